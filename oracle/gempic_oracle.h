@@ -1,0 +1,132 @@
+/*
+ * gempic_oracle.h -- CPU restatement of GEMPIC.jl's particle-field path.
+ *
+ * TEST INFRASTRUCTURE. This library is the checker, not the product: only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.
+ * The product (gempic.jl_amd/csrc, include/gempic_hip.h) never links it.
+ *
+ * Every function follows one function of the reference (Julia, src/...) line
+ * by line, including its rounding order; the file is compiled with
+ * -ffp-contract=off so that no product/sum pair is fused into an FMA (Julia
+ * does not contract by default). Citations are file:line in the reference.
+ *
+ * Layout conventions (same as the reference):
+ *   - particle array: column-major (D+V+n_weights) x N, i.e. element (r, i)
+ *     (both 0-based here) lives at a[r + i*ld] with ld = D+V+n_weights;
+ *   - grid arrays: nx doubles, node k (0-based) = Julia index k+1.
+ */
+#ifndef GEMPIC_ORACLE_H
+#define GEMPIC_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OR_GALERKIN 0
+#define OR_COLLOCATION 1
+#define OR_MAXDEG 5
+
+/* ---------- Julia Base semantics used on the path ---------- */
+double or_jl_mod_f64(double x, double y);
+int64_t or_jl_mod_i64(int64_t a, int64_t n);
+
+/* ---------- low_level_bsplines.jl ---------- */
+void or_bspline_basis(int degree, double offset, double *bspl);
+void or_eval_uniform_periodic_spline_curve(int degree, int64_t n, const double *scoef, double *sval);
+
+/* ---------- FastGaussQuadrature.gausslegendre (third-party, restated) ---------- */
+int or_gausslegendre(int n, double *x, double *w);
+
+/* ---------- splinepp.jl ---------- */
+int or_splinepp_tables(int degree, double *poly_coeffs, double *poly_coeffs_fp);
+void or_b_to_pp(int degree, int64_t ncells, const double *b_coeffs, double *pp_coeffs);
+
+/* ---------- particle_mesh_coupling_1d.jl ---------- */
+typedef struct {
+    int64_t n_grid;
+    double xmin, Lx, delta_x, scaling;
+    int degree, n_span, n_quad;
+    double quad_x[8], quad_w[8];
+    double pp[(OR_MAXDEG + 1) * (OR_MAXDEG + 1)];    /* [power + piece*(deg+1)] */
+    double pp_fp[(OR_MAXDEG + 1) * (OR_MAXDEG + 1)];
+} or_pmc1d;
+
+int or_pmc1d_init(or_pmc1d *p, double xmin, double xmax, int64_t nx, int degree, int smoothing);
+void or_add_charge(double *rho, const or_pmc1d *p, double x, double q);
+void or_add_charge_pp(double *rho, const or_pmc1d *p, double x, double q);
+double or_evaluate(const or_pmc1d *p, double x, const double *field);
+double or_evaluate_pp(const or_pmc1d *p, double x, const double *field_pp);
+double or_add_current_update_v(double *j, const or_pmc1d *p, double x_old, double x_new,
+                               double q, double qm, const double *b, double v);
+double or_add_current_update_v_nob(double *j, const or_pmc1d *p, double x_old, double x_new,
+                                   double q, double qm, double v);
+double or_add_current_update_v_pp(double *j, const or_pmc1d *p, double x_old, double x_new,
+                                  double q, double qm, const double *b, double v);
+
+/* batched helpers (loops over particles in index order, as the reference's callers do) */
+void or_add_charge_n(double *rho, const or_pmc1d *p, int64_t n, const double *x, const double *q);
+void or_evaluate_n(const or_pmc1d *p, int64_t n, const double *x, const double *field, double *out);
+
+/* ---------- maxwell_1d_fem.jl ---------- */
+typedef struct {
+    double xmin, Lx, delta_x;
+    int64_t n;
+    int s_deg_0, s_deg_1;
+    double mass_0[4], mass_1[4];
+    double *eig_mass0, *eig_mass1, *eig_weak_ampere, *eig_weak_poisson;
+    double *work, *wsave, *eigvals;
+    double *cos_tab, *sin_tab;   /* DFT twiddles, index (j*k mod n) */
+} or_maxwell1d;
+
+or_maxwell1d *or_maxwell1d_create(double xmin, double xmax, int64_t n, int degree);
+void or_maxwell1d_destroy(or_maxwell1d *m);
+void or_maxwell1d_tables(const or_maxwell1d *m, double *eig_mass0, double *eig_mass1,
+                         double *eig_weak_ampere, double *eig_weak_poisson);
+void or_r2hc(const or_maxwell1d *m, const double *in, double *out);
+void or_hc2r(const or_maxwell1d *m, const double *in, double *out);
+void or_solve_circulant(or_maxwell1d *m, const double *eigvals, const double *rhs);
+void or_compute_e_from_rho(double *e, or_maxwell1d *m, const double *rho);
+void or_compute_e_from_j(double *e, or_maxwell1d *m, const double *current, int component);
+void or_compute_e_from_b(double *e, or_maxwell1d *m, double dt, const double *b);
+void or_compute_b_from_e(double *b, or_maxwell1d *m, double dt, const double *e);
+/* func(x) = a_cos*cos(k*x) + a_sin*sin(k*x) + c */
+void or_compute_rhs_from_function(double *coefs, or_maxwell1d *m, double a_cos, double a_sin,
+                                  double k, double c, int degree);
+int or_l2projection(double *coefs, or_maxwell1d *m, double a_cos, double a_sin, double k,
+                    double c, int degree);
+double or_l2norm_squared(or_maxwell1d *m, const double *coefs, int degree);
+double or_inner_product(or_maxwell1d *m, const double *c1, const double *c2, int degree);
+const double *or_maxwell1d_work(const or_maxwell1d *m);
+
+/* ---------- hamiltonian_splitting*.jl ---------- */
+typedef struct or_split or_split;
+
+/* particles/e1/e2/b are ALIASED (caller-owned), as HamiltonianSplitting aliases
+ * the caller's e_dofs/b_dofs and the ParticleGroup array. */
+or_split *or_split_create(int64_t nx, double xmin, double xmax, int maxwell_degree,
+                          int degree0, int smoothing0, int degree1, int smoothing1,
+                          int dim_v, int64_t n_particles, int n_weights, double charge,
+                          double mass, double common_weight, double *particles, double *e1,
+                          double *e2, double *b);
+void or_split_destroy(or_split *h);
+const double *or_split_j(const or_split *h, int component);
+void or_operator_hb(or_split *h, double dt);
+void or_operator_he(or_split *h, double dt);
+void or_operator_hp1(or_split *h, double dt);
+void or_operator_hp2(or_split *h, double dt);
+void or_strang_splitting(or_split *h, double dt, int64_t nsteps);
+int or_split_set_particles(or_split *h, double *particles, int64_t n_particles);
+
+/* ---------- diagnostics.jl ---------- */
+void or_solve_poisson(double *efield, or_split *h, double *rho);
+/* out[0..10] = Time, KineticEnergy, Momentum1, Momentum2, PotentialEnergyE1,
+ * PotentialEnergyE2, PotentialEnergyB3, Transfer, VVB, Poynting, ErrorPoisson */
+void or_write_step(or_split *h, double time, int degree, const double *e1, const double *e2,
+                   const double *b, const double *e1_n, const double *e2_n,
+                   const double *efield_poisson, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,327 @@
+"""ctypes view of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE: this module is the checker. Only tests/, the smoke()
+check in __graft_entry__.py and bench.py's cpu_baseline leg import it; the
+product package (gempic.jl_amd/gempic_amd) never does.
+
+Each wrapper names the reference function it restates (file:line in the
+GEMPIC.jl tree); the C file carries the line-by-line restatement.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_D = ctypes.c_double
+_I64 = ctypes.c_int64
+_I = ctypes.c_int
+_P = ctypes.POINTER(ctypes.c_double)
+_VP = ctypes.c_void_p
+
+GALERKIN = 0
+COLLOCATION = 1
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with the Makefile beside this file."""
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        sig = {
+            "or_jl_mod_f64": (_D, [_D, _D]),
+            "or_jl_mod_i64": (_I64, [_I64, _I64]),
+            "or_bspline_basis": (None, [_I, _D, _P]),
+            "or_eval_uniform_periodic_spline_curve": (None, [_I, _I64, _P, _P]),
+            "or_gausslegendre": (_I, [_I, _P, _P]),
+            "or_splinepp_tables": (_I, [_I, _P, _P]),
+            "or_b_to_pp": (None, [_I, _I64, _P, _P]),
+            "or_pmc1d_init": (_I, [_VP, _D, _D, _I64, _I, _I]),
+            "or_add_charge": (None, [_P, _VP, _D, _D]),
+            "or_add_charge_pp": (None, [_P, _VP, _D, _D]),
+            "or_evaluate": (_D, [_VP, _D, _P]),
+            "or_evaluate_pp": (_D, [_VP, _D, _P]),
+            "or_add_current_update_v": (_D, [_P, _VP, _D, _D, _D, _D, _P, _D]),
+            "or_add_current_update_v_nob": (_D, [_P, _VP, _D, _D, _D, _D, _D]),
+            "or_add_current_update_v_pp": (_D, [_P, _VP, _D, _D, _D, _D, _P, _D]),
+            "or_add_charge_n": (None, [_P, _VP, _I64, _P, _P]),
+            "or_evaluate_n": (None, [_VP, _I64, _P, _P, _P]),
+            "or_maxwell1d_create": (_VP, [_D, _D, _I64, _I]),
+            "or_maxwell1d_destroy": (None, [_VP]),
+            "or_maxwell1d_tables": (None, [_VP, _P, _P, _P, _P]),
+            "or_maxwell1d_work": (_P, [_VP]),
+            "or_r2hc": (None, [_VP, _P, _P]),
+            "or_hc2r": (None, [_VP, _P, _P]),
+            "or_solve_circulant": (None, [_VP, _P, _P]),
+            "or_compute_e_from_rho": (None, [_P, _VP, _P]),
+            "or_compute_e_from_j": (None, [_P, _VP, _P, _I]),
+            "or_compute_e_from_b": (None, [_P, _VP, _D, _P]),
+            "or_compute_b_from_e": (None, [_P, _VP, _D, _P]),
+            "or_compute_rhs_from_function": (None, [_P, _VP, _D, _D, _D, _D, _I]),
+            "or_l2projection": (_I, [_P, _VP, _D, _D, _D, _D, _I]),
+            "or_l2norm_squared": (_D, [_VP, _P, _I]),
+            "or_inner_product": (_D, [_VP, _P, _P, _I]),
+            "or_split_create": (_VP, [_I64, _D, _D, _I, _I, _I, _I, _I, _I, _I64, _I, _D, _D, _D,
+                                      _P, _P, _P, _P]),
+            "or_split_destroy": (None, [_VP]),
+            "or_split_j": (_P, [_VP, _I]),
+            "or_operator_hb": (None, [_VP, _D]),
+            "or_operator_he": (None, [_VP, _D]),
+            "or_operator_hp1": (None, [_VP, _D]),
+            "or_operator_hp2": (None, [_VP, _D]),
+            "or_strang_splitting": (None, [_VP, _D, _I64]),
+            "or_split_set_particles": (_I, [_VP, _P, _I64]),
+            "or_solve_poisson": (None, [_P, _VP, _P]),
+            "or_write_step": (None, [_VP, _D, _I, _P, _P, _P, _P, _P, _P, _P]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    assert a.dtype == np.float64 and (a.flags.c_contiguous or a.flags.f_contiguous)
+    return a.ctypes.data_as(_P)
+
+
+# --------------------------------------------------------------------------
+# low-level helpers
+
+def jl_mod(x: float, y: float) -> float:
+    return lib().or_jl_mod_f64(x, y)
+
+
+def bspline_basis(degree: int, offset: float) -> np.ndarray:
+    """uniform_bsplines_eval_basis (low_level_bsplines.jl:38-61)."""
+    out = np.zeros(degree + 1)
+    lib().or_bspline_basis(degree, offset, _p(out))
+    return out
+
+
+def eval_uniform_periodic_spline_curve(degree: int, scoef: np.ndarray) -> np.ndarray:
+    """low_level_bsplines.jl:121-137."""
+    scoef = np.ascontiguousarray(scoef, dtype=np.float64)
+    out = np.zeros_like(scoef)
+    lib().or_eval_uniform_periodic_spline_curve(degree, scoef.size, _p(scoef), _p(out))
+    return out
+
+
+def gausslegendre(n: int):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    assert lib().or_gausslegendre(n, _p(x), _p(w)) == 0
+    return x, w
+
+
+def splinepp_tables(degree: int):
+    """poly_coeffs / poly_coeffs_fp of SplinePP (splinepp.jl:30-233) as
+    (deg+1, deg+1) arrays indexed [power_row, piece]."""
+    d1 = degree + 1
+    a = np.zeros(d1 * d1)
+    b = np.zeros(d1 * d1)
+    assert lib().or_splinepp_tables(degree, _p(a), _p(b)) == 0
+    return a.reshape(d1, d1, order="F"), b.reshape(d1, d1, order="F")
+
+
+def b_to_pp(degree: int, b: np.ndarray) -> np.ndarray:
+    """b_to_pp (splinepp.jl:241-261): returns (deg+1, ncells) Fortran array."""
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    out = np.zeros((degree + 1, b.size), order="F")
+    lib().or_b_to_pp(degree, b.size, _p(b), _p(out))
+    return out
+
+
+# --------------------------------------------------------------------------
+# ParticleMeshCoupling1D
+
+class PMC1D:
+    """ParticleMeshCoupling1D (particle_mesh_coupling_1d.jl:26-95)."""
+
+    _SIZE = 8 * 1 + 8 * 4 + 4 * 3 + 4 + 16 * 8 + 2 * 36 * 8 + 64
+
+    def __init__(self, xmin, xmax, nx, degree, smoothing=GALERKIN):
+        self._buf = ctypes.create_string_buffer(1024)
+        rc = lib().or_pmc1d_init(ctypes.addressof(self._buf), xmin, xmax, nx, degree, smoothing)
+        if rc != 0:
+            raise ValueError(f"invalid ParticleMeshCoupling1D arguments (rc={rc})")
+        self.nx = nx
+        self.degree = degree
+
+    @property
+    def ptr(self):
+        return ctypes.addressof(self._buf)
+
+    def add_charge(self, rho, x, q):
+        lib().or_add_charge(_p(rho), self.ptr, x, q)
+
+    def add_charge_pp(self, rho, x, q):
+        lib().or_add_charge_pp(_p(rho), self.ptr, x, q)
+
+    def evaluate(self, x, field):
+        return lib().or_evaluate(self.ptr, x, _p(field))
+
+    def evaluate_pp(self, x, field_pp):
+        return lib().or_evaluate_pp(self.ptr, x, _p(field_pp))
+
+    def add_current_update_v(self, j, x_old, x_new, q, qm, b, v):
+        return lib().or_add_current_update_v(_p(j), self.ptr, x_old, x_new, q, qm, _p(b), v)
+
+    def add_current_update_v_nob(self, j, x_old, x_new, q, qm, v):
+        return lib().or_add_current_update_v_nob(_p(j), self.ptr, x_old, x_new, q, qm, v)
+
+    def add_current_update_v_pp(self, j, x_old, x_new, q, qm, b, v):
+        return lib().or_add_current_update_v_pp(_p(j), self.ptr, x_old, x_new, q, qm, _p(b), v)
+
+    def add_charge_n(self, rho, x, q):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        lib().or_add_charge_n(_p(rho), self.ptr, x.size, _p(x), _p(q))
+
+    def evaluate_n(self, x, field):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros_like(x)
+        lib().or_evaluate_n(self.ptr, x.size, _p(x), _p(field), _p(out))
+        return out
+
+
+# --------------------------------------------------------------------------
+# Maxwell1DFEM
+
+class Maxwell1D:
+    """Maxwell1DFEM (maxwell_1d_fem.jl:29-177) and its verbs."""
+
+    def __init__(self, xmin, xmax, n, degree):
+        self.h = lib().or_maxwell1d_create(xmin, xmax, n, degree)
+        if not self.h:
+            raise ValueError("invalid Maxwell1DFEM arguments")
+        self.n = n
+        self.degree = degree
+        self.delta_x = (xmax - xmin) / n
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_maxwell1d_destroy(self.h)
+            self.h = None
+
+    def tables(self):
+        t = [np.zeros(self.n) for _ in range(4)]
+        lib().or_maxwell1d_tables(self.h, *[_p(a) for a in t])
+        return dict(eig_mass0=t[0], eig_mass1=t[1], eig_weak_ampere=t[2], eig_weak_poisson=t[3])
+
+    def work(self):
+        return np.ctypeslib.as_array(lib().or_maxwell1d_work(self.h), shape=(self.n,)).copy()
+
+    def r2hc(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros_like(x)
+        lib().or_r2hc(self.h, _p(x), _p(out))
+        return out
+
+    def hc2r(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros_like(x)
+        lib().or_hc2r(self.h, _p(x), _p(out))
+        return out
+
+    def solve_circulant(self, eig, rhs):
+        lib().or_solve_circulant(self.h, _p(np.ascontiguousarray(eig)), _p(np.ascontiguousarray(rhs)))
+        return self.work()
+
+    def compute_e_from_rho(self, e, rho):
+        lib().or_compute_e_from_rho(_p(e), self.h, _p(rho))
+
+    def compute_e_from_j(self, e, j, component):
+        lib().or_compute_e_from_j(_p(e), self.h, _p(j), component)
+
+    def compute_e_from_b(self, e, dt, b):
+        lib().or_compute_e_from_b(_p(e), self.h, dt, _p(b))
+
+    def compute_b_from_e(self, b, dt, e):
+        lib().or_compute_b_from_e(_p(b), self.h, dt, _p(e))
+
+    def compute_rhs_from_function(self, coefs, degree, a_cos=0.0, a_sin=0.0, k=0.0, c=0.0):
+        lib().or_compute_rhs_from_function(_p(coefs), self.h, a_cos, a_sin, k, c, degree)
+
+    def l2projection(self, coefs, degree, a_cos=0.0, a_sin=0.0, k=0.0, c=0.0):
+        if lib().or_l2projection(_p(coefs), self.h, a_cos, a_sin, k, c, degree) != 0:
+            raise ValueError(f"degree {degree} not available")
+
+    def l2norm_squared(self, coefs, degree):
+        return lib().or_l2norm_squared(self.h, _p(coefs), degree)
+
+    def inner_product(self, c1, c2, degree):
+        return lib().or_inner_product(self.h, _p(c1), _p(c2), degree)
+
+
+# --------------------------------------------------------------------------
+# HamiltonianSplitting
+
+class Splitting:
+    """HamiltonianSplitting{1,V} over caller-owned (aliased) arrays
+    (hamiltonian_splitting.jl:20-86). `particles` is the (1+V+nw, N)
+    column-major particle array (a Fortran-ordered numpy array)."""
+
+    def __init__(self, nx, xmin, xmax, maxwell_degree, degree0, degree1, particles, e1, e2, b,
+                 charge=1.0, mass=1.0, common_weight=0.0, smoothing0=GALERKIN,
+                 smoothing1=GALERKIN, n_weights=1):
+        assert particles.flags.f_contiguous and particles.dtype == np.float64
+        dim_v = particles.shape[0] - 1 - n_weights
+        self.particles, self.e1, self.e2, self.b = particles, e1, e2, b
+        self.nx = nx
+        self.h = lib().or_split_create(nx, xmin, xmax, maxwell_degree, degree0, smoothing0,
+                                       degree1, smoothing1, dim_v, particles.shape[1], n_weights,
+                                       charge, mass, common_weight, _p(particles), _p(e1),
+                                       _p(e2), _p(b))
+        if not self.h:
+            raise ValueError("invalid HamiltonianSplitting arguments")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_split_destroy(self.h)
+            self.h = None
+
+    def j(self, component):
+        return np.ctypeslib.as_array(lib().or_split_j(self.h, component), shape=(self.nx,)).copy()
+
+    def operatorHB(self, dt):
+        lib().or_operator_hb(self.h, dt)
+
+    def operatorHE(self, dt):
+        lib().or_operator_he(self.h, dt)
+
+    def operatorHp1(self, dt):
+        lib().or_operator_hp1(self.h, dt)
+
+    def operatorHp2(self, dt):
+        lib().or_operator_hp2(self.h, dt)
+
+    def strang_splitting(self, dt, nsteps):
+        lib().or_strang_splitting(self.h, dt, nsteps)
+
+    def solve_poisson(self, efield, rho):
+        lib().or_solve_poisson(_p(efield), self.h, _p(rho))
+
+    def write_step(self, time, degree, e1, e2, b, e1_n, e2_n, efield_poisson):
+        out = np.zeros(11)
+        lib().or_write_step(self.h, time, degree, _p(e1), _p(e2), _p(b), _p(e1_n), _p(e2_n),
+                            _p(efield_poisson), _p(out))
+        return out
