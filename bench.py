@@ -1,0 +1,177 @@
+#!/usr/bin/env python3
+"""Benchmark: 1D2V Weibel instability, 4,194,304 particles, FP64, on one GPU.
+
+A step is one Strang step (HB HE Hp2 Hp1 Hp2 HE HB) of all particles, run by
+the fused path of the HIP library (one particle pass per step plus the field
+solves in its last workgroup). Inputs are resident in GPU memory before the
+timed region starts.
+
+Prints one JSON line: particle-steps/s over the timed steps, the roofline of
+the fused pass (HIP-event kernel times on the library's stream, algorithmic
+56 bytes per particle-step: read x, v1, v2, w; write x, v1, v2), and a CPU
+baseline (the C oracle, one thread, on a bounded sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "gempic.jl_amd"))
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+BYTES_FUSED = 56          # per particle-step
+BYTES_SUBSTEP = 208       # per particle-step, sub-step kernels (HE 40x2, Hp2 40x2, Hp1 48)
+
+
+def make_splitting(G, w, sync=False):
+    n = w["particles"].shape[1]
+    nx = w["nx"]
+    mesh = G.OneDGrid(w["xmin"], w["xmax"], nx)
+    pg = G.ParticleGroup(1, 2, n, common_weight=w["common_weight"])
+    pg.array[:] = w["particles"]
+    ks0 = G.ParticleMeshCoupling1D(mesh, n, w["degree"], "galerkin")
+    ks1 = G.ParticleMeshCoupling1D(mesh, n, w["degree"] - 1, "galerkin")
+    mx = G.Maxwell1DFEM(mesh, w["degree"])
+    e1, e2, b = np.zeros(nx), np.zeros(nx), w["b0"].copy()
+    h = G.HamiltonianSplitting(1, 2, mx, ks0, ks1, pg, [e1, e2], b, sync=False)
+    rho = np.zeros(nx)
+    G.solve_poisson_(e1, h, rho)          # E1 from Poisson (diagnostics.jl:15-31)
+    h.to_device(particles=False)
+    return h
+
+
+def cpu_baseline(n, steps, seed=1):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as orc
+    from gempic_amd import workloads
+    w = workloads.weibel_1d2v(n, seed=seed)
+    nx = w["nx"]
+    parts = np.asfortranarray(w["particles"])
+    e1, e2, b = np.zeros(nx), np.zeros(nx), w["b0"].copy()
+    osp = orc.Splitting(nx, w["xmin"], w["xmax"], 3, 3, 2, parts, e1, e2, b,
+                        common_weight=w["common_weight"])
+    rho = np.zeros(nx)
+    osp.solve_poisson(e1, rho)
+    t0 = time.perf_counter()
+    osp.strang_splitting(w["dt"], steps)
+    dt = time.perf_counter() - t0
+    return n * steps / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--particles", type=int, default=4194304)
+    ap.add_argument("--sort-interval", type=int, default=20)
+    ap.add_argument("--cpu-particles", type=int, default=262144)
+    ap.add_argument("--cpu-steps", type=int, default=100)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--substep", action="store_true", help="also time the sub-step path")
+    ap.add_argument("--blocks", type=int, default=0, help="workgroups per particle pass (0 = auto)")
+    ap.add_argument("--fusion", type=int, default=1, help="particle kernels per fused step (1 or 2)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(particles would shard with one all-reduce of the nx-sized deposits per "
+              "sub-step; see the design notes). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import gempic_amd as G
+    from gempic_amd import workloads
+    from gempic_amd import _lib
+
+    n = args.particles
+    print(f"[bench] generating 1D2V Weibel, {n} particles", file=sys.stderr, flush=True)
+    w = workloads.weibel_1d2v(n, seed=1)
+    h = make_splitting(G, w)
+    L = _lib.load()
+    _lib.check(L.gempic_set_sort_interval(h._h, args.sort_interval))
+    _lib.check(L.gempic_set_launch(h._h, args.blocks))
+    _lib.check(L.gempic_set_fusion(h._h, args.fusion))
+    dt = w["dt"]
+    print(f"[bench] warmup {args.warmup} steps", file=sys.stderr, flush=True)
+    if args.warmup:
+        _lib.check(L.gempic_strang_splitting(h._h, dt, args.warmup, _lib.STRANG_FUSED))
+    _lib.check(L.gempic_synchronize(h._h))
+    h.profile(True)
+    _lib.check(L.gempic_profile_reset(h._h))
+    print(f"[bench] timing {args.steps} steps", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    _lib.check(L.gempic_strang_splitting(h._h, dt, args.steps, _lib.STRANG_FUSED))
+    _lib.check(L.gempic_synchronize(h._h))
+    elapsed = time.perf_counter() - t0
+    fused_ms, fused_cnt = h.profile_read(0)
+    kernels = {}
+    for kid, name in enumerate(["fused_pass", "trailing_pass", "he", "hp2", "hp1", "grid",
+                                "sort", "other", "fused_pass_a"]):
+        ms, cnt = h.profile_read(kid)
+        if cnt:
+            kernels[name] = {"avg_us": 1e3 * ms / cnt, "count": cnt}
+    h.profile(False)
+    value = n * args.steps / elapsed
+    avg_s = (fused_ms / fused_cnt) * 1e-3 if fused_cnt else float("nan")
+    achieved = BYTES_FUSED * n / avg_s / 1e9
+    print(f"[bench] {value:.4e} particle-steps/s, fused pass {avg_s * 1e6:.2f} us",
+          file=sys.stderr, flush=True)
+
+    substep = None
+    if args.substep:
+        _lib.check(L.gempic_synchronize(h._h))
+        t0 = time.perf_counter()
+        ks = max(args.steps // 10, 1)
+        _lib.check(L.gempic_strang_splitting(h._h, dt, ks, _lib.STRANG_SUBSTEP))
+        _lib.check(L.gempic_synchronize(h._h))
+        substep = n * ks / (time.perf_counter() - t0)
+
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "fused_pass_weibel_4m_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        print(f"[bench] CPU baseline: oracle, {args.cpu_particles} particles x "
+              f"{args.cpu_steps} steps", file=sys.stderr, flush=True)
+        rate, secs = cpu_baseline(args.cpu_particles, args.cpu_steps)
+        cpu = {"value": rate, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+               "sample": f"1D2V Weibel, {args.cpu_particles} particles x {args.cpu_steps} Strang "
+                         f"steps ({secs:.1f} s), C restatement of the reference, one thread"}
+
+    line = {
+        "metric": "particle-steps/s (1D2V Weibel, 4M particles, FP64, fused Strang step)",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Weibel initial condition)",
+        "config": {"workload": "1D2V Weibel instability", "particles": n, "nx": w["nx"],
+                   "spline_degree": w["degree"], "dt": dt, "sort_interval": args.sort_interval,
+                   "blocks": args.blocks, "kernels_per_step": args.fusion,
+                   "path": "fused Strang step"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_pass1d fused Strang pass",
+                     "bytes_per_particle_step": BYTES_FUSED},
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+    }
+    if substep is not None:
+        line["substep_path_value"] = substep
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

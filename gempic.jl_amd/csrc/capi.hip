@@ -1,0 +1,1078 @@
+// capi.hip -- the C ABI of include/gempic_hip.h.
+//
+// Owns the device memory of each handle (SoA particles with a sort double
+// buffer, the field/current grid slots, the constant tables, reduction
+// scratch), drives the kernels of pass1d.hip on the handle's stream, and
+// maps kernel error flags onto the reference's exceptions.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gempic_hip.h"
+#include "host_setup.hpp"
+#include "lib_internal.hpp"
+
+using namespace gl;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                     \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail(GEMPIC_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int check_device()
+{
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n < 1) {
+        const char *hv = getenv("HIP_VISIBLE_DEVICES");
+        const char *rv = getenv("ROCR_VISIBLE_DEVICES");
+        return fail(GEMPIC_ERR_DEVICE,
+                    std::string("no usable AMD GPU: the particle-field path runs only on the GPU "
+                                "(hipGetDeviceCount: ") +
+                        hipGetErrorName(e) + ", count " + std::to_string(n) +
+                        ", HIP_VISIBLE_DEVICES=" + (hv ? hv : "unset") +
+                        ", ROCR_VISIBLE_DEVICES=" + (rv ? rv : "unset") + ")");
+    }
+    return GEMPIC_OK;
+}
+
+int device_cus()
+{
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+        return 256;
+    return cus;
+}
+
+bool pair_supported(int p0, int p1)
+{
+    return (p0 == 1 && (p1 == 0 || p1 == 1)) || (p0 == 2 && (p1 == 1 || p1 == 2)) ||
+           (p0 == 3 && (p1 == 2 || p1 == 3));
+}
+
+struct Launch {
+    int blocks;
+    int64_t chunk;
+    int iters;
+    int ngroups;
+};
+
+Launch plan(int64_t n, int max_blocks)
+{
+    Launch L;
+    int64_t want = (n + (int64_t)kBlock * 8 - 1) / ((int64_t)kBlock * 8);   // >= 8 particles per thread
+    if (want < 1) want = 1;
+    int64_t blocks = std::min<int64_t>(want, max_blocks);
+    int64_t per = (n + blocks - 1) / blocks;
+    L.chunk = ((per + kBlock - 1) / kBlock) * kBlock;
+    if (L.chunk < kBlock) L.chunk = kBlock;
+    L.iters = (int)(L.chunk / kBlock);
+    L.blocks = (int)((n + L.chunk - 1) / L.chunk);
+    if (L.blocks < 1) L.blocks = 1;
+    L.ngroups = (L.blocks + kGroup - 1) / kGroup;
+    return L;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// splitting handle
+// ---------------------------------------------------------------------------
+struct gempic_splitting {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int dim_v = 2, n_weights = 1, ld = 4;
+    int64_t n = 0, nx = 0;
+    double xmin = 0, xmax = 0, Lx = 0, dx = 0;
+    int mdeg = 3, deg0 = 3, deg1 = 2, sm0 = 0, sm1 = 0;
+    double scaling0 = 1, scaling1 = 1, charge = 1, mass = 1, cw = 0, qm = 1;
+    double qx1[3] = {0, 0, 0}, qw1[3] = {0, 0, 0};
+    gh::Maxwell1DTables tabs;
+
+    double *d_x = nullptr, *d_v1 = nullptr, *d_v2 = nullptr, *d_w = nullptr;
+    int64_t *d_id = nullptr;
+    double *d_x2 = nullptr, *d_v12 = nullptr, *d_v22 = nullptr, *d_w2 = nullptr;
+    int64_t *d_id2 = nullptr;
+    double *d_grid = nullptr, *d_tabs = nullptr;
+    double *d_part = nullptr, *d_gpart = nullptr;
+    unsigned *d_cnt = nullptr;
+    int *d_err = nullptr;
+    double *d_diag = nullptr;
+    int *d_sort_counts = nullptr, *d_sort_offsets = nullptr;
+    int ntiles = 0;
+    double *d_stage = nullptr;
+
+    int max_blocks = 1024;            // capacity of the reduction scratch
+    int dep_blocks = 0;               // workgroups of every deposit-bearing pass (0: not set)
+    int override_blocks = 0;          // gempic_set_launch
+    int cus = 256;
+    int kernels_per_step = 1;         // fused Strang step in 1 or 2 particle kernels
+    int64_t sort_interval = 20;
+    int64_t steps_since_sort = 0;
+    bool particles_set = false;
+
+    // profiling
+    bool profile = false;
+    struct Pend { int id; hipEvent_t a, b; };
+    std::vector<Pend> pending;
+    std::vector<hipEvent_t> pool;
+    double prof_ms[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    int64_t prof_cnt[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+struct gempic_maxwell1d {
+    hipStream_t stream = nullptr;
+    int64_t nx = 0;
+    double xmin = 0, xmax = 0, dx = 0;
+    int degree = 3;
+    gh::Maxwell1DTables tabs;
+    double *d_grid = nullptr, *d_tabs = nullptr, *d_scalar = nullptr;
+};
+
+namespace {
+
+hipEvent_t take_event(gempic_splitting *h)
+{
+    if (!h->pool.empty()) {
+        hipEvent_t e = h->pool.back();
+        h->pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+struct ProfScope {
+    gempic_splitting *h;
+    int id;
+    hipEvent_t a = nullptr, b = nullptr;
+    ProfScope(gempic_splitting *hh, int kid) : h(hh), id(kid)
+    {
+        if (h->profile) {
+            a = take_event(h);
+            b = take_event(h);
+            if (a) (void)hipEventRecord(a, h->stream);
+        }
+    }
+    ~ProfScope()
+    {
+        if (h->profile && a && b) {
+            (void)hipEventRecord(b, h->stream);
+            h->pending.push_back({id, a, b});
+        }
+    }
+};
+
+void resolve_profile(gempic_splitting *h)
+{
+    if (h->pending.empty()) return;
+    (void)hipStreamSynchronize(h->stream);
+    for (auto &p : h->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            h->prof_ms[p.id] += ms;
+            h->prof_cnt[p.id] += 1;
+        }
+        h->pool.push_back(p.a);
+        h->pool.push_back(p.b);
+    }
+    h->pending.clear();
+}
+
+PassArgs pass_args(gempic_splitting *h)
+{
+    PassArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.n = h->n;
+    a.nx = (int)h->nx;
+    a.xmin = h->xmin;
+    a.dx = h->dx;
+    a.Lx = h->Lx;
+    a.scaling0 = h->scaling0;
+    a.scaling1 = h->scaling1;
+    for (int k = 0; k < 3; ++k) { a.qx[k] = h->qx1[k]; a.qw[k] = h->qw1[k]; }
+    a.charge = h->charge;
+    a.mass = h->mass;
+    a.cw = h->cw;
+    a.qm = h->qm;
+    a.x = h->d_x; a.v1 = h->d_v1; a.v2 = h->d_v2; a.w = h->d_w;
+    a.grid = h->d_grid;
+    a.tabs = h->d_tabs;
+    a.part = h->d_part;
+    a.gpart = h->d_gpart;
+    a.cnt = h->d_cnt;
+    a.err = h->d_err;
+    a.diag = h->d_diag;
+    return a;
+}
+
+bool mode_deposits(int mode)
+{
+    return (mode & (M_HP2A | M_HP1 | M_HP2B | M_RHO | M_DIAG | M_WITH_J2A)) != 0;
+}
+
+// Deposit-bearing passes all use one workgroup count, so the deposits of the
+// fused and the sub-step paths are summed in the same order: CUs times the
+// fewest workgroups per CU among those kernels (one full round, no tail).
+void set_dep_blocks(gempic_splitting *h)
+{
+    const size_t sh = pass_shmem_bytes((int)h->nx);
+    std::vector<int> modes;
+    if (h->dim_v == 2)
+        modes = {M_HP2A, M_HP1, M_HE | M_HP2A | M_HP1 | M_HP2B,
+                 M_HE_PREV | M_HE | M_HP2A | M_HP1 | M_HP2B, M_HE_PREV | M_HE | M_HP2A | M_NOEPI,
+                 M_HP1 | M_HP2B | M_WITH_J2A, M_RHO, M_DIAG};
+    else
+        modes = {M_V1 | M_HP1, M_V1 | M_HE | M_HP1, M_V1 | M_HE_PREV | M_HE | M_HP1, M_RHO};
+    int bpc = 64;
+    for (int m : modes) bpc = std::min(bpc, pass_blocks_per_cu(h->deg0, h->deg1, m, sh));
+    h->dep_blocks = std::max(1, std::min(h->max_blocks, h->cus * bpc));
+}
+
+int run_pass(gempic_splitting *h, int mode, double dt_he, double dt_hp2, double dt_hp1, int more,
+             int prof_id)
+{
+    PassArgs a = pass_args(h);
+    int target;
+    if (h->override_blocks) target = h->override_blocks;
+    else if (mode_deposits(mode)) target = h->dep_blocks;
+    else target = std::min(h->max_blocks,
+                           h->cus * pass_blocks_per_cu(h->deg0, h->deg1, mode,
+                                                       pass_shmem_bytes((int)h->nx)));
+    Launch L = plan(h->n, target);
+    a.chunk = L.chunk;
+    a.iters = L.iters;
+    a.ngroups = L.ngroups;
+    a.dt_he = dt_he;
+    a.dt_hp2 = dt_hp2;
+    a.dt_hp1 = dt_hp1;
+    a.more = more;
+    ProfScope ps(h, prof_id);
+    hipError_t e = launch_pass(h->deg0, h->deg1, mode, a, L.blocks, pass_shmem_bytes((int)h->nx), h->stream);
+    if (e != hipSuccess) return fail(GEMPIC_ERR_DEVICE, std::string("particle pass launch: ") + hipGetErrorString(e));
+    return GEMPIC_OK;
+}
+
+int run_grid(gempic_splitting *h, const GridProg &g)
+{
+    ProfScope ps(h, 5);
+    hipError_t e = launch_grid(g, (int)h->nx, h->dx, h->d_grid, h->d_tabs, nullptr, h->stream);
+    if (e != hipSuccess) return fail(GEMPIC_ERR_DEVICE, std::string("field program launch: ") + hipGetErrorString(e));
+    return GEMPIC_OK;
+}
+
+GridProg prog1(int op, double dt, int a, int b, int out)
+{
+    GridProg g;
+    std::memset(&g, 0, sizeof(g));
+    g.nops = 1;
+    g.op[0] = op; g.dt[0] = dt; g.slot_a[0] = a; g.slot_b[0] = b; g.slot_out[0] = out;
+    return g;
+}
+
+void prog_add(GridProg &g, int op, double dt, int a, int b, int out)
+{
+    int k = g.nops++;
+    g.op[k] = op; g.dt[k] = dt; g.slot_a[k] = a; g.slot_b[k] = b; g.slot_out[k] = out;
+}
+
+int run_sort(gempic_splitting *h)
+{
+    ProfScope ps(h, 6);
+    SortArgs s;
+    s.n = h->n;
+    s.nx = (int)h->nx;
+    s.ntiles = h->ntiles;
+    s.xmin = h->xmin;
+    s.dx = h->dx;
+    s.x = h->d_x; s.v1 = h->d_v1; s.v2 = h->d_v2; s.w = h->d_w; s.id = h->d_id;
+    s.x2 = h->d_x2; s.v12 = h->d_v12; s.v22 = h->d_v22; s.w2 = h->d_w2; s.id2 = h->d_id2;
+    s.counts = h->d_sort_counts;
+    s.offsets = h->d_sort_offsets;
+    hipError_t e = launch_sort(s, h->dim_v == 2, h->stream);
+    if (e != hipSuccess) return fail(GEMPIC_ERR_DEVICE, std::string("sort launch: ") + hipGetErrorString(e));
+    std::swap(h->d_x, h->d_x2);
+    std::swap(h->d_v1, h->d_v12);
+    std::swap(h->d_v2, h->d_v22);
+    std::swap(h->d_w, h->d_w2);
+    std::swap(h->d_id, h->d_id2);
+    h->steps_since_sort = 0;
+    return GEMPIC_OK;
+}
+
+int finish(gempic_splitting *h)
+{
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    int err = 0;
+    HIP_TRY(hipMemcpy(&err, h->d_err, sizeof(int), hipMemcpyDeviceToHost));
+    if (err) {
+        HIP_TRY(hipMemset(h->d_err, 0, sizeof(int)));
+        if (err & E_INEXACT)
+            return fail(GEMPIC_ERR_INEXACT,
+                        "InexactError: trunc(Int64, x): a particle position is not finite or lies "
+                        "beyond 1e9 cells of the domain");
+        if (err & E_FAR)
+            return fail(GEMPIC_ERR_INEXACT,
+                        "a particle crossed more than 65536 cells in one Hp1 push");
+        return fail(GEMPIC_ERR_ARGUMENT, "BoundsError: a particle lies outside the pp matrix");
+    }
+    return GEMPIC_OK;
+}
+
+#define CHECK_H(h)                                                                    \
+    do {                                                                              \
+        if (!(h)) return fail(GEMPIC_ERR_ARGUMENT, "null handle");                    \
+        if (hipSetDevice((h)->device) != hipSuccess)                                  \
+            return fail(GEMPIC_ERR_DEVICE, "hipSetDevice failed");                    \
+    } while (0)
+
+#define CHECK_P(h)                                                                    \
+    do {                                                                              \
+        if (!(h)->particles_set) return fail(GEMPIC_ERR_STATE, "particles not set");  \
+    } while (0)
+
+template <typename T>
+hipError_t dalloc(T **p, size_t count)
+{
+    return hipMalloc((void **)p, sizeof(T) * std::max<size_t>(count, 1));
+}
+
+}  // namespace
+
+extern "C" {
+
+int gempic_version(void) { return 100; }
+
+const char *gempic_last_error(void) { return g_last_error.c_str(); }
+
+int gempic_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int gempic_splitting_create(int64_t nx, double xmin, double xmax, int maxwell_degree, int degree0,
+                            int smoothing0, int degree1, int smoothing1, int dim_v,
+                            int64_t n_particles, int n_weights, double charge, double mass,
+                            double common_weight, gempic_splitting **out)
+{
+    if (!out) return fail(GEMPIC_ERR_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    if (maxwell_degree < 1 || maxwell_degree > 3)
+        return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: Wrong value of degree (1,2 or 3)");
+    if (smoothing0 != GEMPIC_GALERKIN && smoothing0 != GEMPIC_COLLOCATION)
+        return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: smoothing type not implemented");
+    if (smoothing1 != GEMPIC_GALERKIN && smoothing1 != GEMPIC_COLLOCATION)
+        return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: smoothing type not implemented");
+    if (!pair_supported(degree0, degree1))
+        return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: smoother degrees (p, p-1) or (p, p) with p in 1..3 are supported");
+    if (dim_v != 1 && dim_v != 2) return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: dim_v must be 1 or 2");
+    if (n_weights != 1) return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: n_weights must be 1");
+    if (nx < 2 || nx > kNxMax || (nx % 2) != 0)
+        return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: nx must be even and in [2, 256]");
+    if (n_particles < 1) return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: n_particles must be positive");
+    if (!(xmax > xmin)) return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: xmax must exceed xmin");
+    int rc = check_device();
+    if (rc) return rc;
+
+    gempic_splitting *h = new gempic_splitting();
+    (void)hipGetDevice(&h->device);
+    h->dim_v = dim_v;
+    h->n_weights = n_weights;
+    h->ld = 1 + dim_v + n_weights;
+    h->n = n_particles;
+    h->nx = nx;
+    h->xmin = xmin;
+    h->xmax = xmax;
+    // Maxwell1DFEM: Lx = xmax - xmin, delta_x = Lx / n_dofs (maxwell_1d_fem.jl:50-53);
+    // ParticleMeshCoupling1D: delta_x = (xmax - xmin) / n_grid, the same value.
+    h->Lx = xmax - xmin;
+    h->dx = h->Lx / (double)nx;
+    h->mdeg = maxwell_degree;
+    h->deg0 = degree0;
+    h->deg1 = degree1;
+    h->sm0 = smoothing0;
+    h->sm1 = smoothing1;
+    h->scaling0 = smoothing0 == GEMPIC_COLLOCATION ? 1.0 / h->dx : 1.0;
+    h->scaling1 = smoothing1 == GEMPIC_COLLOCATION ? 1.0 / h->dx : 1.0;
+    h->charge = charge;
+    h->mass = mass;
+    h->cw = common_weight == 0.0 ? 1.0 / (double)n_particles : common_weight;
+    h->qm = charge / mass;
+    gh::gauss_legendre((degree1 + 2) / 2, h->qx1, h->qw1);
+    if (!gh::maxwell1d_tables(nx, maxwell_degree, h->tabs)) {
+        delete h;
+        return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: invalid Maxwell1DFEM size");
+    }
+    h->cus = device_cus();
+    h->max_blocks = h->cus * 8;
+    const int max_groups = (h->max_blocks + kGroup - 1) / kGroup;
+    const size_t n = (size_t)n_particles;
+    h->ntiles = (int)((n + kSortTile - 1) / kSortTile);
+    hipError_t e = hipSuccess;
+    auto ok = [&](hipError_t r) { if (r != hipSuccess && e == hipSuccess) e = r; };
+    ok(hipStreamCreate(&h->stream));
+    ok(dalloc(&h->d_x, n)); ok(dalloc(&h->d_v1, n)); ok(dalloc(&h->d_w, n)); ok(dalloc(&h->d_id, n));
+    ok(dalloc(&h->d_x2, n)); ok(dalloc(&h->d_v12, n)); ok(dalloc(&h->d_w2, n)); ok(dalloc(&h->d_id2, n));
+    if (dim_v == 2) { ok(dalloc(&h->d_v2, n)); ok(dalloc(&h->d_v22, n)); }
+    ok(dalloc(&h->d_grid, (size_t)NSLOT * nx));
+    ok(dalloc(&h->d_tabs, (size_t)NTAB * nx));
+    ok(dalloc(&h->d_part, (size_t)h->max_blocks * NDEP * nx + 64));
+    ok(dalloc(&h->d_gpart, (size_t)max_groups * NDEP * nx + 64));
+    ok(dalloc(&h->d_cnt, (size_t)max_groups + 2));
+    ok(dalloc(&h->d_err, 1));
+    ok(dalloc(&h->d_diag, 16));
+    ok(dalloc(&h->d_sort_counts, (size_t)h->ntiles * nx));
+    ok(dalloc(&h->d_sort_offsets, (size_t)h->ntiles * nx));
+    if (e == hipSuccess) {
+        ok(hipMemset(h->d_grid, 0, sizeof(double) * NSLOT * nx));
+        ok(hipMemset(h->d_cnt, 0, sizeof(unsigned) * (max_groups + 2)));
+        ok(hipMemset(h->d_err, 0, sizeof(int)));
+        ok(hipMemset(h->d_diag, 0, sizeof(double) * 16));
+        std::vector<double> T((size_t)NTAB * nx, 0.0);
+        auto put = [&](int slot, const std::vector<double> &v) {
+            std::copy(v.begin(), v.end(), T.begin() + (size_t)slot * nx);
+        };
+        put(T_COS, h->tabs.cos_tab); put(T_SIN, h->tabs.sin_tab);
+        put(T_INVM0, h->tabs.inv_mass0); put(T_INVM1, h->tabs.inv_mass1);
+        put(T_AMP, h->tabs.eig_weak_ampere); put(T_POIS, h->tabs.eig_weak_poisson);
+        put(T_M0, h->tabs.eig_mass0); put(T_M1, h->tabs.eig_mass1);
+        ok(hipMemcpy(h->d_tabs, T.data(), sizeof(double) * T.size(), hipMemcpyHostToDevice));
+    }
+    if (e != hipSuccess) {
+        gempic_splitting_destroy(h);
+        return fail(GEMPIC_ERR_DEVICE, std::string("device allocation: ") + hipGetErrorString(e));
+    }
+    set_dep_blocks(h);
+    *out = h;
+    return GEMPIC_OK;
+}
+
+int gempic_splitting_destroy(gempic_splitting *h)
+{
+    if (!h) return GEMPIC_OK;
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    void *bufs[] = {h->d_x, h->d_v1, h->d_v2, h->d_w, h->d_id, h->d_x2, h->d_v12, h->d_v22,
+                    h->d_w2, h->d_id2, h->d_grid, h->d_tabs, h->d_part, h->d_gpart, h->d_cnt,
+                    h->d_err, h->d_diag, h->d_sort_counts, h->d_sort_offsets, h->d_stage};
+    for (void *p : bufs)
+        if (p) (void)hipFree(p);
+    for (auto &p : h->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+    for (auto e : h->pool) (void)hipEventDestroy(e);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return GEMPIC_OK;
+}
+
+static int ensure_stage(gempic_splitting *h)
+{
+    if (!h->d_stage) HIP_TRY(dalloc(&h->d_stage, (size_t)h->n * h->ld));
+    return GEMPIC_OK;
+}
+
+int gempic_set_particles(gempic_splitting *h, const double *aos, int64_t ld, int64_t n)
+{
+    CHECK_H(h);
+    if (!aos) return fail(GEMPIC_ERR_ARGUMENT, "particle array is NULL");
+    if (ld != h->ld) return fail(GEMPIC_ERR_ASSERTION, "AssertionError: leading dimension must be D+V+n_weights");
+    if (n != h->n) return fail(GEMPIC_ERR_ASSERTION, "AssertionError: particle count differs from the handle's");
+    int rc = ensure_stage(h);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(h->d_stage, aos, sizeof(double) * (size_t)n * ld, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(launch_aos_to_soa(h->d_stage, ld, n, h->dim_v, h->d_x, h->d_v1, h->d_v2, h->d_w, h->d_id, h->stream));
+    h->particles_set = true;
+    rc = run_sort(h);
+    if (rc) return rc;
+    return finish(h);
+}
+
+int gempic_get_particles(gempic_splitting *h, double *aos, int64_t ld, int64_t n)
+{
+    CHECK_H(h);
+    CHECK_P(h);
+    if (!aos) return fail(GEMPIC_ERR_ARGUMENT, "particle array is NULL");
+    if (ld != h->ld) return fail(GEMPIC_ERR_ASSERTION, "AssertionError: leading dimension must be D+V+n_weights");
+    if (n != h->n) return fail(GEMPIC_ERR_ASSERTION, "AssertionError: particle count differs from the handle's");
+    int rc = ensure_stage(h);
+    if (rc) return rc;
+    HIP_TRY(launch_soa_to_aos(h->d_stage, ld, n, h->dim_v, h->d_x, h->d_v1, h->d_v2, h->d_w, h->d_id, h->stream));
+    HIP_TRY(hipMemcpyAsync(aos, h->d_stage, sizeof(double) * (size_t)n * ld, hipMemcpyDeviceToHost, h->stream));
+    return finish(h);
+}
+
+int gempic_set_fields(gempic_splitting *h, const double *e1, const double *e2, const double *b)
+{
+    CHECK_H(h);
+    const size_t nb = sizeof(double) * h->nx;
+    if (e1) HIP_TRY(hipMemcpyAsync(h->d_grid + S_E1 * h->nx, e1, nb, hipMemcpyHostToDevice, h->stream));
+    if (e2) HIP_TRY(hipMemcpyAsync(h->d_grid + S_E2 * h->nx, e2, nb, hipMemcpyHostToDevice, h->stream));
+    if (b) HIP_TRY(hipMemcpyAsync(h->d_grid + S_B * h->nx, b, nb, hipMemcpyHostToDevice, h->stream));
+    return finish(h);
+}
+
+int gempic_get_fields(gempic_splitting *h, double *e1, double *e2, double *b)
+{
+    CHECK_H(h);
+    const size_t nb = sizeof(double) * h->nx;
+    if (e1) HIP_TRY(hipMemcpyAsync(e1, h->d_grid + S_E1 * h->nx, nb, hipMemcpyDeviceToHost, h->stream));
+    if (e2) HIP_TRY(hipMemcpyAsync(e2, h->d_grid + S_E2 * h->nx, nb, hipMemcpyDeviceToHost, h->stream));
+    if (b) HIP_TRY(hipMemcpyAsync(b, h->d_grid + S_B * h->nx, nb, hipMemcpyDeviceToHost, h->stream));
+    return finish(h);
+}
+
+int gempic_get_j(gempic_splitting *h, double *j1, double *j2)
+{
+    CHECK_H(h);
+    const size_t nb = sizeof(double) * h->nx;
+    if (j1) HIP_TRY(hipMemcpyAsync(j1, h->d_grid + S_J1 * h->nx, nb, hipMemcpyDeviceToHost, h->stream));
+    if (j2) HIP_TRY(hipMemcpyAsync(j2, h->d_grid + S_J2 * h->nx, nb, hipMemcpyDeviceToHost, h->stream));
+    return finish(h);
+}
+
+int gempic_operator_hb(gempic_splitting *h, double dt)
+{
+    CHECK_H(h);
+    int rc;
+    if (h->dim_v == 2) {
+        rc = run_grid(h, prog1(G_HB, dt, S_B, S_B, S_E2));
+    } else {
+        CHECK_P(h);
+        rc = run_pass(h, M_V1 | M_HE, dt, 0.0, 0.0, 0, 7);
+    }
+    if (rc) return rc;
+    return finish(h);
+}
+
+int gempic_operator_he(gempic_splitting *h, double dt)
+{
+    CHECK_H(h);
+    CHECK_P(h);
+    if (h->dim_v != 2) return fail(GEMPIC_ERR_ARGUMENT, "MethodError: operatorHE is defined for 1d2v only");
+    int rc = run_pass(h, M_HE, dt, 0.0, 0.0, 0, 2);
+    if (rc) return rc;
+    return finish(h);
+}
+
+int gempic_operator_hp2(gempic_splitting *h, double dt)
+{
+    CHECK_H(h);
+    CHECK_P(h);
+    if (h->dim_v != 2) return fail(GEMPIC_ERR_ARGUMENT, "MethodError: operatorHp2 is defined for 1d2v only");
+    int rc = run_pass(h, M_HP2A, 0.0, dt, 0.0, 0, 3);
+    if (rc) return rc;
+    return finish(h);
+}
+
+int gempic_operator_hp1(gempic_splitting *h, double dt)
+{
+    CHECK_H(h);
+    CHECK_P(h);
+    int mode = h->dim_v == 2 ? M_HP1 : (M_V1 | M_HP1);
+    int rc = run_pass(h, mode, 0.0, 0.0, dt, 0, 4);
+    if (rc) return rc;
+    return finish(h);
+}
+
+static int maybe_sort(gempic_splitting *h)
+{
+    if (h->sort_interval > 0 && h->steps_since_sort >= h->sort_interval) return run_sort(h);
+    return GEMPIC_OK;
+}
+
+int gempic_strang_splitting(gempic_splitting *h, double dt, int64_t n_steps, int flags)
+{
+    CHECK_H(h);
+    CHECK_P(h);
+    if (n_steps < 0) return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: number_steps must be >= 0");
+    if (n_steps == 0) return finish(h);
+    const double hdt = 0.5 * dt;
+    int rc = GEMPIC_OK;
+    const bool substep = (flags & GEMPIC_STRANG_SUBSTEP) != 0;
+    if (h->dim_v == 2) {
+        if (substep) {
+            for (int64_t s = 0; s < n_steps && !rc; ++s) {
+                rc = maybe_sort(h);
+                if (!rc) rc = run_grid(h, prog1(G_HB, hdt, S_B, S_B, S_E2));
+                if (!rc) rc = run_pass(h, M_HE, hdt, 0, 0, 0, 2);
+                if (!rc) rc = run_pass(h, M_HP2A, 0, hdt, 0, 0, 3);
+                if (!rc) rc = run_pass(h, M_HP1, 0, 0, 1.0 * dt, 0, 4);
+                if (!rc) rc = run_pass(h, M_HP2A, 0, hdt, 0, 0, 3);
+                if (!rc) rc = run_pass(h, M_HE, hdt, 0, 0, 0, 2);
+                if (!rc) rc = run_grid(h, prog1(G_HB, hdt, S_B, S_B, S_E2));
+                h->steps_since_sort++;
+            }
+        } else {
+            // HB(dt/2) and HE's B update of the first step touch only fields
+            GridProg g = prog1(G_HB, hdt, S_B, S_B, S_E2);
+            prog_add(g, G_HE_B, hdt, S_E2, S_E2, S_B);
+            rc = run_grid(h, g);
+            for (int64_t s = 0; s < n_steps && !rc; ++s) {
+                rc = maybe_sort(h);
+                const int prev = s == 0 ? 0 : M_HE_PREV;
+                const int more = s + 1 < n_steps;
+                if (h->kernels_per_step == 1) {
+                    const int mode = prev | M_HE | M_HP2A | M_HP1 | M_HP2B;
+                    if (!rc) rc = run_pass(h, mode, hdt, hdt, 1.0 * dt, more, 0);
+                } else {
+                    // HE pair + first Hp2, then Hp1 + second Hp2 with the field solves;
+                    // Hp1 reads only B, which the first Hp2's solve leaves unchanged
+                    if (!rc) rc = run_pass(h, prev | M_HE | M_HP2A | M_NOEPI, hdt, hdt, 0, 0, 8);
+                    if (!rc) rc = run_pass(h, M_HP1 | M_HP2B | M_WITH_J2A, hdt, hdt, 1.0 * dt, more, 0);
+                }
+                h->steps_since_sort++;
+            }
+            if (!rc) rc = run_pass(h, M_HE_PREV, hdt, 0, 0, 0, 1);
+        }
+    } else {
+        if (substep) {
+            for (int64_t s = 0; s < n_steps && !rc; ++s) {
+                rc = maybe_sort(h);
+                if (!rc) rc = run_pass(h, M_V1 | M_HE, hdt, 0, 0, 0, 7);
+                if (!rc) rc = run_pass(h, M_V1 | M_HP1, 0, 0, dt, 0, 4);
+                if (!rc) rc = run_pass(h, M_V1 | M_HE, hdt, 0, 0, 0, 7);
+                h->steps_since_sort++;
+            }
+        } else {
+            for (int64_t s = 0; s < n_steps && !rc; ++s) {
+                rc = maybe_sort(h);
+                const int mode = M_V1 | (s == 0 ? 0 : M_HE_PREV) | M_HE | M_HP1;
+                if (!rc) rc = run_pass(h, mode, hdt, 0, dt, s + 1 < n_steps, 0);
+                h->steps_since_sort++;
+            }
+            if (!rc) rc = run_pass(h, M_V1 | M_HE_PREV, hdt, 0, 0, 0, 1);
+        }
+    }
+    if (rc) return rc;
+    if (flags & GEMPIC_ASYNC) return GEMPIC_OK;
+    return finish(h);
+}
+
+int gempic_synchronize(gempic_splitting *h)
+{
+    CHECK_H(h);
+    return finish(h);
+}
+
+int gempic_set_sort_interval(gempic_splitting *h, int64_t interval)
+{
+    CHECK_H(h);
+    if (interval < 0) return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: interval must be >= 0");
+    h->sort_interval = interval;
+    return GEMPIC_OK;
+}
+
+int gempic_sort_particles(gempic_splitting *h)
+{
+    CHECK_H(h);
+    CHECK_P(h);
+    int rc = run_sort(h);
+    if (rc) return rc;
+    return finish(h);
+}
+
+int gempic_solve_poisson(gempic_splitting *h, double *efield, double *rho)
+{
+    CHECK_H(h);
+    CHECK_P(h);
+    int rc = run_pass(h, M_RHO, 0, 0, 0, 0, 7);
+    if (rc) return rc;
+    const size_t nb = sizeof(double) * h->nx;
+    if (efield) HIP_TRY(hipMemcpyAsync(efield, h->d_grid + S_EPOIS * h->nx, nb, hipMemcpyDeviceToHost, h->stream));
+    if (rho) HIP_TRY(hipMemcpyAsync(rho, h->d_grid + S_RHO * h->nx, nb, hipMemcpyDeviceToHost, h->stream));
+    return finish(h);
+}
+
+int gempic_write_step(gempic_splitting *h, double time, int degree, const double *e1_n,
+                      const double *e2_n, const double *efield_poisson, double *out11)
+{
+    CHECK_H(h);
+    CHECK_P(h);
+    if (!out11) return fail(GEMPIC_ERR_ARGUMENT, "out is NULL");
+    if (h->dim_v != 2) return fail(GEMPIC_ERR_ARGUMENT, "write_step! reads v1, v2: 1d2v only");
+    if (degree != h->mdeg) return fail(GEMPIC_ERR_ARGUMENT, "degree must be the Maxwell solver's s_deg_0");
+    const size_t nb = sizeof(double) * h->nx;
+    const int64_t nx = h->nx;
+    if (e1_n) HIP_TRY(hipMemcpyAsync(h->d_grid + S_E1N * nx, e1_n, nb, hipMemcpyHostToDevice, h->stream));
+    else HIP_TRY(hipMemcpyAsync(h->d_grid + S_E1N * nx, h->d_grid + S_E1 * nx, nb, hipMemcpyDeviceToDevice, h->stream));
+    if (e2_n) HIP_TRY(hipMemcpyAsync(h->d_grid + S_E2N * nx, e2_n, nb, hipMemcpyHostToDevice, h->stream));
+    else HIP_TRY(hipMemcpyAsync(h->d_grid + S_E2N * nx, h->d_grid + S_E2 * nx, nb, hipMemcpyDeviceToDevice, h->stream));
+    int rc;
+    if (efield_poisson) {
+        HIP_TRY(hipMemcpyAsync(h->d_grid + S_EPOIS * nx, efield_poisson, nb, hipMemcpyHostToDevice, h->stream));
+    } else {
+        rc = run_pass(h, M_RHO, 0, 0, 0, 0, 7);
+        if (rc) return rc;
+    }
+    rc = run_pass(h, M_DIAG, 0, 0, 0, 0, 7);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(out11, h->d_diag, sizeof(double) * 11, hipMemcpyDeviceToHost, h->stream));
+    rc = finish(h);
+    out11[0] = time;
+    return rc;
+}
+
+int gempic_profile_enable(gempic_splitting *h, int enable)
+{
+    CHECK_H(h);
+    if (!enable) resolve_profile(h);
+    h->profile = enable != 0;
+    return GEMPIC_OK;
+}
+
+int gempic_profile_read(gempic_splitting *h, int kernel_id, double *total_ms, int64_t *count)
+{
+    CHECK_H(h);
+    if (kernel_id < 0 || kernel_id > 8) return fail(GEMPIC_ERR_ARGUMENT, "kernel id in 0..8");
+    resolve_profile(h);
+    if (total_ms) *total_ms = h->prof_ms[kernel_id];
+    if (count) *count = h->prof_cnt[kernel_id];
+    return GEMPIC_OK;
+}
+
+int gempic_profile_reset(gempic_splitting *h)
+{
+    CHECK_H(h);
+    resolve_profile(h);
+    for (int k = 0; k < 9; ++k) { h->prof_ms[k] = 0; h->prof_cnt[k] = 0; }
+    return GEMPIC_OK;
+}
+
+int gempic_set_fusion(gempic_splitting *h, int kernels_per_step)
+{
+    CHECK_H(h);
+    if (kernels_per_step != 1 && kernels_per_step != 2)
+        return fail(GEMPIC_ERR_ARGUMENT, "kernels_per_step must be 1 or 2");
+    h->kernels_per_step = kernels_per_step;
+    return GEMPIC_OK;
+}
+
+int gempic_set_launch(gempic_splitting *h, int blocks)
+{
+    CHECK_H(h);
+    if (blocks < 0) return fail(GEMPIC_ERR_ARGUMENT, "blocks must be >= 0");
+    if (blocks > h->max_blocks)
+        return fail(GEMPIC_ERR_ARGUMENT, "blocks exceeds the reduction scratch (8 per CU)");
+    h->override_blocks = blocks;
+    return GEMPIC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batched ParticleMeshCoupling1D verbs
+// ---------------------------------------------------------------------------
+static hipStream_t pmc_stream()
+{
+    static thread_local hipStream_t s = nullptr;
+    if (!s) (void)hipStreamCreate(&s);
+    return s;
+}
+
+static int pmc_run(int op, int64_t nx, double xmin, double xmax, int degree, int smoothing,
+                   int64_t n, const double *x, const double *x_new, const double *q, double qm,
+                   const double *field, int64_t field_len, double *out, double *acc, bool out_in)
+{
+    if (nx < 1 || nx > kNxMax) return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: nx must be in [1, 256]");
+    if (degree < 0 || degree > 5) return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: degree must be in 0..5");
+    if (smoothing != GEMPIC_GALERKIN && smoothing != GEMPIC_COLLOCATION)
+        return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: smoothing type not implemented");
+    if (n < 0) return fail(GEMPIC_ERR_ARGUMENT, "n must be >= 0");
+    if (!(xmax > xmin)) return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: xmax must exceed xmin");
+    int rc = check_device();
+    if (rc) return rc;
+    if (n == 0) return GEMPIC_OK;
+    hipStream_t s = pmc_stream();
+    PmcArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.n = n;
+    a.nx = (int)nx;
+    a.degree = degree;
+    a.xmin = xmin;
+    a.dx = (xmax - xmin) / (double)nx;
+    a.scaling = smoothing == GEMPIC_COLLOCATION ? 1.0 / a.dx : 1.0;
+    gh::gauss_legendre((degree + 2) / 2, a.qx, a.qw);
+    if (degree >= 1) gh::splinepp_tables(degree, a.pc, a.pf);
+    a.qm = qm;
+    const int max_blocks = device_cus() * 4;
+    Launch L = plan(n, max_blocks);   // k_pmc: at least 4 workgroups fit per CU
+    a.chunk = L.chunk;
+    a.iters = L.iters;
+    a.ngroups = L.ngroups;
+    std::vector<void *> bufs;
+    hipError_t e = hipSuccess;
+    auto upload = [&](const double *src, size_t cnt) -> double * {
+        if (!src) return nullptr;
+        double *d = nullptr;
+        if (e == hipSuccess) e = dalloc(&d, cnt);
+        if (e == hipSuccess) { bufs.push_back(d); e = hipMemcpyAsync(d, src, sizeof(double) * cnt, hipMemcpyHostToDevice, s); }
+        return d;
+    };
+    a.x = upload(x, n);
+    a.x_new = upload(x_new, n);
+    a.q = upload(q, n);
+    a.field = upload(field, (size_t)field_len);
+    double *d_out = nullptr, *d_acc = nullptr;
+    if (out) {
+        if (out_in) d_out = upload(out, n);
+        else { if (e == hipSuccess) e = dalloc(&d_out, n); if (e == hipSuccess) bufs.push_back(d_out); }
+    }
+    if (acc) d_acc = upload(acc, nx);
+    a.out = d_out;
+    a.acc = d_acc;
+    if (e == hipSuccess) e = dalloc(&a.part, (size_t)L.blocks * nx + 8);
+    if (e == hipSuccess) bufs.push_back(a.part);
+    if (e == hipSuccess) e = dalloc(&a.gpart, (size_t)L.ngroups * nx + 8);
+    if (e == hipSuccess) bufs.push_back(a.gpart);
+    if (e == hipSuccess) e = dalloc(&a.cnt, (size_t)L.ngroups + 2);
+    if (e == hipSuccess) { bufs.push_back(a.cnt); e = hipMemsetAsync(a.cnt, 0, sizeof(unsigned) * (L.ngroups + 2), s); }
+    if (e == hipSuccess) e = dalloc(&a.err, 1);
+    if (e == hipSuccess) { bufs.push_back(a.err); e = hipMemsetAsync(a.err, 0, sizeof(int), s); }
+    if (e == hipSuccess) e = launch_pmc(op, a, L.blocks, s);
+    if (e == hipSuccess && out) e = hipMemcpyAsync(out, d_out, sizeof(double) * n, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && acc) e = hipMemcpyAsync(acc, d_acc, sizeof(double) * nx, hipMemcpyDeviceToHost, s);
+    int err = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&err, a.err, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipStreamSynchronize(s);
+    for (void *p : bufs) (void)hipFree(p);
+    if (e != hipSuccess) return fail(GEMPIC_ERR_DEVICE, std::string("particle-mesh coupling: ") + hipGetErrorString(e));
+    if (err & E_INEXACT) return fail(GEMPIC_ERR_INEXACT, "InexactError: trunc(Int64, x): non-finite particle position");
+    if (err & E_FAR) return fail(GEMPIC_ERR_INEXACT, "a particle crossed more than 65536 cells in one push");
+    if (err & E_BOUNDS) return fail(GEMPIC_ERR_ARGUMENT, "BoundsError: position outside the pp matrix");
+    return GEMPIC_OK;
+}
+
+int gempic_pmc1d_add_charge(int64_t nx, double xmin, double xmax, int degree, int smoothing,
+                            int64_t n, const double *x, const double *q, double *rho)
+{
+    if ((n > 0 && (!x || !q)) || !rho) return fail(GEMPIC_ERR_ARGUMENT, "NULL array");
+    return pmc_run(PMC_CHARGE, nx, xmin, xmax, degree, smoothing, n, x, nullptr, q, 0.0, nullptr,
+                   0, nullptr, rho, false);
+}
+
+int gempic_pmc1d_add_charge_pp(int64_t nx, double xmin, double xmax, int degree, int smoothing,
+                               int64_t n, const double *x, const double *q, double *rho)
+{
+    if ((n > 0 && (!x || !q)) || !rho) return fail(GEMPIC_ERR_ARGUMENT, "NULL array");
+    if (degree < 1) return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: degree must be in 1..5 for SplinePP");
+    return pmc_run(PMC_CHARGE_PP, nx, xmin, xmax, degree, smoothing, n, x, nullptr, q, 0.0,
+                   nullptr, 0, nullptr, rho, false);
+}
+
+int gempic_pmc1d_evaluate(int64_t nx, double xmin, double xmax, int degree, int smoothing,
+                          int64_t n, const double *x, const double *field, double *out)
+{
+    if ((n > 0 && (!x || !out)) || !field) return fail(GEMPIC_ERR_ARGUMENT, "NULL array");
+    return pmc_run(PMC_EVAL, nx, xmin, xmax, degree, smoothing, n, x, nullptr, nullptr, 0.0,
+                   field, nx, out, nullptr, false);
+}
+
+int gempic_pmc1d_evaluate_pp(int64_t nx, double xmin, double xmax, int degree, int smoothing,
+                             int64_t n, const double *x, const double *field_pp, double *out)
+{
+    if ((n > 0 && (!x || !out)) || !field_pp) return fail(GEMPIC_ERR_ARGUMENT, "NULL array");
+    if (degree < 1) return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: degree must be in 1..5 for SplinePP");
+    return pmc_run(PMC_EVAL_PP, nx, xmin, xmax, degree, smoothing, n, x, nullptr, nullptr, 0.0,
+                   field_pp, (degree + 1) * nx, out, nullptr, false);
+}
+
+int gempic_pmc1d_add_current_update_v(int64_t nx, double xmin, double xmax, int degree,
+                                      int smoothing, int64_t n, const double *x_old,
+                                      const double *x_new, const double *q, double qm,
+                                      const double *bfield, double *v, double *j)
+{
+    if ((n > 0 && (!x_old || !x_new || !q || !v)) || !j) return fail(GEMPIC_ERR_ARGUMENT, "NULL array");
+    if (bfield)
+        return pmc_run(PMC_CURRENT_B, nx, xmin, xmax, degree, smoothing, n, x_old, x_new, q, qm,
+                       bfield, nx, v, j, true);
+    return pmc_run(PMC_CURRENT_NOB, nx, xmin, xmax, degree, smoothing, n, x_old, x_new, q, qm,
+                   nullptr, 0, nullptr, j, false);
+}
+
+int gempic_pmc1d_add_current_update_v_pp(int64_t nx, double xmin, double xmax, int degree,
+                                         int smoothing, int64_t n, const double *x_old,
+                                         const double *x_new, const double *q, double qm,
+                                         const double *bfield, double *v, double *j)
+{
+    if ((n > 0 && (!x_old || !x_new || !q || !v || !bfield)) || !j)
+        return fail(GEMPIC_ERR_ARGUMENT, "NULL array");
+    if (degree < 1) return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: degree must be in 1..5 for SplinePP");
+    return pmc_run(PMC_CURRENT_PP, nx, xmin, xmax, degree, smoothing, n, x_old, x_new, q, qm,
+                   bfield, nx, v, j, true);
+}
+
+// b_to_pp (splinepp.jl:241-261): a set-up conversion of nx coefficients,
+// done on the host.
+int gempic_b_to_pp(int degree, int64_t ncells, const double *b, double *pp)
+{
+    if (!b || !pp) return fail(GEMPIC_ERR_ARGUMENT, "NULL array");
+    if (degree < 1 || degree > 5) return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: degree not implemented");
+    if (ncells < degree) return fail(GEMPIC_ERR_ASSERTION, "AssertionError: ncells >= degree");
+    double pc[36], pf[36];
+    gh::splinepp_tables(degree, pc, pf);
+    const int dp1 = degree + 1;
+    double coeffs[6];
+    for (int64_t i = 1; i <= ncells; ++i) {
+        if (i <= degree) {
+            int c = 0;
+            for (int64_t k = ncells - degree + i; k <= ncells; ++k) coeffs[c++] = b[k - 1];
+            for (int64_t k = 1; k <= i; ++k) coeffs[c++] = b[k - 1];
+        } else {
+            for (int k = 0; k < dp1; ++k) coeffs[k] = b[i - degree - 1 + k];
+        }
+        for (int j = 0; j < dp1; ++j) {
+            double s = coeffs[0] * pc[j];
+            for (int k = 1; k < dp1; ++k) s = s + coeffs[k] * pc[j + k * dp1];
+            pp[j + (i - 1) * dp1] = s;
+        }
+    }
+    return GEMPIC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Maxwell1DFEM verbs
+// ---------------------------------------------------------------------------
+int gempic_maxwell1d_create(int64_t nx, double xmin, double xmax, int degree, gempic_maxwell1d **out)
+{
+    if (!out) return fail(GEMPIC_ERR_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    if (degree < 1 || degree > 3) return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: Wrong value of degree (1,2 or 3)");
+    if (nx < 2 || nx > kNxMax || (nx % 2) != 0) return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: nx must be even and in [2, 256]");
+    if (!(xmax > xmin)) return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: xmax must exceed xmin");
+    int rc = check_device();
+    if (rc) return rc;
+    gempic_maxwell1d *m = new gempic_maxwell1d();
+    m->nx = nx;
+    m->xmin = xmin;
+    m->xmax = xmax;
+    m->dx = (xmax - xmin) / (double)nx;
+    m->degree = degree;
+    gh::maxwell1d_tables(nx, degree, m->tabs);
+    hipError_t e = hipStreamCreate(&m->stream);
+    if (e == hipSuccess) e = dalloc(&m->d_grid, (size_t)NSLOT * nx);
+    if (e == hipSuccess) e = dalloc(&m->d_tabs, (size_t)NTAB * nx);
+    if (e == hipSuccess) e = dalloc(&m->d_scalar, 8);
+    if (e == hipSuccess) e = hipMemset(m->d_grid, 0, sizeof(double) * NSLOT * nx);
+    if (e == hipSuccess) {
+        std::vector<double> T((size_t)NTAB * nx, 0.0);
+        auto put = [&](int slot, const std::vector<double> &v) { std::copy(v.begin(), v.end(), T.begin() + (size_t)slot * nx); };
+        put(T_COS, m->tabs.cos_tab); put(T_SIN, m->tabs.sin_tab);
+        put(T_INVM0, m->tabs.inv_mass0); put(T_INVM1, m->tabs.inv_mass1);
+        put(T_AMP, m->tabs.eig_weak_ampere); put(T_POIS, m->tabs.eig_weak_poisson);
+        put(T_M0, m->tabs.eig_mass0); put(T_M1, m->tabs.eig_mass1);
+        e = hipMemcpy(m->d_tabs, T.data(), sizeof(double) * T.size(), hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) {
+        gempic_maxwell1d_destroy(m);
+        return fail(GEMPIC_ERR_DEVICE, std::string("Maxwell1DFEM allocation: ") + hipGetErrorString(e));
+    }
+    *out = m;
+    return GEMPIC_OK;
+}
+
+int gempic_maxwell1d_destroy(gempic_maxwell1d *m)
+{
+    if (!m) return GEMPIC_OK;
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    if (m->d_grid) (void)hipFree(m->d_grid);
+    if (m->d_tabs) (void)hipFree(m->d_tabs);
+    if (m->d_scalar) (void)hipFree(m->d_scalar);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+    return GEMPIC_OK;
+}
+
+int gempic_maxwell1d_tables(gempic_maxwell1d *m, double *e0, double *e1, double *ea, double *ep)
+{
+    if (!m) return fail(GEMPIC_ERR_ARGUMENT, "null handle");
+    const size_t nb = sizeof(double) * m->nx;
+    if (e0) std::memcpy(e0, m->tabs.eig_mass0.data(), nb);
+    if (e1) std::memcpy(e1, m->tabs.eig_mass1.data(), nb);
+    if (ea) std::memcpy(ea, m->tabs.eig_weak_ampere.data(), nb);
+    if (ep) std::memcpy(ep, m->tabs.eig_weak_poisson.data(), nb);
+    return GEMPIC_OK;
+}
+
+// run one field op: inputs uploaded to slots a/b/out, output slot downloaded
+static int mx_run(gempic_maxwell1d *m, int op, double dt, const double *in_a, const double *in_b,
+                  double *inout, double *scalar)
+{
+    if (!m) return fail(GEMPIC_ERR_ARGUMENT, "null handle");
+    const size_t nb = sizeof(double) * m->nx;
+    const int64_t nx = m->nx;
+    const int SA = S_J1, SB = S_J2, SO = S_E1;
+    if (in_a) HIP_TRY(hipMemcpyAsync(m->d_grid + SA * nx, in_a, nb, hipMemcpyHostToDevice, m->stream));
+    if (in_b) HIP_TRY(hipMemcpyAsync(m->d_grid + SB * nx, in_b, nb, hipMemcpyHostToDevice, m->stream));
+    if (inout) HIP_TRY(hipMemcpyAsync(m->d_grid + SO * nx, inout, nb, hipMemcpyHostToDevice, m->stream));
+    GridProg g = prog1(op, dt, SA, SB, SO);
+    HIP_TRY(launch_grid(g, (int)nx, m->dx, m->d_grid, m->d_tabs, m->d_scalar, m->stream));
+    if (inout) HIP_TRY(hipMemcpyAsync(inout, m->d_grid + SO * nx, nb, hipMemcpyDeviceToHost, m->stream));
+    if (scalar) HIP_TRY(hipMemcpyAsync(scalar, m->d_scalar, sizeof(double), hipMemcpyDeviceToHost, m->stream));
+    HIP_TRY(hipStreamSynchronize(m->stream));
+    return GEMPIC_OK;
+}
+
+int gempic_maxwell1d_compute_e_from_rho(gempic_maxwell1d *m, double *e, const double *rho)
+{
+    if (!e || !rho) return fail(GEMPIC_ERR_ARGUMENT, "NULL array");
+    return mx_run(m, G_E_FROM_RHO, 0.0, rho, nullptr, e, nullptr);
+}
+
+int gempic_maxwell1d_compute_e_from_j(gempic_maxwell1d *m, double *e, const double *current, int component)
+{
+    if (!e || !current) return fail(GEMPIC_ERR_ARGUMENT, "NULL array");
+    if (component != 1 && component != 2)
+        return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: Component not implemented");
+    return mx_run(m, component == 1 ? G_E_FROM_J1 : G_E_FROM_J2, 0.0, current, nullptr, e, nullptr);
+}
+
+int gempic_maxwell1d_compute_e_from_b(gempic_maxwell1d *m, double *e, double dt, const double *b)
+{
+    if (!e || !b) return fail(GEMPIC_ERR_ARGUMENT, "NULL array");
+    return mx_run(m, G_HB, dt, b, nullptr, e, nullptr);
+}
+
+int gempic_maxwell1d_compute_b_from_e(gempic_maxwell1d *m, double *b, double dt, const double *e)
+{
+    if (!e || !b) return fail(GEMPIC_ERR_ARGUMENT, "NULL array");
+    return mx_run(m, G_HE_B, dt, e, nullptr, b, nullptr);
+}
+
+int gempic_maxwell1d_inner_product(gempic_maxwell1d *m, const double *c1, const double *c2,
+                                   int degree, double *out)
+{
+    if (!c1 || !c2 || !out) return fail(GEMPIC_ERR_ARGUMENT, "NULL array");
+    if (!m) return fail(GEMPIC_ERR_ARGUMENT, "null handle");
+    int op;
+    if (degree == m->degree) op = G_INNER_M0;
+    else if (degree == m->degree - 1) op = G_INNER_M1;
+    else return fail(GEMPIC_ERR_ARGUMENT, "degree must be s_deg_0 or s_deg_1");
+    return mx_run(m, op, 0.0, c1, c2, nullptr, out);
+}
+
+int gempic_maxwell1d_l2norm_squared(gempic_maxwell1d *m, const double *c, int degree, double *out)
+{
+    return gempic_maxwell1d_inner_product(m, c, c, degree, out);
+}
+
+}  // extern "C"

@@ -1,0 +1,209 @@
+// device_common.hpp -- device helpers shared by the 1D particle-field kernels.
+//
+// Everything here is FP64 and compiled with -ffp-contract=off so that each
+// product and sum rounds exactly as in GEMPIC.jl's Julia code (no FMA
+// contraction); per-particle arithmetic is then bit-identical to the
+// reference's, and the only difference left is the summation order of the
+// charge/current deposits.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gk {
+
+constexpr int kWave = 64;
+
+// Base.mod(x::Float64, y::Float64) of Julia: r = rem(x, y) (exact);
+// zero takes y's sign; a nonzero r of the other sign gets y added.
+// Used at hamiltonian_splitting_1d2v.jl:79 and hamiltonian_splitting_1d1v.jl:91.
+// For y > 0 and |x| < 2y, rem(x, y) is x or x - sign(x)*y, and the latter is
+// exact (Sterbenz: y <= |x| < 2y), so the library fmod is only needed for
+// positions more than one period away.
+__device__ __forceinline__ double jl_mod(double x, double y)
+{
+    double r;
+    const double ax = fabs(x);
+    if (ax < y) r = x;
+    else if (ax < 2.0 * y) r = x - copysign(y, x);
+    else r = fmod(x, y);
+    if (r == 0.0) return copysign(0.0, y);
+    if ((r > 0.0) != (y > 0.0)) return r + y;
+    return r;
+}
+
+// floored integer modulus for n > 0; indices within one period of [0, n)
+// take a compare-and-add path instead of an integer division
+__device__ __forceinline__ int imod(int a, int n)
+{
+    if ((unsigned)a < (unsigned)n) return a;
+    if (a < 0 && a >= -n) return a + n;
+    if (a >= n && a < 2 * n) return a - n;
+    int r = a % n;
+    return r < 0 ? r + n : r;
+}
+
+// uniform_bsplines_eval_basis! (low_level_bsplines.jl:63-80): the P+1
+// non-zero uniform B-splines of degree P at offset `off`, in registers.
+template <int P>
+__device__ __forceinline__ void bspline(double off, double (&b)[P + 1])
+{
+    b[0] = 1.0;
+#pragma unroll
+    for (int j = 1; j <= P; ++j) {
+        double xx = -off;
+        const double jr = (double)j;
+        const double inv_j = 1.0 / jr;
+        double saved = 0.0;
+#pragma unroll
+        for (int r = 0; r < j; ++r) {
+            xx = xx + 1.0;
+            double temp = b[r] * inv_j;
+            b[r] = saved + xx * temp;
+            saved = (jr - xx) * temp;
+        }
+        b[j] = saved;
+    }
+}
+
+// Node index of the first of the P+1 nodes touched from cell `cell`
+// (reference: index - p .. index, each taken mod n).
+__device__ __forceinline__ int first_node(int cell, int P, int nx)
+{
+    // cell is in [0, nx)
+    int k = cell - P;
+    while (k < 0) k += nx;
+    return k;
+}
+
+__device__ __forceinline__ int next_node(int k, int nx)
+{
+    k += 1;
+    return k >= nx ? k - nx : k;
+}
+
+// Sum over the 64 lanes of a wave by an xor butterfly: every lane ends with
+// the bitwise-identical total (each stage adds the same two operands in
+// either order), so the result is deterministic.
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
+
+// ---------------------------------------------------------------------------
+// Deterministic deposit accumulation.
+//
+// Each lane keeps two cached cells (A, B) with the W = P+1 node sums of the
+// particles it has seen in each. Particles are kept sorted by cell, so a
+// lane's particles mostly fall in one cell and the odd one that has drifted
+// into a neighbour lands in the second entry. When a lane meets a third
+// cell, its least-recently-used entry is flushed: the wave runs a segmented
+// reduction over all lanes flushing the same cell (xor butterfly, fixed
+// order) and lane 0 adds the W sums into the wave's private LDS row. The
+// order of every addition is a fixed function of the particle order, so the
+// deposit is bit-reproducible from run to run.
+// ---------------------------------------------------------------------------
+template <int WN, int R = 1>
+struct Cache2 {
+    static constexpr int W = WN * R;
+    int ca, cb;
+    bool mru_b;
+    double a[W], b[W];
+
+    __device__ __forceinline__ void init()
+    {
+        ca = -1;
+        cb = -1;
+        mru_b = false;
+#pragma unroll
+        for (int w = 0; w < W; ++w) { a[w] = 0.0; b[w] = 0.0; }
+    }
+};
+
+// Segmented flush: lanes with `want` set contribute v[r*WN + w] to node
+// first_node(cell, WN-1) + w of row r (rows are nx doubles apart); called by
+// all 64 lanes of the wave.
+template <int WN, int R>
+__device__ __forceinline__ void seg_flush(bool want, int cell, const double (&v)[WN * R],
+                                          double *row, int nx)
+{
+    unsigned long long pending = __ballot(want);
+    const int lane = lane_id();
+    while (pending) {
+        int leader = __ffsll((long long)pending) - 1;
+        int c = __shfl(cell, leader, kWave);
+        bool mine = want && (cell == c);
+        unsigned long long m = __ballot(mine);
+        const int k0 = first_node(c, WN - 1, nx);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            int k = k0;
+#pragma unroll
+            for (int w = 0; w < WN; ++w) {
+                double s = wave_sum(mine ? v[r * WN + w] : 0.0);
+                if (lane == 0) row[r * nx + k] += s;
+                k = next_node(k, nx);
+            }
+        }
+        pending &= ~m;
+    }
+}
+
+template <int WN, int R>
+__device__ __forceinline__ void cache_add(Cache2<WN, R> &C, bool valid, int cell,
+                                          const double (&v)[WN * R], double *row, int nx)
+{
+    constexpr int W = WN * R;
+    bool hit_a = valid && cell == C.ca;
+    bool hit_b = valid && !hit_a && cell == C.cb;
+    bool miss = valid && !hit_a && !hit_b;
+    if (__ballot(miss)) {
+        bool use_a = miss && (C.ca < 0 || (C.cb >= 0 && C.mru_b));
+        bool use_b = miss && !use_a;
+        bool ev_a = use_a && C.ca >= 0;
+        bool ev_b = use_b && C.cb >= 0;
+        int ecell = ev_a ? C.ca : C.cb;
+        double ev[W];
+#pragma unroll
+        for (int w = 0; w < W; ++w) ev[w] = ev_a ? C.a[w] : C.b[w];
+        seg_flush<WN, R>(ev_a || ev_b, ecell, ev, row, nx);
+        if (use_a) {
+            C.ca = cell;
+#pragma unroll
+            for (int w = 0; w < W; ++w) C.a[w] = 0.0;
+            hit_a = true;
+        }
+        if (use_b) {
+            C.cb = cell;
+#pragma unroll
+            for (int w = 0; w < W; ++w) C.b[w] = 0.0;
+            hit_b = true;
+        }
+    }
+    // branches rather than selects: when every lane hits entry A (sorted
+    // particles) the B update is skipped for the whole wave
+    if (hit_a) {
+#pragma unroll
+        for (int w = 0; w < W; ++w) C.a[w] += v[w];
+        C.mru_b = false;
+    }
+    if (hit_b) {
+#pragma unroll
+        for (int w = 0; w < W; ++w) C.b[w] += v[w];
+        C.mru_b = true;
+    }
+}
+
+template <int WN, int R>
+__device__ __forceinline__ void cache_drain(Cache2<WN, R> &C, double *row, int nx)
+{
+    seg_flush<WN, R>(C.ca >= 0, C.ca, C.a, row, nx);
+    seg_flush<WN, R>(C.cb >= 0, C.cb, C.b, row, nx);
+    C.init();
+}
+
+}  // namespace gk

@@ -1,0 +1,136 @@
+// lib_internal.hpp -- declarations shared by the kernels (pass1d.hip) and
+// the C ABI (capi.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gl {
+
+constexpr int kBlock = 256;               // threads per workgroup of the particle passes
+constexpr int kWaves = kBlock / 64;
+constexpr int kNxMax = 256;               // largest 1D grid handled by the device path
+constexpr int kGroup = 16;                // workgroups per first-level reduction group
+
+// grid slots (nx doubles each) of a splitting's device field buffer
+enum GridSlot { S_E1 = 0, S_E2, S_B, S_E2P, S_J1, S_J2, S_RHO, S_EPOIS, S_E1N, S_E2N, NSLOT };
+// constant tables (nx doubles each)
+enum TabSlot { T_COS = 0, T_SIN, T_INVM0, T_INVM1, T_AMP, T_POIS, T_M0, T_M1, NTAB };
+// deposit slots of one pass
+enum DepSlot { D_J2A = 0, D_J1, D_RHO, D_J2B, NDEP };
+
+// Sub-step content of one particle pass.
+enum ModeBits : int {
+    M_HE_PREV = 1,   // HE (or 1d1v HB) of the previous Strang step, with E2 saved before it
+    M_HE = 2,        // HE (1d2v) / HB (1d1v) with the current fields
+    M_HP2A = 4,      // Hp2 before Hp1
+    M_HP1 = 8,       // Hp1 (+ rho deposit in 1d2v)
+    M_HP2B = 16,     // Hp2 after Hp1
+    M_V1 = 32,       // 1d1v variant
+    M_RHO = 64,      // charge deposit only (solve_poisson!)
+    M_DIAG = 128,    // write_step! particle sums
+    M_NOEPI = 256,   // write the deposit rows and stop (the next pass reduces them)
+    M_WITH_J2A = 512,  // also reduce the Hp2 rows written by the previous pass
+};
+
+struct PassArgs {
+    int64_t n;
+    int64_t chunk;       // particles per workgroup (multiple of kBlock)
+    int iters;           // chunk / kBlock
+    int nx;
+    int ngroups;         // number of reduction groups
+    int more;            // fused pass: another Strang step follows in this call
+    int degree;          // write_step! degree argument (diagnostics)
+    double xmin, dx, Lx;
+    double scaling0, scaling1;
+    double qx[3], qw[3];          // Gauss-Legendre rule of kernel_smoother_1
+    double charge, mass, cw, qm;
+    double dt_he, dt_hp2, dt_hp1;
+    double *x, *v1, *v2, *w;
+    double *grid;                 // NSLOT x nx
+    const double *tabs;           // NTAB x nx
+    double *part, *gpart;         // reduction partials
+    unsigned *cnt;                // ngroups + 1 counters (zero between launches)
+    int *err;                     // error flags
+    double *diag;                 // diagnostics output (11 doubles)
+};
+
+// error flag bits set by kernels
+enum ErrBits { E_INEXACT = 1, E_BOUNDS = 2, E_FAR = 4 };
+// largest number of cells a particle may cross in one Hp1 push; beyond it
+// the move is reported as an error instead of being integrated
+constexpr int kMaxCellsPerStep = 1 << 16;
+
+hipError_t launch_pass(int p0, int p1, int mode, const PassArgs &a, int blocks, size_t shmem,
+                       hipStream_t s);
+// workgroups of the (p0, p1, mode) pass kernel resident on one CU
+int pass_blocks_per_cu(int p0, int p1, int mode, size_t shmem);
+size_t pass_shmem_bytes(int nx);
+
+// Single-workgroup field program on the splitting's grid buffer.
+enum GridOp : int {
+    G_HB = 1,            // E2 += dt/dx * W(B)       (compute_e_from_b!)
+    G_HE_B,              // B  += dt/dx * (E2[i-1] - E2[i])  (compute_b_from_e!)
+    G_E_FROM_J1,         // E1 -= M1^{-1} J1 / dx
+    G_E_FROM_J2,         // E2 -= M0^{-1} J2 / dx
+    G_E_FROM_RHO,        // EPOIS from RHO           (compute_e_from_rho!)
+    G_INNER_M0,          // scalar: sum(c1 .* M0 c2) dx
+    G_INNER_M1,
+};
+struct GridProg {
+    int nops;
+    int op[8];
+    double dt[8];
+    int slot_a[8], slot_b[8], slot_out[8];   // generic operands for the Maxwell1DFEM verbs
+};
+hipError_t launch_grid(const GridProg &g, int nx, double dx, double *grid, const double *tabs,
+                       double *scalar_out, hipStream_t s);
+
+// Stable counting sort of SoA particles by cell.
+struct SortArgs {
+    int64_t n;
+    int nx;
+    int ntiles;
+    double xmin, dx;
+    const double *x, *v1, *v2, *w;
+    const int64_t *id;
+    double *x2, *v12, *v22, *w2;
+    int64_t *id2;
+    int *counts;      // ntiles x nx
+    int *offsets;     // ntiles x nx
+};
+constexpr int kSortTile = 64 * 32;     // particles per wave tile
+hipError_t launch_sort(const SortArgs &a, int has_v2, hipStream_t s);
+
+// AoS (reference ParticleGroup layout) <-> SoA with the permutation id.
+hipError_t launch_aos_to_soa(const double *aos, int64_t ld, int64_t n, int dim_v, double *x,
+                             double *v1, double *v2, double *w, int64_t *id, hipStream_t s);
+hipError_t launch_soa_to_aos(double *aos, int64_t ld, int64_t n, int dim_v, const double *x,
+                             const double *v1, const double *v2, const double *w,
+                             const int64_t *id, hipStream_t s);
+
+// Batched ParticleMeshCoupling1D verbs.
+struct PmcArgs {
+    int64_t n;
+    int64_t chunk;
+    int iters;
+    int nx;
+    int ngroups;
+    int degree;
+    double xmin, dx, scaling;
+    double qx[3], qw[3];
+    double pc[36], pf[36];        // SplinePP tables
+    double qm;
+    const double *x, *x_new, *q;
+    const double *field;          // field dofs / pp matrix / bfield
+    double *out;                  // evaluate output / v in-out
+    double *acc;                  // rho or j (accumulated into)
+    double *part, *gpart;
+    unsigned *cnt;
+    int *err;
+};
+enum PmcOp { PMC_EVAL = 0, PMC_EVAL_PP, PMC_CHARGE, PMC_CHARGE_PP, PMC_CURRENT_B, PMC_CURRENT_NOB,
+             PMC_CURRENT_PP };
+hipError_t launch_pmc(int op, const PmcArgs &a, int blocks, hipStream_t s);
+
+}  // namespace gl

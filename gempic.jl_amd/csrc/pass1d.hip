@@ -1,0 +1,1336 @@
+// pass1d.hip -- CDNA4 kernels of the 1D particle-field path.
+//
+// One kernel template (k_pass1d) runs a particle pass: it streams the SoA
+// particle arrays once, with the fields of the pass in LDS, applies the
+// sub-steps selected by MODE to each particle in registers, and accumulates
+// the charge/current deposits with the deterministic per-lane cell caches of
+// device_common.hpp. Deposits are reduced workgroup -> group -> grid in a
+// fixed order, and the last workgroup to finish runs the field solves of
+// the sub-steps (the "epilogue") so that a whole Strang step is one launch.
+#include "device_common.hpp"
+#include "lib_internal.hpp"
+
+using namespace gk;
+
+namespace gl {
+
+// ---------------------------------------------------------------------------
+// Field-side helpers run by one workgroup on LDS arrays
+// ---------------------------------------------------------------------------
+
+// solve_circulant! (maxwell_1d_fem.jl:222-240) with the FFTW halfcomplex
+// transform pair written out as direct sums over a twiddle table:
+// out = HC2R(eig (.) R2HC(rhs)) / n. Threads [t0, t0 + nt) of the
+// workgroup take part (several independent solves can run side by side);
+// `sync` is a barrier over the whole workgroup, so every thread calls this
+// with the same sequence of solves. Each sum runs left to right over j (or
+// k), with the twiddle index advanced incrementally instead of (j*k) % n.
+__device__ void circ_solve_part(const double *eig, const double *rhs, double *out, double *hc,
+                                const double *cos_t, const double *sin_t, int n, int t0, int nt)
+{
+    const int tid = (int)threadIdx.x - t0;
+    const bool mine = tid >= 0 && tid < nt;
+    if (mine)
+        for (int k = tid; k <= n / 2; k += nt) {
+            double re = 0.0, im = 0.0;
+            int t = 0;
+#pragma unroll 8
+            for (int j = 0; j < n; ++j) {
+                re += rhs[j] * cos_t[t];
+                im -= rhs[j] * sin_t[t];
+                t += k;
+                if (t >= n) t -= n;
+            }
+            hc[k] = re;
+            if (k > 0 && k < n - k) hc[n - k] = im;
+        }
+    __syncthreads();
+    if (mine)
+        for (int k = tid; k <= n / 2; k += nt) {
+            if (k == 0 || k == n / 2) {
+                hc[k] = hc[k] * eig[k];
+            } else {
+                double re_p = hc[k] * eig[k] - hc[n - k] * eig[n - k];
+                double im_p = hc[k] * eig[n - k] + hc[n - k] * eig[k];
+                hc[k] = re_p;
+                hc[n - k] = im_p;
+            }
+        }
+    __syncthreads();
+    if (mine)
+        for (int j = tid; j < n; j += nt) {
+            double s = hc[0];
+            int t = j;
+#pragma unroll 8
+            for (int k = 1; k < n - k; ++k) {
+                s += 2.0 * (hc[k] * cos_t[t] - hc[n - k] * sin_t[t]);
+                t += j;
+                if (t >= n) t -= n;
+            }
+            if ((n & 1) == 0) s += hc[n / 2] * ((j & 1) ? -1.0 : 1.0);
+            out[j] = s / (double)n;
+        }
+    __syncthreads();
+}
+
+__device__ void circ_solve(const double *eig, const double *rhs, double *out, double *hc,
+                           const double *cos_t, const double *sin_t, int n)
+{
+    circ_solve_part(eig, rhs, out, hc, cos_t, sin_t, n, 0, blockDim.x);
+}
+
+// Three independent solves side by side: thread group g (a third of the
+// workgroup) computes out[g] = HC2R(eig[g] (.) R2HC(rhs[g])) / n with the
+// same sums, in the same order, as circ_solve_part.
+struct Solve3 {
+    const double *eig[3];
+    const double *rhs[3];
+    double *out[3];
+    double *hc[3];
+};
+
+__device__ void circ_solve3(const Solve3 &S, const double *cos_t, const double *sin_t, int n)
+{
+    const int G = blockDim.x / 3;
+    const int g = (int)threadIdx.x / G;
+    const int tid = (int)threadIdx.x - g * G;
+    const bool mine = g < 3;
+    const double *eig = S.eig[mine ? g : 0];
+    const double *rhs = S.rhs[mine ? g : 0];
+    double *out = S.out[mine ? g : 0];
+    double *hc = S.hc[mine ? g : 0];
+    if (mine)
+        for (int k = tid; k <= n / 2; k += G) {
+            double re = 0.0, im = 0.0;
+            int t = 0;
+#pragma unroll 8
+            for (int j = 0; j < n; ++j) {
+                re += rhs[j] * cos_t[t];
+                im -= rhs[j] * sin_t[t];
+                t += k;
+                if (t >= n) t -= n;
+            }
+            hc[k] = re;
+            if (k > 0 && k < n - k) hc[n - k] = im;
+        }
+    __syncthreads();
+    if (mine)
+        for (int k = tid; k <= n / 2; k += G) {
+            if (k == 0 || k == n / 2) {
+                hc[k] = hc[k] * eig[k];
+            } else {
+                double re_p = hc[k] * eig[k] - hc[n - k] * eig[n - k];
+                double im_p = hc[k] * eig[n - k] + hc[n - k] * eig[k];
+                hc[k] = re_p;
+                hc[n - k] = im_p;
+            }
+        }
+    __syncthreads();
+    if (mine)
+        for (int j = tid; j < n; j += G) {
+            double s = hc[0];
+            int t = j;
+#pragma unroll 8
+            for (int k = 1; k < n - k; ++k) {
+                s += 2.0 * (hc[k] * cos_t[t] - hc[n - k] * sin_t[t]);
+                t += j;
+                if (t >= n) t -= n;
+            }
+            if ((n & 1) == 0) s += hc[n / 2] * ((j & 1) ? -1.0 : 1.0);
+            out[j] = s / (double)n;
+        }
+    __syncthreads();
+}
+
+// compute_e_from_j! (maxwell_1d_fem.jl:263-289): E -= M^{-1} j / dx
+__device__ void ep_e_from_j(double *E, const double *j, const double *inv_mass, double dx,
+                            double *work, double *hc, const double *tabs, int n)
+{
+    circ_solve(inv_mass, j, work, hc, tabs + T_COS * n, tabs + T_SIN * n, n);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        double wv = work[i] / dx;
+        E[i] -= wv;
+    }
+    __syncthreads();
+}
+
+// compute_b_from_e! (maxwell_1d_fem.jl:407-420): B[i] += dt/dx (E[i-1] - E[i])
+__device__ void ep_b_from_e(double *B, const double *E, double dt, double dx, int n)
+{
+    const double coef = dt / dx;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        int im = i == 0 ? n - 1 : i - 1;
+        B[i] = B[i] + coef * (E[im] - E[i]);
+    }
+    __syncthreads();
+}
+
+// compute_e_from_b! (maxwell_1d_fem.jl:384-396): E += dt/dx W(B)
+__device__ void ep_e_from_b(double *E, const double *B, double dt, double dx, double *work,
+                            double *hc, const double *tabs, int n)
+{
+    const double coef = dt / dx;
+    circ_solve(tabs + T_AMP * n, B, work, hc, tabs + T_COS * n, tabs + T_SIN * n, n);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) E[i] += coef * work[i];
+    __syncthreads();
+}
+
+// compute_e_from_rho! (maxwell_1d_fem.jl:244-255)
+__device__ void ep_e_from_rho(double *E, const double *rho, double *work, double *hc,
+                              const double *tabs, int n)
+{
+    circ_solve(tabs + T_POIS * n, rho, work, hc, tabs + T_COS * n, tabs + T_SIN * n, n);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        int im = i == 0 ? n - 1 : i - 1;
+        E[i] = work[im] - work[i];
+    }
+    __syncthreads();
+}
+
+// sum(c1 .* solve_circulant(eig, c2)) * dx (inner_product, maxwell_1d_fem.jl:461-475),
+// summed left to right by one thread.
+__device__ double ep_inner(const double *c1, const double *c2, const double *eig, double dx,
+                           double *work, double *hc, const double *tabs, int n)
+{
+    circ_solve(eig, c2, work, hc, tabs + T_COS * n, tabs + T_SIN * n, n);
+    __shared__ double s_r;
+    if (threadIdx.x == 0) {
+        double r = 0.0;
+        for (int i = 0; i < n; ++i) r += c1[i] * work[i];
+        s_r = r * dx;
+    }
+    __syncthreads();
+    double r = s_r;
+    __syncthreads();
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic grid-wide reduction of the per-workgroup deposit rows.
+// Workgroups are grouped by kGroup; the last one to finish in a group sums
+// the group's rows, and the last group-finisher sums the group rows into
+// `red` (LDS). Every sum runs in a fixed order that depends only on the
+// launch geometry. Returns true only in that final workgroup. Counters are
+// reset for the next launch.
+// ---------------------------------------------------------------------------
+
+// out[idx] = sum over rows r in [0, nrows) of base[r*len + idx], for the
+// entries idx whose deposit slot (idx / nx) is set in depmask. The rows are
+// split into `k` contiguous chunks summed by k threads (loads issued eight at
+// a time), then the k chunk sums are added in chunk order. `scratch` is LDS
+// of at least blockDim.x doubles. Called by all threads of the workgroup.
+__device__ void sum_rows(const double *base, int nrows, int len, unsigned depmask, int nx,
+                         double *out, double *scratch)
+{
+    const int tid = threadIdx.x;
+    const int nt = blockDim.x;
+    const int k = len >= nt ? 1 : nt / len;
+    for (int idx0 = 0; idx0 < len; idx0 += nt / k) {
+        const int idx = idx0 + tid % (nt / k);
+        const int h = tid / (nt / k);
+        double s = 0.0;
+        const bool live = idx < len && h < k && ((depmask >> (idx / nx)) & 1u);
+        if (live) {
+            const int r0 = (int)(((long)nrows * h) / k);
+            const int r1 = (int)(((long)nrows * (h + 1)) / k);
+            int r = r0;
+            for (; r + 8 <= r1; r += 8) {
+                double t[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) t[u] = base[(size_t)(r + u) * len + idx];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) s += t[u];
+            }
+            for (; r < r1; ++r) s += base[(size_t)r * len + idx];
+        }
+        scratch[tid] = s;
+        __syncthreads();
+        if (h == 0 && idx < len) {
+            double tot = 0.0;
+            if ((depmask >> (idx / nx)) & 1u)
+                for (int hh = 0; hh < k; ++hh) tot += scratch[hh * (nt / k) + tid];
+            out[idx] = tot;
+        }
+        __syncthreads();
+    }
+}
+
+__device__ bool grid_reduce(double *part, double *gpart, unsigned *cnt, int ngroups, int len,
+                            unsigned depmask, int nx, double *red, double *scratch)
+{
+    __shared__ int s_last;
+    const int tid = threadIdx.x;
+    __threadfence();
+    __syncthreads();
+    const int g = blockIdx.x / kGroup;
+    const int g0 = g * kGroup;
+    const int gn = min(kGroup, (int)gridDim.x - g0);
+    if (tid == 0) {
+        unsigned prev = atomicAdd(&cnt[g], 1u);
+        s_last = (prev == (unsigned)(gn - 1));
+    }
+    __syncthreads();
+    if (!s_last) return false;
+    __threadfence();
+    sum_rows(part + (size_t)g0 * len, gn, len, depmask, nx, red, scratch);
+    for (int idx = tid; idx < len; idx += blockDim.x) gpart[(size_t)g * len + idx] = red[idx];
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) {
+        cnt[g] = 0u;
+        unsigned prev = atomicAdd(&cnt[ngroups], 1u);
+        s_last = (prev == (unsigned)(ngroups - 1));
+    }
+    __syncthreads();
+    if (!s_last) return false;
+    __threadfence();
+    sum_rows(gpart, ngroups, len, depmask, nx, red, scratch);
+    for (int idx = tid; idx < len; idx += blockDim.x)
+        if (!((depmask >> (idx / nx)) & 1u)) red[idx] = 0.0;
+    if (tid == 0) cnt[ngroups] = 0u;
+    __syncthreads();
+    return true;
+}
+
+// Copy the constant tables into LDS (all threads).
+__device__ void load_tabs(const double *tabs, double *lds, int nx)
+{
+    for (int i = threadIdx.x; i < NTAB * nx; i += blockDim.x) lds[i] = tabs[i];
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Per-particle pieces
+// ---------------------------------------------------------------------------
+
+// field value from precomputed basis values at cell `cell` (evaluate,
+// particle_mesh_coupling_1d.jl:438-453)
+template <int P>
+__device__ __forceinline__ double gather(const double (&b)[P + 1], const double *f, int cell, int nx)
+{
+    int k = first_node(cell, P, nx);
+    double v = 0.0;
+#pragma unroll
+    for (int i = 0; i <= P; ++i) {
+        v += f[k] * b[i];
+        k = next_node(k, nx);
+    }
+    return v;
+}
+
+// Fields copied to LDS with kPad wrapped nodes on each side: node k of the
+// periodic grid, k in [-kPad, nx + kPad), sits at fp[k] with fp = base + kPad.
+constexpr int kPad = 4;
+
+__device__ __forceinline__ void load_padded(double *dst, const double *src, int nx)
+{
+    for (int i = threadIdx.x; i < nx + 2 * kPad; i += blockDim.x) {
+        int k = i - kPad;
+        while (k < 0) k += nx;
+        while (k >= nx) k -= nx;
+        dst[i] = src[k];
+    }
+}
+
+// evaluate with basis values b at cell `cell` in [0, nx): nodes cell-P..cell
+template <int P>
+__device__ __forceinline__ double gather_p(const double (&b)[P + 1], const double *fp, int cell)
+{
+    const double *f = fp + (cell - P);
+    double v = 0.0;
+#pragma unroll
+    for (int i = 0; i <= P; ++i) v += f[i] * b[i];
+    return v;
+}
+
+// one update_jv! segment (particle_mesh_coupling_1d.jl:385-414): the
+// Gauss-Legendre line integral of the degree-P1 splines over [lower, upper],
+// scaled by sign*dx.
+template <int P1, int NQ>
+__device__ __forceinline__ void segment(double lower, double upper, double sign, const double *qx,
+                                        const double *qw, double dx, double (&sv)[P1 + 1])
+{
+    double c1 = 0.5 * (upper - lower);
+    double c2 = 0.5 * (upper + lower);
+    bspline<P1>(c1 * qx[0] + c2, sv);
+    const double s0 = qw[0] * c1;
+#pragma unroll
+    for (int k = 0; k <= P1; ++k) sv[k] = sv[k] * s0;
+#pragma unroll
+    for (int jq = 1; jq < NQ; ++jq) {
+        double svm[P1 + 1];
+        bspline<P1>(c1 * qx[jq] + c2, svm);
+#pragma unroll
+        for (int k = 0; k <= P1; ++k) sv[k] = sv[k] + svm[k] * qw[jq] * c1;
+    }
+    const double s1 = sign * dx;
+#pragma unroll
+    for (int k = 0; k <= P1; ++k) sv[k] = sv[k] * s1;
+}
+
+// Segment s of add_current_update_v! (:296-376 / :471-529): the old cell,
+// the new cell, then the cells in between, each with its integration bounds.
+__device__ __forceinline__ void segment_bounds(int s, int io, double ro, int inw, double rn,
+                                               double &lo, double &up, double &sg, int &cell)
+{
+    if (io == inw) {
+        cell = io;
+        if (ro < rn) { lo = ro; up = rn; sg = 1.0; }
+        else { lo = rn; up = ro; sg = -1.0; }
+    } else if (io < inw) {
+        sg = 1.0;
+        if (s == 0) { lo = ro; up = 1.0; cell = io; }
+        else if (s == 1) { lo = 0.0; up = rn; cell = inw; }
+        else { lo = 0.0; up = 1.0; cell = io + s - 1; }
+    } else {
+        sg = -1.0;
+        if (s == 0) { lo = rn; up = 1.0; cell = inw; }
+        else if (s == 1) { lo = 0.0; up = ro; cell = io; }
+        else { lo = 0.0; up = 1.0; cell = inw + s - 1; }
+    }
+}
+
+__device__ __forceinline__ bool pos_ok(double xi) { return fabs(xi) < 1.0e9; }
+
+// ---------------------------------------------------------------------------
+// The particle pass
+// ---------------------------------------------------------------------------
+template <int P0, int P1, int MODE>
+__global__ void __launch_bounds__(kBlock) k_pass1d(PassArgs a)
+{
+    constexpr bool HE_PREV = (MODE & M_HE_PREV) != 0;
+    constexpr bool HE = (MODE & M_HE) != 0;
+    constexpr bool HP2A = (MODE & M_HP2A) != 0;
+    constexpr bool HP1 = (MODE & M_HP1) != 0;
+    constexpr bool HP2B = (MODE & M_HP2B) != 0;
+    constexpr bool V1 = (MODE & M_V1) != 0;
+    constexpr bool RHO_ONLY = (MODE & M_RHO) != 0;
+    constexpr bool NOEPI = (MODE & M_NOEPI) != 0;
+    constexpr bool WITH_J2A = (MODE & M_WITH_J2A) != 0;
+    constexpr bool DEP_RHO = (HP1 && !V1) || RHO_ONLY;
+    constexpr bool NEED_NEW = DEP_RHO || HP2B;           // work at the post-Hp1 position
+    constexpr int RR = HP2B ? 2 : 1;                     // rows sharing the new-position cache
+    constexpr unsigned DEPMASK = (HP2A ? 1u << D_J2A : 0u) | (HP1 ? 1u << D_J1 : 0u) |
+                                 (DEP_RHO ? 1u << D_RHO : 0u) | (HP2B ? 1u << D_J2B : 0u);
+    constexpr unsigned REDMASK = DEPMASK | (WITH_J2A ? 1u << D_J2A : 0u);
+    constexpr int NQ = (P1 + 2) / 2;
+    constexpr int W0 = P0 + 1, W1 = P1 + 1;
+    static_assert(P0 <= kPad && P1 <= kPad, "LDS field padding must cover the spline support");
+
+    extern __shared__ double smem[];
+    const int nx = a.nx;
+    const int tid = threadIdx.x;
+    const int wave = tid / kWave;
+    const int nxp = nx + 2 * kPad;
+    double *pE1 = smem;                                   // padded LDS copies of the fields
+    double *pE2p = pE1 + nxp;
+    double *pE2 = pE2p + nxp;
+    double *pB = pE2 + nxp;
+    double *sRed = pB + nxp;                              // [NDEP][nx]
+    double *sWork = sRed + NDEP * nx;                     // [2][nx]
+    double *rows = sWork + 2 * nx;                        // [kWaves][NDEP][nx], later scratch
+    // unpadded views (node k at s*[k]) for the epilogue's field solves
+    double *sE1 = pE1 + kPad, *sE2p = pE2p + kPad, *sE2 = pE2 + kPad, *sB = pB + kPad;
+
+    load_padded(pE1, a.grid + S_E1 * nx, nx);
+    load_padded(pE2p, a.grid + S_E2P * nx, nx);
+    load_padded(pE2, a.grid + S_E2 * nx, nx);
+    load_padded(pB, a.grid + S_B * nx, nx);
+    for (int i = tid; i < kWaves * NDEP * nx; i += kBlock) rows[i] = 0.0;
+    __syncthreads();
+    double *myrow = rows + wave * NDEP * nx;
+
+    Cache2<W0> c_j2a;
+    Cache2<W1> c_j1;
+    Cache2<W0, RR> c_new;
+    c_j2a.init();
+    c_j1.init();
+    c_new.init();
+
+    const double xmin = a.xmin, dx = a.dx, qm = a.qm;
+    const int64_t base = (int64_t)blockIdx.x * a.chunk;
+    int errs = 0;
+
+    // software pipeline: the loads of iteration it+1 are issued before the
+    // work of iteration it
+    double nx_x = 0.0, nx_v1 = 0.0, nx_v2 = 0.0, nx_w = 0.0;
+    {
+        const int64_t i0 = base + tid;
+        if (a.iters > 0 && i0 < a.n) {
+            nx_x = a.x[i0];
+            nx_w = a.w[i0];
+            if (!RHO_ONLY) nx_v1 = a.v1[i0];
+            if (!V1 && !RHO_ONLY) nx_v2 = a.v2[i0];
+        }
+    }
+    for (int it = 0; it < a.iters; ++it) {
+        const int64_t i = base + (int64_t)it * kBlock + tid;
+        const bool valid = i < a.n;
+        double x = nx_x, v1 = nx_v1, v2 = nx_v2, w = nx_w;
+        {
+            const int64_t in = i + kBlock;
+            if (it + 1 < a.iters && in < a.n) {
+                nx_x = a.x[in];
+                nx_w = a.w[in];
+                if (!RHO_ONLY) nx_v1 = a.v1[in];
+                if (!V1 && !RHO_ONLY) nx_v2 = a.v2[in];
+            }
+        }
+        // marker charge, as (w*charge)*common_weight (hamiltonian_splitting_1d2v.jl:62-64)
+        const double q = (w * a.charge) * a.cw;
+
+        double xi = (x - xmin) / dx;
+        if (valid && !pos_ok(xi)) errs |= E_INEXACT;
+        const int ic = (int)xi;                          // trunc(Int, xi)
+        const double off = xi - (double)ic;
+        const int key_old = imod(ic, nx);
+
+        double b1[W1];
+        double b0[W0];
+        if constexpr (HE_PREV || HE || HP2A) bspline<P1>(off, b1);
+        if constexpr ((HE_PREV || HE || HP2A) && !V1) bspline<P0>(off, b0);
+
+        if constexpr (V1 && (HE_PREV || HE)) {
+            // operatorHB 1d1v (hamiltonian_splitting_1d1v.jl:113-125): v += dt * E1(x)
+            const double e1 = gather_p<P1>(b1, sE1, key_old);
+            if (HE_PREV) v1 = v1 + a.dt_he * e1;
+            if (HE) v1 = v1 + a.dt_he * e1;
+        }
+        if constexpr (!V1 && (HE_PREV || HE)) {
+            // operatorHE 1d2v (hamiltonian_splitting_1d2v.jl:198-215)
+            const double e1 = gather_p<P1>(b1, sE1, key_old);
+            const double dtqm = a.dt_he * qm;
+            if (HE_PREV) {
+                const double e2 = gather_p<P0>(b0, sE2p, key_old);
+                v1 = v1 + dtqm * e1;
+                v2 = v2 + dtqm * e2;
+            }
+            if (HE) {
+                const double e2 = gather_p<P0>(b0, sE2, key_old);
+                v1 = v1 + dtqm * e1;
+                v2 = v2 + dtqm * e2;
+            }
+        }
+        if constexpr (HP2A) {
+            // operatorHp2 (hamiltonian_splitting_1d2v.jl:146-163)
+            const double bf = gather_p<P1>(b1, sB, key_old);
+            v1 = v1 + a.dt_hp2 * qm * v2 * bf;
+            const double wq = q * v2;
+            double vals[W0];
+#pragma unroll
+            for (int k = 0; k < W0; ++k) vals[k] = wq * b0[k] * a.scaling0;
+            cache_add<W0, 1>(c_j2a, valid, key_old, vals, myrow + D_J2A * nx, nx);
+        }
+        if constexpr (HP1) {
+            // operatorHp1 (hamiltonian_splitting_1d2v.jl:53-82 / _1d1v.jl:308-329)
+            const double xn = x + a.dt_hp1 * v1;
+            const double xin = (xn - xmin) / dx;
+            if (valid && !pos_ok(xin)) errs |= E_INEXACT;
+            const int inw = V1 ? (int)floor(xin) : (int)xin;
+            const double rn = xin - (double)inw;
+            const int dseg = inw - ic;
+            // a move across more than kMaxCellsPerStep cells (or a non-finite
+            // position) is reported instead of integrated segment by segment
+            const bool far = !pos_ok(xin) || !pos_ok(xi) || dseg > kMaxCellsPerStep ||
+                             dseg < -kMaxCellsPerStep;
+            if (valid && far) errs |= E_FAR;
+            const int nseg = far ? 1 : 1 + (dseg < 0 ? -dseg : dseg);
+            for (int s = 0;; ++s) {
+                const bool act = valid && s < nseg;
+                if (!__ballot(act)) break;
+                double lo, up, sg;
+                int cell;
+                segment_bounds(s, ic, off, inw, rn, lo, up, sg, cell);
+                double sv[W1];
+                segment<P1, NQ>(lo, up, sg, a.qx, a.qw, dx, sv);
+                const int key = imod(cell, nx);
+                double vals[W1];
+                const double *bk = sB + (key - P1);
+                double v2s = v2;
+#pragma unroll
+                for (int m = 0; m < W1; ++m) {
+                    vals[m] = q * sv[m] * a.scaling1;
+                    if (!V1) v2s = v2s - qm * sv[m] * bk[m];
+                }
+                if (act) v2 = v2s;
+                cache_add<W1, 1>(c_j1, act, key, vals, myrow + D_J1 * nx, nx);
+            }
+            x = jl_mod(xn, a.Lx);
+        }
+        if constexpr (NEED_NEW) {
+            // rho deposit at the new position (hamiltonian_splitting_1d2v.jl:93-102,
+            // diagnostics.jl:189-193) and the second Hp2
+            const double xi2 = (x - xmin) / dx;
+            if (valid && RHO_ONLY && !pos_ok(xi2)) errs |= E_INEXACT;
+            const int ic2 = (int)xi2;
+            const double off2 = xi2 - (double)ic2;
+            const int key2 = imod(ic2, nx);
+            double c0[W0];
+            bspline<P0>(off2, c0);
+            if constexpr (HP2B) {
+                double c1[W1];
+                bspline<P1>(off2, c1);
+                const double bf = gather_p<P1>(c1, sB, key2);
+                v1 = v1 + a.dt_hp2 * qm * v2 * bf;
+            }
+            double vals[W0 * RR];
+#pragma unroll
+            for (int k = 0; k < W0; ++k) vals[k] = q * c0[k] * a.scaling0;
+            if constexpr (HP2B) {
+                const double wq = q * v2;
+#pragma unroll
+                for (int k = 0; k < W0; ++k) vals[W0 + k] = wq * c0[k] * a.scaling0;
+            }
+            cache_add<W0, RR>(c_new, valid, key2, vals, myrow + D_RHO * nx, nx);
+        }
+        if (valid) {
+            if constexpr (HP1) a.x[i] = x;
+            if constexpr (HE_PREV || HE || HP2A || HP2B) a.v1[i] = v1;
+            if constexpr (!V1 && (HE_PREV || HE || HP1)) a.v2[i] = v2;
+        }
+    }
+    if (errs) atomicOr(a.err, errs);
+
+    if constexpr (DEPMASK != 0u) {
+        if constexpr (HP2A) cache_drain<W0, 1>(c_j2a, myrow + D_J2A * nx, nx);
+        if constexpr (HP1) cache_drain<W1, 1>(c_j1, myrow + D_J1 * nx, nx);
+        if constexpr (NEED_NEW) cache_drain<W0, RR>(c_new, myrow + D_RHO * nx, nx);
+        __syncthreads();
+        const int len = NDEP * nx;
+        for (int idx = tid; idx < len; idx += kBlock) {
+            if (!((DEPMASK >> (idx / nx)) & 1u)) continue;
+            double s = 0.0;
+            for (int wv = 0; wv < kWaves; ++wv) s += rows[wv * len + idx];
+            a.part[(size_t)blockIdx.x * len + idx] = s;
+        }
+        if constexpr (NOEPI) return;
+        // rows are dead once written out: reuse them as reduction scratch, then
+        // as the LDS copy of the constant tables
+        if (!grid_reduce(a.part, a.gpart, a.cnt, a.ngroups, len, REDMASK, nx, sRed, rows)) return;
+
+        // ---- epilogue: field solves of the sub-steps, in the reference's order
+        double *work = sWork, *hc = sWork + nx;
+        load_tabs(a.tabs, rows, nx);
+        const double *T = rows;
+        double *r_j2a = sRed + D_J2A * nx, *r_j1 = sRed + D_J1 * nx;
+        double *r_rho = sRed + D_RHO * nx, *r_j2b = sRed + D_J2B * nx;
+        double *G = a.grid;
+        if constexpr (RHO_ONLY) {
+            ep_e_from_rho(work + 0, r_rho, sE2p, hc, T, nx);   // sE2p reused as scratch
+            for (int i = tid; i < nx; i += kBlock) {
+                G[S_RHO * nx + i] = r_rho[i];
+                G[S_EPOIS * nx + i] = work[i];
+            }
+        } else if constexpr (V1) {
+            // 1d1v Hp1: j_dofs[1] = j, j_dofs[2] = 0; E1 -= M1^{-1} j / dx
+            for (int i = tid; i < nx; i += kBlock) {
+                G[S_J1 * nx + i] = r_j1[i];
+                G[S_J2 * nx + i] = 0.0;
+            }
+            ep_e_from_j(sE1, r_j1, T + T_INVM1 * nx, dx, work, hc, T, nx);
+            for (int i = tid; i < nx; i += kBlock) G[S_E1 * nx + i] = sE1[i];
+        } else if constexpr (NOEPI) {
+            // unreachable: returned above
+        } else if constexpr (HP2A && !HP1) {
+            // operatorHp2 alone (:169-175)
+            for (int i = tid; i < nx; i += kBlock) r_j2a[i] = r_j2a[i] * a.dt_hp2;
+            __syncthreads();
+            ep_e_from_j(sE2, r_j2a, T + T_INVM0 * nx, dx, work, hc, T, nx);
+            for (int i = tid; i < nx; i += kBlock) {
+                G[S_J1 * nx + i] = 0.0;
+                G[S_J2 * nx + i] = r_j2a[i];
+                G[S_E2 * nx + i] = sE2[i];
+            }
+        } else if constexpr (HP1 && !HP2A && !HP2B) {
+            // operatorHp1 alone (:88-111)
+            for (int i = tid; i < nx; i += kBlock) {
+                G[S_J1 * nx + i] = r_j1[i];
+                G[S_J2 * nx + i] = r_rho[i];
+            }
+            ep_e_from_j(sE1, r_j1, T + T_INVM1 * nx, dx, work, hc, T, nx);
+            for (int i = tid; i < nx; i += kBlock) G[S_E1 * nx + i] = sE1[i];
+        } else {
+            // fused Strang step: Hp2 (1/2) - Hp1 - Hp2 (1/2) field solves, then
+            // HE's B update and HB of this step, and (if another step follows)
+            // HB and HE's B update of the next step.
+            // The three inverse-mass solves (compute_e_from_j! of Hp2, Hp1,
+            // Hp2) are independent: run them side by side, then apply the
+            // updates in the reference's order, E2 -= w_a / dx, E1 -= w_1 / dx,
+            // E2 -= w_b / dx.
+            for (int i = tid; i < nx; i += kBlock) {
+                r_j2a[i] = r_j2a[i] * a.dt_hp2;
+                r_j2b[i] = r_j2b[i] * a.dt_hp2;
+            }
+            __syncthreads();
+            double *w3 = rows + NTAB * nx;          // free LDS after the tables: 6 nx doubles
+            Solve3 S;
+            S.eig[0] = T + T_INVM0 * nx; S.rhs[0] = r_j2a; S.out[0] = w3;          S.hc[0] = w3 + nx;
+            S.eig[1] = T + T_INVM1 * nx; S.rhs[1] = r_j1;  S.out[1] = w3 + 2 * nx; S.hc[1] = w3 + 3 * nx;
+            S.eig[2] = T + T_INVM0 * nx; S.rhs[2] = r_j2b; S.out[2] = w3 + 4 * nx; S.hc[2] = w3 + 5 * nx;
+            circ_solve3(S, T + T_COS * nx, T + T_SIN * nx, nx);
+            for (int i = tid; i < nx; i += kBlock) {
+                double wa = S.out[0][i] / dx;
+                sE2[i] -= wa;
+                double w1 = S.out[1][i] / dx;
+                sE1[i] -= w1;
+                double wb = S.out[2][i] / dx;
+                sE2[i] -= wb;
+            }
+            __syncthreads();
+            for (int i = tid; i < nx; i += kBlock) {
+                G[S_J1 * nx + i] = 0.0;
+                G[S_J2 * nx + i] = r_j2b[i];
+                G[S_RHO * nx + i] = r_rho[i];
+                G[S_E2P * nx + i] = sE2[i];
+            }
+            ep_b_from_e(sB, sE2, a.dt_he, dx, nx);
+            ep_e_from_b(sE2, sB, a.dt_he, dx, work, hc, T, nx);
+            if (a.more) {
+                ep_e_from_b(sE2, sB, a.dt_he, dx, work, hc, T, nx);
+                ep_b_from_e(sB, sE2, a.dt_he, dx, nx);
+            }
+            for (int i = tid; i < nx; i += kBlock) {
+                G[S_E1 * nx + i] = sE1[i];
+                G[S_E2 * nx + i] = sE2[i];
+                G[S_B * nx + i] = sB[i];
+            }
+        }
+    } else if constexpr (HE && !V1 && !HE_PREV) {
+        // operatorHE alone: compute_b_from_e! after the push (:217-218). B is
+        // not read by this pass, so workgroup 0 can update it at any time.
+        if (blockIdx.x == 0) {
+            ep_b_from_e(sB, sE2, a.dt_he, dx, nx);
+            for (int i = tid; i < nx; i += kBlock) a.grid[S_B * nx + i] = sB[i];
+        }
+    }
+}
+
+size_t pass_shmem_bytes(int nx)
+{
+    // particle pass: padded fields, reduced deposits, epilogue work, then the
+    // per-wave rows reused as reduction scratch and LDS tables
+    size_t rows = (size_t)kWaves * NDEP * nx;
+    if (rows < (size_t)kBlock) rows = kBlock;
+    if (rows < (size_t)(NTAB + 6) * nx) rows = (size_t)(NTAB + 6) * nx;
+    const size_t pass = (size_t)4 * (nx + 2 * kPad) + (size_t)nx * (NDEP + 2) + rows;
+    // diagnostics: 5 fields, work, hc, 8 sums, scratch, then scratch/tables
+    size_t tail = (size_t)NTAB * nx;
+    if (tail < (size_t)kBlock) tail = kBlock;
+    const size_t diag = (size_t)nx * 8 + 8 + tail;
+    return sizeof(double) * (pass > diag ? pass : diag);
+}
+
+// ---------------------------------------------------------------------------
+// write_step! particle sums (diagnostics.jl:186-250) + field diagnostics
+// ---------------------------------------------------------------------------
+template <int P0, int P1>
+__global__ void __launch_bounds__(kBlock) k_diag1d2v(PassArgs a)
+{
+    extern __shared__ double smem[];
+    const int nx = a.nx;
+    const int tid = threadIdx.x;
+    double *sE1 = smem, *sE2 = sE1 + nx, *sB = sE2 + nx, *sE1n = sB + nx, *sE2n = sE1n + nx;
+    double *sWork = sE2n + nx, *sHc = sWork + nx, *sScr = sHc + nx;
+    __shared__ double s_part[kWaves][8];
+    __shared__ double s_red[8];
+    for (int i = tid; i < nx; i += kBlock) {
+        sE1[i] = a.grid[S_E1 * nx + i];
+        sE2[i] = a.grid[S_E2 * nx + i];
+        sB[i] = a.grid[S_B * nx + i];
+        sE1n[i] = a.grid[S_E1N * nx + i];
+        sE2n[i] = a.grid[S_E2N * nx + i];
+    }
+    __syncthreads();
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    const int64_t base = (int64_t)blockIdx.x * a.chunk;
+    for (int it = 0; it < a.iters; ++it) {
+        const int64_t i = base + (int64_t)it * kBlock + tid;
+        if (i >= a.n) continue;
+        const double x = a.x[i], v1 = a.v1[i], v2 = a.v2[i], w = a.w[i];
+        const double wm = (w * a.mass) * a.cw;
+        acc[0] += (v1 * v1 + v2 * v2) * wm;
+        acc[1] += v1 * wm;
+        acc[2] += v2 * wm;
+        const double xi = (x - a.xmin) / a.dx;
+        const int ic = (int)xi;
+        const double off = xi - (double)ic;
+        const int key = imod(ic, nx);
+        double b1[P1 + 1], b0[P0 + 1];
+        bspline<P1>(off, b1);
+        bspline<P0>(off, b0);
+        const double e1 = gather<P1>(b1, sE1, key, nx);
+        const double e2 = gather<P0>(b0, sE2, key, nx);
+        const double bf = gather<P1>(b1, sB, key, nx);
+        const double qa = (a.charge * w) * a.cw;
+        acc[3] += (v1 * e1 + v2 * e2) * qa;
+        const double qb = (w * a.charge) * a.cw;
+        acc[4] += qb * v1 * v2 * bf;
+    }
+    // deterministic block sum: wave butterflies, then waves in order
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        double s = wave_sum(acc[k]);
+        if (lane_id() == 0) s_part[tid / kWave][k] = s;
+    }
+    __syncthreads();
+    const int len = 8;
+    if (tid < len) {
+        double s = 0.0;
+        if (tid < 5)
+            for (int wv = 0; wv < kWaves; ++wv) s += s_part[wv][tid];
+        a.part[(size_t)blockIdx.x * len + tid] = s;
+    }
+    double *red = sScr;
+    double *rscr = sScr + 8 + nx;     // kBlock doubles of reduction scratch, then the tables
+    if (!grid_reduce(a.part, a.gpart, a.cnt, a.ngroups, len, 0x1fu, len, red, rscr)) return;
+    // field diagnostics (diagnostics.jl:221-246)
+    load_tabs(a.tabs, rscr, nx);
+    const double *T = rscr;
+    double *scratch = sScr + 8;
+    for (int i = tid; i < nx; i += kBlock) scratch[i] = 0.0;
+    __syncthreads();
+    ep_e_from_b(scratch, sB, 1.0, a.dx, sWork, sHc, T, nx);
+    const double poynting = ep_inner(sE2, scratch, T + T_M0 * nx, a.dx, sWork, sHc, T, nx);
+    const double pe1 = ep_inner(sE1, sE1n, T + T_M1 * nx, a.dx, sWork, sHc, T, nx);
+    const double pe2 = ep_inner(sE2, sE2n, T + T_M0 * nx, a.dx, sWork, sHc, T, nx);
+    const double pb3 = ep_inner(sB, sB, T + T_M1 * nx, a.dx, sWork, sHc, T, nx);
+    if (tid == 0) {
+        double err = 0.0;
+        for (int i = 0; i < nx; ++i) {
+            double d = fabs(sE1[i] - a.grid[S_EPOIS * nx + i]);
+            if (d > err) err = d;
+        }
+        a.diag[1] = red[0];
+        a.diag[2] = red[1];
+        a.diag[3] = red[2];
+        a.diag[4] = pe1;
+        a.diag[5] = pe2;
+        a.diag[6] = pb3;
+        a.diag[7] = red[3];
+        a.diag[8] = red[4];
+        a.diag[9] = poynting;
+        a.diag[10] = err;
+    }
+    (void)s_red;
+}
+
+// ---------------------------------------------------------------------------
+// Launch dispatch
+// ---------------------------------------------------------------------------
+template <int P0, int P1>
+static hipError_t launch_pass_pd(int mode, const PassArgs &a, int blocks, size_t shmem,
+                                 hipStream_t s)
+{
+    // blocks < 0: query how many workgroups of this kernel fit on one CU
+#define GK_CASE(M)                                                                          \
+    case (M):                                                                               \
+        if (blocks < 0) {                                                                   \
+            int nb = 0;                                                                     \
+            hipError_t e_ = hipOccupancyMaxActiveBlocksPerMultiprocessor(                   \
+                &nb, k_pass1d<P0, P1, (M)>, kBlock, shmem);                                 \
+            *(int *)a.diag = nb;                                                            \
+            return e_;                                                                      \
+        }                                                                                   \
+        hipLaunchKernelGGL((k_pass1d<P0, P1, (M)>), dim3(blocks), dim3(kBlock), shmem, s, a); \
+        return hipGetLastError();
+    switch (mode) {
+        GK_CASE(M_HE)
+        GK_CASE(M_HP2A)
+        GK_CASE(M_HP1)
+        GK_CASE(M_HE | M_HP2A | M_HP1 | M_HP2B)
+        GK_CASE(M_HE_PREV | M_HE | M_HP2A | M_HP1 | M_HP2B)
+        GK_CASE(M_HE_PREV)
+        GK_CASE(M_HE | M_HP2A | M_NOEPI)
+        GK_CASE(M_HE_PREV | M_HE | M_HP2A | M_NOEPI)
+        GK_CASE(M_HP1 | M_HP2B | M_WITH_J2A)
+        GK_CASE(M_V1 | M_HE)
+        GK_CASE(M_V1 | M_HP1)
+        GK_CASE(M_V1 | M_HE | M_HP1)
+        GK_CASE(M_V1 | M_HE_PREV | M_HE | M_HP1)
+        GK_CASE(M_V1 | M_HE_PREV)
+        GK_CASE(M_RHO)
+    case M_DIAG:
+        if (blocks < 0) {
+            int nb = 0;
+            hipError_t e_ = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_diag1d2v<P0, P1>,
+                                                                         kBlock, shmem);
+            *(int *)a.diag = nb;
+            return e_;
+        }
+        hipLaunchKernelGGL((k_diag1d2v<P0, P1>), dim3(blocks), dim3(kBlock), shmem, s, a);
+        return hipGetLastError();
+    default:
+        return hipErrorInvalidValue;
+    }
+#undef GK_CASE
+}
+
+int pass_blocks_per_cu(int p0, int p1, int mode, size_t shmem)
+{
+    PassArgs a;
+    int nb = 0;
+    a.diag = (double *)&nb;
+    if (launch_pass(p0, p1, mode, a, -1, shmem, nullptr) != hipSuccess || nb < 1) return 1;
+    return nb;
+}
+
+hipError_t launch_pass(int p0, int p1, int mode, const PassArgs &a, int blocks, size_t shmem,
+                       hipStream_t s)
+{
+#define GK_PAIR(A, B) \
+    if (p0 == (A) && p1 == (B)) return launch_pass_pd<A, B>(mode, a, blocks, shmem, s);
+    GK_PAIR(1, 0)
+    GK_PAIR(1, 1)
+    GK_PAIR(2, 1)
+    GK_PAIR(2, 2)
+    GK_PAIR(3, 2)
+    GK_PAIR(3, 3)
+#undef GK_PAIR
+    return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------------
+// Single-workgroup field programs (operatorHB, strang pre-step, Maxwell verbs)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_grid1d(GridProg g, int nx, double dx, double *grid,
+                                                   const double *tabs, double *scalar_out)
+{
+    __shared__ double sF[NSLOT * kNxMax];
+    __shared__ double sWork[kNxMax], sHc[kNxMax];
+    __shared__ double sT[NTAB * kNxMax];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < NSLOT * nx; i += kBlock) sF[i] = grid[i];
+    load_tabs(tabs, sT, nx);
+    tabs = sT;
+    for (int o = 0; o < g.nops; ++o) {
+        const double dt = g.dt[o];
+        double *A = sF + g.slot_a[o] * nx;
+        double *B = sF + g.slot_b[o] * nx;
+        double *O = sF + g.slot_out[o] * nx;
+        switch (g.op[o]) {
+        case G_HB: ep_e_from_b(O, A, dt, dx, sWork, sHc, tabs, nx); break;
+        case G_HE_B: ep_b_from_e(O, A, dt, dx, nx); break;
+        case G_E_FROM_J1: ep_e_from_j(O, A, tabs + T_INVM1 * nx, dx, sWork, sHc, tabs, nx); break;
+        case G_E_FROM_J2: ep_e_from_j(O, A, tabs + T_INVM0 * nx, dx, sWork, sHc, tabs, nx); break;
+        case G_E_FROM_RHO: ep_e_from_rho(O, A, sWork, sHc, tabs, nx); break;
+        case G_INNER_M0: {
+            double r = ep_inner(A, B, tabs + T_M0 * nx, dx, sWork, sHc, tabs, nx);
+            if (tid == 0 && scalar_out) scalar_out[o] = r;
+            break;
+        }
+        case G_INNER_M1: {
+            double r = ep_inner(A, B, tabs + T_M1 * nx, dx, sWork, sHc, tabs, nx);
+            if (tid == 0 && scalar_out) scalar_out[o] = r;
+            break;
+        }
+        default: break;
+        }
+        __syncthreads();
+    }
+    for (int i = tid; i < NSLOT * nx; i += kBlock) grid[i] = sF[i];
+}
+
+hipError_t launch_grid(const GridProg &g, int nx, double dx, double *grid, const double *tabs,
+                       double *scalar_out, hipStream_t s)
+{
+    if (nx > kNxMax) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_grid1d, dim3(1), dim3(kBlock), 0, s, g, nx, dx, grid, tabs, scalar_out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Stable counting sort by cell (wave tiles of kSortTile particles)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int sort_key(double x, double xmin, double dx, int nx)
+{
+    double xi = (x - xmin) / dx;
+    if (!(fabs(xi) < 1.0e9)) xi = 0.0;
+    return imod((int)xi, nx);
+}
+
+// counts/offsets are cell-major: [cell][tile]. Each wave owns one tile of
+// kSortTile particles and walks it in batches of kSortBatch iterations whose
+// loads are issued together.
+constexpr int kSortBatch = 8;
+
+__global__ void __launch_bounds__(kBlock) k_sort_count(SortArgs a)
+{
+    extern __shared__ int ish[];
+    const int wave = threadIdx.x / kWave;
+    const int lane = lane_id();
+    int *h = ish + wave * a.nx;
+    const int tile = blockIdx.x * kWaves + wave;
+    for (int c = lane; c < a.nx; c += kWave) h[c] = 0;
+    __syncthreads();
+    if (tile < a.ntiles) {
+        const int64_t t0 = (int64_t)tile * kSortTile;
+        for (int k0 = 0; k0 < kSortTile / kWave; k0 += kSortBatch) {
+            double xs[kSortBatch];
+#pragma unroll
+            for (int u = 0; u < kSortBatch; ++u) {
+                const int64_t i = t0 + (int64_t)(k0 + u) * kWave + lane;
+                xs[u] = i < a.n ? a.x[i] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < kSortBatch; ++u) {
+                const int64_t i = t0 + (int64_t)(k0 + u) * kWave + lane;
+                const bool valid = i < a.n;
+                const int c = valid ? sort_key(xs[u], a.xmin, a.dx, a.nx) : -1;
+                unsigned long long pending = __ballot(valid);
+                while (pending) {
+                    const int leader = __ffsll((long long)pending) - 1;
+                    const int cc = __shfl(c, leader, kWave);
+                    const unsigned long long m = __ballot(valid && c == cc);
+                    if (lane == leader) h[cc] += __popcll(m);
+                    pending &= ~m;
+                }
+            }
+        }
+        for (int c = lane; c < a.nx; c += kWave) a.counts[(size_t)c * a.ntiles + tile] = h[c];
+    }
+}
+
+// Exclusive scan of the cell-major counts (one workgroup of 1024):
+// offsets[c][t] = sum of all counts before (c, t) in cell-major order.
+__global__ void __launch_bounds__(1024) k_sort_scan(SortArgs a)
+{
+    constexpr int PER = 16;
+    __shared__ int s_part[1024];
+    __shared__ int s_carry;
+    const int tid = threadIdx.x;
+    const int64_t total = (int64_t)a.nx * a.ntiles;
+    if (tid == 0) s_carry = 0;
+    __syncthreads();
+    for (int64_t base = 0; base < total; base += 1024 * PER) {
+        const int64_t b0 = base + (int64_t)tid * PER;
+        int v[PER];
+        int s = 0;
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            v[u] = (b0 + u < total) ? a.counts[b0 + u] : 0;
+            s += v[u];
+        }
+        s_part[tid] = s;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            const int add = tid >= off ? s_part[tid - off] : 0;
+            __syncthreads();
+            s_part[tid] += add;
+            __syncthreads();
+        }
+        int run = s_carry + s_part[tid] - s;
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            if (b0 + u < total) a.offsets[b0 + u] = run;
+            run += v[u];
+        }
+        __syncthreads();
+        if (tid == 1023) s_carry += s_part[1023];
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_sort_scatter(SortArgs a, int has_v2)
+{
+    extern __shared__ int ish[];
+    const int wave = threadIdx.x / kWave;
+    const int lane = lane_id();
+    int *cur = ish + wave * a.nx;
+    const int tile = blockIdx.x * kWaves + wave;
+    if (tile < a.ntiles)
+        for (int c = lane; c < a.nx; c += kWave) cur[c] = a.offsets[(size_t)c * a.ntiles + tile];
+    __syncthreads();
+    if (tile >= a.ntiles) return;
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int64_t t0 = (int64_t)tile * kSortTile;
+    for (int k0 = 0; k0 < kSortTile / kWave; k0 += kSortBatch) {
+        double xs[kSortBatch], v1s[kSortBatch], v2s[kSortBatch], ws[kSortBatch];
+        int64_t ids[kSortBatch];
+#pragma unroll
+        for (int u = 0; u < kSortBatch; ++u) {
+            const int64_t i = t0 + (int64_t)(k0 + u) * kWave + lane;
+            const bool valid = i < a.n;
+            xs[u] = valid ? a.x[i] : 0.0;
+            v1s[u] = valid ? a.v1[i] : 0.0;
+            v2s[u] = (valid && has_v2) ? a.v2[i] : 0.0;
+            ws[u] = valid ? a.w[i] : 0.0;
+            ids[u] = valid ? a.id[i] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < kSortBatch; ++u) {
+            const int64_t i = t0 + (int64_t)(k0 + u) * kWave + lane;
+            const bool valid = i < a.n;
+            const int c = valid ? sort_key(xs[u], a.xmin, a.dx, a.nx) : -1;
+            unsigned long long pending = __ballot(valid);
+            int pos = 0;
+            while (pending) {
+                const int leader = __ffsll((long long)pending) - 1;
+                const int cc = __shfl(c, leader, kWave);
+                const unsigned long long m = __ballot(valid && c == cc);
+                const int basepos = cur[cc];
+                if (valid && c == cc) pos = basepos + __popcll(m & lt);
+                if (lane == leader) cur[cc] = basepos + __popcll(m);
+                pending &= ~m;
+            }
+            if (valid) {
+                a.x2[pos] = xs[u];
+                a.v12[pos] = v1s[u];
+                if (has_v2) a.v22[pos] = v2s[u];
+                a.w2[pos] = ws[u];
+                a.id2[pos] = ids[u];
+            }
+        }
+    }
+}
+
+hipError_t launch_sort(const SortArgs &a, int has_v2, hipStream_t s)
+{
+    if (a.nx > kNxMax) return hipErrorInvalidValue;
+    const int blocks = (a.ntiles + kWaves - 1) / kWaves;
+    const size_t sh = sizeof(int) * (size_t)kWaves * a.nx;
+    hipLaunchKernelGGL(k_sort_count, dim3(blocks), dim3(kBlock), sh, s, a);
+    hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, a);
+    hipLaunchKernelGGL(k_sort_scatter, dim3(blocks), dim3(kBlock), sh, s, a, has_v2);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Layout transposes between the reference's particle matrix and SoA
+// ---------------------------------------------------------------------------
+__global__ void k_aos_to_soa(const double *aos, int64_t ld, int64_t n, int dim_v, double *x,
+                             double *v1, double *v2, double *w, int64_t *id)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double *p = aos + i * ld;
+        x[i] = p[0];
+        v1[i] = p[1];
+        if (dim_v == 2) v2[i] = p[2];
+        w[i] = p[1 + dim_v];
+        id[i] = i;
+    }
+}
+
+__global__ void k_soa_to_aos(double *aos, int64_t ld, int64_t n, int dim_v, const double *x,
+                             const double *v1, const double *v2, const double *w,
+                             const int64_t *id)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = id[i];
+        if (j < 0 || j >= n) continue;
+        double *p = aos + j * ld;
+        p[0] = x[i];
+        p[1] = v1[i];
+        if (dim_v == 2) p[2] = v2[i];
+        p[1 + dim_v] = w[i];
+    }
+}
+
+hipError_t launch_aos_to_soa(const double *aos, int64_t ld, int64_t n, int dim_v, double *x,
+                             double *v1, double *v2, double *w, int64_t *id, hipStream_t s)
+{
+    int blocks = (int)((n + 255) / 256);
+    if (blocks > 4096) blocks = 4096;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_aos_to_soa, dim3(blocks), dim3(256), 0, s, aos, ld, n, dim_v, x, v1, v2,
+                       w, id);
+    return hipGetLastError();
+}
+
+hipError_t launch_soa_to_aos(double *aos, int64_t ld, int64_t n, int dim_v, const double *x,
+                             const double *v1, const double *v2, const double *w,
+                             const int64_t *id, hipStream_t s)
+{
+    int blocks = (int)((n + 255) / 256);
+    if (blocks > 4096) blocks = 4096;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_soa_to_aos, dim3(blocks), dim3(256), 0, s, aos, ld, n, dim_v, x, v1, v2,
+                       w, id);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Batched ParticleMeshCoupling1D verbs (particles in the caller's order)
+// ---------------------------------------------------------------------------
+
+// horner_1d (splinepp.jl:268-274)
+__device__ __forceinline__ double horner(int degree, const double *pp, int ld, double x, int col)
+{
+    double res = pp[col * ld];
+    for (int i = 1; i <= degree; ++i) res = res * x + pp[i + col * ld];
+    return res;
+}
+
+template <int P>
+__device__ __forceinline__ void pp_basis(const double *pc, double x, double (&b)[P + 1])
+{
+#pragma unroll
+    for (int i = 0; i <= P; ++i) b[i] = horner(P, pc, P + 1, x, i);
+}
+
+template <int P, int OP>
+__global__ void __launch_bounds__(kBlock) k_pmc(PmcArgs a)
+{
+    constexpr int W = P + 1;
+    constexpr int NQ = (P + 2) / 2;
+    constexpr bool DEPOSIT = OP == PMC_CHARGE || OP == PMC_CHARGE_PP || OP == PMC_CURRENT_B ||
+                             OP == PMC_CURRENT_NOB || OP == PMC_CURRENT_PP;
+    extern __shared__ double smem[];
+    const int nx = a.nx;
+    const int tid = threadIdx.x;
+    const int wave = tid / kWave;
+    double *sRed = smem;                       // [nx]
+    double *rows = sRed + nx;                  // [kWaves][nx], later reduction scratch
+    for (int i = tid; i < kWaves * nx; i += kBlock) rows[i] = 0.0;
+    __syncthreads();
+    double *myrow = rows + wave * nx;
+    Cache2<W> cache;
+    cache.init();
+    int errs = 0;
+    const int64_t base = (int64_t)blockIdx.x * a.chunk;
+    for (int it = 0; it < a.iters; ++it) {
+        const int64_t i = base + (int64_t)it * kBlock + tid;
+        const bool valid = i < a.n;
+        const double x = valid ? a.x[i] : a.xmin;
+        const double xi0 = (x - a.xmin) / a.dx;
+        if (valid && !pos_ok(xi0)) errs |= E_INEXACT;
+        if constexpr (OP == PMC_EVAL || OP == PMC_EVAL_PP || OP == PMC_CHARGE ||
+                      OP == PMC_CHARGE_PP) {
+            const int ic = (int)xi0;
+            const double off = xi0 - (double)ic;
+            double b[W];
+            if constexpr (OP == PMC_EVAL_PP || OP == PMC_CHARGE_PP) pp_basis<P>(a.pc, off, b);
+            else bspline<P>(off, b);
+            if constexpr (OP == PMC_EVAL) {
+                if (valid) a.out[i] = gather<P>(b, a.field, imod(ic, nx), nx);
+            } else if constexpr (OP == PMC_EVAL_PP) {
+                // evaluate_pp: horner on column trunc(xi) of the pp matrix, unwrapped
+                if (valid) {
+                    if (ic < 0 || ic >= nx) errs |= E_BOUNDS;
+                    else a.out[i] = horner(P, a.field, P + 1, off, ic);
+                }
+            } else {
+                const double q = valid ? a.q[i] : 0.0;
+                double vals[W];
+#pragma unroll
+                for (int k = 0; k < W; ++k) vals[k] = q * b[k] * a.scaling;
+                cache_add<W, 1>(cache, valid, imod(ic, nx), vals, myrow, nx);
+            }
+        } else {
+            // add_current_update_v! / _pp!
+            const double xn = valid ? a.x_new[i] : x;
+            const double q = valid ? a.q[i] : 0.0;
+            double v = (valid && OP != PMC_CURRENT_NOB) ? a.out[i] : 0.0;
+            const int io = (int)xi0;
+            const double ro = xi0 - (double)io;
+            const double xin = (xn - a.xmin) / a.dx;
+            if (valid && !pos_ok(xin)) errs |= E_INEXACT;
+            const int inw = (OP == PMC_CURRENT_NOB) ? (int)floor(xin) : (int)xin;
+            const double rn = xin - (double)inw;
+            const int dseg = inw - io;
+            const bool far = !pos_ok(xin) || !pos_ok(xi0) || dseg > kMaxCellsPerStep ||
+                             dseg < -kMaxCellsPerStep;
+            if (valid && far) errs |= E_FAR;
+            const int nseg = far ? 1 : 1 + (dseg < 0 ? -dseg : dseg);
+            for (int s = 0;; ++s) {
+                const bool act = valid && s < nseg;
+                if (!__ballot(act)) break;
+                double sv[W];
+                int cell;
+                if constexpr (OP == PMC_CURRENT_PP) {
+                    // update_jv_pp! bounds (:155-189): reversed segments swap bounds
+                    double lo, up;
+                    if (io == inw) { lo = ro; up = rn; cell = io; }
+                    else if (io < inw) {
+                        if (s == 0) { lo = ro; up = 1.0; cell = io; }
+                        else if (s == 1) { lo = 0.0; up = rn; cell = inw; }
+                        else { lo = 0.0; up = 1.0; cell = io + s - 1; }
+                    } else {
+                        if (s == 0) { lo = 1.0; up = rn; cell = inw; }
+                        else if (s == 1) { lo = ro; up = 0.0; cell = io; }
+                        else { lo = 1.0; up = 0.0; cell = inw + s - 1; }
+                    }
+                    double svm[W];
+#pragma unroll
+                    for (int k = 0; k < W; ++k) sv[k] = horner(P, a.pf, W, lo, k) * lo;
+#pragma unroll
+                    for (int k = 0; k < W; ++k) svm[k] = horner(P, a.pf, W, up, k) * up;
+#pragma unroll
+                    for (int k = 0; k < W; ++k) sv[k] = (svm[k] - sv[k]) * a.dx;
+                } else {
+                    double lo, up, sg;
+                    segment_bounds(s, io, ro, inw, rn, lo, up, sg, cell);
+                    segment<P, NQ>(lo, up, sg, a.qx, a.qw, a.dx, sv);
+                }
+                const int key = imod(cell, nx);
+                double vals[W];
+                int k = first_node(key, P, nx);
+                double vs = v;
+#pragma unroll
+                for (int m = 0; m < W; ++m) {
+                    vals[m] = q * sv[m] * a.scaling;
+                    if constexpr (OP != PMC_CURRENT_NOB) vs = vs - a.qm * sv[m] * a.field[k];
+                    k = next_node(k, nx);
+                }
+                if (act) v = vs;
+                cache_add<W, 1>(cache, act, key, vals, myrow, nx);
+            }
+            if (valid && OP != PMC_CURRENT_NOB) a.out[i] = v;
+        }
+    }
+    if (errs) atomicOr(a.err, errs);
+    if constexpr (DEPOSIT) {
+        cache_drain<W, 1>(cache, myrow, nx);
+        __syncthreads();
+        for (int idx = tid; idx < nx; idx += kBlock) {
+            double s = 0.0;
+            for (int wv = 0; wv < kWaves; ++wv) s += rows[wv * nx + idx];
+            a.part[(size_t)blockIdx.x * nx + idx] = s;
+        }
+        if (!grid_reduce(a.part, a.gpart, a.cnt, a.ngroups, nx, 1u, nx, sRed, rows)) return;
+        for (int idx = tid; idx < nx; idx += kBlock) a.acc[idx] = a.acc[idx] + sRed[idx];
+    }
+}
+
+template <int P>
+static hipError_t launch_pmc_p(int op, const PmcArgs &a, int blocks, hipStream_t s)
+{
+    size_t rows = (size_t)kWaves * a.nx;
+    if (rows < (size_t)kBlock) rows = kBlock;
+    const size_t sh = sizeof(double) * ((size_t)a.nx + rows);
+    switch (op) {
+#define GK_OP(O)                                                                        \
+    case O:                                                                             \
+        hipLaunchKernelGGL((k_pmc<P, O>), dim3(blocks), dim3(kBlock), sh, s, a);        \
+        return hipGetLastError();
+        GK_OP(PMC_EVAL)
+        GK_OP(PMC_EVAL_PP)
+        GK_OP(PMC_CHARGE)
+        GK_OP(PMC_CHARGE_PP)
+        GK_OP(PMC_CURRENT_B)
+        GK_OP(PMC_CURRENT_NOB)
+        GK_OP(PMC_CURRENT_PP)
+#undef GK_OP
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_pmc(int op, const PmcArgs &a, int blocks, hipStream_t s)
+{
+    switch (a.degree) {
+    case 0:
+        if (op == PMC_EVAL_PP || op == PMC_CHARGE_PP || op == PMC_CURRENT_PP)
+            return hipErrorInvalidValue;
+        return launch_pmc_p<0>(op, a, blocks, s);
+    case 1: return launch_pmc_p<1>(op, a, blocks, s);
+    case 2: return launch_pmc_p<2>(op, a, blocks, s);
+    case 3: return launch_pmc_p<3>(op, a, blocks, s);
+    case 4: return launch_pmc_p<4>(op, a, blocks, s);
+    case 5: return launch_pmc_p<5>(op, a, blocks, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace gl

@@ -1,0 +1,467 @@
+"""Host-side mirror of GEMPIC.jl's particle-field API over the HIP library.
+
+Names, argument meaning and error behaviour follow the reference (Julia);
+Julia's mutating `f!` functions are spelled `f_` here. Compute runs on the
+GPU through include/gempic_hip.h. Set-up arithmetic that the reference does
+once on the host (meshes, quadrature-based right-hand sides) stays on the
+host, as in the reference.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, ptr
+
+__all__ = [
+    "OneDGrid", "ParticleGroup", "ParticleMeshCoupling1D", "Maxwell1DFEM",
+    "HamiltonianSplitting", "add_charge_", "add_charge_pp_", "evaluate", "evaluate_pp",
+    "add_current_update_v_", "add_current_update_v_pp_", "b_to_pp",
+    "compute_e_from_rho_", "compute_e_from_j_", "compute_e_from_b_", "compute_b_from_e_",
+    "compute_rhs_from_function_", "l2projection_", "inner_product", "l2norm_squared",
+    "operatorHB", "operatorHE", "operatorHp1", "operatorHp2", "strang_splitting_",
+    "solve_poisson_", "TimeHistoryDiagnostics", "write_step_", "get_charge",
+]
+
+
+# ---------------------------------------------------------------------------
+# mesh.jl
+
+class OneDGrid:
+    """OneDGrid(xmin, xmax, nx) (mesh.jl:52-67)."""
+
+    def __init__(self, xmin: float, xmax: float, nx: int):
+        self.nx = int(nx)
+        self.xmin = float(xmin)
+        self.xmax = float(xmax)
+        self.dimx = self.xmax - self.xmin
+        self.dx = self.dimx / (self.nx - 1)        # as in the reference (unused by the path)
+        self.x = np.linspace(self.xmin, self.xmax, self.nx + 1)
+
+
+# ---------------------------------------------------------------------------
+# particle_group.jl
+
+class ParticleGroup:
+    """ParticleGroup{D,V}(n; charge, mass, n_weights, common_weight)
+    (particle_group.jl:15-46). `array` is the (D+V+n_weights, N) matrix in
+    column-major (Fortran) order, exactly the reference's memory layout."""
+
+    def __init__(self, D: int, V: int, n_particles: int, charge=1.0, mass=1.0, n_weights=1,
+                 common_weight=0.0):
+        self.dims = (int(D), int(V))
+        self.n_particles = int(n_particles)
+        self.array = np.zeros((D + V + n_weights, n_particles), order="F")
+        if common_weight == 0.0:
+            common_weight = 1.0 / n_particles
+        self.common_weight = float(common_weight)
+        self.charge = float(charge)
+        self.mass = float(mass)
+        self.n_weights = int(n_weights)
+        self.q_over_m = self.charge / self.mass
+
+
+def get_charge(p: ParticleGroup, i: int, i_wi: int = 0) -> float:
+    """get_charge (particle_group.jl:67-69); 0-based particle index."""
+    D, V = p.dims
+    return p.charge * p.array[D + V + i_wi, i] * p.common_weight
+
+
+# ---------------------------------------------------------------------------
+# particle_mesh_coupling_1d.jl
+
+_SMOOTHING = {"galerkin": _lib.GALERKIN, "collocation": _lib.COLLOCATION}
+
+
+class ParticleMeshCoupling1D:
+    """ParticleMeshCoupling1D(mesh, no_particles, spline_degree, smoothing_type)
+    (particle_mesh_coupling_1d.jl:26-95)."""
+
+    def __init__(self, mesh: OneDGrid, no_particles: int, spline_degree: int,
+                 smoothing_type: str):
+        if smoothing_type not in _SMOOTHING:
+            raise ValueError(f"Smoothing Type {smoothing_type} not implemented for "
+                             "kernel_smoother_spline_1d.")
+        self.dims = 1
+        self.n_grid = mesh.nx
+        self.n_dofs = mesh.nx
+        self.no_particles = int(no_particles)
+        self.spline_degree = int(spline_degree)
+        self.n_span = self.spline_degree + 1
+        self.xmin = mesh.xmin
+        self.xmax = mesh.xmax
+        self.Lx = mesh.xmax - mesh.xmin
+        self.delta_x = self.Lx / self.n_grid
+        self.smoothing_type = smoothing_type
+        self.smoothing = _SMOOTHING[smoothing_type]
+        self.scaling = 1.0 / self.delta_x if smoothing_type == "collocation" else 1.0
+        self.n_quad_points = (self.spline_degree + 2) // 2
+
+    def _args(self):
+        return (self.n_grid, self.xmin, self.xmax, self.spline_degree, self.smoothing)
+
+
+def _arr(x):
+    return np.ascontiguousarray(np.atleast_1d(np.asarray(x, dtype=np.float64)))
+
+
+def add_charge_(rho: np.ndarray, p: ParticleMeshCoupling1D, position, marker_charge):
+    """add_charge! (particle_mesh_coupling_1d.jl:261-280), batched over arrays
+    of positions/charges (a scalar pair is one particle)."""
+    x = _arr(position)
+    q = _arr(marker_charge) * np.ones_like(x)
+    check(_lib.load().gempic_pmc1d_add_charge(*p._args(), x.size, ptr(x), ptr(q), ptr(rho)))
+
+
+def add_charge_pp_(rho: np.ndarray, p: ParticleMeshCoupling1D, position, marker_charge):
+    """add_charge_pp! (particle_mesh_coupling_1d.jl:106-122)."""
+    x = _arr(position)
+    q = _arr(marker_charge) * np.ones_like(x)
+    check(_lib.load().gempic_pmc1d_add_charge_pp(*p._args(), x.size, ptr(x), ptr(q), ptr(rho)))
+
+
+def evaluate(p: ParticleMeshCoupling1D, position, field_dofs: np.ndarray):
+    """evaluate (particle_mesh_coupling_1d.jl:438-453); array positions give
+    the batched form of diagnostics.jl:260-268."""
+    x = _arr(position)
+    out = np.zeros_like(x)
+    f = np.ascontiguousarray(field_dofs, dtype=np.float64)
+    check(_lib.load().gempic_pmc1d_evaluate(*p._args(), x.size, ptr(x), ptr(f), ptr(out)))
+    return float(out[0]) if np.ndim(position) == 0 else out
+
+
+def evaluate_pp(p: ParticleMeshCoupling1D, position, field_dofs_pp: np.ndarray):
+    """evaluate_pp (particle_mesh_coupling_1d.jl:242-250); field_dofs_pp is
+    the (degree+1, n_cells) matrix of b_to_pp."""
+    x = _arr(position)
+    out = np.zeros_like(x)
+    f = np.asfortranarray(field_dofs_pp, dtype=np.float64)
+    check(_lib.load().gempic_pmc1d_evaluate_pp(*p._args(), x.size, ptr(x), ptr(f), ptr(out)))
+    return float(out[0]) if np.ndim(position) == 0 else out
+
+
+def add_current_update_v_(j_dofs: np.ndarray, p: ParticleMeshCoupling1D, position_old,
+                          position_new, marker_charge, qoverm: float, bfield_dofs, vi):
+    """add_current_update_v! (particle_mesh_coupling_1d.jl:296-376); with
+    bfield_dofs=None the 1d1v form without B (:471-529). Returns the updated
+    v (scalar in, scalar out)."""
+    xo = _arr(position_old)
+    xn = _arr(position_new) * np.ones_like(xo)
+    q = _arr(marker_charge) * np.ones_like(xo)
+    v = _arr(vi) * np.ones_like(xo)
+    b = None if bfield_dofs is None else np.ascontiguousarray(bfield_dofs, dtype=np.float64)
+    check(_lib.load().gempic_pmc1d_add_current_update_v(*p._args(), xo.size, ptr(xo), ptr(xn),
+                                                        ptr(q), float(qoverm), ptr(b), ptr(v),
+                                                        ptr(j_dofs)))
+    return float(v[0]) if np.ndim(vi) == 0 else v
+
+
+def add_current_update_v_pp_(j_dofs: np.ndarray, p: ParticleMeshCoupling1D, position_old,
+                             position_new, marker_charge, qoverm: float, bfield_dofs, vi):
+    """add_current_update_v_pp! (particle_mesh_coupling_1d.jl:131-190)."""
+    xo = _arr(position_old)
+    xn = _arr(position_new) * np.ones_like(xo)
+    q = _arr(marker_charge) * np.ones_like(xo)
+    v = _arr(vi) * np.ones_like(xo)
+    b = np.ascontiguousarray(bfield_dofs, dtype=np.float64)
+    check(_lib.load().gempic_pmc1d_add_current_update_v_pp(*p._args(), xo.size, ptr(xo),
+                                                           ptr(xn), ptr(q), float(qoverm),
+                                                           ptr(b), ptr(v), ptr(j_dofs)))
+    return float(v[0]) if np.ndim(vi) == 0 else v
+
+
+def b_to_pp(p: ParticleMeshCoupling1D, ncells: int, b_coeffs: np.ndarray) -> np.ndarray:
+    """b_to_pp(spline_pp, ncells, b_coeffs) (splinepp.jl:241-261)."""
+    b = np.ascontiguousarray(b_coeffs, dtype=np.float64)
+    out = np.zeros((p.spline_degree + 1, ncells), order="F")
+    check(_lib.load().gempic_b_to_pp(p.spline_degree, ncells, ptr(b), ptr(out)))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# maxwell_1d_fem.jl
+
+class Maxwell1DFEM:
+    """Maxwell1DFEM(mesh, degree) (maxwell_1d_fem.jl:29-177); the circulant
+    solves run on the GPU."""
+
+    def __init__(self, mesh: OneDGrid, degree: int):
+        if degree not in (1, 2, 3):
+            raise ValueError(f"Wrong value of degree = {degree}  (1,2 or 3)")
+        self.xmin = mesh.xmin
+        self.Lx = mesh.xmax - mesh.xmin
+        self.n_dofs = mesh.nx
+        self.delta_x = self.Lx / self.n_dofs
+        self.s_deg_0 = degree
+        self.s_deg_1 = degree - 1
+        self._mesh = mesh
+        h = ctypes.c_void_p()
+        check(_lib.load().gempic_maxwell1d_create(mesh.nx, mesh.xmin, mesh.xmax, degree,
+                                                  ctypes.byref(h)))
+        self._h = h
+        t = [np.zeros(self.n_dofs) for _ in range(4)]
+        check(_lib.load().gempic_maxwell1d_tables(self._h, *[ptr(a) for a in t]))
+        self.eig_mass0, self.eig_mass1, self.eig_weak_ampere, self.eig_weak_poisson = t
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            _lib.load().gempic_maxwell1d_destroy(h)
+            self._h = None
+
+
+def compute_e_from_rho_(e, self: Maxwell1DFEM, rho):
+    """compute_e_from_rho! (maxwell_1d_fem.jl:244-255)."""
+    check(_lib.load().gempic_maxwell1d_compute_e_from_rho(self._h, ptr(e), ptr(_arr(rho))))
+
+
+def compute_e_from_j_(e, self: Maxwell1DFEM, current, component: int):
+    """compute_e_from_j! (maxwell_1d_fem.jl:263-289)."""
+    check(_lib.load().gempic_maxwell1d_compute_e_from_j(self._h, ptr(e), ptr(_arr(current)),
+                                                        int(component)))
+
+
+def compute_e_from_b_(field_out, self: Maxwell1DFEM, delta_t: float, field_in):
+    """compute_e_from_b! (maxwell_1d_fem.jl:384-396)."""
+    check(_lib.load().gempic_maxwell1d_compute_e_from_b(self._h, ptr(field_out), float(delta_t),
+                                                        ptr(_arr(field_in))))
+
+
+def compute_b_from_e_(field_out, self: Maxwell1DFEM, delta_t: float, field_in):
+    """compute_b_from_e! (maxwell_1d_fem.jl:407-420)."""
+    check(_lib.load().gempic_maxwell1d_compute_b_from_e(self._h, ptr(field_out), float(delta_t),
+                                                        ptr(_arr(field_in))))
+
+
+def inner_product(self: Maxwell1DFEM, coefs1_dofs, coefs2_dofs, degree: int) -> float:
+    """inner_product (maxwell_1d_fem.jl:461-475)."""
+    out = np.zeros(1)
+    check(_lib.load().gempic_maxwell1d_inner_product(self._h, ptr(_arr(coefs1_dofs)),
+                                                     ptr(_arr(coefs2_dofs)), int(degree),
+                                                     ptr(out)))
+    return float(out[0])
+
+
+def l2norm_squared(self: Maxwell1DFEM, coefs_dofs, degree: int) -> float:
+    """l2norm_squared (maxwell_1d_fem.jl:299-313)."""
+    out = np.zeros(1)
+    check(_lib.load().gempic_maxwell1d_l2norm_squared(self._h, ptr(_arr(coefs_dofs)),
+                                                      int(degree), ptr(out)))
+    return float(out[0])
+
+
+def _bspline_basis(degree: int, x: float):
+    """uniform_bsplines_eval_basis (low_level_bsplines.jl:38-61), host set-up copy."""
+    b = [0.0] * (degree + 1)
+    b[0] = 1.0
+    for j in range(1, degree + 1):
+        xx = -x
+        jr = float(j)
+        inv_j = 1.0 / jr
+        saved = 0.0
+        for r in range(j):
+            xx = xx + 1.0
+            temp = b[r] * inv_j
+            b[r] = saved + xx * temp
+            saved = (jr - xx) * temp
+        b[j] = saved
+    return b
+
+
+def compute_rhs_from_function_(coefs_dofs, self: Maxwell1DFEM, func, degree: int):
+    """compute_rhs_from_function! (maxwell_1d_fem.jl:188-220): host set-up
+    (the reference evaluates the user's function on the host)."""
+    x, w = np.polynomial.legendre.leggauss(degree + 1)
+    x = 0.5 * (x + 1.0)
+    w = 0.5 * w
+    bspl = [_bspline_basis(degree, xk) for xk in x]
+    dx = self.delta_x
+    for i in range(1, self.n_dofs + 1):
+        coef = 0.0
+        for j in range(1, degree + 2):
+            for k in range(1, degree + 2):
+                coef = coef + w[k - 1] * func(dx * (((x[k - 1] + i) + j) - 2)) * \
+                    bspl[k - 1][degree + 2 - j - 1]
+        coefs_dofs[i - 1] = coef * dx
+
+
+def l2projection_(coefs_dofs, self: Maxwell1DFEM, func, degree: int):
+    """l2projection! (maxwell_1d_fem.jl:347-374). The reference solves into
+    its workspace and then returns coefs_dofs ./ delta_x, i.e. the rhs
+    divided by dx; that result is kept."""
+    if degree not in (self.s_deg_0, self.s_deg_0 - 1):
+        raise ValueError(f"degree {degree} not available")
+    compute_rhs_from_function_(coefs_dofs, self, func, degree)
+    coefs_dofs[:] = coefs_dofs / self.delta_x
+
+
+# ---------------------------------------------------------------------------
+# hamiltonian_splitting.jl, _1d2v.jl, _1d1v.jl
+
+class HamiltonianSplitting:
+    """HamiltonianSplitting{D,V}(maxwell_solver, kernel_smoother_0,
+    kernel_smoother_1, particle_group, e_dofs, b_dofs)
+    (hamiltonian_splitting.jl:20-86).
+
+    e_dofs / b_dofs and particle_group.array are aliased like in the
+    reference. With sync=True (default) every operator uploads them before
+    it runs and writes them back after it, so the caller sees the
+    reference's in-place semantics. With sync=False the state stays in GPU
+    memory between calls; call to_host() / to_device() explicitly.
+    """
+
+    def __init__(self, D: int, V: int, maxwell_solver: Maxwell1DFEM,
+                 kernel_smoother_0: ParticleMeshCoupling1D,
+                 kernel_smoother_1: ParticleMeshCoupling1D, particle_group: ParticleGroup,
+                 e_dofs, b_dofs, sync: bool = True):
+        if (D, V) != particle_group.dims:
+            raise AssertionError("dims == particle_group.dims")
+        if kernel_smoother_0.n_dofs != kernel_smoother_1.n_dofs:
+            raise AssertionError("kernel_smoother_0.n_dofs == kernel_smoother_1.n_dofs")
+        if D != 1:
+            raise ValueError("HamiltonianSplitting: only D = 1 has a reference implementation")
+        self.dims = (D, V)
+        self.maxwell_solver = maxwell_solver
+        self.kernel_smoother_0 = kernel_smoother_0
+        self.kernel_smoother_1 = kernel_smoother_1
+        self.particle_group = particle_group
+        self.spline_degree = 3
+        self.Lx = maxwell_solver.Lx
+        self.x_min = maxwell_solver.xmin
+        self.delta_x = self.Lx / kernel_smoother_1.n_dofs
+        self.e_dofs = e_dofs
+        self.b_dofs = b_dofs
+        nx = kernel_smoother_0.n_dofs
+        self.j_dofs = [np.zeros(nx), np.zeros(nx)]
+        self.sync = sync
+        pg = particle_group
+        mesh = maxwell_solver._mesh
+        h = ctypes.c_void_p()
+        check(_lib.load().gempic_splitting_create(
+            nx, mesh.xmin, mesh.xmax, maxwell_solver.s_deg_0, kernel_smoother_0.spline_degree,
+            kernel_smoother_0.smoothing, kernel_smoother_1.spline_degree,
+            kernel_smoother_1.smoothing, V, pg.n_particles, pg.n_weights, pg.charge, pg.mass,
+            pg.common_weight, ctypes.byref(h)))
+        self._h = h
+        self.to_device()
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            _lib.load().gempic_splitting_destroy(h)
+            self._h = None
+
+    # -- host <-> device
+    def to_device(self, particles: bool = True, fields: bool = True):
+        L = _lib.load()
+        if particles:
+            a = self.particle_group.array
+            assert a.flags.f_contiguous
+            check(L.gempic_set_particles(self._h, ptr(a), a.shape[0], a.shape[1]))
+        if fields:
+            e2 = self.e_dofs[1] if self.dims[1] == 2 else None
+            check(L.gempic_set_fields(self._h, ptr(self.e_dofs[0]), ptr(e2), ptr(self.b_dofs)))
+
+    def to_host(self, particles: bool = True, fields: bool = True):
+        L = _lib.load()
+        if particles:
+            a = self.particle_group.array
+            check(L.gempic_get_particles(self._h, ptr(a), a.shape[0], a.shape[1]))
+        if fields:
+            e2 = self.e_dofs[1] if self.dims[1] == 2 else None
+            check(L.gempic_get_fields(self._h, ptr(self.e_dofs[0]), ptr(e2), ptr(self.b_dofs)))
+            check(L.gempic_get_j(self._h, ptr(self.j_dofs[0]), ptr(self.j_dofs[1])))
+
+    def _run(self, fn, *args):
+        if self.sync:
+            self.to_device()
+        check(fn(self._h, *args))
+        if self.sync:
+            self.to_host()
+
+    def profile(self, enable: bool = True):
+        check(_lib.load().gempic_profile_enable(self._h, int(enable)))
+
+    def profile_read(self, kernel_id: int):
+        ms = np.zeros(1)
+        cnt = ctypes.c_int64()
+        check(_lib.load().gempic_profile_read(self._h, kernel_id, ptr(ms), ctypes.byref(cnt)))
+        return float(ms[0]), int(cnt.value)
+
+    def set_sort_interval(self, k: int):
+        check(_lib.load().gempic_set_sort_interval(self._h, int(k)))
+
+
+def operatorHB(h: HamiltonianSplitting, dt: float):
+    """operatorHB (hamiltonian_splitting_1d2v.jl:234-236; _1d1v.jl:113-125)."""
+    h._run(_lib.load().gempic_operator_hb, float(dt))
+
+
+def operatorHE(h: HamiltonianSplitting, dt: float):
+    """operatorHE (hamiltonian_splitting_1d2v.jl:191-219)."""
+    h._run(_lib.load().gempic_operator_he, float(dt))
+
+
+def operatorHp1(h: HamiltonianSplitting, dt: float):
+    """operatorHp1 (hamiltonian_splitting_1d2v.jl:41-112; _1d1v.jl:63-98)."""
+    h._run(_lib.load().gempic_operator_hp1, float(dt))
+
+
+def operatorHp2(h: HamiltonianSplitting, dt: float):
+    """operatorHp2 (hamiltonian_splitting_1d2v.jl:129-176)."""
+    h._run(_lib.load().gempic_operator_hp2, float(dt))
+
+
+def strang_splitting_(h: HamiltonianSplitting, dt: float, number_steps: int,
+                      substep: bool = False):
+    """strang_splitting! (hamiltonian_splitting.jl:98-108; _1d1v.jl:11-21).
+    The default fused path gives the same particle states as the sub-step
+    sequence (substep=True)."""
+    flags = _lib.STRANG_SUBSTEP if substep else _lib.STRANG_FUSED
+    h._run(_lib.load().gempic_strang_splitting, float(dt), int(number_steps), flags)
+
+
+# ---------------------------------------------------------------------------
+# diagnostics.jl
+
+def solve_poisson_(efield_dofs, h: HamiltonianSplitting, rho):
+    """solve_poisson!(efield, particle_group, kernel_smoother_0, maxwell, rho)
+    (diagnostics.jl:15-31) on the splitting's particles (the GPU copy)."""
+    if h.sync:
+        h.to_device(fields=False)
+    check(_lib.load().gempic_solve_poisson(h._h, ptr(efield_dofs), ptr(rho)))
+
+
+class TimeHistoryDiagnostics:
+    """TimeHistoryDiagnostics (diagnostics.jl:128-170): rows of the 11
+    columns, filled by write_step_."""
+
+    COLUMNS = ("Time", "KineticEnergy", "Momentum1", "Momentum2", "PotentialEnergyE1",
+               "PotentialEnergyE2", "PotentialEnergyB3", "Transfer", "VVB", "Poynting",
+               "ErrorPoisson")
+
+    def __init__(self, h: HamiltonianSplitting):
+        self.splitting = h
+        self.data = []
+
+
+def write_step_(thdiag: TimeHistoryDiagnostics, time: float, degree: int, efield_dofs_n=None,
+                efield_poisson=None):
+    """write_step! (diagnostics.jl:186-250) of the splitting's current fields
+    and particles; efield_dofs_n defaults to the current fields."""
+    h = thdiag.splitting
+    if h.sync:
+        h.to_device()
+    out = np.zeros(11)
+    e1n = e2n = None
+    if efield_dofs_n is not None:
+        e1n = np.ascontiguousarray(efield_dofs_n[0], dtype=np.float64)
+        e2n = np.ascontiguousarray(efield_dofs_n[1], dtype=np.float64)
+    ep = None if efield_poisson is None else np.ascontiguousarray(efield_poisson, dtype=np.float64)
+    check(_lib.load().gempic_write_step(h._h, float(time), int(degree), ptr(e1n), ptr(e2n),
+                                        ptr(ep), ptr(out)))
+    thdiag.data.append(out)
+    return out

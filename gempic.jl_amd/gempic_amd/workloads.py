@@ -1,0 +1,96 @@
+"""Seeded synthetic initial conditions of the benchmark workloads.
+
+There is no data to load: particles are drawn from a seeded numpy
+generator, so the same arrays feed the GPU path and any checker.
+
+* weibel_1d2v: 1D2V Weibel instability, x in [0, 2pi/k), density
+  1 + alpha*cos(kx) (rejection sampling), v1 ~ N(0, s1^2), v2 ~ N(0, s2^2),
+  weight = Lx (the reference sampler's `dimx`, particle_sampling.jl:279),
+  common_weight = 1/N, B0 = l2projection!(beta*cos(kx), p-1).
+* landau_1d1v: 1D1V strong Landau damping, x in [0, 2pi/k), density
+  1 + alpha*cos(kx), v ~ N(0, 1).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+
+def sample_cos_density(rng, n, L, k, alpha):
+    """Rejection sampling of 1 + alpha*cos(k x) on [0, L)."""
+    out = np.empty(n)
+    filled = 0
+    bound = 1.0 + abs(alpha)
+    while filled < n:
+        m = int((n - filled) * 1.1) + 16
+        u = rng.random(m) * L
+        acc = rng.random(m) * bound < (1.0 + alpha * np.cos(k * u))
+        take = u[acc][: n - filled]
+        out[filled: filled + take.size] = take
+        filled += take.size
+    return out
+
+
+def bspline_basis(degree, x):
+    b = [0.0] * (degree + 1)
+    b[0] = 1.0
+    for j in range(1, degree + 1):
+        xx = -x
+        jr = float(j)
+        inv_j = 1.0 / jr
+        saved = 0.0
+        for r in range(j):
+            xx = xx + 1.0
+            temp = b[r] * inv_j
+            b[r] = saved + xx * temp
+            saved = (jr - xx) * temp
+        b[j] = saved
+    return b
+
+
+def l2projection_cos(nx, L, degree, amp, k):
+    """l2projection!(amp*cos(k x), degree) of maxwell_1d_fem.jl:347-374, which
+    returns its right-hand side divided by dx (the mass solve lands in the
+    workspace only)."""
+    dx = L / nx
+    xg, wg = np.polynomial.legendre.leggauss(degree + 1)
+    xg = 0.5 * (xg + 1.0)
+    wg = 0.5 * wg
+    bspl = [bspline_basis(degree, xk) for xk in xg]
+    out = np.zeros(nx)
+    for i in range(1, nx + 1):
+        coef = 0.0
+        for j in range(1, degree + 2):
+            for kq in range(1, degree + 2):
+                xq = ((xg[kq - 1] + i) + j) - 2.0
+                coef = coef + wg[kq - 1] * (amp * math.cos(k * (dx * xq))) * bspl[kq - 1][degree + 1 - j]
+        out[i - 1] = coef * dx
+    return out / dx
+
+
+def weibel_1d2v(n, seed=1, nx=32, k=1.25, alpha=1e-4, beta=1e-4, sigma1=0.02 / math.sqrt(2.0),
+                sigma2=None, degree=3):
+    if sigma2 is None:
+        sigma2 = math.sqrt(12.0) * sigma1
+    L = 2.0 * math.pi / k
+    rng = np.random.default_rng(seed)
+    parts = np.empty((4, n), order="F")
+    parts[0] = sample_cos_density(rng, n, L, k, alpha)
+    parts[1] = rng.standard_normal(n) * sigma1
+    parts[2] = rng.standard_normal(n) * sigma2
+    parts[3] = L
+    b0 = l2projection_cos(nx, L, degree - 1, beta, k)
+    return dict(xmin=0.0, xmax=L, nx=nx, degree=degree, particles=parts, b0=b0,
+                common_weight=1.0 / n, dt=0.05)
+
+
+def landau_1d1v(n, seed=1, nx=32, k=0.5, alpha=0.5, degree=3):
+    L = 2.0 * math.pi / k
+    rng = np.random.default_rng(seed)
+    parts = np.empty((3, n), order="F")
+    parts[0] = sample_cos_density(rng, n, L, k, alpha)
+    parts[1] = rng.standard_normal(n)
+    parts[2] = L
+    return dict(xmin=0.0, xmax=L, nx=nx, degree=degree, particles=parts,
+                common_weight=1.0 / n, dt=0.05)

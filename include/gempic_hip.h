@@ -1,0 +1,211 @@
+/*
+ * gempic_hip.h -- C ABI of the MI355X (gfx950) particle-field library.
+ *
+ * A Julia binding (`ccall`) or any C FFI drives GEMPIC.jl's particle-field
+ * hot path through these entry points. They take plain pointers and sizes;
+ * all device memory belongs to opaque handles. Nothing here falls back to
+ * the CPU: without a usable GPU every compute call fails with
+ * GEMPIC_ERR_DEVICE.
+ *
+ * Conventions (same as the reference, GEMPIC.jl):
+ *   - particle arrays are the ParticleGroup matrix, column-major
+ *     (D+V+n_weights) x N Float64: element (r, i) at aos[r + i*ld]
+ *     (particle_group.jl:15-46);
+ *   - grid arrays (e_dofs, b_dofs, j_dofs, rho) are nx Float64;
+ *   - indices and counts are int64_t;
+ *   - every entry point returns 0 (GEMPIC_OK) or an error code, with a
+ *     message from gempic_last_error(). ArgumentError / AssertionError of
+ *     the reference map to GEMPIC_ERR_ARGUMENT / GEMPIC_ERR_ASSERTION, and
+ *     a non-finite or out-of-range particle position (InexactError from
+ *     trunc(Int, x) in the reference) to GEMPIC_ERR_INEXACT.
+ *   - A handle is not re-entrant: callers serialize calls on one handle.
+ *     Each handle owns one HIP stream; calls return after their work is
+ *     complete unless stated otherwise.
+ */
+#ifndef GEMPIC_HIP_H
+#define GEMPIC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GEMPIC_OK 0
+#define GEMPIC_ERR_ARGUMENT 1
+#define GEMPIC_ERR_ASSERTION 2
+#define GEMPIC_ERR_DEVICE 3
+#define GEMPIC_ERR_INEXACT 4
+#define GEMPIC_ERR_STATE 5
+
+/* smoothing_type of ParticleMeshCoupling1D (particle_mesh_coupling_1d.jl:56-65) */
+#define GEMPIC_GALERKIN 0
+#define GEMPIC_COLLOCATION 1
+
+typedef struct gempic_splitting gempic_splitting;
+typedef struct gempic_maxwell1d gempic_maxwell1d;
+
+/* Library version (major*10000 + minor*100 + patch) and last error message
+ * of the calling thread (valid until the next call on that thread). */
+int gempic_version(void);
+const char *gempic_last_error(void);
+
+/* Number of visible GPUs (0 when none). */
+int gempic_device_count(void);
+
+/* ------------------------------------------------------------------------
+ * HamiltonianSplitting{1,V} (hamiltonian_splitting.jl:20-86) together with
+ * the Maxwell1DFEM (maxwell_1d_fem.jl:49-177), the two
+ * ParticleMeshCoupling1D smoothers (particle_mesh_coupling_1d.jl:44-95) and
+ * the ParticleGroup{1,V} (particle_group.jl:25-45) it holds.
+ *
+ * nx, xmin, xmax: the OneDGrid (mesh.jl:52-67); maxwell_degree: s_deg_0 of
+ * Maxwell1DFEM (1..3); degree0/smoothing0: kernel_smoother_0, degree1/
+ * smoothing1: kernel_smoother_1 (degrees 0..5); dim_v: 1 (1d1v) or 2 (1d2v);
+ * n_weights must be 1; common_weight 0 means 1/N as in ParticleGroup.
+ * ------------------------------------------------------------------------ */
+int gempic_splitting_create(int64_t nx, double xmin, double xmax, int maxwell_degree,
+                            int degree0, int smoothing0, int degree1, int smoothing1,
+                            int dim_v, int64_t n_particles, int n_weights, double charge,
+                            double mass, double common_weight, gempic_splitting **out);
+int gempic_splitting_destroy(gempic_splitting *h);
+
+/* Particle array in and out (the aliased ParticleGroup.array).
+ * ld = 1 + dim_v + n_weights. */
+int gempic_set_particles(gempic_splitting *h, const double *aos, int64_t ld, int64_t n);
+int gempic_get_particles(gempic_splitting *h, double *aos, int64_t ld, int64_t n);
+
+/* e_dofs[1], e_dofs[2], b_dofs (the aliased field vectors,
+ * hamiltonian_splitting.jl:80-81); any pointer may be NULL to skip it. */
+int gempic_set_fields(gempic_splitting *h, const double *e1, const double *e2, const double *b);
+int gempic_get_fields(gempic_splitting *h, double *e1, double *e2, double *b);
+/* j_dofs[1], j_dofs[2] as left by the last operator */
+int gempic_get_j(gempic_splitting *h, double *j1, double *j2);
+
+/* Sub-steps (drop-in for the Julia methods):
+ *   operatorHB  -- hamiltonian_splitting_1d2v.jl:234-236 / _1d1v.jl:113-125
+ *   operatorHE  -- hamiltonian_splitting_1d2v.jl:191-219 (1d2v only)
+ *   operatorHp1 -- hamiltonian_splitting_1d2v.jl:41-112 / _1d1v.jl:63-98
+ *   operatorHp2 -- hamiltonian_splitting_1d2v.jl:129-176 (1d2v only) */
+int gempic_operator_hb(gempic_splitting *h, double dt);
+int gempic_operator_he(gempic_splitting *h, double dt);
+int gempic_operator_hp1(gempic_splitting *h, double dt);
+int gempic_operator_hp2(gempic_splitting *h, double dt);
+
+/* strang_splitting! (hamiltonian_splitting.jl:98-108; _1d1v.jl:11-21).
+ * GEMPIC_STRANG_FUSED (default) runs one particle pass per step: the
+ * step-boundary HE pair and Hp2-Hp1-Hp2 read only B between two field
+ * solves, so they are fused; the field solves run in the pass's last
+ * workgroup. GEMPIC_STRANG_SUBSTEP launches the seven sub-steps one by one.
+ * Particle states and fields after each step are those of the sub-step
+ * sequence. With GEMPIC_ASYNC the call returns once the work is queued. */
+#define GEMPIC_STRANG_FUSED 0
+#define GEMPIC_STRANG_SUBSTEP 1
+#define GEMPIC_ASYNC 16
+int gempic_strang_splitting(gempic_splitting *h, double dt, int64_t n_steps, int flags);
+
+/* Wait for queued work and report any deferred error. */
+int gempic_synchronize(gempic_splitting *h);
+
+/* Particles are kept sorted by cell on the device (their order in the
+ * caller's array is restored by gempic_get_particles). interval = re-sort
+ * every `interval` Strang steps (0 = never after the upload). */
+int gempic_set_sort_interval(gempic_splitting *h, int64_t interval);
+int gempic_sort_particles(gempic_splitting *h);
+
+/* solve_poisson! (diagnostics.jl:15-31) on the handle's particles with
+ * kernel_smoother_0; efield and rho are host arrays of nx (either may be
+ * NULL). */
+int gempic_solve_poisson(gempic_splitting *h, double *efield, double *rho);
+
+/* write_step! (diagnostics.jl:186-250) for 1d2v: out[0..10] = Time,
+ * KineticEnergy, Momentum1, Momentum2, PotentialEnergyE1,
+ * PotentialEnergyE2, PotentialEnergyB3, Transfer, VVB, Poynting,
+ * ErrorPoisson. e*_n are the "efield_dofs_n" arguments (NULL = current
+ * fields), efield_poisson the host Poisson field (NULL = recompute with
+ * gempic_solve_poisson). */
+int gempic_write_step(gempic_splitting *h, double time, int degree, const double *e1_n,
+                      const double *e2_n, const double *efield_poisson, double *out11);
+
+/* Per-kernel timing with HIP events recorded on the handle's stream.
+ * kernel ids: 0 fused pass (the second kernel of a two-kernel step),
+ * 1 trailing pass, 2 HE, 3 Hp2, 4 Hp1, 5 HB/grid, 6 sort, 7 other,
+ * 8 first kernel of a two-kernel step. total_ms / count accumulate since the
+ * last reset. */
+int gempic_profile_enable(gempic_splitting *h, int enable);
+int gempic_profile_read(gempic_splitting *h, int kernel_id, double *total_ms, int64_t *count);
+int gempic_profile_reset(gempic_splitting *h);
+
+/* Workgroups per particle pass (0 = automatic: CUs times the workgroups of
+ * the pass kernel that fit on one CU; at most 8 per CU). */
+int gempic_set_launch(gempic_splitting *h, int blocks);
+
+/* Fused Strang path of 1d2v in one particle kernel per step or in two
+ * (HE pair + first Hp2, then Hp1 + second Hp2 + field solves). Both give
+ * bitwise the same results; two kernels hold fewer registers. Default 1. */
+int gempic_set_fusion(gempic_splitting *h, int kernels_per_step);
+
+/* ------------------------------------------------------------------------
+ * ParticleMeshCoupling1D batched per-particle verbs over host arrays of n
+ * particles, in the reference's particle order (one call = the reference's
+ * loop over particles). nx/xmin/xmax/degree/smoothing as in the
+ * constructor (particle_mesh_coupling_1d.jl:44-95).
+ * ------------------------------------------------------------------------ */
+/* add_charge! (:261-280): rho[nx] += sum_i deposit(x[i], q[i]) */
+int gempic_pmc1d_add_charge(int64_t nx, double xmin, double xmax, int degree, int smoothing,
+                            int64_t n, const double *x, const double *q, double *rho);
+/* add_charge_pp! (:106-122) */
+int gempic_pmc1d_add_charge_pp(int64_t nx, double xmin, double xmax, int degree, int smoothing,
+                               int64_t n, const double *x, const double *q, double *rho);
+/* evaluate (:438-453; batched form diagnostics.jl:260-268): out[i] = f(x[i]) */
+int gempic_pmc1d_evaluate(int64_t nx, double xmin, double xmax, int degree, int smoothing,
+                          int64_t n, const double *x, const double *field, double *out);
+/* evaluate_pp (:242-250) on the (degree+1) x nx pp matrix from b_to_pp */
+int gempic_pmc1d_evaluate_pp(int64_t nx, double xmin, double xmax, int degree, int smoothing,
+                             int64_t n, const double *x, const double *field_pp, double *out);
+/* add_current_update_v! with B (:296-376) when bfield != NULL, else the
+ * 1d1v form without B (:471-529). j[nx] += current; v[i] updated in place
+ * (unchanged without B). */
+int gempic_pmc1d_add_current_update_v(int64_t nx, double xmin, double xmax, int degree,
+                                      int smoothing, int64_t n, const double *x_old,
+                                      const double *x_new, const double *q, double qm,
+                                      const double *bfield, double *v, double *j);
+/* add_current_update_v_pp! (:131-190) */
+int gempic_pmc1d_add_current_update_v_pp(int64_t nx, double xmin, double xmax, int degree,
+                                         int smoothing, int64_t n, const double *x_old,
+                                         const double *x_new, const double *q, double qm,
+                                         const double *bfield, double *v, double *j);
+
+/* b_to_pp (splinepp.jl:241-261): pp[(degree+1) x ncells] from b[ncells] */
+int gempic_b_to_pp(int degree, int64_t ncells, const double *b, double *pp);
+
+/* ------------------------------------------------------------------------
+ * Maxwell1DFEM (maxwell_1d_fem.jl) verbs on host arrays, run on the GPU.
+ * ------------------------------------------------------------------------ */
+int gempic_maxwell1d_create(int64_t nx, double xmin, double xmax, int degree,
+                            gempic_maxwell1d **out);
+int gempic_maxwell1d_destroy(gempic_maxwell1d *m);
+/* eigenvalue tables in halfcomplex layout (:94-153); any may be NULL */
+int gempic_maxwell1d_tables(gempic_maxwell1d *m, double *eig_mass0, double *eig_mass1,
+                            double *eig_weak_ampere, double *eig_weak_poisson);
+/* compute_e_from_rho! (:244-255) */
+int gempic_maxwell1d_compute_e_from_rho(gempic_maxwell1d *m, double *e, const double *rho);
+/* compute_e_from_j! (:263-289), component 1 or 2 */
+int gempic_maxwell1d_compute_e_from_j(gempic_maxwell1d *m, double *e, const double *current,
+                                      int component);
+/* compute_e_from_b! (:384-396) */
+int gempic_maxwell1d_compute_e_from_b(gempic_maxwell1d *m, double *e, double dt,
+                                      const double *b);
+/* compute_b_from_e! (:407-420) */
+int gempic_maxwell1d_compute_b_from_e(gempic_maxwell1d *m, double *b, double dt,
+                                      const double *e);
+/* inner_product (:461-475) and l2norm_squared (:299-313) */
+int gempic_maxwell1d_inner_product(gempic_maxwell1d *m, const double *c1, const double *c2,
+                                   int degree, double *out);
+int gempic_maxwell1d_l2norm_squared(gempic_maxwell1d *m, const double *c, int degree,
+                                    double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
