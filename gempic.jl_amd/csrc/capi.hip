@@ -266,6 +266,8 @@ int run_pass(gempic_splitting *h, int mode, double dt_he, double dt_hp2, double 
     a.dt_he = dt_he;
     a.dt_hp2 = dt_hp2;
     a.dt_hp1 = dt_hp1;
+    a.dtqm_he = dt_he * h->qm;
+    a.dtqm_hp2 = dt_hp2 * h->qm;
     a.more = more;
     ProfScope ps(h, prof_id);
     hipError_t e = launch_pass(h->deg0, h->deg1, mode, a, L.blocks, pass_shmem_bytes((int)h->nx), h->stream);

@@ -36,11 +36,13 @@ __device__ __forceinline__ double jl_mod(double x, double y)
 // take a compare-and-add path instead of an integer division
 __device__ __forceinline__ int imod(int a, int n)
 {
-    if ((unsigned)a < (unsigned)n) return a;
-    if (a < 0 && a >= -n) return a + n;
-    if (a >= n && a < 2 * n) return a - n;
-    int r = a % n;
-    return r < 0 ? r + n : r;
+    int r = a >= n ? a - n : a;
+    r = r < 0 ? r + n : r;
+    if ((unsigned)r >= (unsigned)n) {
+        r = a % n;
+        r = r < 0 ? r + n : r;
+    }
+    return r;
 }
 
 // uniform_bsplines_eval_basis! (low_level_bsplines.jl:63-80): the P+1

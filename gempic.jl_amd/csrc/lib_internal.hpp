@@ -46,6 +46,7 @@ struct PassArgs {
     double qx[3], qw[3];          // Gauss-Legendre rule of kernel_smoother_1
     double charge, mass, cw, qm;
     double dt_he, dt_hp2, dt_hp1;
+    double dtqm_he, dtqm_hp2;     // dt_he*qm and dt_hp2*qm, rounded as in `dt * qm * ...`
     double *x, *v1, *v2, *w;
     double *grid;                 // NSLOT x nx
     const double *tabs;           // NTAB x nx

@@ -499,7 +499,7 @@ __global__ void __launch_bounds__(kBlock) k_pass1d(PassArgs a)
         if constexpr (!V1 && (HE_PREV || HE)) {
             // operatorHE 1d2v (hamiltonian_splitting_1d2v.jl:198-215)
             const double e1 = gather_p<P1>(b1, sE1, key_old);
-            const double dtqm = a.dt_he * qm;
+            const double dtqm = a.dtqm_he;
             if (HE_PREV) {
                 const double e2 = gather_p<P0>(b0, sE2p, key_old);
                 v1 = v1 + dtqm * e1;
@@ -514,13 +514,15 @@ __global__ void __launch_bounds__(kBlock) k_pass1d(PassArgs a)
         if constexpr (HP2A) {
             // operatorHp2 (hamiltonian_splitting_1d2v.jl:146-163)
             const double bf = gather_p<P1>(b1, sB, key_old);
-            v1 = v1 + a.dt_hp2 * qm * v2 * bf;
+            v1 = v1 + a.dtqm_hp2 * v2 * bf;
             const double wq = q * v2;
             double vals[W0];
 #pragma unroll
             for (int k = 0; k < W0; ++k) vals[k] = wq * b0[k] * a.scaling0;
             cache_add<W0, 1>(c_j2a, valid, key_old, vals, myrow + D_J2A * nx, nx);
         }
+        double xi_new = 0.0;       // (x - xmin) / dx after Hp1, when the mod kept x_new
+        bool xi_new_ok = false;
         if constexpr (HP1) {
             // operatorHp1 (hamiltonian_splitting_1d2v.jl:53-82 / _1d1v.jl:308-329)
             const double xn = x + a.dt_hp1 * v1;
@@ -556,11 +558,14 @@ __global__ void __launch_bounds__(kBlock) k_pass1d(PassArgs a)
                 cache_add<W1, 1>(c_j1, act, key, vals, myrow + D_J1 * nx, nx);
             }
             x = jl_mod(xn, a.Lx);
+            xi_new = xin;
+            xi_new_ok = __double_as_longlong(x) == __double_as_longlong(xn);
         }
         if constexpr (NEED_NEW) {
             // rho deposit at the new position (hamiltonian_splitting_1d2v.jl:93-102,
-            // diagnostics.jl:189-193) and the second Hp2
-            const double xi2 = (x - xmin) / dx;
+            // diagnostics.jl:189-193) and the second Hp2; the same division as
+            // Hp1's when the mod left the position unchanged
+            const double xi2 = xi_new_ok ? xi_new : (x - xmin) / dx;
             if (valid && RHO_ONLY && !pos_ok(xi2)) errs |= E_INEXACT;
             const int ic2 = (int)xi2;
             const double off2 = xi2 - (double)ic2;
@@ -571,7 +576,7 @@ __global__ void __launch_bounds__(kBlock) k_pass1d(PassArgs a)
                 double c1[W1];
                 bspline<P1>(off2, c1);
                 const double bf = gather_p<P1>(c1, sB, key2);
-                v1 = v1 + a.dt_hp2 * qm * v2 * bf;
+                v1 = v1 + a.dtqm_hp2 * v2 * bf;
             }
             double vals[W0 * RR];
 #pragma unroll
