@@ -62,6 +62,24 @@ def cpu_baseline(n, steps, seed=1):
     return n * steps / dt, dt
 
 
+def copy_calibration(mib):
+    """Run a plain device copy of `mib` MiB five times (reads and writes mib MiB each).
+
+    Used only under `rocprofv3 --pmc FETCH_SIZE` / `WRITE_SIZE` to scale those
+    counters to bytes: the counters of a copy of known size give the factor.
+    torch provides the device memory and the elementwise copy kernel.
+    """
+    import torch
+    n = mib * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        torch.mul(src, 1.0, out=dst)        # one elementwise kernel: n*8 B read, n*8 B written
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_mib": mib, "launches": 5,
+                      "bytes_read_per_launch": n * 8, "bytes_written_per_launch": n * 8}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,7 +93,12 @@ def main():
     ap.add_argument("--substep", action="store_true", help="also time the sub-step path")
     ap.add_argument("--blocks", type=int, default=0, help="workgroups per particle pass (0 = auto)")
     ap.add_argument("--fusion", type=int, default=1, help="particle kernels per fused step (1 or 2)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(particles would shard with one all-reduce of the nx-sized deposits per "
@@ -145,7 +168,7 @@ def main():
                          f"steps ({secs:.1f} s), C restatement of the reference, one thread"}
 
     line = {
-        "metric": "particle-steps/s (1D2V Weibel, 4M particles, FP64, fused Strang step)",
+        "metric": "particle-steps/s (1D2V, 4M particles, FP64) fwd & fwd+adjoint; % HBM roofline",
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,

@@ -117,8 +117,8 @@ struct gempic_splitting {
     unsigned *d_cnt = nullptr;
     int *d_err = nullptr;
     double *d_diag = nullptr;
-    int *d_sort_counts = nullptr, *d_sort_offsets = nullptr;
-    int ntiles = 0;
+    int *d_sort_counts = nullptr, *d_sort_gsum = nullptr;
+    SortGeometry sortg{};
     double *d_stage = nullptr;
 
     int max_blocks = 1024;            // capacity of the reduction scratch
@@ -304,13 +304,17 @@ int run_sort(gempic_splitting *h)
     SortArgs s;
     s.n = h->n;
     s.nx = (int)h->nx;
-    s.ntiles = h->ntiles;
+    s.sub = h->sortg.sub;
+    s.nbins = h->sortg.nbins;
+    s.ntiles = h->sortg.ntiles;
+    s.tiles_per_group = h->sortg.tiles_per_group;
+    s.ngroups = h->sortg.ngroups;
     s.xmin = h->xmin;
     s.dx = h->dx;
     s.x = h->d_x; s.v1 = h->d_v1; s.v2 = h->d_v2; s.w = h->d_w; s.id = h->d_id;
     s.x2 = h->d_x2; s.v12 = h->d_v12; s.v22 = h->d_v22; s.w2 = h->d_w2; s.id2 = h->d_id2;
     s.counts = h->d_sort_counts;
-    s.offsets = h->d_sort_offsets;
+    s.gsum = h->d_sort_gsum;
     hipError_t e = launch_sort(s, h->dim_v == 2, h->stream);
     if (e != hipSuccess) return fail(GEMPIC_ERR_DEVICE, std::string("sort launch: ") + hipGetErrorString(e));
     std::swap(h->d_x, h->d_x2);
@@ -394,6 +398,9 @@ int gempic_splitting_create(int64_t nx, double xmin, double xmax, int maxwell_de
     if (nx < 2 || nx > kNxMax || (nx % 2) != 0)
         return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: nx must be even and in [2, 256]");
     if (n_particles < 1) return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: n_particles must be positive");
+    if (n_particles > (int64_t)INT32_MAX - kSortTile)
+        return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: n_particles must be below 2^31 - 1024 per handle "
+                                         "(32-bit sort offsets)");
     if (!(xmax > xmin)) return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: xmax must exceed xmin");
     int rc = check_device();
     if (rc) return rc;
@@ -431,7 +438,7 @@ int gempic_splitting_create(int64_t nx, double xmin, double xmax, int maxwell_de
     h->max_blocks = h->cus * 8;
     const int max_groups = (h->max_blocks + kGroup - 1) / kGroup;
     const size_t n = (size_t)n_particles;
-    h->ntiles = (int)((n + kSortTile - 1) / kSortTile);
+    h->sortg = sort_geometry((int64_t)n, (int)nx);
     hipError_t e = hipSuccess;
     auto ok = [&](hipError_t r) { if (r != hipSuccess && e == hipSuccess) e = r; };
     ok(hipStreamCreate(&h->stream));
@@ -445,8 +452,8 @@ int gempic_splitting_create(int64_t nx, double xmin, double xmax, int maxwell_de
     ok(dalloc(&h->d_cnt, (size_t)max_groups + 2));
     ok(dalloc(&h->d_err, 1));
     ok(dalloc(&h->d_diag, 16));
-    ok(dalloc(&h->d_sort_counts, (size_t)h->ntiles * nx));
-    ok(dalloc(&h->d_sort_offsets, (size_t)h->ntiles * nx));
+    ok(dalloc(&h->d_sort_counts, (size_t)h->sortg.ntiles * h->sortg.nbins));
+    ok(dalloc(&h->d_sort_gsum, (size_t)h->sortg.ngroups * h->sortg.nbins));
     if (e == hipSuccess) {
         ok(hipMemset(h->d_grid, 0, sizeof(double) * NSLOT * nx));
         ok(hipMemset(h->d_cnt, 0, sizeof(unsigned) * (max_groups + 2)));
@@ -478,7 +485,7 @@ int gempic_splitting_destroy(gempic_splitting *h)
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     void *bufs[] = {h->d_x, h->d_v1, h->d_v2, h->d_w, h->d_id, h->d_x2, h->d_v12, h->d_v22,
                     h->d_w2, h->d_id2, h->d_grid, h->d_tabs, h->d_part, h->d_gpart, h->d_cnt,
-                    h->d_err, h->d_diag, h->d_sort_counts, h->d_sort_offsets, h->d_stage};
+                    h->d_err, h->d_diag, h->d_sort_counts, h->d_sort_gsum, h->d_stage};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     for (auto &p : h->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }

@@ -88,19 +88,28 @@ hipError_t launch_grid(const GridProg &g, int nx, double dx, double *grid, const
                        double *scalar_out, hipStream_t s);
 
 // Stable counting sort of SoA particles by cell.
+// Stable counting sort by (cell, sub-cell bin): bins = nx * sub, tile-major
+// counts. Sorting finer than the cell keeps each wave's particles within a
+// fraction of a cell, so the drift between sorts only disturbs the waves
+// next to a cell boundary.
 struct SortArgs {
     int64_t n;
-    int nx;
-    int ntiles;
+    int nx, sub, nbins;
+    int ntiles, tiles_per_group, ngroups;
     double xmin, dx;
     const double *x, *v1, *v2, *w;
     const int64_t *id;
     double *x2, *v12, *v22, *w2;
     int64_t *id2;
-    int *counts;      // ntiles x nx
-    int *offsets;     // ntiles x nx
+    int *counts;      // [ntiles][nbins]: bin counts, then (in place) scatter offsets
+    int *gsum;        // [ngroups][nbins]: group sums, then (in place) group bases
 };
-constexpr int kSortTile = 64 * 32;     // particles per wave tile
+constexpr int kSortTile = 64 * 16;     // particles per wave tile
+constexpr int kSortBinsMax = 1024;
+struct SortGeometry {
+    int sub, nbins, ntiles, tiles_per_group, ngroups;
+};
+SortGeometry sort_geometry(int64_t n, int nx);
 hipError_t launch_sort(const SortArgs &a, int has_v2, hipStream_t s);
 
 // AoS (reference ParticleGroup layout) <-> SoA with the permutation id.

@@ -7,6 +7,9 @@
 // device_common.hpp. Deposits are reduced workgroup -> group -> grid in a
 // fixed order, and the last workgroup to finish runs the field solves of
 // the sub-steps (the "epilogue") so that a whole Strang step is one launch.
+#include <algorithm>
+#include <cmath>
+
 #include "device_common.hpp"
 #include "lib_internal.hpp"
 
@@ -944,18 +947,24 @@ hipError_t launch_grid(const GridProg &g, int nx, double dx, double *grid, const
 }
 
 // ---------------------------------------------------------------------------
-// Stable counting sort by cell (wave tiles of kSortTile particles)
+// Stable counting sort by (cell, sub-cell bin), wave tiles of kSortTile
+// particles. Five launches: per-tile bin counts; per-group sums over
+// tiles_per_group tiles; bin bases and group bases (one workgroup); per-tile
+// offsets; scatter. Every step is a fixed function of the input order, so the
+// permutation is deterministic; particles of one bin keep their order.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int sort_key(double x, double xmin, double dx, int nx)
+__device__ __forceinline__ int sort_key(double x, double xmin, double dx, int nx, int sub)
 {
     double xi = (x - xmin) / dx;
     if (!(fabs(xi) < 1.0e9)) xi = 0.0;
-    return imod((int)xi, nx);
+    const int ic = (int)xi;                                  // the cell the passes use
+    int s = (int)((xi - (double)ic) * (double)sub);
+    s = s < 0 ? 0 : (s >= sub ? sub - 1 : s);
+    return imod(ic, nx) * sub + s;
 }
 
-// counts/offsets are cell-major: [cell][tile]. Each wave owns one tile of
-// kSortTile particles and walks it in batches of kSortBatch iterations whose
-// loads are issued together.
+// Each wave owns one tile and walks it in batches of kSortBatch iterations
+// whose loads are issued together.
 constexpr int kSortBatch = 8;
 
 __global__ void __launch_bounds__(kBlock) k_sort_count(SortArgs a)
@@ -963,9 +972,10 @@ __global__ void __launch_bounds__(kBlock) k_sort_count(SortArgs a)
     extern __shared__ int ish[];
     const int wave = threadIdx.x / kWave;
     const int lane = lane_id();
-    int *h = ish + wave * a.nx;
+    const int nb = a.nbins;
+    int *h = ish + wave * nb;
     const int tile = blockIdx.x * kWaves + wave;
-    for (int c = lane; c < a.nx; c += kWave) h[c] = 0;
+    for (int c = lane; c < nb; c += kWave) h[c] = 0;
     __syncthreads();
     if (tile < a.ntiles) {
         const int64_t t0 = (int64_t)tile * kSortTile;
@@ -980,7 +990,7 @@ __global__ void __launch_bounds__(kBlock) k_sort_count(SortArgs a)
             for (int u = 0; u < kSortBatch; ++u) {
                 const int64_t i = t0 + (int64_t)(k0 + u) * kWave + lane;
                 const bool valid = i < a.n;
-                const int c = valid ? sort_key(xs[u], a.xmin, a.dx, a.nx) : -1;
+                const int c = valid ? sort_key(xs[u], a.xmin, a.dx, a.nx, a.sub) : -1;
                 unsigned long long pending = __ballot(valid);
                 while (pending) {
                     const int leader = __ffsll((long long)pending) - 1;
@@ -991,47 +1001,99 @@ __global__ void __launch_bounds__(kBlock) k_sort_count(SortArgs a)
                 }
             }
         }
-        for (int c = lane; c < a.nx; c += kWave) a.counts[(size_t)c * a.ntiles + tile] = h[c];
+        int *row = a.counts + (size_t)tile * nb;
+        for (int c = lane; c < nb; c += kWave) row[c] = h[c];
     }
 }
 
-// Exclusive scan of the cell-major counts (one workgroup of 1024):
-// offsets[c][t] = sum of all counts before (c, t) in cell-major order.
-__global__ void __launch_bounds__(1024) k_sort_scan(SortArgs a)
+// gsum[g][b] = sum of counts[t][b] over the tiles t of group g
+__global__ void __launch_bounds__(kBlock) k_sort_gsum(SortArgs a)
 {
-    constexpr int PER = 16;
-    __shared__ int s_part[1024];
-    __shared__ int s_carry;
-    const int tid = threadIdx.x;
-    const int64_t total = (int64_t)a.nx * a.ntiles;
-    if (tid == 0) s_carry = 0;
+    const int nb = a.nbins;
+    const int b = blockIdx.y * kBlock + threadIdx.x;
+    if (b >= nb) return;
+    const int g = blockIdx.x;
+    const int t0 = g * a.tiles_per_group;
+    const int t1 = min(a.ntiles, t0 + a.tiles_per_group);
+    const int *col = a.counts + b;
+    int s = 0;
+    int t = t0;
+    for (; t + kSortBatch <= t1; t += kSortBatch) {
+        int v[kSortBatch];
+#pragma unroll
+        for (int u = 0; u < kSortBatch; ++u) v[u] = col[(size_t)(t + u) * nb];
+#pragma unroll
+        for (int u = 0; u < kSortBatch; ++u) s += v[u];
+    }
+    for (; t < t1; ++t) s += col[(size_t)t * nb];
+    a.gsum[(size_t)g * nb + b] = s;
+}
+
+// One workgroup: bin totals, their exclusive scan (the bin bases), and
+// gsum[g][b] <- base[b] + sum of gsum[g'][b] over g' < g.
+__global__ void __launch_bounds__(kSortBinsMax) k_sort_bases(SortArgs a)
+{
+    __shared__ int s_scan[kSortBinsMax];
+    const int nb = a.nbins;
+    const int b = threadIdx.x;
+    int *col = a.gsum + b;
+    int total = 0;
+    if (b < nb)
+        for (int g = 0; g < a.ngroups; ++g) total += col[(size_t)g * nb];
+    s_scan[b] = b < nb ? total : 0;
     __syncthreads();
-    for (int64_t base = 0; base < total; base += 1024 * PER) {
-        const int64_t b0 = base + (int64_t)tid * PER;
-        int v[PER];
-        int s = 0;
-#pragma unroll
-        for (int u = 0; u < PER; ++u) {
-            v[u] = (b0 + u < total) ? a.counts[b0 + u] : 0;
-            s += v[u];
-        }
-        s_part[tid] = s;
+    for (int off = 1; off < kSortBinsMax; off <<= 1) {
+        const int add = b >= off ? s_scan[b - off] : 0;
         __syncthreads();
-        for (int off = 1; off < 1024; off <<= 1) {
-            const int add = tid >= off ? s_part[tid - off] : 0;
-            __syncthreads();
-            s_part[tid] += add;
-            __syncthreads();
-        }
-        int run = s_carry + s_part[tid] - s;
+        s_scan[b] += add;
+        __syncthreads();
+    }
+    if (b >= nb) return;
+    int run = s_scan[b] - total;
+    int g = 0;
+    for (; g + kSortBatch <= a.ngroups; g += kSortBatch) {
+        int v[kSortBatch];
 #pragma unroll
-        for (int u = 0; u < PER; ++u) {
-            if (b0 + u < total) a.offsets[b0 + u] = run;
+        for (int u = 0; u < kSortBatch; ++u) v[u] = col[(size_t)(g + u) * nb];
+#pragma unroll
+        for (int u = 0; u < kSortBatch; ++u) {
+            col[(size_t)(g + u) * nb] = run;
             run += v[u];
         }
-        __syncthreads();
-        if (tid == 1023) s_carry += s_part[1023];
-        __syncthreads();
+    }
+    for (; g < a.ngroups; ++g) {
+        const int v = col[(size_t)g * nb];
+        col[(size_t)g * nb] = run;
+        run += v;
+    }
+}
+
+// counts[t][b] <- the first output position of bin b's particles of tile t
+__global__ void __launch_bounds__(kBlock) k_sort_offsets(SortArgs a)
+{
+    const int nb = a.nbins;
+    const int b = blockIdx.y * kBlock + threadIdx.x;
+    if (b >= nb) return;
+    const int g = blockIdx.x;
+    const int t0 = g * a.tiles_per_group;
+    const int t1 = min(a.ntiles, t0 + a.tiles_per_group);
+    int *col = a.counts + b;
+    int run = a.gsum[(size_t)g * nb + b];
+    int t = t0;
+    for (; t + kSortBatch <= t1; t += kSortBatch) {
+        int v[kSortBatch];
+#pragma unroll
+        for (int u = 0; u < kSortBatch; ++u) v[u] = col[(size_t)(t + u) * nb];
+#pragma unroll
+        for (int u = 0; u < kSortBatch; ++u) {
+            col[(size_t)(t + u) * nb] = run;
+            run += v[u];
+        }
+    }
+    for (; t < t1; ++t) {
+        const int v = col[(size_t)t * nb];
+        col[(size_t)t * nb] = run;
+        run += v;
     }
 }
 
@@ -1040,10 +1102,13 @@ __global__ void __launch_bounds__(kBlock) k_sort_scatter(SortArgs a, int has_v2)
     extern __shared__ int ish[];
     const int wave = threadIdx.x / kWave;
     const int lane = lane_id();
-    int *cur = ish + wave * a.nx;
+    const int nb = a.nbins;
+    int *cur = ish + wave * nb;
     const int tile = blockIdx.x * kWaves + wave;
-    if (tile < a.ntiles)
-        for (int c = lane; c < a.nx; c += kWave) cur[c] = a.offsets[(size_t)c * a.ntiles + tile];
+    if (tile < a.ntiles) {
+        const int *row = a.counts + (size_t)tile * nb;
+        for (int c = lane; c < nb; c += kWave) cur[c] = row[c];
+    }
     __syncthreads();
     if (tile >= a.ntiles) return;
     const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -1065,7 +1130,7 @@ __global__ void __launch_bounds__(kBlock) k_sort_scatter(SortArgs a, int has_v2)
         for (int u = 0; u < kSortBatch; ++u) {
             const int64_t i = t0 + (int64_t)(k0 + u) * kWave + lane;
             const bool valid = i < a.n;
-            const int c = valid ? sort_key(xs[u], a.xmin, a.dx, a.nx) : -1;
+            const int c = valid ? sort_key(xs[u], a.xmin, a.dx, a.nx, a.sub) : -1;
             unsigned long long pending = __ballot(valid);
             int pos = 0;
             while (pending) {
@@ -1088,13 +1153,31 @@ __global__ void __launch_bounds__(kBlock) k_sort_scatter(SortArgs a, int has_v2)
     }
 }
 
+SortGeometry sort_geometry(int64_t n, int nx)
+{
+    SortGeometry g;
+    g.sub = nx > 0 ? std::max(1, std::min(32, kSortBinsMax / nx)) : 1;
+    g.nbins = nx * g.sub;
+    g.ntiles = (int)std::max<int64_t>(1, (n + kSortTile - 1) / kSortTile);
+    int tpg = (int)std::ceil(std::sqrt((double)g.ntiles));
+    g.tiles_per_group = std::max(16, tpg);
+    g.ngroups = (g.ntiles + g.tiles_per_group - 1) / g.tiles_per_group;
+    return g;
+}
+
 hipError_t launch_sort(const SortArgs &a, int has_v2, hipStream_t s)
 {
-    if (a.nx > kNxMax) return hipErrorInvalidValue;
+    if (a.nx > kNxMax || a.nbins > kSortBinsMax || a.nbins != a.nx * a.sub || a.ntiles < 1 ||
+        (int64_t)a.ntiles * kSortTile < a.n || a.ngroups < 1 ||
+        (int64_t)a.ngroups * a.tiles_per_group < a.ntiles)
+        return hipErrorInvalidValue;
     const int blocks = (a.ntiles + kWaves - 1) / kWaves;
-    const size_t sh = sizeof(int) * (size_t)kWaves * a.nx;
+    const size_t sh = sizeof(int) * (size_t)kWaves * a.nbins;
+    const dim3 cols(a.ngroups, (a.nbins + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_sort_count, dim3(blocks), dim3(kBlock), sh, s, a);
-    hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, a);
+    hipLaunchKernelGGL(k_sort_gsum, cols, dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_sort_bases, dim3(1), dim3(kSortBinsMax), 0, s, a);
+    hipLaunchKernelGGL(k_sort_offsets, cols, dim3(kBlock), 0, s, a);
     hipLaunchKernelGGL(k_sort_scatter, dim3(blocks), dim3(kBlock), sh, s, a, has_v2);
     return hipGetLastError();
 }
