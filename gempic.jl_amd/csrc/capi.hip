@@ -236,7 +236,6 @@ bool mode_deposits(int mode)
 // fewest workgroups per CU among those kernels (one full round, no tail).
 void set_dep_blocks(gempic_splitting *h)
 {
-    const size_t sh = pass_shmem_bytes((int)h->nx);
     std::vector<int> modes;
     if (h->dim_v == 2)
         modes = {M_HP2A, M_HP1, M_HE | M_HP2A | M_HP1 | M_HP2B,
@@ -245,7 +244,8 @@ void set_dep_blocks(gempic_splitting *h)
     else
         modes = {M_V1 | M_HP1, M_V1 | M_HE | M_HP1, M_V1 | M_HE_PREV | M_HE | M_HP1, M_RHO};
     int bpc = 64;
-    for (int m : modes) bpc = std::min(bpc, pass_blocks_per_cu(h->deg0, h->deg1, m, sh));
+    for (int m : modes)
+        bpc = std::min(bpc, pass_blocks_per_cu(h->deg0, h->deg1, m, pass_shmem_bytes((int)h->nx, m)));
     h->dep_blocks = std::max(1, std::min(h->max_blocks, h->cus * bpc));
 }
 
@@ -258,7 +258,7 @@ int run_pass(gempic_splitting *h, int mode, double dt_he, double dt_hp2, double 
     else if (mode_deposits(mode)) target = h->dep_blocks;
     else target = std::min(h->max_blocks,
                            h->cus * pass_blocks_per_cu(h->deg0, h->deg1, mode,
-                                                       pass_shmem_bytes((int)h->nx)));
+                                                       pass_shmem_bytes((int)h->nx, mode)));
     Launch L = plan(h->n, target);
     a.chunk = L.chunk;
     a.iters = L.iters;
@@ -270,7 +270,7 @@ int run_pass(gempic_splitting *h, int mode, double dt_he, double dt_hp2, double 
     a.dtqm_hp2 = dt_hp2 * h->qm;
     a.more = more;
     ProfScope ps(h, prof_id);
-    hipError_t e = launch_pass(h->deg0, h->deg1, mode, a, L.blocks, pass_shmem_bytes((int)h->nx), h->stream);
+    hipError_t e = launch_pass(h->deg0, h->deg1, mode, a, L.blocks, pass_shmem_bytes((int)h->nx, mode), h->stream);
     if (e != hipSuccess) return fail(GEMPIC_ERR_DEVICE, std::string("particle pass launch: ") + hipGetErrorString(e));
     return GEMPIC_OK;
 }

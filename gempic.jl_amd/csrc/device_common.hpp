@@ -100,38 +100,48 @@ __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1
 // Deterministic deposit accumulation.
 //
 // Each lane keeps two cached cells (A, B) with the W = P+1 node sums of the
-// particles it has seen in each. Particles are kept sorted by cell, so a
-// lane's particles mostly fall in one cell and the odd one that has drifted
-// into a neighbour lands in the second entry. When a lane meets a third
-// cell, its least-recently-used entry is flushed: the wave runs a segmented
-// reduction over all lanes flushing the same cell (xor butterfly, fixed
-// order) and lane 0 adds the W sums into the wave's private LDS row. The
-// order of every addition is a fixed function of the particle order, so the
-// deposit is bit-reproducible from run to run.
+// particles it has seen in each. Particles are kept sorted by position, so a
+// lane's particles mostly fall in one cell (entry A, in registers) and the
+// odd one across a cell boundary lands in entry B, which lives in the lane's
+// private LDS slots (B is touched rarely, and keeping it out of registers
+// lets more waves share a SIMD). When a lane meets a third cell, its
+// least-recently-used entry is flushed: the wave runs a segmented reduction
+// over all lanes flushing the same cell (xor butterfly, fixed order) and lane
+// 0 adds the W sums into the wave's private LDS row. The order of every
+// addition is a fixed function of the particle order, so the deposit is
+// bit-reproducible from run to run.
 // ---------------------------------------------------------------------------
-template <int WN, int R = 1>
+template <int WN>
 struct Cache2 {
-    static constexpr int W = WN * R;
     int ca, cb;
     bool mru_b;
-    double a[W], b[W];
+    double a[WN];
+    double *bs;           // entry B: value w at bs[w * kWave] (this lane's LDS slots)
 
-    __device__ __forceinline__ void init()
+    // slots: the wave's WN * kWave doubles of LDS for this cache
+    __device__ __forceinline__ void init(double *slots)
+    {
+        bs = slots + lane_id();
+        reset();
+    }
+    __device__ __forceinline__ void reset()
     {
         ca = -1;
         cb = -1;
         mru_b = false;
 #pragma unroll
-        for (int w = 0; w < W; ++w) { a[w] = 0.0; b[w] = 0.0; }
+        for (int w = 0; w < WN; ++w) {
+            a[w] = 0.0;
+            bs[w * kWave] = 0.0;
+        }
     }
 };
 
-// Segmented flush: lanes with `want` set contribute v[r*WN + w] to node
-// first_node(cell, WN-1) + w of row r (rows are nx doubles apart); called by
-// all 64 lanes of the wave.
-template <int WN, int R>
-__device__ __forceinline__ void seg_flush(bool want, int cell, const double (&v)[WN * R],
-                                          double *row, int nx)
+// Segmented flush: lanes with `want` set contribute v[w] to node
+// first_node(cell, WN-1) + w of `row`; called by all 64 lanes of the wave.
+template <int WN>
+__device__ __forceinline__ void seg_flush(bool want, int cell, const double (&v)[WN], double *row,
+                                          int nx)
 {
     unsigned long long pending = __ballot(want);
     const int lane = lane_id();
@@ -140,26 +150,21 @@ __device__ __forceinline__ void seg_flush(bool want, int cell, const double (&v)
         int c = __shfl(cell, leader, kWave);
         bool mine = want && (cell == c);
         unsigned long long m = __ballot(mine);
-        const int k0 = first_node(c, WN - 1, nx);
+        int k = first_node(c, WN - 1, nx);
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            int k = k0;
-#pragma unroll
-            for (int w = 0; w < WN; ++w) {
-                double s = wave_sum(mine ? v[r * WN + w] : 0.0);
-                if (lane == 0) row[r * nx + k] += s;
-                k = next_node(k, nx);
-            }
+        for (int w = 0; w < WN; ++w) {
+            double s = wave_sum(mine ? v[w] : 0.0);
+            if (lane == 0) row[k] += s;
+            k = next_node(k, nx);
         }
         pending &= ~m;
     }
 }
 
-template <int WN, int R>
-__device__ __forceinline__ void cache_add(Cache2<WN, R> &C, bool valid, int cell,
-                                          const double (&v)[WN * R], double *row, int nx)
+template <int WN>
+__device__ __forceinline__ void cache_add(Cache2<WN> &C, bool valid, int cell,
+                                          const double (&v)[WN], double *row, int nx)
 {
-    constexpr int W = WN * R;
     bool hit_a = valid && cell == C.ca;
     bool hit_b = valid && !hit_a && cell == C.cb;
     bool miss = valid && !hit_a && !hit_b;
@@ -169,20 +174,20 @@ __device__ __forceinline__ void cache_add(Cache2<WN, R> &C, bool valid, int cell
         bool ev_a = use_a && C.ca >= 0;
         bool ev_b = use_b && C.cb >= 0;
         int ecell = ev_a ? C.ca : C.cb;
-        double ev[W];
+        double ev[WN];
 #pragma unroll
-        for (int w = 0; w < W; ++w) ev[w] = ev_a ? C.a[w] : C.b[w];
-        seg_flush<WN, R>(ev_a || ev_b, ecell, ev, row, nx);
+        for (int w = 0; w < WN; ++w) ev[w] = ev_a ? C.a[w] : C.bs[w * kWave];
+        seg_flush<WN>(ev_a || ev_b, ecell, ev, row, nx);
         if (use_a) {
             C.ca = cell;
 #pragma unroll
-            for (int w = 0; w < W; ++w) C.a[w] = 0.0;
+            for (int w = 0; w < WN; ++w) C.a[w] = 0.0;
             hit_a = true;
         }
         if (use_b) {
             C.cb = cell;
 #pragma unroll
-            for (int w = 0; w < W; ++w) C.b[w] = 0.0;
+            for (int w = 0; w < WN; ++w) C.bs[w * kWave] = 0.0;
             hit_b = true;
         }
     }
@@ -190,22 +195,25 @@ __device__ __forceinline__ void cache_add(Cache2<WN, R> &C, bool valid, int cell
     // particles) the B update is skipped for the whole wave
     if (hit_a) {
 #pragma unroll
-        for (int w = 0; w < W; ++w) C.a[w] += v[w];
+        for (int w = 0; w < WN; ++w) C.a[w] += v[w];
         C.mru_b = false;
     }
     if (hit_b) {
 #pragma unroll
-        for (int w = 0; w < W; ++w) C.b[w] += v[w];
+        for (int w = 0; w < WN; ++w) C.bs[w * kWave] += v[w];
         C.mru_b = true;
     }
 }
 
-template <int WN, int R>
-__device__ __forceinline__ void cache_drain(Cache2<WN, R> &C, double *row, int nx)
+template <int WN>
+__device__ __forceinline__ void cache_drain(Cache2<WN> &C, double *row, int nx)
 {
-    seg_flush<WN, R>(C.ca >= 0, C.ca, C.a, row, nx);
-    seg_flush<WN, R>(C.cb >= 0, C.cb, C.b, row, nx);
-    C.init();
+    seg_flush<WN>(C.ca >= 0, C.ca, C.a, row, nx);
+    double bv[WN];
+#pragma unroll
+    for (int w = 0; w < WN; ++w) bv[w] = C.bs[w * kWave];
+    seg_flush<WN>(C.cb >= 0, C.cb, bv, row, nx);
+    C.reset();
 }
 
 }  // namespace gk

@@ -24,7 +24,7 @@ enum ModeBits : int {
     M_HE_PREV = 1,   // HE (or 1d1v HB) of the previous Strang step, with E2 saved before it
     M_HE = 2,        // HE (1d2v) / HB (1d1v) with the current fields
     M_HP2A = 4,      // Hp2 before Hp1
-    M_HP1 = 8,       // Hp1 (+ rho deposit in 1d2v)
+    M_HP1 = 8,       // Hp1 (+ rho deposit in 1d2v when no Hp2 follows in the pass)
     M_HP2B = 16,     // Hp2 after Hp1
     M_V1 = 32,       // 1d1v variant
     M_RHO = 64,      // charge deposit only (solve_poisson!)
@@ -66,7 +66,10 @@ hipError_t launch_pass(int p0, int p1, int mode, const PassArgs &a, int blocks, 
                        hipStream_t s);
 // workgroups of the (p0, p1, mode) pass kernel resident on one CU
 int pass_blocks_per_cu(int p0, int p1, int mode, size_t shmem);
-size_t pass_shmem_bytes(int nx);
+// LDS of a particle pass of this mode: padded fields, deposit rows (reused as
+// reduction scratch and tables) and, for deposit modes, the cache-B slots.
+size_t pass_shmem_bytes(int nx, int mode);
+constexpr int kSlotsPerLane = 12;         // cache-B doubles per lane: three caches of <= 4 nodes
 
 // Single-workgroup field program on the splitting's grid buffer.
 enum GridOp : int {
@@ -87,7 +90,6 @@ struct GridProg {
 hipError_t launch_grid(const GridProg &g, int nx, double dx, double *grid, const double *tabs,
                        double *scalar_out, hipStream_t s);
 
-// Stable counting sort of SoA particles by cell.
 // Stable counting sort by (cell, sub-cell bin): bins = nx * sub, tile-major
 // counts. Sorting finer than the cell keeps each wave's particles within a
 // fraction of a cell, so the drift between sorts only disturbs the waves

@@ -85,23 +85,26 @@ __device__ void circ_solve(const double *eig, const double *rhs, double *out, do
 // Three independent solves side by side: thread group g (a third of the
 // workgroup) computes out[g] = HC2R(eig[g] (.) R2HC(rhs[g])) / n with the
 // same sums, in the same order, as circ_solve_part.
-struct Solve3 {
-    const double *eig[3];
-    const double *rhs[3];
-    double *out[3];
-    double *hc[3];
+// One circulant solve per third of the workgroup: solve g (0, 1, 2) reads
+// rhs_g, multiplies by eig_g in halfcomplex form and writes out_g, with hc_g
+// as its halfcomplex workspace. The pointers are picked with selects so that
+// they stay in registers.
+struct SolveArgs {
+    const double *eig, *rhs;
+    double *out, *hc;
 };
 
-__device__ void circ_solve3(const Solve3 &S, const double *cos_t, const double *sin_t, int n)
+__device__ void circ_solve3(SolveArgs s0, SolveArgs s1, SolveArgs s2, const double *cos_t,
+                            const double *sin_t, int n)
 {
     const int G = blockDim.x / 3;
     const int g = (int)threadIdx.x / G;
     const int tid = (int)threadIdx.x - g * G;
     const bool mine = g < 3;
-    const double *eig = S.eig[mine ? g : 0];
-    const double *rhs = S.rhs[mine ? g : 0];
-    double *out = S.out[mine ? g : 0];
-    double *hc = S.hc[mine ? g : 0];
+    const double *eig = g == 1 ? s1.eig : (g == 2 ? s2.eig : s0.eig);
+    const double *rhs = g == 1 ? s1.rhs : (g == 2 ? s2.rhs : s0.rhs);
+    double *out = g == 1 ? s1.out : (g == 2 ? s2.out : s0.out);
+    double *hc = g == 1 ? s1.hc : (g == 2 ? s2.hc : s0.hc);
     if (mine)
         for (int k = tid; k <= n / 2; k += G) {
             double re = 0.0, im = 0.0;
@@ -325,6 +328,7 @@ __device__ __forceinline__ double gather(const double (&b)[P + 1], const double 
 // periodic grid, k in [-kPad, nx + kPad), sits at fp[k] with fp = base + kPad.
 constexpr int kPad = 4;
 
+
 __device__ __forceinline__ void load_padded(double *dst, const double *src, int nx)
 {
     for (int i = threadIdx.x; i < nx + 2 * kPad; i += blockDim.x) {
@@ -408,14 +412,15 @@ __global__ void __launch_bounds__(kBlock) k_pass1d(PassArgs a)
     constexpr bool HP2B = (MODE & M_HP2B) != 0;
     constexpr bool V1 = (MODE & M_V1) != 0;
     constexpr bool RHO_ONLY = (MODE & M_RHO) != 0;
-    constexpr bool NOEPI = (MODE & M_NOEPI) != 0;
-    constexpr bool WITH_J2A = (MODE & M_WITH_J2A) != 0;
-    constexpr bool DEP_RHO = (HP1 && !V1) || RHO_ONLY;
+    // operatorHp1 (1d2v) also deposits rho into j_dofs[2]
+    // (hamiltonian_splitting_1d2v.jl:90-108); when Hp2 follows in the same pass
+    // it clears j_dofs[2] first (:132-133) and nothing reads the rho between, so
+    // that deposit is dead and skipped.
+    constexpr bool DEP_RHO = (HP1 && !V1 && !HP2B) || RHO_ONLY;
     constexpr bool NEED_NEW = DEP_RHO || HP2B;           // work at the post-Hp1 position
-    constexpr int RR = HP2B ? 2 : 1;                     // rows sharing the new-position cache
+    constexpr int NEW_ROW = HP2B ? D_J2B : D_RHO;        // the new-position cache's row
     constexpr unsigned DEPMASK = (HP2A ? 1u << D_J2A : 0u) | (HP1 ? 1u << D_J1 : 0u) |
                                  (DEP_RHO ? 1u << D_RHO : 0u) | (HP2B ? 1u << D_J2B : 0u);
-    constexpr unsigned REDMASK = DEPMASK | (WITH_J2A ? 1u << D_J2A : 0u);
     constexpr int NQ = (P1 + 2) / 2;
     constexpr int W0 = P0 + 1, W1 = P1 + 1;
     static_assert(P0 <= kPad && P1 <= kPad, "LDS field padding must cover the spline support");
@@ -429,10 +434,10 @@ __global__ void __launch_bounds__(kBlock) k_pass1d(PassArgs a)
     double *pE2p = pE1 + nxp;
     double *pE2 = pE2p + nxp;
     double *pB = pE2 + nxp;
-    double *sRed = pB + nxp;                              // [NDEP][nx]
-    double *sWork = sRed + NDEP * nx;                     // [2][nx]
-    double *rows = sWork + 2 * nx;                        // [kWaves][NDEP][nx], later scratch
-    // unpadded views (node k at s*[k]) for the epilogue's field solves
+    double *rows = pB + nxp;                              // [kWaves][NDEP][nx] deposit rows
+    // this wave's cache-B slots (deposit modes only)
+    double *slots = rows + kWaves * NDEP * nx + wave * (kSlotsPerLane * kWave);
+    // unpadded views (node k at s*[k])
     double *sE1 = pE1 + kPad, *sE2p = pE2p + kPad, *sE2 = pE2 + kPad, *sB = pB + kPad;
 
     load_padded(pE1, a.grid + S_E1 * nx, nx);
@@ -443,40 +448,46 @@ __global__ void __launch_bounds__(kBlock) k_pass1d(PassArgs a)
     __syncthreads();
     double *myrow = rows + wave * NDEP * nx;
 
+    static_assert(W0 + W1 + W0 <= kSlotsPerLane, "cache-B slots");
     Cache2<W0> c_j2a;
     Cache2<W1> c_j1;
-    Cache2<W0, RR> c_new;
-    c_j2a.init();
-    c_j1.init();
-    c_new.init();
+    Cache2<W0> c_new;
+    if constexpr (HP2A) c_j2a.init(slots);
+    if constexpr (HP1) c_j1.init(slots + W0 * kWave);
+    if constexpr (NEED_NEW) c_new.init(slots + (W0 + W1) * kWave);
 
     const double xmin = a.xmin, dx = a.dx, qm = a.qm;
+    // this workgroup's chunk: particles base .. base + cnt_here - 1, addressed
+    // through per-chunk base pointers and a 32-bit index
     const int64_t base = (int64_t)blockIdx.x * a.chunk;
+    const int64_t left = a.n - base;
+    const int cnt_here = left < a.chunk ? (int)left : (int)a.chunk;
+    double *__restrict__ px = a.x + base;
+    double *__restrict__ pv1 = a.v1 + base;
+    double *__restrict__ pv2 = a.v2 + base;
+    const double *__restrict__ pw = a.w + base;
     int errs = 0;
 
     // software pipeline: the loads of iteration it+1 are issued before the
     // work of iteration it
     double nx_x = 0.0, nx_v1 = 0.0, nx_v2 = 0.0, nx_w = 0.0;
-    {
-        const int64_t i0 = base + tid;
-        if (a.iters > 0 && i0 < a.n) {
-            nx_x = a.x[i0];
-            nx_w = a.w[i0];
-            if (!RHO_ONLY) nx_v1 = a.v1[i0];
-            if (!V1 && !RHO_ONLY) nx_v2 = a.v2[i0];
-        }
+    if (a.iters > 0 && tid < cnt_here) {
+        nx_x = px[tid];
+        nx_w = pw[tid];
+        if (!RHO_ONLY) nx_v1 = pv1[tid];
+        if (!V1 && !RHO_ONLY) nx_v2 = pv2[tid];
     }
     for (int it = 0; it < a.iters; ++it) {
-        const int64_t i = base + (int64_t)it * kBlock + tid;
-        const bool valid = i < a.n;
+        const int i = it * kBlock + tid;
+        const bool valid = i < cnt_here;
         double x = nx_x, v1 = nx_v1, v2 = nx_v2, w = nx_w;
         {
-            const int64_t in = i + kBlock;
-            if (it + 1 < a.iters && in < a.n) {
-                nx_x = a.x[in];
-                nx_w = a.w[in];
-                if (!RHO_ONLY) nx_v1 = a.v1[in];
-                if (!V1 && !RHO_ONLY) nx_v2 = a.v2[in];
+            const int in = i + kBlock;
+            if (it + 1 < a.iters && in < cnt_here) {
+                nx_x = px[in];
+                nx_w = pw[in];
+                if (!RHO_ONLY) nx_v1 = pv1[in];
+                if (!V1 && !RHO_ONLY) nx_v2 = pv2[in];
             }
         }
         // marker charge, as (w*charge)*common_weight (hamiltonian_splitting_1d2v.jl:62-64)
@@ -522,7 +533,7 @@ __global__ void __launch_bounds__(kBlock) k_pass1d(PassArgs a)
             double vals[W0];
 #pragma unroll
             for (int k = 0; k < W0; ++k) vals[k] = wq * b0[k] * a.scaling0;
-            cache_add<W0, 1>(c_j2a, valid, key_old, vals, myrow + D_J2A * nx, nx);
+            cache_add<W0>(c_j2a, valid, key_old, vals, myrow + D_J2A * nx, nx);
         }
         double xi_new = 0.0;       // (x - xmin) / dx after Hp1, when the mod kept x_new
         bool xi_new_ok = false;
@@ -558,7 +569,7 @@ __global__ void __launch_bounds__(kBlock) k_pass1d(PassArgs a)
                     if (!V1) v2s = v2s - qm * sv[m] * bk[m];
                 }
                 if (act) v2 = v2s;
-                cache_add<W1, 1>(c_j1, act, key, vals, myrow + D_J1 * nx, nx);
+                cache_add<W1>(c_j1, act, key, vals, myrow + D_J1 * nx, nx);
             }
             x = jl_mod(xn, a.Lx);
             xi_new = xin;
@@ -581,28 +592,25 @@ __global__ void __launch_bounds__(kBlock) k_pass1d(PassArgs a)
                 const double bf = gather_p<P1>(c1, sB, key2);
                 v1 = v1 + a.dtqm_hp2 * v2 * bf;
             }
-            double vals[W0 * RR];
+            // rho: q * S0; Hp2's j2: (q * v2) * S0 (particle_mesh_coupling_1d.jl:120)
+            const double wq = HP2B ? q * v2 : q;
+            double vals[W0];
 #pragma unroll
-            for (int k = 0; k < W0; ++k) vals[k] = q * c0[k] * a.scaling0;
-            if constexpr (HP2B) {
-                const double wq = q * v2;
-#pragma unroll
-                for (int k = 0; k < W0; ++k) vals[W0 + k] = wq * c0[k] * a.scaling0;
-            }
-            cache_add<W0, RR>(c_new, valid, key2, vals, myrow + D_RHO * nx, nx);
+            for (int k = 0; k < W0; ++k) vals[k] = wq * c0[k] * a.scaling0;
+            cache_add<W0>(c_new, valid, key2, vals, myrow + NEW_ROW * nx, nx);
         }
         if (valid) {
-            if constexpr (HP1) a.x[i] = x;
-            if constexpr (HE_PREV || HE || HP2A || HP2B) a.v1[i] = v1;
-            if constexpr (!V1 && (HE_PREV || HE || HP1)) a.v2[i] = v2;
+            if constexpr (HP1) px[i] = x;
+            if constexpr (HE_PREV || HE || HP2A || HP2B) pv1[i] = v1;
+            if constexpr (!V1 && (HE_PREV || HE || HP1)) pv2[i] = v2;
         }
     }
     if (errs) atomicOr(a.err, errs);
 
     if constexpr (DEPMASK != 0u) {
-        if constexpr (HP2A) cache_drain<W0, 1>(c_j2a, myrow + D_J2A * nx, nx);
-        if constexpr (HP1) cache_drain<W1, 1>(c_j1, myrow + D_J1 * nx, nx);
-        if constexpr (NEED_NEW) cache_drain<W0, RR>(c_new, myrow + D_RHO * nx, nx);
+        if constexpr (HP2A) cache_drain<W0>(c_j2a, myrow + D_J2A * nx, nx);
+        if constexpr (HP1) cache_drain<W1>(c_j1, myrow + D_J1 * nx, nx);
+        if constexpr (NEED_NEW) cache_drain<W0>(c_new, myrow + NEW_ROW * nx, nx);
         __syncthreads();
         const int len = NDEP * nx;
         for (int idx = tid; idx < len; idx += kBlock) {
@@ -611,98 +619,7 @@ __global__ void __launch_bounds__(kBlock) k_pass1d(PassArgs a)
             for (int wv = 0; wv < kWaves; ++wv) s += rows[wv * len + idx];
             a.part[(size_t)blockIdx.x * len + idx] = s;
         }
-        if constexpr (NOEPI) return;
-        // rows are dead once written out: reuse them as reduction scratch, then
-        // as the LDS copy of the constant tables
-        if (!grid_reduce(a.part, a.gpart, a.cnt, a.ngroups, len, REDMASK, nx, sRed, rows)) return;
-
-        // ---- epilogue: field solves of the sub-steps, in the reference's order
-        double *work = sWork, *hc = sWork + nx;
-        load_tabs(a.tabs, rows, nx);
-        const double *T = rows;
-        double *r_j2a = sRed + D_J2A * nx, *r_j1 = sRed + D_J1 * nx;
-        double *r_rho = sRed + D_RHO * nx, *r_j2b = sRed + D_J2B * nx;
-        double *G = a.grid;
-        if constexpr (RHO_ONLY) {
-            ep_e_from_rho(work + 0, r_rho, sE2p, hc, T, nx);   // sE2p reused as scratch
-            for (int i = tid; i < nx; i += kBlock) {
-                G[S_RHO * nx + i] = r_rho[i];
-                G[S_EPOIS * nx + i] = work[i];
-            }
-        } else if constexpr (V1) {
-            // 1d1v Hp1: j_dofs[1] = j, j_dofs[2] = 0; E1 -= M1^{-1} j / dx
-            for (int i = tid; i < nx; i += kBlock) {
-                G[S_J1 * nx + i] = r_j1[i];
-                G[S_J2 * nx + i] = 0.0;
-            }
-            ep_e_from_j(sE1, r_j1, T + T_INVM1 * nx, dx, work, hc, T, nx);
-            for (int i = tid; i < nx; i += kBlock) G[S_E1 * nx + i] = sE1[i];
-        } else if constexpr (NOEPI) {
-            // unreachable: returned above
-        } else if constexpr (HP2A && !HP1) {
-            // operatorHp2 alone (:169-175)
-            for (int i = tid; i < nx; i += kBlock) r_j2a[i] = r_j2a[i] * a.dt_hp2;
-            __syncthreads();
-            ep_e_from_j(sE2, r_j2a, T + T_INVM0 * nx, dx, work, hc, T, nx);
-            for (int i = tid; i < nx; i += kBlock) {
-                G[S_J1 * nx + i] = 0.0;
-                G[S_J2 * nx + i] = r_j2a[i];
-                G[S_E2 * nx + i] = sE2[i];
-            }
-        } else if constexpr (HP1 && !HP2A && !HP2B) {
-            // operatorHp1 alone (:88-111)
-            for (int i = tid; i < nx; i += kBlock) {
-                G[S_J1 * nx + i] = r_j1[i];
-                G[S_J2 * nx + i] = r_rho[i];
-            }
-            ep_e_from_j(sE1, r_j1, T + T_INVM1 * nx, dx, work, hc, T, nx);
-            for (int i = tid; i < nx; i += kBlock) G[S_E1 * nx + i] = sE1[i];
-        } else {
-            // fused Strang step: Hp2 (1/2) - Hp1 - Hp2 (1/2) field solves, then
-            // HE's B update and HB of this step, and (if another step follows)
-            // HB and HE's B update of the next step.
-            // The three inverse-mass solves (compute_e_from_j! of Hp2, Hp1,
-            // Hp2) are independent: run them side by side, then apply the
-            // updates in the reference's order, E2 -= w_a / dx, E1 -= w_1 / dx,
-            // E2 -= w_b / dx.
-            for (int i = tid; i < nx; i += kBlock) {
-                r_j2a[i] = r_j2a[i] * a.dt_hp2;
-                r_j2b[i] = r_j2b[i] * a.dt_hp2;
-            }
-            __syncthreads();
-            double *w3 = rows + NTAB * nx;          // free LDS after the tables: 6 nx doubles
-            Solve3 S;
-            S.eig[0] = T + T_INVM0 * nx; S.rhs[0] = r_j2a; S.out[0] = w3;          S.hc[0] = w3 + nx;
-            S.eig[1] = T + T_INVM1 * nx; S.rhs[1] = r_j1;  S.out[1] = w3 + 2 * nx; S.hc[1] = w3 + 3 * nx;
-            S.eig[2] = T + T_INVM0 * nx; S.rhs[2] = r_j2b; S.out[2] = w3 + 4 * nx; S.hc[2] = w3 + 5 * nx;
-            circ_solve3(S, T + T_COS * nx, T + T_SIN * nx, nx);
-            for (int i = tid; i < nx; i += kBlock) {
-                double wa = S.out[0][i] / dx;
-                sE2[i] -= wa;
-                double w1 = S.out[1][i] / dx;
-                sE1[i] -= w1;
-                double wb = S.out[2][i] / dx;
-                sE2[i] -= wb;
-            }
-            __syncthreads();
-            for (int i = tid; i < nx; i += kBlock) {
-                G[S_J1 * nx + i] = 0.0;
-                G[S_J2 * nx + i] = r_j2b[i];
-                G[S_RHO * nx + i] = r_rho[i];
-                G[S_E2P * nx + i] = sE2[i];
-            }
-            ep_b_from_e(sB, sE2, a.dt_he, dx, nx);
-            ep_e_from_b(sE2, sB, a.dt_he, dx, work, hc, T, nx);
-            if (a.more) {
-                ep_e_from_b(sE2, sB, a.dt_he, dx, work, hc, T, nx);
-                ep_b_from_e(sB, sE2, a.dt_he, dx, nx);
-            }
-            for (int i = tid; i < nx; i += kBlock) {
-                G[S_E1 * nx + i] = sE1[i];
-                G[S_E2 * nx + i] = sE2[i];
-                G[S_B * nx + i] = sB[i];
-            }
-        }
+        // the cross-workgroup sum and the field solves run in k_epi1d
     } else if constexpr (HE && !V1 && !HE_PREV) {
         // operatorHE alone: compute_b_from_e! after the push (:217-218). B is
         // not read by this pass, so workgroup 0 can update it at any time.
@@ -713,14 +630,198 @@ __global__ void __launch_bounds__(kBlock) k_pass1d(PassArgs a)
     }
 }
 
-size_t pass_shmem_bytes(int nx)
+// ---------------------------------------------------------------------------
+// The field part of a deposit pass: sum of the workgroups' partial rows and
+// the field solves of its sub-steps, in one workgroup of kEpiThreads.
+//
+// It runs as a kernel of its own after the particle pass: the partial rows
+// are then visible to it through the kernel boundary, so the pass needs no
+// device-scope fence (on MI355X such a fence writes the XCD's whole L2 back,
+// and hundreds of workgroups each doing so cost more than the particle work).
+// ---------------------------------------------------------------------------
+constexpr int kEpiThreads = 1024;
+
+// red[r*nx + k] = sum over b of part[b*NDEP*nx + r*nx + k] for the rows r in
+// `mask` (other rows zero). Fixed order: thread (e, h) adds the workgroups of
+// slice h in index order, then the slices are added in order of h.
+__device__ void sum_blocks(const double *part, int nb, int nx, unsigned mask, double *red,
+                           double *scratch)
 {
-    // particle pass: padded fields, reduced deposits, epilogue work, then the
-    // per-wave rows reused as reduction scratch and LDS tables
-    size_t rows = (size_t)kWaves * NDEP * nx;
-    if (rows < (size_t)kBlock) rows = kBlock;
-    if (rows < (size_t)(NTAB + 6) * nx) rows = (size_t)(NTAB + 6) * nx;
-    const size_t pass = (size_t)4 * (nx + 2 * kPad) + (size_t)nx * (NDEP + 2) + rows;
+    const int nt = blockDim.x, tid = threadIdx.x;
+    const int len = NDEP * nx;
+    const int E = __popc(mask) * nx;                       // live entries, <= len <= nt
+    const int k = nt / E > 1 ? nt / E : 1;
+    const int e = tid % E, h = tid / E;
+    double s = 0.0;
+    int idx = 0;
+    if (h < k) {
+        const int li = e / nx, node = e - li * nx;
+        int r = 0, c = -1;
+#pragma unroll
+        for (int rr = 0; rr < NDEP; ++rr)
+            if ((mask >> rr) & 1u) {
+                ++c;
+                if (c == li) r = rr;
+            }
+        idx = r * nx + node;
+        const int b0 = (int)(((long)nb * h) / k);
+        const int b1 = (int)(((long)nb * (h + 1)) / k);
+        const double *col = part + idx;
+        int b = b0;
+        constexpr int B = 16;
+        for (; b + B <= b1; b += B) {
+            double t[B];
+#pragma unroll
+            for (int u = 0; u < B; ++u) t[u] = col[(size_t)(b + u) * len];
+#pragma unroll
+            for (int u = 0; u < B; ++u) s += t[u];
+        }
+        for (; b < b1; ++b) s += col[(size_t)b * len];
+    }
+    scratch[tid] = s;
+    __syncthreads();
+    if (h == 0 && tid < E) {
+        double tot = 0.0;
+        for (int hh = 0; hh < k; ++hh) tot += scratch[hh * E + e];
+        red[idx] = tot;
+    }
+    for (int i = tid; i < len; i += nt)
+        if (!((mask >> (i / nx)) & 1u)) red[i] = 0.0;
+    __syncthreads();
+}
+
+template <int P0, int P1, int MODE>
+__global__ void __launch_bounds__(kEpiThreads) k_epi1d(PassArgs a, int nblocks)
+{
+    constexpr bool HE_PREV = (MODE & M_HE_PREV) != 0;
+    constexpr bool HE = (MODE & M_HE) != 0;
+    constexpr bool HP2A = (MODE & M_HP2A) != 0;
+    constexpr bool HP1 = (MODE & M_HP1) != 0;
+    constexpr bool HP2B = (MODE & M_HP2B) != 0;
+    constexpr bool V1 = (MODE & M_V1) != 0;
+    constexpr bool RHO_ONLY = (MODE & M_RHO) != 0;
+    constexpr bool WITH_J2A = (MODE & M_WITH_J2A) != 0;
+    constexpr bool DEP_RHO = (HP1 && !V1 && !HP2B) || RHO_ONLY;
+    constexpr unsigned DEPMASK = (HP2A ? 1u << D_J2A : 0u) | (HP1 ? 1u << D_J1 : 0u) |
+                                 (DEP_RHO ? 1u << D_RHO : 0u) | (HP2B ? 1u << D_J2B : 0u);
+    constexpr unsigned REDMASK = DEPMASK | (WITH_J2A ? 1u << D_J2A : 0u);
+    (void)HE_PREV;
+    (void)HE;
+
+    extern __shared__ double smem[];
+    const int nx = a.nx;
+    const int tid = threadIdx.x;
+    const int nt = blockDim.x;
+    const double dx = a.dx;
+    double *sE1 = smem;                     // [nx] each
+    double *sE2 = sE1 + nx;
+    double *sB = sE2 + nx;
+    double *sRed = sB + nx;                 // [NDEP][nx]
+    double *work = sRed + NDEP * nx;        // [nx]
+    double *hc = work + nx;                 // [nx]
+    double *spare = hc + nx;                // [nx]
+    double *T = spare + nx;                 // [NTAB][nx] constant tables
+    double *w3 = T + NTAB * nx;             // [6][nx]
+    double *scratch = w3 + 6 * nx;          // [blockDim.x]
+    double *G = a.grid;
+    for (int i = tid; i < nx; i += nt) {
+        sE1[i] = G[S_E1 * nx + i];
+        sE2[i] = G[S_E2 * nx + i];
+        sB[i] = G[S_B * nx + i];
+    }
+    for (int i = tid; i < NTAB * nx; i += nt) T[i] = a.tabs[i];
+    sum_blocks(a.part, nblocks, nx, REDMASK, sRed, scratch);   // ends with a barrier
+
+    double *r_j2a = sRed + D_J2A * nx, *r_j1 = sRed + D_J1 * nx;
+    double *r_rho = sRed + D_RHO * nx, *r_j2b = sRed + D_J2B * nx;
+    if constexpr (RHO_ONLY) {
+        ep_e_from_rho(work, r_rho, spare, hc, T, nx);
+        for (int i = tid; i < nx; i += nt) {
+            G[S_RHO * nx + i] = r_rho[i];
+            G[S_EPOIS * nx + i] = work[i];
+        }
+    } else if constexpr (V1) {
+        // 1d1v Hp1: j_dofs[1] = j, j_dofs[2] = 0; E1 -= M1^{-1} j / dx
+        for (int i = tid; i < nx; i += nt) {
+            G[S_J1 * nx + i] = r_j1[i];
+            G[S_J2 * nx + i] = 0.0;
+        }
+        ep_e_from_j(sE1, r_j1, T + T_INVM1 * nx, dx, work, hc, T, nx);
+        for (int i = tid; i < nx; i += nt) G[S_E1 * nx + i] = sE1[i];
+    } else if constexpr (HP2A && !HP1) {
+        // operatorHp2 alone (hamiltonian_splitting_1d2v.jl:169-175)
+        for (int i = tid; i < nx; i += nt) r_j2a[i] = r_j2a[i] * a.dt_hp2;
+        __syncthreads();
+        ep_e_from_j(sE2, r_j2a, T + T_INVM0 * nx, dx, work, hc, T, nx);
+        for (int i = tid; i < nx; i += nt) {
+            G[S_J1 * nx + i] = 0.0;
+            G[S_J2 * nx + i] = r_j2a[i];
+            G[S_E2 * nx + i] = sE2[i];
+        }
+    } else if constexpr (HP1 && !HP2A && !HP2B) {
+        // operatorHp1 alone (hamiltonian_splitting_1d2v.jl:88-111)
+        for (int i = tid; i < nx; i += nt) {
+            G[S_J1 * nx + i] = r_j1[i];
+            G[S_J2 * nx + i] = r_rho[i];
+        }
+        ep_e_from_j(sE1, r_j1, T + T_INVM1 * nx, dx, work, hc, T, nx);
+        for (int i = tid; i < nx; i += nt) G[S_E1 * nx + i] = sE1[i];
+    } else {
+        // fused Strang step: Hp2 (1/2) - Hp1 - Hp2 (1/2) field solves, then
+        // HE's B update and HB of this step, and (if another step follows)
+        // HB and HE's B update of the next step.
+        // The three inverse-mass solves (compute_e_from_j! of Hp2, Hp1, Hp2)
+        // are independent: run them side by side, then apply the updates in
+        // the reference's order, E2 -= w_a / dx, E1 -= w_1 / dx, E2 -= w_b / dx.
+        for (int i = tid; i < nx; i += nt) {
+            r_j2a[i] = r_j2a[i] * a.dt_hp2;
+            r_j2b[i] = r_j2b[i] * a.dt_hp2;
+        }
+        __syncthreads();
+        const SolveArgs sa{T + T_INVM0 * nx, r_j2a, w3, w3 + nx};
+        const SolveArgs s1{T + T_INVM1 * nx, r_j1, w3 + 2 * nx, w3 + 3 * nx};
+        const SolveArgs sb{T + T_INVM0 * nx, r_j2b, w3 + 4 * nx, w3 + 5 * nx};
+        circ_solve3(sa, s1, sb, T + T_COS * nx, T + T_SIN * nx, nx);
+        for (int i = tid; i < nx; i += nt) {
+            double wa = sa.out[i] / dx;
+            sE2[i] -= wa;
+            double w1 = s1.out[i] / dx;
+            sE1[i] -= w1;
+            double wb = sb.out[i] / dx;
+            sE2[i] -= wb;
+        }
+        __syncthreads();
+        for (int i = tid; i < nx; i += nt) {
+            G[S_J1 * nx + i] = 0.0;
+            G[S_J2 * nx + i] = r_j2b[i];
+            G[S_E2P * nx + i] = sE2[i];
+        }
+        ep_b_from_e(sB, sE2, a.dt_he, dx, nx);
+        ep_e_from_b(sE2, sB, a.dt_he, dx, work, hc, T, nx);
+        if (a.more) {
+            ep_e_from_b(sE2, sB, a.dt_he, dx, work, hc, T, nx);
+            ep_b_from_e(sB, sE2, a.dt_he, dx, nx);
+        }
+        for (int i = tid; i < nx; i += nt) {
+            G[S_E1 * nx + i] = sE1[i];
+            G[S_E2 * nx + i] = sE2[i];
+            G[S_B * nx + i] = sB[i];
+        }
+    }
+}
+
+size_t epi_shmem_bytes(int nx)
+{
+    return sizeof(double) * ((size_t)(3 + NDEP + 3 + NTAB + 6) * nx + kEpiThreads);
+}
+
+size_t pass_shmem_bytes(int nx, int mode)
+{
+    // particle pass: padded fields, the per-wave deposit rows and, for
+    // deposit modes, the cache-B slots
+    const bool dep = (mode & (M_HP2A | M_HP1 | M_HP2B | M_RHO | M_WITH_J2A)) != 0;
+    const size_t pass = (size_t)4 * (nx + 2 * kPad) + (size_t)kWaves * NDEP * nx +
+                        (dep ? (size_t)kWaves * kSlotsPerLane * kWave : 0);
     // diagnostics: 5 fields, work, hc, 8 sums, scratch, then scratch/tables
     size_t tail = (size_t)NTAB * nx;
     if (tail < (size_t)kBlock) tail = kBlock;
@@ -825,6 +926,13 @@ __global__ void __launch_bounds__(kBlock) k_diag1d2v(PassArgs a)
 // ---------------------------------------------------------------------------
 // Launch dispatch
 // ---------------------------------------------------------------------------
+// deposit passes are followed by k_epi1d, except those whose rows the next
+// pass reduces (M_NOEPI)
+constexpr bool mode_has_epi(int m)
+{
+    return (m & (M_HP2A | M_HP1 | M_HP2B | M_RHO)) != 0 && (m & M_NOEPI) == 0;
+}
+
 template <int P0, int P1>
 static hipError_t launch_pass_pd(int mode, const PassArgs &a, int blocks, size_t shmem,
                                  hipStream_t s)
@@ -840,6 +948,9 @@ static hipError_t launch_pass_pd(int mode, const PassArgs &a, int blocks, size_t
             return e_;                                                                      \
         }                                                                                   \
         hipLaunchKernelGGL((k_pass1d<P0, P1, (M)>), dim3(blocks), dim3(kBlock), shmem, s, a); \
+        if constexpr (mode_has_epi(M))                                                       \
+            hipLaunchKernelGGL((k_epi1d<P0, P1, (M)>), dim3(1), dim3(kEpiThreads),           \
+                               epi_shmem_bytes(a.nx), s, a, blocks);                        \
         return hipGetLastError();
     switch (mode) {
         GK_CASE(M_HE)
@@ -1270,11 +1381,13 @@ __global__ void __launch_bounds__(kBlock) k_pmc(PmcArgs a)
     const int wave = tid / kWave;
     double *sRed = smem;                       // [nx]
     double *rows = sRed + nx;                  // [kWaves][nx], later reduction scratch
+    const int rows_len = kWaves * nx > kBlock ? kWaves * nx : kBlock;
+    double *slots = rows + rows_len + wave * (W * kWave);   // this wave's cache-B slots
     for (int i = tid; i < kWaves * nx; i += kBlock) rows[i] = 0.0;
     __syncthreads();
     double *myrow = rows + wave * nx;
     Cache2<W> cache;
-    cache.init();
+    if constexpr (DEPOSIT) cache.init(slots);
     int errs = 0;
     const int64_t base = (int64_t)blockIdx.x * a.chunk;
     for (int it = 0; it < a.iters; ++it) {
@@ -1303,7 +1416,7 @@ __global__ void __launch_bounds__(kBlock) k_pmc(PmcArgs a)
                 double vals[W];
 #pragma unroll
                 for (int k = 0; k < W; ++k) vals[k] = q * b[k] * a.scaling;
-                cache_add<W, 1>(cache, valid, imod(ic, nx), vals, myrow, nx);
+                cache_add<W>(cache, valid, imod(ic, nx), vals, myrow, nx);
             }
         } else {
             // add_current_update_v! / _pp!
@@ -1362,14 +1475,14 @@ __global__ void __launch_bounds__(kBlock) k_pmc(PmcArgs a)
                     k = next_node(k, nx);
                 }
                 if (act) v = vs;
-                cache_add<W, 1>(cache, act, key, vals, myrow, nx);
+                cache_add<W>(cache, act, key, vals, myrow, nx);
             }
             if (valid && OP != PMC_CURRENT_NOB) a.out[i] = v;
         }
     }
     if (errs) atomicOr(a.err, errs);
     if constexpr (DEPOSIT) {
-        cache_drain<W, 1>(cache, myrow, nx);
+        cache_drain<W>(cache, myrow, nx);
         __syncthreads();
         for (int idx = tid; idx < nx; idx += kBlock) {
             double s = 0.0;
@@ -1386,7 +1499,7 @@ static hipError_t launch_pmc_p(int op, const PmcArgs &a, int blocks, hipStream_t
 {
     size_t rows = (size_t)kWaves * a.nx;
     if (rows < (size_t)kBlock) rows = kBlock;
-    const size_t sh = sizeof(double) * ((size_t)a.nx + rows);
+    const size_t sh = sizeof(double) * ((size_t)a.nx + rows + (size_t)kWaves * (P + 1) * kWave);
     switch (op) {
 #define GK_OP(O)                                                                        \
     case O:                                                                             \

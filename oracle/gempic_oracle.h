@@ -10,6 +10,26 @@
  * -ffp-contract=off so that no product/sum pair is fused into an FMA (Julia
  * does not contract by default). Citations are file:line in the reference.
  *
+ * How each function is pinned (tests/test_oracle_kat.py, KAT = the
+ * reference's own known-answer tests, copied into tests/golden/;
+ * tests/test_oracle_invariants.py for the rest):
+ *   - bspline basis, add_charge(_pp), evaluate(_pp), add_current_update_v(_pp):
+ *     KATs of test/test_particle_mesh_coupling_spline_1d.jl, partition of
+ *     unity, charge continuity of the line integral;
+ *   - add_current_update_v_nob (1d1v): charge continuity for every move;
+ *   - gausslegendre: polynomial exactness; splinepp tables, b_to_pp: de Boor;
+ *   - Maxwell1DFEM tables and solves: eigenvalues = DFT of the mass lines,
+ *     dense circulant solve, numpy rfft, the analytic tests of
+ *     test/test_maxwell_1d_fem.jl, the splitting KAT's fields;
+ *     l2norm_squared / inner_product: Gauss quadrature of the spline;
+ *   - 1d2v operators: KAT of test/test_hamiltonian_splitting.jl (after each
+ *     sub-step); 1d1v operators: Gauss's law kept by Hp1, HB's kick;
+ *   - strang_splitting: second-order energy conservation;
+ *   - write_step: energy conservation and the diagnostics as rates of change
+ *     of the energies and momenta under HE, Hp2 and HB; ErrorPoisson by
+ *     Gauss's law.
+ * No function here is left unpinned.
+ *
  * Layout conventions (same as the reference):
  *   - particle array: column-major (D+V+n_weights) x N, i.e. element (r, i)
  *     (both 0-based here) lives at a[r + i*ld] with ld = D+V+n_weights;
