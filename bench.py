@@ -122,7 +122,9 @@ def main():
     if args.warmup:
         _lib.check(L.gempic_strang_splitting(h._h, dt, args.warmup, _lib.STRANG_FUSED))
     _lib.check(L.gempic_synchronize(h._h))
-    h.profile(True)
+    # HIP events around every 10th launch of each kernel: the kernel times
+    # come from the timed run itself at a small cost to its stream
+    _lib.check(L.gempic_profile_enable(h._h, 10))
     _lib.check(L.gempic_profile_reset(h._h))
     print(f"[bench] timing {args.steps} steps", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
@@ -132,7 +134,7 @@ def main():
     fused_ms, fused_cnt = h.profile_read(0)
     kernels = {}
     for kid, name in enumerate(["fused_pass", "trailing_pass", "he", "hp2", "hp1", "grid",
-                                "sort", "other", "fused_pass_a"]):
+                                "sort", "other", "fused_pass_a", "deposit_sum_and_solves"]):
         ms, cnt = h.profile_read(kid)
         if cnt:
             kernels[name] = {"avg_us": 1e3 * ms / cnt, "count": cnt}

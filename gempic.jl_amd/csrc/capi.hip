@@ -132,11 +132,13 @@ struct gempic_splitting {
 
     // profiling
     bool profile = false;
+    int profile_every = 1;            // sample every n-th launch of each kernel id
+    int64_t prof_seen[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     struct Pend { int id; hipEvent_t a, b; };
     std::vector<Pend> pending;
     std::vector<hipEvent_t> pool;
-    double prof_ms[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    int64_t prof_cnt[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double prof_ms[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    int64_t prof_cnt[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 struct gempic_maxwell1d {
@@ -168,7 +170,7 @@ struct ProfScope {
     hipEvent_t a = nullptr, b = nullptr;
     ProfScope(gempic_splitting *hh, int kid) : h(hh), id(kid)
     {
-        if (h->profile) {
+        if (h->profile && h->prof_seen[id]++ % h->profile_every == 0) {
             a = take_event(h);
             b = take_event(h);
             if (a) (void)hipEventRecord(a, h->stream);
@@ -176,9 +178,12 @@ struct ProfScope {
     }
     ~ProfScope()
     {
-        if (h->profile && a && b) {
+        if (a && b) {
             (void)hipEventRecord(b, h->stream);
             h->pending.push_back({id, a, b});
+        } else {
+            if (a) h->pool.push_back(a);
+            if (b) h->pool.push_back(b);
         }
     }
 };
@@ -269,9 +274,19 @@ int run_pass(gempic_splitting *h, int mode, double dt_he, double dt_hp2, double 
     a.dtqm_he = dt_he * h->qm;
     a.dtqm_hp2 = dt_hp2 * h->qm;
     a.more = more;
-    ProfScope ps(h, prof_id);
-    hipError_t e = launch_pass(h->deg0, h->deg1, mode, a, L.blocks, pass_shmem_bytes((int)h->nx, mode), h->stream);
-    if (e != hipSuccess) return fail(GEMPIC_ERR_DEVICE, std::string("particle pass launch: ") + hipGetErrorString(e));
+    {
+        ProfScope ps(h, prof_id);
+        hipError_t e = launch_pass(h->deg0, h->deg1, mode, a, L.blocks,
+                                   pass_shmem_bytes((int)h->nx, mode), h->stream);
+        if (e != hipSuccess)
+            return fail(GEMPIC_ERR_DEVICE, std::string("particle pass launch: ") + hipGetErrorString(e));
+    }
+    if (mode_has_epi(mode)) {
+        ProfScope ps(h, 9);
+        hipError_t e = launch_epi(h->deg0, h->deg1, mode, a, L.blocks, h->stream);
+        if (e != hipSuccess)
+            return fail(GEMPIC_ERR_DEVICE, std::string("deposit sum launch: ") + hipGetErrorString(e));
+    }
     return GEMPIC_OK;
 }
 
@@ -436,7 +451,7 @@ int gempic_splitting_create(int64_t nx, double xmin, double xmax, int maxwell_de
     }
     h->cus = device_cus();
     h->max_blocks = h->cus * 8;
-    const int max_groups = (h->max_blocks + kGroup - 1) / kGroup;
+    const int max_groups = std::max((h->max_blocks + kGroup - 1) / kGroup, kEpiGroupsMax);
     const size_t n = (size_t)n_particles;
     h->sortg = sort_geometry((int64_t)n, (int)nx);
     hipError_t e = hipSuccess;
@@ -745,15 +760,18 @@ int gempic_write_step(gempic_splitting *h, double time, int degree, const double
 int gempic_profile_enable(gempic_splitting *h, int enable)
 {
     CHECK_H(h);
+    if (enable < 0) return fail(GEMPIC_ERR_ARGUMENT, "enable must be >= 0");
     if (!enable) resolve_profile(h);
     h->profile = enable != 0;
+    h->profile_every = enable > 1 ? enable : 1;
+    for (int k = 0; k < 10; ++k) h->prof_seen[k] = 0;
     return GEMPIC_OK;
 }
 
 int gempic_profile_read(gempic_splitting *h, int kernel_id, double *total_ms, int64_t *count)
 {
     CHECK_H(h);
-    if (kernel_id < 0 || kernel_id > 8) return fail(GEMPIC_ERR_ARGUMENT, "kernel id in 0..8");
+    if (kernel_id < 0 || kernel_id > 9) return fail(GEMPIC_ERR_ARGUMENT, "kernel id in 0..9");
     resolve_profile(h);
     if (total_ms) *total_ms = h->prof_ms[kernel_id];
     if (count) *count = h->prof_cnt[kernel_id];
@@ -764,7 +782,7 @@ int gempic_profile_reset(gempic_splitting *h)
 {
     CHECK_H(h);
     resolve_profile(h);
-    for (int k = 0; k < 9; ++k) { h->prof_ms[k] = 0; h->prof_cnt[k] = 0; }
+    for (int k = 0; k < 10; ++k) { h->prof_ms[k] = 0; h->prof_cnt[k] = 0; }
     return GEMPIC_OK;
 }
 

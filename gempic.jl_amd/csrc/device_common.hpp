@@ -96,6 +96,32 @@ __device__ __forceinline__ double wave_sum(double v)
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
 
+// A double moved across lanes by a DPP pattern (two 32-bit DPP moves); lanes
+// whose source is outside the row, or whose row is not in ROW_MASK, get 0.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, ROW_MASK, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xf, true);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Sum over the 64 lanes (all active), complete in lane 63, in a fixed order:
+// running sums within each row of 16 lanes (row_shr 1, 2, 4, 8), then the
+// row totals combined with row_bcast 15 and row_bcast 31. Register-only: no
+// LDS round trips.
+__device__ __forceinline__ double wave_sum_lane63(double v)
+{
+    v += dpp_f64<0x111>(v);          // row_shr:1
+    v += dpp_f64<0x112>(v);          // row_shr:2
+    v += dpp_f64<0x114>(v);          // row_shr:4
+    v += dpp_f64<0x118>(v);          // row_shr:8
+    v += dpp_f64<0x142, 0xa>(v);     // row_bcast:15 into rows 1 and 3
+    v += dpp_f64<0x143, 0xc>(v);     // row_bcast:31 into rows 2 and 3
+    return v;
+}
+
 // ---------------------------------------------------------------------------
 // Deterministic deposit accumulation.
 //
@@ -106,8 +132,8 @@ __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1
 // private LDS slots (B is touched rarely, and keeping it out of registers
 // lets more waves share a SIMD). When a lane meets a third cell, its
 // least-recently-used entry is flushed: the wave runs a segmented reduction
-// over all lanes flushing the same cell (xor butterfly, fixed order) and lane
-// 0 adds the W sums into the wave's private LDS row. The order of every
+// over all lanes flushing the same cell (DPP sums, fixed order) and lane 63
+// adds the W sums into the wave's private LDS row. The order of every
 // addition is a fixed function of the particle order, so the deposit is
 // bit-reproducible from run to run.
 // ---------------------------------------------------------------------------
@@ -153,8 +179,8 @@ __device__ __forceinline__ void seg_flush(bool want, int cell, const double (&v)
         int k = first_node(c, WN - 1, nx);
 #pragma unroll
         for (int w = 0; w < WN; ++w) {
-            double s = wave_sum(mine ? v[w] : 0.0);
-            if (lane == 0) row[k] += s;
+            double s = wave_sum_lane63(mine ? v[w] : 0.0);
+            if (lane == kWave - 1) row[k] += s;
             k = next_node(k, nx);
         }
         pending &= ~m;

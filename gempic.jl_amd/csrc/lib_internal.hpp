@@ -11,6 +11,7 @@ constexpr int kBlock = 256;               // threads per workgroup of the partic
 constexpr int kWaves = kBlock / 64;
 constexpr int kNxMax = 256;               // largest 1D grid handled by the device path
 constexpr int kGroup = 16;                // workgroups per first-level reduction group
+constexpr int kEpiGroupsMax = 64;         // workgroups of the deposit-sum kernel
 
 // grid slots (nx doubles each) of a splitting's device field buffer
 enum GridSlot { S_E1 = 0, S_E2, S_B, S_E2P, S_J1, S_J2, S_RHO, S_EPOIS, S_E1N, S_E2N, NSLOT };
@@ -64,6 +65,10 @@ constexpr int kMaxCellsPerStep = 1 << 16;
 
 hipError_t launch_pass(int p0, int p1, int mode, const PassArgs &a, int blocks, size_t shmem,
                        hipStream_t s);
+// The deposit sum and field solves that follow a deposit pass (k_epi1d);
+// pass_blocks = workgroups of that pass. Modes with M_NOEPI have none.
+bool mode_has_epi(int mode);
+hipError_t launch_epi(int p0, int p1, int mode, const PassArgs &a, int pass_blocks, hipStream_t s);
 // workgroups of the (p0, p1, mode) pass kernel resident on one CU
 int pass_blocks_per_cu(int p0, int p1, int mode, size_t shmem);
 // LDS of a particle pass of this mode: padded fields, deposit rows (reused as

@@ -632,66 +632,86 @@ __global__ void __launch_bounds__(kBlock) k_pass1d(PassArgs a)
 
 // ---------------------------------------------------------------------------
 // The field part of a deposit pass: sum of the workgroups' partial rows and
-// the field solves of its sub-steps, in one workgroup of kEpiThreads.
+// the field solves of its sub-steps.
 //
-// It runs as a kernel of its own after the particle pass: the partial rows
-// are then visible to it through the kernel boundary, so the pass needs no
-// device-scope fence (on MI355X such a fence writes the XCD's whole L2 back,
-// and hundreds of workgroups each doing so cost more than the particle work).
+// It runs as a kernel of its own after the particle pass. The partial rows
+// are then visible to it through the kernel boundary, so the particle pass
+// needs no device-scope fence: on MI355X such a fence writes the XCD's whole
+// L2 back, and with the particle stores in flight hundreds of workgroups each
+// doing so cost more than the particle work. Here the L2 holds only this
+// kernel's few group rows, so its own fences are cheap: G workgroups each sum
+// a slice of the partial rows, and the last one to finish sums the G group
+// rows and runs the field solves.
 // ---------------------------------------------------------------------------
-constexpr int kEpiThreads = 1024;
 
-// red[r*nx + k] = sum over b of part[b*NDEP*nx + r*nx + k] for the rows r in
-// `mask` (other rows zero). Fixed order: thread (e, h) adds the workgroups of
-// slice h in index order, then the slices are added in order of h.
-__device__ void sum_blocks(const double *part, int nb, int nx, unsigned mask, double *red,
-                           double *scratch)
+// index r*nx + k of live entry e (the e-th entry of the rows in mask)
+__device__ __forceinline__ int live_index(int e, int nx, unsigned mask)
+{
+    const int li = e / nx, node = e - li * nx;
+    int r = 0, c = -1;
+#pragma unroll
+    for (int rr = 0; rr < NDEP; ++rr)
+        if ((mask >> rr) & 1u) {
+            ++c;
+            if (c == li) r = rr;
+        }
+    return r * nx + node;
+}
+
+// out[i] = sum of base[b*NDEP*nx + i] over b in [b0, b1) for the live
+// entries i of the rows in mask. Fixed order: the k threads of an entry take
+// consecutive slices of [b0, b1), and their slice sums are added in slice
+// order. Called by all threads; scratch holds blockDim.x doubles.
+__device__ void sum_rows_range(const double *base, int b0, int b1, int nx, unsigned mask,
+                               double *out, double *scratch)
 {
     const int nt = blockDim.x, tid = threadIdx.x;
     const int len = NDEP * nx;
-    const int E = __popc(mask) * nx;                       // live entries, <= len <= nt
-    const int k = nt / E > 1 ? nt / E : 1;
-    const int e = tid % E, h = tid / E;
-    double s = 0.0;
-    int idx = 0;
-    if (h < k) {
-        const int li = e / nx, node = e - li * nx;
-        int r = 0, c = -1;
+    const int E = __popc(mask) * nx;
+    const int n = b1 - b0;
+    for (int e0 = 0; e0 < E; e0 += nt) {
+        const int ec = E - e0 < nt ? E - e0 : nt;
+        const int k = nt / ec;
+        const int h = tid / ec;
+        const int e = e0 + (tid - h * ec);
+        double s = 0.0;
+        int idx = 0;
+        if (h < k) {
+            idx = live_index(e, nx, mask);
+            const int r0 = b0 + (int)(((long)n * h) / k);
+            const int r1 = b0 + (int)(((long)n * (h + 1)) / k);
+            const double *col = base + idx;
+            int b = r0;
+            constexpr int B = 16;
+            for (; b + B <= r1; b += B) {
+                double t[B];
 #pragma unroll
-        for (int rr = 0; rr < NDEP; ++rr)
-            if ((mask >> rr) & 1u) {
-                ++c;
-                if (c == li) r = rr;
+                for (int u = 0; u < B; ++u) t[u] = col[(size_t)(b + u) * len];
+#pragma unroll
+                for (int u = 0; u < B; ++u) s += t[u];
             }
-        idx = r * nx + node;
-        const int b0 = (int)(((long)nb * h) / k);
-        const int b1 = (int)(((long)nb * (h + 1)) / k);
-        const double *col = part + idx;
-        int b = b0;
-        constexpr int B = 16;
-        for (; b + B <= b1; b += B) {
-            double t[B];
-#pragma unroll
-            for (int u = 0; u < B; ++u) t[u] = col[(size_t)(b + u) * len];
-#pragma unroll
-            for (int u = 0; u < B; ++u) s += t[u];
+            for (; b < r1; ++b) s += col[(size_t)b * len];
         }
-        for (; b < b1; ++b) s += col[(size_t)b * len];
+        scratch[tid] = s;
+        __syncthreads();
+        if (tid < ec) {
+            double tot = 0.0;
+            for (int hh = 0; hh < k; ++hh) tot += scratch[hh * ec + tid];
+            out[idx] = tot;
+        }
+        __syncthreads();
     }
-    scratch[tid] = s;
-    __syncthreads();
-    if (h == 0 && tid < E) {
-        double tot = 0.0;
-        for (int hh = 0; hh < k; ++hh) tot += scratch[hh * E + e];
-        red[idx] = tot;
-    }
-    for (int i = tid; i < len; i += nt)
-        if (!((mask >> (i / nx)) & 1u)) red[i] = 0.0;
-    __syncthreads();
+}
+
+// workgroups of k_epi1d for nblocks partial rows
+int epi_groups(int nblocks)
+{
+    int g = (int)std::ceil(std::sqrt((double)nblocks));
+    return g < 1 ? 1 : (g > kEpiGroupsMax ? kEpiGroupsMax : g);
 }
 
 template <int P0, int P1, int MODE>
-__global__ void __launch_bounds__(kEpiThreads) k_epi1d(PassArgs a, int nblocks)
+__global__ void __launch_bounds__(kBlock) k_epi1d(PassArgs a, int nblocks)
 {
     constexpr bool HE_PREV = (MODE & M_HE_PREV) != 0;
     constexpr bool HE = (MODE & M_HE) != 0;
@@ -709,9 +729,11 @@ __global__ void __launch_bounds__(kEpiThreads) k_epi1d(PassArgs a, int nblocks)
     (void)HE;
 
     extern __shared__ double smem[];
+    __shared__ int s_last;
     const int nx = a.nx;
     const int tid = threadIdx.x;
     const int nt = blockDim.x;
+    const int len = NDEP * nx;
     const double dx = a.dx;
     double *sE1 = smem;                     // [nx] each
     double *sE2 = sE1 + nx;
@@ -723,6 +745,19 @@ __global__ void __launch_bounds__(kEpiThreads) k_epi1d(PassArgs a, int nblocks)
     double *T = spare + nx;                 // [NTAB][nx] constant tables
     double *w3 = T + NTAB * nx;             // [6][nx]
     double *scratch = w3 + 6 * nx;          // [blockDim.x]
+
+    // group sums of a slice of the workgroups' rows
+    const int ng = gridDim.x, g = blockIdx.x;
+    const int b0 = (int)(((long)nblocks * g) / ng), b1 = (int)(((long)nblocks * (g + 1)) / ng);
+    sum_rows_range(a.part, b0, b1, nx, REDMASK, a.gpart + (size_t)g * len, scratch);
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) s_last = atomicAdd(&a.cnt[0], 1u) == (unsigned)(ng - 1);
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    if (tid == 0) a.cnt[0] = 0u;            // counters are zero between launches
+
     double *G = a.grid;
     for (int i = tid; i < nx; i += nt) {
         sE1[i] = G[S_E1 * nx + i];
@@ -730,8 +765,9 @@ __global__ void __launch_bounds__(kEpiThreads) k_epi1d(PassArgs a, int nblocks)
         sB[i] = G[S_B * nx + i];
     }
     for (int i = tid; i < NTAB * nx; i += nt) T[i] = a.tabs[i];
-    sum_blocks(a.part, nblocks, nx, REDMASK, sRed, scratch);   // ends with a barrier
-
+    for (int i = tid; i < len; i += nt) sRed[i] = 0.0;
+    __syncthreads();
+    sum_rows_range(a.gpart, 0, ng, nx, REDMASK, sRed, scratch);  // ends with a barrier
     double *r_j2a = sRed + D_J2A * nx, *r_j1 = sRed + D_J1 * nx;
     double *r_rho = sRed + D_RHO * nx, *r_j2b = sRed + D_J2B * nx;
     if constexpr (RHO_ONLY) {
@@ -812,7 +848,7 @@ __global__ void __launch_bounds__(kEpiThreads) k_epi1d(PassArgs a, int nblocks)
 
 size_t epi_shmem_bytes(int nx)
 {
-    return sizeof(double) * ((size_t)(3 + NDEP + 3 + NTAB + 6) * nx + kEpiThreads);
+    return sizeof(double) * ((size_t)(3 + NDEP + 3 + NTAB + 6) * nx + kBlock);
 }
 
 size_t pass_shmem_bytes(int nx, int mode)
@@ -926,11 +962,48 @@ __global__ void __launch_bounds__(kBlock) k_diag1d2v(PassArgs a)
 // ---------------------------------------------------------------------------
 // Launch dispatch
 // ---------------------------------------------------------------------------
-// deposit passes are followed by k_epi1d, except those whose rows the next
-// pass reduces (M_NOEPI)
-constexpr bool mode_has_epi(int m)
+bool mode_has_epi(int m)
 {
     return (m & (M_HP2A | M_HP1 | M_HP2B | M_RHO)) != 0 && (m & M_NOEPI) == 0;
+}
+
+template <int P0, int P1>
+static hipError_t launch_epi_pd(int mode, const PassArgs &a, int pass_blocks, hipStream_t s)
+{
+#define GK_EPI(M)                                                                             \
+    case (M):                                                                                 \
+        hipLaunchKernelGGL((k_epi1d<P0, P1, (M)>), dim3(epi_groups(pass_blocks)), dim3(kBlock), \
+                           epi_shmem_bytes(a.nx), s, a, pass_blocks);                         \
+        return hipGetLastError();
+    switch (mode) {
+        GK_EPI(M_HP2A)
+        GK_EPI(M_HP1)
+        GK_EPI(M_HE | M_HP2A | M_HP1 | M_HP2B)
+        GK_EPI(M_HE_PREV | M_HE | M_HP2A | M_HP1 | M_HP2B)
+        GK_EPI(M_HP1 | M_HP2B | M_WITH_J2A)
+        GK_EPI(M_V1 | M_HP1)
+        GK_EPI(M_V1 | M_HE | M_HP1)
+        GK_EPI(M_V1 | M_HE_PREV | M_HE | M_HP1)
+        GK_EPI(M_RHO)
+    default:
+        return hipErrorInvalidValue;
+    }
+#undef GK_EPI
+}
+
+hipError_t launch_epi(int p0, int p1, int mode, const PassArgs &a, int pass_blocks, hipStream_t s)
+{
+    if (a.nx > kNxMax || pass_blocks < 1) return hipErrorInvalidValue;
+#define GK_PAIR(A, B) \
+    if (p0 == (A) && p1 == (B)) return launch_epi_pd<A, B>(mode, a, pass_blocks, s);
+    GK_PAIR(1, 0)
+    GK_PAIR(1, 1)
+    GK_PAIR(2, 1)
+    GK_PAIR(2, 2)
+    GK_PAIR(3, 2)
+    GK_PAIR(3, 3)
+#undef GK_PAIR
+    return hipErrorInvalidValue;
 }
 
 template <int P0, int P1>
@@ -948,9 +1021,6 @@ static hipError_t launch_pass_pd(int mode, const PassArgs &a, int blocks, size_t
             return e_;                                                                      \
         }                                                                                   \
         hipLaunchKernelGGL((k_pass1d<P0, P1, (M)>), dim3(blocks), dim3(kBlock), shmem, s, a); \
-        if constexpr (mode_has_epi(M))                                                       \
-            hipLaunchKernelGGL((k_epi1d<P0, P1, (M)>), dim3(1), dim3(kEpiThreads),           \
-                               epi_shmem_bytes(a.nx), s, a, blocks);                        \
         return hipGetLastError();
     switch (mode) {
         GK_CASE(M_HE)

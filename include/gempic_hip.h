@@ -130,8 +130,11 @@ int gempic_write_step(gempic_splitting *h, double time, int degree, const double
 /* Per-kernel timing with HIP events recorded on the handle's stream.
  * kernel ids: 0 fused pass (the second kernel of a two-kernel step),
  * 1 trailing pass, 2 HE, 3 Hp2, 4 Hp1, 5 HB/grid, 6 sort, 7 other,
- * 8 first kernel of a two-kernel step. total_ms / count accumulate since the
- * last reset. */
+ * 8 first kernel of a two-kernel step, 9 the deposit sum and field solves
+ * that follow a deposit pass. total_ms / count accumulate since the
+ * last reset. enable: 0 off, 1 every launch, n > 1 every n-th launch of each
+ * kernel id (the event pair costs a few microseconds of stream time, so
+ * sampling keeps it out of a timed run). */
 int gempic_profile_enable(gempic_splitting *h, int enable);
 int gempic_profile_read(gempic_splitting *h, int kernel_id, double *total_ms, int64_t *count);
 int gempic_profile_reset(gempic_splitting *h);
