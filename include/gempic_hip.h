@@ -107,9 +107,10 @@ int gempic_strang_splitting(gempic_splitting *h, double dt, int64_t n_steps, int
 /* Wait for queued work and report any deferred error. */
 int gempic_synchronize(gempic_splitting *h);
 
-/* Particles are kept sorted by cell on the device (their order in the
- * caller's array is restored by gempic_get_particles). interval = re-sort
- * every `interval` Strang steps (0 = never after the upload). */
+/* Particles are kept sorted by position on the device: by cell, then by
+ * sub-cell bin (their order in the caller's array is restored by
+ * gempic_get_particles). interval = re-sort every `interval` Strang steps
+ * (0 = never after the upload). */
 int gempic_set_sort_interval(gempic_splitting *h, int64_t interval);
 int gempic_sort_particles(gempic_splitting *h);
 
