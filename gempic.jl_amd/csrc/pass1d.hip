@@ -28,9 +28,21 @@ namespace gl {
 // `sync` is a barrier over the whole workgroup, so every thread calls this
 // with the same sequence of solves. Each sum runs left to right over j (or
 // k), with the twiddle index advanced incrementally instead of (j*k) % n.
-__device__ void circ_solve_part(const double *eig, const double *rhs, double *out, double *hc,
-                                const double *cos_t, const double *sin_t, int n, int t0, int nt)
+// Output of a circulant solve: out[j] = value (the default), or a fused
+// update of a field by the thread that owns node j.
+struct StoreOut {
+    double *out;
+    __device__ void operator()(int j, double v) const { out[j] = v; }
+};
+
+template <class Emit>
+__device__ void circ_solve_part(const double *eig, const double *rhs, double rhs_scale, Emit emit,
+                                double *hc, const double *cos_t, const double *sin_t, int n, int t0,
+                                int nt)
 {
+    // first pass: R2HC of rhs * rhs_scale (1, or the sub-step's dt that the
+    // reference applies to j_dofs before the solve), then the halfcomplex
+    // product with eig by the thread that holds both halves of mode k
     const int tid = (int)threadIdx.x - t0;
     const bool mine = tid >= 0 && tid < nt;
     if (mine)
@@ -39,27 +51,23 @@ __device__ void circ_solve_part(const double *eig, const double *rhs, double *ou
             int t = 0;
 #pragma unroll 8
             for (int j = 0; j < n; ++j) {
-                re += rhs[j] * cos_t[t];
-                im -= rhs[j] * sin_t[t];
+                const double r = rhs[j] * rhs_scale;
+                re += r * cos_t[t];
+                im -= r * sin_t[t];
                 t += k;
                 if (t >= n) t -= n;
             }
-            hc[k] = re;
-            if (k > 0 && k < n - k) hc[n - k] = im;
-        }
-    __syncthreads();
-    if (mine)
-        for (int k = tid; k <= n / 2; k += nt) {
             if (k == 0 || k == n / 2) {
-                hc[k] = hc[k] * eig[k];
+                hc[k] = re * eig[k];
             } else {
-                double re_p = hc[k] * eig[k] - hc[n - k] * eig[n - k];
-                double im_p = hc[k] * eig[n - k] + hc[n - k] * eig[k];
+                double re_p = re * eig[k] - im * eig[n - k];
+                double im_p = re * eig[n - k] + im * eig[k];
                 hc[k] = re_p;
                 hc[n - k] = im_p;
             }
         }
     __syncthreads();
+    // second pass: HC2R / n
     if (mine)
         for (int j = tid; j < n; j += nt) {
             double s = hc[0];
@@ -71,26 +79,26 @@ __device__ void circ_solve_part(const double *eig, const double *rhs, double *ou
                 if (t >= n) t -= n;
             }
             if ((n & 1) == 0) s += hc[n / 2] * ((j & 1) ? -1.0 : 1.0);
-            out[j] = s / (double)n;
+            emit(j, s / (double)n);
         }
     __syncthreads();
 }
 
+// out = solve_circulant(eig, rhs) (maxwell_1d_fem.jl:222-240) by the whole
+// workgroup
 __device__ void circ_solve(const double *eig, const double *rhs, double *out, double *hc,
                            const double *cos_t, const double *sin_t, int n)
 {
-    circ_solve_part(eig, rhs, out, hc, cos_t, sin_t, n, 0, blockDim.x);
+    circ_solve_part(eig, rhs, 1.0, StoreOut{out}, hc, cos_t, sin_t, n, 0, blockDim.x);
 }
 
-// Three independent solves side by side: thread group g (a third of the
-// workgroup) computes out[g] = HC2R(eig[g] (.) R2HC(rhs[g])) / n with the
-// same sums, in the same order, as circ_solve_part.
 // One circulant solve per third of the workgroup: solve g (0, 1, 2) reads
-// rhs_g, multiplies by eig_g in halfcomplex form and writes out_g, with hc_g
-// as its halfcomplex workspace. The pointers are picked with selects so that
-// they stay in registers.
+// rhs_g (times scale_g), multiplies by eig_g in halfcomplex form and writes
+// out_g, with hc_g as its halfcomplex workspace. The arguments are picked
+// with selects so that they stay in registers.
 struct SolveArgs {
     const double *eig, *rhs;
+    double scale;
     double *out, *hc;
 };
 
@@ -99,65 +107,24 @@ __device__ void circ_solve3(SolveArgs s0, SolveArgs s1, SolveArgs s2, const doub
 {
     const int G = blockDim.x / 3;
     const int g = (int)threadIdx.x / G;
-    const int tid = (int)threadIdx.x - g * G;
-    const bool mine = g < 3;
     const double *eig = g == 1 ? s1.eig : (g == 2 ? s2.eig : s0.eig);
     const double *rhs = g == 1 ? s1.rhs : (g == 2 ? s2.rhs : s0.rhs);
+    const double scale = g == 1 ? s1.scale : (g == 2 ? s2.scale : s0.scale);
     double *out = g == 1 ? s1.out : (g == 2 ? s2.out : s0.out);
     double *hc = g == 1 ? s1.hc : (g == 2 ? s2.hc : s0.hc);
-    if (mine)
-        for (int k = tid; k <= n / 2; k += G) {
-            double re = 0.0, im = 0.0;
-            int t = 0;
-#pragma unroll 8
-            for (int j = 0; j < n; ++j) {
-                re += rhs[j] * cos_t[t];
-                im -= rhs[j] * sin_t[t];
-                t += k;
-                if (t >= n) t -= n;
-            }
-            hc[k] = re;
-            if (k > 0 && k < n - k) hc[n - k] = im;
-        }
-    __syncthreads();
-    if (mine)
-        for (int k = tid; k <= n / 2; k += G) {
-            if (k == 0 || k == n / 2) {
-                hc[k] = hc[k] * eig[k];
-            } else {
-                double re_p = hc[k] * eig[k] - hc[n - k] * eig[n - k];
-                double im_p = hc[k] * eig[n - k] + hc[n - k] * eig[k];
-                hc[k] = re_p;
-                hc[n - k] = im_p;
-            }
-        }
-    __syncthreads();
-    if (mine)
-        for (int j = tid; j < n; j += G) {
-            double s = hc[0];
-            int t = j;
-#pragma unroll 8
-            for (int k = 1; k < n - k; ++k) {
-                s += 2.0 * (hc[k] * cos_t[t] - hc[n - k] * sin_t[t]);
-                t += j;
-                if (t >= n) t -= n;
-            }
-            if ((n & 1) == 0) s += hc[n / 2] * ((j & 1) ? -1.0 : 1.0);
-            out[j] = s / (double)n;
-        }
-    __syncthreads();
+    circ_solve_part(eig, rhs, scale, StoreOut{out}, hc, cos_t, sin_t, n, (g < 3 ? g : 3) * G, G);
 }
 
-// compute_e_from_j! (maxwell_1d_fem.jl:263-289): E -= M^{-1} j / dx
 __device__ void ep_e_from_j(double *E, const double *j, const double *inv_mass, double dx,
                             double *work, double *hc, const double *tabs, int n)
 {
-    circ_solve(inv_mass, j, work, hc, tabs + T_COS * n, tabs + T_SIN * n, n);
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        double wv = work[i] / dx;
+    (void)work;
+    auto emit = [E, dx](int i, double v) {
+        double wv = v / dx;
         E[i] -= wv;
-    }
-    __syncthreads();
+    };
+    circ_solve_part(inv_mass, j, 1.0, emit, hc, tabs + T_COS * n, tabs + T_SIN * n, n, 0,
+                    blockDim.x);
 }
 
 // compute_b_from_e! (maxwell_1d_fem.jl:407-420): B[i] += dt/dx (E[i-1] - E[i])
@@ -176,9 +143,10 @@ __device__ void ep_e_from_b(double *E, const double *B, double dt, double dx, do
                             double *hc, const double *tabs, int n)
 {
     const double coef = dt / dx;
-    circ_solve(tabs + T_AMP * n, B, work, hc, tabs + T_COS * n, tabs + T_SIN * n, n);
-    for (int i = threadIdx.x; i < n; i += blockDim.x) E[i] += coef * work[i];
-    __syncthreads();
+    (void)work;
+    auto emit = [E, coef](int i, double v) { E[i] += coef * v; };
+    circ_solve_part(tabs + T_AMP * n, B, 1.0, emit, hc, tabs + T_COS * n, tabs + T_SIN * n, n, 0,
+                    blockDim.x);
 }
 
 // compute_e_from_rho! (maxwell_1d_fem.jl:244-255)
@@ -746,6 +714,17 @@ __global__ void __launch_bounds__(kBlock) k_epi1d(PassArgs a, int nblocks)
     double *w3 = T + NTAB * nx;             // [6][nx]
     double *scratch = w3 + 6 * nx;          // [blockDim.x]
 
+    // every workgroup stages the fields and tables first: the one that ends up
+    // last has them in LDS without another round trip
+    double *G = a.grid;
+    for (int i = tid; i < nx; i += nt) {
+        sE1[i] = G[S_E1 * nx + i];
+        sE2[i] = G[S_E2 * nx + i];
+        sB[i] = G[S_B * nx + i];
+    }
+    for (int i = tid; i < NTAB * nx; i += nt) T[i] = a.tabs[i];
+    for (int i = tid; i < len; i += nt) sRed[i] = 0.0;
+
     // group sums of a slice of the workgroups' rows
     const int ng = gridDim.x, g = blockIdx.x;
     const int b0 = (int)(((long)nblocks * g) / ng), b1 = (int)(((long)nblocks * (g + 1)) / ng);
@@ -758,15 +737,6 @@ __global__ void __launch_bounds__(kBlock) k_epi1d(PassArgs a, int nblocks)
     __threadfence();
     if (tid == 0) a.cnt[0] = 0u;            // counters are zero between launches
 
-    double *G = a.grid;
-    for (int i = tid; i < nx; i += nt) {
-        sE1[i] = G[S_E1 * nx + i];
-        sE2[i] = G[S_E2 * nx + i];
-        sB[i] = G[S_B * nx + i];
-    }
-    for (int i = tid; i < NTAB * nx; i += nt) T[i] = a.tabs[i];
-    for (int i = tid; i < len; i += nt) sRed[i] = 0.0;
-    __syncthreads();
     sum_rows_range(a.gpart, 0, ng, nx, REDMASK, sRed, scratch);  // ends with a barrier
     double *r_j2a = sRed + D_J2A * nx, *r_j1 = sRed + D_J1 * nx;
     double *r_rho = sRed + D_RHO * nx, *r_j2b = sRed + D_J2B * nx;
@@ -809,14 +779,11 @@ __global__ void __launch_bounds__(kBlock) k_epi1d(PassArgs a, int nblocks)
         // The three inverse-mass solves (compute_e_from_j! of Hp2, Hp1, Hp2)
         // are independent: run them side by side, then apply the updates in
         // the reference's order, E2 -= w_a / dx, E1 -= w_1 / dx, E2 -= w_b / dx.
-        for (int i = tid; i < nx; i += nt) {
-            r_j2a[i] = r_j2a[i] * a.dt_hp2;
-            r_j2b[i] = r_j2b[i] * a.dt_hp2;
-        }
-        __syncthreads();
-        const SolveArgs sa{T + T_INVM0 * nx, r_j2a, w3, w3 + nx};
-        const SolveArgs s1{T + T_INVM1 * nx, r_j1, w3 + 2 * nx, w3 + 3 * nx};
-        const SolveArgs sb{T + T_INVM0 * nx, r_j2b, w3 + 4 * nx, w3 + 5 * nx};
+        // j2 of each Hp2 is scaled by dt inside the solve
+        // (hamiltonian_splitting_1d2v.jl:173)
+        const SolveArgs sa{T + T_INVM0 * nx, r_j2a, a.dt_hp2, w3, w3 + nx};
+        const SolveArgs s1{T + T_INVM1 * nx, r_j1, 1.0, w3 + 2 * nx, w3 + 3 * nx};
+        const SolveArgs sb{T + T_INVM0 * nx, r_j2b, a.dt_hp2, w3 + 4 * nx, w3 + 5 * nx};
         circ_solve3(sa, s1, sb, T + T_COS * nx, T + T_SIN * nx, nx);
         for (int i = tid; i < nx; i += nt) {
             double wa = sa.out[i] / dx;
@@ -829,7 +796,7 @@ __global__ void __launch_bounds__(kBlock) k_epi1d(PassArgs a, int nblocks)
         __syncthreads();
         for (int i = tid; i < nx; i += nt) {
             G[S_J1 * nx + i] = 0.0;
-            G[S_J2 * nx + i] = r_j2b[i];
+            G[S_J2 * nx + i] = r_j2b[i] * a.dt_hp2;
             G[S_E2P * nx + i] = sE2[i];
         }
         ep_b_from_e(sB, sE2, a.dt_he, dx, nx);
