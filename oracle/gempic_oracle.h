@@ -119,6 +119,51 @@ double or_l2norm_squared(or_maxwell1d *m, const double *coefs, int degree);
 double or_inner_product(or_maxwell1d *m, const double *c1, const double *c2, int degree);
 const double *or_maxwell1d_work(const or_maxwell1d *m);
 
+/* ---------- particle_mesh_coupling_2d.jl (gempic_oracle2d.c) ---------- */
+typedef struct {
+    int64_t nx, ny;
+    double xmin, ymin, dx, dy, scaling;
+    int degree, n_span;
+    double pp[(OR_MAXDEG + 1) * (OR_MAXDEG + 1)];
+    double pp_fp[(OR_MAXDEG + 1) * (OR_MAXDEG + 1)];
+} or_pmc2d;
+
+int or_pmc2d_init(or_pmc2d *p, double xmin, double xmax, int64_t nx, double ymin, double ymax,
+                  int64_t ny, int degree, int smoothing);
+void or_pmc2d_add_charge(double *rho, const or_pmc2d *p, double x, double y, double wp);
+void or_pmc2d_add_charge_pp(double *rho, const or_pmc2d *p, double x, double y, double wp);
+double or_pmc2d_evaluate(const or_pmc2d *p, double x, double y, const double *field);
+void or_pmc2d_evaluate_multiple(const or_pmc2d *p, double x, double y, const double *f1,
+                                const double *f2, double *out2);
+void or_pmc2d_b_to_pp(const or_pmc2d *p, const double *b, double *pp);
+double or_pmc2d_evaluate_pp(const or_pmc2d *p, double x, double y, const double *pp,
+                            int *bounds_error);
+
+/* ---------- maxwell_2d_fem.jl, poisson_2d_fem.jl, linear_solver_spline_mass_2d.jl ---------- */
+typedef struct or_maxwell2d or_maxwell2d;
+void or_spline_fem_mass_line(int degree, double *mass_line);
+void or_spline_fem_compute_mass_eig(int64_t n_cells, int degree, const double *mass_line,
+                                    double *eig);
+or_maxwell2d *or_maxwell2d_create(double xmin, double xmax, int64_t nx, double ymin, double ymax,
+                                  int64_t ny, int degree);
+void or_maxwell2d_destroy(or_maxwell2d *m);
+const double *or_maxwell2d_table(const or_maxwell2d *m, int which);
+void or_maxwell2d_solve(or_maxwell2d *m, int form, int component, const double *rhs, double *out);
+void or_maxwell2d_multiply_mass(or_maxwell2d *m, const double *line1, int deg1,
+                                const double *line2, int deg2, const double *in, double *out);
+double or_maxwell2d_inner_product(or_maxwell2d *m, const double *c1, const double *c2,
+                                  int component, int form);
+void or_maxwell2d_compute_e_from_b(or_maxwell2d *m, double dt, double *const *b, double *const *e);
+void or_maxwell2d_compute_b_from_e(or_maxwell2d *m, double dt, double *const *e, double *const *b);
+void or_maxwell2d_compute_e_from_j(or_maxwell2d *m, const double *current, int component,
+                                   double *e);
+void or_maxwell2d_compute_rho_from_e(or_maxwell2d *m, double *const *e, double *rho);
+void or_maxwell2d_compute_e_from_rho(or_maxwell2d *m, const double *rho, double *const *e);
+void or_maxwell2d_fem_rhs(or_maxwell2d *m, double (*f)(double, double, void *), void *ctx,
+                          int component, int form, double *coefs);
+void or_maxwell2d_l2projection(or_maxwell2d *m, double (*f)(double, double, void *), void *ctx,
+                               int component, int form, double *out);
+
 /* ---------- hamiltonian_splitting*.jl ---------- */
 typedef struct or_split or_split;
 

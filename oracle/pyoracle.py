@@ -23,6 +23,8 @@ _I64 = ctypes.c_int64
 _I = ctypes.c_int
 _P = ctypes.POINTER(ctypes.c_double)
 _VP = ctypes.c_void_p
+_PP = ctypes.POINTER(ctypes.POINTER(ctypes.c_double))
+_FN2 = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_void_p)
 
 GALERKIN = 0
 COLLOCATION = 1
@@ -89,6 +91,28 @@ def lib():
             "or_split_set_particles": (_I, [_VP, _P, _I64]),
             "or_solve_poisson": (None, [_P, _VP, _P]),
             "or_write_step": (None, [_VP, _D, _I, _P, _P, _P, _P, _P, _P, _P]),
+            "or_pmc2d_init": (_I, [_VP, _D, _D, _I64, _D, _D, _I64, _I, _I]),
+            "or_pmc2d_add_charge": (None, [_P, _VP, _D, _D, _D]),
+            "or_pmc2d_add_charge_pp": (None, [_P, _VP, _D, _D, _D]),
+            "or_pmc2d_evaluate": (_D, [_VP, _D, _D, _P]),
+            "or_pmc2d_evaluate_multiple": (None, [_VP, _D, _D, _P, _P, _P]),
+            "or_pmc2d_b_to_pp": (None, [_VP, _P, _P]),
+            "or_pmc2d_evaluate_pp": (_D, [_VP, _D, _D, _P, ctypes.POINTER(_I)]),
+            "or_spline_fem_mass_line": (None, [_I, _P]),
+            "or_spline_fem_compute_mass_eig": (None, [_I64, _I, _P, _P]),
+            "or_maxwell2d_create": (_VP, [_D, _D, _I64, _D, _D, _I64, _I]),
+            "or_maxwell2d_destroy": (None, [_VP]),
+            "or_maxwell2d_table": (_P, [_VP, _I]),
+            "or_maxwell2d_solve": (None, [_VP, _I, _I, _P, _P]),
+            "or_maxwell2d_multiply_mass": (None, [_VP, _P, _I, _P, _I, _P, _P]),
+            "or_maxwell2d_inner_product": (_D, [_VP, _P, _P, _I, _I]),
+            "or_maxwell2d_compute_e_from_b": (None, [_VP, _D, _PP, _PP]),
+            "or_maxwell2d_compute_b_from_e": (None, [_VP, _D, _PP, _PP]),
+            "or_maxwell2d_compute_e_from_j": (None, [_VP, _P, _I, _P]),
+            "or_maxwell2d_compute_rho_from_e": (None, [_VP, _PP, _P]),
+            "or_maxwell2d_compute_e_from_rho": (None, [_VP, _P, _PP]),
+            "or_maxwell2d_fem_rhs": (None, [_VP, _FN2, _VP, _I, _I, _P]),
+            "or_maxwell2d_l2projection": (None, [_VP, _FN2, _VP, _I, _I, _P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -324,4 +348,135 @@ class Splitting:
         out = np.zeros(11)
         lib().or_write_step(self.h, time, degree, _p(e1), _p(e2), _p(b), _p(e1_n), _p(e2_n),
                             _p(efield_poisson), _p(out))
+        return out
+
+
+# --------------------------------------------------------------------------
+# 2D: ParticleMeshCoupling2D, TwoDMaxwell (gempic_oracle2d.c)
+
+class _Pmc2dStruct(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int64), ("ny", ctypes.c_int64), ("xmin", _D), ("ymin", _D),
+                ("dx", _D), ("dy", _D), ("scaling", _D), ("degree", _I), ("n_span", _I),
+                ("pp", _D * 36), ("pp_fp", _D * 36)]
+
+
+class PMC2D:
+    """ParticleMeshCoupling2D (particle_mesh_coupling_2d.jl:12-47) on a
+    TwoDGrid (mesh.jl:17-42). Grid arrays are nx*ny, x fastest."""
+
+    def __init__(self, xmin, xmax, nx, ymin, ymax, ny, degree, smoothing=GALERKIN):
+        self.s = _Pmc2dStruct()
+        rc = lib().or_pmc2d_init(ctypes.byref(self.s), xmin, xmax, nx, ymin, ymax, ny, degree,
+                                 smoothing)
+        if rc != 0:
+            raise ValueError("invalid ParticleMeshCoupling2D arguments")
+        self.nx, self.ny, self.degree = nx, ny, degree
+
+    def ptr(self):
+        return ctypes.byref(self.s)
+
+    def add_charge(self, rho, x, y, w):
+        lib().or_pmc2d_add_charge(_p(rho), self.ptr(), x, y, w)
+
+    def add_charge_pp(self, rho, x, y, w):
+        lib().or_pmc2d_add_charge_pp(_p(rho), self.ptr(), x, y, w)
+
+    def evaluate(self, x, y, field):
+        return lib().or_pmc2d_evaluate(self.ptr(), x, y, _p(field))
+
+    def evaluate_multiple(self, x, y, f1, f2):
+        out = np.zeros(2)
+        lib().or_pmc2d_evaluate_multiple(self.ptr(), x, y, _p(f1), _p(f2), _p(out))
+        return out[0], out[1]
+
+    def b_to_pp(self, b):
+        d = self.degree + 1
+        pp = np.zeros((d * d, self.nx * self.ny), order="F")
+        lib().or_pmc2d_b_to_pp(self.ptr(), _p(b), _p(pp))
+        return pp
+
+    def evaluate_pp(self, x, y, pp):
+        err = ctypes.c_int(0)
+        v = lib().or_pmc2d_evaluate_pp(self.ptr(), x, y, _p(pp), ctypes.byref(err))
+        if err.value:
+            raise IndexError("BoundsError: position outside the pp table")
+        return v
+
+
+def spline_fem_mass_line(degree):
+    out = np.zeros(degree + 1)
+    lib().or_spline_fem_mass_line(degree, _p(out))
+    return out
+
+
+def spline_fem_compute_mass_eig(n, degree, mass_line):
+    out = np.zeros(n)
+    ml = np.ascontiguousarray(mass_line, dtype=np.float64)
+    lib().or_spline_fem_compute_mass_eig(n, degree, _p(ml), _p(out))
+    return out
+
+
+def _ptrs3(arrs):
+    return (ctypes.POINTER(_D) * 3)(*[_p(a) for a in arrs])
+
+
+class Maxwell2D:
+    """TwoDMaxwell (maxwell_2d_fem.jl:11-87) with its TwoDPoisson and
+    TwoDLinearSolverSplineMass solvers."""
+
+    def __init__(self, xmin, xmax, nx, ymin, ymax, ny, degree):
+        self.h = lib().or_maxwell2d_create(xmin, xmax, nx, ymin, ymax, ny, degree)
+        if not self.h:
+            raise ValueError("invalid TwoDMaxwell arguments")
+        self.nx, self.ny, self.degree = nx, ny, degree
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_maxwell2d_destroy(self.h)
+            self.h = None
+
+    def table(self, which, n):
+        return np.ctypeslib.as_array(lib().or_maxwell2d_table(self.h, which), shape=(n,)).copy()
+
+    def solve(self, form, component, rhs):
+        out = np.zeros(self.nx * self.ny)
+        lib().or_maxwell2d_solve(self.h, form, component, _p(rhs), _p(out))
+        return out
+
+    def multiply_mass(self, line1, line2, cin):
+        out = np.zeros(self.nx * self.ny)
+        l1 = np.ascontiguousarray(line1, dtype=np.float64)
+        l2 = np.ascontiguousarray(line2, dtype=np.float64)
+        lib().or_maxwell2d_multiply_mass(self.h, _p(l1), len(l1) - 1, _p(l2), len(l2) - 1,
+                                         _p(cin), _p(out))
+        return out
+
+    def inner_product(self, c1, c2, component, form):
+        return lib().or_maxwell2d_inner_product(self.h, _p(c1), _p(c2), component, form)
+
+    def compute_e_from_b(self, e, dt, b):
+        lib().or_maxwell2d_compute_e_from_b(self.h, dt, _ptrs3(b), _ptrs3(e))
+
+    def compute_b_from_e(self, b, dt, e):
+        lib().or_maxwell2d_compute_b_from_e(self.h, dt, _ptrs3(e), _ptrs3(b))
+
+    def compute_e_from_j(self, e, current, component):
+        lib().or_maxwell2d_compute_e_from_j(self.h, _p(current), component, _p(e))
+
+    def compute_rho_from_e(self, rho, e):
+        lib().or_maxwell2d_compute_rho_from_e(self.h, _ptrs3(e), _p(rho))
+
+    def compute_e_from_rho(self, e, rho):
+        lib().or_maxwell2d_compute_e_from_rho(self.h, _p(rho), _ptrs3(e))
+
+    def fem_rhs(self, f, component, form):
+        cb = _FN2(lambda x, y, ctx: float(f(x, y)))
+        out = np.zeros(self.nx * self.ny)
+        lib().or_maxwell2d_fem_rhs(self.h, cb, None, component, form, _p(out))
+        return out
+
+    def l2projection(self, f, component, form):
+        cb = _FN2(lambda x, y, ctx: float(f(x, y)))
+        out = np.zeros(self.nx * self.ny)
+        lib().or_maxwell2d_l2projection(self.h, cb, None, component, form, _p(out))
         return out
