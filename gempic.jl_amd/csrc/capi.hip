@@ -63,6 +63,14 @@ int device_cus()
     return cus;
 }
 
+}  // namespace
+
+int gl::abi_fail(int code, const char *msg) { return fail(code, msg); }
+int gl::abi_check_device() { return check_device(); }
+int gl::abi_device_cus() { return device_cus(); }
+
+namespace {
+
 bool pair_supported(int p0, int p1)
 {
     return (p0 == 1 && (p1 == 0 || p1 == 1)) || (p0 == 2 && (p1 == 1 || p1 == 2)) ||

@@ -1,5 +1,5 @@
-// lib_internal.hpp -- declarations shared by the kernels (pass1d.hip) and
-// the C ABI (capi.hip).
+// lib_internal.hpp -- declarations shared by the kernels (pass1d.hip), the
+// C ABI (capi.hip) and the 2D coupling and field solvers (pic2d.hip).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -149,5 +149,11 @@ struct PmcArgs {
 enum PmcOp { PMC_EVAL = 0, PMC_EVAL_PP, PMC_CHARGE, PMC_CHARGE_PP, PMC_CURRENT_B, PMC_CURRENT_NOB,
              PMC_CURRENT_PP };
 hipError_t launch_pmc(int op, const PmcArgs &a, int blocks, hipStream_t s);
+
+// shared by the C ABI files: record the thread's last error message and
+// return code; check that a GPU is visible; compute units of the device
+int abi_fail(int code, const char *msg);
+int abi_check_device();
+int abi_device_cus();
 
 }  // namespace gl

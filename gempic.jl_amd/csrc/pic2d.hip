@@ -1,0 +1,916 @@
+// pic2d.hip -- the 2D particle-mesh coupling and the 2D Maxwell solver on
+// the GPU (FP64): batched ParticleMeshCoupling2D verbs and the TwoDMaxwell /
+// TwoDPoisson / TwoDLinearSolverSplineMass operations, with their C ABI.
+//
+// Grid arrays are nx*ny doubles, x fastest (the reference's
+// vec(reshape(..., nx, ny))); node (i, j) at i + j*nx.
+//
+// The transforms of the reference (FFTW) are direct DFT sums over twiddle
+// tables computed in long double on the host, one output per thread, summed
+// in index order; the field operations apply the reference's formulas with
+// its rounding order. Deposits are deterministic: each lane caches the 2D
+// patch of its current cell; evicted patches are staged in LDS and added to
+// the wave's private grid in lane order, one node per lane; the waves' grids
+// are summed over workgroups in a fixed order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "../../include/gempic_hip.h"
+#include "device_common.hpp"
+#include "host_setup.hpp"
+#include "lib_internal.hpp"
+
+using namespace gl;
+using namespace gk;
+
+namespace {
+
+constexpr int kGrid2Max = 4096;            // nx*ny of the device deposit (LDS grid per wave)
+
+// ---------------------------------------------------------------------------
+// device: ParticleMeshCoupling2D (particle_mesh_coupling_2d.jl)
+// ---------------------------------------------------------------------------
+struct Pmc2Args {
+    int64_t n;
+    int64_t chunk;              // particles per workgroup (multiple of 64)
+    int nx, ny;
+    double xmin, ymin, dx, dy, scaling;
+    const double *x, *y, *w;    // w: wp = weight * charge (add_charge! argument)
+    const double *f1, *f2;
+    double *out1, *out2;
+    double *part;               // [workgroups][nx*ny]
+    int *err;
+};
+
+// compute_shape_factor (particle_mesh_coupling_2d.jl:54-65): ceil() index, offset (0, 1]; returns the
+// first node (sx, sy) of the (P+1)^2 patch, wrapped into the grid
+template <int P>
+__device__ __forceinline__ bool shape2(const Pmc2Args &a, double x, double y, int &sx, int &sy,
+                                       double (&vx)[P + 1], double (&vy)[P + 1])
+{
+    const double xp = (x - a.xmin) / a.dx;
+    const double yp = (y - a.ymin) / a.dy;
+    const bool ok = fabs(xp) < 1.0e9 && fabs(yp) < 1.0e9;
+    const double cx = ok ? ceil(xp) : 0.0, cy = ok ? ceil(yp) : 0.0;
+    const int ip = (int)cx, jp = (int)cy;
+    bspline<P>(ok ? xp - (double)(ip - 1) : 1.0, vx);
+    bspline<P>(ok ? yp - (double)(jp - 1) : 1.0, vy);
+    // index_1dto2d_column_major (:76-82): nodes ip - P - 1 + a, mod n
+    sx = imod(ip - P - 1, a.nx);
+    sy = imod(jp - P - 1, a.ny);
+    return ok;
+}
+
+template <int P>
+__global__ void __launch_bounds__(kWave) k_pmc2d_charge(Pmc2Args a)
+{
+    constexpr int W = P + 1, WW = W * W;
+    static_assert(WW <= kWave, "one node per lane");
+    extern __shared__ double smem[];
+    const int nn = a.nx * a.ny;
+    double *grid = smem;                             // [nx*ny] this wave's grid
+    double *stage = grid + nn;                       // [64][WW] evicted patches
+    int *skey = (int *)(stage + kWave * WW);         // [64]
+    const int lane = threadIdx.x;
+    for (int i = lane; i < nn; i += kWave) grid[i] = 0.0;
+    __syncthreads();
+
+    int ckey = -1;                                   // cached patch: first node sx + sy*nx
+    double acc[WW];
+#pragma unroll
+    for (int k = 0; k < WW; ++k) acc[k] = 0.0;
+    int errs = 0;
+
+    // stage the evicting lanes' patches, then add them in lane order, node
+    // (a, b) of each patch by lane a*W + b (distinct nodes: nx, ny >= W)
+    auto flush = [&](bool evict) {
+        const unsigned long long m = __ballot(evict);
+        if (!m) return;
+        if (evict) {
+#pragma unroll
+            for (int k = 0; k < WW; ++k) stage[lane * WW + k] = acc[k];
+            skey[lane] = ckey;
+        }
+        __syncthreads();
+        unsigned long long mm = m;
+        while (mm) {
+            const int l = __ffsll((long long)mm) - 1;
+            mm &= mm - 1;
+            if (lane < WW) {
+                const int key = skey[l];
+                const int sx = key % a.nx, sy = key / a.nx;
+                const int ia = lane / W, ib = lane - ia * W;
+                int gx = sx + ia, gy = sy + ib;
+                if (gx >= a.nx) gx -= a.nx;
+                if (gy >= a.ny) gy -= a.ny;
+                grid[gx + gy * a.nx] += stage[l * WW + lane];
+            }
+            __syncthreads();
+        }
+    };
+
+    const int64_t base = (int64_t)blockIdx.x * a.chunk;
+    for (int64_t off = 0; off < a.chunk; off += kWave) {
+        const int64_t i = base + off + lane;
+        const bool valid = off + lane < a.chunk && i < a.n;
+        double vx[W], vy[W];
+        int sx = 0, sy = 0;
+        const double x = valid ? a.x[i] : a.xmin, y = valid ? a.y[i] : a.ymin;
+        const double wp = valid ? a.w[i] : 0.0;
+        const bool ok = shape2<P>(a, x, y, sx, sy, vx, vy);
+        if (valid && !ok) errs |= E_INEXACT;
+        const int key = sx + sy * a.nx;
+        const bool miss = valid && key != ckey;
+        flush(miss && ckey >= 0);
+        if (miss) {
+            ckey = key;
+#pragma unroll
+            for (int k = 0; k < WW; ++k) acc[k] = 0.0;
+        }
+        if (valid) {
+            // add_charge! (:94-105): wp * scaling * Sx[a] * Sy[b]
+            const double ws = wp * a.scaling;
+#pragma unroll
+            for (int ia = 0; ia < W; ++ia)
+#pragma unroll
+                for (int ib = 0; ib < W; ++ib) acc[ia * W + ib] += ws * vx[ia] * vy[ib];
+        }
+    }
+    flush(ckey >= 0);
+    if (errs) atomicOr(a.err, errs);
+    __syncthreads();
+    double *out = a.part + (size_t)blockIdx.x * nn;
+    for (int k = lane; k < nn; k += kWave) out[k] = grid[k];
+}
+
+// rho[k] += sum over workgroups b (in order) of part[b][k]
+__global__ void k_pmc2d_reduce(const double *part, int nblocks, int nn, double *rho)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nn) return;
+    double s = 0.0;
+    for (int b = 0; b < nblocks; ++b) s += part[(size_t)b * nn + k];
+    rho[k] = rho[k] + s;
+}
+
+// evaluate (:189-203) and evaluate_multiple (:214-231)
+template <int P, bool MULTI>
+__global__ void __launch_bounds__(256) k_pmc2d_eval(Pmc2Args a)
+{
+    constexpr int W = P + 1;
+    int errs = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        double vx[W], vy[W];
+        int sx, sy;
+        if (!shape2<P>(a, a.x[i], a.y[i], sx, sy, vx, vy)) errs |= E_INEXACT;
+        double v1 = 0.0, v2 = 0.0;
+        int gx = sx;
+#pragma unroll
+        for (int ia = 0; ia < W; ++ia) {
+            int gy = sy;
+#pragma unroll
+            for (int ib = 0; ib < W; ++ib) {
+                const int k = gx + gy * a.nx;
+                if constexpr (MULTI) {
+                    const double c = vx[ia] * vy[ib];
+                    v1 += a.f1[k] * c;
+                    v2 += a.f2[k] * c;
+                } else {
+                    v1 += a.f1[k] * vx[ia] * vy[ib];
+                }
+                gy = gy + 1 == a.ny ? 0 : gy + 1;
+            }
+            gx = gx + 1 == a.nx ? 0 : gx + 1;
+        }
+        a.out1[i] = v1;
+        if constexpr (MULTI) a.out2[i] = v2;
+    }
+    if (errs) atomicOr(a.err, errs);
+}
+
+// ---------------------------------------------------------------------------
+// device: TwoDMaxwell, TwoDPoisson, TwoDLinearSolverSplineMass
+// ---------------------------------------------------------------------------
+
+// one complex DFT per row (x direction, DIR_X) or per column (y direction):
+// X[k] = sum_j x[j] e^{-+2 pi i jk/n}, backward divided by n; in place
+template <bool DIR_X>
+__global__ void __launch_bounds__(256) k_dft(double *re, double *im, int nx, int ny,
+                                             const double *cs, const double *sn, int backward)
+{
+    __shared__ double sr[kNxMax], si[kNxMax];
+    const int n = DIR_X ? nx : ny;
+    const int line = blockIdx.x;
+    const int64_t b0 = DIR_X ? (int64_t)line * nx : line;
+    const int64_t st = DIR_X ? 1 : nx;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        sr[j] = re[b0 + j * st];
+        si[j] = im[b0 + j * st];
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        double ar = 0.0, ai = 0.0;
+        int q = 0;
+        for (int j = 0; j < n; ++j) {
+            const double xr = sr[j], xi = si[j], c = cs[q], s = sn[q];
+            if (!backward) {
+                ar += xr * c + xi * s;
+                ai += xi * c - xr * s;
+            } else {
+                ar += xr * c - xi * s;
+                ai += xi * c + xr * s;
+            }
+            q += k;
+            if (q >= n) q -= n;
+        }
+        if (backward) {
+            ar = ar / (double)n;
+            ai = ai / (double)n;
+        }
+        re[b0 + k * st] = ar;
+        im[b0 + k * st] = ai;
+    }
+}
+
+struct Field2 {
+    int nx, ny;
+    double dx, dy;
+};
+
+#define FOR_NODES(f)                                                        \
+    const int64_t nn_ = (int64_t)(f).nx * (f).ny;                           \
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nn_; \
+         k += (int64_t)gridDim.x * blockDim.x)
+
+// re = in, im = 0
+__global__ void k_load_real(Field2 f, const double *in, double *re, double *im)
+{
+    FOR_NODES(f) {
+        re[k] = in[k];
+        im[k] = 0.0;
+    }
+}
+
+// linear_solver_spline_mass_2d.jl:23-27: wk[i, j] /= (eig1[i] * eig2[j] * factor)
+__global__ void k_kron_div(Field2 f, double *re, double *im, const double *e1, const double *e2)
+{
+    FOR_NODES(f) {
+        const int i = (int)(k % f.nx), j = (int)(k / f.nx);
+        const double d = e1[i] * e2[j] * 1.0;
+        re[k] = re[k] / d;
+        im[k] = im[k] / d;
+    }
+}
+
+// spline_fem_multiply_mass (maxwell_2d_fem.jl:300-336) along x (DIR_X) or y
+template <bool DIR_X>
+__global__ void k_mass(Field2 f, const double *in, double *out, const double *line, int deg)
+{
+    FOR_NODES(f) {
+        const int i = (int)(k % f.nx), j = (int)(k / f.nx);
+        const int n = DIR_X ? f.nx : f.ny;
+        const int r = DIR_X ? i : j;
+        const int64_t b0 = DIR_X ? (int64_t)j * f.nx : i;
+        const int64_t st = DIR_X ? 1 : f.nx;
+        double o = line[0] * in[b0 + r * st];
+        for (int c = 1; c <= deg; ++c) {
+            int rp = r + c, rm = r - c;
+            if (rp >= n) rp -= n;
+            if (rm < 0) rm += n;
+            o += line[c] * (in[b0 + rp * st] + in[b0 + rm * st]);
+        }
+        out[k] = o;
+    }
+}
+
+// curl of the weak Ampere step (compute_e_from_b!, maxwell_2d_fem.jl:389-402)
+__global__ void k_curl_b(Field2 f, const double *w1, const double *w2, const double *w3,
+                         double *c1, double *c2, double *c3)
+{
+    FOR_NODES(f) {
+        const int i = (int)(k % f.nx), j = (int)(k / f.nx);
+        const int64_t k1 = (i + 1 == f.nx ? 0 : i + 1) + (int64_t)j * f.nx;
+        const int64_t k2 = i + (int64_t)(j + 1 == f.ny ? 0 : j + 1) * f.nx;
+        c1[k] = -(w3[k] - w3[k2]) / f.dy;
+        c2[k] = (w3[k] - w3[k1]) / f.dx;
+        c3[k] = (w1[k] - w1[k2]) / f.dy - (w2[k] - w2[k1]) / f.dx;
+    }
+}
+
+// e = e + dt * w  (:409-411); e = e - w (:458)
+__global__ void k_axpy(Field2 f, double *e, const double *w, double dt)
+{
+    FOR_NODES(f) e[k] = e[k] + dt * w[k];
+}
+
+__global__ void k_sub(Field2 f, double *e, const double *w)
+{
+    FOR_NODES(f) e[k] = e[k] - w[k];
+}
+
+// compute_b_from_e! 2D (maxwell_2d_fem.jl:423-444)
+__global__ void k_b_from_e(Field2 f, double dt, const double *e1, const double *e2,
+                           const double *e3, double *b1, double *b2, double *b3)
+{
+    FOR_NODES(f) {
+        const int i = (int)(k % f.nx), j = (int)(k / f.nx);
+        const int64_t k1 = (i == 0 ? f.nx - 1 : i - 1) + (int64_t)j * f.nx;
+        const int64_t k2 = i + (int64_t)(j == 0 ? f.ny - 1 : j - 1) * f.nx;
+        b1[k] += -dt * (e3[k] - e3[k2]) / f.dy;
+        b2[k] += dt * (e3[k] - e3[k1]) / f.dx;
+        b3[k] += -dt * ((e2[k] - e2[k1]) / f.dx - (e1[k] - e1[k2]) / f.dy);
+    }
+}
+
+// compute_rho_from_e! divergence (maxwell_2d_fem.jl:484-498)
+__global__ void k_div(Field2 f, const double *w1, const double *w2, double *rho)
+{
+    FOR_NODES(f) {
+        const int i = (int)(k % f.nx), j = (int)(k / f.nx);
+        const int64_t k1 = (i + 1 == f.nx ? 0 : i + 1) + (int64_t)j * f.nx;
+        const int64_t k2 = i + (int64_t)(j + 1 == f.ny ? 0 : j + 1) * f.nx;
+        const double r = (w1[k] - w1[k1]) / f.dx + (w2[k] - w2[k2]) / f.dy;
+        rho[k] = r * -1.0;
+    }
+}
+
+struct PoissonTabs {
+    const double *eig0x, *eig0y, *dtm1x, *dtm1y, *d1re, *d1im, *d2re, *d2im;
+};
+
+// TwoDPoisson mode update (poisson_2d_fem.jl:243-256)
+__global__ void k_poisson_modes(Field2 f, PoissonTabs t, const double *sr, const double *si,
+                                double *xr, double *xi, double *yr, double *yi)
+{
+    FOR_NODES(f) {
+        const int i = (int)(k % f.nx), j = (int)(k / f.nx);
+        double r, m;
+        if (i == 0 && j == 0) {
+            r = 0.0;
+            m = 0.0;
+        } else {
+            const double ev = t.dtm1x[i] * t.eig0y[j] + t.eig0x[i] * t.dtm1y[j];
+            r = sr[k] / ev;
+            m = si[k] / ev;
+        }
+        const double ar = -r, ai = -m;
+        xr[k] = ar * t.d1re[i] - ai * t.d1im[i];
+        xi[k] = ar * t.d1im[i] + ai * t.d1re[i];
+        yr[k] = ar * t.d2re[j] - ai * t.d2im[j];
+        yi[k] = ar * t.d2im[j] + ai * t.d2re[j];
+    }
+}
+
+// sum(c1 .* w) in a fixed order: contiguous chunks per thread, then the
+// chunk sums in thread order (one workgroup)
+__global__ void __launch_bounds__(1024) k_dot(Field2 f, const double *c1, const double *w,
+                                              double *out)
+{
+    __shared__ double part[1024];
+    const int64_t nn = (int64_t)f.nx * f.ny;
+    const int nt = blockDim.x, t = threadIdx.x;
+    const int64_t per = (nn + nt - 1) / nt;
+    const int64_t a0 = std::min<int64_t>(nn, per * t), a1 = std::min<int64_t>(nn, a0 + per);
+    double s = 0.0;
+    for (int64_t k = a0; k < a1; ++k) s += c1[k] * w[k];
+    part[t] = s;
+    __syncthreads();
+    if (t == 0) {
+        double r = 0.0;
+        for (int k = 0; k < nt; ++k) r += part[k];
+        out[0] = r;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host set-up (poisson_2d_fem.jl:23-126, maxwell_2d_fem.jl:26-87)
+// ---------------------------------------------------------------------------
+void host_bspline(int degree, double off, double *b)
+{
+    b[0] = 1.0;
+    for (int j = 1; j <= degree; ++j) {
+        double xx = -off;
+        const double jr = (double)j;
+        const double inv_j = 1.0 / jr;
+        double saved = 0.0;
+        for (int r = 0; r < j; ++r) {
+            xx = xx + 1.0;
+            const double temp = b[r] * inv_j;
+            b[r] = saved + xx * temp;
+            saved = (jr - xx) * temp;
+        }
+        b[j] = saved;
+    }
+}
+
+// spline_fem_mass_line (poisson_2d_fem.jl:87-106)
+void host_mass_line(int degree, double *line)
+{
+    const int n = degree + 1;
+    double x[8], w[8], val[8 * 8];
+    gh::gauss_legendre(n, x, w);
+    for (int k = 0; k < n; ++k) {
+        x[k] += 1.0;
+        x[k] *= 0.5;
+        w[k] *= 0.5;
+    }
+    for (int j = 0; j < n; ++j) host_bspline(degree, x[j], val + j * (degree + 1));
+    for (int j = 0; j <= degree; ++j) line[j] = 0.0;
+    for (int j = 1; j <= degree + 1; ++j)
+        for (int i = j; i <= degree + 1; ++i)
+            for (int k = 1; k <= n; ++k)
+                line[j - 1] += val[(i - 1) + (k - 1) * (degree + 1)] *
+                               val[(i - j) + (k - 1) * (degree + 1)] * w[k - 1];
+}
+
+// spline_fem_compute_mass_eig (poisson_2d_fem.jl:113-126)
+void host_mass_eig(int64_t n, int degree, const double *line, double *eig)
+{
+    const double factor = 6.283185307179586 / (double)n;
+    for (int64_t k = 1; k <= n; ++k) {
+        double e = line[0];
+        for (int j = 1; j <= degree; ++j)
+            e += line[j] * 2.0 * std::cos(factor * (double)(k - 1) * (double)j);
+        eig[k - 1] = e;
+    }
+}
+
+void host_twiddles(int64_t n, double *c, double *s)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)i / (long double)n;
+        c[i] = (double)cosl(ang);
+        s[i] = (double)sinl(ang);
+    }
+}
+
+template <typename T>
+hipError_t dalloc2(T **p, size_t count)
+{
+    return hipMalloc((void **)p, sizeof(T) * std::max<size_t>(count, 1));
+}
+
+unsigned grid_blocks(int64_t nn) { return (unsigned)std::min<int64_t>(1024, (nn + 255) / 256); }
+
+}  // namespace
+
+// table slots of a TwoDMaxwell handle (each of length kNxMax)
+enum Tab2 {
+    T2_CX = 0, T2_SX, T2_CY, T2_SY, T2_E0X, T2_E1X, T2_E0Y, T2_E1Y,
+    T2_DTMX, T2_DTMY, T2_D1R, T2_D1I, T2_D2R, T2_D2I, T2_L0X, T2_L1X, T2_L0Y, T2_L1Y, NT2
+};
+
+struct gempic_maxwell2d {
+    hipStream_t stream = nullptr;
+    int nx = 0, ny = 0, degree = 3;
+    double dx = 0, dy = 0;
+    double *d_tab = nullptr;            // NT2 x kNxMax
+    double *d_buf = nullptr;            // 16 work arrays of nx*ny
+    double *d_scalar = nullptr;
+    const double *tab(int s) const { return d_tab + (size_t)s * kNxMax; }
+    double *buf(int i) const { return d_buf + (size_t)i * nx * ny; }
+    Field2 f() const { return Field2{nx, ny, dx, dy}; }
+};
+
+namespace {
+
+// out = TwoDLinearSolverSplineMass(e1, e2).solve(in) (buffers on the device)
+void kron_solve(gempic_maxwell2d *m, const double *e1, const double *e2, const double *in,
+                double *out)
+{
+    const Field2 f = m->f();
+    const unsigned gb = grid_blocks((int64_t)m->nx * m->ny);
+    double *re = m->buf(14), *im = m->buf(15);
+    hipLaunchKernelGGL(k_load_real, dim3(gb), dim3(256), 0, m->stream, f, in, re, im);
+    hipLaunchKernelGGL(k_dft<true>, dim3(m->ny), dim3(256), 0, m->stream, re, im, m->nx, m->ny,
+                       m->tab(T2_CX), m->tab(T2_SX), 0);
+    hipLaunchKernelGGL(k_dft<false>, dim3(m->nx), dim3(256), 0, m->stream, re, im, m->nx, m->ny,
+                       m->tab(T2_CY), m->tab(T2_SY), 0);
+    hipLaunchKernelGGL(k_kron_div, dim3(gb), dim3(256), 0, m->stream, f, re, im, e1, e2);
+    hipLaunchKernelGGL(k_dft<true>, dim3(m->ny), dim3(256), 0, m->stream, re, im, m->nx, m->ny,
+                       m->tab(T2_CX), m->tab(T2_SX), 1);
+    hipLaunchKernelGGL(k_dft<false>, dim3(m->nx), dim3(256), 0, m->stream, re, im, m->nx, m->ny,
+                       m->tab(T2_CY), m->tab(T2_SY), 1);
+    (void)hipMemcpyAsync(out, re, sizeof(double) * m->nx * m->ny, hipMemcpyDeviceToDevice,
+                         m->stream);
+}
+
+// the inverse mass of a 1-form (form 1) or 2-form (form 2) component
+// (maxwell_2d_fem.jl:56-66)
+void inv_mass(gempic_maxwell2d *m, int form, int component, const double *in, double *out)
+{
+    const double *ex, *ey;
+    if (form == 1) {
+        ex = m->tab(component == 1 ? T2_E1X : T2_E0X);
+        ey = m->tab(component == 2 ? T2_E1Y : T2_E0Y);
+    } else {
+        ex = m->tab(component == 1 ? T2_E0X : T2_E1X);
+        ey = m->tab(component == 2 ? T2_E0Y : T2_E1Y);
+    }
+    kron_solve(m, ex, ey, in, out);
+}
+
+// multiply_mass_2dkron! (maxwell_2d_fem.jl:343-363) with lines (a: mass 1 along x, b: along y)
+void mass2(gempic_maxwell2d *m, bool onex, bool oney, const double *in, double *out)
+{
+    const Field2 f = m->f();
+    const unsigned gb = grid_blocks((int64_t)m->nx * m->ny);
+    double *tmp = m->buf(13);
+    hipLaunchKernelGGL(k_mass<true>, dim3(gb), dim3(256), 0, m->stream, f, in, tmp,
+                       m->tab(onex ? T2_L1X : T2_L0X), onex ? m->degree - 1 : m->degree);
+    hipLaunchKernelGGL(k_mass<false>, dim3(gb), dim3(256), 0, m->stream, f, tmp, out,
+                       m->tab(oney ? T2_L1Y : T2_L0Y), oney ? m->degree - 1 : m->degree);
+}
+
+// which mass lines a (form, component) uses (inner_product, maxwell_2d_fem.jl:511-575)
+void form_lines(int form, int component, bool &onex, bool &oney)
+{
+    onex = oney = false;
+    if (form == 1) { onex = component == 1; oney = component == 2; }
+    else if (form == 2) { onex = component != 1; oney = component != 2; }
+    else if (form == 3) { onex = oney = true; }
+}
+
+int finish2(gempic_maxwell2d *m)
+{
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+    if (e != hipSuccess) return abi_fail(GEMPIC_ERR_DEVICE, (std::string("2D field kernels: ") + hipGetErrorString(e)).c_str());
+    return GEMPIC_OK;
+}
+
+size_t nn2(const gempic_maxwell2d *m) { return (size_t)m->nx * m->ny; }
+
+#define H2D(dst, src) \
+    (void)hipMemcpyAsync((dst), (src), sizeof(double) * nn2(m), hipMemcpyHostToDevice, m->stream)
+#define D2H(dst, src) \
+    (void)hipMemcpyAsync((dst), (src), sizeof(double) * nn2(m), hipMemcpyDeviceToHost, m->stream)
+
+}  // namespace
+
+extern "C" {
+
+int gempic_maxwell2d_create(int64_t nx, double xmin, double xmax, int64_t ny, double ymin,
+                            double ymax, int degree, gempic_maxwell2d **out)
+{
+    if (!out) return abi_fail(GEMPIC_ERR_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    if (degree < 1 || degree > 3)
+        return abi_fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: TwoDMaxwell degree must be 1, 2 or 3");
+    if (nx < 2 * degree + 1 || ny < 2 * degree + 1 || nx > kNxMax || ny > kNxMax)
+        return abi_fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: nx, ny must be in [2*degree+1, 256]");
+    if (!(xmax > xmin) || !(ymax > ymin))
+        return abi_fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: empty domain");
+    int rc = abi_check_device();
+    if (rc) return rc;
+    gempic_maxwell2d *m = new gempic_maxwell2d();
+    m->nx = (int)nx;
+    m->ny = (int)ny;
+    m->degree = degree;
+    m->dx = (xmax - xmin) / (double)nx;     // TwoDGrid (mesh.jl:38-39)
+    m->dy = (ymax - ymin) / (double)ny;
+    std::vector<double> T((size_t)NT2 * kNxMax, 0.0);
+    auto slot = [&](int s) { return T.data() + (size_t)s * kNxMax; };
+    host_twiddles(nx, slot(T2_CX), slot(T2_SX));
+    host_twiddles(ny, slot(T2_CY), slot(T2_SY));
+    double l0[8], l1[8];
+    host_mass_line(degree, l0);
+    host_mass_line(degree - 1, l1);
+    for (int k = 0; k <= degree; ++k) {
+        slot(T2_L0X)[k] = l0[k] * m->dx;
+        slot(T2_L0Y)[k] = l0[k] * m->dy;
+    }
+    for (int k = 0; k < degree; ++k) {
+        slot(T2_L1X)[k] = l1[k] * m->dx;
+        slot(T2_L1Y)[k] = l1[k] * m->dy;
+    }
+    host_mass_eig(nx, degree, slot(T2_L0X), slot(T2_E0X));
+    host_mass_eig(ny, degree, slot(T2_L0Y), slot(T2_E0Y));
+    host_mass_eig(nx, degree - 1, slot(T2_L1X), slot(T2_E1X));
+    host_mass_eig(ny, degree - 1, slot(T2_L1Y), slot(T2_E1Y));
+    // TwoDPoisson (poisson_2d_fem.jl:48-64)
+    const double twopi = 6.283185307179586;
+    for (int64_t j = 2; j <= nx; ++j) {
+        const double angle = twopi * (double)(j - 1) / (double)nx;
+        slot(T2_D1R)[j - 1] = (1.0 - std::cos(angle)) / m->dx;
+        slot(T2_D1I)[j - 1] = std::sin(angle) / m->dx;
+        slot(T2_DTMX)[j - 1] = 2.0 / (m->dx * m->dx) * (1.0 - std::cos(angle)) * slot(T2_E1X)[j - 1];
+    }
+    for (int64_t j = 2; j <= ny; ++j) {
+        const double angle = twopi * (double)(j - 1) / (double)ny;
+        slot(T2_D2R)[j - 1] = (1.0 - std::cos(angle)) / m->dy;
+        slot(T2_D2I)[j - 1] = std::sin(angle) / m->dy;
+        slot(T2_DTMY)[j - 1] = 2.0 / (m->dy * m->dy) * (1.0 - std::cos(angle)) * slot(T2_E1Y)[j - 1];
+    }
+    hipError_t e = hipStreamCreate(&m->stream);
+    if (e == hipSuccess) e = dalloc2(&m->d_tab, T.size());
+    if (e == hipSuccess) e = dalloc2(&m->d_buf, (size_t)16 * nx * ny);
+    if (e == hipSuccess) e = dalloc2(&m->d_scalar, 8);
+    if (e == hipSuccess)
+        e = hipMemcpy(m->d_tab, T.data(), sizeof(double) * T.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        gempic_maxwell2d_destroy(m);
+        return abi_fail(GEMPIC_ERR_DEVICE, (std::string("TwoDMaxwell allocation: ") + hipGetErrorString(e)).c_str());
+    }
+    *out = m;
+    return GEMPIC_OK;
+}
+
+int gempic_maxwell2d_destroy(gempic_maxwell2d *m)
+{
+    if (!m) return GEMPIC_OK;
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    if (m->d_tab) (void)hipFree(m->d_tab);
+    if (m->d_buf) (void)hipFree(m->d_buf);
+    if (m->d_scalar) (void)hipFree(m->d_scalar);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+    return GEMPIC_OK;
+}
+
+int gempic_maxwell2d_tables(gempic_maxwell2d *m, double *eig0x, double *eig0y, double *eig1x,
+                            double *eig1y)
+{
+    if (!m) return abi_fail(GEMPIC_ERR_ARGUMENT, "null handle");
+    const int slots[4] = {T2_E0X, T2_E0Y, T2_E1X, T2_E1Y};
+    double *outs[4] = {eig0x, eig0y, eig1x, eig1y};
+    for (int s = 0; s < 4; ++s)
+        if (outs[s])
+            (void)hipMemcpyAsync(outs[s], m->tab(slots[s]), sizeof(double) * (s % 2 ? m->ny : m->nx),
+                                 hipMemcpyDeviceToHost, m->stream);
+    return finish2(m);
+}
+
+int gempic_maxwell2d_solve(gempic_maxwell2d *m, int form, int component, const double *rhs,
+                           double *out)
+{
+    if (!m || !rhs || !out) return abi_fail(GEMPIC_ERR_ARGUMENT, "NULL argument");
+    if ((form != 1 && form != 2) || component < 1 || component > 3)
+        return abi_fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: form must be 1 or 2, component 1..3");
+    H2D(m->buf(0), rhs);
+    inv_mass(m, form, component, m->buf(0), m->buf(1));
+    D2H(out, m->buf(1));
+    return finish2(m);
+}
+
+int gempic_maxwell2d_multiply_mass(gempic_maxwell2d *m, int form, int component, const double *in,
+                                   double *out)
+{
+    if (!m || !in || !out) return abi_fail(GEMPIC_ERR_ARGUMENT, "NULL argument");
+    if (form < 0 || form > 3 || component < 1 || component > 3)
+        return abi_fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: form 0..3, component 1..3");
+    bool ox, oy;
+    form_lines(form, component, ox, oy);
+    H2D(m->buf(0), in);
+    mass2(m, ox, oy, m->buf(0), m->buf(1));
+    D2H(out, m->buf(1));
+    return finish2(m);
+}
+
+int gempic_maxwell2d_inner_product(gempic_maxwell2d *m, const double *c1, const double *c2,
+                                   int component, int form, double *out)
+{
+    if (!m || !c1 || !c2 || !out) return abi_fail(GEMPIC_ERR_ARGUMENT, "NULL argument");
+    if (form < 0 || form > 3 || component < 1 || component > 3)
+        return abi_fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: form 0..3, component 1..3");
+    bool ox, oy;
+    form_lines(form, component, ox, oy);
+    H2D(m->buf(0), c1);
+    H2D(m->buf(1), c2);
+    mass2(m, ox, oy, m->buf(1), m->buf(2));
+    hipLaunchKernelGGL(k_dot, dim3(1), dim3(1024), 0, m->stream, m->f(), m->buf(0), m->buf(2),
+                       m->d_scalar);
+    (void)hipMemcpyAsync(out, m->d_scalar, sizeof(double), hipMemcpyDeviceToHost, m->stream);
+    return finish2(m);
+}
+
+int gempic_maxwell2d_compute_e_from_b(gempic_maxwell2d *m, double dt, const double *b1,
+                                      const double *b2, const double *b3, double *e1, double *e2,
+                                      double *e3)
+{
+    if (!m || !b1 || !b2 || !b3 || !e1 || !e2 || !e3)
+        return abi_fail(GEMPIC_ERR_ARGUMENT, "NULL argument");
+    const Field2 f = m->f();
+    const unsigned gb = grid_blocks((int64_t)m->nx * m->ny);
+    H2D(m->buf(0), b1); H2D(m->buf(1), b2); H2D(m->buf(2), b3);
+    H2D(m->buf(3), e1); H2D(m->buf(4), e2); H2D(m->buf(5), e3);
+    mass2(m, false, true, m->buf(0), m->buf(6));
+    mass2(m, true, false, m->buf(1), m->buf(7));
+    mass2(m, true, true, m->buf(2), m->buf(8));
+    hipLaunchKernelGGL(k_curl_b, dim3(gb), dim3(256), 0, m->stream, f, m->buf(6), m->buf(7),
+                       m->buf(8), m->buf(9), m->buf(10), m->buf(11));
+    for (int c = 0; c < 3; ++c) {
+        inv_mass(m, 1, c + 1, m->buf(9 + c), m->buf(12));
+        hipLaunchKernelGGL(k_axpy, dim3(gb), dim3(256), 0, m->stream, f, m->buf(3 + c), m->buf(12), dt);
+    }
+    D2H(e1, m->buf(3)); D2H(e2, m->buf(4)); D2H(e3, m->buf(5));
+    return finish2(m);
+}
+
+int gempic_maxwell2d_compute_b_from_e(gempic_maxwell2d *m, double dt, const double *e1,
+                                      const double *e2, const double *e3, double *b1, double *b2,
+                                      double *b3)
+{
+    if (!m || !b1 || !b2 || !b3 || !e1 || !e2 || !e3)
+        return abi_fail(GEMPIC_ERR_ARGUMENT, "NULL argument");
+    const unsigned gb = grid_blocks((int64_t)m->nx * m->ny);
+    H2D(m->buf(0), e1); H2D(m->buf(1), e2); H2D(m->buf(2), e3);
+    H2D(m->buf(3), b1); H2D(m->buf(4), b2); H2D(m->buf(5), b3);
+    hipLaunchKernelGGL(k_b_from_e, dim3(gb), dim3(256), 0, m->stream, m->f(), dt, m->buf(0),
+                       m->buf(1), m->buf(2), m->buf(3), m->buf(4), m->buf(5));
+    D2H(b1, m->buf(3)); D2H(b2, m->buf(4)); D2H(b3, m->buf(5));
+    return finish2(m);
+}
+
+int gempic_maxwell2d_compute_e_from_j(gempic_maxwell2d *m, const double *current, int component,
+                                      double *e)
+{
+    if (!m || !current || !e) return abi_fail(GEMPIC_ERR_ARGUMENT, "NULL argument");
+    if (component < 1 || component > 3) return abi_fail(GEMPIC_ERR_ARGUMENT, "component 1..3");
+    const unsigned gb = grid_blocks((int64_t)m->nx * m->ny);
+    H2D(m->buf(0), current);
+    H2D(m->buf(1), e);
+    inv_mass(m, 1, component, m->buf(0), m->buf(2));
+    hipLaunchKernelGGL(k_sub, dim3(gb), dim3(256), 0, m->stream, m->f(), m->buf(1), m->buf(2));
+    D2H(e, m->buf(1));
+    return finish2(m);
+}
+
+int gempic_maxwell2d_compute_rho_from_e(gempic_maxwell2d *m, const double *e1, const double *e2,
+                                        const double *e3, double *rho)
+{
+    if (!m || !e1 || !e2 || !rho) return abi_fail(GEMPIC_ERR_ARGUMENT, "NULL argument");
+    (void)e3;   // the third component's mass product is not used (maxwell_2d_fem.jl:476-478)
+    const unsigned gb = grid_blocks((int64_t)m->nx * m->ny);
+    H2D(m->buf(0), e1); H2D(m->buf(1), e2);
+    mass2(m, true, false, m->buf(0), m->buf(2));
+    mass2(m, false, true, m->buf(1), m->buf(3));
+    hipLaunchKernelGGL(k_div, dim3(gb), dim3(256), 0, m->stream, m->f(), m->buf(2), m->buf(3),
+                       m->buf(4));
+    D2H(rho, m->buf(4));
+    return finish2(m);
+}
+
+int gempic_maxwell2d_compute_e_from_rho(gempic_maxwell2d *m, const double *rho, double *e1,
+                                        double *e2)
+{
+    if (!m || !rho || !e1 || !e2) return abi_fail(GEMPIC_ERR_ARGUMENT, "NULL argument");
+    const Field2 f = m->f();
+    const unsigned gb = grid_blocks((int64_t)m->nx * m->ny);
+    double *sr = m->buf(0), *si = m->buf(1);
+    H2D(m->buf(2), rho);
+    hipLaunchKernelGGL(k_load_real, dim3(gb), dim3(256), 0, m->stream, f, m->buf(2), sr, si);
+    // fft2d! (poisson_2d_fem.jl:211-235): x then y
+    hipLaunchKernelGGL(k_dft<true>, dim3(m->ny), dim3(256), 0, m->stream, sr, si, m->nx, m->ny,
+                       m->tab(T2_CX), m->tab(T2_SX), 0);
+    hipLaunchKernelGGL(k_dft<false>, dim3(m->nx), dim3(256), 0, m->stream, sr, si, m->nx, m->ny,
+                       m->tab(T2_CY), m->tab(T2_SY), 0);
+    PoissonTabs t{m->tab(T2_E0X), m->tab(T2_E0Y), m->tab(T2_DTMX), m->tab(T2_DTMY),
+                  m->tab(T2_D1R), m->tab(T2_D1I), m->tab(T2_D2R), m->tab(T2_D2I)};
+    hipLaunchKernelGGL(k_poisson_modes, dim3(gb), dim3(256), 0, m->stream, f, t, sr, si,
+                       m->buf(3), m->buf(4), m->buf(5), m->buf(6));
+    // ifft2d! (:184-209): y then x, real part
+    for (int c = 0; c < 2; ++c) {
+        double *r = m->buf(3 + 2 * c), *i = m->buf(4 + 2 * c);
+        hipLaunchKernelGGL(k_dft<false>, dim3(m->nx), dim3(256), 0, m->stream, r, i, m->nx, m->ny,
+                           m->tab(T2_CY), m->tab(T2_SY), 1);
+        hipLaunchKernelGGL(k_dft<true>, dim3(m->ny), dim3(256), 0, m->stream, r, i, m->nx, m->ny,
+                           m->tab(T2_CX), m->tab(T2_SX), 1);
+    }
+    D2H(e1, m->buf(3));
+    D2H(e2, m->buf(5));
+    return finish2(m);
+}
+
+// ---------------------------------------------------------------------------
+// batched ParticleMeshCoupling2D verbs
+// ---------------------------------------------------------------------------
+static hipStream_t pmc2_stream()
+{
+    static thread_local hipStream_t s = nullptr;
+    if (!s) (void)hipStreamCreate(&s);
+    return s;
+}
+
+static int pmc2_run(int op, int64_t nx, int64_t ny, double xmin, double xmax, double ymin,
+                    double ymax, int degree, int smoothing, int64_t n, const double *x,
+                    const double *y, const double *w, const double *f1, const double *f2,
+                    double *out1, double *out2, double *rho)
+{
+    if (degree < 1 || degree > 3)
+        return abi_fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: 2D coupling degree must be 1, 2 or 3");
+    if (nx < degree + 1 || ny < degree + 1 || nx > kNxMax || ny > kNxMax)
+        return abi_fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: nx, ny must be in [degree+1, 256]");
+    if (op == 0 && nx * ny > kGrid2Max)
+        return abi_fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: the device deposit holds nx*ny <= 4096");
+    if (smoothing != GEMPIC_GALERKIN && smoothing != GEMPIC_COLLOCATION)
+        return abi_fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: smoothing type not implemented");
+    if (!(xmax > xmin) || !(ymax > ymin)) return abi_fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: empty domain");
+    if (n < 0) return abi_fail(GEMPIC_ERR_ARGUMENT, "n must be >= 0");
+    int rc = abi_check_device();
+    if (rc) return rc;
+    if (n == 0) return GEMPIC_OK;
+    hipStream_t s = pmc2_stream();
+    const size_t nn = (size_t)nx * ny;
+    Pmc2Args a{};
+    a.n = n;
+    a.nx = (int)nx;
+    a.ny = (int)ny;
+    a.xmin = xmin;
+    a.ymin = ymin;
+    a.dx = (xmax - xmin) / (double)nx;
+    a.dy = (ymax - ymin) / (double)ny;
+    a.scaling = smoothing == GEMPIC_COLLOCATION ? 1.0 / (a.dx * a.dy) : 1.0;
+    double *dx_ = nullptr, *dy_ = nullptr, *dw = nullptr, *df1 = nullptr, *df2 = nullptr;
+    double *do1 = nullptr, *do2 = nullptr, *dpart = nullptr, *drho = nullptr;
+    int *derr = nullptr;
+    const int cus = abi_device_cus();
+    const int blocks = (int)std::min<int64_t>((int64_t)cus * 4, (n + 63) / 64);
+    a.chunk = ((n + blocks - 1) / blocks + 63) / 64 * 64;
+    const int nblk = (int)((n + a.chunk - 1) / a.chunk);
+    hipError_t e = hipSuccess;
+    auto ok = [&](hipError_t r) { if (r != hipSuccess && e == hipSuccess) e = r; };
+    ok(dalloc2(&dx_, n)); ok(dalloc2(&dy_, n)); ok(dalloc2(&derr, 1));
+    if (op == 0) { ok(dalloc2(&dw, n)); ok(dalloc2(&dpart, (size_t)nblk * nn)); ok(dalloc2(&drho, nn)); }
+    else { ok(dalloc2(&df1, nn)); ok(dalloc2(&do1, n)); if (op == 2) { ok(dalloc2(&df2, nn)); ok(dalloc2(&do2, n)); } }
+    if (e == hipSuccess) {
+        ok(hipMemcpyAsync(dx_, x, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        ok(hipMemcpyAsync(dy_, y, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        ok(hipMemsetAsync(derr, 0, sizeof(int), s));
+        a.x = dx_; a.y = dy_; a.err = derr;
+        if (op == 0) {
+            ok(hipMemcpyAsync(dw, w, sizeof(double) * n, hipMemcpyHostToDevice, s));
+            ok(hipMemcpyAsync(drho, rho, sizeof(double) * nn, hipMemcpyHostToDevice, s));
+            a.w = dw; a.part = dpart;
+            const size_t sh = sizeof(double) * (nn + (size_t)kWave * 16) + sizeof(int) * kWave;
+            if (degree == 1) hipLaunchKernelGGL(k_pmc2d_charge<1>, dim3(nblk), dim3(kWave), sh, s, a);
+            else if (degree == 2) hipLaunchKernelGGL(k_pmc2d_charge<2>, dim3(nblk), dim3(kWave), sh, s, a);
+            else hipLaunchKernelGGL(k_pmc2d_charge<3>, dim3(nblk), dim3(kWave), sh, s, a);
+            hipLaunchKernelGGL(k_pmc2d_reduce, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s,
+                               dpart, nblk, (int)nn, drho);
+            ok(hipGetLastError());
+            ok(hipMemcpyAsync(rho, drho, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
+        } else {
+            ok(hipMemcpyAsync(df1, f1, sizeof(double) * nn, hipMemcpyHostToDevice, s));
+            a.f1 = df1; a.out1 = do1;
+            if (op == 2) {
+                ok(hipMemcpyAsync(df2, f2, sizeof(double) * nn, hipMemcpyHostToDevice, s));
+                a.f2 = df2; a.out2 = do2;
+            }
+            const unsigned gb = (unsigned)std::min<int64_t>((int64_t)cus * 8, (n + 255) / 256);
+#define EV(P, M) hipLaunchKernelGGL((k_pmc2d_eval<P, M>), dim3(gb), dim3(256), 0, s, a)
+            if (op == 1) { if (degree == 1) EV(1, false); else if (degree == 2) EV(2, false); else EV(3, false); }
+            else { if (degree == 1) EV(1, true); else if (degree == 2) EV(2, true); else EV(3, true); }
+#undef EV
+            ok(hipGetLastError());
+            ok(hipMemcpyAsync(out1, do1, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+            if (op == 2) ok(hipMemcpyAsync(out2, do2, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        }
+    }
+    int herr = 0;
+    if (e == hipSuccess) ok(hipMemcpyAsync(&herr, derr, sizeof(int), hipMemcpyDeviceToHost, s));
+    if (e == hipSuccess) ok(hipStreamSynchronize(s));
+    void *bufs[] = {dx_, dy_, dw, df1, df2, do1, do2, dpart, drho, derr};
+    for (void *p : bufs)
+        if (p) (void)hipFree(p);
+    if (e != hipSuccess) return abi_fail(GEMPIC_ERR_DEVICE, (std::string("2D coupling: ") + hipGetErrorString(e)).c_str());
+    if (herr & E_INEXACT)
+        return abi_fail(GEMPIC_ERR_INEXACT, "InexactError: ceil(Int, x) of a non-finite or out-of-range position");
+    return GEMPIC_OK;
+}
+
+int gempic_pmc2d_add_charge(int64_t nx, int64_t ny, double xmin, double xmax, double ymin,
+                            double ymax, int degree, int smoothing, int64_t n, const double *x,
+                            const double *y, const double *wp, double *rho)
+{
+    if ((n > 0 && (!x || !y || !wp)) || !rho) return abi_fail(GEMPIC_ERR_ARGUMENT, "NULL array");
+    return pmc2_run(0, nx, ny, xmin, xmax, ymin, ymax, degree, smoothing, n, x, y, wp, nullptr,
+                    nullptr, nullptr, nullptr, rho);
+}
+
+int gempic_pmc2d_evaluate(int64_t nx, int64_t ny, double xmin, double xmax, double ymin,
+                          double ymax, int degree, int smoothing, int64_t n, const double *x,
+                          const double *y, const double *field, double *out)
+{
+    if ((n > 0 && (!x || !y || !out)) || !field) return abi_fail(GEMPIC_ERR_ARGUMENT, "NULL array");
+    return pmc2_run(1, nx, ny, xmin, xmax, ymin, ymax, degree, smoothing, n, x, y, nullptr, field,
+                    nullptr, out, nullptr, nullptr);
+}
+
+int gempic_pmc2d_evaluate_multiple(int64_t nx, int64_t ny, double xmin, double xmax, double ymin,
+                                   double ymax, int degree, int smoothing, int64_t n,
+                                   const double *x, const double *y, const double *field1,
+                                   const double *field2, double *out1, double *out2)
+{
+    if ((n > 0 && (!x || !y || !out1 || !out2)) || !field1 || !field2)
+        return abi_fail(GEMPIC_ERR_ARGUMENT, "NULL array");
+    return pmc2_run(2, nx, ny, xmin, xmax, ymin, ymax, degree, smoothing, n, x, y, nullptr,
+                    field1, field2, out1, out2, nullptr);
+}
+
+}  // extern "C"

@@ -76,6 +76,21 @@ SIGNATURES = {
     "gempic_maxwell1d_compute_b_from_e": (_I, [_VP, _P, _D, _P]),
     "gempic_maxwell1d_inner_product": (_I, [_VP, _P, _P, _I, _P]),
     "gempic_maxwell1d_l2norm_squared": (_I, [_VP, _P, _I, _P]),
+    "gempic_pmc2d_add_charge": (_I, [_I64, _I64, _D, _D, _D, _D, _I, _I, _I64, _P, _P, _P, _P]),
+    "gempic_pmc2d_evaluate": (_I, [_I64, _I64, _D, _D, _D, _D, _I, _I, _I64, _P, _P, _P, _P]),
+    "gempic_pmc2d_evaluate_multiple": (_I, [_I64, _I64, _D, _D, _D, _D, _I, _I, _I64, _P, _P,
+                                            _P, _P, _P, _P]),
+    "gempic_maxwell2d_create": (_I, [_I64, _D, _D, _I64, _D, _D, _I, _PP]),
+    "gempic_maxwell2d_destroy": (_I, [_VP]),
+    "gempic_maxwell2d_tables": (_I, [_VP, _P, _P, _P, _P]),
+    "gempic_maxwell2d_solve": (_I, [_VP, _I, _I, _P, _P]),
+    "gempic_maxwell2d_multiply_mass": (_I, [_VP, _I, _I, _P, _P]),
+    "gempic_maxwell2d_inner_product": (_I, [_VP, _P, _P, _I, _I, _P]),
+    "gempic_maxwell2d_compute_e_from_b": (_I, [_VP, _D, _P, _P, _P, _P, _P, _P]),
+    "gempic_maxwell2d_compute_b_from_e": (_I, [_VP, _D, _P, _P, _P, _P, _P, _P]),
+    "gempic_maxwell2d_compute_e_from_j": (_I, [_VP, _P, _I, _P]),
+    "gempic_maxwell2d_compute_rho_from_e": (_I, [_VP, _P, _P, _P, _P]),
+    "gempic_maxwell2d_compute_e_from_rho": (_I, [_VP, _P, _P, _P]),
 }
 
 

@@ -108,9 +108,13 @@ def _arr(x):
     return np.ascontiguousarray(np.atleast_1d(np.asarray(x, dtype=np.float64)))
 
 
-def add_charge_(rho: np.ndarray, p: ParticleMeshCoupling1D, position, marker_charge):
+def add_charge_(rho: np.ndarray, p: ParticleMeshCoupling1D, position, marker_charge, *wp):
     """add_charge! (particle_mesh_coupling_1d.jl:261-280), batched over arrays
-    of positions/charges (a scalar pair is one particle)."""
+    of positions/charges (a scalar pair is one particle). With a
+    ParticleMeshCoupling2D: add_charge_(rho, pm, xp, yp, wp)
+    (particle_mesh_coupling_2d.jl:94-105)."""
+    if isinstance(p, ParticleMeshCoupling2D):
+        return add_charge_2d_(rho, p, position, marker_charge, *wp)
     x = _arr(position)
     q = _arr(marker_charge) * np.ones_like(x)
     check(_lib.load().gempic_pmc1d_add_charge(*p._args(), x.size, ptr(x), ptr(q), ptr(rho)))
@@ -123,9 +127,12 @@ def add_charge_pp_(rho: np.ndarray, p: ParticleMeshCoupling1D, position, marker_
     check(_lib.load().gempic_pmc1d_add_charge_pp(*p._args(), x.size, ptr(x), ptr(q), ptr(rho)))
 
 
-def evaluate(p: ParticleMeshCoupling1D, position, field_dofs: np.ndarray):
+def evaluate(p: ParticleMeshCoupling1D, position, field_dofs: np.ndarray, *field2d):
     """evaluate (particle_mesh_coupling_1d.jl:438-453); array positions give
-    the batched form of diagnostics.jl:260-268."""
+    the batched form of diagnostics.jl:260-268. With a ParticleMeshCoupling2D:
+    evaluate(pm, xp, yp, field_dofs) (particle_mesh_coupling_2d.jl:189-203)."""
+    if isinstance(p, ParticleMeshCoupling2D):
+        return evaluate_2d(p, position, field_dofs, *field2d)
     x = _arr(position)
     out = np.zeros_like(x)
     f = np.ascontiguousarray(field_dofs, dtype=np.float64)
@@ -214,30 +221,46 @@ class Maxwell1DFEM:
 
 
 def compute_e_from_rho_(e, self: Maxwell1DFEM, rho):
-    """compute_e_from_rho! (maxwell_1d_fem.jl:244-255)."""
+    """compute_e_from_rho! (maxwell_1d_fem.jl:244-255); with a TwoDMaxwell,
+    e is the list of field components (maxwell_2d_fem.jl:199-201)."""
+    if isinstance(self, TwoDMaxwell):
+        return compute_e_from_rho_2d_(e, self, rho)
     check(_lib.load().gempic_maxwell1d_compute_e_from_rho(self._h, ptr(e), ptr(_arr(rho))))
 
 
 def compute_e_from_j_(e, self: Maxwell1DFEM, current, component: int):
-    """compute_e_from_j! (maxwell_1d_fem.jl:263-289)."""
+    """compute_e_from_j! (maxwell_1d_fem.jl:263-289; TwoDMaxwell:
+    maxwell_2d_fem.jl:455-459)."""
+    if isinstance(self, TwoDMaxwell):
+        return compute_e_from_j_2d_(e, self, current, component)
     check(_lib.load().gempic_maxwell1d_compute_e_from_j(self._h, ptr(e), ptr(_arr(current)),
                                                         int(component)))
 
 
 def compute_e_from_b_(field_out, self: Maxwell1DFEM, delta_t: float, field_in):
-    """compute_e_from_b! (maxwell_1d_fem.jl:384-396)."""
+    """compute_e_from_b! (maxwell_1d_fem.jl:384-396); with a TwoDMaxwell,
+    compute_e_from_b_(e, solver, dt, b) on three components each
+    (maxwell_2d_fem.jl:370-412)."""
+    if isinstance(self, TwoDMaxwell):
+        return compute_e_from_b_2d_(field_out, self, delta_t, field_in)
     check(_lib.load().gempic_maxwell1d_compute_e_from_b(self._h, ptr(field_out), float(delta_t),
                                                         ptr(_arr(field_in))))
 
 
 def compute_b_from_e_(field_out, self: Maxwell1DFEM, delta_t: float, field_in):
-    """compute_b_from_e! (maxwell_1d_fem.jl:407-420)."""
+    """compute_b_from_e! (maxwell_1d_fem.jl:407-420); with a TwoDMaxwell,
+    compute_b_from_e_(b, solver, dt, e) (maxwell_2d_fem.jl:423-444)."""
+    if isinstance(self, TwoDMaxwell):
+        return compute_b_from_e_2d_(field_out, self, delta_t, field_in)
     check(_lib.load().gempic_maxwell1d_compute_b_from_e(self._h, ptr(field_out), float(delta_t),
                                                         ptr(_arr(field_in))))
 
 
-def inner_product(self: Maxwell1DFEM, coefs1_dofs, coefs2_dofs, degree: int) -> float:
-    """inner_product (maxwell_1d_fem.jl:461-475)."""
+def inner_product(self: Maxwell1DFEM, coefs1_dofs, coefs2_dofs, degree: int, *form) -> float:
+    """inner_product (maxwell_1d_fem.jl:461-475); with a TwoDMaxwell,
+    inner_product(solver, c1, c2, component, form) (maxwell_2d_fem.jl:511-575)."""
+    if isinstance(self, TwoDMaxwell):
+        return inner_product_2d(self, coefs1_dofs, coefs2_dofs, degree, *form)
     out = np.zeros(1)
     check(_lib.load().gempic_maxwell1d_inner_product(self._h, ptr(_arr(coefs1_dofs)),
                                                      ptr(_arr(coefs2_dofs)), int(degree),
@@ -465,3 +488,13 @@ def write_step_(thdiag: TimeHistoryDiagnostics, time: float, degree: int, efield
                                         ptr(ep), ptr(out)))
     thdiag.data.append(out)
     return out
+
+
+# 2D coupling and field solver (defined in core2d, dispatched from the verbs above)
+from .core2d import (  # noqa: E402
+    ParticleMeshCoupling2D, TwoDMaxwell, add_charge_2d_, evaluate_2d, compute_e_from_rho_2d_,
+    compute_e_from_j_2d_, compute_e_from_b_2d_, compute_b_from_e_2d_, inner_product_2d)
+from .core2d import __all__ as _all2d  # noqa: E402
+from .core2d import *  # noqa: E402,F401,F403
+
+__all__ = __all__ + list(_all2d)

@@ -44,6 +44,7 @@ extern "C" {
 
 typedef struct gempic_splitting gempic_splitting;
 typedef struct gempic_maxwell1d gempic_maxwell1d;
+typedef struct gempic_maxwell2d gempic_maxwell2d;
 
 /* Library version (major*10000 + minor*100 + patch) and last error message
  * of the calling thread (valid until the next call on that thread). */
@@ -208,6 +209,69 @@ int gempic_maxwell1d_inner_product(gempic_maxwell1d *m, const double *c1, const 
                                    int degree, double *out);
 int gempic_maxwell1d_l2norm_squared(gempic_maxwell1d *m, const double *c, int degree,
                                     double *out);
+
+/* ------------------------------------------------------------------------
+ * ParticleMeshCoupling2D batched verbs (particle_mesh_coupling_2d.jl) over
+ * host arrays of n particles. nx/ny/bounds/degree/smoothing as in the
+ * constructor (:22-45) on TwoDGrid (mesh.jl:31-43). Grid arrays hold nx*ny
+ * doubles, x fastest (node (i, j) at i + j*nx). 1 <= degree <= 3,
+ * degree+1 <= nx, ny <= 256.
+ * ------------------------------------------------------------------------ */
+/* add_charge! (:94-105): rho[nx*ny] += sum_i deposit(x[i], y[i], wp[i]);
+ * the device deposit holds nx*ny <= 4096 */
+int gempic_pmc2d_add_charge(int64_t nx, int64_t ny, double xmin, double xmax, double ymin,
+                            double ymax, int degree, int smoothing, int64_t n, const double *x,
+                            const double *y, const double *wp, double *rho);
+/* evaluate (:189-203): out[i] = field(x[i], y[i]) */
+int gempic_pmc2d_evaluate(int64_t nx, int64_t ny, double xmin, double xmax, double ymin,
+                          double ymax, int degree, int smoothing, int64_t n, const double *x,
+                          const double *y, const double *field, double *out);
+/* evaluate_multiple (:214-231): two fields at once */
+int gempic_pmc2d_evaluate_multiple(int64_t nx, int64_t ny, double xmin, double xmax, double ymin,
+                                   double ymax, int degree, int smoothing, int64_t n,
+                                   const double *x, const double *y, const double *field1,
+                                   const double *field2, double *out1, double *out2);
+
+/* ------------------------------------------------------------------------
+ * TwoDMaxwell (maxwell_2d_fem.jl:11-88) with its TwoDPoisson
+ * (poisson_2d_fem.jl:3-85) and TwoDLinearSolverSplineMass
+ * (linear_solver_spline_mass_2d.jl) on host arrays of nx*ny doubles, run on
+ * the GPU. 1 <= degree <= 3, 2*degree+1 <= nx, ny <= 256.
+ * ------------------------------------------------------------------------ */
+int gempic_maxwell2d_create(int64_t nx, double xmin, double xmax, int64_t ny, double ymin,
+                            double ymax, int degree, gempic_maxwell2d **out);
+int gempic_maxwell2d_destroy(gempic_maxwell2d *m);
+/* eig_values_mass_{0,1}_{1,2} (:51-54); any may be NULL */
+int gempic_maxwell2d_tables(gempic_maxwell2d *m, double *eig0x, double *eig0y, double *eig1x,
+                            double *eig1y);
+/* solve(inv_mass_1[component] or inv_mass_2[component], rhs) (:56-66,
+ * linear_solver_spline_mass_2d.jl:16-32); form 1 or 2 */
+int gempic_maxwell2d_solve(gempic_maxwell2d *m, int form, int component, const double *rhs,
+                           double *out);
+/* multiply_mass_2dkron! (:343-363) with the mass lines of inner_product's
+ * (form, component) (:511-575) */
+int gempic_maxwell2d_multiply_mass(gempic_maxwell2d *m, int form, int component, const double *in,
+                                   double *out);
+/* inner_product (:511-575) */
+int gempic_maxwell2d_inner_product(gempic_maxwell2d *m, const double *c1, const double *c2,
+                                   int component, int form, double *out);
+/* compute_e_from_b! (:370-412): e1..e3 updated in place */
+int gempic_maxwell2d_compute_e_from_b(gempic_maxwell2d *m, double dt, const double *b1,
+                                      const double *b2, const double *b3, double *e1, double *e2,
+                                      double *e3);
+/* compute_b_from_e! (:423-444): b1..b3 updated in place */
+int gempic_maxwell2d_compute_b_from_e(gempic_maxwell2d *m, double dt, const double *e1,
+                                      const double *e2, const double *e3, double *b1, double *b2,
+                                      double *b3);
+/* compute_e_from_j! (:455-459), component 1..3 */
+int gempic_maxwell2d_compute_e_from_j(gempic_maxwell2d *m, const double *current, int component,
+                                      double *e);
+/* compute_rho_from_e! (:468-499); e3 is not read and may be NULL */
+int gempic_maxwell2d_compute_rho_from_e(gempic_maxwell2d *m, const double *e1, const double *e2,
+                                        const double *e3, double *rho);
+/* compute_e_from_rho! (:199-201 -> poisson_2d_fem.jl:237-262) */
+int gempic_maxwell2d_compute_e_from_rho(gempic_maxwell2d *m, const double *rho, double *e1,
+                                        double *e2);
 
 #ifdef __cplusplus
 }

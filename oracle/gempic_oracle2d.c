@@ -29,7 +29,7 @@ static int64_t imod(int64_t a, int64_t n)
 /* particle_mesh_coupling_2d.jl                                              */
 /* ------------------------------------------------------------------------ */
 
-/* ParticleMeshCoupling2D constructor (particle_mesh_coupling_2d.jl:22-47). */
+/* ParticleMeshCoupling2D constructor (particle_mesh_coupling_2d.jl:22-45). */
 int or_pmc2d_init(or_pmc2d *p, double xmin, double xmax, int64_t nx, double ymin, double ymax,
                   int64_t ny, int degree, int smoothing)
 {
@@ -50,7 +50,7 @@ int or_pmc2d_init(or_pmc2d *p, double xmin, double xmax, int64_t nx, double ymin
     return 0;
 }
 
-/* compute_shape_factor (:57-69): ceil() cell index, offset in (0, 1]; the
+/* compute_shape_factor (:54-65): ceil() cell index, offset in (0, 1]; the
  * 2D uniform_bsplines_eval_basis! (low_level_bsplines.jl:82-110) runs the
  * x and y recurrences with the same operations as the 1D one. */
 static void shape_factor(const or_pmc2d *p, double xp, double yp, int64_t *ix, int64_t *iy,
@@ -68,7 +68,7 @@ static void shape_factor(const or_pmc2d *p, double xp, double yp, int64_t *ix, i
     *iy = jp - p->degree;
 }
 
-/* index_1dto2d_column_major (:80-86), 0-based result */
+/* index_1dto2d_column_major (:76-82), 0-based result */
 static int64_t idx2d(const or_pmc2d *p, int64_t i1, int64_t i2)
 {
     return imod(i1, p->nx) + imod(i2, p->ny) * p->nx;
@@ -97,7 +97,7 @@ static double horner_col(int degree, const double *pp, int64_t ld, double x, int
     return res;
 }
 
-/* add_charge_pp! 2D (:125-144): floor() + 1 cell index, horner_m_2d! weights
+/* add_charge_pp! 2D (:127-148): floor() + 1 cell index, horner_m_2d! weights
  * (splinepp.jl:295-300) */
 void or_pmc2d_add_charge_pp(double *rho, const or_pmc2d *p, double x, double y, double wp)
 {
@@ -200,7 +200,7 @@ void or_pmc2d_b_to_pp(const or_pmc2d *p, const double *b, double *pp)
         }
 }
 
-/* evaluate_pp 2D (:160-180) with horner_2d (splinepp.jl:412-434); indices
+/* evaluate_pp 2D (:162-178) with horner_2d (splinepp.jl:412-434); indices
  * floor()+1 are not wrapped (the reference indexes the pp matrix directly).
  * Returns NAN and sets *bounds_error for a position outside the table. */
 double or_pmc2d_evaluate_pp(const or_pmc2d *p, double x, double y, const double *pp,
@@ -236,7 +236,7 @@ double or_pmc2d_evaluate_pp(const or_pmc2d *p, double x, double y, const double 
 /* poisson_2d_fem.jl helpers                                                 */
 /* ------------------------------------------------------------------------ */
 
-/* spline_fem_mass_line (poisson_2d_fem.jl:87-108) */
+/* spline_fem_mass_line (poisson_2d_fem.jl:87-106) */
 void or_spline_fem_mass_line(int degree, double *mass_line)
 {
     const int n = degree + 1;
@@ -257,7 +257,7 @@ void or_spline_fem_mass_line(int degree, double *mass_line)
                                     val[(i - j) + (k - 1) * (degree + 1)] * w[k - 1];
 }
 
-/* spline_fem_compute_mass_eig (poisson_2d_fem.jl:115-128) */
+/* spline_fem_compute_mass_eig (poisson_2d_fem.jl:113-126) */
 void or_spline_fem_compute_mass_eig(int64_t n_cells, int degree, const double *mass_line,
                                     double *eig)
 {
@@ -503,7 +503,7 @@ void or_maxwell2d_solve(or_maxwell2d *m, int form, int component, const double *
     inv_mass(m, form, component, rhs, out);
 }
 
-/* spline_fem_multiply_mass (maxwell_2d_fem.jl:285-317): banded circulant
+/* spline_fem_multiply_mass (maxwell_2d_fem.jl:300-336): banded circulant
  * product, out[r] = m[0] in[r] + sum_c m[c] (in[r+c] + in[r-c]), c ascending */
 static void multiply_mass_1d(int64_t n, int degree, const double *mass, const double *in,
                              int64_t stride, double *out)
@@ -516,7 +516,7 @@ static void multiply_mass_1d(int64_t n, int degree, const double *mass, const do
     }
 }
 
-/* multiply_mass_2dkron! (:324-342): along x for each column, then along y */
+/* multiply_mass_2dkron! (:343-363): along x for each column, then along y */
 void or_maxwell2d_multiply_mass(or_maxwell2d *m, const double *line1, int deg1,
                                 const double *line2, int deg2, const double *in, double *out)
 {
@@ -538,7 +538,7 @@ void or_maxwell2d_multiply_mass(or_maxwell2d *m, const double *line1, int deg1,
 static void mass_lines_for(or_maxwell2d *m, int form, int component, const double **l1, int *d1,
                            const double **l2, int *d2)
 {
-    /* inner_product's table (:532-575): 0-form (0,0); 1-form c1 (1,0), c2 (0,1),
+    /* inner_product's table (:511-575): 0-form (0,0); 1-form c1 (1,0), c2 (0,1),
      * c3 (0,0); 2-form c1 (0,1), c2 (1,0), c3 (1,1); 3-form (1,1) */
     int a = 0, b = 0;
     if (form == 1) { a = component == 1; b = component == 2; }
@@ -550,7 +550,7 @@ static void mass_lines_for(or_maxwell2d *m, int form, int component, const doubl
     *d2 = b ? m->s_deg_1 : m->s_deg_0;
 }
 
-/* inner_product (:532-575): sum(c1 .* M c2), summed left to right */
+/* inner_product (:511-575): sum(c1 .* M c2), summed left to right */
 double or_maxwell2d_inner_product(or_maxwell2d *m, const double *c1, const double *c2,
                                   int component, int form)
 {
@@ -566,7 +566,7 @@ double or_maxwell2d_inner_product(or_maxwell2d *m, const double *c1, const doubl
     return r;
 }
 
-/* compute_e_from_b! 2D (:349-395); e, b: 3 arrays of nx*ny */
+/* compute_e_from_b! 2D (:370-412); e, b: 3 arrays of nx*ny */
 void or_maxwell2d_compute_e_from_b(or_maxwell2d *m, double dt, double *const *b, double *const *e)
 {
     const int64_t nx = m->nx, ny = m->ny, nn = nx * ny;
@@ -597,7 +597,7 @@ void or_maxwell2d_compute_e_from_b(or_maxwell2d *m, double dt, double *const *b,
     }
 }
 
-/* compute_b_from_e! 2D (:405-433) */
+/* compute_b_from_e! 2D (:423-444) */
 void or_maxwell2d_compute_b_from_e(or_maxwell2d *m, double dt, double *const *e, double *const *b)
 {
     const int64_t nx = m->nx, ny = m->ny;
@@ -612,7 +612,7 @@ void or_maxwell2d_compute_b_from_e(or_maxwell2d *m, double dt, double *const *e,
         }
 }
 
-/* compute_e_from_j! 2D (:445-449): e -= M1^{-1} current (no dx factor) */
+/* compute_e_from_j! 2D (:455-459): e -= M1^{-1} current (no dx factor) */
 void or_maxwell2d_compute_e_from_j(or_maxwell2d *m, const double *current, int component, double *e)
 {
     const int64_t nn = m->nx * m->ny;
@@ -622,7 +622,7 @@ void or_maxwell2d_compute_e_from_j(or_maxwell2d *m, const double *current, int c
     free(w);
 }
 
-/* compute_rho_from_e! (:456-488): rho = -(G^T M1 e) */
+/* compute_rho_from_e! (:468-499): rho = -(G^T M1 e) */
 void or_maxwell2d_compute_rho_from_e(or_maxwell2d *m, double *const *e, double *rho)
 {
     const int64_t nx = m->nx, ny = m->ny, nn = nx * ny;
@@ -690,7 +690,7 @@ void or_maxwell2d_compute_e_from_rho(or_maxwell2d *m, const double *rho, double 
     free(sr); free(si); free(xr); free(xi); free(yr); free(yi);
 }
 
-/* compute_fem_rhs! (:227-283) / compute_rhs_from_function (:112-180) for a
+/* compute_fem_rhs! (:212-282) / compute_rhs_from_function (:123-197) for a
  * callback f(x, y, ctx); form/component pick the degrees as the reference. */
 void or_maxwell2d_fem_rhs(or_maxwell2d *m, double (*f)(double, double, void *), void *ctx,
                           int component, int form, double *coefs)
@@ -737,7 +737,7 @@ void or_maxwell2d_fem_rhs(or_maxwell2d *m, double (*f)(double, double, void *), 
         }
 }
 
-/* l2projection (:291-303): the FEM rhs, then the inverse mass of the form
+/* l2projection (:288-298): the FEM rhs, then the inverse mass of the form
  * (forms 1 and 2; other forms return the rhs) */
 void or_maxwell2d_l2projection(or_maxwell2d *m, double (*f)(double, double, void *), void *ctx,
                                int component, int form, double *out)

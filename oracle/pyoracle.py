@@ -361,7 +361,7 @@ class _Pmc2dStruct(ctypes.Structure):
 
 
 class PMC2D:
-    """ParticleMeshCoupling2D (particle_mesh_coupling_2d.jl:12-47) on a
+    """ParticleMeshCoupling2D (particle_mesh_coupling_2d.jl:12-45) on a
     TwoDGrid (mesh.jl:17-42). Grid arrays are nx*ny, x fastest."""
 
     def __init__(self, xmin, xmax, nx, ymin, ymax, ny, degree, smoothing=GALERKIN):
@@ -421,7 +421,7 @@ def _ptrs3(arrs):
 
 
 class Maxwell2D:
-    """TwoDMaxwell (maxwell_2d_fem.jl:11-87) with its TwoDPoisson and
+    """TwoDMaxwell (maxwell_2d_fem.jl:11-88) with its TwoDPoisson and
     TwoDLinearSolverSplineMass solvers."""
 
     def __init__(self, xmin, xmax, nx, ymin, ymax, ny, degree):

@@ -92,7 +92,7 @@ def test_pmc2d_partition_of_unity_and_total_charge():
 # Mass lines, eigenvalues, Kronecker solves
 
 def test_mass_line_is_the_spline_autocorrelation():
-    """spline_fem_mass_line (poisson_2d_fem.jl:87-108): int N_0 N_j over the
+    """spline_fem_mass_line (poisson_2d_fem.jl:87-106): int N_0 N_j over the
     line, the Eulerian-number lines of Maxwell1DFEM (maxwell_1d_fem.jl:78-88)."""
     np.testing.assert_allclose(orc.spline_fem_mass_line(3),
                                np.array([2416, 1191, 120, 1]) / 5040, rtol=1e-14)
