@@ -62,6 +62,94 @@ def cpu_baseline(n, steps, seed=1):
     return n * steps / dt, dt
 
 
+def pieces_2d(n, reps, cpu_n):
+    """The 2D3V configuration's particle-mesh pieces on one GPU: 64 x 64 grid
+    on [0, 4 pi)^2, degree-3 tensor splines, n particles sampled from the
+    density 1 + 0.01 cos(0.5 x) cos(0.5 y) (seeded, rejection sampling), kept
+    in GPU memory. Times the device-resident add_charge! (fast and
+    deterministic, particles in sampled order and sorted by cell), evaluate
+    and evaluate_multiple with HIP events on the stream the kernels run on.
+    Algorithmic bytes per particle: deposit 24 (x, y, wp read), evaluate 24
+    (x, y read, value written), evaluate_multiple 32."""
+    import gempic_amd as G
+    from gempic_amd.device import DeviceArray, Event, Stream
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    nx = ny = 64
+    L = 4.0 * np.pi
+    rng = np.random.default_rng(3)
+    xs, ys, have = [], [], 0
+    while have < n:
+        m = int((n - have) * 1.02) + 1024
+        x, y, u = L * rng.random(m), L * rng.random(m), rng.random(m)
+        keep = u * 1.01 < 1.0 + 0.01 * np.cos(0.5 * x) * np.cos(0.5 * y)
+        xs.append(x[keep])
+        ys.append(y[keep])
+        have += int(keep.sum())
+    x = np.concatenate(xs)[:n]
+    y = np.concatenate(ys)[:n]
+    w = np.full(n, 1.0 / n)
+    cell = np.floor(x / (L / nx)).astype(np.int64) + nx * np.floor(y / (L / ny)).astype(np.int64)
+    order = np.argsort(cell, kind="stable")
+    D = DeviceArray.from_numpy
+    dx_, dy_, dw = D(x), D(y), D(w)
+    dxs, dys = D(x[order]), D(y[order])
+    grid = G.TwoDGrid(0.0, L, nx, 0.0, L, ny)
+    pm = G.ParticleMeshCoupling2D(G.ParticleGroup(2, 2, n), grid, 3, "galerkin")
+    rho = DeviceArray.zeros(nx * ny)
+    f1, f2 = D(rng.normal(size=nx * ny)), D(rng.normal(size=nx * ny))
+    o1, o2 = DeviceArray(n), DeviceArray(n)
+    st = Stream()
+
+    def timed(fn, nbytes):
+        for _ in range(3):
+            fn()
+        G.pmc2d_sync(st)
+        e0, e1 = Event(), Event()
+        e0.record(st)
+        for _ in range(reps):
+            fn()
+        e1.record(st)
+        G.pmc2d_sync(st)
+        us = 1e3 * e0.elapsed_ms(e1) / reps
+        gbs = nbytes * n / (us * 1e-6) / 1e9
+        return {"us_per_call": us, "particles_per_s": n / (us * 1e-6), "achieved_GBs": gbs,
+                "roofline_frac": gbs / HBM_PEAK_GBS, "bytes_per_particle": nbytes}
+
+    out = {"workload": f"2D3V Landau pieces: {n} particles, 64x64 grid on [0,4pi)^2, degree 3",
+           "bound": "hbm", "peak_GBs": HBM_PEAK_GBS}
+    kw = dict(async_=True, stream=st)
+    out["add_charge_fast"] = timed(lambda: G.add_charge_device_(rho, pm, dx_, dy_, dw, **kw), 24)
+    out["add_charge_fast_sorted"] = timed(
+        lambda: G.add_charge_device_(rho, pm, dxs, dys, dw, **kw), 24)
+    out["add_charge_deterministic"] = timed(
+        lambda: G.add_charge_device_(rho, pm, dx_, dy_, dw, deterministic=True, **kw), 24)
+    out["add_charge_deterministic_sorted"] = timed(
+        lambda: G.add_charge_device_(rho, pm, dxs, dys, dw, deterministic=True, **kw), 24)
+    out["evaluate"] = timed(lambda: G.evaluate_device(pm, dx_, dy_, f1, o1, **kw), 24)
+    out["evaluate_sorted"] = timed(lambda: G.evaluate_device(pm, dxs, dys, f1, o1, **kw), 24)
+    out["evaluate_multiple"] = timed(
+        lambda: G.evaluate_multiple_device(pm, (dx_, dy_), (f1, f2), (o1, o2), **kw), 32)
+    # TwoDMaxwell at 64 x 64 through the host-array verbs (copies included)
+    m = G.TwoDMaxwell(grid, 3)
+    e = [np.random.default_rng(1).normal(size=nx * ny) for _ in range(3)]
+    b = [np.random.default_rng(2).normal(size=nx * ny) for _ in range(3)]
+    t0 = time.perf_counter()
+    for _ in range(20):
+        G.compute_e_from_b_(e, m, 0.05, b)
+    out["maxwell2d_compute_e_from_b_host_call_us"] = 1e6 * (time.perf_counter() - t0) / 20
+    if cpu_n:
+        import pyoracle as orc
+        p = orc.PMC2D(0.0, L, nx, 0.0, L, ny, 3, orc.GALERKIN)
+        r = np.zeros(nx * ny)
+        t0 = time.perf_counter()
+        p.add_charge_n(r, x[:cpu_n], y[:cpu_n], w[:cpu_n])
+        secs = time.perf_counter() - t0
+        out["cpu_baseline_add_charge"] = {
+            "value": cpu_n / secs, "unit": "particles/s", "cores": 1, "kind": "port",
+            "sample": f"{cpu_n} particles of the same sample ({secs:.2f} s), C restatement, one thread"}
+    return out
+
+
 def copy_calibration(mib):
     """Run a plain device copy of `mib` MiB five times (reads and writes mib MiB each).
 
@@ -93,6 +181,12 @@ def main():
     ap.add_argument("--substep", action="store_true", help="also time the sub-step path")
     ap.add_argument("--blocks", type=int, default=0, help="workgroups per particle pass (0 = auto)")
     ap.add_argument("--fusion", type=int, default=1, help="particle kernels per fused step (1 or 2)")
+    ap.add_argument("--particles-2d", type=int, default=16777216,
+                    help="particles of the 2D pieces measurement (0 = skip it)")
+    ap.add_argument("--reps-2d", type=int, default=20)
+    ap.add_argument("--only-2d", action="store_true",
+                    help="print only the 2D pieces measurement (for profiling its kernels)")
+    ap.add_argument("--cpu-particles-2d", type=int, default=4194304)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -104,6 +198,11 @@ def main():
               "(particles would shard with one all-reduce of the nx-sized deposits per "
               "sub-step; see the design notes). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+
+    if args.only_2d:
+        print(json.dumps(pieces_2d(args.particles_2d, args.reps_2d,
+                                   0 if args.no_cpu else args.cpu_particles_2d)))
+        return
 
     import gempic_amd as G
     from gempic_amd import workloads
@@ -169,6 +268,11 @@ def main():
                "sample": f"1D2V Weibel, {args.cpu_particles} particles x {args.cpu_steps} Strang "
                          f"steps ({secs:.1f} s), C restatement of the reference, one thread"}
 
+    p2d = None
+    if args.particles_2d:
+        print(f"[bench] 2D pieces, {args.particles_2d} particles", file=sys.stderr, flush=True)
+        p2d = pieces_2d(args.particles_2d, args.reps_2d, 0 if args.no_cpu else args.cpu_particles_2d)
+
     line = {
         "metric": "particle-steps/s (1D2V, 4M particles, FP64) fwd & fwd+adjoint; % HBM roofline",
         "value": value,
@@ -193,6 +297,8 @@ def main():
         "cpu_baseline": cpu,
         "kernels": kernels,
     }
+    if p2d is not None:
+        line["pieces_2d"] = p2d
     if substep is not None:
         line["substep_path_value"] = substep
     print(json.dumps(line))

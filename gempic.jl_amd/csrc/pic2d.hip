@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -29,7 +30,24 @@ using namespace gk;
 
 namespace {
 
-constexpr int kGrid2Max = 4096;            // nx*ny of the device deposit (LDS grid per wave)
+constexpr int kGrid2Max = 4096;            // nx*ny of the device deposit (LDS grid)
+
+// launch shape of the evaluation kernels: workgroups per CU, and the largest
+// field (doubles, both fields of evaluate_multiple together) staged in LDS
+#ifndef GEMPIC_EVAL_BLOCKS_PER_CU
+#define GEMPIC_EVAL_BLOCKS_PER_CU 4
+#endif
+#ifndef GEMPIC_EVAL_STAGE_MAX
+#define GEMPIC_EVAL_STAGE_MAX 8192
+#endif
+// deposit: waves per workgroup (at most) and workgroups per CU (0: as many
+// as fit)
+#ifndef GEMPIC_DEP_WAVES
+#define GEMPIC_DEP_WAVES 4
+#endif
+#ifndef GEMPIC_DEP_BLOCKS_PER_CU
+#define GEMPIC_DEP_BLOCKS_PER_CU 0
+#endif
 
 // ---------------------------------------------------------------------------
 // device: ParticleMeshCoupling2D (particle_mesh_coupling_2d.jl)
@@ -65,62 +83,103 @@ __device__ __forceinline__ bool shape2(const Pmc2Args &a, double x, double y, in
     return ok;
 }
 
-template <int P>
-__global__ void __launch_bounds__(kWave) k_pmc2d_charge(Pmc2Args a)
+// Deposit (add_charge!, :94-105). Each lane keeps the (P+1)^2 node sums of
+// the patch its particles fall in, in registers; when its next particle
+// starts another patch, the cached sums are flushed into an LDS grid:
+//  * if every flushing lane of the wave holds the same patch (particles
+//    sorted by cell), the wave sums them (DPP, fixed order) and 16 lanes add
+//    the patch to the grid;
+//  * otherwise, in the fast mode (DET = false), each flushing lane adds its
+//    patch with LDS atomics into the workgroup's shared grid;
+//  * in the deterministic mode (DET = true) every wave owns its grid and the
+//    flushing lanes add in passes: per pass, the lowest lane of each hash
+//    slot of its patch adds its whole patch (distinct patches, so no two
+//    lanes of one instruction hit one node), in lane order of the slots.
+// In the deterministic mode every addition's order is a fixed function of
+// the particle order; the waves' grids are summed in wave order at the end.
+template <int P, bool DET>
+__global__ void __launch_bounds__(256) k_pmc2d_deposit(Pmc2Args a)
 {
     constexpr int W = P + 1, WW = W * W;
-    static_assert(WW <= kWave, "one node per lane");
+    static_assert(WW <= 16, "one node per lane of a 16-lane row");
     extern __shared__ double smem[];
     const int nn = a.nx * a.ny;
-    double *grid = smem;                             // [nx*ny] this wave's grid
-    double *stage = grid + nn;                       // [64][WW] evicted patches
-    int *skey = (int *)(stage + kWave * WW);         // [64]
-    const int lane = threadIdx.x;
-    for (int i = lane; i < nn; i += kWave) grid[i] = 0.0;
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const int nw = blockDim.x / kWave;
+    const int ngrids = DET ? nw : 1;
+    double *grid = smem + (DET ? (size_t)wave * nn : 0);
+    double *stage = smem + (size_t)ngrids * nn + wave * 16;
+    int *hash = (int *)(smem + (size_t)ngrids * nn + nw * 16) + wave * 256;
+    for (int k = threadIdx.x; k < ngrids * nn; k += blockDim.x) smem[k] = 0.0;
     __syncthreads();
 
-    int ckey = -1;                                   // cached patch: first node sx + sy*nx
+    // node j (= ia * W + ib) of the patch starting at node `key`
+    auto node = [&](int key, int j) {
+        const int sy = key / a.nx, sx = key - sy * a.nx;
+        const int ia = j / W, ib = j - ia * W;
+        int gx = sx + ia, gy = sy + ib;
+        if (gx >= a.nx) gx -= a.nx;
+        if (gy >= a.ny) gy -= a.ny;
+        return gx + gy * a.nx;
+    };
+
+    int ckey = -1;
     double acc[WW];
 #pragma unroll
-    for (int k = 0; k < WW; ++k) acc[k] = 0.0;
+    for (int j = 0; j < WW; ++j) acc[j] = 0.0;
     int errs = 0;
 
-    // stage the evicting lanes' patches, then add them in lane order, node
-    // (a, b) of each patch by lane a*W + b (distinct nodes: nx, ny >= W)
     auto flush = [&](bool evict) {
         const unsigned long long m = __ballot(evict);
         if (!m) return;
-        if (evict) {
+        const int k0 = __shfl(ckey, __ffsll((long long)m) - 1, kWave);
+        if (__ballot(evict && ckey != k0) == 0) {
 #pragma unroll
-            for (int k = 0; k < WW; ++k) stage[lane * WW + k] = acc[k];
-            skey[lane] = ckey;
-        }
-        __syncthreads();
-        unsigned long long mm = m;
-        while (mm) {
-            const int l = __ffsll((long long)mm) - 1;
-            mm &= mm - 1;
-            if (lane < WW) {
-                const int key = skey[l];
-                const int sx = key % a.nx, sy = key / a.nx;
-                const int ia = lane / W, ib = lane - ia * W;
-                int gx = sx + ia, gy = sy + ib;
-                if (gx >= a.nx) gx -= a.nx;
-                if (gy >= a.ny) gy -= a.ny;
-                grid[gx + gy * a.nx] += stage[l * WW + lane];
+            for (int j = 0; j < WW; ++j) {
+                const double sj = wave_sum_lane63(evict ? acc[j] : 0.0);
+                if (lane == kWave - 1) stage[j] = sj;
             }
-            __syncthreads();
+            if (lane < WW) atomicAdd(&grid[node(k0, lane)], stage[lane]);
+        } else if constexpr (!DET) {
+            if (evict) {
+#pragma unroll
+                for (int j = 0; j < WW; ++j) atomicAdd(&grid[node(ckey, j)], acc[j]);
+            }
+        } else {
+            bool rem = evict;
+            while (__ballot(rem)) {
+                const int slot = ckey & 255;
+                if (rem) hash[slot] = 0x7fffffff;
+                if (rem) atomicMin(&hash[slot], lane);
+                const bool lead = rem && hash[slot] == lane;
+                if (lead) {
+#pragma unroll
+                    for (int j = 0; j < WW; ++j) atomicAdd(&grid[node(ckey, j)], acc[j]);
+                }
+                rem = rem && !lead;
+            }
         }
     };
 
-    const int64_t base = (int64_t)blockIdx.x * a.chunk;
+    const int64_t wbase = ((int64_t)blockIdx.x * nw + wave) * a.chunk;
+    // the next particle's inputs are loaded while the current one is deposited
+    double xn = a.xmin, yn = a.ymin, wn = 0.0;
+    if (wbase + lane < a.n) {
+        xn = a.x[wbase + lane];
+        yn = a.y[wbase + lane];
+        wn = a.w[wbase + lane];
+    }
     for (int64_t off = 0; off < a.chunk; off += kWave) {
-        const int64_t i = base + off + lane;
-        const bool valid = off + lane < a.chunk && i < a.n;
+        const int64_t i = wbase + off + lane;
+        const bool valid = i < a.n;
+        const double x = xn, y = yn, wp = wn;
+        if (off + kWave < a.chunk && i + kWave < a.n) {
+            xn = a.x[i + kWave];
+            yn = a.y[i + kWave];
+            wn = a.w[i + kWave];
+        }
         double vx[W], vy[W];
         int sx = 0, sy = 0;
-        const double x = valid ? a.x[i] : a.xmin, y = valid ? a.y[i] : a.ymin;
-        const double wp = valid ? a.w[i] : 0.0;
         const bool ok = shape2<P>(a, x, y, sx, sy, vx, vy);
         if (valid && !ok) errs |= E_INEXACT;
         const int key = sx + sy * a.nx;
@@ -129,63 +188,129 @@ __global__ void __launch_bounds__(kWave) k_pmc2d_charge(Pmc2Args a)
         if (miss) {
             ckey = key;
 #pragma unroll
-            for (int k = 0; k < WW; ++k) acc[k] = 0.0;
+            for (int j = 0; j < WW; ++j) acc[j] = 0.0;
         }
         if (valid) {
-            // add_charge! (:94-105): wp * scaling * Sx[a] * Sy[b]
+            // wp * scaling * Sx[a] * Sy[b]
             const double ws = wp * a.scaling;
 #pragma unroll
-            for (int ia = 0; ia < W; ++ia)
+            for (int ia = 0; ia < W; ++ia) {
+                const double wx = ws * vx[ia];
 #pragma unroll
-                for (int ib = 0; ib < W; ++ib) acc[ia * W + ib] += ws * vx[ia] * vy[ib];
+                for (int ib = 0; ib < W; ++ib) acc[ia * W + ib] += wx * vy[ib];
+            }
         }
     }
     flush(ckey >= 0);
     if (errs) atomicOr(a.err, errs);
     __syncthreads();
     double *out = a.part + (size_t)blockIdx.x * nn;
-    for (int k = lane; k < nn; k += kWave) out[k] = grid[k];
+    for (int k = threadIdx.x; k < nn; k += blockDim.x) {
+        double t = smem[k];
+        for (int w = 1; w < ngrids; ++w) t += smem[(size_t)w * nn + k];
+        out[k] = t;
+    }
 }
 
-// rho[k] += sum over workgroups b (in order) of part[b][k]
-__global__ void k_pmc2d_reduce(const double *part, int nblocks, int nn, double *rho)
+// part2[s][k] = sum of part[b][k] over the workgroups b of split s, in order
+__global__ void k_pmc2d_reduce1(const double *part, int nblocks, int nn, double *part2)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nn) return;
-    double s = 0.0;
-    for (int b = 0; b < nblocks; ++b) s += part[(size_t)b * nn + k];
-    rho[k] = rho[k] + s;
+    const int splits = gridDim.y, sp = blockIdx.y;
+    const int b0 = (int)((int64_t)sp * nblocks / splits), b1 = (int)((int64_t)(sp + 1) * nblocks / splits);
+    double t = 0.0;
+    for (int b = b0; b < b1; ++b) t += part[(size_t)b * nn + k];
+    part2[(size_t)sp * nn + k] = t;
 }
 
-// evaluate (:189-203) and evaluate_multiple (:214-231)
-template <int P, bool MULTI>
+// rho[k] += sum of part2[s][k] over the splits, in order
+__global__ void k_pmc2d_reduce2(const double *part2, int splits, int nn, double *rho)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nn) return;
+    double t = 0.0;
+    for (int sp = 0; sp < splits; ++sp) t += part2[(size_t)sp * nn + k];
+    rho[k] = rho[k] + t;
+}
+
+// evaluate (:189-203) and evaluate_multiple (:214-231). With STAGE the
+// field is copied into LDS first: evaluate's with a periodic halo of P
+// columns and rows ((nx+P) x (ny+P), so the 16 reads of a particle are
+// constant offsets from its first node), evaluate_multiple's two fields
+// interleaved (one 16-byte read per node). The next particle's position is
+// loaded while the current one is evaluated.
+template <int P, bool MULTI, bool STAGE>
 __global__ void __launch_bounds__(256) k_pmc2d_eval(Pmc2Args a)
 {
     constexpr int W = P + 1;
+    extern __shared__ __align__(16) double sfield[];
+    const int sw = STAGE && !MULTI ? a.nx + P : a.nx;       // row stride of the staged field
+    if constexpr (STAGE) {
+        if constexpr (MULTI) {
+            const int nn = a.nx * a.ny;
+            for (int k = threadIdx.x; k < nn; k += blockDim.x) {
+                sfield[2 * k] = a.f1[k];
+                sfield[2 * k + 1] = a.f2[k];
+            }
+        } else {
+            const int ns = sw * (a.ny + P);
+            for (int k = threadIdx.x; k < ns; k += blockDim.x) {
+                const int gy = k / sw, gx = k - gy * sw;
+                sfield[k] = a.f1[(gx < a.nx ? gx : gx - a.nx) + (gy < a.ny ? gy : gy - a.ny) * a.nx];
+            }
+        }
+        __syncthreads();
+    }
+    const double2 *f12 = (const double2 *)sfield;
     int errs = 0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n;
-         i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double xn = i < a.n ? a.x[i] : 0.0, yn = i < a.n ? a.y[i] : 0.0;
+    for (; i < a.n; i += stride) {
+        const double x = xn, y = yn;
+        if (i + stride < a.n) {
+            xn = a.x[i + stride];
+            yn = a.y[i + stride];
+        }
         double vx[W], vy[W];
         int sx, sy;
-        if (!shape2<P>(a, a.x[i], a.y[i], sx, sy, vx, vy)) errs |= E_INEXACT;
+        if (!shape2<P>(a, x, y, sx, sy, vx, vy)) errs |= E_INEXACT;
         double v1 = 0.0, v2 = 0.0;
-        int gx = sx;
+        if constexpr (STAGE && !MULTI) {
+            const double *f = sfield + sx + sy * sw;
 #pragma unroll
-        for (int ia = 0; ia < W; ++ia) {
-            int gy = sy;
+            for (int ia = 0; ia < W; ++ia)
 #pragma unroll
-            for (int ib = 0; ib < W; ++ib) {
-                const int k = gx + gy * a.nx;
-                if constexpr (MULTI) {
-                    const double c = vx[ia] * vy[ib];
-                    v1 += a.f1[k] * c;
-                    v2 += a.f2[k] * c;
-                } else {
-                    v1 += a.f1[k] * vx[ia] * vy[ib];
+                for (int ib = 0; ib < W; ++ib) v1 += f[ia + ib * sw] * vx[ia] * vy[ib];
+        } else {
+            int gx = sx;
+#pragma unroll
+            for (int ia = 0; ia < W; ++ia) {
+                int gy = sy;
+#pragma unroll
+                for (int ib = 0; ib < W; ++ib) {
+                    const int k = gx + gy * a.nx;
+                    if constexpr (MULTI) {
+                        const double c = vx[ia] * vy[ib];
+                        double g1, g2;
+                        if constexpr (STAGE) {
+                            const double2 g = f12[k];
+                            g1 = g.x;
+                            g2 = g.y;
+                        } else {
+                            g1 = a.f1[k];
+                            g2 = a.f2[k];
+                        }
+                        v1 += g1 * c;
+                        v2 += g2 * c;
+                    } else {
+                        v1 += a.f1[k] * vx[ia] * vy[ib];
+                    }
+                    gy = gy + 1 == a.ny ? 0 : gy + 1;
                 }
-                gy = gy + 1 == a.ny ? 0 : gy + 1;
+                gx = gx + 1 == a.nx ? 0 : gx + 1;
             }
-            gx = gx + 1 == a.nx ? 0 : gx + 1;
         }
         a.out1[i] = v1;
         if constexpr (MULTI) a.out2[i] = v2;
@@ -788,36 +913,31 @@ int gempic_maxwell2d_compute_e_from_rho(gempic_maxwell2d *m, const double *rho, 
 }
 
 // ---------------------------------------------------------------------------
-// batched ParticleMeshCoupling2D verbs
+// ParticleMeshCoupling2D verbs: device-resident forms and the batched host
+// forms built on them
 // ---------------------------------------------------------------------------
-static hipStream_t pmc2_stream()
-{
-    static thread_local hipStream_t s = nullptr;
-    if (!s) (void)hipStreamCreate(&s);
-    return s;
-}
+}  // extern "C"
 
-static int pmc2_run(int op, int64_t nx, int64_t ny, double xmin, double xmax, double ymin,
-                    double ymax, int degree, int smoothing, int64_t n, const double *x,
-                    const double *y, const double *w, const double *f1, const double *f2,
-                    double *out1, double *out2, double *rho)
+namespace {
+
+enum Pmc2Op { P2_CHARGE = 0, P2_EVAL, P2_EVAL2 };
+
+// validated geometry of a ParticleMeshCoupling2D (constructor :22-45 on
+// TwoDGrid, mesh.jl:31-43)
+int pmc2_geometry(int op, int64_t nx, int64_t ny, double xmin, double xmax, double ymin,
+                  double ymax, int degree, int smoothing, int64_t n, Pmc2Args &a)
 {
     if (degree < 1 || degree > 3)
         return abi_fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: 2D coupling degree must be 1, 2 or 3");
     if (nx < degree + 1 || ny < degree + 1 || nx > kNxMax || ny > kNxMax)
         return abi_fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: nx, ny must be in [degree+1, 256]");
-    if (op == 0 && nx * ny > kGrid2Max)
+    if (op == P2_CHARGE && nx * ny > kGrid2Max)
         return abi_fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: the device deposit holds nx*ny <= 4096");
     if (smoothing != GEMPIC_GALERKIN && smoothing != GEMPIC_COLLOCATION)
         return abi_fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: smoothing type not implemented");
     if (!(xmax > xmin) || !(ymax > ymin)) return abi_fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: empty domain");
     if (n < 0) return abi_fail(GEMPIC_ERR_ARGUMENT, "n must be >= 0");
-    int rc = abi_check_device();
-    if (rc) return rc;
-    if (n == 0) return GEMPIC_OK;
-    hipStream_t s = pmc2_stream();
-    const size_t nn = (size_t)nx * ny;
-    Pmc2Args a{};
+    a = Pmc2Args{};
     a.n = n;
     a.nx = (int)nx;
     a.ny = (int)ny;
@@ -826,71 +946,272 @@ static int pmc2_run(int op, int64_t nx, int64_t ny, double xmin, double xmax, do
     a.dx = (xmax - xmin) / (double)nx;
     a.dy = (ymax - ymin) / (double)ny;
     a.scaling = smoothing == GEMPIC_COLLOCATION ? 1.0 / (a.dx * a.dy) : 1.0;
-    double *dx_ = nullptr, *dy_ = nullptr, *dw = nullptr, *df1 = nullptr, *df2 = nullptr;
-    double *do1 = nullptr, *do2 = nullptr, *dpart = nullptr, *drho = nullptr;
-    int *derr = nullptr;
-    const int cus = abi_device_cus();
-    const int blocks = (int)std::min<int64_t>((int64_t)cus * 4, (n + 63) / 64);
-    a.chunk = ((n + blocks - 1) / blocks + 63) / 64 * 64;
-    const int nblk = (int)((n + a.chunk - 1) / a.chunk);
-    hipError_t e = hipSuccess;
-    auto ok = [&](hipError_t r) { if (r != hipSuccess && e == hipSuccess) e = r; };
-    ok(dalloc2(&dx_, n)); ok(dalloc2(&dy_, n)); ok(dalloc2(&derr, 1));
-    if (op == 0) { ok(dalloc2(&dw, n)); ok(dalloc2(&dpart, (size_t)nblk * nn)); ok(dalloc2(&drho, nn)); }
-    else { ok(dalloc2(&df1, nn)); ok(dalloc2(&do1, n)); if (op == 2) { ok(dalloc2(&df2, nn)); ok(dalloc2(&do2, n)); } }
-    if (e == hipSuccess) {
-        ok(hipMemcpyAsync(dx_, x, sizeof(double) * n, hipMemcpyHostToDevice, s));
-        ok(hipMemcpyAsync(dy_, y, sizeof(double) * n, hipMemcpyHostToDevice, s));
-        ok(hipMemsetAsync(derr, 0, sizeof(int), s));
-        a.x = dx_; a.y = dy_; a.err = derr;
-        if (op == 0) {
-            ok(hipMemcpyAsync(dw, w, sizeof(double) * n, hipMemcpyHostToDevice, s));
-            ok(hipMemcpyAsync(drho, rho, sizeof(double) * nn, hipMemcpyHostToDevice, s));
-            a.w = dw; a.part = dpart;
-            const size_t sh = sizeof(double) * (nn + (size_t)kWave * 16) + sizeof(int) * kWave;
-            if (degree == 1) hipLaunchKernelGGL(k_pmc2d_charge<1>, dim3(nblk), dim3(kWave), sh, s, a);
-            else if (degree == 2) hipLaunchKernelGGL(k_pmc2d_charge<2>, dim3(nblk), dim3(kWave), sh, s, a);
-            else hipLaunchKernelGGL(k_pmc2d_charge<3>, dim3(nblk), dim3(kWave), sh, s, a);
-            hipLaunchKernelGGL(k_pmc2d_reduce, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s,
-                               dpart, nblk, (int)nn, drho);
-            ok(hipGetLastError());
-            ok(hipMemcpyAsync(rho, drho, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
-        } else {
-            ok(hipMemcpyAsync(df1, f1, sizeof(double) * nn, hipMemcpyHostToDevice, s));
-            a.f1 = df1; a.out1 = do1;
-            if (op == 2) {
-                ok(hipMemcpyAsync(df2, f2, sizeof(double) * nn, hipMemcpyHostToDevice, s));
-                a.f2 = df2; a.out2 = do2;
-            }
-            const unsigned gb = (unsigned)std::min<int64_t>((int64_t)cus * 8, (n + 255) / 256);
-#define EV(P, M) hipLaunchKernelGGL((k_pmc2d_eval<P, M>), dim3(gb), dim3(256), 0, s, a)
-            if (op == 1) { if (degree == 1) EV(1, false); else if (degree == 2) EV(2, false); else EV(3, false); }
-            else { if (degree == 1) EV(1, true); else if (degree == 2) EV(2, true); else EV(3, true); }
-#undef EV
-            ok(hipGetLastError());
-            ok(hipMemcpyAsync(out1, do1, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-            if (op == 2) ok(hipMemcpyAsync(out2, do2, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    return GEMPIC_OK;
+}
+
+// Per-device workspace of the device-resident verbs: the per-workgroup
+// deposit grids and the error flag. Grown on demand (hipMalloc only then);
+// calls that overlap on one device share it and race.
+struct Ws2 {
+    double *part = nullptr;
+    size_t cap = 0;          // doubles
+    int *err = nullptr;
+};
+std::mutex g_ws2_mu;
+Ws2 g_ws2[64];
+
+int ws2_get(size_t part_doubles, Ws2 *&w)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64)
+        return abi_fail(GEMPIC_ERR_DEVICE, "hipGetDevice failed");
+    std::lock_guard<std::mutex> lock(g_ws2_mu);
+    w = &g_ws2[dev];
+    if (!w->err) {
+        hipError_t e = hipMalloc((void **)&w->err, sizeof(int));
+        if (e == hipSuccess) e = hipMemset(w->err, 0, sizeof(int));
+        if (e != hipSuccess) {
+            w->err = nullptr;
+            return abi_fail(GEMPIC_ERR_DEVICE, (std::string("2D coupling workspace: ") + hipGetErrorString(e)).c_str());
         }
     }
+    if (part_doubles > w->cap) {
+        if (w->part) (void)hipFree(w->part);     // synchronizes with kernels still using it
+        w->part = nullptr;
+        w->cap = 0;
+        hipError_t e = hipMalloc((void **)&w->part, sizeof(double) * part_doubles);
+        if (e != hipSuccess)
+            return abi_fail(GEMPIC_ERR_DEVICE, (std::string("2D coupling workspace: ") + hipGetErrorString(e)).c_str());
+        w->cap = part_doubles;
+    }
+    return GEMPIC_OK;
+}
+
+template <int P, bool DET>
+const void *dep_fn() { return (const void *)k_pmc2d_deposit<P, DET>; }
+
+const void *dep_kernel(int degree, bool det)
+{
+    if (det) return degree == 1 ? dep_fn<1, true>() : degree == 2 ? dep_fn<2, true>() : dep_fn<3, true>();
+    return degree == 1 ? dep_fn<1, false>() : degree == 2 ? dep_fn<2, false>() : dep_fn<3, false>();
+}
+
+// LDS of k_pmc2d_deposit: grids (one per wave when deterministic), 16
+// staging doubles per wave, 256 hash slots per wave (deterministic)
+size_t dep_shmem(bool det, int nw, size_t nn)
+{
+    return sizeof(double) * ((det ? (size_t)nw : 1) * nn + (size_t)nw * 16) +
+           (det ? sizeof(int) * 256 * (size_t)nw : 0);
+}
+
+struct DepPlan {
+    int nw = 1, blocks = 1;
+    size_t sh = 0;
+};
+
+// waves per workgroup (4 unless the deterministic per-wave grids do not fit
+// in the workgroup's LDS) and workgroups (what the CUs hold at once)
+int dep_plan(int degree, bool det, int64_t n, size_t nn, DepPlan &pl)
+{
+    int dev = 0, maxsh = 65536, cus = abi_device_cus();
+    if (hipGetDevice(&dev) == hipSuccess)
+        (void)hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    const void *fn = dep_kernel(degree, det);
+    int nw = GEMPIC_DEP_WAVES;
+    while (nw > 1 && dep_shmem(det, nw, nn) > (size_t)maxsh) nw /= 2;
+    size_t sh = dep_shmem(det, nw, nn);
+    if (sh > 65536 &&
+        hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh) != hipSuccess) {
+        (void)hipGetLastError();
+        while (nw > 1 && dep_shmem(det, nw, nn) > 65536) nw /= 2;
+        sh = dep_shmem(det, nw, nn);
+    }
+    if (sh > (size_t)std::max(maxsh, 65536))
+        return abi_fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: the 2D grid does not fit in the deposit's LDS");
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, nw * kWave, sh) != hipSuccess || occ < 1) {
+        (void)hipGetLastError();
+        occ = 1;
+    }
+    if (GEMPIC_DEP_BLOCKS_PER_CU > 0) occ = std::min(occ, GEMPIC_DEP_BLOCKS_PER_CU);
+    const int64_t want = (n + (int64_t)nw * kWave - 1) / ((int64_t)nw * kWave);
+    pl.nw = nw;
+    pl.sh = sh;
+    pl.blocks = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)cus * occ, want));
+    return GEMPIC_OK;
+}
+
+// Enqueue one verb on `s` over device arrays (a.x, a.y, ... set by the caller).
+int pmc2_enqueue(int op, int degree, bool deterministic, Pmc2Args a, double *rho, hipStream_t s)
+{
+    const int cus = abi_device_cus();
+    const size_t nn = (size_t)a.nx * a.ny;
+    int nblk = 0, splits = 0;
+    DepPlan pl;
+    if (op == P2_CHARGE) {
+        int rc = dep_plan(degree, deterministic, a.n, nn, pl);
+        if (rc) return rc;
+        const int64_t waves = (int64_t)pl.blocks * pl.nw;
+        a.chunk = ((a.n + waves - 1) / waves + kWave - 1) / kWave * kWave;      // per wave
+        nblk = (int)((a.n + (int64_t)pl.nw * a.chunk - 1) / ((int64_t)pl.nw * a.chunk));
+        splits = std::min(nblk, 64);
+    }
+    Ws2 *w = nullptr;
+    int rc = ws2_get((size_t)(nblk + splits) * nn, w);
+    if (rc) return rc;
+    a.err = w->err;
+    a.part = w->part;
+    if (op == P2_CHARGE) {
+        const void *fn = dep_kernel(degree, deterministic);
+        void *args[] = {&a};
+        hipError_t e = hipLaunchKernel(fn, dim3(nblk), dim3(pl.nw * kWave), args, pl.sh, s);
+        if (e != hipSuccess)
+            return abi_fail(GEMPIC_ERR_DEVICE, (std::string("2D deposit launch: ") + hipGetErrorString(e)).c_str());
+        double *part2 = w->part + (size_t)nblk * nn;
+        hipLaunchKernelGGL(k_pmc2d_reduce1, dim3((unsigned)((nn + 255) / 256), (unsigned)splits), dim3(256),
+                           0, s, (const double *)w->part, nblk, (int)nn, part2);
+        hipLaunchKernelGGL(k_pmc2d_reduce2, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s,
+                           (const double *)part2, splits, (int)nn, rho);
+    } else {
+        const bool multi = op == P2_EVAL2;
+        const size_t ns = multi ? 2 * nn : (size_t)(a.nx + degree) * (a.ny + degree);
+        const bool stage = ns <= GEMPIC_EVAL_STAGE_MAX;                 // <= 64 KiB of LDS
+        const size_t sh = stage ? sizeof(double) * ns : 0;
+        const unsigned gb = (unsigned)std::max<int64_t>(
+            1, std::min<int64_t>((int64_t)cus * GEMPIC_EVAL_BLOCKS_PER_CU, (a.n + 255) / 256));
+#define EV(P, M, S) hipLaunchKernelGGL((k_pmc2d_eval<P, M, S>), dim3(gb), dim3(256), sh, s, a)
+#define EVD(M, S) do { if (degree == 1) EV(1, M, S); else if (degree == 2) EV(2, M, S); else EV(3, M, S); } while (0)
+        if (multi) { if (stage) EVD(true, true); else EVD(true, false); }
+        else { if (stage) EVD(false, true); else EVD(false, false); }
+#undef EVD
+#undef EV
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return abi_fail(GEMPIC_ERR_DEVICE, (std::string("2D coupling launch: ") + hipGetErrorString(e)).c_str());
+    return GEMPIC_OK;
+}
+
+// wait for `s`, then report and clear the device's error flag
+int pmc2_finish(hipStream_t s)
+{
+    Ws2 *w = nullptr;
+    int rc = ws2_get(0, w);
+    if (rc) return rc;
     int herr = 0;
-    if (e == hipSuccess) ok(hipMemcpyAsync(&herr, derr, sizeof(int), hipMemcpyDeviceToHost, s));
-    if (e == hipSuccess) ok(hipStreamSynchronize(s));
-    void *bufs[] = {dx_, dy_, dw, df1, df2, do1, do2, dpart, drho, derr};
-    for (void *p : bufs)
-        if (p) (void)hipFree(p);
-    if (e != hipSuccess) return abi_fail(GEMPIC_ERR_DEVICE, (std::string("2D coupling: ") + hipGetErrorString(e)).c_str());
+    hipError_t e = hipMemcpyAsync(&herr, w->err, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e == hipSuccess && herr) e = hipMemsetAsync(w->err, 0, sizeof(int), s);
+    if (e == hipSuccess && herr) e = hipStreamSynchronize(s);
+    if (e != hipSuccess)
+        return abi_fail(GEMPIC_ERR_DEVICE, (std::string("2D coupling: ") + hipGetErrorString(e)).c_str());
     if (herr & E_INEXACT)
         return abi_fail(GEMPIC_ERR_INEXACT, "InexactError: ceil(Int, x) of a non-finite or out-of-range position");
     return GEMPIC_OK;
 }
+
+hipStream_t pmc2_stream()
+{
+    static thread_local hipStream_t s = nullptr;
+    if (!s) (void)hipStreamCreate(&s);
+    return s;
+}
+
+// the batched host-array forms: copy in, run deterministically, copy out
+int pmc2_host(int op, int64_t nx, int64_t ny, double xmin, double xmax, double ymin,
+              double ymax, int degree, int smoothing, int64_t n, const double *x,
+              const double *y, const double *w, const double *f1, const double *f2,
+              double *out1, double *out2, double *rho)
+{
+    Pmc2Args a;
+    int rc = pmc2_geometry(op, nx, ny, xmin, xmax, ymin, ymax, degree, smoothing, n, a);
+    if (rc) return rc;
+    rc = abi_check_device();
+    if (rc) return rc;
+    if (n == 0) return GEMPIC_OK;
+    hipStream_t s = pmc2_stream();
+    const size_t nn = (size_t)nx * ny;
+    double *dx_ = nullptr, *dy_ = nullptr, *dw = nullptr, *df1 = nullptr, *df2 = nullptr;
+    double *do1 = nullptr, *do2 = nullptr, *drho = nullptr;
+    hipError_t e = hipSuccess;
+    auto ok = [&](hipError_t r) { if (r != hipSuccess && e == hipSuccess) e = r; };
+    ok(dalloc2(&dx_, n));
+    ok(dalloc2(&dy_, n));
+    if (op == P2_CHARGE) { ok(dalloc2(&dw, n)); ok(dalloc2(&drho, nn)); }
+    else { ok(dalloc2(&df1, nn)); ok(dalloc2(&do1, n)); if (op == P2_EVAL2) { ok(dalloc2(&df2, nn)); ok(dalloc2(&do2, n)); } }
+    rc = GEMPIC_OK;
+    if (e == hipSuccess) {
+        ok(hipMemcpyAsync(dx_, x, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        ok(hipMemcpyAsync(dy_, y, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        a.x = dx_;
+        a.y = dy_;
+        if (op == P2_CHARGE) {
+            ok(hipMemcpyAsync(dw, w, sizeof(double) * n, hipMemcpyHostToDevice, s));
+            ok(hipMemcpyAsync(drho, rho, sizeof(double) * nn, hipMemcpyHostToDevice, s));
+            a.w = dw;
+        } else {
+            ok(hipMemcpyAsync(df1, f1, sizeof(double) * nn, hipMemcpyHostToDevice, s));
+            a.f1 = df1;
+            a.out1 = do1;
+            if (op == P2_EVAL2) {
+                ok(hipMemcpyAsync(df2, f2, sizeof(double) * nn, hipMemcpyHostToDevice, s));
+                a.f2 = df2;
+                a.out2 = do2;
+            }
+        }
+        if (e == hipSuccess) rc = pmc2_enqueue(op, degree, true, a, drho, s);
+        if (e == hipSuccess && rc == GEMPIC_OK) {
+            if (op == P2_CHARGE) ok(hipMemcpyAsync(rho, drho, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
+            else {
+                ok(hipMemcpyAsync(out1, do1, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+                if (op == P2_EVAL2) ok(hipMemcpyAsync(out2, do2, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+            }
+        }
+    }
+    int frc = GEMPIC_OK;
+    if (e == hipSuccess && rc == GEMPIC_OK) frc = pmc2_finish(s);
+    else (void)hipStreamSynchronize(s);
+    void *bufs[] = {dx_, dy_, dw, df1, df2, do1, do2, drho};
+    for (void *p : bufs)
+        if (p) (void)hipFree(p);
+    if (e != hipSuccess) return abi_fail(GEMPIC_ERR_DEVICE, (std::string("2D coupling: ") + hipGetErrorString(e)).c_str());
+    if (rc) return rc;
+    return frc;
+}
+
+// the device-resident forms: enqueue on the caller's stream; synchronous
+// unless GEMPIC_ASYNC (then errors surface at the next synchronous call or
+// gempic_pmc2d_sync)
+int pmc2_dev(int op, int64_t nx, int64_t ny, double xmin, double xmax, double ymin,
+             double ymax, int degree, int smoothing, int64_t n, Pmc2Args io, double *rho,
+             int flags, void *stream)
+{
+    Pmc2Args a;
+    int rc = pmc2_geometry(op, nx, ny, xmin, xmax, ymin, ymax, degree, smoothing, n, a);
+    if (rc) return rc;
+    rc = abi_check_device();
+    if (rc) return rc;
+    if (n == 0) return GEMPIC_OK;
+    a.x = io.x; a.y = io.y; a.w = io.w;
+    a.f1 = io.f1; a.f2 = io.f2; a.out1 = io.out1; a.out2 = io.out2;
+    hipStream_t s = (hipStream_t)stream;
+    rc = pmc2_enqueue(op, degree, (flags & GEMPIC_DETERMINISTIC) != 0, a, rho, s);
+    if (rc) return rc;
+    if (flags & GEMPIC_ASYNC) return GEMPIC_OK;
+    return pmc2_finish(s);
+}
+
+}  // namespace
+
+extern "C" {
 
 int gempic_pmc2d_add_charge(int64_t nx, int64_t ny, double xmin, double xmax, double ymin,
                             double ymax, int degree, int smoothing, int64_t n, const double *x,
                             const double *y, const double *wp, double *rho)
 {
     if ((n > 0 && (!x || !y || !wp)) || !rho) return abi_fail(GEMPIC_ERR_ARGUMENT, "NULL array");
-    return pmc2_run(0, nx, ny, xmin, xmax, ymin, ymax, degree, smoothing, n, x, y, wp, nullptr,
-                    nullptr, nullptr, nullptr, rho);
+    return pmc2_host(P2_CHARGE, nx, ny, xmin, xmax, ymin, ymax, degree, smoothing, n, x, y, wp,
+                     nullptr, nullptr, nullptr, nullptr, rho);
 }
 
 int gempic_pmc2d_evaluate(int64_t nx, int64_t ny, double xmin, double xmax, double ymin,
@@ -898,8 +1219,8 @@ int gempic_pmc2d_evaluate(int64_t nx, int64_t ny, double xmin, double xmax, doub
                           const double *y, const double *field, double *out)
 {
     if ((n > 0 && (!x || !y || !out)) || !field) return abi_fail(GEMPIC_ERR_ARGUMENT, "NULL array");
-    return pmc2_run(1, nx, ny, xmin, xmax, ymin, ymax, degree, smoothing, n, x, y, nullptr, field,
-                    nullptr, out, nullptr, nullptr);
+    return pmc2_host(P2_EVAL, nx, ny, xmin, xmax, ymin, ymax, degree, smoothing, n, x, y, nullptr,
+                     field, nullptr, out, nullptr, nullptr);
 }
 
 int gempic_pmc2d_evaluate_multiple(int64_t nx, int64_t ny, double xmin, double xmax, double ymin,
@@ -909,8 +1230,53 @@ int gempic_pmc2d_evaluate_multiple(int64_t nx, int64_t ny, double xmin, double x
 {
     if ((n > 0 && (!x || !y || !out1 || !out2)) || !field1 || !field2)
         return abi_fail(GEMPIC_ERR_ARGUMENT, "NULL array");
-    return pmc2_run(2, nx, ny, xmin, xmax, ymin, ymax, degree, smoothing, n, x, y, nullptr,
-                    field1, field2, out1, out2, nullptr);
+    return pmc2_host(P2_EVAL2, nx, ny, xmin, xmax, ymin, ymax, degree, smoothing, n, x, y,
+                     nullptr, field1, field2, out1, out2, nullptr);
+}
+
+int gempic_pmc2d_add_charge_dev(int64_t nx, int64_t ny, double xmin, double xmax, double ymin,
+                                double ymax, int degree, int smoothing, int64_t n,
+                                const double *x, const double *y, const double *wp, double *rho,
+                                int flags, void *stream)
+{
+    if ((n > 0 && (!x || !y || !wp)) || !rho) return abi_fail(GEMPIC_ERR_ARGUMENT, "NULL array");
+    Pmc2Args io{};
+    io.x = x; io.y = y; io.w = wp;
+    return pmc2_dev(P2_CHARGE, nx, ny, xmin, xmax, ymin, ymax, degree, smoothing, n, io, rho,
+                    flags, stream);
+}
+
+int gempic_pmc2d_evaluate_dev(int64_t nx, int64_t ny, double xmin, double xmax, double ymin,
+                              double ymax, int degree, int smoothing, int64_t n, const double *x,
+                              const double *y, const double *field, double *out, int flags,
+                              void *stream)
+{
+    if ((n > 0 && (!x || !y || !out)) || !field) return abi_fail(GEMPIC_ERR_ARGUMENT, "NULL array");
+    Pmc2Args io{};
+    io.x = x; io.y = y; io.f1 = field; io.out1 = out;
+    return pmc2_dev(P2_EVAL, nx, ny, xmin, xmax, ymin, ymax, degree, smoothing, n, io, nullptr,
+                    flags, stream);
+}
+
+int gempic_pmc2d_evaluate_multiple_dev(int64_t nx, int64_t ny, double xmin, double xmax,
+                                       double ymin, double ymax, int degree, int smoothing,
+                                       int64_t n, const double *x, const double *y,
+                                       const double *field1, const double *field2, double *out1,
+                                       double *out2, int flags, void *stream)
+{
+    if ((n > 0 && (!x || !y || !out1 || !out2)) || !field1 || !field2)
+        return abi_fail(GEMPIC_ERR_ARGUMENT, "NULL array");
+    Pmc2Args io{};
+    io.x = x; io.y = y; io.f1 = field1; io.f2 = field2; io.out1 = out1; io.out2 = out2;
+    return pmc2_dev(P2_EVAL2, nx, ny, xmin, xmax, ymin, ymax, degree, smoothing, n, io, nullptr,
+                    flags, stream);
+}
+
+int gempic_pmc2d_sync(void *stream)
+{
+    int rc = abi_check_device();
+    if (rc) return rc;
+    return pmc2_finish((hipStream_t)stream);
 }
 
 }  // extern "C"

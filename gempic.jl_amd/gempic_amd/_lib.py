@@ -24,6 +24,7 @@ COLLOCATION = 1
 STRANG_FUSED = 0
 STRANG_SUBSTEP = 1
 ASYNC = 16
+DETERMINISTIC = 32
 
 _D = ctypes.c_double
 _I64 = ctypes.c_int64
@@ -80,6 +81,13 @@ SIGNATURES = {
     "gempic_pmc2d_evaluate": (_I, [_I64, _I64, _D, _D, _D, _D, _I, _I, _I64, _P, _P, _P, _P]),
     "gempic_pmc2d_evaluate_multiple": (_I, [_I64, _I64, _D, _D, _D, _D, _I, _I, _I64, _P, _P,
                                             _P, _P, _P, _P]),
+    "gempic_pmc2d_add_charge_dev": (_I, [_I64, _I64, _D, _D, _D, _D, _I, _I, _I64, _VP, _VP, _VP,
+                                         _VP, _I, _VP]),
+    "gempic_pmc2d_evaluate_dev": (_I, [_I64, _I64, _D, _D, _D, _D, _I, _I, _I64, _VP, _VP, _VP,
+                                       _VP, _I, _VP]),
+    "gempic_pmc2d_evaluate_multiple_dev": (_I, [_I64, _I64, _D, _D, _D, _D, _I, _I, _I64, _VP, _VP,
+                                                _VP, _VP, _VP, _VP, _I, _VP]),
+    "gempic_pmc2d_sync": (_I, [_VP]),
     "gempic_maxwell2d_create": (_I, [_I64, _D, _D, _I64, _D, _D, _I, _PP]),
     "gempic_maxwell2d_destroy": (_I, [_VP]),
     "gempic_maxwell2d_tables": (_I, [_VP, _P, _P, _P, _P]),

@@ -25,6 +25,7 @@ __all__ = [
     "TwoDGrid", "ParticleMeshCoupling2D", "evaluate_multiple", "TwoDMaxwell",
     "TwoDLinearSolverSplineMass", "solve", "multiply_mass_2dkron_", "compute_rho_from_e_",
     "compute_fem_rhs_", "l2projection", "spline_fem_mass_line", "spline_fem_compute_mass_eig",
+    "add_charge_device_", "evaluate_device", "evaluate_multiple_device", "pmc2d_sync",
 ]
 
 
@@ -135,6 +136,69 @@ def evaluate_multiple(pm: ParticleMeshCoupling2D, position, field_dofs):
     if np.ndim(xp) == 0:
         return float(o1[0]), float(o2[0])
     return o1, o2
+
+
+# device-resident forms: arrays are gempic_amd.device.DeviceArray (GPU memory
+# of the library's own HIP runtime), streams gempic_amd.device.Stream
+
+def _dev(t, n, name):
+    from .device import DeviceArray
+    if not isinstance(t, DeviceArray):
+        raise TypeError(f"{name}: DeviceArray expected")
+    if t.size != n:
+        raise ValueError(f"{name}: {n} entries expected, got {t.size}")
+    return t.ptr
+
+
+def _flags(deterministic, async_):
+    return (_lib.DETERMINISTIC if deterministic else 0) | (_lib.ASYNC if async_ else 0)
+
+
+def _stream(stream):
+    return None if stream is None else stream.handle
+
+
+def add_charge_device_(rho, pm: ParticleMeshCoupling2D, x, y, wp, deterministic=False,
+                       async_=False, stream=None):
+    """add_charge! (particle_mesh_coupling_2d.jl:94-105) on particles in GPU
+    memory: rho (nx*ny), x, y, wp (n) are DeviceArrays. The work runs on
+    `stream` (the null stream by default)."""
+    n = x.size
+    args = (_dev(x, n, "x"), _dev(y, n, "y"), _dev(wp, n, "wp"),
+            _dev(rho, pm.grid.nx * pm.grid.ny, "rho"))
+    check(_lib.load().gempic_pmc2d_add_charge_dev(*pm._args(), n, *args,
+                                                  _flags(deterministic, async_), _stream(stream)))
+
+
+def evaluate_device(pm: ParticleMeshCoupling2D, x, y, field_dofs, out, async_=False,
+                    stream=None):
+    """evaluate (particle_mesh_coupling_2d.jl:189-203) into `out` on the GPU."""
+    n = x.size
+    args = (_dev(x, n, "x"), _dev(y, n, "y"),
+            _dev(field_dofs, pm.grid.nx * pm.grid.ny, "field_dofs"), _dev(out, n, "out"))
+    check(_lib.load().gempic_pmc2d_evaluate_dev(*pm._args(), n, *args, _flags(False, async_),
+                                                _stream(stream)))
+    return out
+
+
+def evaluate_multiple_device(pm: ParticleMeshCoupling2D, position, field_dofs, out,
+                             async_=False, stream=None):
+    """evaluate_multiple (particle_mesh_coupling_2d.jl:214-231) into the two
+    tensors of `out` on the GPU."""
+    x, y = position
+    n = x.size
+    nn = pm.grid.nx * pm.grid.ny
+    args = (_dev(x, n, "x"), _dev(y, n, "y"), _dev(field_dofs[0], nn, "field_dofs[1]"),
+            _dev(field_dofs[1], nn, "field_dofs[2]"), _dev(out[0], n, "out[1]"),
+            _dev(out[1], n, "out[2]"))
+    check(_lib.load().gempic_pmc2d_evaluate_multiple_dev(*pm._args(), n, *args,
+                                                         _flags(False, async_), _stream(stream)))
+    return out
+
+
+def pmc2d_sync(stream=None):
+    """Wait for the stream; raise errors of earlier asynchronous 2D calls."""
+    check(_lib.load().gempic_pmc2d_sync(_stream(stream)))
 
 
 # ---------------------------------------------------------------------------

@@ -232,6 +232,33 @@ int gempic_pmc2d_evaluate_multiple(int64_t nx, int64_t ny, double xmin, double x
                                    const double *x, const double *y, const double *field1,
                                    const double *field2, double *out1, double *out2);
 
+/* Device-resident forms of the three verbs above, for particles kept in GPU
+ * memory (a ROCArray on the Julia side): x, y, wp, rho, field*, out* are
+ * device pointers and `stream` a hipStream_t (NULL: the null stream).
+ * flags: GEMPIC_DETERMINISTIC makes the deposit bit-reproducible from run
+ * to run (per-lane patch caches added in lane order); without it the deposit
+ * adds into a per-workgroup LDS grid with LDS atomics, and only the order of
+ * those additions varies. GEMPIC_ASYNC returns once the work is queued; an
+ * InexactError is then reported by the next synchronous call or by
+ * gempic_pmc2d_sync. The calls share one workspace per device: calls that
+ * overlap on one device race. */
+#define GEMPIC_DETERMINISTIC 32
+int gempic_pmc2d_add_charge_dev(int64_t nx, int64_t ny, double xmin, double xmax, double ymin,
+                                double ymax, int degree, int smoothing, int64_t n,
+                                const double *x, const double *y, const double *wp, double *rho,
+                                int flags, void *stream);
+int gempic_pmc2d_evaluate_dev(int64_t nx, int64_t ny, double xmin, double xmax, double ymin,
+                              double ymax, int degree, int smoothing, int64_t n, const double *x,
+                              const double *y, const double *field, double *out, int flags,
+                              void *stream);
+int gempic_pmc2d_evaluate_multiple_dev(int64_t nx, int64_t ny, double xmin, double xmax,
+                                       double ymin, double ymax, int degree, int smoothing,
+                                       int64_t n, const double *x, const double *y,
+                                       const double *field1, const double *field2, double *out1,
+                                       double *out2, int flags, void *stream);
+/* wait for `stream` and report errors of earlier GEMPIC_ASYNC 2D calls */
+int gempic_pmc2d_sync(void *stream);
+
 /* ------------------------------------------------------------------------
  * TwoDMaxwell (maxwell_2d_fem.jl:11-88) with its TwoDPoisson
  * (poisson_2d_fem.jl:3-85) and TwoDLinearSolverSplineMass

@@ -135,6 +135,13 @@ void or_pmc2d_add_charge_pp(double *rho, const or_pmc2d *p, double x, double y, 
 double or_pmc2d_evaluate(const or_pmc2d *p, double x, double y, const double *field);
 void or_pmc2d_evaluate_multiple(const or_pmc2d *p, double x, double y, const double *f1,
                                 const double *f2, double *out2);
+void or_pmc2d_add_charge_n(double *rho, const or_pmc2d *p, int64_t n, const double *x,
+                           const double *y, const double *wp);
+void or_pmc2d_evaluate_n(const or_pmc2d *p, int64_t n, const double *x, const double *y,
+                         const double *field, double *out);
+void or_pmc2d_evaluate_multiple_n(const or_pmc2d *p, int64_t n, const double *x,
+                                  const double *y, const double *f1, const double *f2,
+                                  double *out1, double *out2);
 void or_pmc2d_b_to_pp(const or_pmc2d *p, const double *b, double *pp);
 double or_pmc2d_evaluate_pp(const or_pmc2d *p, double x, double y, const double *pp,
                             int *bounds_error);

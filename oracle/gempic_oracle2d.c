@@ -163,6 +163,32 @@ void or_pmc2d_evaluate_multiple(const or_pmc2d *p, double x, double y, const dou
     out2[1] = v2;
 }
 
+/* The reference's loop over particles (one call per particle, in order) for
+ * n particles: add_charge! and evaluate / evaluate_multiple. */
+void or_pmc2d_add_charge_n(double *rho, const or_pmc2d *p, int64_t n, const double *x,
+                           const double *y, const double *wp)
+{
+    for (int64_t i = 0; i < n; ++i) or_pmc2d_add_charge(rho, p, x[i], y[i], wp[i]);
+}
+
+void or_pmc2d_evaluate_n(const or_pmc2d *p, int64_t n, const double *x, const double *y,
+                         const double *field, double *out)
+{
+    for (int64_t i = 0; i < n; ++i) out[i] = or_pmc2d_evaluate(p, x[i], y[i], field);
+}
+
+void or_pmc2d_evaluate_multiple_n(const or_pmc2d *p, int64_t n, const double *x,
+                                  const double *y, const double *f1, const double *f2,
+                                  double *out1, double *out2)
+{
+    double v[2];
+    for (int64_t i = 0; i < n; ++i) {
+        or_pmc2d_evaluate_multiple(p, x[i], y[i], f1, f2, v);
+        out1[i] = v[0];
+        out2[i] = v[1];
+    }
+}
+
 /* b_to_pp_1d_cell! (splinepp.jl:310-316): pp[j] = sum_i b[i] * coeffs[j, i],
  * accumulated from zero with i outer */
 static void b_to_pp_1d_cell(double *pp, int64_t stride, const double *pc, int dp1,

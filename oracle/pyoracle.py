@@ -96,6 +96,9 @@ def lib():
             "or_pmc2d_add_charge_pp": (None, [_P, _VP, _D, _D, _D]),
             "or_pmc2d_evaluate": (_D, [_VP, _D, _D, _P]),
             "or_pmc2d_evaluate_multiple": (None, [_VP, _D, _D, _P, _P, _P]),
+            "or_pmc2d_add_charge_n": (None, [_P, _VP, _I64, _P, _P, _P]),
+            "or_pmc2d_evaluate_n": (None, [_VP, _I64, _P, _P, _P, _P]),
+            "or_pmc2d_evaluate_multiple_n": (None, [_VP, _I64, _P, _P, _P, _P, _P, _P]),
             "or_pmc2d_b_to_pp": (None, [_VP, _P, _P]),
             "or_pmc2d_evaluate_pp": (_D, [_VP, _D, _D, _P, ctypes.POINTER(_I)]),
             "or_spline_fem_mass_line": (None, [_I, _P]),
@@ -388,6 +391,24 @@ class PMC2D:
         out = np.zeros(2)
         lib().or_pmc2d_evaluate_multiple(self.ptr(), x, y, _p(f1), _p(f2), _p(out))
         return out[0], out[1]
+
+    def add_charge_n(self, rho, x, y, w):
+        """The reference's per-particle loop over arrays (in order)."""
+        x, y, w = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, w))
+        lib().or_pmc2d_add_charge_n(_p(rho), self.ptr(), x.size, _p(x), _p(y), _p(w))
+
+    def evaluate_n(self, x, y, field):
+        x, y = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y))
+        out = np.zeros(x.size)
+        lib().or_pmc2d_evaluate_n(self.ptr(), x.size, _p(x), _p(y), _p(field), _p(out))
+        return out
+
+    def evaluate_multiple_n(self, x, y, f1, f2):
+        x, y = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y))
+        o1, o2 = np.zeros(x.size), np.zeros(x.size)
+        lib().or_pmc2d_evaluate_multiple_n(self.ptr(), x.size, _p(x), _p(y), _p(f1), _p(f2),
+                                           _p(o1), _p(o2))
+        return o1, o2
 
     def b_to_pp(self, b):
         d = self.degree + 1

@@ -5,9 +5,9 @@ C ABI, against the oracle and the reference's own 2D tests.
 Tolerances: evaluation, the field solves and the field updates use the
 oracle's operations in its order and are compared bit for bit. The deposit
 sums each particle's contribution in a different order than the reference's
-sequential loop (deterministic per-lane, per-wave and per-workgroup sums), so
-it is compared at 1e-13 relative to the largest grid value; the same holds for
-inner_product's sum over the grid.
+sequential loop (deterministic per-lane, per-wave and per-workgroup sums, or
+LDS atomics in the fast device form), so it is compared at 1e-13 relative to
+the largest grid value; the same holds for inner_product's sum over the grid.
 """
 import math
 
@@ -31,8 +31,7 @@ def _coupling(nx, ny, bounds, degree, smoothing, n=1):
 
 
 def _oracle_deposit(p, rho, x, y, w):
-    for a, b, c in zip(x, y, w):
-        p.add_charge(rho, float(a), float(b), float(c))
+    p.add_charge_n(rho, x, y, w)          # the reference's loop over particles, in order
 
 
 def _assert_deposit(got, want):
@@ -353,3 +352,77 @@ def test_gpu_maxwell_2d_analytic():
     rho = np.zeros(nn)
     G.compute_rho_from_e_(rho, m, ef)
     np.testing.assert_allclose(rho, rho_ref, rtol=1e-6, atol=1e-8)
+
+
+# ---------------------------------------------------------------------------
+# device-resident forms (particles in GPU memory)
+
+from gempic_amd.device import DeviceArray  # noqa: E402
+
+
+@pytest.mark.parametrize("degree", [1, 2, 3])
+def test_gpu_pmc2d_device_forms(degree):
+    """The device-resident verbs: the LDS-atomic deposit within 1e-13 of the
+    oracle, the deterministic deposit bitwise equal to the host form, and
+    the staged and unstaged evaluations bitwise equal to the oracle."""
+    rng = np.random.default_rng(21 + degree)
+    nx, ny = 48, 40
+    bounds = (0.0, 3.0, -1.0, 1.0)
+    n = 200003
+    x, y = _particles(rng, n, bounds, nx, ny)
+    w = rng.random(n)
+    _, pm = _coupling(nx, ny, bounds, degree, "galerkin", n)
+    p = orc.PMC2D(bounds[0], bounds[1], nx, bounds[2], bounds[3], ny, degree, orc.GALERKIN)
+    want = np.zeros(nx * ny)
+    _oracle_deposit(p, want, x, y, w)
+    dx_, dy_, dw = DeviceArray.from_numpy(x), DeviceArray.from_numpy(y), DeviceArray.from_numpy(w)
+    rho = DeviceArray.zeros(nx * ny)
+    G.add_charge_device_(rho, pm, dx_, dy_, dw)
+    _assert_deposit(rho.numpy(), want)
+    rho_det = DeviceArray.zeros(nx * ny)
+    G.add_charge_device_(rho_det, pm, dx_, dy_, dw, deterministic=True)
+    host = np.zeros(nx * ny)
+    G.add_charge_(host, pm, x, y, w)
+    np.testing.assert_array_equal(rho_det.numpy(), host)
+    # 1920 nodes: one field (15 KiB) and two fields are staged in LDS
+    f1, f2 = rng.normal(size=nx * ny), rng.normal(size=nx * ny)
+    out = DeviceArray(n)
+    G.evaluate_device(pm, dx_, dy_, DeviceArray.from_numpy(f1), out)
+    np.testing.assert_array_equal(out.numpy(), p.evaluate_n(x, y, f1))
+    o1, o2 = DeviceArray(n), DeviceArray(n)
+    G.evaluate_multiple_device(pm, (dx_, dy_), (DeviceArray.from_numpy(f1),
+                                                DeviceArray.from_numpy(f2)), (o1, o2))
+    r1, r2 = p.evaluate_multiple_n(x, y, f1, f2)
+    np.testing.assert_array_equal(o1.numpy(), r1)
+    np.testing.assert_array_equal(o2.numpy(), r2)
+
+
+def test_gpu_pmc2d_device_unstaged_eval_and_async_errors():
+    rng = np.random.default_rng(33)
+    nx, ny = 96, 80                    # 7680 nodes: two fields are read from global memory
+    _, pm = _coupling(nx, ny, (0.0, 1.0, 0.0, 1.0), 3, "galerkin")
+    p = orc.PMC2D(0.0, 1.0, nx, 0.0, 1.0, ny, 3, orc.GALERKIN)
+    n = 10000
+    x, y = rng.random(n), rng.random(n)
+    f1, f2 = rng.normal(size=nx * ny), rng.normal(size=nx * ny)
+    D = DeviceArray.from_numpy
+    o1, o2 = DeviceArray(n), DeviceArray(n)
+    G.evaluate_device(pm, D(x), D(y), D(f1), o1)
+    np.testing.assert_array_equal(o1.numpy(), p.evaluate_n(x, y, f1))
+    G.evaluate_multiple_device(pm, (D(x), D(y)), (D(f1), D(f2)), (o1, o2))
+    r1, r2 = p.evaluate_multiple_n(x, y, f1, f2)
+    np.testing.assert_array_equal(o1.numpy(), r1)
+    np.testing.assert_array_equal(o2.numpy(), r2)
+    # an InexactError of an asynchronous call surfaces at the sync, once
+    _, small = _coupling(16, 16, (0.0, 1.0, 0.0, 1.0), 3, "galerkin")
+    xb = x.copy()
+    xb[17] = np.nan
+    rho = DeviceArray.zeros(256)
+    G.add_charge_device_(rho, small, D(xb), D(y), D(np.ones(n)), async_=True)
+    with pytest.raises(G.InexactError):
+        G.pmc2d_sync()
+    G.pmc2d_sync()
+    with pytest.raises(ValueError):                          # the deposit's LDS grid limit
+        G.add_charge_device_(DeviceArray.zeros(nx * ny), pm, D(x), D(y), D(np.ones(n)))
+    with pytest.raises(TypeError):
+        G.add_charge_device_(rho, small, x, y, np.ones(n))

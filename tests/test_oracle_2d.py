@@ -278,3 +278,20 @@ def test_maxwell_2d_analytic():
     rho = np.zeros(nx1 * nx2)
     m.compute_rho_from_e(rho, ef)
     np.testing.assert_allclose(rho, rho_ref, rtol=1e-6, atol=1e-8)
+
+
+def test_pmc2d_batched_loops_are_the_per_particle_calls():
+    rng = np.random.default_rng(9)
+    p = orc.PMC2D(0.0, 1.0, 7, -1.0, 1.0, 6, 2, orc.GALERKIN)
+    x, y, w = rng.random(300), 2.0 * rng.random(300) - 1.0, rng.random(300)
+    r1, r2 = np.zeros(42), np.zeros(42)
+    p.add_charge_n(r1, x, y, w)
+    for a, b, c in zip(x, y, w):
+        p.add_charge(r2, a, b, c)
+    np.testing.assert_array_equal(r1, r2)
+    f1, f2 = rng.normal(size=42), rng.normal(size=42)
+    np.testing.assert_array_equal(p.evaluate_n(x, y, f1), [p.evaluate(a, b, f1) for a, b in zip(x, y)])
+    o1, o2 = p.evaluate_multiple_n(x, y, f1, f2)
+    ref = np.array([p.evaluate_multiple(a, b, f1, f2) for a, b in zip(x, y)])
+    np.testing.assert_array_equal(o1, ref[:, 0])
+    np.testing.assert_array_equal(o2, ref[:, 1])
