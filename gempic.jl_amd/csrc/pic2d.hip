@@ -32,18 +32,26 @@ namespace {
 
 constexpr int kGrid2Max = 4096;            // nx*ny of the device deposit (LDS grid)
 
-// launch shape of the evaluation kernels: workgroups per CU, and the largest
+// launch shape of the evaluation kernels: threads per workgroup (the waves
+// of a workgroup share its staged field), workgroups per CU, and the largest
 // field (doubles, both fields of evaluate_multiple together) staged in LDS
+#ifndef GEMPIC_EVAL_THREADS
+#define GEMPIC_EVAL_THREADS 1024
+#endif
 #ifndef GEMPIC_EVAL_BLOCKS_PER_CU
 #define GEMPIC_EVAL_BLOCKS_PER_CU 4
 #endif
 #ifndef GEMPIC_EVAL_STAGE_MAX
 #define GEMPIC_EVAL_STAGE_MAX 8192
 #endif
-// deposit: waves per workgroup (at most) and workgroups per CU (0: as many
-// as fit)
+// deposit: waves per workgroup (at most; the fast mode's waves share one
+// LDS grid, the deterministic mode's own one each) and workgroups per CU (0:
+// as many as fit)
 #ifndef GEMPIC_DEP_WAVES
 #define GEMPIC_DEP_WAVES 4
+#endif
+#ifndef GEMPIC_DEP_WAVES_FAST
+#define GEMPIC_DEP_WAVES_FAST 4
 #endif
 #ifndef GEMPIC_DEP_BLOCKS_PER_CU
 #define GEMPIC_DEP_BLOCKS_PER_CU 0
@@ -98,7 +106,7 @@ __device__ __forceinline__ bool shape2(const Pmc2Args &a, double x, double y, in
 // In the deterministic mode every addition's order is a fixed function of
 // the particle order; the waves' grids are summed in wave order at the end.
 template <int P, bool DET>
-__global__ void __launch_bounds__(256) k_pmc2d_deposit(Pmc2Args a)
+__global__ void __launch_bounds__(1024) k_pmc2d_deposit(Pmc2Args a)
 {
     constexpr int W = P + 1, WW = W * W;
     static_assert(WW <= 16, "one node per lane of a 16-lane row");
@@ -212,6 +220,23 @@ __global__ void __launch_bounds__(256) k_pmc2d_deposit(Pmc2Args a)
     }
 }
 
+// sum of col[b * nn] for b in [b0, b1), added in order; the loads are issued
+// eight at a time
+__device__ __forceinline__ double sum_column(const double *col, int b0, int b1, int nn)
+{
+    double t = 0.0;
+    int b = b0;
+    for (; b + 8 <= b1; b += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = col[(size_t)(b + u) * nn];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) t += v[u];
+    }
+    for (; b < b1; ++b) t += col[(size_t)b * nn];
+    return t;
+}
+
 // part2[s][k] = sum of part[b][k] over the workgroups b of split s, in order
 __global__ void k_pmc2d_reduce1(const double *part, int nblocks, int nn, double *part2)
 {
@@ -219,9 +244,7 @@ __global__ void k_pmc2d_reduce1(const double *part, int nblocks, int nn, double 
     if (k >= nn) return;
     const int splits = gridDim.y, sp = blockIdx.y;
     const int b0 = (int)((int64_t)sp * nblocks / splits), b1 = (int)((int64_t)(sp + 1) * nblocks / splits);
-    double t = 0.0;
-    for (int b = b0; b < b1; ++b) t += part[(size_t)b * nn + k];
-    part2[(size_t)sp * nn + k] = t;
+    part2[(size_t)sp * nn + k] = sum_column(part + k, b0, b1, nn);
 }
 
 // rho[k] += sum of part2[s][k] over the splits, in order
@@ -229,9 +252,7 @@ __global__ void k_pmc2d_reduce2(const double *part2, int splits, int nn, double 
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nn) return;
-    double t = 0.0;
-    for (int sp = 0; sp < splits; ++sp) t += part2[(size_t)sp * nn + k];
-    rho[k] = rho[k] + t;
+    rho[k] = rho[k] + sum_column(part2 + k, 0, splits, nn);
 }
 
 // evaluate (:189-203) and evaluate_multiple (:214-231). With STAGE the
@@ -241,7 +262,7 @@ __global__ void k_pmc2d_reduce2(const double *part2, int splits, int nn, double 
 // interleaved (one 16-byte read per node). The next particle's position is
 // loaded while the current one is evaluated.
 template <int P, bool MULTI, bool STAGE>
-__global__ void __launch_bounds__(256) k_pmc2d_eval(Pmc2Args a)
+__global__ void __launch_bounds__(1024) k_pmc2d_eval(Pmc2Args a)
 {
     constexpr int W = P + 1;
     extern __shared__ __align__(16) double sfield[];
@@ -1017,7 +1038,7 @@ int dep_plan(int degree, bool det, int64_t n, size_t nn, DepPlan &pl)
     if (hipGetDevice(&dev) == hipSuccess)
         (void)hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
     const void *fn = dep_kernel(degree, det);
-    int nw = GEMPIC_DEP_WAVES;
+    int nw = det ? GEMPIC_DEP_WAVES : GEMPIC_DEP_WAVES_FAST;
     while (nw > 1 && dep_shmem(det, nw, nn) > (size_t)maxsh) nw /= 2;
     size_t sh = dep_shmem(det, nw, nn);
     if (sh > 65536 &&
@@ -1077,9 +1098,10 @@ int pmc2_enqueue(int op, int degree, bool deterministic, Pmc2Args a, double *rho
         const size_t ns = multi ? 2 * nn : (size_t)(a.nx + degree) * (a.ny + degree);
         const bool stage = ns <= GEMPIC_EVAL_STAGE_MAX;                 // <= 64 KiB of LDS
         const size_t sh = stage ? sizeof(double) * ns : 0;
+        const int nt = GEMPIC_EVAL_THREADS;
         const unsigned gb = (unsigned)std::max<int64_t>(
-            1, std::min<int64_t>((int64_t)cus * GEMPIC_EVAL_BLOCKS_PER_CU, (a.n + 255) / 256));
-#define EV(P, M, S) hipLaunchKernelGGL((k_pmc2d_eval<P, M, S>), dim3(gb), dim3(256), sh, s, a)
+            1, std::min<int64_t>((int64_t)cus * GEMPIC_EVAL_BLOCKS_PER_CU, (a.n + nt - 1) / nt));
+#define EV(P, M, S) hipLaunchKernelGGL((k_pmc2d_eval<P, M, S>), dim3(gb), dim3(nt), sh, s, a)
 #define EVD(M, S) do { if (degree == 1) EV(1, M, S); else if (degree == 2) EV(2, M, S); else EV(3, M, S); } while (0)
         if (multi) { if (stage) EVD(true, true); else EVD(true, false); }
         else { if (stage) EVD(false, true); else EVD(false, false); }
