@@ -796,6 +796,7 @@ struct or_split {
     double *a;               /* aliased particle array */
     double *e1, *e2, *b;     /* aliased field arrays */
     double *j1, *j2;         /* own j_dofs[1:2] */
+    double *e1m, *e2m, *bm;  /* HamiltonianSplittingBoris: e_dofs_mid, b_dofs_mid */
     or_maxwell1d *m;
     or_pmc1d ks0, ks1;
     double Lx, x_min, delta_x;
@@ -829,6 +830,9 @@ or_split *or_split_create(int64_t nx, double xmin, double xmax, int maxwell_degr
     h->e1 = e1; h->e2 = e2; h->b = b;
     h->j1 = (double *)calloc(nx, sizeof(double));
     h->j2 = (double *)calloc(nx, sizeof(double));
+    h->e1m = (double *)calloc(nx, sizeof(double));
+    h->e2m = (double *)calloc(nx, sizeof(double));
+    h->bm = (double *)calloc(nx, sizeof(double));
     h->x_min = h->m->xmin;
     h->Lx = h->m->Lx;
     h->delta_x = h->Lx / (double)nx;
@@ -847,6 +851,7 @@ void or_split_destroy(or_split *h)
     if (!h) return;
     or_maxwell1d_destroy(h->m);
     free(h->j1); free(h->j2);
+    free(h->e1m); free(h->e2m); free(h->bm);
     free(h);
 }
 
@@ -1044,4 +1049,114 @@ void or_write_step(or_split *h, double time, int degree, const double *e1, const
     out[0] = time; out[1] = d0; out[2] = d1; out[3] = d2;
     out[4] = pe1; out[5] = pe2; out[6] = pb3;
     out[7] = transfer; out[8] = vvb; out[9] = poynting; out[10] = err;
+}
+
+
+/* ------------------------------------------------------------------------ */
+/* HamiltonianSplittingBoris (hamiltonian_splitting_boris.jl), on the same   */
+/* state as or_split (dim_v = 2): e_dofs and b_dofs aliased, e_dofs_mid and  */
+/* b_dofs_mid owned (zero at construction, :50-60), j_dofs = j1, j2.         */
+/* ------------------------------------------------------------------------ */
+
+const double *or_boris_mid(const or_split *h, int which)
+{
+    return which == 0 ? h->e1m : which == 1 ? h->e2m : h->bm;
+}
+
+/* push_v_epart! (:189-204): v .+ dt * qm .* [E1_mid(x), E2_mid(x)] */
+void or_boris_push_v_epart(or_split *h, double dt)
+{
+    const double qm = h->q_over_m;
+    for (int64_t i = 0; i < h->np; ++i) {
+        const double x = PX(h, i);
+        const double e1 = or_evaluate(&h->ks1, x, h->e1m);
+        const double e2 = or_evaluate(&h->ks0, x, h->e2m);
+        const double dtqm = dt * qm;
+        PV1(h, i) = PV1(h, i) + dtqm * e1;
+        PV2(h, i) = PV2(h, i) + dtqm * e2;
+    }
+}
+
+/* push_v_bpart! (:211-233): rotation by B_mid(x) */
+void or_boris_push_v_bpart(or_split *h, double dt)
+{
+    const double qmdt = h->q_over_m * 0.5 * dt;
+    for (int64_t i = 0; i < h->np; ++i) {
+        const double v1 = PV1(h, i), v2 = PV2(h, i);
+        double b = or_evaluate(&h->ks1, PX(h, i), h->bm);
+        b = qmdt * b;
+        double m11 = 1.0 / (1.0 + b * b);
+        const double m12 = m11 * b * 2.0;
+        m11 = m11 * (1.0 - b * b);
+        PV1(h, i) = m11 * v1 + m12 * v2;
+        PV2(h, i) = -m12 * v1 + m11 * v2;
+    }
+}
+
+/* push_x_accumulate_j! (:250-288): j_dofs = midpoint deposits of q v1 (ks1)
+ * and q v2 (ks0); x = mod(x + dt v1, Lx) */
+void or_boris_push_x_accumulate_j(or_split *h, double dt)
+{
+    const int64_t nx = h->ks0.n_grid;
+    memset(h->j1, 0, sizeof(double) * nx);
+    memset(h->j2, 0, sizeof(double) * nx);
+    for (int64_t i = 0; i < h->np; ++i) {
+        const double x_old = PX(h, i), v1 = PV1(h, i), v2 = PV2(h, i);
+        const double x_new = x_old + dt * v1;
+        const double wi = h->charge * PW(h, i) * h->common_weight;   /* get_charge */
+        const double xm = (x_old + x_new) * 0.5;
+        or_add_charge(h->j1, &h->ks1, xm, wi * v1);
+        or_add_charge(h->j2, &h->ks0, xm, wi * v2);
+        PX(h, i) = or_jl_mod_f64(x_new, h->Lx);
+    }
+}
+
+/* staggering! (:99-122) */
+void or_boris_staggering(or_split *h, double dt)
+{
+    const int64_t nx = h->ks0.n_grid;
+    or_boris_push_x_accumulate_j(h, dt * 0.5);
+    for (int64_t k = 0; k < nx; ++k) {
+        h->e1m[k] = h->e1[k];
+        h->e2m[k] = h->e2[k];
+        h->j1[k] = 0.5 * dt * h->j1[k];
+        h->j2[k] = 0.5 * dt * h->j2[k];
+    }
+    or_compute_e_from_j(h->e1m, h->m, h->j1, 1);
+    or_compute_e_from_b(h->e2m, h->m, 0.5 * dt, h->b);
+    or_compute_e_from_j(h->e2m, h->m, h->j2, 2);
+}
+
+/* strang_splitting! (:132-177) */
+void or_boris_strang(or_split *h, double dt, int64_t number_steps)
+{
+    const int64_t nx = h->ks0.n_grid;
+    for (int64_t step = 0; step < number_steps; ++step) {
+        for (int64_t k = 0; k < nx; ++k) h->bm[k] = h->b[k];
+        or_compute_b_from_e(h->b, h->m, dt, h->e2m);
+        for (int64_t k = 0; k < nx; ++k) h->bm[k] = (h->bm[k] + h->b[k]) * 0.5;
+        or_boris_push_v_epart(h, 0.5 * dt);
+        or_boris_push_v_bpart(h, dt);
+        or_boris_push_v_epart(h, 0.5 * dt);
+        or_boris_push_x_accumulate_j(h, dt);
+        for (int64_t k = 0; k < nx; ++k) {
+            h->e1[k] = h->e1m[k];
+            h->e2[k] = h->e2m[k];
+            h->j1[k] = dt * h->j1[k];
+            h->j2[k] = dt * h->j2[k];
+        }
+        or_compute_e_from_j(h->e1m, h->m, h->j1, 1);
+        or_compute_e_from_b(h->e2m, h->m, dt, h->b);
+        or_compute_e_from_j(h->e2m, h->m, h->j2, 2);
+    }
+}
+
+void or_boris_set_mid(or_split *h, const double *e1m, const double *e2m, const double *bm)
+{
+    const int64_t nx = h->ks0.n_grid;
+    for (int64_t k = 0; k < nx; ++k) {
+        if (e1m) h->e1m[k] = e1m[k];
+        if (e2m) h->e2m[k] = e2m[k];
+        if (bm) h->bm[k] = bm[k];
+    }
 }

@@ -182,6 +182,17 @@ or_split *or_split_create(int64_t nx, double xmin, double xmax, int maxwell_degr
                           double mass, double common_weight, double *particles, double *e1,
                           double *e2, double *b);
 void or_split_destroy(or_split *h);
+
+/* HamiltonianSplittingBoris (hamiltonian_splitting_boris.jl) on an or_split
+ * state with dim_v = 2. which: 0 = e_dofs_mid[1], 1 = e_dofs_mid[2], 2 =
+ * b_dofs_mid. */
+const double *or_boris_mid(const or_split *h, int which);
+void or_boris_set_mid(or_split *h, const double *e1m, const double *e2m, const double *bm);
+void or_boris_push_v_epart(or_split *h, double dt);
+void or_boris_push_v_bpart(or_split *h, double dt);
+void or_boris_push_x_accumulate_j(or_split *h, double dt);
+void or_boris_staggering(or_split *h, double dt);
+void or_boris_strang(or_split *h, double dt, int64_t number_steps);
 const double *or_split_j(const or_split *h, int component);
 void or_operator_hb(or_split *h, double dt);
 void or_operator_he(or_split *h, double dt);

@@ -88,6 +88,13 @@ def lib():
             "or_operator_hp1": (None, [_VP, _D]),
             "or_operator_hp2": (None, [_VP, _D]),
             "or_strang_splitting": (None, [_VP, _D, _I64]),
+            "or_boris_mid": (_P, [_VP, _I]),
+            "or_boris_set_mid": (None, [_VP, _P, _P, _P]),
+            "or_boris_push_v_epart": (None, [_VP, _D]),
+            "or_boris_push_v_bpart": (None, [_VP, _D]),
+            "or_boris_push_x_accumulate_j": (None, [_VP, _D]),
+            "or_boris_staggering": (None, [_VP, _D]),
+            "or_boris_strang": (None, [_VP, _D, _I64]),
             "or_split_set_particles": (_I, [_VP, _P, _I64]),
             "or_solve_poisson": (None, [_P, _VP, _P]),
             "or_write_step": (None, [_VP, _D, _I, _P, _P, _P, _P, _P, _P, _P]),
@@ -346,6 +353,30 @@ class Splitting:
 
     def solve_poisson(self, efield, rho):
         lib().or_solve_poisson(_p(efield), self.h, _p(rho))
+
+    # HamiltonianSplittingBoris (hamiltonian_splitting_boris.jl) on this state
+    def boris_mid(self):
+        """(e_dofs_mid[1], e_dofs_mid[2], b_dofs_mid)"""
+        return tuple(np.ctypeslib.as_array(lib().or_boris_mid(self.h, k), shape=(self.nx,)).copy()
+                     for k in range(3))
+
+    def boris_set_mid(self, e1m, e2m, bm):
+        lib().or_boris_set_mid(self.h, _p(e1m), _p(e2m), _p(bm))
+
+    def boris_push_v_epart(self, dt):
+        lib().or_boris_push_v_epart(self.h, dt)
+
+    def boris_push_v_bpart(self, dt):
+        lib().or_boris_push_v_bpart(self.h, dt)
+
+    def boris_push_x_accumulate_j(self, dt):
+        lib().or_boris_push_x_accumulate_j(self.h, dt)
+
+    def boris_staggering(self, dt):
+        lib().or_boris_staggering(self.h, dt)
+
+    def boris_strang(self, dt, nsteps):
+        lib().or_boris_strang(self.h, dt, nsteps)
 
     def write_step(self, time, degree, e1, e2, b, e1_n, e2_n, efield_poisson):
         out = np.zeros(11)
