@@ -131,6 +131,7 @@ struct gempic_splitting {
 
     int max_blocks = 1024;            // capacity of the reduction scratch
     int dep_blocks = 0;               // workgroups of every deposit-bearing pass (0: not set)
+    int boris_blocks = 0;             // workgroups of the Boris passes
     int override_blocks = 0;          // gempic_set_launch
     int cus = 256;
     int kernels_per_step = 1;         // fused Strang step in 1 or 2 particle kernels
@@ -497,6 +498,9 @@ int gempic_splitting_create(int64_t nx, double xmin, double xmax, int maxwell_de
         return fail(GEMPIC_ERR_DEVICE, std::string("device allocation: ") + hipGetErrorString(e));
     }
     set_dep_blocks(h);
+    h->boris_blocks = std::max(1, std::min(h->max_blocks,
+                                           h->cus * boris_blocks_per_cu(h->deg0, h->deg1,
+                                                                        boris_shmem_bytes((int)h->nx))));
     *out = h;
     return GEMPIC_OK;
 }
@@ -698,6 +702,137 @@ int gempic_strang_splitting(gempic_splitting *h, double dt, int64_t n_steps, int
     }
     if (rc) return rc;
     if (flags & GEMPIC_ASYNC) return GEMPIC_OK;
+    return finish(h);
+}
+
+// ---------------------------------------------------------------------------
+// HamiltonianSplittingBoris (hamiltonian_splitting_boris.jl) on the handle's
+// state: e_dofs, b_dofs and j_dofs are the handle's fields, e_dofs_mid and
+// b_dofs_mid its S_E1M, S_E2M, S_BM slots (zero at creation, :50-60).
+// ---------------------------------------------------------------------------
+namespace {
+
+int run_boris(gempic_splitting *h, int ops, double dt_e, double dt_b, double dt_x, int kind,
+              int more, int prof_id)
+{
+    PassArgs a = pass_args(h);
+    const int target = h->override_blocks ? h->override_blocks : h->boris_blocks;
+    Launch L = plan(h->n, target);
+    a.chunk = L.chunk;
+    a.iters = L.iters;
+    a.ngroups = L.ngroups;
+    a.dtqm_he = dt_e * h->qm;              // dt * qm .* [E1, E2]
+    a.dtqm_hp2 = h->qm * 0.5 * dt_b;       // qm * 0.5 * dt
+    a.dt_hp1 = dt_x;
+    a.more = more;
+    {
+        ProfScope ps(h, prof_id);
+        hipError_t e = launch_boris(h->deg0, h->deg1, ops, a, L.blocks, boris_shmem_bytes((int)h->nx),
+                                    h->stream);
+        if (e != hipSuccess)
+            return fail(GEMPIC_ERR_DEVICE, std::string("Boris pass launch: ") + hipGetErrorString(e));
+    }
+    if (ops & BO_PUSHX) {
+        ProfScope ps(h, 9);
+        hipError_t e = launch_boris_epi(kind, a, L.blocks, h->stream);
+        if (e != hipSuccess)
+            return fail(GEMPIC_ERR_DEVICE, std::string("Boris field launch: ") + hipGetErrorString(e));
+    }
+    return GEMPIC_OK;
+}
+
+int boris_check(gempic_splitting *h)
+{
+    if (h->dim_v != 2)
+        return fail(GEMPIC_ERR_ARGUMENT, "MethodError: HamiltonianSplittingBoris needs ParticleGroup{1,2}");
+    return GEMPIC_OK;
+}
+
+}  // namespace
+
+int gempic_boris_push_v_epart(gempic_splitting *h, double dt)
+{
+    CHECK_H(h);
+    CHECK_P(h);
+    int rc = boris_check(h);
+    if (!rc) rc = run_boris(h, BO_EPART, dt, 0.0, 0.0, BE_J, 0, 2);
+    if (rc) return rc;
+    return finish(h);
+}
+
+int gempic_boris_push_v_bpart(gempic_splitting *h, double dt)
+{
+    CHECK_H(h);
+    CHECK_P(h);
+    int rc = boris_check(h);
+    if (!rc) rc = run_boris(h, BO_BPART, 0.0, dt, 0.0, BE_J, 0, 3);
+    if (rc) return rc;
+    return finish(h);
+}
+
+int gempic_boris_push_x_accumulate_j(gempic_splitting *h, double dt)
+{
+    CHECK_H(h);
+    CHECK_P(h);
+    int rc = boris_check(h);
+    if (!rc) rc = run_boris(h, BO_PUSHX, 0.0, 0.0, dt, BE_J, 0, 4);
+    if (rc) return rc;
+    return finish(h);
+}
+
+int gempic_boris_staggering(gempic_splitting *h, double dt)
+{
+    CHECK_H(h);
+    CHECK_P(h);
+    int rc = boris_check(h);
+    if (!rc) rc = run_boris(h, BO_PUSHX, 0.0, 0.0, dt * 0.5, BE_STAGGER, 0, 4);
+    if (rc) return rc;
+    return finish(h);
+}
+
+int gempic_boris_strang_splitting(gempic_splitting *h, double dt, int64_t n_steps, int flags)
+{
+    CHECK_H(h);
+    CHECK_P(h);
+    int rc = boris_check(h);
+    if (rc) return rc;
+    if (n_steps < 0) return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: number_steps must be >= 0");
+    if (n_steps == 0) return finish(h);
+    // the first step's B_mid = (B + b_from_e(B, dt, E2_mid)) / 2 (:138-144);
+    // the later steps' run at the end of the previous step's field part
+    GridProg g = prog1(G_COPY, 0.0, S_B, S_B, S_BM);
+    prog_add(g, G_HE_B, dt, S_E2M, S_E2M, S_B);
+    prog_add(g, G_AVG, 0.0, S_B, S_B, S_BM);
+    rc = run_grid(h, g);
+    const int all = BO_EPART | BO_BPART | BO_EPART2 | BO_PUSHX;
+    for (int64_t s = 0; s < n_steps && !rc; ++s) {
+        rc = maybe_sort(h);
+        if (!rc) rc = run_boris(h, all, 0.5 * dt, dt, dt, BE_STEP, s + 1 < n_steps, 0);
+        h->steps_since_sort++;
+    }
+    if (rc) return rc;
+    if (flags & GEMPIC_ASYNC) return GEMPIC_OK;
+    return finish(h);
+}
+
+int gempic_boris_set_mid(gempic_splitting *h, const double *e1_mid, const double *e2_mid,
+                         const double *b_mid)
+{
+    CHECK_H(h);
+    const size_t nb = sizeof(double) * h->nx;
+    if (e1_mid) HIP_TRY(hipMemcpyAsync(h->d_grid + S_E1M * h->nx, e1_mid, nb, hipMemcpyHostToDevice, h->stream));
+    if (e2_mid) HIP_TRY(hipMemcpyAsync(h->d_grid + S_E2M * h->nx, e2_mid, nb, hipMemcpyHostToDevice, h->stream));
+    if (b_mid) HIP_TRY(hipMemcpyAsync(h->d_grid + S_BM * h->nx, b_mid, nb, hipMemcpyHostToDevice, h->stream));
+    return finish(h);
+}
+
+int gempic_boris_get_mid(gempic_splitting *h, double *e1_mid, double *e2_mid, double *b_mid)
+{
+    CHECK_H(h);
+    const size_t nb = sizeof(double) * h->nx;
+    if (e1_mid) HIP_TRY(hipMemcpyAsync(e1_mid, h->d_grid + S_E1M * h->nx, nb, hipMemcpyDeviceToHost, h->stream));
+    if (e2_mid) HIP_TRY(hipMemcpyAsync(e2_mid, h->d_grid + S_E2M * h->nx, nb, hipMemcpyDeviceToHost, h->stream));
+    if (b_mid) HIP_TRY(hipMemcpyAsync(b_mid, h->d_grid + S_BM * h->nx, nb, hipMemcpyDeviceToHost, h->stream));
     return finish(h);
 }
 

@@ -14,7 +14,9 @@ constexpr int kGroup = 16;                // workgroups per first-level reductio
 constexpr int kEpiGroupsMax = 64;         // workgroups of the deposit-sum kernel
 
 // grid slots (nx doubles each) of a splitting's device field buffer
-enum GridSlot { S_E1 = 0, S_E2, S_B, S_E2P, S_J1, S_J2, S_RHO, S_EPOIS, S_E1N, S_E2N, NSLOT };
+// (S_E1M, S_E2M, S_BM: the Boris splitting's e_dofs_mid and b_dofs_mid)
+enum GridSlot { S_E1 = 0, S_E2, S_B, S_E2P, S_J1, S_J2, S_RHO, S_EPOIS, S_E1N, S_E2N, S_E1M, S_E2M,
+                S_BM, NSLOT };
 // constant tables (nx doubles each)
 enum TabSlot { T_COS = 0, T_SIN, T_INVM0, T_INVM1, T_AMP, T_POIS, T_M0, T_M1, NTAB };
 // deposit slots of one pass
@@ -85,6 +87,8 @@ enum GridOp : int {
     G_E_FROM_RHO,        // EPOIS from RHO           (compute_e_from_rho!)
     G_INNER_M0,          // scalar: sum(c1 .* M0 c2) dx
     G_INNER_M1,
+    G_COPY,              // out = A
+    G_AVG,               // out = (out + A) * 0.5
 };
 struct GridProg {
     int nops;
@@ -94,6 +98,20 @@ struct GridProg {
 };
 hipError_t launch_grid(const GridProg &g, int nx, double dx, double *grid, const double *tabs,
                        double *scalar_out, hipStream_t s);
+
+// HamiltonianSplittingBoris particle pass: which pushers it runs, in this
+// order (E kick with dtqm_he, B rotation with dtqm_hp2 = qm*0.5*dt, E kick,
+// x push with dt_hp1 and the midpoint deposits of j1 and j2)
+enum BorisOps : int { BO_EPART = 1, BO_BPART = 2, BO_EPART2 = 4, BO_PUSHX = 8 };
+// field part after a BO_PUSHX pass: j_dofs only (push_x_accumulate_j!), the
+// staggering!, or a strang_splitting! step (a.more: the next step's B
+// update follows)
+enum BorisEpi : int { BE_J = 0, BE_STAGGER, BE_STEP };
+hipError_t launch_boris(int p0, int p1, int ops, const PassArgs &a, int blocks, size_t shmem,
+                        hipStream_t s);
+hipError_t launch_boris_epi(int kind, const PassArgs &a, int pass_blocks, hipStream_t s);
+size_t boris_shmem_bytes(int nx);
+int boris_blocks_per_cu(int p0, int p1, size_t shmem);
 
 // Stable counting sort by (cell, sub-cell bin): bins = nx * sub, tile-major
 // counts. Sorting finer than the cell keeps each wave's particles within a

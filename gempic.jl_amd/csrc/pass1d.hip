@@ -1046,6 +1046,335 @@ hipError_t launch_pass(int p0, int p1, int mode, const PassArgs &a, int blocks, 
 }
 
 // ---------------------------------------------------------------------------
+// HamiltonianSplittingBoris (hamiltonian_splitting_boris.jl): the particle
+// work of one strang_splitting! step is one pass (push_v_epart!(dt/2),
+// push_v_bpart!(dt), push_v_epart!(dt/2), push_x_accumulate_j!(dt), :146-155):
+// the E and B kicks read the mid-step fields at the old position, then the
+// position moves and the currents are deposited at the midpoint. OPS selects
+// the pushers for the stand-alone verbs.
+// ---------------------------------------------------------------------------
+template <int P0, int P1, int OPS>
+__global__ void __launch_bounds__(kBlock) k_boris1d(PassArgs a)
+{
+    constexpr bool EPART = (OPS & BO_EPART) != 0;
+    constexpr bool BPART = (OPS & BO_BPART) != 0;
+    constexpr bool EPART2 = (OPS & BO_EPART2) != 0;
+    constexpr bool PUSHX = (OPS & BO_PUSHX) != 0;
+    constexpr bool KICK = EPART || BPART || EPART2;
+    constexpr int W0 = P0 + 1, W1 = P1 + 1;
+    static_assert(P0 <= kPad && P1 <= kPad, "LDS field padding must cover the spline support");
+    static_assert(W0 + W1 <= kSlotsPerLane, "cache-B slots");
+
+    extern __shared__ double smem[];
+    const int nx = a.nx;
+    const int tid = threadIdx.x;
+    const int wave = tid / kWave;
+    const int nxp = nx + 2 * kPad;
+    double *pE1 = smem, *pE2 = pE1 + nxp, *pB = pE2 + nxp;     // padded E1_mid, E2_mid, B_mid
+    double *rows = pB + nxp;                                     // [kWaves][NDEP][nx]
+    double *slots = rows + kWaves * NDEP * nx + wave * (kSlotsPerLane * kWave);
+    const double *sE1 = pE1 + kPad, *sE2 = pE2 + kPad, *sB = pB + kPad;
+    if constexpr (KICK) {
+        load_padded(pE1, a.grid + S_E1M * nx, nx);
+        load_padded(pE2, a.grid + S_E2M * nx, nx);
+        load_padded(pB, a.grid + S_BM * nx, nx);
+    }
+    for (int i = tid; i < kWaves * NDEP * nx; i += kBlock) rows[i] = 0.0;
+    __syncthreads();
+    double *myrow = rows + wave * NDEP * nx;
+    Cache2<W1> c_j1;
+    Cache2<W0> c_j2;
+    if constexpr (PUSHX) {
+        c_j1.init(slots);
+        c_j2.init(slots + W1 * kWave);
+    }
+
+    const double xmin = a.xmin, dx = a.dx;
+    const int64_t base = (int64_t)blockIdx.x * a.chunk;
+    const int64_t left = a.n - base;
+    const int cnt_here = left < a.chunk ? (int)left : (int)a.chunk;
+    double *__restrict__ px = a.x + base;
+    double *__restrict__ pv1 = a.v1 + base;
+    double *__restrict__ pv2 = a.v2 + base;
+    const double *__restrict__ pw = a.w + base;
+    int errs = 0;
+
+    double nx_x = 0.0, nx_v1 = 0.0, nx_v2 = 0.0, nx_w = 0.0;
+    if (a.iters > 0 && tid < cnt_here) {
+        nx_x = px[tid];
+        nx_v1 = pv1[tid];
+        nx_v2 = pv2[tid];
+        if (PUSHX) nx_w = pw[tid];
+    }
+    for (int it = 0; it < a.iters; ++it) {
+        const int i = it * kBlock + tid;
+        const bool valid = i < cnt_here;
+        double x = nx_x, v1 = nx_v1, v2 = nx_v2;
+        const double w = nx_w;
+        {
+            const int in = i + kBlock;
+            if (it + 1 < a.iters && in < cnt_here) {
+                nx_x = px[in];
+                nx_v1 = pv1[in];
+                nx_v2 = pv2[in];
+                if (PUSHX) nx_w = pw[in];
+            }
+        }
+        if constexpr (KICK) {
+            // evaluate (particle_mesh_coupling_1d.jl:438-453) at the old position
+            const double xi = (x - xmin) / dx;
+            if (valid && !pos_ok(xi)) errs |= E_INEXACT;
+            const int ic = (int)xi;
+            const double off = xi - (double)ic;
+            const int key = imod(ic, nx);
+            double b1[W1];
+            bspline<P1>(off, b1);
+            double e1 = 0.0, e2 = 0.0;
+            if constexpr (EPART || EPART2) {
+                double b0[W0];
+                bspline<P0>(off, b0);
+                e1 = gather_p<P1>(b1, sE1, key);
+                e2 = gather_p<P0>(b0, sE2, key);
+            }
+            if constexpr (EPART) {
+                // push_v_epart! (:189-204): v .+ dt * qm .* [E1, E2]
+                v1 = v1 + a.dtqm_he * e1;
+                v2 = v2 + a.dtqm_he * e2;
+            }
+            if constexpr (BPART) {
+                // push_v_bpart! (:211-233)
+                double bf = gather_p<P1>(b1, sB, key);
+                bf = a.dtqm_hp2 * bf;
+                double m11 = 1.0 / (1.0 + bf * bf);
+                const double m12 = m11 * bf * 2.0;
+                m11 = m11 * (1.0 - bf * bf);
+                const double n1 = m11 * v1 + m12 * v2;
+                const double n2 = -m12 * v1 + m11 * v2;
+                v1 = n1;
+                v2 = n2;
+            }
+            if constexpr (EPART2) {
+                v1 = v1 + a.dtqm_he * e1;
+                v2 = v2 + a.dtqm_he * e2;
+            }
+        }
+        if constexpr (PUSHX) {
+            // push_x_accumulate_j! (:250-288): add_charge! of q v1 (degree P1)
+            // and q v2 (degree P0) at the midpoint, then x = mod(x_new, Lx)
+            const double q = (w * a.charge) * a.cw;
+            const double xn = x + a.dt_hp1 * v1;
+            const double xm = (x + xn) * 0.5;
+            const double xim = (xm - xmin) / dx;
+            if (valid && !pos_ok(xim)) errs |= E_INEXACT;
+            const int icm = (int)xim;                    // trunc(Int, xi)
+            const double offm = xim - (double)icm;
+            const int keym = imod(icm, nx);
+            double c1[W1], c0[W0];
+            bspline<P1>(offm, c1);
+            bspline<P0>(offm, c0);
+            const double q1 = q * v1, q2 = q * v2;
+            double vals1[W1], vals0[W0];
+#pragma unroll
+            for (int k = 0; k < W1; ++k) vals1[k] = q1 * c1[k] * a.scaling1;
+#pragma unroll
+            for (int k = 0; k < W0; ++k) vals0[k] = q2 * c0[k] * a.scaling0;
+            cache_add<W1>(c_j1, valid, keym, vals1, myrow + D_J1 * nx, nx);
+            cache_add<W0>(c_j2, valid, keym, vals0, myrow + D_J2A * nx, nx);
+            x = jl_mod(xn, a.Lx);
+        }
+        if (valid) {
+            if constexpr (PUSHX) px[i] = x;
+            if constexpr (KICK) {
+                pv1[i] = v1;
+                pv2[i] = v2;
+            }
+        }
+    }
+    if (errs) atomicOr(a.err, errs);
+    if constexpr (PUSHX) {
+        cache_drain<W1>(c_j1, myrow + D_J1 * nx, nx);
+        cache_drain<W0>(c_j2, myrow + D_J2A * nx, nx);
+        __syncthreads();
+        constexpr unsigned MASK = (1u << D_J1) | (1u << D_J2A);
+        const int len = NDEP * nx;
+        for (int idx = tid; idx < len; idx += kBlock) {
+            if (!((MASK >> (idx / nx)) & 1u)) continue;
+            double t = 0.0;
+            for (int wv = 0; wv < kWaves; ++wv) t += rows[wv * len + idx];
+            a.part[(size_t)blockIdx.x * len + idx] = t;
+        }
+    }
+}
+
+// The field part after a Boris x push: the deposit sum over the workgroups'
+// rows (as k_epi1d: group slices, then the last workgroup), then
+//  BE_J:       j_dofs = the deposits (push_x_accumulate_j!);
+//  BE_STAGGER: staggering! (:103-121) with h = a.dt_hp1: E_mid = E,
+//              j = h j, E1_mid -= M1^-1 j1/dx, E2_mid += h W(B),
+//              E2_mid -= M0^-1 j2/dx;
+//  BE_STEP:    strang_splitting! (:158-175) with dt = a.dt_hp1: E = E_mid,
+//              j = dt j, the same E_mid updates with dt, and with a.more the
+//              next step's B_mid = (B + b_from_e(B, dt, E2_mid)) / 2 (:138-144).
+template <int KIND>
+__global__ void __launch_bounds__(kBlock) k_boris_epi(PassArgs a, int nblocks)
+{
+    extern __shared__ double smem[];
+    __shared__ int s_last;
+    constexpr unsigned REDMASK = (1u << D_J1) | (1u << D_J2A);
+    const int nx = a.nx;
+    const int tid = threadIdx.x;
+    const int nt = blockDim.x;
+    const int len = NDEP * nx;
+    const double dx = a.dx, h = a.dt_hp1;
+    double *sE1 = smem;                     // [nx] each: E1_mid (or E1), E2_mid (or E2), B, B_mid
+    double *sE2 = sE1 + nx;
+    double *sB = sE2 + nx;
+    double *sBM = sB + nx;
+    double *sRed = sBM + nx;                // [NDEP][nx]
+    double *work = sRed + NDEP * nx;
+    double *hc = work + nx;
+    double *T = hc + nx;                    // [NTAB][nx]
+    double *scratch = T + NTAB * nx;        // [blockDim.x]
+    double *G = a.grid;
+    const int se1 = KIND == BE_STAGGER ? S_E1 : S_E1M, se2 = KIND == BE_STAGGER ? S_E2 : S_E2M;
+    for (int i = tid; i < nx; i += nt) {
+        sE1[i] = G[se1 * nx + i];
+        sE2[i] = G[se2 * nx + i];
+        sB[i] = G[S_B * nx + i];
+    }
+    for (int i = tid; i < NTAB * nx; i += nt) T[i] = a.tabs[i];
+    for (int i = tid; i < len; i += nt) sRed[i] = 0.0;
+
+    const int ng = gridDim.x, g = blockIdx.x;
+    const int b0 = (int)(((long)nblocks * g) / ng), b1 = (int)(((long)nblocks * (g + 1)) / ng);
+    sum_rows_range(a.part, b0, b1, nx, REDMASK, a.gpart + (size_t)g * len, scratch);
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) s_last = atomicAdd(&a.cnt[0], 1u) == (unsigned)(ng - 1);
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    if (tid == 0) a.cnt[0] = 0u;
+    sum_rows_range(a.gpart, 0, ng, nx, REDMASK, sRed, scratch);
+    double *r_j1 = sRed + D_J1 * nx, *r_j2 = sRed + D_J2A * nx;
+    if constexpr (KIND == BE_J) {
+        for (int i = tid; i < nx; i += nt) {
+            G[S_J1 * nx + i] = r_j1[i];
+            G[S_J2 * nx + i] = r_j2[i];
+        }
+        return;
+    } else {
+        for (int i = tid; i < nx; i += nt) {
+            if (KIND == BE_STEP) {               // e_dofs .= e_dofs_mid
+                G[S_E1 * nx + i] = sE1[i];
+                G[S_E2 * nx + i] = sE2[i];
+            }
+            r_j1[i] = h * r_j1[i];
+            r_j2[i] = h * r_j2[i];
+        }
+        __syncthreads();
+        ep_e_from_j(sE1, r_j1, T + T_INVM1 * nx, dx, work, hc, T, nx);
+        ep_e_from_b(sE2, sB, h, dx, work, hc, T, nx);
+        ep_e_from_j(sE2, r_j2, T + T_INVM0 * nx, dx, work, hc, T, nx);
+        if (KIND == BE_STEP && a.more) {
+            for (int i = tid; i < nx; i += nt) sBM[i] = sB[i];
+            __syncthreads();
+            ep_b_from_e(sB, sE2, h, dx, nx);
+            for (int i = tid; i < nx; i += nt) sBM[i] = (sBM[i] + sB[i]) * 0.5;
+            __syncthreads();
+            for (int i = tid; i < nx; i += nt) {
+                G[S_B * nx + i] = sB[i];
+                G[S_BM * nx + i] = sBM[i];
+            }
+        }
+        for (int i = tid; i < nx; i += nt) {
+            G[S_J1 * nx + i] = r_j1[i];
+            G[S_J2 * nx + i] = r_j2[i];
+            G[S_E1M * nx + i] = sE1[i];
+            G[S_E2M * nx + i] = sE2[i];
+        }
+    }
+}
+
+size_t boris_shmem_bytes(int nx)
+{
+    return sizeof(double) * ((size_t)3 * (nx + 2 * kPad) + (size_t)kWaves * NDEP * nx +
+                             (size_t)kWaves * kSlotsPerLane * kWave);
+}
+
+static size_t boris_epi_shmem_bytes(int nx)
+{
+    return sizeof(double) * ((size_t)(4 + NDEP + 2 + NTAB) * nx + kBlock);
+}
+
+template <int P0, int P1>
+static hipError_t launch_boris_pd(int ops, const PassArgs &a, int blocks, size_t shmem, hipStream_t s)
+{
+#define GK_BO(O)                                                                             \
+    case (O):                                                                                \
+        if (blocks < 0) {                                                                    \
+            int nb = 0;                                                                      \
+            hipError_t e_ = hipOccupancyMaxActiveBlocksPerMultiprocessor(                    \
+                &nb, k_boris1d<P0, P1, (O)>, kBlock, shmem);                                 \
+            *(int *)a.diag = nb;                                                             \
+            return e_;                                                                       \
+        }                                                                                    \
+        hipLaunchKernelGGL((k_boris1d<P0, P1, (O)>), dim3(blocks), dim3(kBlock), shmem, s, a); \
+        return hipGetLastError();
+    switch (ops) {
+        GK_BO(BO_EPART | BO_BPART | BO_EPART2 | BO_PUSHX)
+        GK_BO(BO_PUSHX)
+        GK_BO(BO_EPART)
+        GK_BO(BO_BPART)
+    default:
+        return hipErrorInvalidValue;
+    }
+#undef GK_BO
+}
+
+hipError_t launch_boris(int p0, int p1, int ops, const PassArgs &a, int blocks, size_t shmem,
+                        hipStream_t s)
+{
+    if (a.nx > kNxMax) return hipErrorInvalidValue;
+#define GK_PAIR(A, B) \
+    if (p0 == (A) && p1 == (B)) return launch_boris_pd<A, B>(ops, a, blocks, shmem, s);
+    GK_PAIR(1, 0)
+    GK_PAIR(1, 1)
+    GK_PAIR(2, 1)
+    GK_PAIR(2, 2)
+    GK_PAIR(3, 2)
+    GK_PAIR(3, 3)
+#undef GK_PAIR
+    return hipErrorInvalidValue;
+}
+
+int boris_blocks_per_cu(int p0, int p1, size_t shmem)
+{
+    PassArgs a;
+    int nb = 0;
+    a.diag = (double *)&nb;
+    a.nx = 1;
+    if (launch_boris(p0, p1, BO_EPART | BO_BPART | BO_EPART2 | BO_PUSHX, a, -1, shmem, nullptr) !=
+            hipSuccess || nb < 1)
+        return 1;
+    return nb;
+}
+
+hipError_t launch_boris_epi(int kind, const PassArgs &a, int pass_blocks, hipStream_t s)
+{
+    if (a.nx > kNxMax || pass_blocks < 1) return hipErrorInvalidValue;
+    const dim3 grid(epi_groups(pass_blocks));
+    const size_t sh = boris_epi_shmem_bytes(a.nx);
+    switch (kind) {
+    case BE_J: hipLaunchKernelGGL(k_boris_epi<BE_J>, grid, dim3(kBlock), sh, s, a, pass_blocks); break;
+    case BE_STAGGER: hipLaunchKernelGGL(k_boris_epi<BE_STAGGER>, grid, dim3(kBlock), sh, s, a, pass_blocks); break;
+    case BE_STEP: hipLaunchKernelGGL(k_boris_epi<BE_STEP>, grid, dim3(kBlock), sh, s, a, pass_blocks); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Single-workgroup field programs (operatorHB, strang pre-step, Maxwell verbs)
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_grid1d(GridProg g, int nx, double dx, double *grid,
@@ -1079,6 +1408,12 @@ __global__ void __launch_bounds__(kBlock) k_grid1d(GridProg g, int nx, double dx
             if (tid == 0 && scalar_out) scalar_out[o] = r;
             break;
         }
+        case G_COPY:
+            for (int i = tid; i < nx; i += kBlock) O[i] = A[i];
+            break;
+        case G_AVG:
+            for (int i = tid; i < nx; i += kBlock) O[i] = (O[i] + A[i]) * 0.5;
+            break;
         default: break;
         }
         __syncthreads();

@@ -61,6 +61,13 @@ SIGNATURES = {
     "gempic_profile_reset": (_I, [_VP]),
     "gempic_set_launch": (_I, [_VP, _I]),
     "gempic_set_fusion": (_I, [_VP, _I]),
+    "gempic_boris_staggering": (_I, [_VP, _D]),
+    "gempic_boris_strang_splitting": (_I, [_VP, _D, _I64, _I]),
+    "gempic_boris_push_v_epart": (_I, [_VP, _D]),
+    "gempic_boris_push_v_bpart": (_I, [_VP, _D]),
+    "gempic_boris_push_x_accumulate_j": (_I, [_VP, _D]),
+    "gempic_boris_set_mid": (_I, [_VP, _P, _P, _P]),
+    "gempic_boris_get_mid": (_I, [_VP, _P, _P, _P]),
     "gempic_pmc1d_add_charge": (_I, [_I64, _D, _D, _I, _I, _I64, _P, _P, _P]),
     "gempic_pmc1d_add_charge_pp": (_I, [_I64, _D, _D, _I, _I, _I64, _P, _P, _P]),
     "gempic_pmc1d_evaluate": (_I, [_I64, _D, _D, _I, _I, _I64, _P, _P, _P]),

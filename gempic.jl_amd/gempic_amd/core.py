@@ -24,6 +24,8 @@ __all__ = [
     "compute_rhs_from_function_", "l2projection_", "inner_product", "l2norm_squared",
     "operatorHB", "operatorHE", "operatorHp1", "operatorHp2", "strang_splitting_",
     "solve_poisson_", "TimeHistoryDiagnostics", "write_step_", "get_charge",
+    "HamiltonianSplittingBoris", "staggering_", "push_v_epart_", "push_v_bpart_",
+    "push_x_accumulate_j_",
 ]
 
 
@@ -443,8 +445,66 @@ def strang_splitting_(h: HamiltonianSplitting, dt: float, number_steps: int,
     """strang_splitting! (hamiltonian_splitting.jl:98-108; _1d1v.jl:11-21).
     The default fused path gives the same particle states as the sub-step
     sequence (substep=True)."""
+    if isinstance(h, HamiltonianSplittingBoris):
+        # hamiltonian_splitting_boris.jl:132-177 (one particle pass per step)
+        h._run(_lib.load().gempic_boris_strang_splitting, float(dt), int(number_steps), 0)
+        return
     flags = _lib.STRANG_SUBSTEP if substep else _lib.STRANG_FUSED
     h._run(_lib.load().gempic_strang_splitting, float(dt), int(number_steps), flags)
+
+
+# ---------------------------------------------------------------------------
+# hamiltonian_splitting_boris.jl
+
+class HamiltonianSplittingBoris(HamiltonianSplitting):
+    """HamiltonianSplittingBoris(maxwell_solver, kernel_smoother_0,
+    kernel_smoother_1, particle_group, e_dofs, b_dofs)
+    (hamiltonian_splitting_boris.jl:23-88), for a ParticleGroup{1,2}.
+    e_dofs_mid and b_dofs_mid start at zero (:50-60) and live on the GPU; with
+    sync=True they are mirrored here after every call."""
+
+    def __init__(self, maxwell_solver: Maxwell1DFEM, kernel_smoother_0: ParticleMeshCoupling1D,
+                 kernel_smoother_1: ParticleMeshCoupling1D, particle_group: ParticleGroup,
+                 e_dofs, b_dofs, sync: bool = True):
+        super().__init__(1, 2, maxwell_solver, kernel_smoother_0, kernel_smoother_1,
+                         particle_group, e_dofs, b_dofs, sync=sync)
+        nx = kernel_smoother_1.n_dofs
+        self.e_dofs_mid = [np.zeros(nx), np.zeros(nx)]
+        self.b_dofs_mid = np.zeros(nx)
+        self.cell_integrals_1 = np.array([0.5, 2.0, 0.5]) / 3.0
+        self.cell_integrals_0 = np.array([1.0, 11.0, 11.0, 1.0]) / 24.0
+
+    def to_device(self, particles: bool = True, fields: bool = True):
+        super().to_device(particles, fields)
+        if fields and hasattr(self, "b_dofs_mid"):
+            check(_lib.load().gempic_boris_set_mid(self._h, ptr(self.e_dofs_mid[0]),
+                                                   ptr(self.e_dofs_mid[1]), ptr(self.b_dofs_mid)))
+
+    def to_host(self, particles: bool = True, fields: bool = True):
+        super().to_host(particles, fields)
+        if fields:
+            check(_lib.load().gempic_boris_get_mid(self._h, ptr(self.e_dofs_mid[0]),
+                                                   ptr(self.e_dofs_mid[1]), ptr(self.b_dofs_mid)))
+
+
+def staggering_(splitting: HamiltonianSplittingBoris, dt: float):
+    """staggering! (hamiltonian_splitting_boris.jl:99-122)."""
+    splitting._run(_lib.load().gempic_boris_staggering, float(dt))
+
+
+def push_v_epart_(splitting: HamiltonianSplittingBoris, dt: float):
+    """push_v_epart! (hamiltonian_splitting_boris.jl:189-204)."""
+    splitting._run(_lib.load().gempic_boris_push_v_epart, float(dt))
+
+
+def push_v_bpart_(splitting: HamiltonianSplittingBoris, dt: float):
+    """push_v_bpart! (hamiltonian_splitting_boris.jl:211-233)."""
+    splitting._run(_lib.load().gempic_boris_push_v_bpart, float(dt))
+
+
+def push_x_accumulate_j_(splitting: HamiltonianSplittingBoris, dt: float):
+    """push_x_accumulate_j! (hamiltonian_splitting_boris.jl:250-288)."""
+    splitting._run(_lib.load().gempic_boris_push_x_accumulate_j, float(dt))
 
 
 # ---------------------------------------------------------------------------

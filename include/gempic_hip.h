@@ -151,6 +151,23 @@ int gempic_set_launch(gempic_splitting *h, int blocks);
 int gempic_set_fusion(gempic_splitting *h, int kernels_per_step);
 
 /* ------------------------------------------------------------------------
+ * HamiltonianSplittingBoris (hamiltonian_splitting_boris.jl:23-88) on a
+ * splitting handle with V = 2 (the same particles, e_dofs, b_dofs and
+ * j_dofs); its e_dofs_mid and b_dofs_mid live on the handle (zero at
+ * creation, :50-60). One strang_splitting! step is one particle pass.
+ * ------------------------------------------------------------------------ */
+int gempic_boris_staggering(gempic_splitting *h, double dt);              /* staggering! :99-122 */
+int gempic_boris_strang_splitting(gempic_splitting *h, double dt, int64_t n_steps,
+                                  int flags);                             /* :132-177 */
+int gempic_boris_push_v_epart(gempic_splitting *h, double dt);            /* :189-204 */
+int gempic_boris_push_v_bpart(gempic_splitting *h, double dt);            /* :211-233 */
+int gempic_boris_push_x_accumulate_j(gempic_splitting *h, double dt);     /* :250-288 */
+/* e_dofs_mid[1], e_dofs_mid[2], b_dofs_mid (nx each; any may be NULL) */
+int gempic_boris_set_mid(gempic_splitting *h, const double *e1_mid, const double *e2_mid,
+                         const double *b_mid);
+int gempic_boris_get_mid(gempic_splitting *h, double *e1_mid, double *e2_mid, double *b_mid);
+
+/* ------------------------------------------------------------------------
  * ParticleMeshCoupling1D batched per-particle verbs over host arrays of n
  * particles, in the reference's particle order (one call = the reference's
  * loop over particles). nx/xmin/xmax/degree/smoothing as in the
