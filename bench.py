@@ -150,6 +150,51 @@ def pieces_2d(n, reps, cpu_n):
     return out
 
 
+def boris_run(G, _lib, w, steps, warmup):
+    """HamiltonianSplittingBoris on the same 1D2V Weibel particles (E1 from
+    Poisson): staggering!, then strang_splitting! steps, one fused particle
+    pass per step (56 algorithmic bytes per particle-step, as the Strang
+    pass), with HIP-event kernel times on the library's stream."""
+    n = w["particles"].shape[1]
+    nx = w["nx"]
+    mesh = G.OneDGrid(w["xmin"], w["xmax"], nx)
+    pg = G.ParticleGroup(1, 2, n, common_weight=w["common_weight"])
+    pg.array[:] = w["particles"]
+    ks0 = G.ParticleMeshCoupling1D(mesh, n, w["degree"], "galerkin")
+    ks1 = G.ParticleMeshCoupling1D(mesh, n, w["degree"] - 1, "galerkin")
+    mx = G.Maxwell1DFEM(mesh, w["degree"])
+    e1, e2, b = np.zeros(nx), np.zeros(nx), w["b0"].copy()
+    h = G.HamiltonianSplittingBoris(mx, ks0, ks1, pg, [e1, e2], b, sync=False)
+    G.solve_poisson_(e1, h, np.zeros(nx))
+    h.to_device(particles=False)
+    L = _lib.load()
+    dt = w["dt"]
+    _lib.check(L.gempic_boris_staggering(h._h, dt))
+    if warmup:
+        _lib.check(L.gempic_boris_strang_splitting(h._h, dt, warmup, _lib.ASYNC))
+    _lib.check(L.gempic_synchronize(h._h))
+    _lib.check(L.gempic_profile_enable(h._h, 10))
+    _lib.check(L.gempic_profile_reset(h._h))
+    t0 = time.perf_counter()
+    _lib.check(L.gempic_boris_strang_splitting(h._h, dt, steps, _lib.ASYNC))
+    _lib.check(L.gempic_synchronize(h._h))
+    elapsed = time.perf_counter() - t0
+    pass_ms, pass_cnt = h.profile_read(0)
+    epi_ms, epi_cnt = h.profile_read(9)
+    h.profile(False)
+    pass_s = pass_ms / pass_cnt * 1e-3 if pass_cnt else float("nan")
+    achieved = BYTES_FUSED * n / pass_s / 1e9
+    return {"workload": f"1D2V Weibel, {n} particles, HamiltonianSplittingBoris",
+            "value": n * steps / elapsed, "unit": "particle-steps/s", "steps": steps,
+            "ms_per_step": 1e3 * elapsed / steps,
+            "pass_us": pass_s * 1e6,
+            "field_part_us": (epi_ms / epi_cnt * 1e3) if epi_cnt else None,
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                         "bytes_per_particle_step": BYTES_FUSED,
+                         "kernel": "k_boris1d fused Boris step"}}
+
+
 def copy_calibration(mib):
     """Run a plain device copy of `mib` MiB five times (reads and writes mib MiB each).
 
@@ -181,6 +226,8 @@ def main():
     ap.add_argument("--substep", action="store_true", help="also time the sub-step path")
     ap.add_argument("--blocks", type=int, default=0, help="workgroups per particle pass (0 = auto)")
     ap.add_argument("--fusion", type=int, default=1, help="particle kernels per fused step (1 or 2)")
+    ap.add_argument("--boris-steps", type=int, default=200,
+                    help="steps of the Boris splitting measurement (0 = skip it)")
     ap.add_argument("--particles-2d", type=int, default=16777216,
                     help="particles of the 2D pieces measurement (0 = skip it)")
     ap.add_argument("--reps-2d", type=int, default=20)
@@ -268,6 +315,11 @@ def main():
                "sample": f"1D2V Weibel, {args.cpu_particles} particles x {args.cpu_steps} Strang "
                          f"steps ({secs:.1f} s), C restatement of the reference, one thread"}
 
+    boris = None
+    if args.boris_steps:
+        print(f"[bench] Boris splitting, {args.boris_steps} steps", file=sys.stderr, flush=True)
+        boris = boris_run(G, _lib, w, args.boris_steps, 10)
+
     p2d = None
     if args.particles_2d:
         print(f"[bench] 2D pieces, {args.particles_2d} particles", file=sys.stderr, flush=True)
@@ -297,6 +349,8 @@ def main():
         "cpu_baseline": cpu,
         "kernels": kernels,
     }
+    if boris is not None:
+        line["boris_1d2v"] = boris
     if p2d is not None:
         line["pieces_2d"] = p2d
     if substep is not None:

@@ -1234,7 +1234,8 @@ __global__ void __launch_bounds__(kBlock) k_boris_epi(PassArgs a, int nblocks)
     double *work = sRed + NDEP * nx;
     double *hc = work + nx;
     double *T = hc + nx;                    // [NTAB][nx]
-    double *scratch = T + NTAB * nx;        // [blockDim.x]
+    double *w3 = T + NTAB * nx;             // [6][nx]: outputs and workspaces of the solves
+    double *scratch = w3 + 6 * nx;          // [blockDim.x]
     double *G = a.grid;
     const int se1 = KIND == BE_STAGGER ? S_E1 : S_E1M, se2 = KIND == BE_STAGGER ? S_E2 : S_E2M;
     for (int i = tid; i < nx; i += nt) {
@@ -1273,9 +1274,22 @@ __global__ void __launch_bounds__(kBlock) k_boris_epi(PassArgs a, int nblocks)
             r_j2[i] = h * r_j2[i];
         }
         __syncthreads();
-        ep_e_from_j(sE1, r_j1, T + T_INVM1 * nx, dx, work, hc, T, nx);
-        ep_e_from_b(sE2, sB, h, dx, work, hc, T, nx);
-        ep_e_from_j(sE2, r_j2, T + T_INVM0 * nx, dx, work, hc, T, nx);
+        // the three solves (M1^-1 j1, W(B), M0^-1 j2) are independent: run
+        // them side by side, then apply them in the reference's order
+        // (compute_e_from_j!(E1), compute_e_from_b!(E2), compute_e_from_j!(E2))
+        const SolveArgs s1{T + T_INVM1 * nx, r_j1, 1.0, w3, w3 + nx};
+        const SolveArgs sb{T + T_AMP * nx, sB, 1.0, w3 + 2 * nx, w3 + 3 * nx};
+        const SolveArgs s2{T + T_INVM0 * nx, r_j2, 1.0, w3 + 4 * nx, w3 + 5 * nx};
+        circ_solve3(s1, sb, s2, T + T_COS * nx, T + T_SIN * nx, nx);
+        const double coef = h / dx;
+        for (int i = tid; i < nx; i += nt) {
+            double w1 = s1.out[i] / dx;
+            sE1[i] -= w1;
+            sE2[i] += coef * sb.out[i];
+            double w2 = s2.out[i] / dx;
+            sE2[i] -= w2;
+        }
+        __syncthreads();
         if (KIND == BE_STEP && a.more) {
             for (int i = tid; i < nx; i += nt) sBM[i] = sB[i];
             __syncthreads();
@@ -1304,7 +1318,7 @@ size_t boris_shmem_bytes(int nx)
 
 static size_t boris_epi_shmem_bytes(int nx)
 {
-    return sizeof(double) * ((size_t)(4 + NDEP + 2 + NTAB) * nx + kBlock);
+    return sizeof(double) * ((size_t)(4 + NDEP + 2 + NTAB + 6) * nx + kBlock);
 }
 
 template <int P0, int P1>
