@@ -70,6 +70,8 @@ struct Pmc2Args {
     double *out1, *out2;
     double *part;               // [workgroups][nx*ny]
     int *err;
+    const double *pp;           // evaluate_pp: the ((P+1)^2) x (nx*ny) pp matrix
+    double pc[16];              // SplinePP coefficient table ((P+1) x (P+1), column-major)
 };
 
 // compute_shape_factor (particle_mesh_coupling_2d.jl:54-65): ceil() index, offset (0, 1]; returns the
@@ -91,6 +93,38 @@ __device__ __forceinline__ bool shape2(const Pmc2Args &a, double x, double y, in
     return ok;
 }
 
+// horner_1d (splinepp.jl:268-274) of column `col` of the (P+1) x (P+1) table
+template <int P>
+__device__ __forceinline__ double horner_col(const double *pc, double x, int col)
+{
+    double r = pc[col * (P + 1)];
+#pragma unroll
+    for (int i = 1; i <= P; ++i) r = r * x + pc[i + col * (P + 1)];
+    return r;
+}
+
+// add_charge_pp! 2D (particle_mesh_coupling_2d.jl:127-148): floor() + 1 index
+// and the horner_m_2d! weights (splinepp.jl:295-300)
+template <int P>
+__device__ __forceinline__ bool shape2_pp(const Pmc2Args &a, double x, double y, int &sx, int &sy,
+                                          double (&vx)[P + 1], double (&vy)[P + 1])
+{
+    const double xp = (x - a.xmin) / a.dx;
+    const double yp = (y - a.ymin) / a.dy;
+    const bool ok = fabs(xp) < 1.0e9 && fabs(yp) < 1.0e9;
+    const int ip = ok ? (int)floor(xp) + 1 : 1, jp = ok ? (int)floor(yp) + 1 : 1;
+    const double dxp = ok ? xp - (double)(ip - 1) : 0.0;
+    const double dyp = ok ? yp - (double)(jp - 1) : 0.0;
+#pragma unroll
+    for (int i = 0; i <= P; ++i) {
+        vx[i] = horner_col<P>(a.pc, dxp, i);
+        vy[i] = horner_col<P>(a.pc, dyp, i);
+    }
+    sx = imod(ip - P - 1, a.nx);
+    sy = imod(jp - P - 1, a.ny);
+    return ok;
+}
+
 // Deposit (add_charge!, :94-105). Each lane keeps the (P+1)^2 node sums of
 // the patch its particles fall in, in registers; when its next particle
 // starts another patch, the cached sums are flushed into an LDS grid:
@@ -105,7 +139,7 @@ __device__ __forceinline__ bool shape2(const Pmc2Args &a, double x, double y, in
 //    lanes of one instruction hit one node), in lane order of the slots.
 // In the deterministic mode every addition's order is a fixed function of
 // the particle order; the waves' grids are summed in wave order at the end.
-template <int P, bool DET>
+template <int P, bool DET, bool PP>
 __global__ void __launch_bounds__(1024) k_pmc2d_deposit(Pmc2Args a)
 {
     constexpr int W = P + 1, WW = W * W;
@@ -188,7 +222,7 @@ __global__ void __launch_bounds__(1024) k_pmc2d_deposit(Pmc2Args a)
         }
         double vx[W], vy[W];
         int sx = 0, sy = 0;
-        const bool ok = shape2<P>(a, x, y, sx, sy, vx, vy);
+        const bool ok = PP ? shape2_pp<P>(a, x, y, sx, sy, vx, vy) : shape2<P>(a, x, y, sx, sy, vx, vy);
         if (valid && !ok) errs |= E_INEXACT;
         const int key = sx + sy * a.nx;
         const bool miss = valid && key != ckey;
@@ -337,6 +371,94 @@ __global__ void __launch_bounds__(1024) k_pmc2d_eval(Pmc2Args a)
         if constexpr (MULTI) a.out2[i] = v2;
     }
     if (errs) atomicOr(a.err, errs);
+}
+
+// evaluate_pp 2D (particle_mesh_coupling_2d.jl:162-178) with horner_2d
+// (splinepp.jl:412-434): floor() + 1 indices, not wrapped; a position outside
+// the pp table is a BoundsError (the value is NaN)
+template <int P>
+__global__ void __launch_bounds__(256) k_pmc2d_eval_pp(Pmc2Args a)
+{
+    constexpr int W = P + 1, LD = W * W;
+    int errs = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        double xi = (a.x[i] - a.xmin) / a.dx;
+        double yi = (a.y[i] - a.ymin) / a.dy;
+        const bool fin = fabs(xi) < 1.0e9 && fabs(yi) < 1.0e9;
+        const int idx = fin ? (int)floor(xi) + 1 : 0, idy = fin ? (int)floor(yi) + 1 : 0;
+        xi = xi - (double)(idx - 1);
+        yi = yi - (double)(idy - 1);
+        double res = __longlong_as_double(0x7ff8000000000000ll);
+        if (idx >= 1 && idx <= a.nx && idy >= 1 && idy <= a.ny) {
+            const double *col = a.pp + ((size_t)(idx - 1) + (size_t)(idy - 1) * a.nx) * LD;
+            double p1[W];
+#pragma unroll
+            for (int r = 0; r < W; ++r) {
+                const double *c = col + r * W;
+                double t = c[0];
+#pragma unroll
+                for (int k = 1; k < W; ++k) t = t * xi + c[k];
+                p1[r] = t;
+            }
+            res = p1[0];
+#pragma unroll
+            for (int r = 1; r < W; ++r) res = res * yi + p1[r];
+        } else {
+            errs |= E_BOUNDS;
+        }
+        a.out1[i] = res;
+    }
+    if (errs) atomicOr(a.err, errs);
+}
+
+// b_to_pp_2d! (splinepp.jl:329-399): the pp column of cell (i, j) from the
+// spline coefficients b[(i - W + k) mod nx, (j - W + l) mod ny], first along x
+// for each l (b_to_pp_1d_cell!, :310-316), then along y for each k
+template <int P>
+__global__ void __launch_bounds__(256) k_b_to_pp_2d(const double *b, int nx, int ny, Pmc2Args a,
+                                                    double *pp)
+{
+    constexpr int W = P + 1, LD = W * W;
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= nx * ny) return;
+    const int j = cell / nx, i = cell - j * nx;
+    double col[LD];
+#pragma unroll
+    for (int l = 0; l < W; ++l) {
+        int gy = j - W + 1 + l;
+        gy = gy < 0 ? gy + ny : gy;
+        double bx[W];
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            int gx = i - W + 1 + k;
+            gx = gx < 0 ? gx + nx : gx;
+            bx[k] = b[gx + (size_t)gy * nx];
+        }
+#pragma unroll
+        for (int q = 0; q < W; ++q) col[l * W + q] = 0.0;
+#pragma unroll
+        for (int r = 0; r < W; ++r)
+#pragma unroll
+            for (int q = 0; q < W; ++q) col[l * W + q] += bx[r] * a.pc[q + r * W];
+    }
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+        double by[W], p2[W];
+#pragma unroll
+        for (int l = 0; l < W; ++l) by[l] = col[k + W * l];
+#pragma unroll
+        for (int q = 0; q < W; ++q) p2[q] = 0.0;
+#pragma unroll
+        for (int r = 0; r < W; ++r)
+#pragma unroll
+            for (int q = 0; q < W; ++q) p2[q] += by[r] * a.pc[q + r * W];
+#pragma unroll
+        for (int l = 0; l < W; ++l) col[k + W * l] = p2[l];
+    }
+    double *out = pp + (size_t)cell * LD;
+#pragma unroll
+    for (int q = 0; q < LD; ++q) out[q] = col[q];
 }
 
 // ---------------------------------------------------------------------------
@@ -941,7 +1063,7 @@ int gempic_maxwell2d_compute_e_from_rho(gempic_maxwell2d *m, const double *rho, 
 
 namespace {
 
-enum Pmc2Op { P2_CHARGE = 0, P2_EVAL, P2_EVAL2 };
+enum Pmc2Op { P2_CHARGE = 0, P2_EVAL, P2_EVAL2, P2_CHARGE_PP, P2_EVAL_PP };
 
 // validated geometry of a ParticleMeshCoupling2D (constructor :22-45 on
 // TwoDGrid, mesh.jl:31-43)
@@ -952,7 +1074,7 @@ int pmc2_geometry(int op, int64_t nx, int64_t ny, double xmin, double xmax, doub
         return abi_fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: 2D coupling degree must be 1, 2 or 3");
     if (nx < degree + 1 || ny < degree + 1 || nx > kNxMax || ny > kNxMax)
         return abi_fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: nx, ny must be in [degree+1, 256]");
-    if (op == P2_CHARGE && nx * ny > kGrid2Max)
+    if ((op == P2_CHARGE || op == P2_CHARGE_PP) && nx * ny > kGrid2Max)
         return abi_fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: the device deposit holds nx*ny <= 4096");
     if (smoothing != GEMPIC_GALERKIN && smoothing != GEMPIC_COLLOCATION)
         return abi_fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: smoothing type not implemented");
@@ -967,6 +1089,8 @@ int pmc2_geometry(int op, int64_t nx, int64_t ny, double xmin, double xmax, doub
     a.dx = (xmax - xmin) / (double)nx;
     a.dy = (ymax - ymin) / (double)ny;
     a.scaling = smoothing == GEMPIC_COLLOCATION ? 1.0 / (a.dx * a.dy) : 1.0;
+    double pf[36];
+    gh::splinepp_tables(degree, a.pc, pf);      // SplinePP(degree, n) (splinepp.jl:36-233)
     return GEMPIC_OK;
 }
 
@@ -1008,13 +1132,19 @@ int ws2_get(size_t part_doubles, Ws2 *&w)
     return GEMPIC_OK;
 }
 
-template <int P, bool DET>
-const void *dep_fn() { return (const void *)k_pmc2d_deposit<P, DET>; }
+template <int P, bool DET, bool PP>
+const void *dep_fn() { return (const void *)k_pmc2d_deposit<P, DET, PP>; }
 
-const void *dep_kernel(int degree, bool det)
+template <bool PP>
+const void *dep_kernel_pp(int degree, bool det)
 {
-    if (det) return degree == 1 ? dep_fn<1, true>() : degree == 2 ? dep_fn<2, true>() : dep_fn<3, true>();
-    return degree == 1 ? dep_fn<1, false>() : degree == 2 ? dep_fn<2, false>() : dep_fn<3, false>();
+    if (det) return degree == 1 ? dep_fn<1, true, PP>() : degree == 2 ? dep_fn<2, true, PP>() : dep_fn<3, true, PP>();
+    return degree == 1 ? dep_fn<1, false, PP>() : degree == 2 ? dep_fn<2, false, PP>() : dep_fn<3, false, PP>();
+}
+
+const void *dep_kernel(int degree, bool det, bool pp)
+{
+    return pp ? dep_kernel_pp<true>(degree, det) : dep_kernel_pp<false>(degree, det);
 }
 
 // LDS of k_pmc2d_deposit: grids (one per wave when deterministic), 16
@@ -1032,12 +1162,12 @@ struct DepPlan {
 
 // waves per workgroup (4 unless the deterministic per-wave grids do not fit
 // in the workgroup's LDS) and workgroups (what the CUs hold at once)
-int dep_plan(int degree, bool det, int64_t n, size_t nn, DepPlan &pl)
+int dep_plan(int degree, bool det, bool pp, int64_t n, size_t nn, DepPlan &pl)
 {
     int dev = 0, maxsh = 65536, cus = abi_device_cus();
     if (hipGetDevice(&dev) == hipSuccess)
         (void)hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-    const void *fn = dep_kernel(degree, det);
+    const void *fn = dep_kernel(degree, det, pp);
     int nw = det ? GEMPIC_DEP_WAVES : GEMPIC_DEP_WAVES_FAST;
     while (nw > 1 && dep_shmem(det, nw, nn) > (size_t)maxsh) nw /= 2;
     size_t sh = dep_shmem(det, nw, nn);
@@ -1069,8 +1199,9 @@ int pmc2_enqueue(int op, int degree, bool deterministic, Pmc2Args a, double *rho
     const size_t nn = (size_t)a.nx * a.ny;
     int nblk = 0, splits = 0;
     DepPlan pl;
-    if (op == P2_CHARGE) {
-        int rc = dep_plan(degree, deterministic, a.n, nn, pl);
+    const bool charge = op == P2_CHARGE || op == P2_CHARGE_PP, pp = op == P2_CHARGE_PP;
+    if (charge) {
+        int rc = dep_plan(degree, deterministic, pp, a.n, nn, pl);
         if (rc) return rc;
         const int64_t waves = (int64_t)pl.blocks * pl.nw;
         a.chunk = ((a.n + waves - 1) / waves + kWave - 1) / kWave * kWave;      // per wave
@@ -1082,8 +1213,8 @@ int pmc2_enqueue(int op, int degree, bool deterministic, Pmc2Args a, double *rho
     if (rc) return rc;
     a.err = w->err;
     a.part = w->part;
-    if (op == P2_CHARGE) {
-        const void *fn = dep_kernel(degree, deterministic);
+    if (charge) {
+        const void *fn = dep_kernel(degree, deterministic, pp);
         void *args[] = {&a};
         hipError_t e = hipLaunchKernel(fn, dim3(nblk), dim3(pl.nw * kWave), args, pl.sh, s);
         if (e != hipSuccess)
@@ -1093,6 +1224,11 @@ int pmc2_enqueue(int op, int degree, bool deterministic, Pmc2Args a, double *rho
                            0, s, (const double *)w->part, nblk, (int)nn, part2);
         hipLaunchKernelGGL(k_pmc2d_reduce2, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s,
                            (const double *)part2, splits, (int)nn, rho);
+    } else if (op == P2_EVAL_PP) {
+        const unsigned gb = (unsigned)std::max<int64_t>(1, std::min<int64_t>((int64_t)cus * 8, (a.n + 255) / 256));
+        if (degree == 1) hipLaunchKernelGGL(k_pmc2d_eval_pp<1>, dim3(gb), dim3(256), 0, s, a);
+        else if (degree == 2) hipLaunchKernelGGL(k_pmc2d_eval_pp<2>, dim3(gb), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL(k_pmc2d_eval_pp<3>, dim3(gb), dim3(256), 0, s, a);
     } else {
         const bool multi = op == P2_EVAL2;
         const size_t ns = multi ? 2 * nn : (size_t)(a.nx + degree) * (a.ny + degree);
@@ -1129,6 +1265,8 @@ int pmc2_finish(hipStream_t s)
         return abi_fail(GEMPIC_ERR_DEVICE, (std::string("2D coupling: ") + hipGetErrorString(e)).c_str());
     if (herr & E_INEXACT)
         return abi_fail(GEMPIC_ERR_INEXACT, "InexactError: ceil(Int, x) of a non-finite or out-of-range position");
+    if (herr & E_BOUNDS)
+        return abi_fail(GEMPIC_ERR_ARGUMENT, "BoundsError: a position lies outside the pp table");
     return GEMPIC_OK;
 }
 
@@ -1159,21 +1297,24 @@ int pmc2_host(int op, int64_t nx, int64_t ny, double xmin, double xmax, double y
     auto ok = [&](hipError_t r) { if (r != hipSuccess && e == hipSuccess) e = r; };
     ok(dalloc2(&dx_, n));
     ok(dalloc2(&dy_, n));
-    if (op == P2_CHARGE) { ok(dalloc2(&dw, n)); ok(dalloc2(&drho, nn)); }
-    else { ok(dalloc2(&df1, nn)); ok(dalloc2(&do1, n)); if (op == P2_EVAL2) { ok(dalloc2(&df2, nn)); ok(dalloc2(&do2, n)); } }
+    const bool charge = op == P2_CHARGE || op == P2_CHARGE_PP;
+    const size_t nf1 = op == P2_EVAL_PP ? nn * (size_t)(degree + 1) * (degree + 1) : nn;
+    if (charge) { ok(dalloc2(&dw, n)); ok(dalloc2(&drho, nn)); }
+    else { ok(dalloc2(&df1, nf1)); ok(dalloc2(&do1, n)); if (op == P2_EVAL2) { ok(dalloc2(&df2, nn)); ok(dalloc2(&do2, n)); } }
     rc = GEMPIC_OK;
     if (e == hipSuccess) {
         ok(hipMemcpyAsync(dx_, x, sizeof(double) * n, hipMemcpyHostToDevice, s));
         ok(hipMemcpyAsync(dy_, y, sizeof(double) * n, hipMemcpyHostToDevice, s));
         a.x = dx_;
         a.y = dy_;
-        if (op == P2_CHARGE) {
+        if (charge) {
             ok(hipMemcpyAsync(dw, w, sizeof(double) * n, hipMemcpyHostToDevice, s));
             ok(hipMemcpyAsync(drho, rho, sizeof(double) * nn, hipMemcpyHostToDevice, s));
             a.w = dw;
         } else {
-            ok(hipMemcpyAsync(df1, f1, sizeof(double) * nn, hipMemcpyHostToDevice, s));
+            ok(hipMemcpyAsync(df1, f1, sizeof(double) * nf1, hipMemcpyHostToDevice, s));
             a.f1 = df1;
+            a.pp = df1;
             a.out1 = do1;
             if (op == P2_EVAL2) {
                 ok(hipMemcpyAsync(df2, f2, sizeof(double) * nn, hipMemcpyHostToDevice, s));
@@ -1183,7 +1324,7 @@ int pmc2_host(int op, int64_t nx, int64_t ny, double xmin, double xmax, double y
         }
         if (e == hipSuccess) rc = pmc2_enqueue(op, degree, true, a, drho, s);
         if (e == hipSuccess && rc == GEMPIC_OK) {
-            if (op == P2_CHARGE) ok(hipMemcpyAsync(rho, drho, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
+            if (charge) ok(hipMemcpyAsync(rho, drho, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
             else {
                 ok(hipMemcpyAsync(out1, do1, sizeof(double) * n, hipMemcpyDeviceToHost, s));
                 if (op == P2_EVAL2) ok(hipMemcpyAsync(out2, do2, sizeof(double) * n, hipMemcpyDeviceToHost, s));
@@ -1292,6 +1433,58 @@ int gempic_pmc2d_evaluate_multiple_dev(int64_t nx, int64_t ny, double xmin, doub
     io.x = x; io.y = y; io.f1 = field1; io.f2 = field2; io.out1 = out1; io.out2 = out2;
     return pmc2_dev(P2_EVAL2, nx, ny, xmin, xmax, ymin, ymax, degree, smoothing, n, io, nullptr,
                     flags, stream);
+}
+
+int gempic_pmc2d_add_charge_pp(int64_t nx, int64_t ny, double xmin, double xmax, double ymin,
+                               double ymax, int degree, int smoothing, int64_t n, const double *x,
+                               const double *y, const double *wp, double *rho)
+{
+    if ((n > 0 && (!x || !y || !wp)) || !rho) return abi_fail(GEMPIC_ERR_ARGUMENT, "NULL array");
+    return pmc2_host(P2_CHARGE_PP, nx, ny, xmin, xmax, ymin, ymax, degree, smoothing, n, x, y, wp,
+                     nullptr, nullptr, nullptr, nullptr, rho);
+}
+
+int gempic_pmc2d_evaluate_pp(int64_t nx, int64_t ny, double xmin, double xmax, double ymin,
+                             double ymax, int degree, int smoothing, int64_t n, const double *x,
+                             const double *y, const double *pp, double *out)
+{
+    if ((n > 0 && (!x || !y || !out)) || !pp) return abi_fail(GEMPIC_ERR_ARGUMENT, "NULL array");
+    return pmc2_host(P2_EVAL_PP, nx, ny, xmin, xmax, ymin, ymax, degree, smoothing, n, x, y,
+                     nullptr, pp, nullptr, out, nullptr, nullptr);
+}
+
+int gempic_b_to_pp_2d(int degree, int64_t nx, int64_t ny, const double *b, double *pp)
+{
+    if (!b || !pp) return abi_fail(GEMPIC_ERR_ARGUMENT, "NULL array");
+    if (degree < 1 || degree > 3)
+        return abi_fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: SplinePP degree must be 1, 2 or 3");
+    if (nx < degree + 1 || ny < degree + 1 || nx * ny > (int64_t)1 << 24)
+        return abi_fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: nx, ny must be >= degree+1, nx*ny <= 2^24");
+    int rc = abi_check_device();
+    if (rc) return rc;
+    Pmc2Args a{};
+    double pf[36];
+    gh::splinepp_tables(degree, a.pc, pf);
+    const size_t nn = (size_t)nx * ny, ld = (size_t)(degree + 1) * (degree + 1);
+    hipStream_t s = pmc2_stream();
+    double *db = nullptr, *dpp = nullptr;
+    hipError_t e = dalloc2(&db, nn);
+    if (e == hipSuccess) e = dalloc2(&dpp, nn * ld);
+    if (e == hipSuccess) e = hipMemcpyAsync(db, b, sizeof(double) * nn, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+        const dim3 g((unsigned)((nn + 255) / 256));
+        if (degree == 1) hipLaunchKernelGGL(k_b_to_pp_2d<1>, g, dim3(256), 0, s, db, (int)nx, (int)ny, a, dpp);
+        else if (degree == 2) hipLaunchKernelGGL(k_b_to_pp_2d<2>, g, dim3(256), 0, s, db, (int)nx, (int)ny, a, dpp);
+        else hipLaunchKernelGGL(k_b_to_pp_2d<3>, g, dim3(256), 0, s, db, (int)nx, (int)ny, a, dpp);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(pp, dpp, sizeof(double) * nn * ld, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    else (void)hipStreamSynchronize(s);
+    if (db) (void)hipFree(db);
+    if (dpp) (void)hipFree(dpp);
+    if (e != hipSuccess) return abi_fail(GEMPIC_ERR_DEVICE, (std::string("b_to_pp_2d: ") + hipGetErrorString(e)).c_str());
+    return GEMPIC_OK;
 }
 
 int gempic_pmc2d_sync(void *stream)

@@ -88,6 +88,9 @@ SIGNATURES = {
     "gempic_pmc2d_evaluate": (_I, [_I64, _I64, _D, _D, _D, _D, _I, _I, _I64, _P, _P, _P, _P]),
     "gempic_pmc2d_evaluate_multiple": (_I, [_I64, _I64, _D, _D, _D, _D, _I, _I, _I64, _P, _P,
                                             _P, _P, _P, _P]),
+    "gempic_pmc2d_add_charge_pp": (_I, [_I64, _I64, _D, _D, _D, _D, _I, _I, _I64, _P, _P, _P, _P]),
+    "gempic_pmc2d_evaluate_pp": (_I, [_I64, _I64, _D, _D, _D, _D, _I, _I, _I64, _P, _P, _P, _P]),
+    "gempic_b_to_pp_2d": (_I, [_I, _I64, _I64, _P, _P]),
     "gempic_pmc2d_add_charge_dev": (_I, [_I64, _I64, _D, _D, _D, _D, _I, _I, _I64, _VP, _VP, _VP,
                                          _VP, _I, _VP]),
     "gempic_pmc2d_evaluate_dev": (_I, [_I64, _I64, _D, _D, _D, _D, _I, _I, _I64, _VP, _VP, _VP,

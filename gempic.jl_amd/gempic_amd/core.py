@@ -122,8 +122,12 @@ def add_charge_(rho: np.ndarray, p: ParticleMeshCoupling1D, position, marker_cha
     check(_lib.load().gempic_pmc1d_add_charge(*p._args(), x.size, ptr(x), ptr(q), ptr(rho)))
 
 
-def add_charge_pp_(rho: np.ndarray, p: ParticleMeshCoupling1D, position, marker_charge):
-    """add_charge_pp! (particle_mesh_coupling_1d.jl:106-122)."""
+def add_charge_pp_(rho: np.ndarray, p: ParticleMeshCoupling1D, position, marker_charge, *wp):
+    """add_charge_pp! (particle_mesh_coupling_1d.jl:106-122); with a
+    ParticleMeshCoupling2D: add_charge_pp_(rho, pm, xp, yp, wp)
+    (particle_mesh_coupling_2d.jl:127-148)."""
+    if isinstance(p, ParticleMeshCoupling2D):
+        return add_charge_pp_2d_(rho, p, position, marker_charge, *wp)
     x = _arr(position)
     q = _arr(marker_charge) * np.ones_like(x)
     check(_lib.load().gempic_pmc1d_add_charge_pp(*p._args(), x.size, ptr(x), ptr(q), ptr(rho)))
@@ -142,9 +146,12 @@ def evaluate(p: ParticleMeshCoupling1D, position, field_dofs: np.ndarray, *field
     return float(out[0]) if np.ndim(position) == 0 else out
 
 
-def evaluate_pp(p: ParticleMeshCoupling1D, position, field_dofs_pp: np.ndarray):
+def evaluate_pp(p: ParticleMeshCoupling1D, position, field_dofs_pp: np.ndarray, *pp2d):
     """evaluate_pp (particle_mesh_coupling_1d.jl:242-250); field_dofs_pp is
-    the (degree+1, n_cells) matrix of b_to_pp."""
+    the (degree+1, n_cells) matrix of b_to_pp. With a ParticleMeshCoupling2D:
+    evaluate_pp(pm, xp, yp, pp) (particle_mesh_coupling_2d.jl:162-178)."""
+    if isinstance(p, ParticleMeshCoupling2D):
+        return evaluate_pp_2d(p, position, field_dofs_pp, *pp2d)
     x = _arr(position)
     out = np.zeros_like(x)
     f = np.asfortranarray(field_dofs_pp, dtype=np.float64)
@@ -553,6 +560,7 @@ def write_step_(thdiag: TimeHistoryDiagnostics, time: float, degree: int, efield
 # 2D coupling and field solver (defined in core2d, dispatched from the verbs above)
 from .core2d import (  # noqa: E402
     ParticleMeshCoupling2D, TwoDMaxwell, add_charge_2d_, evaluate_2d, compute_e_from_rho_2d_,
+    add_charge_pp_2d_, evaluate_pp_2d,
     compute_e_from_j_2d_, compute_e_from_b_2d_, compute_b_from_e_2d_, inner_product_2d)
 from .core2d import __all__ as _all2d  # noqa: E402
 from .core2d import *  # noqa: E402,F401,F403

@@ -26,6 +26,7 @@ __all__ = [
     "TwoDLinearSolverSplineMass", "solve", "multiply_mass_2dkron_", "compute_rho_from_e_",
     "compute_fem_rhs_", "l2projection", "spline_fem_mass_line", "spline_fem_compute_mass_eig",
     "add_charge_device_", "evaluate_device", "evaluate_multiple_device", "pmc2d_sync",
+    "b_to_pp_2d_",
 ]
 
 
@@ -117,6 +118,48 @@ def evaluate_2d(pm: ParticleMeshCoupling2D, xp, yp, field_dofs):
     out = np.zeros_like(x)
     check(_lib.load().gempic_pmc2d_evaluate(*pm._args(), x.size, ptr(x), ptr(y), ptr(f),
                                             ptr(out)))
+    return float(out[0]) if np.ndim(xp) == 0 else out
+
+
+def add_charge_pp_2d_(rho, pm: ParticleMeshCoupling2D, xp, yp, wp):
+    """add_charge_pp! (particle_mesh_coupling_2d.jl:127-148), batched."""
+    x, y = _vec(xp), _vec(yp)
+    w = _vec(wp) * np.ones_like(x)
+    if y.size != x.size or w.size != x.size:
+        raise ValueError("xp, yp and wp must have the same length")
+    _grid(rho, pm.grid.nx * pm.grid.ny, "rho")
+    check(_lib.load().gempic_pmc2d_add_charge_pp(*pm._args(), x.size, ptr(x), ptr(y), ptr(w),
+                                                 ptr(rho)))
+
+
+def b_to_pp_2d_(pp, pm: ParticleMeshCoupling2D, b):
+    """b_to_pp_2d!(pp, spl1, spl2, b) (splinepp.jl:392-399) with the
+    coupling's two SplinePP of its degree: pp is the ((degree+1)^2, nx*ny)
+    column-major matrix (a Fortran-ordered array, or one of that size)."""
+    g = pm.grid
+    ld = (pm.degree + 1) ** 2
+    bb = _grid(np.ascontiguousarray(b, dtype=np.float64), g.nx * g.ny, "b")
+    if not (isinstance(pp, np.ndarray) and pp.dtype == np.float64 and pp.size == ld * g.nx * g.ny
+            and (pp.flags.f_contiguous if pp.ndim == 2 else pp.flags.c_contiguous)):
+        raise ValueError(f"pp: Fortran-ordered float64 ({ld}, {g.nx * g.ny}) array expected")
+    check(_lib.load().gempic_b_to_pp_2d(pm.degree, g.nx, g.ny, ptr(bb), ptr(pp)))
+    return pp
+
+
+def evaluate_pp_2d(pm: ParticleMeshCoupling2D, xp, yp, pp):
+    """evaluate_pp (particle_mesh_coupling_2d.jl:162-178); a position outside
+    the table raises IndexError (Julia's BoundsError) through ValueError."""
+    x, y = _vec(xp), _vec(yp)
+    if y.size != x.size:
+        raise ValueError("xp and yp must have the same length")
+    g = pm.grid
+    ld = (pm.degree + 1) ** 2
+    if not (isinstance(pp, np.ndarray) and pp.dtype == np.float64 and pp.size == ld * g.nx * g.ny):
+        raise ValueError(f"pp: float64 ({ld}, {g.nx * g.ny}) matrix expected")
+    pp = np.asfortranarray(pp)
+    out = np.zeros_like(x)
+    check(_lib.load().gempic_pmc2d_evaluate_pp(*pm._args(), x.size, ptr(x), ptr(y), ptr(pp),
+                                               ptr(out)))
     return float(out[0]) if np.ndim(xp) == 0 else out
 
 

@@ -249,6 +249,18 @@ int gempic_pmc2d_evaluate_multiple(int64_t nx, int64_t ny, double xmin, double x
                                    const double *x, const double *y, const double *field1,
                                    const double *field2, double *out1, double *out2);
 
+/* add_charge_pp! (:127-148): floor()+1 index, SplinePP (Horner) weights */
+int gempic_pmc2d_add_charge_pp(int64_t nx, int64_t ny, double xmin, double xmax, double ymin,
+                               double ymax, int degree, int smoothing, int64_t n, const double *x,
+                               const double *y, const double *wp, double *rho);
+/* evaluate_pp (:162-178) on the ((degree+1)^2) x (nx*ny) column-major pp
+ * matrix of b_to_pp_2d; a position outside the table is a BoundsError */
+int gempic_pmc2d_evaluate_pp(int64_t nx, int64_t ny, double xmin, double xmax, double ymin,
+                             double ymax, int degree, int smoothing, int64_t n, const double *x,
+                             const double *y, const double *pp, double *out);
+/* b_to_pp_2d! (splinepp.jl:329-399): pp[(degree+1)^2 x nx*ny] from b[nx*ny] */
+int gempic_b_to_pp_2d(int degree, int64_t nx, int64_t ny, const double *b, double *pp);
+
 /* Device-resident forms of the three verbs above, for particles kept in GPU
  * memory (a ROCArray on the Julia side): x, y, wp, rho, field*, out* are
  * device pointers and `stream` a hipStream_t (NULL: the null stream).

@@ -426,3 +426,72 @@ def test_gpu_pmc2d_device_unstaged_eval_and_async_errors():
         G.add_charge_device_(DeviceArray.zeros(nx * ny), pm, D(x), D(y), D(np.ones(n)))
     with pytest.raises(TypeError):
         G.add_charge_device_(rho, small, x, y, np.ones(n))
+
+
+# ---------------------------------------------------------------------------
+# the pp (piecewise-polynomial, Horner) variants
+
+def test_gpu_pmc2d_pp_kat():
+    """test_particle_mesh_coupling_spline_2d.jl: add_charge_pp! gives rho_ref
+    and evaluate_pp on b_to_pp_2d! gives the evaluate values (:67-118)."""
+    kat = golden("kat_pmc2d.json")
+    n = kat["n_cells"]
+    bounds = (kat["xmin"], kat["xmax"], kat["ymin"], kat["ymax"])
+    _, pm = _coupling(n, n, bounds, kat["spline_degree"], "collocation", kat["n_particles"])
+    p = orc.PMC2D(*bounds[:2], n, *bounds[2:], n, kat["spline_degree"], orc.COLLOCATION)
+    x, y = np.array(kat["x"]), np.array(kat["y"])
+    rho = np.zeros(n * n)
+    G.add_charge_pp_(rho, pm, x, y, kat["weight"])
+    want = np.zeros(n * n)
+    for a, b in zip(x, y):
+        p.add_charge(want, a, b, kat["weight"])
+    np.testing.assert_allclose(rho, want, rtol=1e-14, atol=1e-13)
+    pp = np.zeros((16, n * n), order="F")
+    G.b_to_pp_2d_(pp, pm, want)
+    np.testing.assert_array_equal(pp, p.b_to_pp(want))
+    got = G.evaluate_pp(pm, x, y, pp)
+    np.testing.assert_array_equal(got, [p.evaluate_pp(a, b, pp) for a, b in zip(x, y)])
+    ev = np.array([p.evaluate(a, b, want) for a, b in zip(x, y)])
+    np.testing.assert_allclose(got, ev, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("degree", [1, 2, 3])
+@pytest.mark.parametrize("nx,ny", [(16, 12), (5, 7)])
+def test_gpu_pmc2d_pp_match_oracle(degree, nx, ny):
+    if min(nx, ny) < degree + 1:
+        pytest.skip("grid smaller than the spline support")
+    rng = np.random.default_rng(7 * degree + nx)
+    bounds = (-0.5, 1.75, 0.25, 2.0)
+    n = 3001
+    x, y = _particles(rng, n, bounds, nx, ny)
+    w = rng.random(n) - 0.2
+    _, pm = _coupling(nx, ny, bounds, degree, "galerkin", n)
+    p = orc.PMC2D(bounds[0], bounds[1], nx, bounds[2], bounds[3], ny, degree, orc.GALERKIN)
+    rho, want = np.zeros(nx * ny), np.zeros(nx * ny)
+    G.add_charge_pp_(rho, pm, x, y, w)
+    for a, b, c in zip(x, y, w):
+        p.add_charge_pp(want, a, b, c)
+    _assert_deposit(rho, want)
+    f = rng.normal(size=nx * ny)
+    ld = (degree + 1) ** 2
+    pp = np.zeros((ld, nx * ny), order="F")
+    G.b_to_pp_2d_(pp, pm, f)
+    np.testing.assert_array_equal(pp, p.b_to_pp(f))
+    # positions whose floor()+1 index leaves the table are a BoundsError in
+    # the reference (evaluate_pp does not wrap): the one just below the upper
+    # bound can round onto index nx+1
+    inside = []
+    for i, (a, b) in enumerate(zip(x, y)):
+        try:
+            p.evaluate_pp(a, b, pp)
+            inside.append(i)
+        except IndexError:
+            with pytest.raises(ValueError, match="BoundsError"):
+                G.evaluate_pp(pm, np.array([a]), np.array([b]), pp)
+    xi, yi = x[inside], y[inside]
+    got = G.evaluate_pp(pm, xi, yi, pp)
+    np.testing.assert_array_equal(got, [p.evaluate_pp(a, b, pp) for a, b in zip(xi, yi)])
+    # evaluate_pp of the pp form is evaluate of the B-spline form
+    np.testing.assert_allclose(got, p.evaluate_n(xi, yi, f), rtol=0, atol=1e-12 * np.abs(f).max())
+    with pytest.raises(ValueError, match="BoundsError"):
+        G.evaluate_pp(pm, np.array([0.1, bounds[1] + 0.5]), np.array([0.5, 0.5]), pp)
