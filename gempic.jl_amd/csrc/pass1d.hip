@@ -800,11 +800,19 @@ __global__ void __launch_bounds__(kBlock) k_epi1d(PassArgs a, int nblocks)
             G[S_E2P * nx + i] = sE2[i];
         }
         ep_b_from_e(sB, sE2, a.dt_he, dx, nx);
-        ep_e_from_b(sE2, sB, a.dt_he, dx, work, hc, T, nx);
-        if (a.more) {
-            ep_e_from_b(sE2, sB, a.dt_he, dx, work, hc, T, nx);
-            ep_b_from_e(sB, sE2, a.dt_he, dx, nx);
+        // HB of this step and (a.more) of the next one: compute_e_from_b! twice
+        // with the same B, so the weak curl W(B) is solved once and added twice
+        circ_solve(T + T_AMP * nx, sB, work, hc, T + T_COS * nx, T + T_SIN * nx, nx);
+        {
+            const double coef = a.dt_he / dx;
+            for (int i = tid; i < nx; i += nt) {
+                double e = sE2[i] + coef * work[i];
+                if (a.more) e = e + coef * work[i];
+                sE2[i] = e;
+            }
+            __syncthreads();
         }
+        if (a.more) ep_b_from_e(sB, sE2, a.dt_he, dx, nx);
         for (int i = tid; i < nx; i += nt) {
             G[S_E1 * nx + i] = sE1[i];
             G[S_E2 * nx + i] = sE2[i];
