@@ -544,6 +544,78 @@ int gempic_set_particles(gempic_splitting *h, const double *aos, int64_t ld, int
     return finish(h);
 }
 
+namespace {
+
+int sample_args(int dim_v, int64_t n, double alpha, double k, double sigma, double dimx)
+{
+    if (dim_v != 1 && dim_v != 2)
+        return fail(GEMPIC_ERR_ARGUMENT, "MethodError: sample! is defined for ParticleGroup{1,1} and {1,2}");
+    if (n < 1) return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: n_particles must be >= 1");
+    if (!(k > 0.0) || !(alpha >= 0.0 && alpha < 1.0) || !(sigma >= 0.0) || !(dimx > 0.0))
+        return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: need k > 0, 0 <= alpha < 1, sigma >= 0, dimx > 0");
+    if (dim_v == 2) {
+        // @assert mesh.dimx ≈ 2π / k (particle_sampling.jl:297)
+        const double L = 6.283185307179586 / k;
+        if (!(std::fabs(dimx - L) <= 1.4901161193847656e-8 * std::max(std::fabs(dimx), std::fabs(L))))
+            return fail(GEMPIC_ERR_ASSERTION, "AssertionError: mesh.dimx ≈ 2π / k");
+    }
+    return GEMPIC_OK;
+}
+
+}  // namespace
+
+int gempic_sample_landau(int dim_v, int64_t n, double alpha, double k, double sigma, double dimx,
+                         double *aos, int64_t ld)
+{
+    int rc = sample_args(dim_v, n, alpha, k, sigma, dimx);
+    if (rc) return rc;
+    if (!aos) return fail(GEMPIC_ERR_ARGUMENT, "particle array is NULL");
+    if (ld < 2 + dim_v) return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: ld must be >= 2 + V");
+    rc = check_device();
+    if (rc) return rc;
+    hipStream_t s = nullptr;
+    double *dx_ = nullptr, *dv1 = nullptr, *dv2 = nullptr, *dw = nullptr, *dst = nullptr;
+    int64_t *did = nullptr;
+    int *derr = nullptr;
+    hipError_t e = hipStreamCreate(&s);
+    auto ok = [&](hipError_t r) { if (r != hipSuccess && e == hipSuccess) e = r; };
+    ok(dalloc(&dx_, n)); ok(dalloc(&dv1, n)); ok(dalloc(&dv2, n)); ok(dalloc(&dw, n));
+    ok(dalloc(&did, n)); ok(dalloc(&derr, 1)); ok(dalloc(&dst, (size_t)n * ld));
+    int herr = 0;
+    if (e == hipSuccess) {
+        ok(hipMemsetAsync(derr, 0, sizeof(int), s));
+        ok(hipMemsetAsync(dst, 0, sizeof(double) * (size_t)n * ld, s));
+        ok(launch_sample_landau(n, dim_v, alpha, k, sigma, dimx, dx_, dv1, dv2, dw, did, derr, s));
+        ok(launch_soa_to_aos(dst, ld, n, dim_v, dx_, dv1, dv2, dw, did, s));
+        ok(hipMemcpyAsync(aos, dst, sizeof(double) * (size_t)n * ld, hipMemcpyDeviceToHost, s));
+        ok(hipMemcpyAsync(&herr, derr, sizeof(int), hipMemcpyDeviceToHost, s));
+        ok(hipStreamSynchronize(s));
+    }
+    void *bufs[] = {dx_, dv1, dv2, dw, did, derr, dst};
+    for (void *p : bufs)
+        if (p) (void)hipFree(p);
+    if (s) (void)hipStreamDestroy(s);
+    if (e != hipSuccess) return fail(GEMPIC_ERR_DEVICE, std::string("sample!: ") + hipGetErrorString(e));
+    if (herr) return fail(GEMPIC_ERR_ARGUMENT, "newton did not converge in 200 iterations");
+    return GEMPIC_OK;
+}
+
+int gempic_sample_landau_particles(gempic_splitting *h, double alpha, double k, double sigma)
+{
+    CHECK_H(h);
+    int rc = sample_args(h->dim_v, h->n, alpha, k, sigma, h->Lx);
+    if (rc) return rc;
+    if (h->n_weights != 1) return fail(GEMPIC_ERR_ARGUMENT, "sample! sets one weight per particle");
+    HIP_TRY(launch_sample_landau(h->n, h->dim_v, alpha, k, sigma, h->Lx, h->d_x, h->d_v1, h->d_v2,
+                                 h->d_w, h->d_id, h->d_err, h->stream));
+    h->particles_set = true;
+    rc = run_sort(h);
+    if (rc) return rc;
+    rc = finish(h);
+    if (rc == GEMPIC_ERR_INEXACT) return fail(GEMPIC_ERR_ARGUMENT, "newton did not converge in 200 iterations");
+    return rc;
+}
+
 int gempic_get_particles(gempic_splitting *h, double *aos, int64_t ld, int64_t n)
 {
     CHECK_H(h);

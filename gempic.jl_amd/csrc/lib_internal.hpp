@@ -137,6 +137,12 @@ struct SortGeometry {
 SortGeometry sort_geometry(int64_t n, int nx);
 hipError_t launch_sort(const SortArgs &a, int has_v2, hipStream_t s);
 
+// sample!(pg, alpha, k, sigma, mesh) into SoA arrays (id = identity); a
+// newton that does not converge sets E_FAR in err
+hipError_t launch_sample_landau(int64_t n, int dim_v, double alpha, double k, double sigma,
+                                double dimx, double *x, double *v1, double *v2, double *w,
+                                int64_t *id, int *err, hipStream_t s);
+
 // AoS (reference ParticleGroup layout) <-> SoA with the permutation id.
 hipError_t launch_aos_to_soa(const double *aos, int64_t ld, int64_t n, int dim_v, double *x,
                              double *v1, double *v2, double *w, int64_t *id, hipStream_t s);

@@ -1690,6 +1690,69 @@ hipError_t launch_sort(const SortArgs &a, int has_v2, hipStream_t s)
 // ---------------------------------------------------------------------------
 // Layout transposes between the reference's particle matrix and SoA
 // ---------------------------------------------------------------------------
+// sample!(pg, alpha, k, sigma, mesh) (particle_sampling.jl:266-311): particle
+// i (1-based) from the i-th point of Sobol.jl's SobolSeq(2) (Gray-code order,
+// the zero point skipped; dimension 1 van der Corput, dimension 2 from the
+// polynomial x + 1), its speed sigma sqrt(-2 log((i - 1/2)/n)) at angle
+// 2 pi r1, and x = newton(r2, alpha, k) (:236-245, the reference's two
+// interleaved iterations, at most 200).
+__device__ void sobol2(uint64_t k, double &r1, double &r2)
+{
+    const uint64_t g = k ^ (k >> 1);
+    uint32_t x1 = 0, x2 = 0, m = 1;
+    for (int j = 1; j <= 32; ++j) {
+        if (j > 1) m = (m << 1) ^ m;
+        if ((g >> (j - 1)) & 1u) {
+            x1 ^= 1u << (32 - j);
+            x2 ^= m << (32 - j);
+        }
+    }
+    r1 = (double)x1 * 0x1p-32;
+    r2 = (double)x2 * 0x1p-32;
+}
+
+__global__ void k_sample_landau(int64_t n, int dim_v, double alpha, double k, double sigma,
+                                double dimx, double *x, double *v1, double *v2, double *w,
+                                int64_t *id, int *err)
+{
+    int errs = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double v = sigma * sqrt(-2.0 * log(((double)(i + 1) - 0.5) / (double)n));
+        double r1, r2;
+        sobol2((uint64_t)(i + 1), r1, r2);
+        const double th = r1 * 6.283185307179586;
+        double x0 = 0.0, x1 = 1.0;
+        const double r = r2 * (6.283185307179586 / k);
+        int it = 0;
+        while (fabs(x1 - x0) > 1e-12 && it < 200) {
+            const double p = x0 + alpha * sin(k * x0) / k;
+            const double f = 1.0 + alpha * cos(k * x0);
+            const double nx1 = x0 - (p - r) / f;
+            x0 = x1;
+            x1 = nx1;
+            ++it;
+        }
+        if (it >= 200) errs |= E_FAR;
+        x[i] = x1;
+        v1[i] = v * cos(th);
+        if (dim_v == 2) v2[i] = v * sin(th);
+        w[i] = dimx;
+        id[i] = i;
+    }
+    if (errs) atomicOr(err, errs);
+}
+
+hipError_t launch_sample_landau(int64_t n, int dim_v, double alpha, double k, double sigma,
+                                double dimx, double *x, double *v1, double *v2, double *w,
+                                int64_t *id, int *err, hipStream_t s)
+{
+    const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(4096, (n + kBlock - 1) / kBlock));
+    hipLaunchKernelGGL(k_sample_landau, dim3(blocks), dim3(kBlock), 0, s, n, dim_v, alpha, k, sigma,
+                       dimx, x, v1, v2, w, id, err);
+    return hipGetLastError();
+}
+
 __global__ void k_aos_to_soa(const double *aos, int64_t ld, int64_t n, int dim_v, double *x,
                              double *v1, double *v2, double *w, int64_t *id)
 {

@@ -61,6 +61,8 @@ SIGNATURES = {
     "gempic_profile_reset": (_I, [_VP]),
     "gempic_set_launch": (_I, [_VP, _I]),
     "gempic_set_fusion": (_I, [_VP, _I]),
+    "gempic_sample_landau": (_I, [_I, _I64, _D, _D, _D, _D, _P, _I64]),
+    "gempic_sample_landau_particles": (_I, [_VP, _D, _D, _D]),
     "gempic_boris_staggering": (_I, [_VP, _D]),
     "gempic_boris_strang_splitting": (_I, [_VP, _D, _I64, _I]),
     "gempic_boris_push_v_epart": (_I, [_VP, _D]),

@@ -25,7 +25,7 @@ __all__ = [
     "operatorHB", "operatorHE", "operatorHp1", "operatorHp2", "strang_splitting_",
     "solve_poisson_", "TimeHistoryDiagnostics", "write_step_", "get_charge",
     "HamiltonianSplittingBoris", "staggering_", "push_v_epart_", "push_v_bpart_",
-    "push_x_accumulate_j_",
+    "push_x_accumulate_j_", "sample_", "sample_particles_",
 ]
 
 
@@ -458,6 +458,28 @@ def strang_splitting_(h: HamiltonianSplitting, dt: float, number_steps: int,
         return
     flags = _lib.STRANG_SUBSTEP if substep else _lib.STRANG_FUSED
     h._run(_lib.load().gempic_strang_splitting, float(dt), int(number_steps), flags)
+
+
+# ---------------------------------------------------------------------------
+# particle_sampling.jl
+
+def sample_(pg: ParticleGroup, alpha: float, k: float, sigma: float, mesh: OneDGrid):
+    """sample!(pg, alpha, k, sigma, mesh) (particle_sampling.jl:266-311) for
+    ParticleGroup{1,1} and {1,2}, computed on the GPU into pg.array."""
+    D, V = pg.dims
+    if D != 1 or pg.n_weights != 1:
+        raise TypeError("sample!(pg, alpha, k, sigma, mesh) needs a ParticleGroup{1,V} with one weight")
+    a = pg.array
+    check(_lib.load().gempic_sample_landau(V, pg.n_particles, float(alpha), float(k), float(sigma),
+                                           float(mesh.dimx), ptr(a), a.shape[0]))
+
+
+def sample_particles_(h: "HamiltonianSplitting", alpha: float, k: float, sigma: float):
+    """sample! straight into a splitting's GPU particles (no host round
+    trip); with sync=True the particle group is updated too."""
+    check(_lib.load().gempic_sample_landau_particles(h._h, float(alpha), float(k), float(sigma)))
+    if h.sync:
+        h.to_host(fields=False)
 
 
 # ---------------------------------------------------------------------------

@@ -151,6 +151,19 @@ int gempic_set_launch(gempic_splitting *h, int blocks);
 int gempic_set_fusion(gempic_splitting *h, int kernels_per_step);
 
 /* ------------------------------------------------------------------------
+ * sample!(pg, alpha, k, sigma, mesh) (particle_sampling.jl:266-311) for
+ * ParticleGroup{1,1} and {1,2}: Sobol.jl's SobolSeq(2) points, speeds
+ * sigma sqrt(-2 log((i - 1/2)/n)), x = newton(r2, alpha, k) (:236-245),
+ * weight = mesh.dimx. With V = 2, mesh.dimx must be 2 pi / k (:297).
+ * ------------------------------------------------------------------------ */
+/* into a host array: (2 + dim_v) x n column-major, leading dimension ld */
+int gempic_sample_landau(int dim_v, int64_t n, double alpha, double k, double sigma, double dimx,
+                         double *aos, int64_t ld);
+/* directly into a splitting handle's GPU particles (dimx = the mesh's;
+ * the particles are then sorted as after gempic_set_particles) */
+int gempic_sample_landau_particles(gempic_splitting *h, double alpha, double k, double sigma);
+
+/* ------------------------------------------------------------------------
  * HamiltonianSplittingBoris (hamiltonian_splitting_boris.jl:23-88) on a
  * splitting handle with V = 2 (the same particles, e_dofs, b_dofs and
  * j_dofs); its e_dofs_mid and b_dofs_mid live on the handle (zero at

@@ -1160,3 +1160,76 @@ void or_boris_set_mid(or_split *h, const double *e1m, const double *e2m, const d
         if (bm) h->bm[k] = bm[k];
     }
 }
+
+
+/* ------------------------------------------------------------------------ */
+/* sample!(pg, alpha, k, sigma, mesh) for ParticleGroup{1,1} and {1,2}       */
+/* (particle_sampling.jl:236-311) with Sobol.jl's SobolSeq(2).               */
+/* ------------------------------------------------------------------------ */
+
+/* Sobol.jl (Project.toml compat "1"; not vendored in the reference) follows
+ * the Bratley-Fox / Joe-Kuo construction in Gray-code order and skips the
+ * all-zero point: the k-th next!() (k = 1, 2, ...) is the XOR of the
+ * direction numbers V_j over the set bits j of gray(k) = k ^ (k >> 1).
+ * Dimension 1: V_j = 2^-j (van der Corput); dimension 2: primitive
+ * polynomial x + 1, m_1 = 1, m_j = 2 m_{j-1} XOR m_{j-1}, V_j = m_j 2^-j.
+ * Values are k/2^32 dyadic rationals, exact in double. */
+void or_sobol2(uint64_t k, double *r1, double *r2)
+{
+    uint64_t g = k ^ (k >> 1);
+    uint32_t x1 = 0, x2 = 0, m = 1;
+    for (int j = 1; j <= 32; ++j) {
+        if (j > 1) m = (m << 1) ^ m;
+        if ((g >> (j - 1)) & 1u) {
+            x1 ^= 1u << (32 - j);
+            x2 ^= m << (32 - j);
+        }
+    }
+    *r1 = ldexp((double)x1, -32);
+    *r2 = ldexp((double)x2, -32);
+}
+
+/* newton(r, alpha, k) (particle_sampling.jl:236-245): the inverse of the
+ * cdf of 1 + alpha cos(k x) on [0, 2 pi / k], in the reference's iteration
+ * (two interleaved sequences, stopped when they agree to 1e-12). Returns the
+ * number of iterations (limit 200; the reference has none). */
+int or_landau_newton(double r, double alpha, double k, double *out)
+{
+    double x0 = 0.0, x1 = 1.0;
+    r *= 6.283185307179586 / k;
+    int it = 0;
+    while (fabs(x1 - x0) > 1e-12 && it < 200) {
+        const double p = x0 + alpha * sin(k * x0) / k;
+        const double f = 1.0 + alpha * cos(k * x0);
+        const double nx1 = x0 - (p - r) / f;
+        x0 = x1;
+        x1 = nx1;
+        ++it;
+    }
+    *out = x1;
+    return it;
+}
+
+/* sample! (:266-311): x from the Sobol point's second coordinate by newton,
+ * |v| = sigma sqrt(-2 log((i - 1/2) / N)), angle 2 pi r1; weight = dimx.
+ * aos: (1 + dim_v + 1) x n column-major with leading dimension ld. Returns
+ * the largest newton iteration count. */
+int or_sample_landau(int dim_v, int64_t n, double alpha, double k, double sigma, double dimx,
+                     double *aos, int64_t ld)
+{
+    int itmax = 0;
+    for (int64_t i = 1; i <= n; ++i) {
+        const double v = sigma * sqrt(-2.0 * log(((double)i - 0.5) / (double)n));
+        double r1, r2, x;
+        or_sobol2((uint64_t)i, &r1, &r2);
+        const double th = r1 * 6.283185307179586;
+        const int it = or_landau_newton(r2, alpha, k, &x);
+        if (it > itmax) itmax = it;
+        double *col = aos + (i - 1) * ld;
+        col[0] = x;
+        col[1] = v * cos(th);
+        if (dim_v == 2) col[2] = v * sin(th);
+        col[1 + dim_v] = dimx;
+    }
+    return itmax;
+}

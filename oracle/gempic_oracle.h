@@ -183,6 +183,13 @@ or_split *or_split_create(int64_t nx, double xmin, double xmax, int maxwell_degr
                           double *e2, double *b);
 void or_split_destroy(or_split *h);
 
+/* sample!(pg, alpha, k, sigma, mesh) (particle_sampling.jl:236-311) and the
+ * Sobol.jl SobolSeq(2) points it draws */
+void or_sobol2(uint64_t k, double *r1, double *r2);
+int or_landau_newton(double r, double alpha, double k, double *out);
+int or_sample_landau(int dim_v, int64_t n, double alpha, double k, double sigma, double dimx,
+                     double *aos, int64_t ld);
+
 /* HamiltonianSplittingBoris (hamiltonian_splitting_boris.jl) on an or_split
  * state with dim_v = 2. which: 0 = e_dofs_mid[1], 1 = e_dofs_mid[2], 2 =
  * b_dofs_mid. */

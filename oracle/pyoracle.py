@@ -88,6 +88,9 @@ def lib():
             "or_operator_hp1": (None, [_VP, _D]),
             "or_operator_hp2": (None, [_VP, _D]),
             "or_strang_splitting": (None, [_VP, _D, _I64]),
+            "or_sobol2": (None, [ctypes.c_uint64, _P, _P]),
+            "or_landau_newton": (_I, [_D, _D, _D, _P]),
+            "or_sample_landau": (_I, [_I, _I64, _D, _D, _D, _D, _P, _I64]),
             "or_boris_mid": (_P, [_VP, _I]),
             "or_boris_set_mid": (None, [_VP, _P, _P, _P]),
             "or_boris_push_v_epart": (None, [_VP, _D]),
@@ -532,3 +535,26 @@ class Maxwell2D:
         out = np.zeros(self.nx * self.ny)
         lib().or_maxwell2d_l2projection(self.h, cb, None, component, form, _p(out))
         return out
+
+
+# --------------------------------------------------------------------------
+# sample!(pg, alpha, k, sigma, mesh) (particle_sampling.jl:236-311)
+
+def sobol2(k):
+    """The k-th point (k >= 1) of Sobol.jl's SobolSeq(2)."""
+    r = np.zeros(2)
+    lib().or_sobol2(k, _p(r[:1]), _p(r[1:]))
+    return float(r[0]), float(r[1])
+
+
+def landau_newton(r, alpha, k):
+    out = np.zeros(1)
+    it = lib().or_landau_newton(r, alpha, k, _p(out))
+    return float(out[0]), it
+
+
+def sample_landau(dim_v, n, alpha, k, sigma, dimx):
+    """The (1 + dim_v + 1, n) column-major particle array of sample!."""
+    a = np.zeros((2 + dim_v, n), order="F")
+    it = lib().or_sample_landau(dim_v, n, alpha, k, sigma, dimx, _p(a), 2 + dim_v)
+    return a, it
