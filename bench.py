@@ -195,6 +195,35 @@ def boris_run(G, _lib, w, steps, warmup):
                          "kernel": "k_boris1d fused Boris step"}}
 
 
+def sampling_run(G, n, cpu_n):
+    """sample!(pg, 0.5, 0.5, 1.0, mesh) (the strong-Landau initial condition)
+    of n particles straight into a splitting's GPU particles, including the
+    initial sort; wall time of the call, and the oracle on cpu_n particles."""
+    import math
+    L = 4 * math.pi
+    mesh = G.OneDGrid(0.0, L, 32)
+    pg = G.ParticleGroup(1, 2, n)
+    ks0 = G.ParticleMeshCoupling1D(mesh, n, 3, "galerkin")
+    ks1 = G.ParticleMeshCoupling1D(mesh, n, 2, "galerkin")
+    h = G.HamiltonianSplitting(1, 2, G.Maxwell1DFEM(mesh, 3), ks0, ks1, pg,
+                               [np.zeros(32), np.zeros(32)], np.zeros(32), sync=False)
+    G.sample_particles_(h, 0.5, 0.5, 1.0)               # warm-up (kernel load)
+    t0 = time.perf_counter()
+    G.sample_particles_(h, 0.5, 0.5, 1.0)
+    secs = time.perf_counter() - t0
+    out = {"workload": f"sample!(pg, 0.5, 0.5, 1.0, mesh), {n} particles, 1D2V, into GPU memory",
+           "ms": secs * 1e3, "particles_per_s": n / secs}
+    if cpu_n:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as orc
+        t0 = time.perf_counter()
+        orc.sample_landau(2, cpu_n, 0.5, 0.5, 1.0, L)
+        cs = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": cpu_n / cs, "unit": "particles/s", "cores": 1, "kind": "port",
+                               "sample": f"{cpu_n} particles ({cs:.2f} s), C restatement, one thread"}
+    return out
+
+
 def copy_calibration(mib):
     """Run a plain device copy of `mib` MiB five times (reads and writes mib MiB each).
 
@@ -226,6 +255,7 @@ def main():
     ap.add_argument("--substep", action="store_true", help="also time the sub-step path")
     ap.add_argument("--blocks", type=int, default=0, help="workgroups per particle pass (0 = auto)")
     ap.add_argument("--fusion", type=int, default=1, help="particle kernels per fused step (1 or 2)")
+    ap.add_argument("--no-sampling", action="store_true", help="skip the sample! measurement")
     ap.add_argument("--boris-steps", type=int, default=200,
                     help="steps of the Boris splitting measurement (0 = skip it)")
     ap.add_argument("--particles-2d", type=int, default=16777216,
@@ -320,6 +350,11 @@ def main():
         print(f"[bench] Boris splitting, {args.boris_steps} steps", file=sys.stderr, flush=True)
         boris = boris_run(G, _lib, w, args.boris_steps, 10)
 
+    samp = None
+    if not args.no_sampling:
+        print("[bench] sample! into GPU memory", file=sys.stderr, flush=True)
+        samp = sampling_run(G, n, 0 if args.no_cpu else 262144)
+
     p2d = None
     if args.particles_2d:
         print(f"[bench] 2D pieces, {args.particles_2d} particles", file=sys.stderr, flush=True)
@@ -351,6 +386,8 @@ def main():
     }
     if boris is not None:
         line["boris_1d2v"] = boris
+    if samp is not None:
+        line["sampling"] = samp
     if p2d is not None:
         line["pieces_2d"] = p2d
     if substep is not None:
