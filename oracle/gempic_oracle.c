@@ -1233,3 +1233,311 @@ int or_sample_landau(int dim_v, int64_t n, double alpha, double k, double sigma,
     }
     return itmax;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Reverse mode (adjoint) of the 1D2V Strang splitting                      */
+/* ------------------------------------------------------------------------ */
+/*
+ * The reference has no derivatives. This is the transpose of the Jacobian of
+ * each sub-step of strang_splitting! (hamiltonian_splitting.jl:98-108) as the
+ * functions above compute it, applied in reverse order, sub-step by
+ * sub-step. It is pinned by central finite differences of the forward
+ * functions above (tests/test_oracle_adjoint.py): directional derivatives of
+ * the loss along random perturbations of every input, per sub-step and over
+ * whole runs.
+ *
+ * Loss: wloss * sum over the steps s = 1..T of 1/2 l2norm_squared(B_s, p-1)
+ * (maxwell_1d_fem.jl:299-313), B_s = b_dofs after step s; with wloss = dt
+ * this is the time-integrated magnetic energy of write_step!
+ * (diagnostics.jl:236), plus <ct, final state> for given cotangents ct.
+ *
+ * Derivative of a basis of degree p at offset t: d/dt B_p[r] =
+ * B_{p-1}[r-1] - B_{p-1}[r] (entries outside 0..p-1 are zero); positions
+ * enter through t = (x - xmin)/dx - cell, so d/dx = (1/dx) d/dt.
+ */
+
+void or_bspline_deriv(int degree, double offset, double *d)
+{
+    double lo[OR_MAXDEG + 1];
+    if (degree == 0) { d[0] = 0.0; return; }
+    or_bspline_basis(degree - 1, offset, lo);
+    for (int r = 0; r <= degree; ++r) {
+        double a = r >= 1 ? lo[r - 1] : 0.0;
+        double b = r <= degree - 1 ? lo[r] : 0.0;
+        d[r] = a - b;
+    }
+}
+
+/* value and x-derivative of evaluate(p, x, field) at the trunc cell */
+static void eval_d(const or_pmc1d *p, double x, const double *field, double *val, double *dval)
+{
+    double sv[OR_MAXDEG + 1], dv[OR_MAXDEG + 1];
+    double xi = (x - p->xmin) / p->delta_x;
+    int64_t index = (int64_t)xi;
+    double t = xi - (double)index;
+    index = index - p->degree;
+    or_bspline_basis(p->degree, t, sv);
+    or_bspline_deriv(p->degree, t, dv);
+    double v = 0.0, d = 0.0;
+    for (int i = 1; i <= p->n_span; ++i) {
+        int64_t k = or_jl_mod_i64(index + i - 1, p->n_grid);
+        v += field[k] * sv[i - 1];
+        d += field[k] * dv[i - 1];
+    }
+    *val = v;
+    *dval = d / p->delta_x;
+}
+
+/* out[k] += wgt * N_k(x) at the trunc cell (the transpose of evaluate) */
+static void scatter(const or_pmc1d *p, double x, double wgt, double *out)
+{
+    double sv[OR_MAXDEG + 1];
+    double xi = (x - p->xmin) / p->delta_x;
+    int64_t index = (int64_t)xi;
+    double t = xi - (double)index;
+    index = index - p->degree;
+    or_bspline_basis(p->degree, t, sv);
+    for (int i = 1; i <= p->n_span; ++i)
+        out[or_jl_mod_i64(index + i - 1, p->n_grid)] += wgt * sv[i - 1];
+}
+
+/* y = C^T x for the circulant C of solve_circulant(eig, .): conjugate eigenvalues */
+static void solve_circulant_t(or_maxwell1d *m, const double *eig, const double *x, double *y)
+{
+    int64_t n = m->n;
+    double *ec = (double *)malloc((size_t)n * sizeof(double));
+    for (int64_t k = 0; k < n; ++k) ec[k] = (k > n / 2) ? -eig[k] : eig[k];
+    or_solve_circulant(m, ec, x);
+    memcpy(y, m->work, (size_t)n * sizeof(double));
+    free(ec);
+}
+
+/* lambda_e += coef * D^T lambda_b, (D e)[i] = e[i-1] - e[i] (compute_b_from_e!) */
+static void adj_b_from_e(double *le, double coef, const double *lb, int64_t n)
+{
+    for (int64_t j = 0; j < n; ++j) {
+        int64_t jp = j + 1 == n ? 0 : j + 1;
+        le[j] += coef * (lb[jp] - lb[j]);
+    }
+}
+
+typedef struct {
+    double *lx, *lv1, *lv2;      /* particle cotangents, one per particle */
+    double *le1, *le2, *lb;      /* field cotangents */
+} or_adj_state;
+
+/* operatorHB 1d2v: E2 += dt/dx W(B) */
+static void adj_hb(or_split *h, double dt, or_adj_state *A)
+{
+    int64_t n = h->m->n;
+    double *y = (double *)malloc((size_t)n * sizeof(double));
+    solve_circulant_t(h->m, h->m->eig_weak_ampere, A->le2, y);
+    double coef = dt / h->m->delta_x;
+    for (int64_t i = 0; i < n; ++i) A->lb[i] += coef * y[i];
+    free(y);
+}
+
+/* operatorHE 1d2v; S = particle array and fields before the sub-step */
+static void adj_he(or_split *h, double dt, const double *S, const double *e1, const double *e2,
+                   or_adj_state *A)
+{
+    double qm = h->q_over_m;
+    int ld = h->ld;
+    adj_b_from_e(A->le2, dt / h->m->delta_x, A->lb, h->m->n);
+    for (int64_t i = 0; i < h->np; ++i) {
+        double x = S[i * ld];
+        double f1, d1, f2, d2;
+        eval_d(&h->ks1, x, e1, &f1, &d1);
+        eval_d(&h->ks0, x, e2, &f2, &d2);
+        A->lx[i] += dt * qm * (A->lv1[i] * d1 + A->lv2[i] * d2);
+        scatter(&h->ks1, x, dt * qm * A->lv1[i], A->le1);
+        scatter(&h->ks0, x, dt * qm * A->lv2[i], A->le2);
+    }
+}
+
+/* operatorHp2 1d2v */
+static void adj_hp2(or_split *h, double dt, const double *S, const double *b, or_adj_state *A)
+{
+    or_maxwell1d *m = h->m;
+    int64_t n = m->n;
+    int ld = h->ld;
+    double qm = h->q_over_m;
+    double *mu = (double *)calloc((size_t)n, sizeof(double));
+    double *eig = (double *)calloc((size_t)n, sizeof(double));
+    for (int64_t i = 0; i <= n / 2; ++i) eig[i] = 1.0 / m->eig_mass0[i];
+    /* E2 -= C j2 / dx with j2 = dt * sum q v2 S0(x): dj = -(dt/dx) C^T lambda_E2 */
+    solve_circulant_t(m, eig, A->le2, mu);
+    for (int64_t k = 0; k < n; ++k) mu[k] = -(dt / m->delta_x) * mu[k];
+    for (int64_t i = 0; i < h->np; ++i) {
+        double x = S[i * ld], v2 = S[2 + i * ld], w = S[1 + h->dim_v + i * ld];
+        double q = w * h->charge * h->common_weight;
+        double bv, db, g, dg;
+        eval_d(&h->ks1, x, b, &bv, &db);
+        eval_d(&h->ks0, x, mu, &g, &dg);
+        double lv1 = A->lv1[i];
+        A->lv2[i] += dt * qm * bv * lv1 + q * h->ks0.scaling * g;
+        A->lx[i] += dt * qm * v2 * lv1 * db + q * h->ks0.scaling * v2 * dg;
+        scatter(&h->ks1, x, dt * qm * v2 * lv1, A->lb);
+    }
+    free(mu);
+    free(eig);
+}
+
+/* operatorHp1 1d2v: x_new = x + dt v1, v2 -= qm int_x^x_new B, j1 += q int S1,
+ * x = mod(x_new, Lx), E1 -= C1 j1 / dx */
+static void adj_hp1(or_split *h, double dt, const double *S, const double *b, or_adj_state *A)
+{
+    or_maxwell1d *m = h->m;
+    int64_t n = m->n;
+    int ld = h->ld;
+    double qm = h->q_over_m;
+    double *mu = (double *)calloc((size_t)n, sizeof(double));
+    double *eig = (double *)calloc((size_t)n, sizeof(double));
+    for (int64_t i = 0; i <= n / 2; ++i) eig[i] = 1.0 / m->eig_mass1[i];
+    solve_circulant_t(m, eig, A->le1, mu);
+    for (int64_t k = 0; k < n; ++k) mu[k] = -mu[k] / m->delta_x;
+    or_pmc1d p1 = h->ks1;
+    p1.scaling = 1.0;
+    for (int64_t i = 0; i < h->np; ++i) {
+        double x_old = S[i * ld], v1 = S[1 + i * ld], w = S[1 + h->dim_v + i * ld];
+        double q = w * h->charge * h->common_weight;
+        double x_new = x_old + dt * v1;
+        double bn, dbn, bo, dbo, gn, dgn, go, dgo;
+        /* the end points' cells are trunc() cells, as in add_current_update_v! */
+        eval_d(&h->ks1, x_new, b, &bn, &dbn);
+        eval_d(&h->ks1, x_old, b, &bo, &dbo);
+        eval_d(&h->ks1, x_new, mu, &gn, &dgn);
+        eval_d(&h->ks1, x_old, mu, &go, &dgo);
+        double lv2 = A->lv2[i];
+        double lxn = A->lx[i] - qm * lv2 * bn + q * h->ks1.scaling * gn;
+        A->lx[i] = lxn + qm * lv2 * bo - q * h->ks1.scaling * go;
+        A->lv1[i] += dt * lxn;
+        /* lambda_B[k] += -qm lv2 int S1_k: the same line integral, unscaled */
+        or_add_current_update_v(A->lb, &p1, x_old, x_new, -qm * lv2, qm, NULL, 0.0);
+    }
+    free(mu);
+    free(eig);
+}
+
+/* loss term of B: wloss * 1/2 l2norm_squared(B) and its gradient wloss dx M1 B */
+static double loss_b(or_split *h, double wloss, const double *b, double *lb)
+{
+    or_maxwell1d *m = h->m;
+    double e = or_l2norm_squared(m, b, m->s_deg_1);   /* leaves M1 B in work */
+    if (lb)
+        for (int64_t i = 0; i < m->n; ++i) lb[i] += wloss * m->delta_x * m->work[i];
+    return wloss * 0.5 * e;
+}
+
+double or_strang_loss(or_split *h, double dt, int64_t nsteps, double wloss)
+{
+    double loss = 0.0;
+    for (int64_t s = 0; s < nsteps; ++s) {
+        or_strang_splitting(h, dt, 1);
+        loss += loss_b(h, wloss, h->b, NULL);
+    }
+    return loss;
+}
+
+int or_strang_adjoint(or_split *h, double dt, int64_t nsteps, double wloss, const double *ct_part,
+                      const double *ct_e1, const double *ct_e2, const double *ct_b, double *loss,
+                      double *g_part, double *g_e1, double *g_e2, double *g_b)
+{
+    if (h->dim_v != 2 || nsteps < 0) return -1;
+    const int64_t n = h->m->n, np = h->np;
+    const int ld = h->ld;
+    const size_t psz = (size_t)ld * (size_t)np, fsz = 3 * (size_t)n;
+    const int nsub = 7;
+    /* state before every sub-step of every step: particles, then E1, E2, B */
+    double *hist = (double *)malloc(((size_t)nsteps * nsub) * (psz + fsz) * sizeof(double) + 8);
+    double *bend = (double *)malloc((size_t)nsteps * n * sizeof(double) + 8);
+    if (!hist || !bend) { free(hist); free(bend); return -2; }
+    const double hdt = 0.5 * dt;
+    double L = 0.0;
+    for (int64_t s = 0; s < nsteps; ++s) {
+        for (int k = 0; k < nsub; ++k) {
+            double *rec = hist + ((size_t)s * nsub + k) * (psz + fsz);
+            memcpy(rec, h->a, psz * sizeof(double));
+            memcpy(rec + psz, h->e1, n * sizeof(double));
+            memcpy(rec + psz + n, h->e2, n * sizeof(double));
+            memcpy(rec + psz + 2 * n, h->b, n * sizeof(double));
+            switch (k) {
+            case 0: or_operator_hb(h, hdt); break;
+            case 1: or_operator_he(h, hdt); break;
+            case 2: or_operator_hp2(h, hdt); break;
+            case 3: or_operator_hp1(h, dt); break;
+            case 4: or_operator_hp2(h, hdt); break;
+            case 5: or_operator_he(h, hdt); break;
+            default: or_operator_hb(h, hdt); break;
+            }
+        }
+        memcpy(bend + (size_t)s * n, h->b, n * sizeof(double));
+        L += loss_b(h, wloss, h->b, NULL);
+    }
+    or_adj_state A;
+    A.lx = (double *)calloc((size_t)np, sizeof(double));
+    A.lv1 = (double *)calloc((size_t)np, sizeof(double));
+    A.lv2 = (double *)calloc((size_t)np, sizeof(double));
+    A.le1 = (double *)calloc((size_t)n, sizeof(double));
+    A.le2 = (double *)calloc((size_t)n, sizeof(double));
+    A.lb = (double *)calloc((size_t)n, sizeof(double));
+    for (int64_t i = 0; ct_part && i < np; ++i) {
+        A.lx[i] = ct_part[i * ld];
+        A.lv1[i] = ct_part[1 + i * ld];
+        A.lv2[i] = ct_part[2 + i * ld];
+    }
+    for (int64_t k = 0; k < n; ++k) {
+        if (ct_e1) A.le1[k] = ct_e1[k];
+        if (ct_e2) A.le2[k] = ct_e2[k];
+        if (ct_b) A.lb[k] = ct_b[k];
+    }
+    for (int64_t s = nsteps - 1; s >= 0; --s) {
+        (void)loss_b(h, wloss, bend + (size_t)s * n, A.lb);
+        for (int k = nsub - 1; k >= 0; --k) {
+            const double *rec = hist + ((size_t)s * nsub + k) * (psz + fsz);
+            const double *e1 = rec + psz, *e2 = rec + psz + n, *b = rec + psz + 2 * n;
+            switch (k) {
+            case 0: case 6: adj_hb(h, hdt, &A); break;
+            case 1: case 5: adj_he(h, hdt, rec, e1, e2, &A); break;
+            case 2: case 4: adj_hp2(h, hdt, rec, b, &A); break;
+            default: adj_hp1(h, dt, rec, b, &A); break;
+            }
+        }
+    }
+    if (loss) *loss = L;
+    for (int64_t i = 0; g_part && i < np; ++i) {
+        for (int r = 0; r < ld; ++r) g_part[r + i * ld] = 0.0;
+        g_part[i * ld] = A.lx[i];
+        g_part[1 + i * ld] = A.lv1[i];
+        g_part[2 + i * ld] = A.lv2[i];
+    }
+    if (g_e1) memcpy(g_e1, A.le1, n * sizeof(double));
+    if (g_e2) memcpy(g_e2, A.le2, n * sizeof(double));
+    if (g_b) memcpy(g_b, A.lb, n * sizeof(double));
+    free(A.lx); free(A.lv1); free(A.lv2); free(A.le1); free(A.le2); free(A.lb);
+    free(hist);
+    free(bend);
+    return 0;
+}
+
+/* Reverse mode of solve_poisson! (diagnostics.jl:15-31) with respect to the
+ * positions: E = D C_pois rho(x), rho = sum q S0(x) * scaling (add_charge!),
+ * (D phi)[i] = phi[i-1] - phi[i] (compute_e_from_rho!, maxwell_1d_fem.jl:244-255).
+ * g_x[i] = d <ct_e, E> / d x_i. */
+void or_solve_poisson_adjoint(or_split *h, const double *ct_e, double *g_x)
+{
+    or_maxwell1d *m = h->m;
+    int64_t n = m->n;
+    double *dt_ = (double *)calloc((size_t)n, sizeof(double));
+    double *nu = (double *)calloc((size_t)n, sizeof(double));
+    adj_b_from_e(dt_, 1.0, ct_e, n);
+    solve_circulant_t(m, m->eig_weak_poisson, dt_, nu);
+    for (int64_t i = 0; i < h->np; ++i) {
+        double q = h->charge * PW(h, i) * h->common_weight;
+        double g, dg;
+        eval_d(&h->ks0, PX(h, i), nu, &g, &dg);
+        g_x[i] = q * h->ks0.scaling * dg;
+    }
+    free(dt_);
+    free(nu);
+}

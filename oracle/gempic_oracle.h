@@ -216,6 +216,16 @@ void or_write_step(or_split *h, double time, int degree, const double *e1, const
                    const double *b, const double *e1_n, const double *e2_n,
                    const double *efield_poisson, double *out);
 
+/* ---------- reverse mode of the 1D2V Strang splitting (no reference
+ * counterpart; pinned by central finite differences of the functions above) ---------- */
+void or_bspline_deriv(int degree, double offset, double *d);
+double or_strang_loss(or_split *h, double dt, int64_t nsteps, double wloss);
+int or_strang_adjoint(or_split *h, double dt, int64_t nsteps, double wloss,
+                      const double *ct_part, const double *ct_e1, const double *ct_e2,
+                      const double *ct_b, double *loss, double *g_part, double *g_e1,
+                      double *g_e2, double *g_b);
+void or_solve_poisson_adjoint(or_split *h, const double *ct_e, double *g_x);
+
 #ifdef __cplusplus
 }
 #endif

@@ -99,6 +99,10 @@ def lib():
             "or_boris_staggering": (None, [_VP, _D]),
             "or_boris_strang": (None, [_VP, _D, _I64]),
             "or_split_set_particles": (_I, [_VP, _P, _I64]),
+            "or_bspline_deriv": (None, [_I, _D, _P]),
+            "or_strang_loss": (_D, [_VP, _D, _I64, _D]),
+            "or_strang_adjoint": (_I, [_VP, _D, _I64, _D, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+            "or_solve_poisson_adjoint": (None, [_VP, _P, _P]),
             "or_solve_poisson": (None, [_P, _VP, _P]),
             "or_write_step": (None, [_VP, _D, _I, _P, _P, _P, _P, _P, _P, _P]),
             "or_pmc2d_init": (_I, [_VP, _D, _D, _I64, _D, _D, _I64, _I, _I]),
@@ -136,6 +140,8 @@ def lib():
 
 
 def _p(a: np.ndarray):
+    if a is None:
+        return None
     assert a.dtype == np.float64 and (a.flags.c_contiguous or a.flags.f_contiguous)
     return a.ctypes.data_as(_P)
 
@@ -145,6 +151,12 @@ def _p(a: np.ndarray):
 
 def jl_mod(x: float, y: float) -> float:
     return lib().or_jl_mod_f64(x, y)
+
+
+def bspline_deriv(degree: int, offset: float) -> np.ndarray:
+    out = np.zeros(degree + 1)
+    lib().or_bspline_deriv(degree, offset, _p(out))
+    return out
 
 
 def bspline_basis(degree: int, offset: float) -> np.ndarray:
@@ -356,6 +368,31 @@ class Splitting:
 
     def solve_poisson(self, efield, rho):
         lib().or_solve_poisson(_p(efield), self.h, _p(rho))
+
+    # reverse mode (no reference counterpart): loss = wloss * sum over steps
+    # of 1/2 l2norm_squared(B_s, p-1), plus <ct, final state>
+    def strang_loss(self, dt, nsteps, wloss):
+        return lib().or_strang_loss(self.h, dt, nsteps, wloss)
+
+    def strang_adjoint(self, dt, nsteps, wloss, ct_part=None, ct_e1=None, ct_e2=None, ct_b=None):
+        """Runs nsteps forward (like strang_splitting) and returns
+        (loss, g_part, g_e1, g_e2, g_b): the cotangents of the initial state
+        (g_part has the particle array's shape; the weight row is zero)."""
+        loss = np.zeros(1)
+        g = np.zeros_like(self.particles, order="F")
+        ge1, ge2, gb = np.zeros(self.nx), np.zeros(self.nx), np.zeros(self.nx)
+        cp = None if ct_part is None else np.asfortranarray(ct_part, dtype=np.float64)
+        rc = lib().or_strang_adjoint(self.h, dt, nsteps, wloss, _p(cp), _p(ct_e1), _p(ct_e2),
+                                     _p(ct_b), _p(loss), _p(g), _p(ge1), _p(ge2), _p(gb))
+        if rc:
+            raise ValueError(f"or_strang_adjoint: {rc}")
+        return float(loss[0]), g, ge1, ge2, gb
+
+    def solve_poisson_adjoint(self, ct_e):
+        g = np.zeros(self.particles.shape[1])
+        lib().or_solve_poisson_adjoint(self.h, _p(np.ascontiguousarray(ct_e, dtype=np.float64)),
+                                       _p(g))
+        return g
 
     # HamiltonianSplittingBoris (hamiltonian_splitting_boris.jl) on this state
     def boris_mid(self):
