@@ -139,6 +139,16 @@ struct gempic_splitting {
     int64_t steps_since_sort = 0;
     bool particles_set = false;
 
+    // reverse mode (gempic_strang_splitting_adjoint): field records, particle
+    // checkpoints, the recompute buffer of one checkpoint interval, the
+    // particle cotangents (x, v1, v2) and the adjoint grid slots
+    double *d_hist = nullptr, *d_ckpt = nullptr, *d_seg = nullptr, *d_lam = nullptr;
+    double *d_agrid = nullptr;
+    size_t hist_cap = 0, ckpt_cap = 0, seg_cap = 0;
+    double *cur_hist = nullptr;       // PassArgs::hist of the next launches
+    int adj_blocks = 0;
+    bool adj_ready = false;
+
     // profiling
     bool profile = false;
     int profile_every = 1;            // sample every n-th launch of each kernel id
@@ -237,6 +247,7 @@ PassArgs pass_args(gempic_splitting *h)
     a.cnt = h->d_cnt;
     a.err = h->d_err;
     a.diag = h->d_diag;
+    a.hist = h->cur_hist;
     return a;
 }
 
@@ -512,7 +523,8 @@ int gempic_splitting_destroy(gempic_splitting *h)
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     void *bufs[] = {h->d_x, h->d_v1, h->d_v2, h->d_w, h->d_id, h->d_x2, h->d_v12, h->d_v22,
                     h->d_w2, h->d_id2, h->d_grid, h->d_tabs, h->d_part, h->d_gpart, h->d_cnt,
-                    h->d_err, h->d_diag, h->d_sort_counts, h->d_sort_gsum, h->d_stage};
+                    h->d_err, h->d_diag, h->d_sort_counts, h->d_sort_gsum, h->d_stage,
+                    h->d_hist, h->d_ckpt, h->d_seg, h->d_lam, h->d_agrid};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     for (auto &p : h->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
@@ -775,6 +787,240 @@ int gempic_strang_splitting(gempic_splitting *h, double dt, int64_t n_steps, int
     if (rc) return rc;
     if (flags & GEMPIC_ASYNC) return GEMPIC_OK;
     return finish(h);
+}
+
+// ---------------------------------------------------------------------------
+// Reverse mode of strang_splitting! (1d2v): the forward run records the
+// fields of every step and checkpoints the particles every K steps; the
+// reverse sweep re-runs each interval of K steps forward from its checkpoint
+// (k_adj1d R_FWD, no deposits) and then steps back through it with the
+// reverse passes and their field parts.
+// ---------------------------------------------------------------------------
+namespace {
+
+int grow(double **p, size_t *cap, size_t need)
+{
+    if (*cap >= need && *p) return GEMPIC_OK;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    HIP_TRY(dalloc(p, need));
+    *cap = need;
+    return GEMPIC_OK;
+}
+
+AdjArgs adj_args(gempic_splitting *h, double dt, const Launch &L)
+{
+    AdjArgs A;
+    std::memset(&A, 0, sizeof(A));
+    A.p = pass_args(h);
+    A.p.hist = nullptr;
+    A.p.chunk = L.chunk;
+    A.p.iters = L.iters;
+    A.p.ngroups = L.ngroups;
+    const double hdt = 0.5 * dt;
+    A.p.dt_he = hdt;
+    A.p.dt_hp2 = hdt;
+    A.p.dt_hp1 = 1.0 * dt;
+    A.p.dtqm_he = hdt * h->qm;
+    A.p.dtqm_hp2 = hdt * h->qm;
+    A.agrid = h->d_agrid;
+    A.lx = h->d_lam;
+    A.lv1 = h->d_lam + h->n;
+    A.lv2 = h->d_lam + 2 * h->n;
+    return A;
+}
+
+int adj_launch(gempic_splitting *h, int rm, const AdjArgs &A, int blocks)
+{
+    hipError_t e = launch_adj(h->deg0, h->deg1, rm, A, blocks, h->stream);
+    if (e != hipSuccess) return fail(GEMPIC_ERR_DEVICE, std::string("reverse pass launch: ") + hipGetErrorString(e));
+    return GEMPIC_OK;
+}
+
+int adj_epi(gempic_splitting *h, int kind, const AdjArgs &A, int blocks)
+{
+    hipError_t e = launch_adj_epi(kind, A, blocks, h->stream);
+    if (e != hipSuccess) return fail(GEMPIC_ERR_DEVICE, std::string("reverse field part launch: ") + hipGetErrorString(e));
+    return GEMPIC_OK;
+}
+
+int adj_workspace(gempic_splitting *h, int64_t T, int64_t K)
+{
+    const size_t n = (size_t)h->n, nx = (size_t)h->nx;
+    const size_t nck = (size_t)((T + K - 1) / K);
+    int rc = grow(&h->d_hist, &h->hist_cap, (size_t)(T + 2) * NHIST * nx);
+    if (!rc) rc = grow(&h->d_ckpt, &h->ckpt_cap, nck * 3 * n);
+    if (!rc && K > 1) rc = grow(&h->d_seg, &h->seg_cap, (size_t)(K - 1) * 3 * n);
+    if (!rc && !h->d_lam) HIP_TRY(dalloc(&h->d_lam, 3 * n));
+    if (!rc && !h->d_agrid) HIP_TRY(dalloc(&h->d_agrid, (size_t)NASLOT * nx));
+    if (!rc && !h->adj_blocks)
+        h->adj_blocks = std::max(1, std::min(h->max_blocks,
+                                             h->cus * adj_blocks_per_cu(h->deg0, h->deg1, R_REV)));
+    return rc;
+}
+
+}  // namespace
+
+int gempic_strang_splitting_adjoint(gempic_splitting *h, double dt, int64_t n_steps,
+                                    int64_t checkpoint_every, double loss_weight,
+                                    const double *ct_particles, int64_t ld, const double *ct_e1,
+                                    const double *ct_e2, const double *ct_b, int flags)
+{
+    CHECK_H(h);
+    CHECK_P(h);
+    if (h->dim_v != 2)
+        return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: the reverse mode covers the 1d2v Strang splitting");
+    if (n_steps < 1) return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: number_steps must be >= 1");
+    if (checkpoint_every < 1) return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: checkpoint_every must be >= 1");
+    if (h->nx > kAdjNxMax) return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: the reverse mode needs nx <= 128");
+    if (ct_particles && ld != h->ld)
+        return fail(GEMPIC_ERR_ASSERTION, "AssertionError: leading dimension must be D+V+n_weights");
+    const int64_t T = n_steps, K = std::min<int64_t>(checkpoint_every, n_steps);
+    int rc = adj_workspace(h, T, K);
+    if (rc) return rc;
+    h->adj_ready = false;
+    const size_t n = (size_t)h->n, nx = (size_t)h->nx;
+    const size_t nb = sizeof(double) * n;
+    const double hdt = 0.5 * dt;
+    double *hist = h->d_hist;
+    auto rec = [&](int64_t s) { return hist + (size_t)s * NHIST * nx; };
+
+    // ---- forward, recorded ----
+    GridProg g = prog1(G_HB, hdt, S_B, S_B, S_E2);
+    prog_add(g, G_HE_B, hdt, S_E2, S_E2, S_B);
+    rc = run_grid(h, g);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(rec(1), h->d_grid, sizeof(double) * 4 * nx, hipMemcpyDeviceToDevice, h->stream));
+    for (int64_t s = 1; s <= T && !rc; ++s) {
+        if ((s - 1) % K == 0) {
+            double *ck = h->d_ckpt + (size_t)((s - 1) / K) * 3 * n;
+            HIP_TRY(hipMemcpyAsync(ck, h->d_x, nb, hipMemcpyDeviceToDevice, h->stream));
+            HIP_TRY(hipMemcpyAsync(ck + n, h->d_v1, nb, hipMemcpyDeviceToDevice, h->stream));
+            HIP_TRY(hipMemcpyAsync(ck + 2 * n, h->d_v2, nb, hipMemcpyDeviceToDevice, h->stream));
+        }
+        h->cur_hist = rec(s);
+        const int mode = (s > 1 ? M_HE_PREV : 0) | M_HE | M_HP2A | M_HP1 | M_HP2B;
+        rc = run_pass(h, mode, hdt, hdt, 1.0 * dt, s < T, 0);
+    }
+    h->cur_hist = nullptr;
+    if (!rc) rc = run_pass(h, M_HE_PREV, hdt, 0, 0, 0, 1);
+    if (rc) return rc;
+    h->steps_since_sort += T;
+
+    // ---- reverse ----
+    const Launch L = plan(h->n, h->adj_blocks);
+    AdjArgs A = adj_args(h, dt, L);
+    A.wloss = loss_weight;
+    HIP_TRY(hipMemsetAsync(h->d_agrid, 0, sizeof(double) * NASLOT * nx, h->stream));
+    const size_t fb = sizeof(double) * nx;
+    if (ct_e1) HIP_TRY(hipMemcpyAsync(h->d_agrid + A_E1 * nx, ct_e1, fb, hipMemcpyHostToDevice, h->stream));
+    if (ct_e2) HIP_TRY(hipMemcpyAsync(h->d_agrid + A_E2 * nx, ct_e2, fb, hipMemcpyHostToDevice, h->stream));
+    if (ct_b) HIP_TRY(hipMemcpyAsync(h->d_agrid + A_B * nx, ct_b, fb, hipMemcpyHostToDevice, h->stream));
+    if (ct_particles) {
+        rc = ensure_stage(h);
+        if (rc) return rc;
+        HIP_TRY(hipMemcpyAsync(h->d_stage, ct_particles, sizeof(double) * n * h->ld,
+                               hipMemcpyHostToDevice, h->stream));
+        HIP_TRY(launch_adj_rows_in(h->d_stage, h->ld, h->n, h->d_id, A.lx, A.lv1, A.lv2, h->stream));
+    } else {
+        HIP_TRY(hipMemsetAsync(h->d_lam, 0, 3 * nb, h->stream));
+    }
+    // the trailing HE, then the field part of the last step
+    A.x = h->d_x;
+    A.fields = rec(T + 1);
+    rc = adj_launch(h, R_HEONLY, A, L.blocks);
+    A.bend = rec(T) + H_BEND * nx;
+    if (!rc) rc = adj_epi(h, AE_TAIL, A, L.blocks);
+    const int64_t nck = (T + K - 1) / K;
+    for (int64_t j = nck - 1; j >= 0 && !rc; --j) {
+        const int64_t s_lo = j * K + 1, s_hi = std::min((j + 1) * K, T);
+        // state before step s: the checkpoint for s = s_lo, else the recompute buffer
+        auto state = [&](int64_t s, double **x, double **v1, double **v2) {
+            double *b = s == s_lo ? h->d_ckpt + (size_t)j * 3 * n : h->d_seg + (size_t)(s - s_lo - 1) * 3 * n;
+            *x = b;
+            *v1 = b + n;
+            *v2 = b + 2 * n;
+        };
+        for (int64_t s = s_lo; s < s_hi && !rc; ++s) {
+            double *x, *v1, *v2;
+            state(s, &x, &v1, &v2);
+            A.x = x; A.v1 = v1; A.v2 = v2;
+            state(s + 1, &A.xo, &A.v1o, &A.v2o);
+            A.fields = rec(s);
+            rc = adj_launch(h, R_FWD | (s > 1 ? R_PREV : 0), A, L.blocks);
+        }
+        for (int64_t s = s_hi; s >= s_lo && !rc; --s) {
+            double *x, *v1, *v2;
+            state(s, &x, &v1, &v2);
+            A.x = x; A.v1 = v1; A.v2 = v2;
+            A.fields = rec(s);
+            rc = adj_launch(h, R_REV | (s > 1 ? R_PREV : 0), A, L.blocks);
+            if (rc) break;
+            if (s > 1) {
+                A.bend = rec(s - 1) + H_BEND * nx;
+                rc = adj_epi(h, AE_STEP, A, L.blocks);
+            } else {
+                A.bend = nullptr;
+                rc = adj_epi(h, AE_HEAD, A, L.blocks);
+            }
+        }
+    }
+    if (rc) return rc;
+    h->adj_ready = true;
+    if (flags & GEMPIC_ASYNC) return GEMPIC_OK;
+    return finish(h);
+}
+
+int gempic_get_adjoint(gempic_splitting *h, double *loss, double *g_particles, int64_t ld, int64_t n,
+                       double *g_e1, double *g_e2, double *g_b)
+{
+    CHECK_H(h);
+    if (!h->adj_ready) return fail(GEMPIC_ERR_STATE, "no reverse-mode result: call gempic_strang_splitting_adjoint first");
+    const size_t nx = (size_t)h->nx, fb = sizeof(double) * nx;
+    if (g_particles) {
+        if (ld != h->ld) return fail(GEMPIC_ERR_ASSERTION, "AssertionError: leading dimension must be D+V+n_weights");
+        if (n != h->n) return fail(GEMPIC_ERR_ASSERTION, "AssertionError: particle count differs from the handle's");
+        int rc = ensure_stage(h);
+        if (rc) return rc;
+        HIP_TRY(launch_adj_rows_out(h->d_stage, ld, n, h->d_id, h->d_lam, h->d_lam + n, h->d_lam + 2 * n,
+                                    h->stream));
+        HIP_TRY(hipMemcpyAsync(g_particles, h->d_stage, sizeof(double) * (size_t)n * ld,
+                               hipMemcpyDeviceToHost, h->stream));
+    }
+    if (g_e1) HIP_TRY(hipMemcpyAsync(g_e1, h->d_agrid + A_E1 * nx, fb, hipMemcpyDeviceToHost, h->stream));
+    if (g_e2) HIP_TRY(hipMemcpyAsync(g_e2, h->d_agrid + A_E2 * nx, fb, hipMemcpyDeviceToHost, h->stream));
+    if (g_b) HIP_TRY(hipMemcpyAsync(g_b, h->d_agrid + A_B * nx, fb, hipMemcpyDeviceToHost, h->stream));
+    if (loss) HIP_TRY(hipMemcpyAsync(loss, h->d_agrid + A_LOSS * nx, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    return finish(h);
+}
+
+int gempic_solve_poisson_adjoint(gempic_splitting *h, const double *ct_e, double *g_x)
+{
+    CHECK_H(h);
+    CHECK_P(h);
+    if (!ct_e || !g_x) return fail(GEMPIC_ERR_ARGUMENT, "ct_e and g_x must not be NULL");
+    if (h->nx > kAdjNxMax) return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: the reverse mode needs nx <= 128");
+    int rc = adj_workspace(h, 1, 1);
+    if (!rc) rc = ensure_stage(h);
+    if (rc) return rc;
+    h->adj_ready = false;
+    const size_t nx = (size_t)h->nx, n = (size_t)h->n;
+    const Launch L = plan(h->n, h->adj_blocks);
+    AdjArgs A = adj_args(h, 0.0, L);
+    A.x = h->d_x;
+    HIP_TRY(hipMemcpyAsync(h->d_agrid + A_NU * nx, ct_e, sizeof(double) * nx, hipMemcpyHostToDevice, h->stream));
+    rc = adj_epi(h, AE_POIS, A, 0);
+    if (!rc) rc = adj_launch(h, R_POIS, A, L.blocks);
+    if (rc) return rc;
+    // one row out, through the caller's order: rows 0..2 of a 3-row staging array
+    HIP_TRY(launch_adj_rows_out(h->d_stage, 3, h->n, h->d_id, A.lx, A.lx, A.lx, h->stream));
+    std::vector<double> tmp(3 * n);
+    HIP_TRY(hipMemcpyAsync(tmp.data(), h->d_stage, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, h->stream));
+    rc = finish(h);
+    if (rc) return rc;
+    for (size_t i = 0; i < n; ++i) g_x[i] = tmp[3 * i];
+    return GEMPIC_OK;
 }
 
 // ---------------------------------------------------------------------------

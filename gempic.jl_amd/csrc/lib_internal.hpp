@@ -12,6 +12,7 @@ constexpr int kWaves = kBlock / 64;
 constexpr int kNxMax = 256;               // largest 1D grid handled by the device path
 constexpr int kGroup = 16;                // workgroups per first-level reduction group
 constexpr int kEpiGroupsMax = 64;         // workgroups of the deposit-sum kernel
+constexpr int kAdjNxMax = 128;            // largest 1D grid of the reverse-mode kernels
 
 // grid slots (nx doubles each) of a splitting's device field buffer
 // (S_E1M, S_E2M, S_BM: the Boris splitting's e_dofs_mid and b_dofs_mid)
@@ -57,6 +58,9 @@ struct PassArgs {
     unsigned *cnt;                // ngroups + 1 counters (zero between launches)
     int *err;                     // error flags
     double *diag;                 // diagnostics output (11 doubles)
+    double *hist;                 // fused Strang step with a record: k_epi1d writes B at the
+                                  // step's end to hist[4][nx] and the next pass's fields
+                                  // E1, E2, B, E2P to hist[5..8][nx] (nullptr: no record)
 };
 
 // error flag bits set by kernels
@@ -112,6 +116,46 @@ hipError_t launch_boris(int p0, int p1, int ops, const PassArgs &a, int blocks, 
 hipError_t launch_boris_epi(int kind, const PassArgs &a, int pass_blocks, hipStream_t s);
 size_t boris_shmem_bytes(int nx);
 int boris_blocks_per_cu(int p0, int p1, size_t shmem);
+
+// ---------------------------------------------------------------------------
+// Reverse mode of the fused 1D2V Strang pass (k_pass1d<P0,P1,HE_PREV|HE|HP2A|HP1|HP2B>).
+// ---------------------------------------------------------------------------
+// What a reverse-mode particle kernel does:
+//   R_FWD     re-run the pass forward (no deposits) from x, v1, v2 into xo, v1o, v2o
+//   R_REV     cotangents through the pass: (lx, lv1, lv2) after it -> before it,
+//             deposit rows A_E1 (S1 basis at x), A_E2 (S0 basis at x), A_B (S1)
+//   R_HEONLY  cotangents through the trailing HE of a call
+//   R_POIS    lx = d <ct, compute_e_from_rho!(rho(x))> / dx with nu = C_pois D^T ct
+//   R_PREV    (with R_FWD / R_REV) the pass starts with the previous step's HE
+enum AdjMode : int { R_PREV = 1, R_FWD = 2, R_REV = 4, R_HEONLY = 8, R_POIS = 16 };
+// adjoint grid slots (nx doubles each) and deposit rows of a reverse pass
+enum AdjSlot { A_E1 = 0, A_E2, A_B, A_MU1, A_MU2, A_NU, A_LOSS, NASLOT };
+enum AdjRow { AR_E1 = 0, AR_E2, AR_B };
+// field part of a reverse step: the reverse of the grid work between the
+// passes (k_epi1d and the prologue), see k_adj_epi
+enum AdjEpi : int { AE_TAIL = 0, AE_STEP, AE_HEAD, AE_POIS };
+// field record of one step (nx doubles each): the fields the step's pass
+// reads, then B at the step's end
+enum HistSlot { H_E1 = 0, H_E2, H_B, H_E2P, H_BEND, NHIST };
+
+struct AdjArgs {
+    PassArgs p;                   // geometry, constants, w, reduction scratch, error flags
+    const double *fields;         // this pass's record: [NHIST][nx]
+    const double *bend;           // the field part's B at a step's end (nx), or nullptr
+    double *agrid;                // [NASLOT][nx]
+    const double *x, *v1, *v2;    // primal state before the pass
+    double *xo, *v1o, *v2o;       // R_FWD: state after the pass
+    double *lx, *lv1, *lv2;       // cotangents, updated in place
+    double wloss;                 // weight of 1/2 l2norm_squared(B) per step
+};
+hipError_t launch_adj(int p0, int p1, int rm, const AdjArgs &a, int blocks, hipStream_t s);
+hipError_t launch_adj_epi(int kind, const AdjArgs &a, int pass_blocks, hipStream_t s);
+int adj_blocks_per_cu(int p0, int p1, int rm);
+// AoS rows 0..2 of the caller's array (particle id[i] of the caller) into SoA, and back
+hipError_t launch_adj_rows_in(const double *aos, int64_t ld, int64_t n, const int64_t *id,
+                              double *r0, double *r1, double *r2, hipStream_t s);
+hipError_t launch_adj_rows_out(double *aos, int64_t ld, int64_t n, const int64_t *id,
+                               const double *r0, const double *r1, const double *r2, hipStream_t s);
 
 // Stable counting sort by (cell, sub-cell bin): bins = nx * sub, tile-major
 // counts. Sorting finer than the cell keeps each wave's particles within a

@@ -9,6 +9,7 @@
 // the sub-steps (the "epilogue") so that a whole Strang step is one launch.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 #include "device_common.hpp"
 #include "lib_internal.hpp"
@@ -798,8 +799,11 @@ __global__ void __launch_bounds__(kBlock) k_epi1d(PassArgs a, int nblocks)
             G[S_J1 * nx + i] = 0.0;
             G[S_J2 * nx + i] = r_j2b[i] * a.dt_hp2;
             G[S_E2P * nx + i] = sE2[i];
+            if (a.hist) a.hist[(NHIST + H_E2P) * nx + i] = sE2[i];
         }
         ep_b_from_e(sB, sE2, a.dt_he, dx, nx);
+        if (a.hist)
+            for (int i = tid; i < nx; i += nt) a.hist[H_BEND * nx + i] = sB[i];
         // HB of this step and (a.more) of the next one: compute_e_from_b! twice
         // with the same B, so the weak curl W(B) is solved once and added twice
         circ_solve(T + T_AMP * nx, sB, work, hc, T + T_COS * nx, T + T_SIN * nx, nx);
@@ -818,6 +822,12 @@ __global__ void __launch_bounds__(kBlock) k_epi1d(PassArgs a, int nblocks)
             G[S_E2 * nx + i] = sE2[i];
             G[S_B * nx + i] = sB[i];
         }
+        if (a.hist)
+            for (int i = tid; i < nx; i += nt) {
+                a.hist[(NHIST + H_E1) * nx + i] = sE1[i];
+                a.hist[(NHIST + H_E2) * nx + i] = sE2[i];
+                a.hist[(NHIST + H_B) * nx + i] = sB[i];
+            }
     }
 }
 
@@ -1989,6 +1999,606 @@ hipError_t launch_pmc(int op, const PmcArgs &a, int blocks, hipStream_t s)
     case 5: return launch_pmc_p<5>(op, a, blocks, s);
     default: return hipErrorInvalidValue;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Reverse mode of the fused 1D2V Strang step.
+//
+// The forward pass k_pass1d<P0,P1,HE_PREV|HE|HP2A|HP1|HP2B> maps a particle
+// (x, v1, v2) through HE of the previous step, HE, Hp2, Hp1, Hp2 of this one
+// (hamiltonian_splitting_1d2v.jl:41-219) with fields that stay fixed during
+// the pass. k_adj1d re-runs that map in registers from the state before the
+// pass (bit for bit, with the forward's operations in the forward's order)
+// and applies the transpose of its Jacobian to the particle's cotangents:
+//
+//   Hp2 (v1 += c qm v2 B(x), j2 += q v2 S0(x)):
+//       lv2 += c qm B(x) lv1 + q s0 mu2(x),
+//       lx  += c qm v2 lv1 B'(x) + q s0 v2 mu2'(x),   lambda_B += c qm v2 lv1 S1(x)
+//   Hp1 (x_new = x + dt v1, v2 -= qm int_x^x_new B, j1 += q int S1, x = mod(x_new)):
+//       lxn = lx - qm lv2 B(x_new) + q s1 mu1(x_new),
+//       lx  = lxn + qm lv2 B(x) - q s1 mu1(x),  lv1 += dt lxn,  lambda_B -= qm lv2 int S1
+//   HE (v += c qm E(x)):
+//       lx += c qm (lv1 E1'(x) + lv2 E2'(x)),  lambda_E1 += c qm lv1 S1(x),
+//       lambda_E2 += c qm lv2 S0(x)
+//
+// mu1 = -(1/dx) M1^-1 lambda_E1 and mu2 = -(c/dx) M0^-1 lambda_E2 are the
+// cotangents of the pass's current deposits, from the field part of the
+// reverse step (k_adj_epi). Endpoint values use the trunc() cells of
+// add_current_update_v! (particle_mesh_coupling_1d.jl:307,313), for which the
+// Gauss-Legendre line integral is exact, so its derivatives are the spline
+// values there. B' = (1/dx) sum B_k S1_k'(t), S_p'[r] = S_{p-1}[r-1] - S_{p-1}[r].
+// The cotangent deposits use the same cached, deterministic accumulation as
+// the forward pass.
+// ---------------------------------------------------------------------------
+
+template <int P>
+__device__ __forceinline__ void dbspline(double off, double (&d)[P + 1])
+{
+    if constexpr (P == 0) {
+        d[0] = 0.0;
+    } else {
+        double lo[P];
+        bspline<P - 1>(off, lo);
+        d[0] = -lo[0];
+#pragma unroll
+        for (int r = 1; r < P; ++r) d[r] = lo[r - 1] - lo[r];
+        d[P] = lo[P - 1];
+    }
+}
+
+template <int P0, int P1>
+constexpr int adj_slots() { return 4 * (P1 + 1) + (P0 + 1); }
+
+size_t adj_shmem_bytes(int nx, int slots)
+{
+    return sizeof(double) *
+           ((size_t)6 * (nx + 2 * kPad) + (size_t)kWaves * NDEP * nx + (size_t)kWaves * slots * kWave);
+}
+
+template <int P0, int P1, int RM>
+__global__ void __launch_bounds__(kBlock) k_adj1d(AdjArgs A)
+{
+    constexpr bool PREV = (RM & R_PREV) != 0;
+    constexpr bool FWD = (RM & R_FWD) != 0;
+    constexpr bool REV = (RM & R_REV) != 0;
+    constexpr bool HEONLY = (RM & R_HEONLY) != 0;
+    constexpr bool POIS = (RM & R_POIS) != 0;
+    constexpr bool STEP = FWD || REV;
+    constexpr bool DEP = REV || HEONLY;
+    constexpr int NQ = (P1 + 2) / 2;
+    constexpr int W0 = P0 + 1, W1 = P1 + 1;
+    constexpr int SL = adj_slots<P0, P1>();
+    static_assert(P0 <= kPad && P1 <= kPad, "LDS field padding must cover the spline support");
+
+    const PassArgs &a = A.p;
+    extern __shared__ double smem[];
+    const int nx = a.nx;
+    const int tid = threadIdx.x;
+    const int wave = tid / kWave;
+    const int nxp = nx + 2 * kPad;
+    double *pE1 = smem, *pE2 = pE1 + nxp, *pB = pE2 + nxp, *pE2p = pB + nxp;
+    double *pM1 = pE2p + nxp, *pM2 = pM1 + nxp;
+    double *rows = pM2 + nxp;                               // [kWaves][NDEP][nx]
+    double *slots = rows + kWaves * NDEP * nx + wave * (SL * kWave);
+    const double *sE1 = pE1 + kPad, *sE2 = pE2 + kPad, *sB = pB + kPad, *sE2p = pE2p + kPad;
+    const double *sM1 = pM1 + kPad, *sM2 = pM2 + kPad;
+
+    if constexpr (!POIS) {
+        const double *F = A.fields;
+        load_padded(pE1, F + H_E1 * nx, nx);
+        load_padded(pE2, F + H_E2 * nx, nx);
+        load_padded(pB, F + H_B * nx, nx);
+        load_padded(pE2p, F + H_E2P * nx, nx);
+    }
+    if constexpr (REV) {
+        load_padded(pM1, A.agrid + A_MU1 * nx, nx);
+        load_padded(pM2, A.agrid + A_MU2 * nx, nx);
+    }
+    if constexpr (POIS) load_padded(pM1, A.agrid + A_NU * nx, nx);
+    for (int i = tid; i < kWaves * NDEP * nx; i += kBlock) rows[i] = 0.0;
+    __syncthreads();
+    double *myrow = rows + wave * NDEP * nx;
+
+    Cache2<W1> c_e1, c_b0, c_bl, c_bn;
+    Cache2<W0> c_e2;
+    if constexpr (DEP) {
+        c_e1.init(slots);
+        c_e2.init(slots + W1 * kWave);
+    }
+    if constexpr (REV) {
+        c_b0.init(slots + (W1 + W0) * kWave);
+        c_bl.init(slots + (2 * W1 + W0) * kWave);
+        c_bn.init(slots + (3 * W1 + W0) * kWave);
+    }
+
+    const double xmin = a.xmin, dx = a.dx, qm = a.qm;
+    const double dtqm = a.dtqm_he, g = a.dtqm_hp2, dt1 = a.dt_hp1;
+    const double s0 = a.scaling0, s1 = a.scaling1;
+    const int64_t base = (int64_t)blockIdx.x * a.chunk;
+    const int64_t left = a.n - base;
+    const int cnt_here = left < a.chunk ? (int)left : (int)a.chunk;
+
+    for (int it = 0; it < a.iters; ++it) {
+        const int i = it * kBlock + tid;
+        const bool valid = i < cnt_here;
+        const int64_t gi = base + (valid ? i : 0);
+        double x = valid ? A.x[gi] : xmin;
+        double v1 = 0.0, v2 = 0.0, w = 0.0;
+        if (valid) {
+            w = a.w[gi];
+            if constexpr (STEP) {
+                v1 = A.v1[gi];
+                v2 = A.v2[gi];
+            }
+        }
+        double lx = 0.0, lv1 = 0.0, lv2 = 0.0;
+        if (valid && (DEP || POIS)) lx = A.lx[gi];
+        if (valid && DEP) {
+            lv1 = A.lv1[gi];
+            lv2 = A.lv2[gi];
+        }
+        const double q = (w * a.charge) * a.cw;
+        const double xi = (x - xmin) / dx;
+        const int ic = pos_ok(xi) ? (int)xi : 0;
+        const double off = xi - (double)ic;
+        const int key = imod(ic, nx);
+        double b1[W1], b0[W0];
+        bspline<P1>(off, b1);
+        bspline<P0>(off, b0);
+
+        if constexpr (POIS) {
+            double db0[W0];
+            dbspline<P0>(off, db0);
+            const double dnu = gather_p<P0>(db0, sM1, key) / dx;
+            if (valid) A.lx[gi] = q * s0 * dnu;
+            continue;
+        }
+        if constexpr (HEONLY) {
+            double db1[W1], db0[W0];
+            dbspline<P1>(off, db1);
+            dbspline<P0>(off, db0);
+            const double de1 = gather_p<P1>(db1, sE1, key) / dx;
+            const double de2p = gather_p<P0>(db0, sE2p, key) / dx;
+            lx = lx + dtqm * (lv1 * de1 + lv2 * de2p);
+            double ve[W1], vf[W0];
+#pragma unroll
+            for (int k = 0; k < W1; ++k) ve[k] = (dtqm * lv1) * b1[k];
+#pragma unroll
+            for (int k = 0; k < W0; ++k) vf[k] = (dtqm * lv2) * b0[k];
+            cache_add<W1>(c_e1, valid, key, ve, myrow + AR_E1 * nx, nx);
+            cache_add<W0>(c_e2, valid, key, vf, myrow + AR_E2 * nx, nx);
+            if (valid) A.lx[gi] = lx;
+            continue;
+        }
+
+        // ---- the forward pass, as k_pass1d computes it ----
+        const double e1 = gather_p<P1>(b1, sE1, key);
+        if constexpr (PREV) {
+            const double e2 = gather_p<P0>(b0, sE2p, key);
+            v1 = v1 + dtqm * e1;
+            v2 = v2 + dtqm * e2;
+        }
+        {
+            const double e2 = gather_p<P0>(b0, sE2, key);
+            v1 = v1 + dtqm * e1;
+            v2 = v2 + dtqm * e2;
+        }
+        const double bA = gather_p<P1>(b1, sB, key);
+        const double v2b = v2;
+        v1 = v1 + g * v2 * bA;
+        const double xn = x + dt1 * v1;
+        const double xin = (xn - xmin) / dx;
+        const int inw = pos_ok(xin) ? (int)xin : 0;
+        const double rn = xin - (double)inw;
+        const int dseg = inw - ic;
+        const bool far = !pos_ok(xin) || !pos_ok(xi) || dseg > kMaxCellsPerStep ||
+                         dseg < -kMaxCellsPerStep;
+        const int nseg = far ? 1 : 1 + (dseg < 0 ? -dseg : dseg);
+        for (int sgi = 0; sgi < nseg; ++sgi) {
+            double lo, up, sg;
+            int cell;
+            segment_bounds(sgi, ic, off, inw, rn, lo, up, sg, cell);
+            double sv[W1];
+            segment<P1, NQ>(lo, up, sg, a.qx, a.qw, dx, sv);
+            const double *bk = sB + (imod(cell, nx) - P1);
+#pragma unroll
+            for (int m = 0; m < W1; ++m) v2 = v2 - qm * sv[m] * bk[m];
+        }
+        const double x1 = jl_mod(xn, a.Lx);
+        const bool same = __double_as_longlong(x1) == __double_as_longlong(xn);
+        const double xi2 = same ? xin : (x1 - xmin) / dx;
+        const int ic2 = pos_ok(xi2) ? (int)xi2 : 0;
+        const double off2 = xi2 - (double)ic2;
+        const int key2 = imod(ic2, nx);
+        double c0[W0], c1[W1];
+        bspline<P0>(off2, c0);
+        bspline<P1>(off2, c1);
+        const double bB = gather_p<P1>(c1, sB, key2);
+        if constexpr (FWD) {
+            v1 = v1 + g * v2 * bB;
+            if (valid) {
+                A.xo[gi] = x1;
+                A.v1o[gi] = v1;
+                A.v2o[gi] = v2;
+            }
+            continue;
+        }
+
+        // ---- reverse ----
+        if constexpr (REV) {
+            const double v2d = v2;
+            // second Hp2
+            {
+                double dc0[W0], dc1[W1];
+                dbspline<P0>(off2, dc0);
+                dbspline<P1>(off2, dc1);
+                const double mu = gather_p<P0>(c0, sM2, key2);
+                const double dmu = gather_p<P0>(dc0, sM2, key2) / dx;
+                const double db = gather_p<P1>(dc1, sB, key2) / dx;
+                const double gl1 = g * v2d * lv1;
+                lx = lx + gl1 * db + q * s0 * v2d * dmu;
+                lv2 = lv2 + g * bB * lv1 + q * s0 * mu;
+                double vb[W1];
+#pragma unroll
+                for (int k = 0; k < W1; ++k) vb[k] = gl1 * c1[k];
+                cache_add<W1>(c_bn, valid, key2, vb, myrow + AR_B * nx, nx);
+            }
+            // Hp1
+            {
+                double bn[W1];
+                bspline<P1>(rn, bn);
+                const int keyn = imod(inw, nx);
+                const double bN = gather_p<P1>(bn, sB, keyn);
+                const double muN = gather_p<P1>(bn, sM1, keyn);
+                const double muO = gather_p<P1>(b1, sM1, key);
+                const double lxn = lx - qm * lv2 * bN + q * s1 * muN;
+                lx = lxn + qm * lv2 * bA - q * s1 * muO;
+                lv1 = lv1 + dt1 * lxn;
+                const double wl = -qm * lv2;
+                for (int sgi = 0;; ++sgi) {
+                    const bool act = valid && sgi < nseg;
+                    if (!__ballot(act)) break;
+                    double lo, up, sg;
+                    int cell;
+                    segment_bounds(sgi, ic, off, inw, rn, lo, up, sg, cell);
+                    double sv[W1];
+                    segment<P1, NQ>(lo, up, sg, a.qx, a.qw, dx, sv);
+                    double vb[W1];
+#pragma unroll
+                    for (int m = 0; m < W1; ++m) vb[m] = wl * sv[m];
+                    cache_add<W1>(c_bl, act, imod(cell, nx), vb, myrow + AR_B * nx, nx);
+                }
+            }
+            // first Hp2
+            double db1[W1], db0[W0];
+            dbspline<P1>(off, db1);
+            dbspline<P0>(off, db0);
+            {
+                const double mu = gather_p<P0>(b0, sM2, key);
+                const double dmu = gather_p<P0>(db0, sM2, key) / dx;
+                const double db = gather_p<P1>(db1, sB, key) / dx;
+                const double gl1 = g * v2b * lv1;
+                lx = lx + gl1 * db + q * s0 * v2b * dmu;
+                lv2 = lv2 + g * bA * lv1 + q * s0 * mu;
+                double vb[W1];
+#pragma unroll
+                for (int k = 0; k < W1; ++k) vb[k] = gl1 * b1[k];
+                cache_add<W1>(c_b0, valid, key, vb, myrow + AR_B * nx, nx);
+            }
+            // HE of this step and of the previous one: same x and cotangents
+            {
+                const double de1 = gather_p<P1>(db1, sE1, key) / dx;
+                const double de2 = gather_p<P0>(db0, sE2, key) / dx;
+                lx = lx + dtqm * (lv1 * de1 + lv2 * de2);
+                if constexpr (PREV) {
+                    const double de2p = gather_p<P0>(db0, sE2p, key) / dx;
+                    lx = lx + dtqm * (lv1 * de1 + lv2 * de2p);
+                }
+                double ve[W1], vf[W0];
+#pragma unroll
+                for (int k = 0; k < W1; ++k) ve[k] = (dtqm * lv1) * b1[k];
+#pragma unroll
+                for (int k = 0; k < W0; ++k) vf[k] = (dtqm * lv2) * b0[k];
+                cache_add<W1>(c_e1, valid, key, ve, myrow + AR_E1 * nx, nx);
+                cache_add<W0>(c_e2, valid, key, vf, myrow + AR_E2 * nx, nx);
+            }
+            if (valid) {
+                A.lx[gi] = lx;
+                A.lv1[gi] = lv1;
+                A.lv2[gi] = lv2;
+            }
+        }
+    }
+
+    if constexpr (DEP) {
+        cache_drain<W1>(c_e1, myrow + AR_E1 * nx, nx);
+        cache_drain<W0>(c_e2, myrow + AR_E2 * nx, nx);
+        if constexpr (REV) {
+            cache_drain<W1>(c_b0, myrow + AR_B * nx, nx);
+            cache_drain<W1>(c_bl, myrow + AR_B * nx, nx);
+            cache_drain<W1>(c_bn, myrow + AR_B * nx, nx);
+        }
+        __syncthreads();
+        constexpr unsigned MASK = (1u << AR_E1) | (1u << AR_E2) | (REV ? 1u << AR_B : 0u);
+        const int len = NDEP * nx;
+        for (int idx = tid; idx < len; idx += kBlock) {
+            if (!((MASK >> (idx / nx)) & 1u)) continue;
+            double s = 0.0;
+            for (int wv = 0; wv < kWaves; ++wv) s += rows[wv * len + idx];
+            a.part[(size_t)blockIdx.x * len + idx] = s;
+        }
+    }
+}
+
+// lambda_e[j] += coef (lambda_b[j+1] - lambda_b[j]): the transpose of
+// compute_b_from_e! (maxwell_1d_fem.jl:407-420), by the whole workgroup
+__device__ void adj_b_from_e(double *le, double coef, const double *lb, int n)
+{
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        const int jp = j + 1 == n ? 0 : j + 1;
+        le[j] = le[j] + coef * (lb[jp] - lb[j]);
+    }
+    __syncthreads();
+}
+
+// The field part of a reverse step: the transpose of the grid work between
+// two forward passes, in reverse order, with the deposit rows of the reverse
+// pass just run. With c = dt/2, U = compute_b_from_e!(c), H = compute_e_from_b!(c),
+// L = the loss at a step's end:
+//   AE_TAIL  (after the trailing HE):     L, H, U, then the HE rows at E_c
+//   AE_STEP  (between steps s and s+1):   the pass's rows at (E_f, B_b), U, H, L, H, U,
+//            then the HE rows again at E_c (HE of step s)
+//   AE_HEAD  (the prologue):              the pass's rows at (E_f, B_b), U, H
+// and then, for AE_TAIL and AE_STEP, mu1 = -(1/dx) M1^-1 lambda_E1 and
+// mu2 = -(c/dx) M0^-1 lambda_E2 for the next reverse pass. AE_POIS computes
+// nu = C_pois D^T lambda_E1 for the solve_poisson! gradient.
+template <int KIND>
+__global__ void __launch_bounds__(kBlock) k_adj_epi(AdjArgs A, int nblocks)
+{
+    const PassArgs &a = A.p;
+    extern __shared__ double smem[];
+    __shared__ int s_last;
+    const int nx = a.nx;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int len = NDEP * nx;
+    const double dx = a.dx;
+    double *lE1 = smem, *lE2 = lE1 + nx, *lB = lE2 + nx;
+    double *red = lB + nx;                  // [NDEP][nx]
+    double *w3 = red + NDEP * nx;           // [6][nx]: three outputs, three workspaces
+    double *T = w3 + 6 * nx;                // [NTAB][nx]
+    double *ampt = T + NTAB * nx;           // conjugate weak-Ampere symbol
+    double *bend = ampt + nx;
+    double *scratch = bend + nx;            // [blockDim.x]
+    double *G = A.agrid;
+
+    for (int i = tid; i < nx; i += nt) {
+        lE1[i] = G[A_E1 * nx + i];
+        lE2[i] = G[A_E2 * nx + i];
+        lB[i] = G[A_B * nx + i];
+        bend[i] = A.bend ? A.bend[i] : 0.0;
+    }
+    for (int i = tid; i < NTAB * nx; i += nt) T[i] = a.tabs[i];
+    for (int i = tid; i < len; i += nt) red[i] = 0.0;
+    __syncthreads();
+    for (int k = tid; k < nx; k += nt) ampt[k] = k > nx / 2 ? -T[T_AMP * nx + k] : T[T_AMP * nx + k];
+
+    constexpr unsigned MASK = KIND == AE_TAIL ? (1u << AR_E1) | (1u << AR_E2)
+                              : KIND == AE_POIS ? 0u
+                                                : (1u << AR_E1) | (1u << AR_E2) | (1u << AR_B);
+    if constexpr (MASK != 0u) {
+        const int ng = gridDim.x, gb = blockIdx.x;
+        const int b0 = (int)(((long)nblocks * gb) / ng), b1 = (int)(((long)nblocks * (gb + 1)) / ng);
+        sum_rows_range(a.part, b0, b1, nx, MASK, a.gpart + (size_t)gb * len, scratch);
+        __threadfence();
+        __syncthreads();
+        if (tid == 0) s_last = atomicAdd(&a.cnt[0], 1u) == (unsigned)(ng - 1);
+        __syncthreads();
+        if (!s_last) return;
+        __threadfence();
+        if (tid == 0) a.cnt[0] = 0u;
+        sum_rows_range(a.gpart, 0, ng, nx, MASK, red, scratch);
+    }
+    __syncthreads();
+    const double *dE1 = red + AR_E1 * nx, *dE2 = red + AR_E2 * nx, *dB = red + AR_B * nx;
+    const double c = a.dt_he;
+    const double coef = c / dx;
+    const double *cs = T + T_COS * nx, *sn = T + T_SIN * nx;
+    double *wA = w3, *wM = w3 + nx, *w1 = w3 + 2 * nx;
+
+    if constexpr (KIND == AE_POIS) {
+        // the cotangent of E arrives in the A_NU slot and nu replaces it
+        double *tmp = w3 + 3 * nx;
+        for (int i = tid; i < nx; i += nt) {
+            tmp[i] = 0.0;
+            bend[i] = G[A_NU * nx + i];
+        }
+        __syncthreads();
+        adj_b_from_e(tmp, 1.0, bend, nx);
+        circ_solve(T + T_POIS * nx, tmp, wA, w3 + 4 * nx, cs, sn, nx);
+        for (int i = tid; i < nx; i += nt) G[A_NU * nx + i] = wA[i];
+        return;
+    }
+    if constexpr (KIND == AE_STEP || KIND == AE_HEAD) {
+        // cotangents of the fields the pass read: B_b, E1 and E2 of its first HE
+        for (int i = tid; i < nx; i += nt) {
+            lB[i] = lB[i] + dB[i];
+            lE1[i] = lE1[i] + dE1[i];
+            lE2[i] = lE2[i] + dE2[i];
+        }
+        __syncthreads();
+        adj_b_from_e(lE2, coef, lB, nx);                       // U of the next step's HE
+    }
+    if constexpr (KIND == AE_HEAD) {
+        circ_solve(ampt, lE2, wA, w3 + 3 * nx, cs, sn, nx);     // H of the first step
+        for (int i = tid; i < nx; i += nt) {
+            lB[i] = lB[i] + coef * wA[i];
+            G[A_E1 * nx + i] = lE1[i];
+            G[A_E2 * nx + i] = lE2[i];
+            G[A_B * nx + i] = lB[i];
+        }
+        return;
+    }
+    if constexpr (KIND == AE_STEP || KIND == AE_TAIL) {
+        // lambda_E1 is final once the rows of this step's last HE are added
+        for (int i = tid; i < nx; i += nt) lE1[i] = lE1[i] + dE1[i];
+        __syncthreads();
+        const SolveArgs sa{ampt, lE2, 1.0, wA, w3 + 3 * nx};
+        const SolveArgs sm{T + T_M1 * nx, bend, 1.0, wM, w3 + 4 * nx};
+        const SolveArgs s1{T + T_INVM1 * nx, lE1, 1.0, w1, w3 + 5 * nx};
+        circ_solve3(sa, sm, s1, cs, sn, nx);
+        const double wl = A.wloss;
+        for (int i = tid; i < nx; i += nt) {
+            double b = lB[i];
+            if (KIND == AE_STEP) b = b + coef * wA[i];         // H of the next step
+            b = b + wl * dx * wM[i];                           // L at this step's end
+            b = b + coef * wA[i];                              // H of this step
+            lB[i] = b;
+            G[A_MU1 * nx + i] = -w1[i] / dx;
+        }
+        if (tid == 0) {
+            double r = 0.0;
+            for (int i = 0; i < nx; ++i) r += bend[i] * wM[i];
+            G[A_LOSS * nx] = G[A_LOSS * nx] + wl * 0.5 * (r * dx);
+        }
+        __syncthreads();
+        adj_b_from_e(lE2, coef, lB, nx);                       // U of this step's HE
+        for (int i = tid; i < nx; i += nt) lE2[i] = lE2[i] + dE2[i];
+        __syncthreads();
+        circ_solve(T + T_INVM0 * nx, lE2, wA, w3 + 3 * nx, cs, sn, nx);
+        for (int i = tid; i < nx; i += nt) {
+            G[A_MU2 * nx + i] = -(c / dx) * wA[i];
+            G[A_E1 * nx + i] = lE1[i];
+            G[A_E2 * nx + i] = lE2[i];
+            G[A_B * nx + i] = lB[i];
+        }
+    }
+}
+
+size_t adj_epi_shmem_bytes(int nx)
+{
+    return sizeof(double) * ((size_t)(3 + NDEP + 6 + NTAB + 2) * nx + kBlock);
+}
+
+template <int P0, int P1>
+static hipError_t launch_adj_pd(int rm, const AdjArgs &a, int blocks, hipStream_t s)
+{
+    const size_t sh = adj_shmem_bytes(a.p.nx, adj_slots<P0, P1>());
+#define GK_CASE(M)                                                                          \
+    case (M):                                                                               \
+        hipLaunchKernelGGL((k_adj1d<P0, P1, (M)>), dim3(blocks), dim3(kBlock), sh, s, a); \
+        return hipGetLastError();
+    switch (rm) {
+        GK_CASE(R_FWD)
+        GK_CASE(R_FWD | R_PREV)
+        GK_CASE(R_REV)
+        GK_CASE(R_REV | R_PREV)
+        GK_CASE(R_HEONLY)
+        GK_CASE(R_POIS)
+    default:
+        return hipErrorInvalidValue;
+    }
+#undef GK_CASE
+}
+
+hipError_t launch_adj(int p0, int p1, int rm, const AdjArgs &a, int blocks, hipStream_t s)
+{
+    if (a.p.nx > kAdjNxMax || blocks < 1) return hipErrorInvalidValue;
+#define GK_PAIR(A, B) \
+    if (p0 == (A) && p1 == (B)) return launch_adj_pd<A, B>(rm, a, blocks, s);
+    GK_PAIR(1, 0)
+    GK_PAIR(1, 1)
+    GK_PAIR(2, 1)
+    GK_PAIR(2, 2)
+    GK_PAIR(3, 2)
+    GK_PAIR(3, 3)
+#undef GK_PAIR
+    return hipErrorInvalidValue;
+}
+
+// workgroups of the reverse pass resident on one CU, at the largest supported
+// grid (one launch geometry serves every nx, so the deposit sums of a run
+// always add the same rows in the same order)
+int adj_blocks_per_cu(int p0, int p1, int rm)
+{
+    int nb = 0;
+#define GK_Q(A, B)                                                                              \
+    if (p0 == (A) && p1 == (B)) {                                                               \
+        const size_t s_ = adj_shmem_bytes(kAdjNxMax, adj_slots<A, B>());                        \
+        if (rm & R_REV)                                                                         \
+            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_adj1d<A, B, R_REV | R_PREV>, \
+                                                               kBlock, s_);                      \
+        else                                                                                    \
+            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_adj1d<A, B, R_FWD | R_PREV>, \
+                                                               kBlock, s_);                      \
+    }
+    GK_Q(1, 0)
+    GK_Q(1, 1)
+    GK_Q(2, 1)
+    GK_Q(2, 2)
+    GK_Q(3, 2)
+    GK_Q(3, 3)
+#undef GK_Q
+    return nb > 0 ? nb : 1;
+}
+
+hipError_t launch_adj_epi(int kind, const AdjArgs &a, int pass_blocks, hipStream_t s)
+{
+    if (a.p.nx > kAdjNxMax) return hipErrorInvalidValue;
+    const size_t sh = adj_epi_shmem_bytes(a.p.nx);
+    const dim3 grid(kind == AE_POIS ? 1 : epi_groups(pass_blocks));
+    switch (kind) {
+    case AE_TAIL: hipLaunchKernelGGL(k_adj_epi<AE_TAIL>, grid, dim3(kBlock), sh, s, a, pass_blocks); break;
+    case AE_STEP: hipLaunchKernelGGL(k_adj_epi<AE_STEP>, grid, dim3(kBlock), sh, s, a, pass_blocks); break;
+    case AE_HEAD: hipLaunchKernelGGL(k_adj_epi<AE_HEAD>, grid, dim3(kBlock), sh, s, a, pass_blocks); break;
+    case AE_POIS: hipLaunchKernelGGL(k_adj_epi<AE_POIS>, grid, dim3(kBlock), sh, s, a, 0); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+__global__ void k_adj_rows_in(const double *aos, int64_t ld, int64_t n, const int64_t *id,
+                              double *r0, double *r1, double *r2)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = id[i];
+        if (j < 0 || j >= n) continue;
+        const double *p = aos + j * ld;
+        r0[i] = p[0];
+        r1[i] = p[1];
+        r2[i] = p[2];
+    }
+}
+
+__global__ void k_adj_rows_out(double *aos, int64_t ld, int64_t n, const int64_t *id,
+                               const double *r0, const double *r1, const double *r2)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = id[i];
+        if (j < 0 || j >= n) continue;
+        double *p = aos + j * ld;
+        p[0] = r0[i];
+        p[1] = r1[i];
+        p[2] = r2[i];
+        for (int64_t r = 3; r < ld; ++r) p[r] = 0.0;
+    }
+}
+
+hipError_t launch_adj_rows_in(const double *aos, int64_t ld, int64_t n, const int64_t *id,
+                              double *r0, double *r1, double *r2, hipStream_t s)
+{
+    int blocks = (int)std::min<int64_t>(4096, std::max<int64_t>(1, (n + 255) / 256));
+    hipLaunchKernelGGL(k_adj_rows_in, dim3(blocks), dim3(256), 0, s, aos, ld, n, id, r0, r1, r2);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_rows_out(double *aos, int64_t ld, int64_t n, const int64_t *id,
+                               const double *r0, const double *r1, const double *r2, hipStream_t s)
+{
+    int blocks = (int)std::min<int64_t>(4096, std::max<int64_t>(1, (n + 255) / 256));
+    hipLaunchKernelGGL(k_adj_rows_out, dim3(blocks), dim3(256), 0, s, aos, ld, n, id, r0, r1, r2);
+    return hipGetLastError();
 }
 
 }  // namespace gl

@@ -25,7 +25,8 @@ __all__ = [
     "operatorHB", "operatorHE", "operatorHp1", "operatorHp2", "strang_splitting_",
     "solve_poisson_", "TimeHistoryDiagnostics", "write_step_", "get_charge",
     "HamiltonianSplittingBoris", "staggering_", "push_v_epart_", "push_v_bpart_",
-    "push_x_accumulate_j_", "sample_", "sample_particles_",
+    "push_x_accumulate_j_", "sample_", "sample_particles_", "strang_splitting_adjoint_",
+    "get_adjoint", "solve_poisson_adjoint", "AdjointResult",
 ]
 
 
@@ -458,6 +459,74 @@ def strang_splitting_(h: HamiltonianSplitting, dt: float, number_steps: int,
         return
     flags = _lib.STRANG_SUBSTEP if substep else _lib.STRANG_FUSED
     h._run(_lib.load().gempic_strang_splitting, float(dt), int(number_steps), flags)
+
+
+class AdjointResult:
+    """Loss and cotangents of the state a reverse-mode call started from:
+    g_particles has the particle array's layout (rows x, v1, v2; the weight
+    row is zero), g_e1, g_e2, g_b the fields'."""
+
+    def __init__(self, loss, g_particles, g_e1, g_e2, g_b):
+        self.loss, self.g_particles = loss, g_particles
+        self.g_e1, self.g_e2, self.g_b = g_e1, g_e2, g_b
+
+
+def strang_splitting_adjoint_(h: HamiltonianSplitting, dt: float, number_steps: int,
+                              checkpoint_every: int = 50, loss_weight: float | None = None,
+                              ct_particles=None, ct_e1=None, ct_e2=None, ct_b=None,
+                              fetch: bool = True):
+    """Reverse mode of strang_splitting!(h, dt, number_steps) for 1d2v (the
+    reference has no derivatives; this is the pullback of an rrule). Runs the
+    steps forward, leaving h in the state strang_splitting_ would (without
+    the periodic particle sort), and returns the gradient of
+
+        loss_weight * sum_s 1/2 l2norm_squared(B_s, p-1) + <ct, final state>
+
+    with respect to the particles and fields before the call (loss_weight
+    defaults to dt: the time-integrated magnetic energy). Checkpoints every
+    `checkpoint_every` steps bound the memory; the intervals between them are
+    re-run on the GPU. With fetch=False the result stays on the GPU
+    (get_adjoint reads it)."""
+    if h.dims != (1, 2) or isinstance(h, HamiltonianSplittingBoris):
+        raise ValueError("the reverse mode covers HamiltonianSplitting{1,2}")
+    wl = float(dt) if loss_weight is None else float(loss_weight)
+    cp = None
+    if ct_particles is not None:
+        cp = np.asfortranarray(ct_particles, dtype=np.float64)
+        if cp.shape != h.particle_group.array.shape:
+            raise AssertionError("ct_particles must have the particle array's shape")
+    f = [None if c is None else np.ascontiguousarray(c, dtype=np.float64) for c in (ct_e1, ct_e2, ct_b)]
+    if h.sync:
+        h.to_device()
+    ld = h.particle_group.array.shape[0]
+    check(_lib.load().gempic_strang_splitting_adjoint(h._h, float(dt), int(number_steps),
+                                                      int(checkpoint_every), wl, ptr(cp), ld,
+                                                      ptr(f[0]), ptr(f[1]), ptr(f[2]), 0))
+    if h.sync:
+        h.to_host()
+    return get_adjoint(h) if fetch else None
+
+
+def get_adjoint(h: HamiltonianSplitting) -> AdjointResult:
+    """The result of the last strang_splitting_adjoint_ on h."""
+    a = h.particle_group.array
+    g = np.zeros_like(a, order="F")
+    nx = h.kernel_smoother_0.n_dofs
+    ge1, ge2, gb, loss = np.zeros(nx), np.zeros(nx), np.zeros(nx), np.zeros(1)
+    check(_lib.load().gempic_get_adjoint(h._h, ptr(loss), ptr(g), a.shape[0], a.shape[1],
+                                         ptr(ge1), ptr(ge2), ptr(gb)))
+    return AdjointResult(float(loss[0]), g, ge1, ge2, gb)
+
+
+def solve_poisson_adjoint(h: HamiltonianSplitting, ct_e) -> np.ndarray:
+    """Reverse mode of solve_poisson! (diagnostics.jl:15-31) in the particle
+    positions: d <ct_e, E> / dx_i, in the particle group's order."""
+    if h.sync:
+        h.to_device(fields=False)
+    g = np.zeros(h.particle_group.n_particles)
+    check(_lib.load().gempic_solve_poisson_adjoint(
+        h._h, ptr(np.ascontiguousarray(ct_e, dtype=np.float64)), ptr(g)))
+    return g
 
 
 # ---------------------------------------------------------------------------

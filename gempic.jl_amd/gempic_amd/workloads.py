@@ -94,3 +94,40 @@ def landau_1d1v(n, seed=1, nx=32, k=0.5, alpha=0.5, degree=3):
     parts[2] = L
     return dict(xmin=0.0, xmax=L, nx=nx, degree=degree, particles=parts,
                 common_weight=1.0 / n, dt=0.05)
+
+
+def newton_cos_cdf(u, L, k, alpha, iters=60):
+    """x in [0, L) with x + (alpha/k) sin(k x) = u L, the inverse cdf of the
+    density (1 + alpha cos(k x)) / L, by newton's method from x = u L
+    (vectorised; the same equation as particle_sampling.jl:236-245)."""
+    x = u * L
+    for _ in range(iters):
+        f = x + alpha * np.sin(k * x) / k - u * L
+        x = x - f / (1.0 + alpha * np.cos(k * x))
+    return x
+
+
+def weibel_adjoint_1d2v(n, seed=1, nx=32, k=1.25, alpha=1e-4, beta=1e-4,
+                        sigma1=0.02 / math.sqrt(2.0), sigma2=None, degree=3, dt=0.05):
+    """The 1D2V Weibel initial condition with positions that are a smooth
+    function of alpha, for gradients: x_i = inverse cdf of (i - 1/2)/n
+    (a stratified quiet start), v1 ~ N(0, s1^2), v2 ~ N(0, s2^2) (seeded),
+    weight = Lx, B0 = beta * P with P = l2projection!(cos(kx), p-1).
+    Returns also dx/dalpha per particle and P, so that with the cotangents
+    of the initial state, dJ/dbeta = <g_B, P> and
+    dJ/dalpha = sum_i (g_x[i] + g_pois[i]) dx_i/dalpha, where g_pois is the
+    solve_poisson! pullback of g_E1 (E1 is set by Poisson from x)."""
+    if sigma2 is None:
+        sigma2 = math.sqrt(12.0) * sigma1
+    L = 2.0 * math.pi / k
+    rng = np.random.default_rng(seed)
+    u = (np.arange(n) + 0.5) / n
+    x = newton_cos_cdf(u, L, k, alpha)
+    dx_dalpha = -(np.sin(k * x) / k) / (1.0 + alpha * np.cos(k * x))
+    v1 = sigma1 * rng.standard_normal(n)
+    v2 = sigma2 * rng.standard_normal(n)
+    parts = np.asfortranarray(np.vstack([x, v1, v2, np.full(n, L)]))
+    p = l2projection_cos(nx, L, degree - 1, 1.0, k)
+    return dict(xmin=0.0, xmax=L, nx=nx, degree=degree, particles=parts, b_pattern=p,
+                b0=beta * p, dx_dalpha=dx_dalpha, common_weight=1.0 / n, dt=dt, k=k,
+                alpha=alpha, beta=beta)

@@ -108,6 +108,34 @@ int gempic_strang_splitting(gempic_splitting *h, double dt, int64_t n_steps, int
 /* Wait for queued work and report any deferred error. */
 int gempic_synchronize(gempic_splitting *h);
 
+/* Reverse mode of strang_splitting! (1d2v). The reference has no
+ * derivatives; this is the pullback a ChainRulesCore.rrule of
+ * strang_splitting!(h, dt, n_steps) (hamiltonian_splitting.jl:98-108) would
+ * call. The call runs the n_steps forward (the handle ends in the same state
+ * as after gempic_strang_splitting, without the periodic sort), recording the
+ * fields of every step and the particles every checkpoint_every steps, then
+ * sweeps back, re-running each checkpoint interval forward on the GPU.
+ * The loss is
+ *   loss_weight * sum_{s=1..n_steps} 1/2 l2norm_squared(B_s, p-1)
+ *     + <ct_particles, final x, v1, v2> + <ct_e1, E1> + <ct_e2, E2> + <ct_b, B>
+ * (l2norm_squared: maxwell_1d_fem.jl:299-313; with loss_weight = dt the sum
+ * is the time-integrated magnetic energy of write_step!, diagnostics.jl:236).
+ * ct_* may be NULL (zero); ct_particles is the caller's AoS layout (rows x,
+ * v1, v2 read, ld = D+V+n_weights). nx <= 128. Results: gempic_get_adjoint. */
+int gempic_strang_splitting_adjoint(gempic_splitting *h, double dt, int64_t n_steps,
+                                    int64_t checkpoint_every, double loss_weight,
+                                    const double *ct_particles, int64_t ld, const double *ct_e1,
+                                    const double *ct_e2, const double *ct_b, int flags);
+/* The loss and the cotangents of the state the adjoint call started from:
+ * g_particles in the caller's AoS layout (rows x, v1, v2; other rows zero),
+ * g_e1, g_e2, g_b (nx each). Any output may be NULL. */
+int gempic_get_adjoint(gempic_splitting *h, double *loss, double *g_particles, int64_t ld,
+                       int64_t n, double *g_e1, double *g_e2, double *g_b);
+/* Reverse mode of solve_poisson! (diagnostics.jl:15-31) in the positions:
+ * g_x[i] = d <ct_e, E> / d x_i for E = compute_e_from_rho!(add_charge!(x))
+ * of the handle's particles, in the caller's order (n doubles). */
+int gempic_solve_poisson_adjoint(gempic_splitting *h, const double *ct_e, double *g_x);
+
 /* Particles are kept sorted by position on the device: by cell, then by
  * sub-cell bin (their order in the caller's array is restored by
  * gempic_get_particles). interval = re-sort every `interval` Strang steps

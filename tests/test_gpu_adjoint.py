@@ -1,0 +1,165 @@
+"""GPU reverse mode of the 1D2V Strang splitting through the C ABI, against
+the oracle's reverse mode (itself pinned by central differences) and against
+central differences of the GPU's own forward run.
+
+Tolerances: the GPU forward equals the oracle's to the deposit summation
+order (1e-12), and the cotangents inherit that: GPU and oracle gradients
+agree to 1e-9 of each group's largest entry. Against central differences
+(h = 1e-6 of the parameter's scale) the directional derivatives agree to
+1e-6 relative.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import gempic_amd as G
+from gempic_amd import workloads
+import pyoracle as orc
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-9
+
+
+def _hot(n, nx=16, seed=3, vscale=0.6):
+    rng = np.random.default_rng(seed)
+    L = 4 * math.pi
+    parts = np.asfortranarray(np.vstack([L * rng.random(n), vscale * rng.normal(size=n),
+                                         0.5 * rng.normal(size=n), rng.random(n) + 0.5]))
+    e1, e2, b = (0.3 * rng.normal(size=nx) for _ in range(3))
+    return parts, e1, e2, b, L, nx
+
+
+def _gpu(parts, e1, e2, b, L, nx, deg, cw, sync=True):
+    n = parts.shape[1]
+    mesh = G.OneDGrid(0.0, L, nx)
+    pg = G.ParticleGroup(1, 2, n, common_weight=cw)
+    pg.array[:] = parts
+    ks0 = G.ParticleMeshCoupling1D(mesh, n, deg[0], "galerkin")
+    ks1 = G.ParticleMeshCoupling1D(mesh, n, deg[1], "galerkin")
+    h = G.HamiltonianSplitting(1, 2, G.Maxwell1DFEM(mesh, 3), ks0, ks1, pg,
+                               [e1.copy(), e2.copy()], b.copy(), sync=sync)
+    return h, pg
+
+
+def _close(got, ref, rel=REL):
+    scale = max(np.max(np.abs(ref)), 1e-300)
+    err = np.max(np.abs(got - ref))
+    assert err <= rel * scale, (err, scale)
+
+
+@pytest.mark.parametrize("deg", [(3, 2), (2, 1)])
+@pytest.mark.parametrize("K", [1, 4, 6])
+def test_gpu_adjoint_matches_oracle(deg, K):
+    """Six steps of a hot state (particles cross cells and the domain's
+    ends), with cotangents on the final state and the magnetic-energy loss;
+    K = 4 re-runs an interval of four steps and a partial one of two."""
+    parts, e1, e2, b, L, nx = _hot(20000)
+    n = parts.shape[1]
+    dt, steps, wl, cw = 0.1, 6, 0.1, 0.01
+    rng = np.random.default_rng(11)
+    cp = np.zeros_like(parts, order="F")
+    cp[:3] = rng.normal(size=(3, n))
+    ct = (rng.normal(size=nx), rng.normal(size=nx), rng.normal(size=nx))
+    h, pg = _gpu(parts, e1, e2, b, L, nx, deg, cw)
+    r = G.strang_splitting_adjoint_(h, dt, steps, checkpoint_every=K, loss_weight=wl,
+                                    ct_particles=cp, ct_e1=ct[0], ct_e2=ct[1], ct_b=ct[2])
+    o_parts = parts.copy(order="F")
+    o_f = [e1.copy(), e2.copy(), b.copy()]
+    osp = orc.Splitting(nx, 0.0, L, 3, deg[0], deg[1], o_parts, *o_f, common_weight=cw)
+    J, gp, ge1, ge2, gb = osp.strang_adjoint(dt, steps, wl, cp, *ct)
+    # the forward state after the call
+    np.testing.assert_allclose(pg.array, o_parts, rtol=0, atol=1e-11)
+    assert abs(r.loss - J) <= 1e-12 * abs(J)
+    for row in range(3):
+        _close(r.g_particles[row], gp[row])
+    np.testing.assert_array_equal(r.g_particles[3], 0.0)
+    _close(r.g_e1, ge1)
+    _close(r.g_e2, ge2)
+    _close(r.g_b, gb)
+
+
+def test_gpu_adjoint_loss_only_and_deterministic():
+    parts, e1, e2, b, L, nx = _hot(30000, seed=4)
+    outs = []
+    for _ in range(2):
+        h, pg = _gpu(parts, e1, e2, b, L, nx, (3, 2), 0.01)
+        r = G.strang_splitting_adjoint_(h, 0.1, 5, checkpoint_every=2)
+        outs.append(r)
+    for a in ("g_particles", "g_e1", "g_e2", "g_b"):
+        np.testing.assert_array_equal(getattr(outs[0], a), getattr(outs[1], a))
+    assert outs[0].loss == outs[1].loss
+    o_parts = parts.copy(order="F")
+    osp = orc.Splitting(nx, 0.0, L, 3, 3, 2, o_parts, e1.copy(), e2.copy(), b.copy(),
+                        common_weight=0.01)
+    J, gp, ge1, ge2, gb = osp.strang_adjoint(0.1, 5, 0.1)
+    for row in range(3):
+        _close(outs[0].g_particles[row], gp[row])
+    _close(outs[0].g_b, gb)
+
+
+def test_gpu_solve_poisson_adjoint():
+    parts, e1, e2, b, L, nx = _hot(50000, seed=9)
+    h, pg = _gpu(parts, e1, e2, b, L, nx, (3, 2), 0.001)
+    ct = np.random.default_rng(5).normal(size=nx)
+    g = G.solve_poisson_adjoint(h, ct)
+    osp = orc.Splitting(nx, 0.0, L, 3, 3, 2, parts.copy(order="F"), e1.copy(), e2.copy(),
+                        b.copy(), common_weight=0.001)
+    np.testing.assert_allclose(g, osp.solve_poisson_adjoint(ct), rtol=1e-12, atol=1e-15)
+
+
+def _weibel_loss_and_grad(w, alpha_shift=0.0, beta_shift=0.0, steps=40, K=10, fetch=True):
+    """Loss of the Weibel case with E1 from Poisson; the particle positions
+    move with alpha to first order through dx/dalpha (exact for the test's
+    purposes: the same linearisation on both sides of the difference)."""
+    n, nx = w["particles"].shape[1], w["nx"]
+    parts = w["particles"].copy(order="F")
+    parts[0] = parts[0] + alpha_shift * w["dx_dalpha"]
+    b0 = w["b0"] + beta_shift * w["b_pattern"]
+    h, pg = _gpu(parts, np.zeros(nx), np.zeros(nx), b0, w["xmax"], nx, (3, 2), w["common_weight"])
+    e1, rho = np.zeros(nx), np.zeros(nx)
+    G.solve_poisson_(e1, h, rho)
+    h.e_dofs[0][:] = e1
+    r = G.strang_splitting_adjoint_(h, w["dt"], steps, checkpoint_every=K)
+    gpois = G.solve_poisson_adjoint(_gpu(parts, e1, np.zeros(nx), b0, w["xmax"], nx, (3, 2),
+                                         w["common_weight"])[0], r.g_e1)
+    return r, gpois
+
+
+def test_gpu_adjoint_weibel_parameter_gradients_vs_central_differences():
+    """dJ/dbeta and dJ/dalpha of the time-integrated magnetic energy of a
+    200,000-particle Weibel run (40 steps, checkpoints every 10) against
+    central differences of the GPU's own forward run."""
+    w = workloads.weibel_adjoint_1d2v(200000, seed=2, alpha=1e-2, beta=1e-3)
+    r, gpois = _weibel_loss_and_grad(w)
+    dj_dbeta = float(r.g_b @ w["b_pattern"])
+    dj_dalpha = float((r.g_particles[0] + gpois) @ w["dx_dalpha"])
+    hb = 1e-6 * w["beta"]
+    fd_b = (_weibel_loss_and_grad(w, beta_shift=hb)[0].loss -
+            _weibel_loss_and_grad(w, beta_shift=-hb)[0].loss) / (2 * hb)
+    assert abs(fd_b - dj_dbeta) <= 1e-6 * abs(fd_b), (fd_b, dj_dbeta)
+    ha = 1e-6 * w["alpha"]
+    fd_a = (_weibel_loss_and_grad(w, alpha_shift=ha)[0].loss -
+            _weibel_loss_and_grad(w, alpha_shift=-ha)[0].loss) / (2 * ha)
+    assert abs(fd_a - dj_dalpha) <= 1e-6 * abs(fd_a), (fd_a, dj_dalpha)
+
+
+def test_gpu_adjoint_velocity_gradient_vs_central_differences():
+    """A random direction in all initial velocities (200,000 particles, 40
+    steps): <g_v, d> against the central difference of the GPU loss."""
+    w = workloads.weibel_adjoint_1d2v(200000, seed=3, alpha=1e-2, beta=1e-3)
+    nx = w["nx"]
+    d = np.random.default_rng(1).normal(size=(2, w["particles"].shape[1])) * 0.02
+
+    def run(eps, fetch=True):
+        parts = w["particles"].copy(order="F")
+        parts[1:3] += eps * d
+        h, pg = _gpu(parts, np.zeros(nx), np.zeros(nx), w["b0"], w["xmax"], nx, (3, 2),
+                     w["common_weight"])
+        return G.strang_splitting_adjoint_(h, w["dt"], 40, checkpoint_every=10)
+    r = run(0.0)
+    ad = float(np.sum(r.g_particles[1:3] * d))
+    he = 1e-6
+    fd = (run(he).loss - run(-he).loss) / (2 * he)
+    assert abs(fd - ad) <= 1e-6 * abs(fd), (fd, ad)
