@@ -224,6 +224,80 @@ def sampling_run(G, n, cpu_n):
     return out
 
 
+BYTES_REV = 80            # reverse pass: read x, v1, v2, w and the cotangents, write the cotangents
+BYTES_RECOMPUTE = 56      # forward re-run of a checkpoint interval: read x, v1, v2, w; write x, v1, v2
+
+
+def adjoint_run(G, _lib, n, steps, every, cpu_n, cpu_steps):
+    """strang_splitting! forward, then forward + reverse mode (the gradient
+    of the time-integrated magnetic energy with respect to the initial
+    particles and fields, checkpoints every `every` steps), on the Weibel
+    initial condition whose positions are a smooth function of alpha; wall
+    times of both calls, HIP-event times of the reverse kernels, and the
+    gradients with respect to alpha and beta."""
+    from gempic_amd import workloads
+    L = _lib.load()
+    w = workloads.weibel_adjoint_1d2v(n, seed=4)
+    h = make_splitting(G, w)
+    dt = w["dt"]
+    G.strang_splitting_adjoint_(h, dt, 4, checkpoint_every=2, fetch=False)   # kernel loads
+    h.to_device()
+    _lib.check(L.gempic_synchronize(h._h))
+    t0 = time.perf_counter()
+    _lib.check(L.gempic_strang_splitting(h._h, dt, steps, _lib.STRANG_FUSED))
+    _lib.check(L.gempic_synchronize(h._h))
+    t_fwd = time.perf_counter() - t0
+    h.to_device()
+    _lib.check(L.gempic_profile_enable(h._h, 10))
+    _lib.check(L.gempic_profile_reset(h._h))
+    t0 = time.perf_counter()
+    G.strang_splitting_adjoint_(h, dt, steps, checkpoint_every=every, fetch=False)
+    t_adj = time.perf_counter() - t0
+    kern = {}
+    for kid, name in ((0, "fused_pass"), (9, "deposit_sum_and_solves"), (10, "reverse_pass"),
+                      (11, "recompute_pass"), (12, "reverse_field_part")):
+        ms, cnt = h.profile_read(kid)
+        if cnt:
+            kern[name] = {"avg_us": 1e3 * ms / cnt, "count": cnt}
+    h.profile(False)
+    r = G.get_adjoint(h)
+    h.to_device()                                  # the initial positions again
+    gpois = G.solve_poisson_adjoint(h, r.g_e1)
+    rev_s = kern["reverse_pass"]["avg_us"] * 1e-6 if "reverse_pass" in kern else float("nan")
+    achieved = BYTES_REV * n / rev_s / 1e9
+    out = {"workload": f"1D2V Weibel adjoint, {n} particles, {steps} steps, checkpoints every "
+                       f"{every} steps, loss = dt * sum of 1/2 l2norm_squared(B) per step",
+           "particles": n, "steps": steps, "checkpoint_every": every,
+           "fwd_particle_steps_per_s": n * steps / t_fwd,
+           "fwd_adj_particle_steps_per_s": n * steps / t_adj,
+           "fwd_adj_over_fwd": t_adj / t_fwd,
+           "loss": r.loss,
+           "dloss_dbeta": float(r.g_b @ w["b_pattern"]),
+           "dloss_dalpha": float((r.g_particles[0] + gpois) @ w["dx_dalpha"]),
+           "kernels": kern,
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS, "bytes_per_particle_step": BYTES_REV,
+                        "kernel": "k_adj1d reverse pass"}}
+    if cpu_n:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as orc
+        wc = workloads.weibel_adjoint_1d2v(cpu_n, seed=4)
+        nx = wc["nx"]
+        parts = np.asfortranarray(wc["particles"])
+        e1, e2, b = np.zeros(nx), np.zeros(nx), wc["b0"].copy()
+        osp = orc.Splitting(nx, wc["xmin"], wc["xmax"], 3, 3, 2, parts, e1, e2, b,
+                            common_weight=wc["common_weight"])
+        osp.solve_poisson(e1, np.zeros(nx))
+        t0 = time.perf_counter()
+        osp.strang_adjoint(dt, cpu_steps, dt)
+        cs = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": cpu_n * cpu_steps / cs, "unit": "particle-steps/s (fwd+adj)",
+                               "cores": 1, "kind": "port",
+                               "sample": f"{cpu_n} particles x {cpu_steps} steps ({cs:.2f} s), the "
+                                         "oracle's reverse mode (every sub-step state kept), one thread"}
+    return out
+
+
 def copy_calibration(mib):
     """Run a plain device copy of `mib` MiB five times (reads and writes mib MiB each).
 
@@ -258,6 +332,9 @@ def main():
     ap.add_argument("--no-sampling", action="store_true", help="skip the sample! measurement")
     ap.add_argument("--boris-steps", type=int, default=200,
                     help="steps of the Boris splitting measurement (0 = skip it)")
+    ap.add_argument("--adjoint-steps", type=int, default=1000,
+                    help="steps of the forward + reverse-mode measurements (0 = skip them)")
+    ap.add_argument("--adjoint-every", type=int, default=50, help="checkpoint interval in steps")
     ap.add_argument("--particles-2d", type=int, default=16777216,
                     help="particles of the 2D pieces measurement (0 = skip it)")
     ap.add_argument("--reps-2d", type=int, default=20)
@@ -350,6 +427,15 @@ def main():
         print(f"[bench] Boris splitting, {args.boris_steps} steps", file=sys.stderr, flush=True)
         boris = boris_run(G, _lib, w, args.boris_steps, 10)
 
+    adj = None
+    if args.adjoint_steps:
+        adj = {}
+        for na in (n, 1048576):
+            print(f"[bench] forward and reverse mode, {na} particles x {args.adjoint_steps} steps",
+                  file=sys.stderr, flush=True)
+            adj[str(na)] = adjoint_run(G, _lib, na, args.adjoint_steps, args.adjoint_every,
+                                       0 if (args.no_cpu or na != n) else 32768, 50)
+
     samp = None
     if not args.no_sampling:
         print("[bench] sample! into GPU memory", file=sys.stderr, flush=True)
@@ -386,6 +472,8 @@ def main():
     }
     if boris is not None:
         line["boris_1d2v"] = boris
+    if adj is not None:
+        line["adjoint_1d2v"] = adj
     if samp is not None:
         line["sampling"] = samp
     if p2d is not None:

@@ -144,20 +144,25 @@ struct gempic_splitting {
     // particle cotangents (x, v1, v2) and the adjoint grid slots
     double *d_hist = nullptr, *d_ckpt = nullptr, *d_seg = nullptr, *d_lam = nullptr;
     double *d_agrid = nullptr;
-    size_t hist_cap = 0, ckpt_cap = 0, seg_cap = 0;
+    double *d_wsnap = nullptr;        // weights and particle order (id) of every sort epoch
+    int64_t *d_ids = nullptr;
+    size_t hist_cap = 0, ckpt_cap = 0, seg_cap = 0, ids_cap = 0, wsnap_cap = 0;
     double *cur_hist = nullptr;       // PassArgs::hist of the next launches
+    double *cur_cx = nullptr;         // PassArgs::cx/cv1/cv2 (x, v1, v2 at n-strides)
+    const int64_t *adj_id0 = nullptr; // particle order of the reverse-mode result
     int adj_blocks = 0;
     bool adj_ready = false;
 
     // profiling
     bool profile = false;
     int profile_every = 1;            // sample every n-th launch of each kernel id
-    int64_t prof_seen[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    static constexpr int kProfIds = 13;
+    int64_t prof_seen[kProfIds] = {};
     struct Pend { int id; hipEvent_t a, b; };
     std::vector<Pend> pending;
     std::vector<hipEvent_t> pool;
-    double prof_ms[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    int64_t prof_cnt[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double prof_ms[kProfIds] = {};
+    int64_t prof_cnt[kProfIds] = {};
 };
 
 struct gempic_maxwell1d {
@@ -248,6 +253,11 @@ PassArgs pass_args(gempic_splitting *h)
     a.err = h->d_err;
     a.diag = h->d_diag;
     a.hist = h->cur_hist;
+    if (h->cur_cx) {
+        a.cx = h->cur_cx;
+        a.cv1 = h->cur_cx + h->n;
+        a.cv2 = h->cur_cx + 2 * h->n;
+    }
     return a;
 }
 
@@ -524,7 +534,7 @@ int gempic_splitting_destroy(gempic_splitting *h)
     void *bufs[] = {h->d_x, h->d_v1, h->d_v2, h->d_w, h->d_id, h->d_x2, h->d_v12, h->d_v22,
                     h->d_w2, h->d_id2, h->d_grid, h->d_tabs, h->d_part, h->d_gpart, h->d_cnt,
                     h->d_err, h->d_diag, h->d_sort_counts, h->d_sort_gsum, h->d_stage,
-                    h->d_hist, h->d_ckpt, h->d_seg, h->d_lam, h->d_agrid};
+                    h->d_hist, h->d_ckpt, h->d_seg, h->d_lam, h->d_agrid, h->d_ids, h->d_wsnap};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     for (auto &p : h->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
@@ -833,6 +843,7 @@ AdjArgs adj_args(gempic_splitting *h, double dt, const Launch &L)
 
 int adj_launch(gempic_splitting *h, int rm, const AdjArgs &A, int blocks)
 {
+    ProfScope ps(h, (rm & R_REV) ? 10 : (rm & R_FWD) ? 11 : 7);
     hipError_t e = launch_adj(h->deg0, h->deg1, rm, A, blocks, h->stream);
     if (e != hipSuccess) return fail(GEMPIC_ERR_DEVICE, std::string("reverse pass launch: ") + hipGetErrorString(e));
     return GEMPIC_OK;
@@ -840,6 +851,7 @@ int adj_launch(gempic_splitting *h, int rm, const AdjArgs &A, int blocks)
 
 int adj_epi(gempic_splitting *h, int kind, const AdjArgs &A, int blocks)
 {
+    ProfScope ps(h, 12);
     hipError_t e = launch_adj_epi(kind, A, blocks, h->stream);
     if (e != hipSuccess) return fail(GEMPIC_ERR_DEVICE, std::string("reverse field part launch: ") + hipGetErrorString(e));
     return GEMPIC_OK;
@@ -851,6 +863,14 @@ int adj_workspace(gempic_splitting *h, int64_t T, int64_t K)
     const size_t nck = (size_t)((T + K - 1) / K);
     int rc = grow(&h->d_hist, &h->hist_cap, (size_t)(T + 2) * NHIST * nx);
     if (!rc) rc = grow(&h->d_ckpt, &h->ckpt_cap, nck * 3 * n);
+    if (!rc) rc = grow(&h->d_wsnap, &h->wsnap_cap, nck * n);
+    if (!rc && h->ids_cap < nck * n) {
+        if (h->d_ids) (void)hipFree(h->d_ids);
+        h->d_ids = nullptr;
+        h->ids_cap = 0;
+        HIP_TRY(dalloc(&h->d_ids, nck * n));
+        h->ids_cap = nck * n;
+    }
     if (!rc && K > 1) rc = grow(&h->d_seg, &h->seg_cap, (size_t)(K - 1) * 3 * n);
     if (!rc && !h->d_lam) HIP_TRY(dalloc(&h->d_lam, 3 * n));
     if (!rc && !h->d_agrid) HIP_TRY(dalloc(&h->d_agrid, (size_t)NASLOT * nx));
@@ -877,7 +897,12 @@ int gempic_strang_splitting_adjoint(gempic_splitting *h, double dt, int64_t n_st
     if (ct_particles && ld != h->ld)
         return fail(GEMPIC_ERR_ASSERTION, "AssertionError: leading dimension must be D+V+n_weights");
     const int64_t T = n_steps, K = std::min<int64_t>(checkpoint_every, n_steps);
+    struct RecordOff {      // no record pointers survive the call, on any path
+        gempic_splitting *h;
+        ~RecordOff() { h->cur_hist = nullptr; h->cur_cx = nullptr; }
+    } record_off{h};
     int rc = adj_workspace(h, T, K);
+    if (!rc) rc = ensure_stage(h);
     if (rc) return rc;
     h->adj_ready = false;
     const size_t n = (size_t)h->n, nx = (size_t)h->nx;
@@ -892,21 +917,51 @@ int gempic_strang_splitting_adjoint(gempic_splitting *h, double dt, int64_t n_st
     rc = run_grid(h, g);
     if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(rec(1), h->d_grid, sizeof(double) * 4 * nx, hipMemcpyDeviceToDevice, h->stream));
+    // Particles are re-sorted only where a checkpoint interval starts (when
+    // the sort interval has passed), so each interval runs in one particle
+    // order. Each sort starts an epoch whose weights and order (id) are
+    // kept, to run the interval's reverse passes and to carry the particle
+    // cotangents across the sort on the way back. With one step per
+    // interval the fused pass itself stores the state after it into the next
+    // checkpoint (no copies, and no re-run on the way back).
+    const int64_t nck = (T + K - 1) / K;
+    std::vector<int64_t> epoch_of((size_t)nck, 0);
+    int64_t ne = 0;
+    const bool dual = K == 1;
     for (int64_t s = 1; s <= T && !rc; ++s) {
         if ((s - 1) % K == 0) {
-            double *ck = h->d_ckpt + (size_t)((s - 1) / K) * 3 * n;
-            HIP_TRY(hipMemcpyAsync(ck, h->d_x, nb, hipMemcpyDeviceToDevice, h->stream));
-            HIP_TRY(hipMemcpyAsync(ck + n, h->d_v1, nb, hipMemcpyDeviceToDevice, h->stream));
-            HIP_TRY(hipMemcpyAsync(ck + 2 * n, h->d_v2, nb, hipMemcpyDeviceToDevice, h->stream));
+            const size_t j = (size_t)((s - 1) / K);
+            bool sorted = false;
+            if (s > 1) {
+                rc = maybe_sort(h);
+                if (rc) break;
+                sorted = h->steps_since_sort == 0;
+            }
+            if (s == 1 || sorted) {
+                HIP_TRY(hipMemcpyAsync(h->d_ids + (size_t)ne * n, h->d_id, sizeof(int64_t) * n,
+                                       hipMemcpyDeviceToDevice, h->stream));
+                HIP_TRY(hipMemcpyAsync(h->d_wsnap + (size_t)ne * n, h->d_w, nb, hipMemcpyDeviceToDevice,
+                                       h->stream));
+                ++ne;
+            }
+            epoch_of[j] = ne - 1;
+            if (!dual || s == 1 || sorted) {
+                double *ck = h->d_ckpt + j * 3 * n;
+                HIP_TRY(hipMemcpyAsync(ck, h->d_x, nb, hipMemcpyDeviceToDevice, h->stream));
+                HIP_TRY(hipMemcpyAsync(ck + n, h->d_v1, nb, hipMemcpyDeviceToDevice, h->stream));
+                HIP_TRY(hipMemcpyAsync(ck + 2 * n, h->d_v2, nb, hipMemcpyDeviceToDevice, h->stream));
+            }
         }
         h->cur_hist = rec(s);
+        h->cur_cx = dual && s < T ? h->d_ckpt + (size_t)s * 3 * n : nullptr;
         const int mode = (s > 1 ? M_HE_PREV : 0) | M_HE | M_HP2A | M_HP1 | M_HP2B;
         rc = run_pass(h, mode, hdt, hdt, 1.0 * dt, s < T, 0);
+        h->steps_since_sort++;
     }
+    h->cur_cx = nullptr;
     h->cur_hist = nullptr;
     if (!rc) rc = run_pass(h, M_HE_PREV, hdt, 0, 0, 0, 1);
     if (rc) return rc;
-    h->steps_since_sort += T;
 
     // ---- reverse ----
     const Launch L = plan(h->n, h->adj_blocks);
@@ -918,8 +973,6 @@ int gempic_strang_splitting_adjoint(gempic_splitting *h, double dt, int64_t n_st
     if (ct_e2) HIP_TRY(hipMemcpyAsync(h->d_agrid + A_E2 * nx, ct_e2, fb, hipMemcpyHostToDevice, h->stream));
     if (ct_b) HIP_TRY(hipMemcpyAsync(h->d_agrid + A_B * nx, ct_b, fb, hipMemcpyHostToDevice, h->stream));
     if (ct_particles) {
-        rc = ensure_stage(h);
-        if (rc) return rc;
         HIP_TRY(hipMemcpyAsync(h->d_stage, ct_particles, sizeof(double) * n * h->ld,
                                hipMemcpyHostToDevice, h->stream));
         HIP_TRY(launch_adj_rows_in(h->d_stage, h->ld, h->n, h->d_id, A.lx, A.lv1, A.lv2, h->stream));
@@ -932,7 +985,6 @@ int gempic_strang_splitting_adjoint(gempic_splitting *h, double dt, int64_t n_st
     rc = adj_launch(h, R_HEONLY, A, L.blocks);
     A.bend = rec(T) + H_BEND * nx;
     if (!rc) rc = adj_epi(h, AE_TAIL, A, L.blocks);
-    const int64_t nck = (T + K - 1) / K;
     for (int64_t j = nck - 1; j >= 0 && !rc; --j) {
         const int64_t s_lo = j * K + 1, s_hi = std::min((j + 1) * K, T);
         // state before step s: the checkpoint for s = s_lo, else the recompute buffer
@@ -942,6 +994,7 @@ int gempic_strang_splitting_adjoint(gempic_splitting *h, double dt, int64_t n_st
             *v1 = b + n;
             *v2 = b + 2 * n;
         };
+        A.p.w = h->d_wsnap + (size_t)epoch_of[(size_t)j] * n;   // the interval's particle order
         for (int64_t s = s_lo; s < s_hi && !rc; ++s) {
             double *x, *v1, *v2;
             state(s, &x, &v1, &v2);
@@ -965,8 +1018,16 @@ int gempic_strang_splitting_adjoint(gempic_splitting *h, double dt, int64_t n_st
                 rc = adj_epi(h, AE_HEAD, A, L.blocks);
             }
         }
+        // into the particle order of the previous interval when a sort ran between them
+        if (!rc && j > 0 && epoch_of[(size_t)j] != epoch_of[(size_t)j - 1]) {
+            hipError_t e = launch_adj_reorder(h->d_ids + (size_t)epoch_of[(size_t)j] * n,
+                                              h->d_ids + (size_t)epoch_of[(size_t)j - 1] * n, h->n,
+                                              h->d_lam, h->d_stage, h->stream);
+            if (e != hipSuccess) return fail(GEMPIC_ERR_DEVICE, std::string("reorder launch: ") + hipGetErrorString(e));
+        }
     }
     if (rc) return rc;
+    h->adj_id0 = h->d_ids;
     h->adj_ready = true;
     if (flags & GEMPIC_ASYNC) return GEMPIC_OK;
     return finish(h);
@@ -983,8 +1044,9 @@ int gempic_get_adjoint(gempic_splitting *h, double *loss, double *g_particles, i
         if (n != h->n) return fail(GEMPIC_ERR_ASSERTION, "AssertionError: particle count differs from the handle's");
         int rc = ensure_stage(h);
         if (rc) return rc;
-        HIP_TRY(launch_adj_rows_out(h->d_stage, ld, n, h->d_id, h->d_lam, h->d_lam + n, h->d_lam + 2 * n,
-                                    h->stream));
+        // the cotangents are in the particle order the call started from
+        HIP_TRY(launch_adj_rows_out(h->d_stage, ld, n, h->adj_id0, h->d_lam, h->d_lam + n,
+                                    h->d_lam + 2 * n, h->stream));
         HIP_TRY(hipMemcpyAsync(g_particles, h->d_stage, sizeof(double) * (size_t)n * ld,
                                hipMemcpyDeviceToHost, h->stream));
     }
@@ -1225,14 +1287,15 @@ int gempic_profile_enable(gempic_splitting *h, int enable)
     if (!enable) resolve_profile(h);
     h->profile = enable != 0;
     h->profile_every = enable > 1 ? enable : 1;
-    for (int k = 0; k < 10; ++k) h->prof_seen[k] = 0;
+    for (int k = 0; k < gempic_splitting::kProfIds; ++k) h->prof_seen[k] = 0;
     return GEMPIC_OK;
 }
 
 int gempic_profile_read(gempic_splitting *h, int kernel_id, double *total_ms, int64_t *count)
 {
     CHECK_H(h);
-    if (kernel_id < 0 || kernel_id > 9) return fail(GEMPIC_ERR_ARGUMENT, "kernel id in 0..9");
+    if (kernel_id < 0 || kernel_id >= gempic_splitting::kProfIds)
+        return fail(GEMPIC_ERR_ARGUMENT, "kernel id in 0..12");
     resolve_profile(h);
     if (total_ms) *total_ms = h->prof_ms[kernel_id];
     if (count) *count = h->prof_cnt[kernel_id];
@@ -1243,7 +1306,7 @@ int gempic_profile_reset(gempic_splitting *h)
 {
     CHECK_H(h);
     resolve_profile(h);
-    for (int k = 0; k < 10; ++k) { h->prof_ms[k] = 0; h->prof_cnt[k] = 0; }
+    for (int k = 0; k < gempic_splitting::kProfIds; ++k) { h->prof_ms[k] = 0; h->prof_cnt[k] = 0; }
     return GEMPIC_OK;
 }
 

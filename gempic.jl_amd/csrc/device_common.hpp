@@ -68,6 +68,40 @@ __device__ __forceinline__ void bspline(double off, double (&b)[P + 1])
     }
 }
 
+// Both bases of a smoother pair at one offset: b0 of degree P0 and b1 of
+// degree P1. When P1 = P0 - 1 the recurrence for b0 passes through b1 after
+// P1 rounds (the same operations as bspline<P1>), so b1 is copied out of it.
+template <int P0, int P1>
+__device__ __forceinline__ void bspline_pair(double off, double (&b0)[P0 + 1], double (&b1)[P1 + 1])
+{
+    if constexpr (P1 == P0 - 1) {
+        b0[0] = 1.0;
+#pragma unroll
+        for (int j = 1; j <= P0; ++j) {
+            double xx = -off;
+            const double jr = (double)j;
+            const double inv_j = 1.0 / jr;
+            double saved = 0.0;
+#pragma unroll
+            for (int r = 0; r < j; ++r) {
+                xx = xx + 1.0;
+                double temp = b0[r] * inv_j;
+                b0[r] = saved + xx * temp;
+                saved = (jr - xx) * temp;
+            }
+            b0[j] = saved;
+            if (j == P1) {
+#pragma unroll
+                for (int r = 0; r <= P1; ++r) b1[r] = b0[r];
+            }
+        }
+        if constexpr (P1 == 0) b1[0] = 1.0;
+    } else {
+        bspline<P1>(off, b1);
+        bspline<P0>(off, b0);
+    }
+}
+
 // Node index of the first of the P+1 nodes touched from cell `cell`
 // (reference: index - p .. index, each taken mod n).
 __device__ __forceinline__ int first_node(int cell, int P, int nx)

@@ -61,6 +61,7 @@ struct PassArgs {
     double *hist;                 // fused Strang step with a record: k_epi1d writes B at the
                                   // step's end to hist[4][nx] and the next pass's fields
                                   // E1, E2, B, E2P to hist[5..8][nx] (nullptr: no record)
+    double *cx, *cv1, *cv2;       // fused Strang pass: also store x, v1, v2 here (nullptr: no)
 };
 
 // error flag bits set by kernels
@@ -156,6 +157,10 @@ hipError_t launch_adj_rows_in(const double *aos, int64_t ld, int64_t n, const in
                               double *r0, double *r1, double *r2, hipStream_t s);
 hipError_t launch_adj_rows_out(double *aos, int64_t ld, int64_t n, const int64_t *id,
                                const double *r0, const double *r1, const double *r2, hipStream_t s);
+// lam (3 rows of n) from the particle order id_from into the order id_to;
+// tmp holds 3n doubles
+hipError_t launch_adj_reorder(const int64_t *id_from, const int64_t *id_to, int64_t n, double *lam,
+                              double *tmp, hipStream_t s);
 
 // Stable counting sort by (cell, sub-cell bin): bins = nx * sub, tile-major
 // counts. Sorting finer than the cell keeps each wave's particles within a

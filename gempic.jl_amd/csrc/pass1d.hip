@@ -470,8 +470,8 @@ __global__ void __launch_bounds__(kBlock) k_pass1d(PassArgs a)
 
         double b1[W1];
         double b0[W0];
-        if constexpr (HE_PREV || HE || HP2A) bspline<P1>(off, b1);
-        if constexpr ((HE_PREV || HE || HP2A) && !V1) bspline<P0>(off, b0);
+        if constexpr ((HE_PREV || HE || HP2A) && !V1) bspline_pair<P0, P1>(off, b0, b1);
+        else if constexpr (HE_PREV || HE || HP2A) bspline<P1>(off, b1);
 
         if constexpr (V1 && (HE_PREV || HE)) {
             // operatorHB 1d1v (hamiltonian_splitting_1d1v.jl:113-125): v += dt * E1(x)
@@ -554,10 +554,10 @@ __global__ void __launch_bounds__(kBlock) k_pass1d(PassArgs a)
             const double off2 = xi2 - (double)ic2;
             const int key2 = imod(ic2, nx);
             double c0[W0];
-            bspline<P0>(off2, c0);
+            double c1[W1];
+            if constexpr (HP2B) bspline_pair<P0, P1>(off2, c0, c1);
+            else bspline<P0>(off2, c0);
             if constexpr (HP2B) {
-                double c1[W1];
-                bspline<P1>(off2, c1);
                 const double bf = gather_p<P1>(c1, sB, key2);
                 v1 = v1 + a.dtqm_hp2 * v2 * bf;
             }
@@ -573,6 +573,13 @@ __global__ void __launch_bounds__(kBlock) k_pass1d(PassArgs a)
             if constexpr (HE_PREV || HE || HP2A || HP2B) pv1[i] = v1;
             if constexpr (!V1 && (HE_PREV || HE || HP1)) pv2[i] = v2;
         }
+        // a recorded fused step also keeps the state after the pass
+        if constexpr (HE && HP2A && HP1 && HP2B && !V1)
+            if (a.cx && valid) {
+                a.cx[base + i] = x;
+                a.cv1[base + i] = v1;
+                a.cv2[base + i] = v2;
+            }
     }
     if (errs) atomicOr(a.err, errs);
 
@@ -2046,8 +2053,13 @@ __device__ __forceinline__ void dbspline(double off, double (&d)[P + 1])
     }
 }
 
+// waves per SIMD the reverse pass is compiled for (register budget)
+#ifndef GEMPIC_ADJ_WAVES
+#define GEMPIC_ADJ_WAVES 2
+#endif
+
 template <int P0, int P1>
-constexpr int adj_slots() { return 4 * (P1 + 1) + (P0 + 1); }
+constexpr int adj_slots() { return 2 * (P1 + 1) + (P0 + 1); }
 
 size_t adj_shmem_bytes(int nx, int slots)
 {
@@ -2056,7 +2068,7 @@ size_t adj_shmem_bytes(int nx, int slots)
 }
 
 template <int P0, int P1, int RM>
-__global__ void __launch_bounds__(kBlock) k_adj1d(AdjArgs A)
+__global__ void __launch_bounds__(kBlock, GEMPIC_ADJ_WAVES) k_adj1d(AdjArgs A)
 {
     constexpr bool PREV = (RM & R_PREV) != 0;
     constexpr bool FWD = (RM & R_FWD) != 0;
@@ -2099,17 +2111,13 @@ __global__ void __launch_bounds__(kBlock) k_adj1d(AdjArgs A)
     __syncthreads();
     double *myrow = rows + wave * NDEP * nx;
 
-    Cache2<W1> c_e1, c_b0, c_bl, c_bn;
+    Cache2<W1> c_e1, c_b;
     Cache2<W0> c_e2;
     if constexpr (DEP) {
         c_e1.init(slots);
         c_e2.init(slots + W1 * kWave);
     }
-    if constexpr (REV) {
-        c_b0.init(slots + (W1 + W0) * kWave);
-        c_bl.init(slots + (2 * W1 + W0) * kWave);
-        c_bn.init(slots + (3 * W1 + W0) * kWave);
-    }
+    if constexpr (REV) c_b.init(slots + (W1 + W0) * kWave);
 
     const double xmin = a.xmin, dx = a.dx, qm = a.qm;
     const double dtqm = a.dtqm_he, g = a.dtqm_hp2, dt1 = a.dt_hp1;
@@ -2143,8 +2151,7 @@ __global__ void __launch_bounds__(kBlock) k_adj1d(AdjArgs A)
         const double off = xi - (double)ic;
         const int key = imod(ic, nx);
         double b1[W1], b0[W0];
-        bspline<P1>(off, b1);
-        bspline<P0>(off, b0);
+        bspline_pair<P0, P1>(off, b0, b1);
 
         if constexpr (POIS) {
             double db0[W0];
@@ -2194,6 +2201,7 @@ __global__ void __launch_bounds__(kBlock) k_adj1d(AdjArgs A)
         const bool far = !pos_ok(xin) || !pos_ok(xi) || dseg > kMaxCellsPerStep ||
                          dseg < -kMaxCellsPerStep;
         const int nseg = far ? 1 : 1 + (dseg < 0 ? -dseg : dseg);
+        double sv0[W1];          // the first segment's integrals, kept for the reverse
         for (int sgi = 0; sgi < nseg; ++sgi) {
             double lo, up, sg;
             int cell;
@@ -2202,7 +2210,10 @@ __global__ void __launch_bounds__(kBlock) k_adj1d(AdjArgs A)
             segment<P1, NQ>(lo, up, sg, a.qx, a.qw, dx, sv);
             const double *bk = sB + (imod(cell, nx) - P1);
 #pragma unroll
-            for (int m = 0; m < W1; ++m) v2 = v2 - qm * sv[m] * bk[m];
+            for (int m = 0; m < W1; ++m) {
+                v2 = v2 - qm * sv[m] * bk[m];
+                if (sgi == 0) sv0[m] = sv[m];
+            }
         }
         const double x1 = jl_mod(xn, a.Lx);
         const bool same = __double_as_longlong(x1) == __double_as_longlong(xn);
@@ -2211,8 +2222,7 @@ __global__ void __launch_bounds__(kBlock) k_adj1d(AdjArgs A)
         const double off2 = xi2 - (double)ic2;
         const int key2 = imod(ic2, nx);
         double c0[W0], c1[W1];
-        bspline<P0>(off2, c0);
-        bspline<P1>(off2, c1);
+        bspline_pair<P0, P1>(off2, c0, c1);
         const double bB = gather_p<P1>(c1, sB, key2);
         if constexpr (FWD) {
             v1 = v1 + g * v2 * bB;
@@ -2227,28 +2237,43 @@ __global__ void __launch_bounds__(kBlock) k_adj1d(AdjArgs A)
         // ---- reverse ----
         if constexpr (REV) {
             const double v2d = v2;
+            // lambda_B lands in the particle's old cell (first Hp2, the line
+            // integral's cell when it stays there, second Hp2 when the push
+            // kept the cell): those parts are summed per particle first
+            double vkey[W1];
             // second Hp2
-            {
-                double dc0[W0], dc1[W1];
+            double dc0[W0], dc1[W1];
+            if constexpr (P1 == P0 - 1) {
+                dc0[0] = -c1[0];
+#pragma unroll
+                for (int r = 1; r < P0; ++r) dc0[r] = c1[r - 1] - c1[r];
+                dc0[P0] = c1[P1];
+            } else {
                 dbspline<P0>(off2, dc0);
-                dbspline<P1>(off2, dc1);
+            }
+            dbspline<P1>(off2, dc1);
+            const double gl2 = g * v2d * lv1;
+            {
                 const double mu = gather_p<P0>(c0, sM2, key2);
                 const double dmu = gather_p<P0>(dc0, sM2, key2) / dx;
                 const double db = gather_p<P1>(dc1, sB, key2) / dx;
-                const double gl1 = g * v2d * lv1;
-                lx = lx + gl1 * db + q * s0 * v2d * dmu;
+                lx = lx + gl2 * db + q * s0 * v2d * dmu;
                 lv2 = lv2 + g * bB * lv1 + q * s0 * mu;
-                double vb[W1];
-#pragma unroll
-                for (int k = 0; k < W1; ++k) vb[k] = gl1 * c1[k];
-                cache_add<W1>(c_bn, valid, key2, vb, myrow + AR_B * nx, nx);
             }
-            // Hp1
+            const bool new_here = key2 == key;
+#pragma unroll
+            for (int k = 0; k < W1; ++k) vkey[k] = new_here ? gl2 * c1[k] : 0.0;
+            // Hp1: the basis at the pushed position is c1 when the mod kept it
             {
                 double bn[W1];
-                bspline<P1>(rn, bn);
+                if (same) {
+#pragma unroll
+                    for (int k = 0; k < W1; ++k) bn[k] = c1[k];
+                } else {
+                    bspline<P1>(rn, bn);
+                }
                 const int keyn = imod(inw, nx);
-                const double bN = gather_p<P1>(bn, sB, keyn);
+                const double bN = same ? bB : gather_p<P1>(bn, sB, keyn);
                 const double muN = gather_p<P1>(bn, sM1, keyn);
                 const double muO = gather_p<P1>(b1, sM1, key);
                 const double lxn = lx - qm * lv2 * bN + q * s1 * muN;
@@ -2262,17 +2287,34 @@ __global__ void __launch_bounds__(kBlock) k_adj1d(AdjArgs A)
                     int cell;
                     segment_bounds(sgi, ic, off, inw, rn, lo, up, sg, cell);
                     double sv[W1];
-                    segment<P1, NQ>(lo, up, sg, a.qx, a.qw, dx, sv);
+                    if (sgi == 0) {
+#pragma unroll
+                        for (int m = 0; m < W1; ++m) sv[m] = sv0[m];
+                    } else {
+                        segment<P1, NQ>(lo, up, sg, a.qx, a.qw, dx, sv);
+                    }
+                    const int kc = imod(cell, nx);
+                    const bool here = kc == key;
                     double vb[W1];
 #pragma unroll
-                    for (int m = 0; m < W1; ++m) vb[m] = wl * sv[m];
-                    cache_add<W1>(c_bl, act, imod(cell, nx), vb, myrow + AR_B * nx, nx);
+                    for (int m = 0; m < W1; ++m) {
+                        vb[m] = wl * sv[m];
+                        if (act && here) vkey[m] = vkey[m] + vb[m];
+                    }
+                    cache_add<W1>(c_b, act && !here, kc, vb, myrow + AR_B * nx, nx);
                 }
             }
             // first Hp2
             double db1[W1], db0[W0];
             dbspline<P1>(off, db1);
-            dbspline<P0>(off, db0);
+            if constexpr (P1 == P0 - 1) {
+                db0[0] = -b1[0];
+#pragma unroll
+                for (int r = 1; r < P0; ++r) db0[r] = b1[r - 1] - b1[r];
+                db0[P0] = b1[P1];
+            } else {
+                dbspline<P0>(off, db0);
+            }
             {
                 const double mu = gather_p<P0>(b0, sM2, key);
                 const double dmu = gather_p<P0>(db0, sM2, key) / dx;
@@ -2280,10 +2322,15 @@ __global__ void __launch_bounds__(kBlock) k_adj1d(AdjArgs A)
                 const double gl1 = g * v2b * lv1;
                 lx = lx + gl1 * db + q * s0 * v2b * dmu;
                 lv2 = lv2 + g * bA * lv1 + q * s0 * mu;
+#pragma unroll
+                for (int k = 0; k < W1; ++k) vkey[k] = vkey[k] + gl1 * b1[k];
+            }
+            cache_add<W1>(c_b, valid, key, vkey, myrow + AR_B * nx, nx);
+            {
                 double vb[W1];
 #pragma unroll
-                for (int k = 0; k < W1; ++k) vb[k] = gl1 * b1[k];
-                cache_add<W1>(c_b0, valid, key, vb, myrow + AR_B * nx, nx);
+                for (int k = 0; k < W1; ++k) vb[k] = gl2 * c1[k];
+                cache_add<W1>(c_b, valid && !new_here, key2, vb, myrow + AR_B * nx, nx);
             }
             // HE of this step and of the previous one: same x and cotangents
             {
@@ -2313,11 +2360,7 @@ __global__ void __launch_bounds__(kBlock) k_adj1d(AdjArgs A)
     if constexpr (DEP) {
         cache_drain<W1>(c_e1, myrow + AR_E1 * nx, nx);
         cache_drain<W0>(c_e2, myrow + AR_E2 * nx, nx);
-        if constexpr (REV) {
-            cache_drain<W1>(c_b0, myrow + AR_B * nx, nx);
-            cache_drain<W1>(c_bl, myrow + AR_B * nx, nx);
-            cache_drain<W1>(c_bn, myrow + AR_B * nx, nx);
-        }
+        if constexpr (REV) cache_drain<W1>(c_b, myrow + AR_B * nx, nx);
         __syncthreads();
         constexpr unsigned MASK = (1u << AR_E1) | (1u << AR_E2) | (REV ? 1u << AR_B : 0u);
         const int len = NDEP * nx;
@@ -2583,6 +2626,39 @@ __global__ void k_adj_rows_out(double *aos, int64_t ld, int64_t n, const int64_t
         p[2] = r2[i];
         for (int64_t r = 3; r < ld; ++r) p[r] = 0.0;
     }
+}
+
+__global__ void k_adj_scatter(const int64_t *id, int64_t n, const double *lam, double *tmp)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = id[i];
+        if (j < 0 || j >= n) continue;
+        tmp[j] = lam[i];
+        tmp[n + j] = lam[n + i];
+        tmp[2 * n + j] = lam[2 * n + i];
+    }
+}
+
+__global__ void k_adj_gather(const int64_t *id, int64_t n, const double *tmp, double *lam)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = id[i];
+        if (j < 0 || j >= n) continue;
+        lam[i] = tmp[j];
+        lam[n + i] = tmp[n + j];
+        lam[2 * n + i] = tmp[2 * n + j];
+    }
+}
+
+hipError_t launch_adj_reorder(const int64_t *id_from, const int64_t *id_to, int64_t n, double *lam,
+                              double *tmp, hipStream_t s)
+{
+    int blocks = (int)std::min<int64_t>(4096, std::max<int64_t>(1, (n + 255) / 256));
+    hipLaunchKernelGGL(k_adj_scatter, dim3(blocks), dim3(256), 0, s, id_from, n, lam, tmp);
+    hipLaunchKernelGGL(k_adj_gather, dim3(blocks), dim3(256), 0, s, id_to, n, tmp, lam);
+    return hipGetLastError();
 }
 
 hipError_t launch_adj_rows_in(const double *aos, int64_t ld, int64_t n, const int64_t *id,

@@ -161,7 +161,8 @@ int gempic_write_step(gempic_splitting *h, double time, int degree, const double
  * kernel ids: 0 fused pass (the second kernel of a two-kernel step),
  * 1 trailing pass, 2 HE, 3 Hp2, 4 Hp1, 5 HB/grid, 6 sort, 7 other,
  * 8 first kernel of a two-kernel step, 9 the deposit sum and field solves
- * that follow a deposit pass. total_ms / count accumulate since the
+ * that follow a deposit pass, 10 reverse pass, 11 forward re-run of a
+ * checkpoint interval, 12 field part of a reverse step. total_ms / count accumulate since the
  * last reset. enable: 0 off, 1 every launch, n > 1 every n-th launch of each
  * kernel id (the event pair costs a few microseconds of stream time, so
  * sampling keeps it out of a timed run). */

@@ -163,3 +163,47 @@ def test_gpu_adjoint_velocity_gradient_vs_central_differences():
     he = 1e-6
     fd = (run(he).loss - run(-he).loss) / (2 * he)
     assert abs(fd - ad) <= 1e-6 * abs(fd), (fd, ad)
+
+
+@pytest.mark.parametrize("K", [2, 3])
+def test_gpu_adjoint_across_particle_sorts(K):
+    """With a sort interval of one step the particles are re-sorted at every
+    checkpoint after the first; the cotangents follow the particles through
+    each re-sort back to the caller's order."""
+    parts, e1, e2, b, L, nx = _hot(20000, seed=6, vscale=1.5)
+    dt, steps, wl, cw = 0.1, 7, 0.1, 0.01
+    h, pg = _gpu(parts, e1, e2, b, L, nx, (3, 2), cw)
+    h.set_sort_interval(1)
+    r = G.strang_splitting_adjoint_(h, dt, steps, checkpoint_every=K, loss_weight=wl)
+    o_parts = parts.copy(order="F")
+    osp = orc.Splitting(nx, 0.0, L, 3, 3, 2, o_parts, e1.copy(), e2.copy(), b.copy(),
+                        common_weight=cw)
+    J, gp, ge1, ge2, gb = osp.strang_adjoint(dt, steps, wl)
+    np.testing.assert_allclose(pg.array, o_parts, rtol=0, atol=1e-11)
+    assert abs(r.loss - J) <= 1e-12 * abs(J)
+    for row in range(3):
+        _close(r.g_particles[row], gp[row])
+    _close(r.g_e1, ge1)
+    _close(r.g_e2, ge2)
+    _close(r.g_b, gb)
+
+
+def test_gpu_adjoint_one_step_intervals_store_every_state():
+    """checkpoint_every = 1: the fused pass stores each step's state as it
+    goes (no re-run), across sorts (sort interval 3)."""
+    parts, e1, e2, b, L, nx = _hot(20000, seed=8, vscale=1.5)
+    dt, steps, wl, cw = 0.1, 8, 0.1, 0.01
+    h, pg = _gpu(parts, e1, e2, b, L, nx, (3, 2), cw)
+    h.set_sort_interval(3)
+    r = G.strang_splitting_adjoint_(h, dt, steps, checkpoint_every=1, loss_weight=wl)
+    o_parts = parts.copy(order="F")
+    osp = orc.Splitting(nx, 0.0, L, 3, 3, 2, o_parts, e1.copy(), e2.copy(), b.copy(),
+                        common_weight=cw)
+    J, gp, ge1, ge2, gb = osp.strang_adjoint(dt, steps, wl)
+    np.testing.assert_allclose(pg.array, o_parts, rtol=0, atol=1e-11)
+    assert abs(r.loss - J) <= 1e-12 * abs(J)
+    for row in range(3):
+        _close(r.g_particles[row], gp[row])
+    _close(r.g_e1, ge1)
+    _close(r.g_e2, ge2)
+    _close(r.g_b, gb)
