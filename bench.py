@@ -429,12 +429,17 @@ def main():
 
     adj = None
     if args.adjoint_steps:
+        # the benchmark's particle count with every step's state kept (one-step
+        # checkpoint intervals, ~100 GB at 4M particles x 1000 steps) and with
+        # checkpoints every `--adjoint-every` steps; the 1M-particle case with
+        # checkpoints every `--adjoint-every` steps
         adj = {}
-        for na in (n, 1048576):
-            print(f"[bench] forward and reverse mode, {na} particles x {args.adjoint_steps} steps",
-                  file=sys.stderr, flush=True)
-            adj[str(na)] = adjoint_run(G, _lib, na, args.adjoint_steps, args.adjoint_every,
-                                       0 if (args.no_cpu or na != n) else 32768, 50)
+        for na, every in ((n, 1), (n, args.adjoint_every), (1048576, args.adjoint_every)):
+            print(f"[bench] forward and reverse mode, {na} particles x {args.adjoint_steps} steps, "
+                  f"checkpoints every {every}", file=sys.stderr, flush=True)
+            adj[f"{na}_every{every}"] = adjoint_run(
+                G, _lib, na, args.adjoint_steps, every,
+                0 if (args.no_cpu or na != n or every != 1) else 32768, 50)
 
     samp = None
     if not args.no_sampling:

@@ -2126,17 +2126,32 @@ __global__ void __launch_bounds__(kBlock, GEMPIC_ADJ_WAVES) k_adj1d(AdjArgs A)
     const int64_t left = a.n - base;
     const int cnt_here = left < a.chunk ? (int)left : (int)a.chunk;
 
+    // software pipeline: the primal state of iteration it+1 is loaded before
+    // the work of iteration it (the cotangents are loaded at the top of their
+    // own iteration; the forward re-run covers their latency)
+    double nx_x = xmin, nx_v1 = 0.0, nx_v2 = 0.0, nx_w = 0.0;
+    if (a.iters > 0 && tid < cnt_here) {
+        nx_x = A.x[base + tid];
+        nx_w = a.w[base + tid];
+        if constexpr (STEP) {
+            nx_v1 = A.v1[base + tid];
+            nx_v2 = A.v2[base + tid];
+        }
+    }
     for (int it = 0; it < a.iters; ++it) {
         const int i = it * kBlock + tid;
         const bool valid = i < cnt_here;
         const int64_t gi = base + (valid ? i : 0);
-        double x = valid ? A.x[gi] : xmin;
-        double v1 = 0.0, v2 = 0.0, w = 0.0;
-        if (valid) {
-            w = a.w[gi];
-            if constexpr (STEP) {
-                v1 = A.v1[gi];
-                v2 = A.v2[gi];
+        double x = nx_x, v1 = nx_v1, v2 = nx_v2, w = nx_w;
+        {
+            const int in = i + kBlock;
+            if (it + 1 < a.iters && in < cnt_here) {
+                nx_x = A.x[base + in];
+                nx_w = a.w[base + in];
+                if constexpr (STEP) {
+                    nx_v1 = A.v1[base + in];
+                    nx_v2 = A.v2[base + in];
+                }
             }
         }
         double lx = 0.0, lv1 = 0.0, lv2 = 0.0;
