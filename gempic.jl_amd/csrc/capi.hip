@@ -144,9 +144,11 @@ struct gempic_splitting {
     // particle cotangents (x, v1, v2) and the adjoint grid slots
     double *d_hist = nullptr, *d_ckpt = nullptr, *d_seg = nullptr, *d_lam = nullptr;
     double *d_agrid = nullptr;
-    double *d_wsnap = nullptr;        // weights and particle order (id) of every sort epoch
-    int64_t *d_ids = nullptr;
-    size_t hist_cap = 0, ckpt_cap = 0, seg_cap = 0, ids_cap = 0, wsnap_cap = 0;
+    double *d_wsnap = nullptr;        // weights of every sort epoch
+    int *d_perm = nullptr;            // each epoch's sort: perm[new index] = old index
+    int64_t *d_ids = nullptr;         // the particle order (id) the call started from
+    size_t hist_cap = 0, ckpt_cap = 0, seg_cap = 0, ids_cap = 0, wsnap_cap = 0, perm_cap = 0;
+    int *cur_perm = nullptr;          // SortArgs::perm of the next sort
     double *cur_hist = nullptr;       // PassArgs::hist of the next launches
     double *cur_cx = nullptr;         // PassArgs::cx/cv1/cv2 (x, v1, v2 at n-strides)
     const int64_t *adj_id0 = nullptr; // particle order of the reverse-mode result
@@ -360,6 +362,7 @@ int run_sort(gempic_splitting *h)
     s.x2 = h->d_x2; s.v12 = h->d_v12; s.v22 = h->d_v22; s.w2 = h->d_w2; s.id2 = h->d_id2;
     s.counts = h->d_sort_counts;
     s.gsum = h->d_sort_gsum;
+    s.perm = h->cur_perm;
     hipError_t e = launch_sort(s, h->dim_v == 2, h->stream);
     if (e != hipSuccess) return fail(GEMPIC_ERR_DEVICE, std::string("sort launch: ") + hipGetErrorString(e));
     std::swap(h->d_x, h->d_x2);
@@ -534,7 +537,8 @@ int gempic_splitting_destroy(gempic_splitting *h)
     void *bufs[] = {h->d_x, h->d_v1, h->d_v2, h->d_w, h->d_id, h->d_x2, h->d_v12, h->d_v22,
                     h->d_w2, h->d_id2, h->d_grid, h->d_tabs, h->d_part, h->d_gpart, h->d_cnt,
                     h->d_err, h->d_diag, h->d_sort_counts, h->d_sort_gsum, h->d_stage,
-                    h->d_hist, h->d_ckpt, h->d_seg, h->d_lam, h->d_agrid, h->d_ids, h->d_wsnap};
+                    h->d_hist, h->d_ckpt, h->d_seg, h->d_lam, h->d_agrid, h->d_ids, h->d_wsnap,
+                    h->d_perm};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     for (auto &p : h->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
@@ -864,12 +868,16 @@ int adj_workspace(gempic_splitting *h, int64_t T, int64_t K)
     int rc = grow(&h->d_hist, &h->hist_cap, (size_t)(T + 2) * NHIST * nx);
     if (!rc) rc = grow(&h->d_ckpt, &h->ckpt_cap, nck * 3 * n);
     if (!rc) rc = grow(&h->d_wsnap, &h->wsnap_cap, nck * n);
-    if (!rc && h->ids_cap < nck * n) {
-        if (h->d_ids) (void)hipFree(h->d_ids);
-        h->d_ids = nullptr;
-        h->ids_cap = 0;
-        HIP_TRY(dalloc(&h->d_ids, nck * n));
-        h->ids_cap = nck * n;
+    if (!rc && h->ids_cap < n) {
+        HIP_TRY(dalloc(&h->d_ids, n));
+        h->ids_cap = n;
+    }
+    if (!rc && h->perm_cap < nck * n) {
+        if (h->d_perm) (void)hipFree(h->d_perm);
+        h->d_perm = nullptr;
+        h->perm_cap = 0;
+        HIP_TRY(dalloc(&h->d_perm, nck * n));
+        h->perm_cap = nck * n;
     }
     if (!rc && K > 1) rc = grow(&h->d_seg, &h->seg_cap, (size_t)(K - 1) * 3 * n);
     if (!rc && !h->d_lam) HIP_TRY(dalloc(&h->d_lam, 3 * n));
@@ -899,7 +907,7 @@ int gempic_strang_splitting_adjoint(gempic_splitting *h, double dt, int64_t n_st
     const int64_t T = n_steps, K = std::min<int64_t>(checkpoint_every, n_steps);
     struct RecordOff {      // no record pointers survive the call, on any path
         gempic_splitting *h;
-        ~RecordOff() { h->cur_hist = nullptr; h->cur_cx = nullptr; }
+        ~RecordOff() { h->cur_hist = nullptr; h->cur_cx = nullptr; h->cur_perm = nullptr; }
     } record_off{h};
     int rc = adj_workspace(h, T, K);
     if (!rc) rc = ensure_stage(h);
@@ -933,13 +941,16 @@ int gempic_strang_splitting_adjoint(gempic_splitting *h, double dt, int64_t n_st
             const size_t j = (size_t)((s - 1) / K);
             bool sorted = false;
             if (s > 1) {
+                h->cur_perm = h->d_perm + (size_t)ne * n;     // this epoch's, if a sort runs
                 rc = maybe_sort(h);
+                h->cur_perm = nullptr;
                 if (rc) break;
                 sorted = h->steps_since_sort == 0;
+            } else {
+                HIP_TRY(hipMemcpyAsync(h->d_ids, h->d_id, sizeof(int64_t) * n, hipMemcpyDeviceToDevice,
+                                       h->stream));
             }
             if (s == 1 || sorted) {
-                HIP_TRY(hipMemcpyAsync(h->d_ids + (size_t)ne * n, h->d_id, sizeof(int64_t) * n,
-                                       hipMemcpyDeviceToDevice, h->stream));
                 HIP_TRY(hipMemcpyAsync(h->d_wsnap + (size_t)ne * n, h->d_w, nb, hipMemcpyDeviceToDevice,
                                        h->stream));
                 ++ne;
@@ -1020,13 +1031,18 @@ int gempic_strang_splitting_adjoint(gempic_splitting *h, double dt, int64_t n_st
         }
         // into the particle order of the previous interval when a sort ran between them
         if (!rc && j > 0 && epoch_of[(size_t)j] != epoch_of[(size_t)j - 1]) {
-            hipError_t e = launch_adj_reorder(h->d_ids + (size_t)epoch_of[(size_t)j] * n,
-                                              h->d_ids + (size_t)epoch_of[(size_t)j - 1] * n, h->n,
-                                              h->d_lam, h->d_stage, h->stream);
-            if (e != hipSuccess) return fail(GEMPIC_ERR_DEVICE, std::string("reorder launch: ") + hipGetErrorString(e));
+            double *other = A.lx == h->d_lam ? h->d_stage : h->d_lam;
+            hipError_t e = launch_adj_unsort(h->d_perm + (size_t)epoch_of[(size_t)j] * n, h->n, A.lx,
+                                             other, h->stream);
+            if (e != hipSuccess) return fail(GEMPIC_ERR_DEVICE, std::string("unsort launch: ") + hipGetErrorString(e));
+            A.lx = other;
+            A.lv1 = other + n;
+            A.lv2 = other + 2 * n;
         }
     }
     if (rc) return rc;
+    if (A.lx != h->d_lam)
+        HIP_TRY(hipMemcpyAsync(h->d_lam, A.lx, 3 * nb, hipMemcpyDeviceToDevice, h->stream));
     h->adj_id0 = h->d_ids;
     h->adj_ready = true;
     if (flags & GEMPIC_ASYNC) return GEMPIC_OK;

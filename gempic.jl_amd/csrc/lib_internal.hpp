@@ -157,10 +157,10 @@ hipError_t launch_adj_rows_in(const double *aos, int64_t ld, int64_t n, const in
                               double *r0, double *r1, double *r2, hipStream_t s);
 hipError_t launch_adj_rows_out(double *aos, int64_t ld, int64_t n, const int64_t *id,
                                const double *r0, const double *r1, const double *r2, hipStream_t s);
-// lam (3 rows of n) from the particle order id_from into the order id_to;
-// tmp holds 3n doubles
-hipError_t launch_adj_reorder(const int64_t *id_from, const int64_t *id_to, int64_t n, double *lam,
-                              double *tmp, hipStream_t s);
+// dst[perm[i]] = src[i] for the 3 rows of n cotangents: back across a sort
+// whose perm[new index] = old index
+hipError_t launch_adj_unsort(const int *perm, int64_t n, const double *src, double *dst,
+                             hipStream_t s);
 
 // Stable counting sort by (cell, sub-cell bin): bins = nx * sub, tile-major
 // counts. Sorting finer than the cell keeps each wave's particles within a
@@ -177,6 +177,7 @@ struct SortArgs {
     int64_t *id2;
     int *counts;      // [ntiles][nbins]: bin counts, then (in place) scatter offsets
     int *gsum;        // [ngroups][nbins]: group sums, then (in place) group bases
+    int *perm;        // if not null: perm[new index] = old index
 };
 constexpr int kSortTile = 64 * 16;     // particles per wave tile
 constexpr int kSortBinsMax = 1024;

@@ -1670,6 +1670,7 @@ __global__ void __launch_bounds__(kBlock) k_sort_scatter(SortArgs a, int has_v2)
                 if (has_v2) a.v22[pos] = v2s[u];
                 a.w2[pos] = ws[u];
                 a.id2[pos] = ids[u];
+                if (a.perm) a.perm[pos] = (int)i;
             }
         }
     }
@@ -2643,36 +2644,23 @@ __global__ void k_adj_rows_out(double *aos, int64_t ld, int64_t n, const int64_t
     }
 }
 
-__global__ void k_adj_scatter(const int64_t *id, int64_t n, const double *lam, double *tmp)
+__global__ void k_adj_unsort(const int *perm, int64_t n, const double *src, double *dst)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t j = id[i];
+        const int64_t j = perm[i];
         if (j < 0 || j >= n) continue;
-        tmp[j] = lam[i];
-        tmp[n + j] = lam[n + i];
-        tmp[2 * n + j] = lam[2 * n + i];
+        dst[j] = src[i];
+        dst[n + j] = src[n + i];
+        dst[2 * n + j] = src[2 * n + i];
     }
 }
 
-__global__ void k_adj_gather(const int64_t *id, int64_t n, const double *tmp, double *lam)
-{
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t j = id[i];
-        if (j < 0 || j >= n) continue;
-        lam[i] = tmp[j];
-        lam[n + i] = tmp[n + j];
-        lam[2 * n + i] = tmp[2 * n + j];
-    }
-}
-
-hipError_t launch_adj_reorder(const int64_t *id_from, const int64_t *id_to, int64_t n, double *lam,
-                              double *tmp, hipStream_t s)
+hipError_t launch_adj_unsort(const int *perm, int64_t n, const double *src, double *dst,
+                             hipStream_t s)
 {
     int blocks = (int)std::min<int64_t>(4096, std::max<int64_t>(1, (n + 255) / 256));
-    hipLaunchKernelGGL(k_adj_scatter, dim3(blocks), dim3(256), 0, s, id_from, n, lam, tmp);
-    hipLaunchKernelGGL(k_adj_gather, dim3(blocks), dim3(256), 0, s, id_to, n, tmp, lam);
+    hipLaunchKernelGGL(k_adj_unsort, dim3(blocks), dim3(256), 0, s, perm, n, src, dst);
     return hipGetLastError();
 }
 
