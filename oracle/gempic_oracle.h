@@ -27,7 +27,10 @@
  *   - strang_splitting: second-order energy conservation;
  *   - write_step: energy conservation and the diagnostics as rates of change
  *     of the energies and momenta under HE, Hp2 and HB; ErrorPoisson by
- *     Gauss's law.
+ *     Gauss's law;
+ *   - reverse mode (strang_adjoint, solve_poisson_adjoint, bspline_deriv):
+ *     central differences of the forward functions above along random
+ *     directions of every input (tests/test_oracle_adjoint.py).
  * No function here is left unpinned.
  *
  * Layout conventions (same as the reference):
