@@ -263,6 +263,10 @@ def adjoint_run(G, _lib, n, steps, every, cpu_n, cpu_steps):
     r = G.get_adjoint(h)
     h.to_device()                                  # the initial positions again
     gpois = G.solve_poisson_adjoint(h, r.g_e1)
+    # k moves Lx = 2 pi / k: positions x = Lx xi, weights w = Lx, E1 (Poisson) ~ Lx
+    L, k = w["xmax"], w["k"]
+    dj_dl = (r.g_length + r.g_weight_sum + float(r.g_particles[0] @ w["particles"][0]) / L +
+             float(r.g_e1 @ h.e_dofs[0]) / L)
     rev_s = kern["reverse_pass"]["avg_us"] * 1e-6 if "reverse_pass" in kern else float("nan")
     achieved = BYTES_REV * n / rev_s / 1e9
     out = {"workload": f"1D2V Weibel adjoint, {n} particles, {steps} steps, checkpoints every "
@@ -274,6 +278,7 @@ def adjoint_run(G, _lib, n, steps, every, cpu_n, cpu_steps):
            "loss": r.loss,
            "dloss_dbeta": float(r.g_b @ w["b_pattern"]),
            "dloss_dalpha": float((r.g_particles[0] + gpois) @ w["dx_dalpha"]),
+           "dloss_dk": -(L / k) * dj_dl,
            "kernels": kern,
            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": achieved / HBM_PEAK_GBS, "bytes_per_particle_step": BYTES_REV,

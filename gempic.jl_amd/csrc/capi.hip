@@ -901,7 +901,8 @@ int gempic_strang_splitting_adjoint(gempic_splitting *h, double dt, int64_t n_st
         return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: the reverse mode covers the 1d2v Strang splitting");
     if (n_steps < 1) return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: number_steps must be >= 1");
     if (checkpoint_every < 1) return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: checkpoint_every must be >= 1");
-    if (h->nx > kAdjNxMax) return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: the reverse mode needs nx <= 128");
+    if (h->nx > kAdjNxMax || h->nx < 4)
+        return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: the reverse mode needs 4 <= nx <= 128");
     if (ct_particles && ld != h->ld)
         return fail(GEMPIC_ERR_ASSERTION, "AssertionError: leading dimension must be D+V+n_weights");
     const int64_t T = n_steps, K = std::min<int64_t>(checkpoint_every, n_steps);
@@ -920,6 +921,9 @@ int gempic_strang_splitting_adjoint(gempic_splitting *h, double dt, int64_t n_st
     auto rec = [&](int64_t s) { return hist + (size_t)s * NHIST * nx; };
 
     // ---- forward, recorded ----
+    // B before the first step (the prologue's HB reads it)
+    HIP_TRY(hipMemcpyAsync(rec(0) + H_BEND * nx, h->d_grid + S_B * nx, sizeof(double) * nx,
+                           hipMemcpyDeviceToDevice, h->stream));
     GridProg g = prog1(G_HB, hdt, S_B, S_B, S_E2);
     prog_add(g, G_HE_B, hdt, S_E2, S_E2, S_B);
     rc = run_grid(h, g);
@@ -1025,7 +1029,7 @@ int gempic_strang_splitting_adjoint(gempic_splitting *h, double dt, int64_t n_st
                 A.bend = rec(s - 1) + H_BEND * nx;
                 rc = adj_epi(h, AE_STEP, A, L.blocks);
             } else {
-                A.bend = nullptr;
+                A.bend = rec(0) + H_BEND * nx;
                 rc = adj_epi(h, AE_HEAD, A, L.blocks);
             }
         }
@@ -1071,6 +1075,20 @@ int gempic_get_adjoint(gempic_splitting *h, double *loss, double *g_particles, i
     if (g_b) HIP_TRY(hipMemcpyAsync(g_b, h->d_agrid + A_B * nx, fb, hipMemcpyDeviceToHost, h->stream));
     if (loss) HIP_TRY(hipMemcpyAsync(loss, h->d_agrid + A_LOSS * nx, sizeof(double), hipMemcpyDeviceToHost, h->stream));
     return finish(h);
+}
+
+int gempic_get_adjoint_domain(gempic_splitting *h, double *g_length, double *g_weight_sum)
+{
+    CHECK_H(h);
+    if (!h->adj_ready) return fail(GEMPIC_ERR_STATE, "no reverse-mode result: call gempic_strang_splitting_adjoint first");
+    double sc[4] = {0, 0, 0, 0};
+    HIP_TRY(hipMemcpyAsync(sc, h->d_agrid + A_LOSS * h->nx, sizeof(sc), hipMemcpyDeviceToHost, h->stream));
+    int rc = finish(h);
+    if (rc) return rc;
+    // dx = Lx / nx: d/d Lx = (1/nx) d/d dx, plus the mod's periods
+    if (g_length) *g_length = sc[AS_DX] / (double)h->nx + sc[AS_MOD];
+    if (g_weight_sum) *g_weight_sum = sc[AS_W];
+    return GEMPIC_OK;
 }
 
 int gempic_solve_poisson_adjoint(gempic_splitting *h, const double *ct_e, double *g_x)

@@ -131,7 +131,9 @@ int boris_blocks_per_cu(int p0, int p1, size_t shmem);
 enum AdjMode : int { R_PREV = 1, R_FWD = 2, R_REV = 4, R_HEONLY = 8, R_POIS = 16 };
 // adjoint grid slots (nx doubles each) and deposit rows of a reverse pass
 enum AdjSlot { A_E1 = 0, A_E2, A_B, A_MU1, A_MU2, A_NU, A_LOSS, NASLOT };
-enum AdjRow { AR_E1 = 0, AR_E2, AR_B };
+enum AdjRow { AR_E1 = 0, AR_E2, AR_B, AR_SCAL };   // AR_SCAL: [0] d/d dx, [1] d/d Lx (mod), [2] sum d/d w
+// scalars in the A_LOSS slot: the loss, then its explicit derivatives
+enum AdjScalar { AS_LOSS = 0, AS_DX, AS_MOD, AS_W };
 // field part of a reverse step: the reverse of the grid work between the
 // passes (k_epi1d and the prologue), see k_adj_epi
 enum AdjEpi : int { AE_TAIL = 0, AE_STEP, AE_HEAD, AE_POIS };

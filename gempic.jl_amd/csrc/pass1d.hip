@@ -2127,6 +2127,12 @@ __global__ void __launch_bounds__(kBlock, GEMPIC_ADJ_WAVES) k_adj1d(AdjArgs A)
     const int64_t left = a.n - base;
     const int cnt_here = left < a.chunk ? (int)left : (int)a.chunk;
 
+    // the loss's explicit dependence on the cell size dx, on the domain
+    // length through the mod, and on the particle weights (summed): per lane,
+    // then per wave and workgroup in a fixed order
+    double acc_dx = 0.0, acc_mod = 0.0, acc_w = 0.0;
+    const double cwq = a.charge * a.cw;         // d q / d w
+
     // software pipeline: the primal state of iteration it+1 is loaded before
     // the work of iteration it (the cotangents are loaded at the top of their
     // own iteration; the forward re-run covers their latency)
@@ -2182,7 +2188,9 @@ __global__ void __launch_bounds__(kBlock, GEMPIC_ADJ_WAVES) k_adj1d(AdjArgs A)
             dbspline<P0>(off, db0);
             const double de1 = gather_p<P1>(db1, sE1, key) / dx;
             const double de2p = gather_p<P0>(db0, sE2p, key) / dx;
-            lx = lx + dtqm * (lv1 * de1 + lv2 * de2p);
+            const double dhe = dtqm * (lv1 * de1 + lv2 * de2p);
+            lx = lx + dhe;
+            if (valid) acc_dx = acc_dx - xi * dhe;
             double ve[W1], vf[W0];
 #pragma unroll
             for (int k = 0; k < W1; ++k) ve[k] = (dtqm * lv1) * b1[k];
@@ -2269,12 +2277,19 @@ __global__ void __launch_bounds__(kBlock, GEMPIC_ADJ_WAVES) k_adj1d(AdjArgs A)
             }
             dbspline<P1>(off2, dc1);
             const double gl2 = g * v2d * lv1;
+            double lx1;                 // cotangent of the position after the mod
             {
                 const double mu = gather_p<P0>(c0, sM2, key2);
                 const double dmu = gather_p<P0>(dc0, sM2, key2) / dx;
                 const double db = gather_p<P1>(dc1, sB, key2) / dx;
-                lx = lx + gl2 * db + q * s0 * v2d * dmu;
+                const double d2 = gl2 * db + q * s0 * v2d * dmu;
+                lx = lx + d2;
                 lv2 = lv2 + g * bB * lv1 + q * s0 * mu;
+                lx1 = lx;
+                if (valid) {
+                    acc_dx = acc_dx - xi2 * d2 - q * s0 * v2d * mu / dx;
+                    acc_w = acc_w + cwq * s0 * v2d * mu;
+                }
             }
             const bool new_here = key2 == key;
 #pragma unroll
@@ -2296,6 +2311,8 @@ __global__ void __launch_bounds__(kBlock, GEMPIC_ADJ_WAVES) k_adj1d(AdjArgs A)
                 lx = lxn + qm * lv2 * bA - q * s1 * muO;
                 lv1 = lv1 + dt1 * lxn;
                 const double wl = -qm * lv2;
+                const double lv2h = lv2;
+                double ib = 0.0, im1 = 0.0;     // int B and int mu1 along the path
                 for (int sgi = 0;; ++sgi) {
                     const bool act = valid && sgi < nseg;
                     if (!__ballot(act)) break;
@@ -2311,13 +2328,27 @@ __global__ void __launch_bounds__(kBlock, GEMPIC_ADJ_WAVES) k_adj1d(AdjArgs A)
                     }
                     const int kc = imod(cell, nx);
                     const bool here = kc == key;
+                    const double *bk = sB + (kc - P1);
+                    const double *mk = sM1 + (kc - P1);
                     double vb[W1];
 #pragma unroll
                     for (int m = 0; m < W1; ++m) {
                         vb[m] = wl * sv[m];
                         if (act && here) vkey[m] = vkey[m] + vb[m];
+                        if (act) {
+                            ib = ib + sv[m] * bk[m];
+                            im1 = im1 + sv[m] * mk[m];
+                        }
                     }
                     cache_add<W1>(c_b, act && !here, kc, vb, myrow + AR_B * nx, nx);
+                }
+                if (valid) {
+                    // d/d dx of int_x^xn f = (1/dx) int f - xin f(xn) + xi f(x)
+                    acc_dx = acc_dx - qm * lv2h * (ib / dx - xin * bN + xi * bA) +
+                             q * s1 * (xi * muO - xin * muN);
+                    acc_w = acc_w + cwq * s1 * im1;
+                    // x = mod(xn, Lx): d x / d Lx = the number of periods added
+                    if (!same) acc_mod = acc_mod + lx1 * (x1 > xn ? 1.0 : -1.0);
                 }
             }
             // first Hp2
@@ -2336,8 +2367,13 @@ __global__ void __launch_bounds__(kBlock, GEMPIC_ADJ_WAVES) k_adj1d(AdjArgs A)
                 const double dmu = gather_p<P0>(db0, sM2, key) / dx;
                 const double db = gather_p<P1>(db1, sB, key) / dx;
                 const double gl1 = g * v2b * lv1;
-                lx = lx + gl1 * db + q * s0 * v2b * dmu;
+                const double d1 = gl1 * db + q * s0 * v2b * dmu;
+                lx = lx + d1;
                 lv2 = lv2 + g * bA * lv1 + q * s0 * mu;
+                if (valid) {
+                    acc_dx = acc_dx - xi * d1 - q * s0 * v2b * mu / dx;
+                    acc_w = acc_w + cwq * s0 * v2b * mu;
+                }
 #pragma unroll
                 for (int k = 0; k < W1; ++k) vkey[k] = vkey[k] + gl1 * b1[k];
             }
@@ -2352,11 +2388,16 @@ __global__ void __launch_bounds__(kBlock, GEMPIC_ADJ_WAVES) k_adj1d(AdjArgs A)
             {
                 const double de1 = gather_p<P1>(db1, sE1, key) / dx;
                 const double de2 = gather_p<P0>(db0, sE2, key) / dx;
-                lx = lx + dtqm * (lv1 * de1 + lv2 * de2);
+                const double dh = dtqm * (lv1 * de1 + lv2 * de2);
+                lx = lx + dh;
+                double dhs = dh;
                 if constexpr (PREV) {
                     const double de2p = gather_p<P0>(db0, sE2p, key) / dx;
-                    lx = lx + dtqm * (lv1 * de1 + lv2 * de2p);
+                    const double dhp = dtqm * (lv1 * de1 + lv2 * de2p);
+                    lx = lx + dhp;
+                    dhs = dhs + dhp;
                 }
+                if (valid) acc_dx = acc_dx - xi * dhs;
                 double ve[W1], vf[W0];
 #pragma unroll
                 for (int k = 0; k < W1; ++k) ve[k] = (dtqm * lv1) * b1[k];
@@ -2374,11 +2415,22 @@ __global__ void __launch_bounds__(kBlock, GEMPIC_ADJ_WAVES) k_adj1d(AdjArgs A)
     }
 
     if constexpr (DEP) {
+        {
+            // wave sums (xor butterfly: identical in every lane) into the
+            // wave's row AR_SCAL; the loop below adds the waves in order
+            const double sdx = wave_sum(acc_dx), smod = wave_sum(acc_mod), sw = wave_sum(acc_w);
+            if ((tid & (kWave - 1)) == 0) {
+                myrow[AR_SCAL * nx + 0] = sdx;
+                myrow[AR_SCAL * nx + 1] = smod;
+                myrow[AR_SCAL * nx + 2] = sw;
+            }
+        }
         cache_drain<W1>(c_e1, myrow + AR_E1 * nx, nx);
         cache_drain<W0>(c_e2, myrow + AR_E2 * nx, nx);
         if constexpr (REV) cache_drain<W1>(c_b, myrow + AR_B * nx, nx);
         __syncthreads();
-        constexpr unsigned MASK = (1u << AR_E1) | (1u << AR_E2) | (REV ? 1u << AR_B : 0u);
+        constexpr unsigned MASK = (1u << AR_E1) | (1u << AR_E2) | (REV ? 1u << AR_B : 0u) |
+                                  (1u << AR_SCAL);
         const int len = NDEP * nx;
         for (int idx = tid; idx < len; idx += kBlock) {
             if (!((MASK >> (idx / nx)) & 1u)) continue;
@@ -2441,9 +2493,10 @@ __global__ void __launch_bounds__(kBlock) k_adj_epi(AdjArgs A, int nblocks)
     __syncthreads();
     for (int k = tid; k < nx; k += nt) ampt[k] = k > nx / 2 ? -T[T_AMP * nx + k] : T[T_AMP * nx + k];
 
-    constexpr unsigned MASK = KIND == AE_TAIL ? (1u << AR_E1) | (1u << AR_E2)
+    constexpr unsigned MASK = KIND == AE_TAIL ? (1u << AR_E1) | (1u << AR_E2) | (1u << AR_SCAL)
                               : KIND == AE_POIS ? 0u
-                                                : (1u << AR_E1) | (1u << AR_E2) | (1u << AR_B);
+                                                : (1u << AR_E1) | (1u << AR_E2) | (1u << AR_B) |
+                                                      (1u << AR_SCAL);
     if constexpr (MASK != 0u) {
         const int ng = gridDim.x, gb = blockIdx.x;
         const int b0 = (int)(((long)nblocks * gb) / ng), b1 = (int)(((long)nblocks * (gb + 1)) / ng);
@@ -2461,6 +2514,24 @@ __global__ void __launch_bounds__(kBlock) k_adj_epi(AdjArgs A, int nblocks)
     const double *dE1 = red + AR_E1 * nx, *dE2 = red + AR_E2 * nx, *dB = red + AR_B * nx;
     const double c = a.dt_he;
     const double coef = c / dx;
+    // explicit derivatives of the loss in dx: the particle sums of the pass,
+    // and (thread 0, below) the grid terms
+    //   U = compute_b_from_e!:  -(1/dx) lambda_B . (c/dx) D E2
+    //   H = compute_e_from_b!:  -(1/dx) lambda_E2 . (c/dx) W B = -(1/dx) (c/dx) (W^T lambda_E2) . B
+    //   L = w/2 dx B.M1 B:      w/2 B.M1 B
+    double gdx = 0.0;
+    const double *E2f = A.fields ? A.fields + H_E2 * nx : nullptr;
+    const double *E2c = A.fields ? A.fields + H_E2P * nx : nullptr;
+    auto dot_d = [&](const double *lb, const double *e) {      // lb . D e, D e[j] = e[j-1] - e[j]
+        double r = 0.0;
+        for (int j = 0; j < nx; ++j) r += lb[j] * (e[j == 0 ? nx - 1 : j - 1] - e[j]);
+        return r;
+    };
+    if (KIND != AE_POIS && tid == 0) {
+        G[A_LOSS * nx + AS_DX] = G[A_LOSS * nx + AS_DX] + red[AR_SCAL * nx + 0];
+        G[A_LOSS * nx + AS_MOD] = G[A_LOSS * nx + AS_MOD] + red[AR_SCAL * nx + 1];
+        G[A_LOSS * nx + AS_W] = G[A_LOSS * nx + AS_W] + red[AR_SCAL * nx + 2];
+    }
     const double *cs = T + T_COS * nx, *sn = T + T_SIN * nx;
     double *wA = w3, *wM = w3 + nx, *w1 = w3 + 2 * nx;
 
@@ -2485,6 +2556,7 @@ __global__ void __launch_bounds__(kBlock) k_adj_epi(AdjArgs A, int nblocks)
             lE2[i] = lE2[i] + dE2[i];
         }
         __syncthreads();
+        if (tid == 0) gdx = gdx - (coef / dx) * dot_d(lB, E2f);
         adj_b_from_e(lE2, coef, lB, nx);                       // U of the next step's HE
     }
     if constexpr (KIND == AE_HEAD) {
@@ -2494,6 +2566,12 @@ __global__ void __launch_bounds__(kBlock) k_adj_epi(AdjArgs A, int nblocks)
             G[A_E1 * nx + i] = lE1[i];
             G[A_E2 * nx + i] = lE2[i];
             G[A_B * nx + i] = lB[i];
+        }
+        if (tid == 0) {
+            double r = 0.0;                                    // (W^T lambda_E2) . B_0
+            for (int i = 0; i < nx; ++i) r += wA[i] * bend[i];
+            gdx = gdx - (coef / dx) * r;
+            G[A_LOSS * nx + AS_DX] = G[A_LOSS * nx + AS_DX] + gdx;
         }
         return;
     }
@@ -2515,11 +2593,19 @@ __global__ void __launch_bounds__(kBlock) k_adj_epi(AdjArgs A, int nblocks)
             G[A_MU1 * nx + i] = -w1[i] / dx;
         }
         if (tid == 0) {
-            double r = 0.0;
-            for (int i = 0; i < nx; ++i) r += bend[i] * wM[i];
+            double r = 0.0, rh = 0.0;
+            for (int i = 0; i < nx; ++i) {
+                r += bend[i] * wM[i];
+                rh += wA[i] * bend[i];
+            }
             G[A_LOSS * nx] = G[A_LOSS * nx] + wl * 0.5 * (r * dx);
+            gdx = gdx + wl * 0.5 * r - (KIND == AE_STEP ? 2.0 : 1.0) * (coef / dx) * rh;
         }
         __syncthreads();
+        if (tid == 0) {
+            gdx = gdx - (coef / dx) * dot_d(lB, E2c);
+            G[A_LOSS * nx + AS_DX] = G[A_LOSS * nx + AS_DX] + gdx;
+        }
         adj_b_from_e(lE2, coef, lB, nx);                       // U of this step's HE
         for (int i = tid; i < nx; i += nt) lE2[i] = lE2[i] + dE2[i];
         __syncthreads();

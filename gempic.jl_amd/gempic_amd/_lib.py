@@ -55,6 +55,7 @@ SIGNATURES = {
     "gempic_strang_splitting_adjoint": (_I, [_VP, _D, _I64, _I64, _D, _P, _I64, _P, _P, _P, _I]),
     "gempic_get_adjoint": (_I, [_VP, _P, _P, _I64, _I64, _P, _P, _P]),
     "gempic_solve_poisson_adjoint": (_I, [_VP, _P, _P]),
+    "gempic_get_adjoint_domain": (_I, [_VP, _P, _P]),
     "gempic_set_sort_interval": (_I, [_VP, _I64]),
     "gempic_sort_particles": (_I, [_VP]),
     "gempic_solve_poisson": (_I, [_VP, _P, _P]),

@@ -466,9 +466,11 @@ class AdjointResult:
     g_particles has the particle array's layout (rows x, v1, v2; the weight
     row is zero), g_e1, g_e2, g_b the fields'."""
 
-    def __init__(self, loss, g_particles, g_e1, g_e2, g_b):
+    def __init__(self, loss, g_particles, g_e1, g_e2, g_b, g_length=0.0, g_weight_sum=0.0):
         self.loss, self.g_particles = loss, g_particles
         self.g_e1, self.g_e2, self.g_b = g_e1, g_e2, g_b
+        # explicit d loss / d Lx (state held fixed) and sum_i d loss / d w_i
+        self.g_length, self.g_weight_sum = g_length, g_weight_sum
 
 
 def strang_splitting_adjoint_(h: HamiltonianSplitting, dt: float, number_steps: int,
@@ -515,7 +517,9 @@ def get_adjoint(h: HamiltonianSplitting) -> AdjointResult:
     ge1, ge2, gb, loss = np.zeros(nx), np.zeros(nx), np.zeros(nx), np.zeros(1)
     check(_lib.load().gempic_get_adjoint(h._h, ptr(loss), ptr(g), a.shape[0], a.shape[1],
                                          ptr(ge1), ptr(ge2), ptr(gb)))
-    return AdjointResult(float(loss[0]), g, ge1, ge2, gb)
+    gl, gw = np.zeros(1), np.zeros(1)
+    check(_lib.load().gempic_get_adjoint_domain(h._h, ptr(gl), ptr(gw)))
+    return AdjointResult(float(loss[0]), g, ge1, ge2, gb, float(gl[0]), float(gw[0]))
 
 
 def solve_poisson_adjoint(h: HamiltonianSplitting, ct_e) -> np.ndarray:

@@ -121,7 +121,7 @@ int gempic_synchronize(gempic_splitting *h);
  * (l2norm_squared: maxwell_1d_fem.jl:299-313; with loss_weight = dt the sum
  * is the time-integrated magnetic energy of write_step!, diagnostics.jl:236).
  * ct_* may be NULL (zero); ct_particles is the caller's AoS layout (rows x,
- * v1, v2 read, ld = D+V+n_weights). nx <= 128. Results: gempic_get_adjoint. */
+ * v1, v2 read, ld = D+V+n_weights). 4 <= nx <= 128. Results: gempic_get_adjoint. */
 int gempic_strang_splitting_adjoint(gempic_splitting *h, double dt, int64_t n_steps,
                                     int64_t checkpoint_every, double loss_weight,
                                     const double *ct_particles, int64_t ld, const double *ct_e1,
@@ -131,6 +131,15 @@ int gempic_strang_splitting_adjoint(gempic_splitting *h, double dt, int64_t n_st
  * g_e1, g_e2, g_b (nx each). Any output may be NULL. */
 int gempic_get_adjoint(gempic_splitting *h, double *loss, double *g_particles, int64_t ld,
                        int64_t n, double *g_e1, double *g_e2, double *g_b);
+/* The same call's explicit derivatives of the loss in the domain: g_length =
+ * d loss / d Lx with the particles' positions, velocities and weights and the
+ * fields held fixed (through dx = Lx/nx in every operator, and the periods the
+ * mod adds in Hp1, hamiltonian_splitting_1d2v.jl:79); g_weight_sum = the sum
+ * over particles of d loss / d w_i. With the cotangents of gempic_get_adjoint
+ * they give the derivative in a parameter that moves Lx (a wavenumber k with
+ * Lx = 2 pi / k) by the chain rule through the initial state. Needs a
+ * smoother pair with p - 1 >= 1 (continuous splines along Hp1's path). */
+int gempic_get_adjoint_domain(gempic_splitting *h, double *g_length, double *g_weight_sum);
 /* Reverse mode of solve_poisson! (diagnostics.jl:15-31) in the positions:
  * g_x[i] = d <ct_e, E> / d x_i for E = compute_e_from_rho!(add_charge!(x))
  * of the handle's particles, in the caller's order (n doubles). */
