@@ -1250,6 +1250,194 @@ int gempic_boris_get_mid(gempic_splitting *h, double *e1_mid, double *e2_mid, do
     return finish(h);
 }
 
+// ---------------------------------------------------------------------------
+// Ensembles: independent splittings (each with its own particles, grid,
+// domain and tables) stepped together, one particle-pass launch and one
+// field-part launch per Strang step for all of them. Each instance keeps the
+// launch geometry of its own strang_splitting!, so its results are bit for
+// bit those of a stand-alone run.
+// ---------------------------------------------------------------------------
+struct gempic_ensemble {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<gempic_splitting *> hs;
+    int p0 = 3, p1 = 2, max_nx = 0;
+    PassArgs *d_args = nullptr;        // [2][ninst]: another step follows / the last step
+    int *d_bstart = nullptr, *d_estart = nullptr;
+    std::vector<int> bstart, estart;
+};
+
+namespace {
+
+// per-instance arguments of the fused pass (more = 1 and more = 0) and the
+// workgroup tables; uploaded again after a re-sort moves the particle arrays
+int ens_upload(gempic_ensemble *e, const double *dts)
+{
+    const size_t ni = e->hs.size();
+    std::vector<PassArgs> args(2 * ni);
+    e->bstart.assign(ni + 1, 0);
+    e->estart.assign(ni + 1, 0);
+    for (size_t i = 0; i < ni; ++i) {
+        gempic_splitting *h = e->hs[i];
+        const double dt = dts[i], hdt = 0.5 * dt;
+        const int target = h->override_blocks ? h->override_blocks : h->dep_blocks;
+        const Launch L = plan(h->n, target);
+        PassArgs a = pass_args(h);
+        a.chunk = L.chunk;
+        a.iters = L.iters;
+        a.ngroups = L.ngroups;
+        a.dt_he = hdt;
+        a.dt_hp2 = hdt;
+        a.dt_hp1 = 1.0 * dt;
+        a.dtqm_he = hdt * h->qm;
+        a.dtqm_hp2 = hdt * h->qm;
+        a.more = 1;
+        args[i] = a;
+        a.more = 0;
+        args[ni + i] = a;
+        e->bstart[i + 1] = e->bstart[i] + L.blocks;
+        e->estart[i + 1] = e->estart[i] + epi_groups(L.blocks);
+    }
+    HIP_TRY(hipMemcpy(e->d_args, args.data(), sizeof(PassArgs) * args.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(e->d_bstart, e->bstart.data(), sizeof(int) * (ni + 1), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(e->d_estart, e->estart.data(), sizeof(int) * (ni + 1), hipMemcpyHostToDevice));
+    return GEMPIC_OK;
+}
+
+// the instances' own work (sorts, field programs) goes to the ensemble's
+// stream for the duration of an ensemble call
+struct EnsStreams {
+    gempic_ensemble *e;
+    std::vector<hipStream_t> saved;
+    explicit EnsStreams(gempic_ensemble *ee) : e(ee)
+    {
+        for (auto *h : e->hs) {
+            (void)hipStreamSynchronize(h->stream);
+            saved.push_back(h->stream);
+            h->stream = e->stream;
+        }
+    }
+    ~EnsStreams()
+    {
+        (void)hipStreamSynchronize(e->stream);
+        for (size_t i = 0; i < e->hs.size(); ++i) e->hs[i]->stream = saved[i];
+    }
+};
+
+}  // namespace
+
+int gempic_ensemble_create(int n_instances, gempic_splitting *const *handles, gempic_ensemble **out)
+{
+    if (!out) return fail(GEMPIC_ERR_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    if (n_instances < 1 || !handles) return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: need at least one splitting");
+    gempic_splitting *h0 = handles[0];
+    if (!h0) return fail(GEMPIC_ERR_ARGUMENT, "null handle");
+    for (int i = 0; i < n_instances; ++i) {
+        gempic_splitting *h = handles[i];
+        if (!h) return fail(GEMPIC_ERR_ARGUMENT, "null handle");
+        if (h->dim_v != 2) return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: ensembles step 1d2v splittings");
+        if (h->device != h0->device) return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: every splitting must live on one GPU");
+        if (h->deg0 != h0->deg0 || h->deg1 != h0->deg1)
+            return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: every splitting must use the same smoother degrees");
+        for (int j = 0; j < i; ++j)
+            if (handles[j] == h) return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: a splitting appears twice");
+    }
+    if (hipSetDevice(h0->device) != hipSuccess) return fail(GEMPIC_ERR_DEVICE, "hipSetDevice failed");
+    gempic_ensemble *e = new gempic_ensemble();
+    e->device = h0->device;
+    e->hs.assign(handles, handles + n_instances);
+    e->p0 = h0->deg0;
+    e->p1 = h0->deg1;
+    for (auto *h : e->hs) e->max_nx = std::max(e->max_nx, (int)h->nx);
+    hipError_t r = hipStreamCreate(&e->stream);
+    if (r == hipSuccess) r = dalloc(&e->d_args, 2 * (size_t)n_instances);
+    if (r == hipSuccess) r = dalloc(&e->d_bstart, (size_t)n_instances + 1);
+    if (r == hipSuccess) r = dalloc(&e->d_estart, (size_t)n_instances + 1);
+    if (r != hipSuccess) {
+        gempic_ensemble_destroy(e);
+        return fail(GEMPIC_ERR_DEVICE, std::string("ensemble allocation: ") + hipGetErrorString(r));
+    }
+    *out = e;
+    return GEMPIC_OK;
+}
+
+int gempic_ensemble_destroy(gempic_ensemble *e)
+{
+    if (!e) return GEMPIC_OK;
+    (void)hipSetDevice(e->device);
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    void *bufs[] = {e->d_args, e->d_bstart, e->d_estart};
+    for (void *p : bufs)
+        if (p) (void)hipFree(p);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+    return GEMPIC_OK;
+}
+
+int gempic_ensemble_strang_splitting(gempic_ensemble *e, const double *dt, int64_t n_steps, int flags)
+{
+    if (!e) return fail(GEMPIC_ERR_ARGUMENT, "null ensemble");
+    if (!dt) return fail(GEMPIC_ERR_ARGUMENT, "dt is NULL");
+    if (hipSetDevice(e->device) != hipSuccess) return fail(GEMPIC_ERR_DEVICE, "hipSetDevice failed");
+    if (n_steps < 0) return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: number_steps must be >= 0");
+    for (auto *h : e->hs)
+        if (!h->particles_set) return fail(GEMPIC_ERR_STATE, "particles not set");
+    if (n_steps == 0) return GEMPIC_OK;
+    EnsStreams streams(e);
+    const int ni = (int)e->hs.size();
+    // HB(dt/2) and HE's B update of the first step, per instance
+    for (int i = 0; i < ni; ++i) {
+        const double hdt = 0.5 * dt[i];
+        GridProg g = prog1(G_HB, hdt, S_B, S_B, S_E2);
+        prog_add(g, G_HE_B, hdt, S_E2, S_E2, S_B);
+        int rc = run_grid(e->hs[i], g);
+        if (rc) return rc;
+    }
+    int rc = ens_upload(e, dt);
+    if (rc) return rc;
+    const size_t sh_pass = pass_shmem_bytes(e->max_nx, M_HE_PREV | M_HE | M_HP2A | M_HP1 | M_HP2B);
+    const size_t sh_epi = epi_shmem_bytes(e->max_nx);
+    for (int64_t s = 0; s < n_steps; ++s) {
+        bool moved = false;
+        for (auto *h : e->hs) {
+            if (h->sort_interval > 0 && h->steps_since_sort >= h->sort_interval) {
+                rc = run_sort(h);
+                if (rc) return rc;
+                moved = true;
+            }
+        }
+        if (moved) {
+            rc = ens_upload(e, dt);
+            if (rc) return rc;
+        }
+        const int mode = (s == 0 ? 0 : M_HE_PREV) | M_HE | M_HP2A | M_HP1 | M_HP2B;
+        BatchArgs b;
+        b.args = e->d_args + (s + 1 < n_steps ? 0 : ni);
+        b.block_start = e->d_bstart;
+        b.epi_start = e->d_estart;
+        b.ninst = ni;
+        HIP_TRY(launch_pass_batch(e->p0, e->p1, mode, b, e->bstart[ni], sh_pass, e->stream));
+        HIP_TRY(launch_epi_batch(e->p0, e->p1, mode, b, e->estart[ni], sh_epi, e->stream));
+        for (auto *h : e->hs) h->steps_since_sort++;
+    }
+    // the last step's trailing HE
+    BatchArgs b;
+    b.args = e->d_args + ni;
+    b.block_start = e->d_bstart;
+    b.epi_start = e->d_estart;
+    b.ninst = ni;
+    HIP_TRY(launch_pass_batch(e->p0, e->p1, M_HE_PREV, b, e->bstart[ni],
+                              pass_shmem_bytes(e->max_nx, M_HE_PREV), e->stream));
+    (void)flags;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    for (auto *h : e->hs) {
+        rc = finish(h);
+        if (rc) return rc;
+    }
+    return GEMPIC_OK;
+}
+
 int gempic_synchronize(gempic_splitting *h)
 {
     CHECK_H(h);

@@ -72,6 +72,22 @@ constexpr int kMaxCellsPerStep = 1 << 16;
 
 hipError_t launch_pass(int p0, int p1, int mode, const PassArgs &a, int blocks, size_t shmem,
                        hipStream_t s);
+
+// An ensemble's instances in one launch (device arrays): PassArgs per
+// instance, the first particle-pass workgroup of each instance (ninst + 1
+// entries) and the first field-part workgroup of each (ninst + 1 entries).
+struct BatchArgs {
+    const PassArgs *args;
+    const int *block_start;
+    const int *epi_start;
+    int ninst;
+};
+hipError_t launch_pass_batch(int p0, int p1, int mode, const BatchArgs &b, int total_blocks,
+                             size_t shmem, hipStream_t s);
+hipError_t launch_epi_batch(int p0, int p1, int mode, const BatchArgs &b, int total_groups,
+                            size_t shmem, hipStream_t s);
+size_t epi_shmem_bytes(int nx);
+int epi_groups(int nblocks);      // workgroups of the field part after a pass of nblocks
 // The deposit sum and field solves that follow a deposit pass (k_epi1d);
 // pass_blocks = workgroups of that pass. Modes with M_NOEPI have none.
 bool mode_has_epi(int mode);

@@ -372,7 +372,7 @@ __device__ __forceinline__ bool pos_ok(double xi) { return fabs(xi) < 1.0e9; }
 // The particle pass
 // ---------------------------------------------------------------------------
 template <int P0, int P1, int MODE>
-__global__ void __launch_bounds__(kBlock) k_pass1d(PassArgs a)
+__device__ __forceinline__ void pass1d_body(const PassArgs &a, int bid, double *smem)
 {
     constexpr bool HE_PREV = (MODE & M_HE_PREV) != 0;
     constexpr bool HE = (MODE & M_HE) != 0;
@@ -394,7 +394,6 @@ __global__ void __launch_bounds__(kBlock) k_pass1d(PassArgs a)
     constexpr int W0 = P0 + 1, W1 = P1 + 1;
     static_assert(P0 <= kPad && P1 <= kPad, "LDS field padding must cover the spline support");
 
-    extern __shared__ double smem[];
     const int nx = a.nx;
     const int tid = threadIdx.x;
     const int wave = tid / kWave;
@@ -428,7 +427,7 @@ __global__ void __launch_bounds__(kBlock) k_pass1d(PassArgs a)
     const double xmin = a.xmin, dx = a.dx, qm = a.qm;
     // this workgroup's chunk: particles base .. base + cnt_here - 1, addressed
     // through per-chunk base pointers and a 32-bit index
-    const int64_t base = (int64_t)blockIdx.x * a.chunk;
+    const int64_t base = (int64_t)bid * a.chunk;
     const int64_t left = a.n - base;
     const int cnt_here = left < a.chunk ? (int)left : (int)a.chunk;
     double *__restrict__ px = a.x + base;
@@ -593,17 +592,47 @@ __global__ void __launch_bounds__(kBlock) k_pass1d(PassArgs a)
             if (!((DEPMASK >> (idx / nx)) & 1u)) continue;
             double s = 0.0;
             for (int wv = 0; wv < kWaves; ++wv) s += rows[wv * len + idx];
-            a.part[(size_t)blockIdx.x * len + idx] = s;
+            a.part[(size_t)bid * len + idx] = s;
         }
         // the cross-workgroup sum and the field solves run in k_epi1d
     } else if constexpr (HE && !V1 && !HE_PREV) {
         // operatorHE alone: compute_b_from_e! after the push (:217-218). B is
         // not read by this pass, so workgroup 0 can update it at any time.
-        if (blockIdx.x == 0) {
+        if (bid == 0) {
             ep_b_from_e(sB, sE2, a.dt_he, dx, nx);
             for (int i = tid; i < nx; i += kBlock) a.grid[S_B * nx + i] = sB[i];
         }
     }
+}
+
+// The batch of an ensemble: instance i's workgroups are [block_start[i], block_start[i+1])
+__device__ __forceinline__ int batch_instance(const int *start, int ninst, int block)
+{
+    int lo = 0, hi = ninst;              // start[lo] <= block < start[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (start[mid] <= block) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <int P0, int P1, int MODE>
+__global__ void __launch_bounds__(kBlock) k_pass1d(PassArgs a)
+{
+    extern __shared__ double smem[];
+    pass1d_body<P0, P1, MODE>(a, blockIdx.x, smem);
+}
+
+// One launch runs the same pass for every instance of an ensemble; each
+// workgroup works for one instance with that instance's own launch geometry,
+// so every instance's result is the one its own launch would give.
+template <int P0, int P1, int MODE>
+__global__ void __launch_bounds__(kBlock) k_pass1d_batch(BatchArgs b)
+{
+    extern __shared__ double smem[];
+    const int inst = batch_instance(b.block_start, b.ninst, blockIdx.x);
+    pass1d_body<P0, P1, MODE>(b.args[inst], blockIdx.x - b.block_start[inst], smem);
 }
 
 // ---------------------------------------------------------------------------
@@ -687,7 +716,8 @@ int epi_groups(int nblocks)
 }
 
 template <int P0, int P1, int MODE>
-__global__ void __launch_bounds__(kBlock) k_epi1d(PassArgs a, int nblocks)
+__device__ __forceinline__ void epi1d_body(const PassArgs &a, int nblocks, int g, int ng,
+                                           double *smem)
 {
     constexpr bool HE_PREV = (MODE & M_HE_PREV) != 0;
     constexpr bool HE = (MODE & M_HE) != 0;
@@ -704,7 +734,6 @@ __global__ void __launch_bounds__(kBlock) k_epi1d(PassArgs a, int nblocks)
     (void)HE_PREV;
     (void)HE;
 
-    extern __shared__ double smem[];
     __shared__ int s_last;
     const int nx = a.nx;
     const int tid = threadIdx.x;
@@ -734,7 +763,6 @@ __global__ void __launch_bounds__(kBlock) k_epi1d(PassArgs a, int nblocks)
     for (int i = tid; i < len; i += nt) sRed[i] = 0.0;
 
     // group sums of a slice of the workgroups' rows
-    const int ng = gridDim.x, g = blockIdx.x;
     const int b0 = (int)(((long)nblocks * g) / ng), b1 = (int)(((long)nblocks * (g + 1)) / ng);
     sum_rows_range(a.part, b0, b1, nx, REDMASK, a.gpart + (size_t)g * len, scratch);
     __threadfence();
@@ -836,6 +864,25 @@ __global__ void __launch_bounds__(kBlock) k_epi1d(PassArgs a, int nblocks)
                 a.hist[(NHIST + H_B) * nx + i] = sB[i];
             }
     }
+}
+
+template <int P0, int P1, int MODE>
+__global__ void __launch_bounds__(kBlock) k_epi1d(PassArgs a, int nblocks)
+{
+    extern __shared__ double smem[];
+    epi1d_body<P0, P1, MODE>(a, nblocks, blockIdx.x, gridDim.x, smem);
+}
+
+// The field parts of every instance of an ensemble in one launch: instance
+// i's epi_groups(its pass workgroups) workgroups are [epi_start[i], epi_start[i+1]).
+template <int P0, int P1, int MODE>
+__global__ void __launch_bounds__(kBlock) k_epi1d_batch(BatchArgs b)
+{
+    extern __shared__ double smem[];
+    const int inst = batch_instance(b.epi_start, b.ninst, blockIdx.x);
+    const int g0 = b.epi_start[inst];
+    epi1d_body<P0, P1, MODE>(b.args[inst], b.block_start[inst + 1] - b.block_start[inst],
+                             blockIdx.x - g0, b.epi_start[inst + 1] - g0, smem);
 }
 
 size_t epi_shmem_bytes(int nx)
@@ -996,6 +1043,58 @@ hipError_t launch_epi(int p0, int p1, int mode, const PassArgs &a, int pass_bloc
     GK_PAIR(3, 3)
 #undef GK_PAIR
     return hipErrorInvalidValue;
+}
+
+template <int P0, int P1>
+static hipError_t launch_batch_pd(bool epi, int mode, const BatchArgs &b, int total, size_t shmem,
+                                  hipStream_t s)
+{
+#define GK_B(M)                                                                                  \
+    case (M):                                                                                    \
+        if (epi)                                                                                 \
+            hipLaunchKernelGGL((k_epi1d_batch<P0, P1, (M)>), dim3(total), dim3(kBlock), shmem, s, b); \
+        else                                                                                     \
+            hipLaunchKernelGGL((k_pass1d_batch<P0, P1, (M)>), dim3(total), dim3(kBlock), shmem, s, b); \
+        return hipGetLastError();
+    switch (mode) {
+        GK_B(M_HE | M_HP2A | M_HP1 | M_HP2B)
+        GK_B(M_HE_PREV | M_HE | M_HP2A | M_HP1 | M_HP2B)
+    case M_HE_PREV:
+        if (epi) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_pass1d_batch<P0, P1, M_HE_PREV>), dim3(total), dim3(kBlock), shmem, s, b);
+        return hipGetLastError();
+    default:
+        return hipErrorInvalidValue;
+    }
+#undef GK_B
+}
+
+static hipError_t launch_batch(bool epi, int p0, int p1, int mode, const BatchArgs &b, int total,
+                               size_t shmem, hipStream_t s)
+{
+    if (total < 1 || b.ninst < 1) return hipErrorInvalidValue;
+#define GK_PAIR(A, B) \
+    if (p0 == (A) && p1 == (B)) return launch_batch_pd<A, B>(epi, mode, b, total, shmem, s);
+    GK_PAIR(1, 0)
+    GK_PAIR(1, 1)
+    GK_PAIR(2, 1)
+    GK_PAIR(2, 2)
+    GK_PAIR(3, 2)
+    GK_PAIR(3, 3)
+#undef GK_PAIR
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_pass_batch(int p0, int p1, int mode, const BatchArgs &b, int total_blocks,
+                             size_t shmem, hipStream_t s)
+{
+    return launch_batch(false, p0, p1, mode, b, total_blocks, shmem, s);
+}
+
+hipError_t launch_epi_batch(int p0, int p1, int mode, const BatchArgs &b, int total_groups,
+                            size_t shmem, hipStream_t s)
+{
+    return launch_batch(true, p0, p1, mode, b, total_groups, shmem, s);
 }
 
 template <int P0, int P1>

@@ -26,7 +26,8 @@ __all__ = [
     "solve_poisson_", "TimeHistoryDiagnostics", "write_step_", "get_charge",
     "HamiltonianSplittingBoris", "staggering_", "push_v_epart_", "push_v_bpart_",
     "push_x_accumulate_j_", "sample_", "sample_particles_", "strang_splitting_adjoint_",
-    "get_adjoint", "solve_poisson_adjoint", "AdjointResult",
+    "get_adjoint", "solve_poisson_adjoint", "AdjointResult", "Ensemble",
+    "ensemble_strang_splitting_",
 ]
 
 
@@ -459,6 +460,43 @@ def strang_splitting_(h: HamiltonianSplitting, dt: float, number_steps: int,
         return
     flags = _lib.STRANG_SUBSTEP if substep else _lib.STRANG_FUSED
     h._run(_lib.load().gempic_strang_splitting, float(dt), int(number_steps), flags)
+
+
+class Ensemble:
+    """Independent HamiltonianSplitting{1,2}s advanced together: one
+    particle-pass launch and one field-part launch per Strang step for all
+    of them (no counterpart in the reference: the batched form of
+    strang_splitting!). Each splitting keeps its own particles, grid,
+    domain and fields, and its results equal a stand-alone
+    strang_splitting_ call's bit for bit."""
+
+    def __init__(self, splittings):
+        self.splittings = list(splittings)
+        if not self.splittings:
+            raise ValueError("an ensemble needs at least one splitting")
+        arr = (ctypes.c_void_p * len(self.splittings))(*[h._h.value for h in self.splittings])
+        e = ctypes.c_void_p()
+        check(_lib.load().gempic_ensemble_create(len(self.splittings), arr, ctypes.byref(e)))
+        self._e = e
+
+    def __del__(self):
+        e = getattr(self, "_e", None)
+        if e is not None and e.value:
+            _lib.load().gempic_ensemble_destroy(e)
+            self._e = None
+
+
+def ensemble_strang_splitting_(ens: Ensemble, dt, number_steps: int):
+    """strang_splitting!(h, dt_i, number_steps) for every splitting h of ens
+    (dt: one value for all, or one per splitting)."""
+    dts = np.broadcast_to(np.asarray(dt, dtype=np.float64), (len(ens.splittings),)).copy()
+    for h in ens.splittings:
+        if h.sync:
+            h.to_device()
+    check(_lib.load().gempic_ensemble_strang_splitting(ens._e, ptr(dts), int(number_steps), 0))
+    for h in ens.splittings:
+        if h.sync:
+            h.to_host()
 
 
 class AdjointResult:

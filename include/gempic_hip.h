@@ -108,6 +108,21 @@ int gempic_strang_splitting(gempic_splitting *h, double dt, int64_t n_steps, int
 /* Wait for queued work and report any deferred error. */
 int gempic_synchronize(gempic_splitting *h);
 
+/* Ensembles (no counterpart in the reference; the batched form of
+ * strang_splitting!, hamiltonian_splitting.jl:98-108): independent 1d2v
+ * splittings, each with its own particles, grid size, domain and fields,
+ * advanced together with one particle-pass launch and one field-part launch
+ * per Strang step for all of them. Each splitting's results equal those of
+ * its own gempic_strang_splitting call bit for bit. The handles stay owned
+ * by the caller and must outlive the ensemble; they must share one GPU and
+ * one smoother pair. The call is synchronous. One step of every splitting is
+ * taken per launch pair, each with its own dt. */
+typedef struct gempic_ensemble gempic_ensemble;
+int gempic_ensemble_create(int n_instances, gempic_splitting *const *handles, gempic_ensemble **out);
+int gempic_ensemble_destroy(gempic_ensemble *e);
+/* dt: one time step per splitting (n_instances doubles) */
+int gempic_ensemble_strang_splitting(gempic_ensemble *e, const double *dt, int64_t n_steps, int flags);
+
 /* Reverse mode of strang_splitting! (1d2v). The reference has no
  * derivatives; this is the pullback a ChainRulesCore.rrule of
  * strang_splitting!(h, dt, n_steps) (hamiltonian_splitting.jl:98-108) would
