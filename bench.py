@@ -303,6 +303,61 @@ def adjoint_run(G, _lib, n, steps, every, cpu_n, cpu_steps):
     return out
 
 
+def ensemble_run(G, _lib, n_inst, steps, seq_steps, seed=5):
+    """Independent 1D2V Weibel instances: N uniform in [65536, 1048576],
+    nx in {16, 32, 64}, k uniform in [1.0, 1.5], alpha and beta log-uniform
+    in [1e-5, 1e-3] (seeded). Particles come from sample!(pg, alpha, k,
+    sigma, mesh) on the GPU (one sigma for both velocity components), E1
+    from Poisson, B0 = l2projection!(beta cos(kx)). dt = min(0.05, dx/2):
+    the degree-3 FEM Maxwell solver is unstable for dt > 0.64 dx, which
+    0.05 exceeds on the finest grids. Times `steps` batched Strang steps of
+    the whole ensemble against `seq_steps` steps run instance by instance."""
+    import math
+    from gempic_amd import workloads
+    L_ = _lib.load()
+    rng = np.random.default_rng(seed)
+    hs, dts, total = [], [], 0
+    for i in range(n_inst):
+        n = int(rng.integers(65536, 1048577))
+        nx = int(rng.choice([16, 32, 64]))
+        k = float(rng.uniform(1.0, 1.5))
+        alpha = float(np.exp(rng.uniform(math.log(1e-5), math.log(1e-3))))
+        beta = float(np.exp(rng.uniform(math.log(1e-5), math.log(1e-3))))
+        L = 2 * math.pi / k
+        mesh = G.OneDGrid(0.0, L, nx)
+        pg = G.ParticleGroup(1, 2, n)
+        ks0 = G.ParticleMeshCoupling1D(mesh, n, 3, "galerkin")
+        ks1 = G.ParticleMeshCoupling1D(mesh, n, 2, "galerkin")
+        e1, e2 = np.zeros(nx), np.zeros(nx)
+        b = workloads.l2projection_cos(nx, L, 2, beta, k)
+        h = G.HamiltonianSplitting(1, 2, G.Maxwell1DFEM(mesh, 3), ks0, ks1, pg, [e1, e2], b,
+                                   sync=False)
+        G.sample_particles_(h, alpha, k, 0.02 / math.sqrt(2.0))
+        G.solve_poisson_(e1, h, np.zeros(nx))
+        h.to_device(particles=False)
+        hs.append(h)
+        dts.append(min(0.05, 0.5 * L / nx))
+        total += n
+    ens = G.Ensemble(hs)
+    G.ensemble_strang_splitting_(ens, dts, 2)                # kernel loads
+    t0 = time.perf_counter()
+    G.ensemble_strang_splitting_(ens, dts, steps)
+    t_ens = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for h, dt in zip(hs, dts):
+        _lib.check(L_.gempic_strang_splitting(h._h, dt, seq_steps, _lib.STRANG_FUSED | _lib.ASYNC))
+    for h in hs:
+        _lib.check(L_.gempic_synchronize(h._h))
+    t_seq = time.perf_counter() - t0
+    return {"workload": f"{n_inst} independent 1D2V Weibel instances, {total} particles in all, "
+                        f"N in [65536, 1048576], nx in {{16, 32, 64}}, k in [1, 1.5], "
+                        f"dt = min(0.05, dx/2)",
+            "instances": n_inst, "particles": total, "steps": steps,
+            "particle_steps_per_s": total * steps / t_ens,
+            "one_by_one_particle_steps_per_s": total * seq_steps / t_seq,
+            "one_by_one_steps": seq_steps}
+
+
 def copy_calibration(mib):
     """Run a plain device copy of `mib` MiB five times (reads and writes mib MiB each).
 
@@ -340,6 +395,9 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=1000,
                     help="steps of the forward + reverse-mode measurements (0 = skip them)")
     ap.add_argument("--adjoint-every", type=int, default=50, help="checkpoint interval in steps")
+    ap.add_argument("--ensemble", type=int, default=256,
+                    help="instances of the ensemble measurement (0 = skip it)")
+    ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--particles-2d", type=int, default=16777216,
                     help="particles of the 2D pieces measurement (0 = skip it)")
     ap.add_argument("--reps-2d", type=int, default=20)
@@ -446,6 +504,13 @@ def main():
                 G, _lib, na, args.adjoint_steps, every,
                 0 if (args.no_cpu or na != n or every != 1) else 32768, 50)
 
+    ens = None
+    if args.ensemble:
+        print(f"[bench] ensemble of {args.ensemble} instances, {args.ensemble_steps} steps",
+              file=sys.stderr, flush=True)
+        ens = ensemble_run(G, _lib, args.ensemble, args.ensemble_steps,
+                           max(1, args.ensemble_steps // 5))
+
     samp = None
     if not args.no_sampling:
         print("[bench] sample! into GPU memory", file=sys.stderr, flush=True)
@@ -484,6 +549,8 @@ def main():
         line["boris_1d2v"] = boris
     if adj is not None:
         line["adjoint_1d2v"] = adj
+    if ens is not None:
+        line["ensemble_1d2v"] = ens
     if samp is not None:
         line["sampling"] = samp
     if p2d is not None:
