@@ -1265,6 +1265,8 @@ struct gempic_ensemble {
     PassArgs *d_args = nullptr;        // [2][ninst]: another step follows / the last step
     int *d_bstart = nullptr, *d_estart = nullptr;
     std::vector<int> bstart, estart;
+    SortArgs *d_sargs = nullptr;       // the batched sort's tables
+    int *d_tstart = nullptr, *d_gstart = nullptr;
 };
 
 namespace {
@@ -1301,6 +1303,70 @@ int ens_upload(gempic_ensemble *e, const double *dts)
     HIP_TRY(hipMemcpy(e->d_args, args.data(), sizeof(PassArgs) * args.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(e->d_bstart, e->bstart.data(), sizeof(int) * (ni + 1), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(e->d_estart, e->estart.data(), sizeof(int) * (ni + 1), hipMemcpyHostToDevice));
+    return GEMPIC_OK;
+}
+
+// SortArgs of a handle's sort (run_sort's), into its double buffer
+SortArgs sort_args(gempic_splitting *h)
+{
+    SortArgs s;
+    std::memset(&s, 0, sizeof(s));
+    s.n = h->n;
+    s.nx = (int)h->nx;
+    s.sub = h->sortg.sub;
+    s.nbins = h->sortg.nbins;
+    s.ntiles = h->sortg.ntiles;
+    s.tiles_per_group = h->sortg.tiles_per_group;
+    s.ngroups = h->sortg.ngroups;
+    s.xmin = h->xmin;
+    s.dx = h->dx;
+    s.x = h->d_x; s.v1 = h->d_v1; s.v2 = h->d_v2; s.w = h->d_w; s.id = h->d_id;
+    s.x2 = h->d_x2; s.v12 = h->d_v12; s.v22 = h->d_v22; s.w2 = h->d_w2; s.id2 = h->d_id2;
+    s.counts = h->d_sort_counts;
+    s.gsum = h->d_sort_gsum;
+    s.perm = h->cur_perm;
+    return s;
+}
+
+void sort_swap(gempic_splitting *h)
+{
+    std::swap(h->d_x, h->d_x2);
+    std::swap(h->d_v1, h->d_v12);
+    std::swap(h->d_v2, h->d_v22);
+    std::swap(h->d_w, h->d_w2);
+    std::swap(h->d_id, h->d_id2);
+    h->steps_since_sort = 0;
+}
+
+// the sorts that are due, in one launch per stage
+int ens_sort(gempic_ensemble *e, bool *any)
+{
+    std::vector<SortArgs> args;
+    std::vector<int> tstart(1, 0), gstart(1, 0);
+    std::vector<gempic_splitting *> due;
+    int max_bins = 1;
+    for (auto *h : e->hs)
+        if (h->sort_interval > 0 && h->steps_since_sort >= h->sort_interval) {
+            SortArgs a = sort_args(h);
+            args.push_back(a);
+            tstart.push_back(tstart.back() + (a.ntiles + kWaves - 1) / kWaves);
+            gstart.push_back(gstart.back() + a.ngroups);
+            max_bins = std::max(max_bins, a.nbins);
+            due.push_back(h);
+        }
+    *any = !due.empty();
+    if (due.empty()) return GEMPIC_OK;
+    const size_t nd = due.size();
+    HIP_TRY(hipMemcpy(e->d_sargs, args.data(), sizeof(SortArgs) * nd, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(e->d_tstart, tstart.data(), sizeof(int) * (nd + 1), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(e->d_gstart, gstart.data(), sizeof(int) * (nd + 1), hipMemcpyHostToDevice));
+    SortBatch b;
+    b.args = e->d_sargs;
+    b.tile_start = e->d_tstart;
+    b.group_start = e->d_gstart;
+    b.ninst = (int)nd;
+    HIP_TRY(launch_sort_batch(b, tstart.back(), gstart.back(), max_bins, e->stream));
+    for (auto *h : due) sort_swap(h);
     return GEMPIC_OK;
 }
 
@@ -1354,6 +1420,9 @@ int gempic_ensemble_create(int n_instances, gempic_splitting *const *handles, ge
     if (r == hipSuccess) r = dalloc(&e->d_args, 2 * (size_t)n_instances);
     if (r == hipSuccess) r = dalloc(&e->d_bstart, (size_t)n_instances + 1);
     if (r == hipSuccess) r = dalloc(&e->d_estart, (size_t)n_instances + 1);
+    if (r == hipSuccess) r = dalloc(&e->d_sargs, (size_t)n_instances);
+    if (r == hipSuccess) r = dalloc(&e->d_tstart, (size_t)n_instances + 1);
+    if (r == hipSuccess) r = dalloc(&e->d_gstart, (size_t)n_instances + 1);
     if (r != hipSuccess) {
         gempic_ensemble_destroy(e);
         return fail(GEMPIC_ERR_DEVICE, std::string("ensemble allocation: ") + hipGetErrorString(r));
@@ -1367,7 +1436,7 @@ int gempic_ensemble_destroy(gempic_ensemble *e)
     if (!e) return GEMPIC_OK;
     (void)hipSetDevice(e->device);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
-    void *bufs[] = {e->d_args, e->d_bstart, e->d_estart};
+    void *bufs[] = {e->d_args, e->d_bstart, e->d_estart, e->d_sargs, e->d_tstart, e->d_gstart};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -1400,13 +1469,8 @@ int gempic_ensemble_strang_splitting(gempic_ensemble *e, const double *dt, int64
     const size_t sh_epi = epi_shmem_bytes(e->max_nx);
     for (int64_t s = 0; s < n_steps; ++s) {
         bool moved = false;
-        for (auto *h : e->hs) {
-            if (h->sort_interval > 0 && h->steps_since_sort >= h->sort_interval) {
-                rc = run_sort(h);
-                if (rc) return rc;
-                moved = true;
-            }
-        }
+        rc = ens_sort(e, &moved);
+        if (rc) return rc;
         if (moved) {
             rc = ens_upload(e, dt);
             if (rc) return rc;

@@ -204,6 +204,17 @@ struct SortGeometry {
 };
 SortGeometry sort_geometry(int64_t n, int nx);
 hipError_t launch_sort(const SortArgs &a, int has_v2, hipStream_t s);
+// the sorts of an ensemble's instances (1d2v), one launch per stage (device
+// arrays: SortArgs per instance, first tile workgroup and first group
+// column of each instance, ninst + 1 entries each)
+struct SortBatch {
+    const SortArgs *args;
+    const int *tile_start;
+    const int *group_start;
+    int ninst;
+};
+hipError_t launch_sort_batch(const SortBatch &b, int total_tiles, int total_groups, int max_bins,
+                             hipStream_t s);
 
 // sample!(pg, alpha, k, sigma, mesh) into SoA arrays (id = identity); a
 // newton that does not converge sets E_FAR in err

@@ -715,7 +715,12 @@ int epi_groups(int nblocks)
     return g < 1 ? 1 : (g > kEpiGroupsMax ? kEpiGroupsMax : g);
 }
 
-template <int P0, int P1, int MODE>
+// PHASE 0: the whole field part (the slice sums, then the last workgroup
+// to finish sums the group rows and runs the solves); PHASE 1: only this
+// workgroup's slice sum into its group row; PHASE 2: only the part after the
+// slice sums (a later launch, so no fences or counters are needed). Phases 1
+// and 2 together add the same numbers in the same order as phase 0.
+template <int P0, int P1, int MODE, int PHASE = 0>
 __device__ __forceinline__ void epi1d_body(const PassArgs &a, int nblocks, int g, int ng,
                                            double *smem)
 {
@@ -751,6 +756,12 @@ __device__ __forceinline__ void epi1d_body(const PassArgs &a, int nblocks, int g
     double *w3 = T + NTAB * nx;             // [6][nx]
     double *scratch = w3 + 6 * nx;          // [blockDim.x]
 
+    if constexpr (PHASE == 1) {
+        const int b0 = (int)(((long)nblocks * g) / ng), b1 = (int)(((long)nblocks * (g + 1)) / ng);
+        sum_rows_range(a.part, b0, b1, nx, REDMASK, a.gpart + (size_t)g * len, scratch);
+        return;
+    }
+
     // every workgroup stages the fields and tables first: the one that ends up
     // last has them in LDS without another round trip
     double *G = a.grid;
@@ -762,16 +773,21 @@ __device__ __forceinline__ void epi1d_body(const PassArgs &a, int nblocks, int g
     for (int i = tid; i < NTAB * nx; i += nt) T[i] = a.tabs[i];
     for (int i = tid; i < len; i += nt) sRed[i] = 0.0;
 
-    // group sums of a slice of the workgroups' rows
-    const int b0 = (int)(((long)nblocks * g) / ng), b1 = (int)(((long)nblocks * (g + 1)) / ng);
-    sum_rows_range(a.part, b0, b1, nx, REDMASK, a.gpart + (size_t)g * len, scratch);
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) s_last = atomicAdd(&a.cnt[0], 1u) == (unsigned)(ng - 1);
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-    if (tid == 0) a.cnt[0] = 0u;            // counters are zero between launches
+    if constexpr (PHASE == 0) {
+        // group sums of a slice of the workgroups' rows
+        const int b0 = (int)(((long)nblocks * g) / ng), b1 = (int)(((long)nblocks * (g + 1)) / ng);
+        sum_rows_range(a.part, b0, b1, nx, REDMASK, a.gpart + (size_t)g * len, scratch);
+        __threadfence();
+        __syncthreads();
+        if (tid == 0) s_last = atomicAdd(&a.cnt[0], 1u) == (unsigned)(ng - 1);
+        __syncthreads();
+        if (!s_last) return;
+        __threadfence();
+        if (tid == 0) a.cnt[0] = 0u;            // counters are zero between launches
+    } else {
+        (void)s_last;
+        __syncthreads();
+    }
 
     sum_rows_range(a.gpart, 0, ng, nx, REDMASK, sRed, scratch);  // ends with a barrier
     double *r_j2a = sRed + D_J2A * nx, *r_j1 = sRed + D_J1 * nx;
@@ -873,16 +889,28 @@ __global__ void __launch_bounds__(kBlock) k_epi1d(PassArgs a, int nblocks)
     epi1d_body<P0, P1, MODE>(a, nblocks, blockIdx.x, gridDim.x, smem);
 }
 
-// The field parts of every instance of an ensemble in one launch: instance
-// i's epi_groups(its pass workgroups) workgroups are [epi_start[i], epi_start[i+1]).
+// The field parts of every instance of an ensemble in two launches: the
+// slice sums (instance i's epi_groups(its pass workgroups) workgroups are
+// [epi_start[i], epi_start[i+1])), then one workgroup per instance for the
+// rest. In one launch, the fences and counters of hundreds of instances'
+// last-workgroup handshakes serialise on the L2 write-backs.
 template <int P0, int P1, int MODE>
-__global__ void __launch_bounds__(kBlock) k_epi1d_batch(BatchArgs b)
+__global__ void __launch_bounds__(kBlock) k_epi1d_batch_sum(BatchArgs b)
 {
     extern __shared__ double smem[];
     const int inst = batch_instance(b.epi_start, b.ninst, blockIdx.x);
     const int g0 = b.epi_start[inst];
-    epi1d_body<P0, P1, MODE>(b.args[inst], b.block_start[inst + 1] - b.block_start[inst],
-                             blockIdx.x - g0, b.epi_start[inst + 1] - g0, smem);
+    epi1d_body<P0, P1, MODE, 1>(b.args[inst], b.block_start[inst + 1] - b.block_start[inst],
+                                blockIdx.x - g0, b.epi_start[inst + 1] - g0, smem);
+}
+
+template <int P0, int P1, int MODE>
+__global__ void __launch_bounds__(kBlock) k_epi1d_batch_fin(BatchArgs b)
+{
+    extern __shared__ double smem[];
+    const int inst = blockIdx.x;
+    epi1d_body<P0, P1, MODE, 2>(b.args[inst], b.block_start[inst + 1] - b.block_start[inst], 0,
+                                b.epi_start[inst + 1] - b.epi_start[inst], smem);
 }
 
 size_t epi_shmem_bytes(int nx)
@@ -1051,10 +1079,12 @@ static hipError_t launch_batch_pd(bool epi, int mode, const BatchArgs &b, int to
 {
 #define GK_B(M)                                                                                  \
     case (M):                                                                                    \
-        if (epi)                                                                                 \
-            hipLaunchKernelGGL((k_epi1d_batch<P0, P1, (M)>), dim3(total), dim3(kBlock), shmem, s, b); \
-        else                                                                                     \
+        if (epi) {                                                                               \
+            hipLaunchKernelGGL((k_epi1d_batch_sum<P0, P1, (M)>), dim3(total), dim3(kBlock), shmem, s, b); \
+            hipLaunchKernelGGL((k_epi1d_batch_fin<P0, P1, (M)>), dim3(b.ninst), dim3(kBlock), shmem, s, b); \
+        } else {                                                                                 \
             hipLaunchKernelGGL((k_pass1d_batch<P0, P1, (M)>), dim3(total), dim3(kBlock), shmem, s, b); \
+        }                                                                                        \
         return hipGetLastError();
     switch (mode) {
         GK_B(M_HE | M_HP2A | M_HP1 | M_HP2B)
@@ -1588,14 +1618,13 @@ __device__ __forceinline__ int sort_key(double x, double xmin, double dx, int nx
 // whose loads are issued together.
 constexpr int kSortBatch = 8;
 
-__global__ void __launch_bounds__(kBlock) k_sort_count(SortArgs a)
+__device__ __forceinline__ void sort_count_body(const SortArgs &a, int bx, int *ish)
 {
-    extern __shared__ int ish[];
     const int wave = threadIdx.x / kWave;
     const int lane = lane_id();
     const int nb = a.nbins;
     int *h = ish + wave * nb;
-    const int tile = blockIdx.x * kWaves + wave;
+    const int tile = bx * kWaves + wave;
     for (int c = lane; c < nb; c += kWave) h[c] = 0;
     __syncthreads();
     if (tile < a.ntiles) {
@@ -1628,12 +1657,11 @@ __global__ void __launch_bounds__(kBlock) k_sort_count(SortArgs a)
 }
 
 // gsum[g][b] = sum of counts[t][b] over the tiles t of group g
-__global__ void __launch_bounds__(kBlock) k_sort_gsum(SortArgs a)
+__device__ __forceinline__ void sort_gsum_body(const SortArgs &a, int g, int by)
 {
     const int nb = a.nbins;
-    const int b = blockIdx.y * kBlock + threadIdx.x;
+    const int b = by * kBlock + threadIdx.x;
     if (b >= nb) return;
-    const int g = blockIdx.x;
     const int t0 = g * a.tiles_per_group;
     const int t1 = min(a.ntiles, t0 + a.tiles_per_group);
     const int *col = a.counts + b;
@@ -1652,9 +1680,8 @@ __global__ void __launch_bounds__(kBlock) k_sort_gsum(SortArgs a)
 
 // One workgroup: bin totals, their exclusive scan (the bin bases), and
 // gsum[g][b] <- base[b] + sum of gsum[g'][b] over g' < g.
-__global__ void __launch_bounds__(kSortBinsMax) k_sort_bases(SortArgs a)
+__device__ __forceinline__ void sort_bases_body(const SortArgs &a, int *s_scan)
 {
-    __shared__ int s_scan[kSortBinsMax];
     const int nb = a.nbins;
     const int b = threadIdx.x;
     int *col = a.gsum + b;
@@ -1690,12 +1717,11 @@ __global__ void __launch_bounds__(kSortBinsMax) k_sort_bases(SortArgs a)
 }
 
 // counts[t][b] <- the first output position of bin b's particles of tile t
-__global__ void __launch_bounds__(kBlock) k_sort_offsets(SortArgs a)
+__device__ __forceinline__ void sort_offsets_body(const SortArgs &a, int g, int by)
 {
     const int nb = a.nbins;
-    const int b = blockIdx.y * kBlock + threadIdx.x;
+    const int b = by * kBlock + threadIdx.x;
     if (b >= nb) return;
-    const int g = blockIdx.x;
     const int t0 = g * a.tiles_per_group;
     const int t1 = min(a.ntiles, t0 + a.tiles_per_group);
     int *col = a.counts + b;
@@ -1718,14 +1744,13 @@ __global__ void __launch_bounds__(kBlock) k_sort_offsets(SortArgs a)
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_sort_scatter(SortArgs a, int has_v2)
+__device__ __forceinline__ void sort_scatter_body(const SortArgs &a, int has_v2, int bx, int *ish)
 {
-    extern __shared__ int ish[];
     const int wave = threadIdx.x / kWave;
     const int lane = lane_id();
     const int nb = a.nbins;
     int *cur = ish + wave * nb;
-    const int tile = blockIdx.x * kWaves + wave;
+    const int tile = bx * kWaves + wave;
     if (tile < a.ntiles) {
         const int *row = a.counts + (size_t)tile * nb;
         for (int c = lane; c < nb; c += kWave) cur[c] = row[c];
@@ -1773,6 +1798,85 @@ __global__ void __launch_bounds__(kBlock) k_sort_scatter(SortArgs a, int has_v2)
             }
         }
     }
+}
+
+__global__ void __launch_bounds__(kBlock) k_sort_count(SortArgs a)
+{
+    extern __shared__ int ish[];
+    sort_count_body(a, blockIdx.x, ish);
+}
+
+__global__ void __launch_bounds__(kBlock) k_sort_gsum(SortArgs a)
+{
+    sort_gsum_body(a, blockIdx.x, blockIdx.y);
+}
+
+__global__ void __launch_bounds__(kSortBinsMax) k_sort_bases(SortArgs a)
+{
+    __shared__ int s_scan[kSortBinsMax];
+    sort_bases_body(a, s_scan);
+}
+
+__global__ void __launch_bounds__(kBlock) k_sort_offsets(SortArgs a)
+{
+    sort_offsets_body(a, blockIdx.x, blockIdx.y);
+}
+
+__global__ void __launch_bounds__(kBlock) k_sort_scatter(SortArgs a, int has_v2)
+{
+    extern __shared__ int ish[];
+    sort_scatter_body(a, has_v2, blockIdx.x, ish);
+}
+
+// The sorts of every instance of an ensemble, one launch per stage: tile
+// workgroups of instance i are [tile_start[i], tile_start[i+1]), group
+// columns [group_start[i], group_start[i+1]) (y: bins / kBlock, at most that
+// of the largest grid), one workgroup per instance for the bases.
+__global__ void __launch_bounds__(kBlock) k_sort_count_batch(SortBatch b)
+{
+    extern __shared__ int ish[];
+    const int i = batch_instance(b.tile_start, b.ninst, blockIdx.x);
+    sort_count_body(b.args[i], blockIdx.x - b.tile_start[i], ish);
+}
+
+__global__ void __launch_bounds__(kBlock) k_sort_gsum_batch(SortBatch b)
+{
+    const int i = batch_instance(b.group_start, b.ninst, blockIdx.x);
+    sort_gsum_body(b.args[i], blockIdx.x - b.group_start[i], blockIdx.y);
+}
+
+__global__ void __launch_bounds__(kSortBinsMax) k_sort_bases_batch(SortBatch b)
+{
+    __shared__ int s_scan[kSortBinsMax];
+    sort_bases_body(b.args[blockIdx.x], s_scan);
+}
+
+__global__ void __launch_bounds__(kBlock) k_sort_offsets_batch(SortBatch b)
+{
+    const int i = batch_instance(b.group_start, b.ninst, blockIdx.x);
+    sort_offsets_body(b.args[i], blockIdx.x - b.group_start[i], blockIdx.y);
+}
+
+__global__ void __launch_bounds__(kBlock) k_sort_scatter_batch(SortBatch b)
+{
+    extern __shared__ int ish[];
+    const int i = batch_instance(b.tile_start, b.ninst, blockIdx.x);
+    sort_scatter_body(b.args[i], 1, blockIdx.x - b.tile_start[i], ish);
+}
+
+hipError_t launch_sort_batch(const SortBatch &b, int total_tiles, int total_groups, int max_bins,
+                             hipStream_t s)
+{
+    if (b.ninst < 1 || total_tiles < 1 || total_groups < 1 || max_bins > kSortBinsMax)
+        return hipErrorInvalidValue;
+    const size_t sh = sizeof(int) * (size_t)kWaves * max_bins;
+    const dim3 cols(total_groups, (max_bins + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_sort_count_batch, dim3(total_tiles), dim3(kBlock), sh, s, b);
+    hipLaunchKernelGGL(k_sort_gsum_batch, cols, dim3(kBlock), 0, s, b);
+    hipLaunchKernelGGL(k_sort_bases_batch, dim3(b.ninst), dim3(kSortBinsMax), 0, s, b);
+    hipLaunchKernelGGL(k_sort_offsets_batch, cols, dim3(kBlock), 0, s, b);
+    hipLaunchKernelGGL(k_sort_scatter_batch, dim3(total_tiles), dim3(kBlock), sh, s, b);
+    return hipGetLastError();
 }
 
 SortGeometry sort_geometry(int64_t n, int nx)
