@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -1267,6 +1268,8 @@ struct gempic_ensemble {
     std::vector<int> bstart, estart;
     SortArgs *d_sargs = nullptr;       // the batched sort's tables
     int *d_tstart = nullptr, *d_gstart = nullptr;
+    AdjArgs *d_aargs = nullptr;        // the batched reverse mode's arguments and tables
+    int *d_abstart = nullptr, *d_aestart = nullptr;
 };
 
 namespace {
@@ -1423,6 +1426,9 @@ int gempic_ensemble_create(int n_instances, gempic_splitting *const *handles, ge
     if (r == hipSuccess) r = dalloc(&e->d_sargs, (size_t)n_instances);
     if (r == hipSuccess) r = dalloc(&e->d_tstart, (size_t)n_instances + 1);
     if (r == hipSuccess) r = dalloc(&e->d_gstart, (size_t)n_instances + 1);
+    if (r == hipSuccess) r = dalloc(&e->d_aargs, (size_t)n_instances);
+    if (r == hipSuccess) r = dalloc(&e->d_abstart, (size_t)n_instances + 1);
+    if (r == hipSuccess) r = dalloc(&e->d_aestart, (size_t)n_instances + 1);
     if (r != hipSuccess) {
         gempic_ensemble_destroy(e);
         return fail(GEMPIC_ERR_DEVICE, std::string("ensemble allocation: ") + hipGetErrorString(r));
@@ -1436,7 +1442,8 @@ int gempic_ensemble_destroy(gempic_ensemble *e)
     if (!e) return GEMPIC_OK;
     (void)hipSetDevice(e->device);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
-    void *bufs[] = {e->d_args, e->d_bstart, e->d_estart, e->d_sargs, e->d_tstart, e->d_gstart};
+    void *bufs[] = {e->d_args, e->d_bstart, e->d_estart, e->d_sargs, e->d_tstart, e->d_gstart,
+                    e->d_aargs, e->d_abstart, e->d_aestart};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -1493,6 +1500,261 @@ int gempic_ensemble_strang_splitting(gempic_ensemble *e, const double *dt, int64
     b.ninst = ni;
     HIP_TRY(launch_pass_batch(e->p0, e->p1, M_HE_PREV, b, e->bstart[ni],
                               pass_shmem_bytes(e->max_nx, M_HE_PREV), e->stream));
+    (void)flags;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    for (auto *h : e->hs) {
+        rc = finish(h);
+        if (rc) return rc;
+    }
+    return GEMPIC_OK;
+}
+
+
+// Reverse mode of an ensemble: gempic_strang_splitting_adjoint for every
+// instance, with the recorded forward, the re-runs, the reverse passes and
+// their field parts batched over the instances (one launch each per step;
+// the field parts as two). Each instance's loss and cotangents equal those
+// of its own gempic_strang_splitting_adjoint bit for bit (gempic_get_adjoint
+// reads them per splitting). No cotangents of the final state: the loss is
+// loss_weight[i] * sum_s 1/2 l2norm_squared(B_s).
+int gempic_ensemble_strang_splitting_adjoint(gempic_ensemble *e, const double *dt, int64_t n_steps,
+                                             int64_t checkpoint_every, const double *loss_weight,
+                                             int flags)
+{
+    if (!e) return fail(GEMPIC_ERR_ARGUMENT, "null ensemble");
+    if (!dt || !loss_weight) return fail(GEMPIC_ERR_ARGUMENT, "dt and loss_weight must not be NULL");
+    if (hipSetDevice(e->device) != hipSuccess) return fail(GEMPIC_ERR_DEVICE, "hipSetDevice failed");
+    if (n_steps < 1) return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: number_steps must be >= 1");
+    if (checkpoint_every < 1) return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: checkpoint_every must be >= 1");
+    for (auto *h : e->hs) {
+        if (!h->particles_set) return fail(GEMPIC_ERR_STATE, "particles not set");
+        if (h->nx > kAdjNxMax || h->nx < 4)
+            return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: the reverse mode needs 4 <= nx <= 128");
+    }
+    EnsStreams streams(e);
+    struct RecordOff {
+        gempic_ensemble *e;
+        ~RecordOff()
+        {
+            for (auto *h : e->hs) { h->cur_hist = nullptr; h->cur_cx = nullptr; h->cur_perm = nullptr; }
+        }
+    } record_off{e};
+    const int ni = (int)e->hs.size();
+    const int64_t T = n_steps, K = std::min<int64_t>(checkpoint_every, n_steps);
+    const int64_t nck = (T + K - 1) / K;
+    const bool dual = K == 1;
+    int rc = GEMPIC_OK;
+    for (auto *h : e->hs) {
+        rc = adj_workspace(h, T, K);
+        if (!rc) rc = ensure_stage(h);
+        if (rc) return rc;
+        h->adj_ready = false;
+    }
+    auto rec = [](gempic_splitting *h, int64_t s) { return h->d_hist + (size_t)s * NHIST * h->nx; };
+    struct Inst {
+        std::vector<int64_t> epoch_of;
+        int64_t ne = 0;
+        double *lam = nullptr;
+        Launch L{};
+    };
+    std::vector<Inst> st((size_t)ni);
+
+    // ---- forward, recorded ----
+    for (int i = 0; i < ni; ++i) {
+        gempic_splitting *h = e->hs[i];
+        const size_t nx = (size_t)h->nx;
+        const double hdt = 0.5 * dt[i];
+        HIP_TRY(hipMemcpyAsync(rec(h, 0) + H_BEND * nx, h->d_grid + S_B * nx, sizeof(double) * nx,
+                               hipMemcpyDeviceToDevice, e->stream));
+        GridProg g = prog1(G_HB, hdt, S_B, S_B, S_E2);
+        prog_add(g, G_HE_B, hdt, S_E2, S_E2, S_B);
+        rc = run_grid(h, g);
+        if (rc) return rc;
+        HIP_TRY(hipMemcpyAsync(rec(h, 1), h->d_grid, sizeof(double) * 4 * nx, hipMemcpyDeviceToDevice,
+                               e->stream));
+        st[i].epoch_of.assign((size_t)nck, 0);
+    }
+    rc = ens_upload(e, dt);          // workgroup tables (the arguments are rebuilt every step)
+    if (rc) return rc;
+    const size_t sh_pass = pass_shmem_bytes(e->max_nx, M_HE_PREV | M_HE | M_HP2A | M_HP1 | M_HP2B);
+    const size_t sh_epi = epi_shmem_bytes(e->max_nx);
+    std::vector<PassArgs> pa((size_t)ni);
+    auto pass_step = [&](int64_t s, bool trailing) -> int {
+        for (int i = 0; i < ni; ++i) {
+            gempic_splitting *h = e->hs[i];
+            const int target = h->override_blocks ? h->override_blocks : h->dep_blocks;
+            const Launch L = plan(h->n, target);
+            h->cur_hist = trailing ? nullptr : rec(h, s);
+            h->cur_cx = (!trailing && dual && s < T) ? h->d_ckpt + (size_t)s * 3 * h->n : nullptr;
+            PassArgs a = pass_args(h);
+            const double hdt = 0.5 * dt[i];
+            a.chunk = L.chunk;
+            a.iters = L.iters;
+            a.ngroups = L.ngroups;
+            a.dt_he = hdt;
+            a.dt_hp2 = hdt;
+            a.dt_hp1 = 1.0 * dt[i];
+            a.dtqm_he = hdt * h->qm;
+            a.dtqm_hp2 = hdt * h->qm;
+            a.more = (!trailing && s < T) ? 1 : 0;
+            pa[(size_t)i] = a;
+            h->cur_hist = nullptr;
+            h->cur_cx = nullptr;
+        }
+        HIP_TRY(hipMemcpy(e->d_args, pa.data(), sizeof(PassArgs) * (size_t)ni, hipMemcpyHostToDevice));
+        BatchArgs b;
+        b.args = e->d_args;
+        b.block_start = e->d_bstart;
+        b.epi_start = e->d_estart;
+        b.ninst = ni;
+        if (trailing) {
+            HIP_TRY(launch_pass_batch(e->p0, e->p1, M_HE_PREV, b, e->bstart[ni],
+                                      pass_shmem_bytes(e->max_nx, M_HE_PREV), e->stream));
+            return GEMPIC_OK;
+        }
+        const int mode = (s > 1 ? M_HE_PREV : 0) | M_HE | M_HP2A | M_HP1 | M_HP2B;
+        HIP_TRY(launch_pass_batch(e->p0, e->p1, mode, b, e->bstart[ni], sh_pass, e->stream));
+        HIP_TRY(launch_epi_batch(e->p0, e->p1, mode, b, e->estart[ni], sh_epi, e->stream));
+        return GEMPIC_OK;
+    };
+    for (int64_t s = 1; s <= T; ++s) {
+        if ((s - 1) % K == 0) {
+            const size_t j = (size_t)((s - 1) / K);
+            std::vector<char> sorted((size_t)ni, 0);
+            if (s > 1) {
+                for (int i = 0; i < ni; ++i)
+                    e->hs[i]->cur_perm = e->hs[i]->d_perm + (size_t)st[i].ne * e->hs[i]->n;
+                bool any = false;
+                rc = ens_sort(e, &any);
+                for (auto *h : e->hs) h->cur_perm = nullptr;
+                if (rc) return rc;
+                for (int i = 0; i < ni; ++i) sorted[i] = e->hs[i]->steps_since_sort == 0;
+                if (any) {
+                    rc = ens_upload(e, dt);
+                    if (rc) return rc;
+                }
+            }
+            for (int i = 0; i < ni; ++i) {
+                gempic_splitting *h = e->hs[i];
+                const size_t n = (size_t)h->n, nb = sizeof(double) * n;
+                if (s == 1)
+                    HIP_TRY(hipMemcpyAsync(h->d_ids, h->d_id, sizeof(int64_t) * n, hipMemcpyDeviceToDevice,
+                                           e->stream));
+                if (s == 1 || sorted[i]) {
+                    HIP_TRY(hipMemcpyAsync(h->d_wsnap + (size_t)st[i].ne * n, h->d_w, nb,
+                                           hipMemcpyDeviceToDevice, e->stream));
+                    ++st[i].ne;
+                }
+                st[i].epoch_of[j] = st[i].ne - 1;
+                if (!dual || s == 1 || sorted[i]) {
+                    double *ck = h->d_ckpt + j * 3 * n;
+                    HIP_TRY(hipMemcpyAsync(ck, h->d_x, nb, hipMemcpyDeviceToDevice, e->stream));
+                    HIP_TRY(hipMemcpyAsync(ck + n, h->d_v1, nb, hipMemcpyDeviceToDevice, e->stream));
+                    HIP_TRY(hipMemcpyAsync(ck + 2 * n, h->d_v2, nb, hipMemcpyDeviceToDevice, e->stream));
+                }
+            }
+        }
+        rc = pass_step(s, false);
+        if (rc) return rc;
+        for (auto *h : e->hs) h->steps_since_sort++;
+    }
+    rc = pass_step(T, true);
+    if (rc) return rc;
+
+    // ---- reverse ----
+    std::vector<int> abstart((size_t)ni + 1, 0), aestart((size_t)ni + 1, 0);
+    for (int i = 0; i < ni; ++i) {
+        gempic_splitting *h = e->hs[i];
+        st[i].L = plan(h->n, h->adj_blocks);
+        st[i].lam = h->d_lam;
+        abstart[i + 1] = abstart[i] + st[i].L.blocks;
+        aestart[i + 1] = aestart[i] + epi_groups(st[i].L.blocks);
+        HIP_TRY(hipMemsetAsync(h->d_agrid, 0, sizeof(double) * NASLOT * h->nx, e->stream));
+        HIP_TRY(hipMemsetAsync(h->d_lam, 0, sizeof(double) * 3 * h->n, e->stream));
+    }
+    HIP_TRY(hipMemcpy(e->d_abstart, abstart.data(), sizeof(int) * (ni + 1), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(e->d_aestart, aestart.data(), sizeof(int) * (ni + 1), hipMemcpyHostToDevice));
+    AdjBatch ab;
+    ab.args = e->d_aargs;
+    ab.block_start = e->d_abstart;
+    ab.epi_start = e->d_aestart;
+    ab.ninst = ni;
+    ab.max_nx = e->max_nx;
+    std::vector<AdjArgs> aa((size_t)ni);
+    // fill one AdjArgs per instance, upload, launch the pass and (kind >= 0) its field part
+    auto rev_launch = [&](int rm, int kind, const std::function<void(int, AdjArgs &)> &set) -> int {
+        for (int i = 0; i < ni; ++i) {
+            gempic_splitting *h = e->hs[i];
+            AdjArgs A = adj_args(h, dt[i], st[i].L);
+            A.wloss = loss_weight[i];
+            A.lx = st[i].lam;
+            A.lv1 = st[i].lam + h->n;
+            A.lv2 = st[i].lam + 2 * h->n;
+            set(i, A);
+            aa[(size_t)i] = A;
+        }
+        HIP_TRY(hipMemcpy(e->d_aargs, aa.data(), sizeof(AdjArgs) * (size_t)ni, hipMemcpyHostToDevice));
+        HIP_TRY(launch_adj_batch(e->p0, e->p1, rm, ab, abstart[ni], e->stream));
+        if (kind >= 0) HIP_TRY(launch_adj_epi_batch(kind, ab, aestart[ni], e->stream));
+        return GEMPIC_OK;
+    };
+    rc = rev_launch(R_HEONLY, AE_TAIL, [&](int i, AdjArgs &A) {
+        gempic_splitting *h = e->hs[i];
+        A.x = h->d_x;
+        A.fields = rec(h, T + 1);
+        A.bend = rec(h, T) + H_BEND * h->nx;
+    });
+    for (int64_t j = nck - 1; j >= 0 && !rc; --j) {
+        const int64_t s_lo = j * K + 1, s_hi = std::min((j + 1) * K, T);
+        auto state = [&](gempic_splitting *h, int64_t s) {
+            const size_t n = (size_t)h->n;
+            return s == s_lo ? h->d_ckpt + (size_t)j * 3 * n : h->d_seg + (size_t)(s - s_lo - 1) * 3 * n;
+        };
+        auto common = [&](int i, AdjArgs &A, int64_t s) {
+            gempic_splitting *h = e->hs[i];
+            const size_t n = (size_t)h->n;
+            double *b0 = state(h, s);
+            A.x = b0;
+            A.v1 = b0 + n;
+            A.v2 = b0 + 2 * n;
+            A.fields = rec(h, s);
+            A.p.w = h->d_wsnap + (size_t)st[i].epoch_of[(size_t)j] * n;
+        };
+        for (int64_t s = s_lo; s < s_hi && !rc; ++s)
+            rc = rev_launch(R_FWD | (s > 1 ? R_PREV : 0), -1, [&](int i, AdjArgs &A) {
+                common(i, A, s);
+                gempic_splitting *h = e->hs[i];
+                double *b1 = state(h, s + 1);
+                A.xo = b1;
+                A.v1o = b1 + h->n;
+                A.v2o = b1 + 2 * h->n;
+            });
+        for (int64_t s = s_hi; s >= s_lo && !rc; --s)
+            rc = rev_launch(R_REV | (s > 1 ? R_PREV : 0), s > 1 ? AE_STEP : AE_HEAD,
+                            [&](int i, AdjArgs &A) {
+                                common(i, A, s);
+                                gempic_splitting *h = e->hs[i];
+                                A.bend = rec(h, s > 1 ? s - 1 : 0) + H_BEND * h->nx;
+                            });
+        if (!rc && j > 0)
+            for (int i = 0; i < ni; ++i) {
+                if (st[i].epoch_of[(size_t)j] == st[i].epoch_of[(size_t)j - 1]) continue;
+                gempic_splitting *h = e->hs[i];
+                double *other = st[i].lam == h->d_lam ? h->d_stage : h->d_lam;
+                HIP_TRY(launch_adj_unsort(h->d_perm + (size_t)st[i].epoch_of[(size_t)j] * h->n, h->n,
+                                          st[i].lam, other, e->stream));
+                st[i].lam = other;
+            }
+    }
+    if (rc) return rc;
+    for (int i = 0; i < ni; ++i) {
+        gempic_splitting *h = e->hs[i];
+        if (st[i].lam != h->d_lam)
+            HIP_TRY(hipMemcpyAsync(h->d_lam, st[i].lam, sizeof(double) * 3 * h->n, hipMemcpyDeviceToDevice,
+                                   e->stream));
+        h->adj_id0 = h->d_ids;
+        h->adj_ready = true;
+    }
     (void)flags;
     HIP_TRY(hipStreamSynchronize(e->stream));
     for (auto *h : e->hs) {

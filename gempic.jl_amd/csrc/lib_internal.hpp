@@ -168,6 +168,18 @@ struct AdjArgs {
     double wloss;                 // weight of 1/2 l2norm_squared(B) per step
 };
 hipError_t launch_adj(int p0, int p1, int rm, const AdjArgs &a, int blocks, hipStream_t s);
+// an ensemble's instances in one launch (device arrays as BatchArgs'); the
+// field part runs as two launches (slice sums, then one workgroup per instance)
+struct AdjBatch {
+    const AdjArgs *args;
+    const int *block_start;
+    const int *epi_start;
+    int ninst;
+    int max_nx;
+};
+hipError_t launch_adj_batch(int p0, int p1, int rm, const AdjBatch &b, int total_blocks,
+                            hipStream_t s);
+hipError_t launch_adj_epi_batch(int kind, const AdjBatch &b, int total_groups, hipStream_t s);
 hipError_t launch_adj_epi(int kind, const AdjArgs &a, int pass_blocks, hipStream_t s);
 int adj_blocks_per_cu(int p0, int p1, int rm);
 // AoS rows 0..2 of the caller's array (particle id[i] of the caller) into SoA, and back

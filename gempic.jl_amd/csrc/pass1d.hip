@@ -2272,7 +2272,7 @@ size_t adj_shmem_bytes(int nx, int slots)
 }
 
 template <int P0, int P1, int RM>
-__global__ void __launch_bounds__(kBlock, GEMPIC_ADJ_WAVES) k_adj1d(AdjArgs A)
+__device__ __forceinline__ void adj1d_body(const AdjArgs &A, int bid, double *smem)
 {
     constexpr bool PREV = (RM & R_PREV) != 0;
     constexpr bool FWD = (RM & R_FWD) != 0;
@@ -2287,7 +2287,6 @@ __global__ void __launch_bounds__(kBlock, GEMPIC_ADJ_WAVES) k_adj1d(AdjArgs A)
     static_assert(P0 <= kPad && P1 <= kPad, "LDS field padding must cover the spline support");
 
     const PassArgs &a = A.p;
-    extern __shared__ double smem[];
     const int nx = a.nx;
     const int tid = threadIdx.x;
     const int wave = tid / kWave;
@@ -2326,7 +2325,7 @@ __global__ void __launch_bounds__(kBlock, GEMPIC_ADJ_WAVES) k_adj1d(AdjArgs A)
     const double xmin = a.xmin, dx = a.dx, qm = a.qm;
     const double dtqm = a.dtqm_he, g = a.dtqm_hp2, dt1 = a.dt_hp1;
     const double s0 = a.scaling0, s1 = a.scaling1;
-    const int64_t base = (int64_t)blockIdx.x * a.chunk;
+    const int64_t base = (int64_t)bid * a.chunk;
     const int64_t left = a.n - base;
     const int cnt_here = left < a.chunk ? (int)left : (int)a.chunk;
 
@@ -2639,9 +2638,25 @@ __global__ void __launch_bounds__(kBlock, GEMPIC_ADJ_WAVES) k_adj1d(AdjArgs A)
             if (!((MASK >> (idx / nx)) & 1u)) continue;
             double s = 0.0;
             for (int wv = 0; wv < kWaves; ++wv) s += rows[wv * len + idx];
-            a.part[(size_t)blockIdx.x * len + idx] = s;
+            a.part[(size_t)bid * len + idx] = s;
         }
     }
+}
+
+template <int P0, int P1, int RM>
+__global__ void __launch_bounds__(kBlock, GEMPIC_ADJ_WAVES) k_adj1d(AdjArgs A)
+{
+    extern __shared__ double smem[];
+    adj1d_body<P0, P1, RM>(A, blockIdx.x, smem);
+}
+
+// every instance of an ensemble in one launch (as k_pass1d_batch)
+template <int P0, int P1, int RM>
+__global__ void __launch_bounds__(kBlock, GEMPIC_ADJ_WAVES) k_adj1d_batch(AdjBatch b)
+{
+    extern __shared__ double smem[];
+    const int inst = batch_instance(b.block_start, b.ninst, blockIdx.x);
+    adj1d_body<P0, P1, RM>(b.args[inst], blockIdx.x - b.block_start[inst], smem);
 }
 
 // lambda_e[j] += coef (lambda_b[j+1] - lambda_b[j]): the transpose of
@@ -2666,11 +2681,11 @@ __device__ void adj_b_from_e(double *le, double coef, const double *lb, int n)
 // and then, for AE_TAIL and AE_STEP, mu1 = -(1/dx) M1^-1 lambda_E1 and
 // mu2 = -(c/dx) M0^-1 lambda_E2 for the next reverse pass. AE_POIS computes
 // nu = C_pois D^T lambda_E1 for the solve_poisson! gradient.
-template <int KIND>
-__global__ void __launch_bounds__(kBlock) k_adj_epi(AdjArgs A, int nblocks)
+template <int KIND, int PHASE = 0>
+__device__ __forceinline__ void adj_epi_body(const AdjArgs &A, int nblocks, int gb, int ng,
+                                             double *smem)
 {
     const PassArgs &a = A.p;
-    extern __shared__ double smem[];
     __shared__ int s_last;
     const int nx = a.nx;
     const int tid = threadIdx.x, nt = blockDim.x;
@@ -2701,16 +2716,22 @@ __global__ void __launch_bounds__(kBlock) k_adj_epi(AdjArgs A, int nblocks)
                                                 : (1u << AR_E1) | (1u << AR_E2) | (1u << AR_B) |
                                                       (1u << AR_SCAL);
     if constexpr (MASK != 0u) {
-        const int ng = gridDim.x, gb = blockIdx.x;
-        const int b0 = (int)(((long)nblocks * gb) / ng), b1 = (int)(((long)nblocks * (gb + 1)) / ng);
-        sum_rows_range(a.part, b0, b1, nx, MASK, a.gpart + (size_t)gb * len, scratch);
-        __threadfence();
-        __syncthreads();
-        if (tid == 0) s_last = atomicAdd(&a.cnt[0], 1u) == (unsigned)(ng - 1);
-        __syncthreads();
-        if (!s_last) return;
-        __threadfence();
-        if (tid == 0) a.cnt[0] = 0u;
+        // PHASE 1: only this workgroup's slice sum; PHASE 2: only the rest (a
+        // later launch, no fences or counters); PHASE 0: both, as one launch
+        if constexpr (PHASE != 2) {
+            const int b0 = (int)(((long)nblocks * gb) / ng), b1 = (int)(((long)nblocks * (gb + 1)) / ng);
+            sum_rows_range(a.part, b0, b1, nx, MASK, a.gpart + (size_t)gb * len, scratch);
+            if constexpr (PHASE == 1) return;
+            __threadfence();
+            __syncthreads();
+            if (tid == 0) s_last = atomicAdd(&a.cnt[0], 1u) == (unsigned)(ng - 1);
+            __syncthreads();
+            if (!s_last) return;
+            __threadfence();
+            if (tid == 0) a.cnt[0] = 0u;
+        } else {
+            (void)s_last;
+        }
         sum_rows_range(a.gpart, 0, ng, nx, MASK, red, scratch);
     }
     __syncthreads();
@@ -2822,6 +2843,32 @@ __global__ void __launch_bounds__(kBlock) k_adj_epi(AdjArgs A, int nblocks)
     }
 }
 
+template <int KIND>
+__global__ void __launch_bounds__(kBlock) k_adj_epi(AdjArgs A, int nblocks)
+{
+    extern __shared__ double smem[];
+    adj_epi_body<KIND>(A, nblocks, blockIdx.x, gridDim.x, smem);
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(kBlock) k_adj_epi_batch_sum(AdjBatch b)
+{
+    extern __shared__ double smem[];
+    const int inst = batch_instance(b.epi_start, b.ninst, blockIdx.x);
+    const int g0 = b.epi_start[inst];
+    adj_epi_body<KIND, 1>(b.args[inst], b.block_start[inst + 1] - b.block_start[inst],
+                          blockIdx.x - g0, b.epi_start[inst + 1] - g0, smem);
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(kBlock) k_adj_epi_batch_fin(AdjBatch b)
+{
+    extern __shared__ double smem[];
+    const int inst = blockIdx.x;
+    adj_epi_body<KIND, 2>(b.args[inst], b.block_start[inst + 1] - b.block_start[inst], 0,
+                          b.epi_start[inst + 1] - b.epi_start[inst], smem);
+}
+
 size_t adj_epi_shmem_bytes(int nx)
 {
     return sizeof(double) * ((size_t)(3 + NDEP + 6 + NTAB + 2) * nx + kBlock);
@@ -2887,6 +2934,61 @@ int adj_blocks_per_cu(int p0, int p1, int rm)
     GK_Q(3, 3)
 #undef GK_Q
     return nb > 0 ? nb : 1;
+}
+
+template <int P0, int P1>
+static hipError_t launch_adj_batch_pd(int rm, const AdjBatch &b, int total, hipStream_t s)
+{
+    const size_t sh = adj_shmem_bytes(b.max_nx, adj_slots<P0, P1>());
+#define GK_CASE(M)                                                                              \
+    case (M):                                                                                   \
+        hipLaunchKernelGGL((k_adj1d_batch<P0, P1, (M)>), dim3(total), dim3(kBlock), sh, s, b); \
+        return hipGetLastError();
+    switch (rm) {
+        GK_CASE(R_FWD)
+        GK_CASE(R_FWD | R_PREV)
+        GK_CASE(R_REV)
+        GK_CASE(R_REV | R_PREV)
+        GK_CASE(R_HEONLY)
+    default:
+        return hipErrorInvalidValue;
+    }
+#undef GK_CASE
+}
+
+hipError_t launch_adj_batch(int p0, int p1, int rm, const AdjBatch &b, int total_blocks,
+                            hipStream_t s)
+{
+    if (b.max_nx > kAdjNxMax || total_blocks < 1 || b.ninst < 1) return hipErrorInvalidValue;
+#define GK_PAIR(A, B) \
+    if (p0 == (A) && p1 == (B)) return launch_adj_batch_pd<A, B>(rm, b, total_blocks, s);
+    GK_PAIR(1, 0)
+    GK_PAIR(1, 1)
+    GK_PAIR(2, 1)
+    GK_PAIR(2, 2)
+    GK_PAIR(3, 2)
+    GK_PAIR(3, 3)
+#undef GK_PAIR
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_adj_epi_batch(int kind, const AdjBatch &b, int total_groups, hipStream_t s)
+{
+    if (b.max_nx > kAdjNxMax || total_groups < 1 || b.ninst < 1) return hipErrorInvalidValue;
+    const size_t sh = adj_epi_shmem_bytes(b.max_nx);
+#define GK_K(K)                                                                                        \
+    case (K):                                                                                          \
+        hipLaunchKernelGGL(k_adj_epi_batch_sum<K>, dim3(total_groups), dim3(kBlock), sh, s, b);    \
+        hipLaunchKernelGGL(k_adj_epi_batch_fin<K>, dim3(b.ninst), dim3(kBlock), sh, s, b);          \
+        return hipGetLastError();
+    switch (kind) {
+        GK_K(AE_TAIL)
+        GK_K(AE_STEP)
+        GK_K(AE_HEAD)
+    default:
+        return hipErrorInvalidValue;
+    }
+#undef GK_K
 }
 
 hipError_t launch_adj_epi(int kind, const AdjArgs &a, int pass_blocks, hipStream_t s)

@@ -27,7 +27,7 @@ __all__ = [
     "HamiltonianSplittingBoris", "staggering_", "push_v_epart_", "push_v_bpart_",
     "push_x_accumulate_j_", "sample_", "sample_particles_", "strang_splitting_adjoint_",
     "get_adjoint", "solve_poisson_adjoint", "AdjointResult", "Ensemble",
-    "ensemble_strang_splitting_",
+    "ensemble_strang_splitting_", "ensemble_strang_splitting_adjoint_",
 ]
 
 
@@ -497,6 +497,27 @@ def ensemble_strang_splitting_(ens: Ensemble, dt, number_steps: int):
     for h in ens.splittings:
         if h.sync:
             h.to_host()
+
+
+def ensemble_strang_splitting_adjoint_(ens: Ensemble, dt, number_steps: int,
+                                       checkpoint_every: int = 50, loss_weight=None,
+                                       fetch: bool = True):
+    """strang_splitting_adjoint_ for every splitting of ens, batched (no
+    cotangents of the final state; loss_weight defaults to each dt). Returns
+    one AdjointResult per splitting (or None with fetch=False)."""
+    m = len(ens.splittings)
+    dts = np.broadcast_to(np.asarray(dt, dtype=np.float64), (m,)).copy()
+    wl = dts.copy() if loss_weight is None else \
+        np.broadcast_to(np.asarray(loss_weight, dtype=np.float64), (m,)).copy()
+    for h in ens.splittings:
+        if h.sync:
+            h.to_device()
+    check(_lib.load().gempic_ensemble_strang_splitting_adjoint(
+        ens._e, ptr(dts), int(number_steps), int(checkpoint_every), ptr(wl), 0))
+    for h in ens.splittings:
+        if h.sync:
+            h.to_host()
+    return [get_adjoint(h) for h in ens.splittings] if fetch else None
 
 
 class AdjointResult:

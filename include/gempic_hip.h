@@ -122,6 +122,15 @@ int gempic_ensemble_create(int n_instances, gempic_splitting *const *handles, ge
 int gempic_ensemble_destroy(gempic_ensemble *e);
 /* dt: one time step per splitting (n_instances doubles) */
 int gempic_ensemble_strang_splitting(gempic_ensemble *e, const double *dt, int64_t n_steps, int flags);
+/* The reverse mode of every splitting of the ensemble (gempic_strang_splitting_adjoint
+ * per splitting, batched: one launch per pass, re-run and field part for all
+ * of them), with loss loss_weight[i] * sum_s 1/2 l2norm_squared(B_s, p-1)
+ * and no cotangents of the final state. Each splitting's results equal those
+ * of its own gempic_strang_splitting_adjoint bit for bit; gempic_get_adjoint
+ * and gempic_get_adjoint_domain read them per splitting. */
+int gempic_ensemble_strang_splitting_adjoint(gempic_ensemble *e, const double *dt, int64_t n_steps,
+                                             int64_t checkpoint_every, const double *loss_weight,
+                                             int flags);
 
 /* Reverse mode of strang_splitting! (1d2v). The reference has no
  * derivatives; this is the pullback a ChainRulesCore.rrule of

@@ -85,3 +85,33 @@ def test_gpu_ensemble_rejects_mixed_handles():
         G.Ensemble([h, other])
     with pytest.raises(Exception):
         G.Ensemble([h, h])
+
+
+@pytest.mark.parametrize("K,sort_every", [(3, 2), (1, 3)])
+def test_gpu_ensemble_adjoint_equals_standalone(K, sort_every):
+    """The batched reverse mode of an ensemble (instances of different N, nx,
+    k and dt) gives every instance the loss, cotangents and domain
+    derivatives of its own strang_splitting_adjoint_, bit for bit, across
+    particle sorts."""
+    specs = [(20000, 16, 1.0, 1), (8192, 64, 1.5, 3), (65536, 32, 1.25, 2)]
+    steps = 8
+    alone = []
+    for (n, nx, k, seed) in specs:
+        h, pg, w = _instance(n, nx, k, seed)
+        h.set_sort_interval(sort_every)
+        r = G.strang_splitting_adjoint_(h, _dt(nx), steps, checkpoint_every=K)
+        alone.append((r, pg.array.copy()))
+    hs = []
+    for (n, nx, k, seed) in specs:
+        h, pg, w = _instance(n, nx, k, seed)
+        h.set_sort_interval(sort_every)
+        hs.append((h, pg))
+    ens = G.Ensemble([h for h, _ in hs])
+    res = G.ensemble_strang_splitting_adjoint_(ens, [_dt(nx) for (_, nx, _, _) in specs], steps,
+                                               checkpoint_every=K)
+    for (h, pg), r, (ra, parts) in zip(hs, res, alone):
+        np.testing.assert_array_equal(pg.array, parts)
+        assert r.loss == ra.loss
+        for a in ("g_particles", "g_e1", "g_e2", "g_b"):
+            np.testing.assert_array_equal(getattr(r, a), getattr(ra, a))
+        assert r.g_length == ra.g_length and r.g_weight_sum == ra.g_weight_sum
