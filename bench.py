@@ -303,7 +303,7 @@ def adjoint_run(G, _lib, n, steps, every, cpu_n, cpu_steps):
     return out
 
 
-def ensemble_run(G, _lib, n_inst, steps, seq_steps, seed=5):
+def ensemble_run(G, _lib, n_inst, steps, seq_steps, adj_steps=100, adj_every=10, seed=5):
     """Independent 1D2V Weibel instances: N uniform in [65536, 1048576],
     nx in {16, 32, 64}, k uniform in [1.0, 1.5], alpha and beta log-uniform
     in [1e-5, 1e-3] (seeded). Particles come from sample!(pg, alpha, k,
@@ -311,7 +311,10 @@ def ensemble_run(G, _lib, n_inst, steps, seq_steps, seed=5):
     from Poisson, B0 = l2projection!(beta cos(kx)). dt = min(0.05, dx/2):
     the degree-3 FEM Maxwell solver is unstable for dt > 0.64 dx, which
     0.05 exceeds on the finest grids. Times `steps` batched Strang steps of
-    the whole ensemble against `seq_steps` steps run instance by instance."""
+    the whole ensemble against `seq_steps` steps run instance by instance,
+    then `adj_steps` steps forward against forward + reverse mode of every
+    instance (checkpoints every `adj_every` steps: about 85 GB for 146M
+    particles x 100 steps)."""
     import math
     from gempic_amd import workloads
     L_ = _lib.load()
@@ -349,13 +352,28 @@ def ensemble_run(G, _lib, n_inst, steps, seq_steps, seed=5):
     for h in hs:
         _lib.check(L_.gempic_synchronize(h._h))
     t_seq = time.perf_counter() - t0
+    adj = None
+    if adj_steps:
+        # forward and forward + reverse mode of every instance (loss: dt times the
+        # magnetic energy summed over the steps), same number of steps
+        t0 = time.perf_counter()
+        G.ensemble_strang_splitting_(ens, dts, adj_steps)
+        t_f = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        G.ensemble_strang_splitting_adjoint_(ens, dts, adj_steps, checkpoint_every=adj_every,
+                                             fetch=False)
+        t_a = time.perf_counter() - t0
+        adj = {"steps": adj_steps, "checkpoint_every": adj_every,
+               "fwd_particle_steps_per_s": total * adj_steps / t_f,
+               "fwd_adj_particle_steps_per_s": total * adj_steps / t_a,
+               "fwd_adj_over_fwd": t_a / t_f}
     return {"workload": f"{n_inst} independent 1D2V Weibel instances, {total} particles in all, "
                         f"N in [65536, 1048576], nx in {{16, 32, 64}}, k in [1, 1.5], "
                         f"dt = min(0.05, dx/2)",
             "instances": n_inst, "particles": total, "steps": steps,
             "particle_steps_per_s": total * steps / t_ens,
             "one_by_one_particle_steps_per_s": total * seq_steps / t_seq,
-            "one_by_one_steps": seq_steps}
+            "one_by_one_steps": seq_steps, "adjoint": adj}
 
 
 def copy_calibration(mib):

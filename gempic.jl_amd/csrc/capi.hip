@@ -934,13 +934,13 @@ int gempic_strang_splitting_adjoint(gempic_splitting *h, double dt, int64_t n_st
     // the sort interval has passed), so each interval runs in one particle
     // order. Each sort starts an epoch whose weights and order (id) are
     // kept, to run the interval's reverse passes and to carry the particle
-    // cotangents across the sort on the way back. With one step per
-    // interval the fused pass itself stores the state after it into the next
-    // checkpoint (no copies, and no re-run on the way back).
+    // cotangents across the sort on the way back. The fused pass that ends an
+    // interval stores its state into the next checkpoint itself (a copy is
+    // made only after a sort); with one step per interval nothing is re-run on
+    // the way back.
     const int64_t nck = (T + K - 1) / K;
     std::vector<int64_t> epoch_of((size_t)nck, 0);
     int64_t ne = 0;
-    const bool dual = K == 1;
     for (int64_t s = 1; s <= T && !rc; ++s) {
         if ((s - 1) % K == 0) {
             const size_t j = (size_t)((s - 1) / K);
@@ -961,7 +961,7 @@ int gempic_strang_splitting_adjoint(gempic_splitting *h, double dt, int64_t n_st
                 ++ne;
             }
             epoch_of[j] = ne - 1;
-            if (!dual || s == 1 || sorted) {
+            if (s == 1 || sorted) {
                 double *ck = h->d_ckpt + j * 3 * n;
                 HIP_TRY(hipMemcpyAsync(ck, h->d_x, nb, hipMemcpyDeviceToDevice, h->stream));
                 HIP_TRY(hipMemcpyAsync(ck + n, h->d_v1, nb, hipMemcpyDeviceToDevice, h->stream));
@@ -969,7 +969,8 @@ int gempic_strang_splitting_adjoint(gempic_splitting *h, double dt, int64_t n_st
             }
         }
         h->cur_hist = rec(s);
-        h->cur_cx = dual && s < T ? h->d_ckpt + (size_t)s * 3 * n : nullptr;
+        // the pass that ends an interval stores its state as the next checkpoint
+        h->cur_cx = (s % K == 0 && s < T) ? h->d_ckpt + (size_t)(s / K) * 3 * n : nullptr;
         const int mode = (s > 1 ? M_HE_PREV : 0) | M_HE | M_HP2A | M_HP1 | M_HP2B;
         rc = run_pass(h, mode, hdt, hdt, 1.0 * dt, s < T, 0);
         h->steps_since_sort++;
@@ -1542,7 +1543,6 @@ int gempic_ensemble_strang_splitting_adjoint(gempic_ensemble *e, const double *d
     const int ni = (int)e->hs.size();
     const int64_t T = n_steps, K = std::min<int64_t>(checkpoint_every, n_steps);
     const int64_t nck = (T + K - 1) / K;
-    const bool dual = K == 1;
     int rc = GEMPIC_OK;
     for (auto *h : e->hs) {
         rc = adj_workspace(h, T, K);
@@ -1585,7 +1585,8 @@ int gempic_ensemble_strang_splitting_adjoint(gempic_ensemble *e, const double *d
             const int target = h->override_blocks ? h->override_blocks : h->dep_blocks;
             const Launch L = plan(h->n, target);
             h->cur_hist = trailing ? nullptr : rec(h, s);
-            h->cur_cx = (!trailing && dual && s < T) ? h->d_ckpt + (size_t)s * 3 * h->n : nullptr;
+            h->cur_cx = (!trailing && s % K == 0 && s < T) ? h->d_ckpt + (size_t)(s / K) * 3 * h->n
+                                                            : nullptr;
             PassArgs a = pass_args(h);
             const double hdt = 0.5 * dt[i];
             a.chunk = L.chunk;
@@ -1646,7 +1647,7 @@ int gempic_ensemble_strang_splitting_adjoint(gempic_ensemble *e, const double *d
                     ++st[i].ne;
                 }
                 st[i].epoch_of[j] = st[i].ne - 1;
-                if (!dual || s == 1 || sorted[i]) {
+                if (s == 1 || sorted[i]) {
                     double *ck = h->d_ckpt + j * 3 * n;
                     HIP_TRY(hipMemcpyAsync(ck, h->d_x, nb, hipMemcpyDeviceToDevice, e->stream));
                     HIP_TRY(hipMemcpyAsync(ck + n, h->d_v1, nb, hipMemcpyDeviceToDevice, e->stream));
