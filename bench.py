@@ -400,7 +400,10 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--particles", type=int, default=4194304)
-    ap.add_argument("--sort-interval", type=int, default=20)
+    # sort interval of the fused-path run: 40 steps measured fastest for this
+    # workload (1000 steps: 4.07e10 at 10, 4.39e10 at 20, 4.52e10 at 30,
+    # 4.59e10 at 40, 4.55e10 at 60); the library's default is 20
+    ap.add_argument("--sort-interval", type=int, default=40)
     ap.add_argument("--cpu-particles", type=int, default=262144)
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
