@@ -797,6 +797,7 @@ struct or_split {
     double *e1, *e2, *b;     /* aliased field arrays */
     double *j1, *j2;         /* own j_dofs[1:2] */
     double *e1m, *e2m, *bm;  /* HamiltonianSplittingBoris: e_dofs_mid, b_dofs_mid */
+    double adj_g_length, adj_g_wsum;   /* the last or_strang_adjoint's domain derivatives */
     or_maxwell1d *m;
     or_pmc1d ks0, ks1;
     double Lx, x_min, delta_x;
@@ -1324,15 +1325,26 @@ static void adj_b_from_e(double *le, double coef, const double *lb, int64_t n)
 typedef struct {
     double *lx, *lv1, *lv2;      /* particle cotangents, one per particle */
     double *le1, *le2, *lb;      /* field cotangents */
+    /* explicit derivatives of the loss, state held fixed: in dx (through
+     * xi = (x - xmin)/dx and the dt/dx, 1/dx factors), in Lx through the mod
+     * of Hp1, and the sum over particles of those in the weights */
+    double g_dx, g_mod, g_w;
 } or_adj_state;
 
+/* (x - xmin) / dx of a position */
+static double xi_of(const or_split *h, double x) { return (x - h->x_min) / h->delta_x; }
+
 /* operatorHB 1d2v: E2 += dt/dx W(B) */
-static void adj_hb(or_split *h, double dt, or_adj_state *A)
+static void adj_hb(or_split *h, double dt, const double *b, or_adj_state *A)
 {
     int64_t n = h->m->n;
     double *y = (double *)malloc((size_t)n * sizeof(double));
     solve_circulant_t(h->m, h->m->eig_weak_ampere, A->le2, y);
     double coef = dt / h->m->delta_x;
+    /* d/d dx of E2 += (dt/dx) W B: -(1/dx) lambda_E2 . (dt/dx) W B = -(1/dx) coef (W^T lambda_E2) . B */
+    double r = 0.0;
+    for (int64_t i = 0; i < n; ++i) r += y[i] * b[i];
+    A->g_dx += -(coef / h->m->delta_x) * r;
     for (int64_t i = 0; i < n; ++i) A->lb[i] += coef * y[i];
     free(y);
 }
@@ -1343,13 +1355,22 @@ static void adj_he(or_split *h, double dt, const double *S, const double *e1, co
 {
     double qm = h->q_over_m;
     int ld = h->ld;
+    int64_t n = h->m->n;
+    {
+        /* d/d dx of B += (dt/dx) D E2: -(1/dx) lambda_B . (dt/dx) D E2 */
+        double coef = dt / h->m->delta_x, r = 0.0;
+        for (int64_t j = 0; j < n; ++j) r += A->lb[j] * (e2[j == 0 ? n - 1 : j - 1] - e2[j]);
+        A->g_dx += -(coef / h->m->delta_x) * r;
+    }
     adj_b_from_e(A->le2, dt / h->m->delta_x, A->lb, h->m->n);
     for (int64_t i = 0; i < h->np; ++i) {
         double x = S[i * ld];
         double f1, d1, f2, d2;
         eval_d(&h->ks1, x, e1, &f1, &d1);
         eval_d(&h->ks0, x, e2, &f2, &d2);
-        A->lx[i] += dt * qm * (A->lv1[i] * d1 + A->lv2[i] * d2);
+        double dlx = dt * qm * (A->lv1[i] * d1 + A->lv2[i] * d2);
+        A->g_dx += -xi_of(h, x) * dlx;          /* d f(x)/d dx = -xi f'(x) */
+        A->lx[i] += dlx;
         scatter(&h->ks1, x, dt * qm * A->lv1[i], A->le1);
         scatter(&h->ks0, x, dt * qm * A->lv2[i], A->le2);
     }
@@ -1376,7 +1397,11 @@ static void adj_hp2(or_split *h, double dt, const double *S, const double *b, or
         eval_d(&h->ks0, x, mu, &g, &dg);
         double lv1 = A->lv1[i];
         A->lv2[i] += dt * qm * bv * lv1 + q * h->ks0.scaling * g;
-        A->lx[i] += dt * qm * v2 * lv1 * db + q * h->ks0.scaling * v2 * dg;
+        double dlx = dt * qm * v2 * lv1 * db + q * h->ks0.scaling * v2 * dg;
+        A->lx[i] += dlx;
+        /* mu carries the solve's 1/dx: d/d dx of -(dt/dx) M0^-1 J adds -(1/dx) mu . J */
+        A->g_dx += -xi_of(h, x) * dlx - q * h->ks0.scaling * v2 * g / m->delta_x;
+        A->g_w += h->charge * h->common_weight * h->ks0.scaling * v2 * g;
         scatter(&h->ks1, x, dt * qm * v2 * lv1, A->lb);
     }
     free(mu);
@@ -1409,9 +1434,24 @@ static void adj_hp1(or_split *h, double dt, const double *S, const double *b, or
         eval_d(&h->ks1, x_new, mu, &gn, &dgn);
         eval_d(&h->ks1, x_old, mu, &go, &dgo);
         double lv2 = A->lv2[i];
+        double lx1 = A->lx[i];
         double lxn = A->lx[i] - qm * lv2 * bn + q * h->ks1.scaling * gn;
         A->lx[i] = lxn + qm * lv2 * bo - q * h->ks1.scaling * go;
         A->lv1[i] += dt * lxn;
+        {
+            /* int_x^xn f for f = B and f = mu1 (the same trunc-cell pieces):
+             * d/d dx = (1/dx) int f - xi_n f(xn) + xi f(x), by parts */
+            double *tmp = (double *)calloc((size_t)n, sizeof(double));
+            double ib = -or_add_current_update_v(tmp, &p1, x_old, x_new, 0.0, 1.0, b, 0.0);
+            double im = -or_add_current_update_v(tmp, &p1, x_old, x_new, 0.0, 1.0, mu, 0.0);
+            free(tmp);
+            double xo = xi_of(h, x_old), xn = xi_of(h, x_new);
+            A->g_dx += -qm * lv2 * (ib / m->delta_x - xn * bn + xo * bo) +
+                       q * h->ks1.scaling * (xo * go - xn * gn);
+            A->g_w += h->charge * h->common_weight * h->ks1.scaling * im;
+            double xm = or_jl_mod_f64(x_new, h->Lx);
+            if (xm != x_new) A->g_mod += lx1 * (xm > x_new ? 1.0 : -1.0);
+        }
         /* lambda_B[k] += -qm lv2 int S1_k: the same line integral, unscaled */
         or_add_current_update_v(A->lb, &p1, x_old, x_new, -qm * lv2, qm, NULL, 0.0);
     }
@@ -1481,6 +1521,9 @@ int or_strang_adjoint(or_split *h, double dt, int64_t nsteps, double wloss, cons
     A.le1 = (double *)calloc((size_t)n, sizeof(double));
     A.le2 = (double *)calloc((size_t)n, sizeof(double));
     A.lb = (double *)calloc((size_t)n, sizeof(double));
+    A.g_dx = 0.0;
+    A.g_mod = 0.0;
+    A.g_w = 0.0;
     for (int64_t i = 0; ct_part && i < np; ++i) {
         A.lx[i] = ct_part[i * ld];
         A.lv1[i] = ct_part[1 + i * ld];
@@ -1492,12 +1535,12 @@ int or_strang_adjoint(or_split *h, double dt, int64_t nsteps, double wloss, cons
         if (ct_b) A.lb[k] = ct_b[k];
     }
     for (int64_t s = nsteps - 1; s >= 0; --s) {
-        (void)loss_b(h, wloss, bend + (size_t)s * n, A.lb);
+        A.g_dx += loss_b(h, wloss, bend + (size_t)s * n, A.lb) / h->m->delta_x;
         for (int k = nsub - 1; k >= 0; --k) {
             const double *rec = hist + ((size_t)s * nsub + k) * (psz + fsz);
             const double *e1 = rec + psz, *e2 = rec + psz + n, *b = rec + psz + 2 * n;
             switch (k) {
-            case 0: case 6: adj_hb(h, hdt, &A); break;
+            case 0: case 6: adj_hb(h, hdt, b, &A); break;
             case 1: case 5: adj_he(h, hdt, rec, e1, e2, &A); break;
             case 2: case 4: adj_hp2(h, hdt, rec, b, &A); break;
             default: adj_hp1(h, dt, rec, b, &A); break;
@@ -1505,6 +1548,9 @@ int or_strang_adjoint(or_split *h, double dt, int64_t nsteps, double wloss, cons
         }
     }
     if (loss) *loss = L;
+    /* dx = Lx / nx: d/d Lx = (1/nx) d/d dx, plus the periods of the mod */
+    h->adj_g_length = A.g_dx / (double)n + A.g_mod;
+    h->adj_g_wsum = A.g_w;
     for (int64_t i = 0; g_part && i < np; ++i) {
         for (int r = 0; r < ld; ++r) g_part[r + i * ld] = 0.0;
         g_part[i * ld] = A.lx[i];
@@ -1540,4 +1586,13 @@ void or_solve_poisson_adjoint(or_split *h, const double *ct_e, double *g_x)
     }
     free(dt_);
     free(nu);
+}
+
+/* The explicit derivatives of the last or_strang_adjoint's loss in the domain
+ * length (state held fixed; through dx = Lx/nx and the mod) and the sum of
+ * its derivatives in the particle weights. */
+void or_strang_adjoint_domain(const or_split *h, double *g_length, double *g_wsum)
+{
+    if (g_length) *g_length = h->adj_g_length;
+    if (g_wsum) *g_wsum = h->adj_g_wsum;
 }

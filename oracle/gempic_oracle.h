@@ -28,9 +28,10 @@
  *   - write_step: energy conservation and the diagnostics as rates of change
  *     of the energies and momenta under HE, Hp2 and HB; ErrorPoisson by
  *     Gauss's law;
- *   - reverse mode (strang_adjoint, solve_poisson_adjoint, bspline_deriv):
- *     central differences of the forward functions above along random
- *     directions of every input (tests/test_oracle_adjoint.py).
+ *   - reverse mode (strang_adjoint, solve_poisson_adjoint, bspline_deriv,
+ *     strang_adjoint_domain): central differences of the forward functions
+ *     above along random directions of every input, and in the wavenumber k
+ *     that moves the domain (tests/test_oracle_adjoint.py).
  * No function here is left unpinned.
  *
  * Layout conventions (same as the reference):
@@ -228,6 +229,7 @@ int or_strang_adjoint(or_split *h, double dt, int64_t nsteps, double wloss,
                       const double *ct_b, double *loss, double *g_part, double *g_e1,
                       double *g_e2, double *g_b);
 void or_solve_poisson_adjoint(or_split *h, const double *ct_e, double *g_x);
+void or_strang_adjoint_domain(const or_split *h, double *g_length, double *g_wsum);
 
 #ifdef __cplusplus
 }

@@ -103,6 +103,7 @@ def lib():
             "or_strang_loss": (_D, [_VP, _D, _I64, _D]),
             "or_strang_adjoint": (_I, [_VP, _D, _I64, _D, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
             "or_solve_poisson_adjoint": (None, [_VP, _P, _P]),
+            "or_strang_adjoint_domain": (None, [_VP, _P, _P]),
             "or_solve_poisson": (None, [_P, _VP, _P]),
             "or_write_step": (None, [_VP, _D, _I, _P, _P, _P, _P, _P, _P, _P]),
             "or_pmc2d_init": (_I, [_VP, _D, _D, _I64, _D, _D, _I64, _I, _I]),
@@ -387,6 +388,13 @@ class Splitting:
         if rc:
             raise ValueError(f"or_strang_adjoint: {rc}")
         return float(loss[0]), g, ge1, ge2, gb
+
+    def strang_adjoint_domain(self):
+        """(d loss / d Lx with the state held fixed, sum_i d loss / d w_i) of
+        the last strang_adjoint."""
+        gl, gw = np.zeros(1), np.zeros(1)
+        lib().or_strang_adjoint_domain(self.h, _p(gl), _p(gw))
+        return float(gl[0]), float(gw[0])
 
     def solve_poisson_adjoint(self, ct_e):
         g = np.zeros(self.particles.shape[1])

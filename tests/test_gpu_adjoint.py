@@ -78,6 +78,10 @@ def test_gpu_adjoint_matches_oracle(deg, K):
     _close(r.g_e1, ge1)
     _close(r.g_e2, ge2)
     _close(r.g_b, gb)
+    # the explicit derivatives in the domain length and the weights
+    gl, gw = osp.strang_adjoint_domain()
+    assert abs(r.g_length - gl) <= REL * abs(gl), (r.g_length, gl)
+    assert abs(r.g_weight_sum - gw) <= REL * abs(gw), (r.g_weight_sum, gw)
 
 
 def test_gpu_adjoint_loss_only_and_deterministic():

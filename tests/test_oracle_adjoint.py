@@ -129,3 +129,37 @@ def test_solve_poisson_adjoint():
         return ct @ e
     fd = (f(1) - f(-1)) / (2 * H)
     assert abs(fd - g @ d) <= TOL * abs(fd)
+
+
+def _k_loss(k, n, steps, with_grad):
+    import sys, os
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..", "gempic.jl_amd"))
+    from gempic_amd import workloads
+    w = workloads.weibel_adjoint_1d2v(n, seed=2, nx=16, k=k, alpha=1e-2, beta=1e-2,
+                                      sigma1=0.2)
+    nx = w["nx"]
+    parts = np.asfortranarray(w["particles"].copy())
+    e1, e2, b = np.zeros(nx), np.zeros(nx), w["b0"].copy()
+    sp = orc.Splitting(nx, 0.0, w["xmax"], 3, 3, 2, parts, e1, e2, b,
+                       common_weight=w["common_weight"])
+    sp.solve_poisson(e1, np.zeros(nx))
+    e1_0 = e1.copy()
+    if not with_grad:
+        return sp.strang_loss(0.1, steps, 0.1)
+    J, gp, ge1, ge2, gb = sp.strang_adjoint(0.1, steps, 0.1)
+    g_len, g_w = sp.strang_adjoint_domain()
+    L = w["xmax"]
+    dj_dl = g_len + g_w + float(gp[0] @ w["particles"][0]) / L + float(ge1 @ e1_0) / L
+    return J, -(L / k) * dj_dl
+
+
+def test_adjoint_wavenumber_gradient():
+    """dJ/dk, where k moves Lx = 2 pi / k and with it dx, the quiet-start
+    positions (x = Lx xi), the weights (w = Lx) and E1 (Poisson, ~ Lx): the
+    oracle's explicit domain derivatives plus the chain rule through the
+    initial state, against central differences in k."""
+    k, n, steps = 1.25, 600, 6
+    J, dj_dk = _k_loss(k, n, steps, True)
+    hk = 1e-6 * k
+    fd = (_k_loss(k + hk, n, steps, False) - _k_loss(k - hk, n, steps, False)) / (2 * hk)
+    assert abs(fd - dj_dk) <= 1e-6 * abs(fd), (fd, dj_dk)
