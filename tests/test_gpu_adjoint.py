@@ -240,3 +240,27 @@ def test_gpu_adjoint_wavenumber_gradient_vs_central_differences():
     hk = 1e-6 * k
     fd = (_k_run(k + hk, 200000, 40, 10)[0].loss - _k_run(k - hk, 200000, 40, 10)[0].loss) / (2 * hk)
     assert abs(fd - dj_dk) <= 1e-6 * abs(fd), (fd, dj_dk)
+
+
+@pytest.mark.parametrize("nx", [4, 128])
+def test_gpu_adjoint_grid_limits(nx):
+    """The reverse mode at the smallest (4) and largest (128) grids it
+    supports, against the oracle; 2 and 130 cells are rejected."""
+    parts, e1, e2, b, L, _ = _hot(6000, nx=nx, seed=12)
+    dt, steps, wl, cw = 0.05, 4, 0.05, 0.01
+    h, pg = _gpu(parts, e1, e2, b, L, nx, (3, 2), cw)
+    r = G.strang_splitting_adjoint_(h, dt, steps, checkpoint_every=2, loss_weight=wl)
+    o_parts = parts.copy(order="F")
+    osp = orc.Splitting(nx, 0.0, L, 3, 3, 2, o_parts, e1.copy(), e2.copy(), b.copy(),
+                        common_weight=cw)
+    J, gp, ge1, ge2, gb = osp.strang_adjoint(dt, steps, wl)
+    assert abs(r.loss - J) <= 1e-12 * abs(J)
+    for row in range(3):
+        _close(r.g_particles[row], gp[row])
+    _close(r.g_e2, ge2)
+    _close(r.g_b, gb)
+    for bad in (2, 130):
+        pb, e1b, e2b, bb, Lb, _ = _hot(500, nx=bad, seed=1)
+        hb, _ = _gpu(pb, e1b, e2b, bb, Lb, bad, (3, 2), cw)
+        with pytest.raises(ValueError):
+            G.strang_splitting_adjoint_(hb, dt, 2)
