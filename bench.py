@@ -405,7 +405,7 @@ def main():
     # 4.59e10 at 40, 4.55e10 at 60); the library's default is 20
     ap.add_argument("--sort-interval", type=int, default=40)
     ap.add_argument("--cpu-particles", type=int, default=262144)
-    ap.add_argument("--cpu-steps", type=int, default=100)
+    ap.add_argument("--cpu-steps", type=int, default=300)      # about 10 s on one core
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--substep", action="store_true", help="also time the sub-step path")
     ap.add_argument("--blocks", type=int, default=0, help="workgroups per particle pass (0 = auto)")
