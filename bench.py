@@ -432,9 +432,9 @@ def main():
         copy_calibration(args.copy_calibration)
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
+        print("bench.py measures one GPU: the library has no multi-GPU path "
               "(particles would shard with one all-reduce of the nx-sized deposits per "
-              "sub-step; see the design notes). Run with --gpus 1.", file=sys.stderr)
+              "step). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     if args.only_2d:
