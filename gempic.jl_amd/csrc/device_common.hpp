@@ -276,4 +276,94 @@ __device__ __forceinline__ void cache_drain(Cache2<WN> &C, double *row, int nx)
     C.reset();
 }
 
+// Two rows cached together: entries of 2*WN values, the first WN for row0's
+// nodes first_node(cell, WN-1) + w and the next WN for row1's same nodes
+// (two deposits that always land in the same cell). Same caching as
+// Cache2/cache_add; a flush writes both rows.
+template <int WN>
+__device__ __forceinline__ void seg_flush_rows2(bool want, int cell, const double (&v)[2 * WN],
+                                                double *row0, double *row1, int nx)
+{
+    unsigned long long pending = __ballot(want);
+    const int lane = lane_id();
+    while (pending) {
+        int leader = __ffsll((long long)pending) - 1;
+        int c = __shfl(cell, leader, kWave);
+        bool mine = want && (cell == c);
+        unsigned long long m = __ballot(mine);
+        const int k0 = first_node(c, WN - 1, nx);
+        int k = k0;
+#pragma unroll
+        for (int w = 0; w < WN; ++w) {
+            double s = wave_sum_lane63(mine ? v[w] : 0.0);
+            if (lane == kWave - 1) row0[k] += s;
+            k = next_node(k, nx);
+        }
+        k = k0;
+#pragma unroll
+        for (int w = 0; w < WN; ++w) {
+            double s = wave_sum_lane63(mine ? v[WN + w] : 0.0);
+            if (lane == kWave - 1) row1[k] += s;
+            k = next_node(k, nx);
+        }
+        pending &= ~m;
+    }
+}
+
+template <int WN>
+__device__ __forceinline__ void cache_add_rows2(Cache2<2 * WN> &C, bool valid, int cell,
+                                                const double (&v)[2 * WN], double *row0,
+                                                double *row1, int nx)
+{
+    constexpr int W2 = 2 * WN;
+    bool hit_a = valid && cell == C.ca;
+    bool hit_b = valid && !hit_a && cell == C.cb;
+    bool miss = valid && !hit_a && !hit_b;
+    if (__ballot(miss)) {
+        bool use_a = miss && (C.ca < 0 || (C.cb >= 0 && C.mru_b));
+        bool use_b = miss && !use_a;
+        bool ev_a = use_a && C.ca >= 0;
+        bool ev_b = use_b && C.cb >= 0;
+        int ecell = ev_a ? C.ca : C.cb;
+        double ev[W2];
+#pragma unroll
+        for (int w = 0; w < W2; ++w) ev[w] = ev_a ? C.a[w] : C.bs[w * kWave];
+        seg_flush_rows2<WN>(ev_a || ev_b, ecell, ev, row0, row1, nx);
+        if (use_a) {
+            C.ca = cell;
+#pragma unroll
+            for (int w = 0; w < W2; ++w) C.a[w] = 0.0;
+            hit_a = true;
+        }
+        if (use_b) {
+            C.cb = cell;
+#pragma unroll
+            for (int w = 0; w < W2; ++w) C.bs[w * kWave] = 0.0;
+            hit_b = true;
+        }
+    }
+    if (hit_a) {
+#pragma unroll
+        for (int w = 0; w < W2; ++w) C.a[w] += v[w];
+        C.mru_b = false;
+    }
+    if (hit_b) {
+#pragma unroll
+        for (int w = 0; w < W2; ++w) C.bs[w * kWave] += v[w];
+        C.mru_b = true;
+    }
+}
+
+template <int WN>
+__device__ __forceinline__ void cache_drain_rows2(Cache2<2 * WN> &C, double *row0, double *row1,
+                                                  int nx)
+{
+    seg_flush_rows2<WN>(C.ca >= 0, C.ca, C.a, row0, row1, nx);
+    double bv[2 * WN];
+#pragma unroll
+    for (int w = 0; w < 2 * WN; ++w) bv[w] = C.bs[w * kWave];
+    seg_flush_rows2<WN>(C.cb >= 0, C.cb, bv, row0, row1, nx);
+    C.reset();
+}
+
 }  // namespace gk

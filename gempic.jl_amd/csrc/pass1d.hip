@@ -2314,13 +2314,20 @@ __device__ __forceinline__ void adj1d_body(const AdjArgs &A, int bid, double *sm
     __syncthreads();
     double *myrow = rows + wave * NDEP * nx;
 
-    Cache2<W1> c_e1, c_b;
+    // HEONLY: lambda_E1 and lambda_E2 caches; REV: lambda_E1 and lambda_B
+    // cached together (both are S1-basis deposits, mostly in the old cell),
+    // and lambda_E2
+    Cache2<W1> c_e1;
+    Cache2<2 * W1> c_eb;
     Cache2<W0> c_e2;
-    if constexpr (DEP) {
+    if constexpr (HEONLY) {
         c_e1.init(slots);
         c_e2.init(slots + W1 * kWave);
     }
-    if constexpr (REV) c_b.init(slots + (W1 + W0) * kWave);
+    if constexpr (REV) {
+        c_eb.init(slots);
+        c_e2.init(slots + 2 * W1 * kWave);
+    }
 
     const double xmin = a.xmin, dx = a.dx, qm = a.qm;
     const double dtqm = a.dtqm_he, g = a.dtqm_hp2, dt1 = a.dt_hp1;
@@ -2532,17 +2539,19 @@ __device__ __forceinline__ void adj1d_body(const AdjArgs &A, int bid, double *sm
                     const bool here = kc == key;
                     const double *bk = sB + (kc - P1);
                     const double *mk = sM1 + (kc - P1);
-                    double vb[W1];
+                    double vb[2 * W1];
 #pragma unroll
                     for (int m = 0; m < W1; ++m) {
-                        vb[m] = wl * sv[m];
-                        if (act && here) vkey[m] = vkey[m] + vb[m];
+                        vb[m] = 0.0;
+                        vb[W1 + m] = wl * sv[m];
+                        if (act && here) vkey[m] = vkey[m] + vb[W1 + m];
                         if (act) {
                             ib = ib + sv[m] * bk[m];
                             im1 = im1 + sv[m] * mk[m];
                         }
                     }
-                    cache_add<W1>(c_b, act && !here, kc, vb, myrow + AR_B * nx, nx);
+                    cache_add_rows2<W1>(c_eb, act && !here, kc, vb, myrow + AR_E1 * nx,
+                                        myrow + AR_B * nx, nx);
                 }
                 if (valid) {
                     // d/d dx of int_x^xn f = (1/dx) int f - xin f(xn) + xi f(x)
@@ -2579,12 +2588,23 @@ __device__ __forceinline__ void adj1d_body(const AdjArgs &A, int bid, double *sm
 #pragma unroll
                 for (int k = 0; k < W1; ++k) vkey[k] = vkey[k] + gl1 * b1[k];
             }
-            cache_add<W1>(c_b, valid, key, vkey, myrow + AR_B * nx, nx);
             {
-                double vb[W1];
+                // lambda_E1 of the two HE (the same cotangent and cell) with lambda_B
+                double veb[2 * W1];
 #pragma unroll
-                for (int k = 0; k < W1; ++k) vb[k] = gl2 * c1[k];
-                cache_add<W1>(c_b, valid && !new_here, key2, vb, myrow + AR_B * nx, nx);
+                for (int k = 0; k < W1; ++k) {
+                    veb[k] = (dtqm * lv1) * b1[k];
+                    veb[W1 + k] = vkey[k];
+                }
+                cache_add_rows2<W1>(c_eb, valid, key, veb, myrow + AR_E1 * nx, myrow + AR_B * nx, nx);
+                double vb[2 * W1];
+#pragma unroll
+                for (int k = 0; k < W1; ++k) {
+                    vb[k] = 0.0;
+                    vb[W1 + k] = gl2 * c1[k];
+                }
+                cache_add_rows2<W1>(c_eb, valid && !new_here, key2, vb, myrow + AR_E1 * nx,
+                                    myrow + AR_B * nx, nx);
             }
             // HE of this step and of the previous one: same x and cotangents
             {
@@ -2600,12 +2620,9 @@ __device__ __forceinline__ void adj1d_body(const AdjArgs &A, int bid, double *sm
                     dhs = dhs + dhp;
                 }
                 if (valid) acc_dx = acc_dx - xi * dhs;
-                double ve[W1], vf[W0];
-#pragma unroll
-                for (int k = 0; k < W1; ++k) ve[k] = (dtqm * lv1) * b1[k];
+                double vf[W0];
 #pragma unroll
                 for (int k = 0; k < W0; ++k) vf[k] = (dtqm * lv2) * b0[k];
-                cache_add<W1>(c_e1, valid, key, ve, myrow + AR_E1 * nx, nx);
                 cache_add<W0>(c_e2, valid, key, vf, myrow + AR_E2 * nx, nx);
             }
             if (valid) {
@@ -2627,9 +2644,9 @@ __device__ __forceinline__ void adj1d_body(const AdjArgs &A, int bid, double *sm
                 myrow[AR_SCAL * nx + 2] = sw;
             }
         }
-        cache_drain<W1>(c_e1, myrow + AR_E1 * nx, nx);
+        if constexpr (HEONLY) cache_drain<W1>(c_e1, myrow + AR_E1 * nx, nx);
+        if constexpr (REV) cache_drain_rows2<W1>(c_eb, myrow + AR_E1 * nx, myrow + AR_B * nx, nx);
         cache_drain<W0>(c_e2, myrow + AR_E2 * nx, nx);
-        if constexpr (REV) cache_drain<W1>(c_b, myrow + AR_B * nx, nx);
         __syncthreads();
         constexpr unsigned MASK = (1u << AR_E1) | (1u << AR_E2) | (REV ? 1u << AR_B : 0u) |
                                   (1u << AR_SCAL);
