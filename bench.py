@@ -404,6 +404,9 @@ def main():
     # workload (1000 steps: 4.07e10 at 10, 4.39e10 at 20, 4.52e10 at 30,
     # 4.59e10 at 40, 4.55e10 at 60); the library's default is 20
     ap.add_argument("--sort-interval", type=int, default=40)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep --blocks and --sort-interval instead of autotuning them "
+                         "(HamiltonianSplitting.autotune on the same state, before the warmup)")
     ap.add_argument("--cpu-particles", type=int, default=262144)
     ap.add_argument("--cpu-steps", type=int, default=300)      # about 10 s on one core
     ap.add_argument("--no-cpu", action="store_true")
@@ -455,6 +458,20 @@ def main():
     _lib.check(L.gempic_set_launch(h._h, args.blocks))
     _lib.check(L.gempic_set_fusion(h._h, args.fusion))
     dt = w["dt"]
+    tune = None
+    blocks, interval = args.blocks, args.sort_interval
+    if not args.no_autotune:
+        # every pair runs 120 steps twice from the same state, which is
+        # restored after each trial; the fastest pair stays set
+        print("[bench] autotuning workgroups per pass and sort interval", file=sys.stderr,
+              flush=True)
+        t0 = time.perf_counter()
+        tune = h.autotune(dt, n_steps=120, blocks=(0, 768, 1024, 1536),
+                          sort_intervals=(20, 30, 40, 60))
+        tune["seconds"] = time.perf_counter() - t0
+        blocks, interval = tune["blocks"], tune["sort_interval"]
+        print(f"[bench] autotune: blocks {blocks}, sort interval {interval} "
+              f"({tune['seconds']:.2f} s)", file=sys.stderr, flush=True)
     print(f"[bench] warmup {args.warmup} steps", file=sys.stderr, flush=True)
     if args.warmup:
         _lib.check(L.gempic_strang_splitting(h._h, dt, args.warmup, _lib.STRANG_FUSED))
@@ -556,8 +573,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded Weibel initial condition)",
         "config": {"workload": "1D2V Weibel instability", "particles": n, "nx": w["nx"],
-                   "spline_degree": w["degree"], "dt": dt, "sort_interval": args.sort_interval,
-                   "blocks": args.blocks, "kernels_per_step": args.fusion,
+                   "spline_degree": w["degree"], "dt": dt, "sort_interval": interval,
+                   "blocks": blocks, "kernels_per_step": args.fusion,
                    "path": "fused Strang step"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
@@ -566,6 +583,8 @@ def main():
         "cpu_baseline": cpu,
         "kernels": kernels,
     }
+    if tune is not None:
+        line["autotune"] = tune
     if boris is not None:
         line["boris_1d2v"] = boris
     if adj is not None:

@@ -1878,6 +1878,121 @@ int gempic_set_launch(gempic_splitting *h, int blocks)
     return GEMPIC_OK;
 }
 
+int gempic_autotune(gempic_splitting *h, double dt, int64_t n_steps, const int *blocks,
+                    int n_blocks, const int64_t *sort_intervals, int n_intervals, int *best_blocks,
+                    int64_t *best_interval, double *ms_per_step)
+{
+    CHECK_H(h);
+    CHECK_P(h);
+    if (n_steps < 1 || n_steps > 100000)
+        return fail(GEMPIC_ERR_ARGUMENT, "autotune: n_steps must be in [1, 100000]");
+    if (!blocks || n_blocks < 1 || n_blocks > 64 || !sort_intervals || n_intervals < 1 ||
+        n_intervals > 64)
+        return fail(GEMPIC_ERR_ARGUMENT, "autotune: 1 to 64 candidates of each kind");
+    for (int i = 0; i < n_blocks; ++i)
+        if (blocks[i] < 0 || blocks[i] > h->max_blocks)
+            return fail(GEMPIC_ERR_ARGUMENT,
+                        "autotune: a blocks candidate is negative or exceeds the reduction scratch "
+                        "(8 per CU)");
+    for (int j = 0; j < n_intervals; ++j)
+        if (sort_intervals[j] < 0)
+            return fail(GEMPIC_ERR_ARGUMENT, "autotune: a sort interval candidate is negative");
+
+    // snapshot of everything a Strang call changes: particles in their device
+    // order, the field slots and the steps since the last sort
+    const size_t n = (size_t)h->n, ng = (size_t)NSLOT * (size_t)h->nx;
+    const int nv = h->dim_v == 2 ? 2 : 1;
+    double *snap = nullptr;
+    int64_t *snap_id = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    hipError_t e = dalloc(&snap, (2 + (size_t)nv) * n + ng);
+    if (e == hipSuccess) e = dalloc(&snap_id, n);
+    if (e == hipSuccess) e = hipEventCreate(&ev[0]);
+    if (e == hipSuccess) e = hipEventCreate(&ev[1]);
+    const int64_t saved_since = h->steps_since_sort;
+    const int saved_blocks = h->override_blocks;
+    const int64_t saved_interval = h->sort_interval;
+    const bool saved_profile = h->profile;
+    auto copy_state = [&](bool save) -> hipError_t {
+        double *dev[4] = {h->d_x, h->d_v1, h->d_w, h->d_v2};
+        hipError_t c = hipSuccess;
+        for (int k = 0; k < 2 + nv && c == hipSuccess; ++k) {
+            double *s = snap + (size_t)k * n;
+            c = hipMemcpyAsync(save ? s : dev[k], save ? dev[k] : s, sizeof(double) * n,
+                               hipMemcpyDeviceToDevice, h->stream);
+        }
+        double *sg = snap + (2 + (size_t)nv) * n;
+        if (c == hipSuccess)
+            c = hipMemcpyAsync(save ? sg : h->d_grid, save ? h->d_grid : sg, sizeof(double) * ng,
+                               hipMemcpyDeviceToDevice, h->stream);
+        if (c == hipSuccess)
+            c = hipMemcpyAsync(save ? snap_id : h->d_id, save ? h->d_id : snap_id,
+                               sizeof(int64_t) * n, hipMemcpyDeviceToDevice, h->stream);
+        if (c == hipSuccess) c = hipStreamSynchronize(h->stream);
+        h->steps_since_sort = saved_since;
+        return c;
+    };
+    if (e == hipSuccess) e = copy_state(true);
+
+    int rc = e == hipSuccess ? GEMPIC_OK
+                             : fail(GEMPIC_ERR_DEVICE, std::string("autotune set-up: ") +
+                                                           hipGetErrorString(e));
+    h->profile = false;
+    double best = 0;
+    int bi = -1;
+    for (int i = 0; i < n_blocks && !rc; ++i) {
+        for (int j = 0; j < n_intervals && !rc; ++j) {
+            h->override_blocks = blocks[i];
+            h->sort_interval = sort_intervals[j];
+            // two timed trials from the same state; the faster one counts
+            float t = 0;
+            for (int r = 0; r < 2 && !rc; ++r) {
+                float ms = 0;
+                e = hipEventRecord(ev[0], h->stream);
+                if (e != hipSuccess) {
+                    rc = fail(GEMPIC_ERR_DEVICE, std::string("autotune: ") + hipGetErrorString(e));
+                    break;
+                }
+                rc = gempic_strang_splitting(h, dt, n_steps, 0);
+                if (!rc) e = hipEventRecord(ev[1], h->stream);
+                if (!rc && e == hipSuccess) e = hipEventSynchronize(ev[1]);
+                if (!rc && e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
+                if (!rc && e == hipSuccess) e = copy_state(false);
+                if (!rc && e != hipSuccess)
+                    rc = fail(GEMPIC_ERR_DEVICE, std::string("autotune: ") + hipGetErrorString(e));
+                if (r == 0 || ms < t) t = ms;
+            }
+            if (rc) break;
+            const double per = (double)t / (double)n_steps;
+            if (ms_per_step) ms_per_step[(size_t)i * n_intervals + j] = per;
+            if (bi < 0 || per < best) {
+                best = per;
+                bi = i * n_intervals + j;
+            }
+        }
+    }
+    h->profile = saved_profile;
+    if (rc) {
+        // a failed trial leaves the caller's state and settings as they were
+        const std::string msg = gempic_last_error();
+        if (snap && snap_id) (void)copy_state(false);
+        h->override_blocks = saved_blocks;
+        h->sort_interval = saved_interval;
+        g_last_error = msg;
+    } else {
+        h->override_blocks = blocks[bi / n_intervals];
+        h->sort_interval = sort_intervals[bi % n_intervals];
+        if (best_blocks) *best_blocks = h->override_blocks;
+        if (best_interval) *best_interval = h->sort_interval;
+    }
+    if (ev[0]) (void)hipEventDestroy(ev[0]);
+    if (ev[1]) (void)hipEventDestroy(ev[1]);
+    if (snap) (void)hipFree(snap);
+    if (snap_id) (void)hipFree(snap_id);
+    return rc;
+}
+
 // ---------------------------------------------------------------------------
 // batched ParticleMeshCoupling1D verbs
 // ---------------------------------------------------------------------------

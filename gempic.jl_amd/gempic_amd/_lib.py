@@ -68,6 +68,8 @@ SIGNATURES = {
     "gempic_profile_read": (_I, [_VP, _I, _P, _PI64]),
     "gempic_profile_reset": (_I, [_VP]),
     "gempic_set_launch": (_I, [_VP, _I]),
+    "gempic_autotune": (_I, [_VP, _D, _I64, ctypes.POINTER(_I), _I, _PI64, _I,
+                             ctypes.POINTER(_I), _PI64, _P]),
     "gempic_set_fusion": (_I, [_VP, _I]),
     "gempic_sample_landau": (_I, [_I, _I64, _D, _D, _D, _D, _P, _I64]),
     "gempic_sample_landau_particles": (_I, [_VP, _D, _D, _D]),

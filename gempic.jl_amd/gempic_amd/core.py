@@ -333,6 +333,53 @@ def l2projection_(coefs_dofs, self: Maxwell1DFEM, func, degree: int):
 
 
 # ---------------------------------------------------------------------------
+# launch autotuning (HamiltonianSplitting.autotune)
+
+# default candidates: workgroups per particle pass (0 = automatic; MI355X has
+# 256 CUs and the reduction scratch holds 8 workgroups per CU) and Strang
+# steps between sorts
+AUTOTUNE_BLOCKS = (0, 512, 1024, 2048)
+AUTOTUNE_SORT_INTERVALS = (10, 20, 40, 80)
+
+
+def autotune_key(V, n, nx, deg0, deg1, dt, n_steps, blocks, sort_intervals) -> str:
+    """Cache key of one autotuning problem: everything the timing depends on."""
+    return (f"V={int(V)},n={int(n)},nx={int(nx)},deg={int(deg0)}/{int(deg1)},dt={float(dt)!r},"
+            f"steps={int(n_steps)},blocks={list(map(int, blocks))},"
+            f"intervals={list(map(int, sort_intervals))}")
+
+
+def autotune_cache_lookup(path: str, key: str):
+    """The stored result for `key` in the JSON cache file, or None."""
+    import json
+    import os
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        table = json.load(f)
+    hit = table.get(key)
+    if hit is None or not isinstance(hit.get("blocks"), int) or \
+            not isinstance(hit.get("sort_interval"), int):
+        return None
+    return hit
+
+
+def autotune_cache_store(path: str, key: str, result: dict):
+    """Add `result` under `key` to the JSON cache file (written atomically)."""
+    import json
+    import os
+    table = {}
+    if os.path.exists(path):
+        with open(path) as f:
+            table = json.load(f)
+    table[key] = {k: v for k, v in result.items() if k != "cached"}
+    tmp = f"{path}.tmp{os.getpid()}"
+    with open(tmp, "w") as f:
+        json.dump(table, f, indent=1, sort_keys=True)
+    os.replace(tmp, path)
+
+
+# ---------------------------------------------------------------------------
 # hamiltonian_splitting.jl, _1d2v.jl, _1d1v.jl
 
 class HamiltonianSplitting:
@@ -427,6 +474,56 @@ class HamiltonianSplitting:
 
     def set_sort_interval(self, k: int):
         check(_lib.load().gempic_set_sort_interval(self._h, int(k)))
+
+    def set_launch(self, blocks: int):
+        """Workgroups per particle pass (0 = automatic)."""
+        check(_lib.load().gempic_set_launch(self._h, int(blocks)))
+
+    def autotune(self, dt: float, n_steps: int | None = None,
+                 blocks=AUTOTUNE_BLOCKS, sort_intervals=AUTOTUNE_SORT_INTERVALS,
+                 cache: str | None = None) -> dict:
+        """Pick the workgroups per particle pass and the sort interval by timing.
+
+        Stands in for the reference's only launch knob, the Julia thread count
+        (hamiltonian_splitting.jl:62). Every (blocks, sort interval) pair runs
+        n_steps fused Strang steps of dt on this splitting's own state
+        (gempic_autotune), which is restored after each trial; the fastest
+        pair stays set. n_steps defaults to twice the largest sort interval
+        (at least 20). With `cache` (a JSON file path) a result found under
+        the same particle count, grid, degrees, dt and candidates is applied
+        without timing, and a new one is stored there.
+        """
+        blocks = [int(b) for b in blocks]
+        sort_intervals = [int(k) for k in sort_intervals]
+        if n_steps is None:
+            n_steps = max(20, 2 * max(sort_intervals))
+        key = autotune_key(self.dims[1], self.particle_group.n_particles,
+                           self.kernel_smoother_0.n_dofs, self.kernel_smoother_0.spline_degree,
+                           self.kernel_smoother_1.spline_degree, dt, n_steps, blocks,
+                           sort_intervals)
+        if cache is not None:
+            hit = autotune_cache_lookup(cache, key)
+            if hit is not None:
+                self.set_launch(hit["blocks"])
+                self.set_sort_interval(hit["sort_interval"])
+                return dict(hit, cached=True)
+        if self.sync:
+            self.to_device()
+        cb = (ctypes.c_int * len(blocks))(*blocks)
+        ci = (ctypes.c_int64 * len(sort_intervals))(*sort_intervals)
+        ms = np.zeros(len(blocks) * len(sort_intervals))
+        bb = ctypes.c_int()
+        bi = ctypes.c_int64()
+        check(_lib.load().gempic_autotune(self._h, float(dt), int(n_steps), cb, len(blocks), ci,
+                                          len(sort_intervals), ctypes.byref(bb),
+                                          ctypes.byref(bi), ptr(ms)))
+        res = {"blocks": int(bb.value), "sort_interval": int(bi.value),
+               "ms_per_step": ms.reshape(len(blocks), len(sort_intervals)).tolist(),
+               "candidates": {"blocks": blocks, "sort_intervals": sort_intervals},
+               "n_steps": int(n_steps)}
+        if cache is not None:
+            autotune_cache_store(cache, key, res)
+        return dict(res, cached=False)
 
 
 def operatorHB(h: HamiltonianSplitting, dt: float):

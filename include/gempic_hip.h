@@ -207,6 +207,22 @@ int gempic_profile_reset(gempic_splitting *h);
  * the pass kernel that fit on one CU; at most 8 per CU). */
 int gempic_set_launch(gempic_splitting *h, int blocks);
 
+/* Launch autotuner. No counterpart in the reference, whose parallelism is the
+ * Julia thread count (hamiltonian_splitting.jl:62, nthreads()); it replaces
+ * that knob. Times n_steps fused Strang steps of dt (gempic_strang_splitting
+ * with flags 0) for every pair (blocks[i], sort_intervals[j]) on the handle's
+ * own particles and fields, twice each, restoring particles, fields and the
+ * sort phase after every trial, then keeps the fastest pair (as
+ * gempic_set_launch and gempic_set_sort_interval would). blocks: workgroups
+ * per particle pass, 0 = automatic; sort_intervals: Strang steps between
+ * sorts, 0 = never. ms_per_step (may be NULL): n_blocks * n_intervals values,
+ * row i for blocks[i]. On an error the state and both settings are left as
+ * they were. Every pair is deterministic, but different pairs add the
+ * deposits in different orders, so results agree to rounding, not bitwise. */
+int gempic_autotune(gempic_splitting *h, double dt, int64_t n_steps, const int *blocks,
+                    int n_blocks, const int64_t *sort_intervals, int n_intervals,
+                    int *best_blocks, int64_t *best_interval, double *ms_per_step);
+
 /* Fused Strang path of 1d2v in one particle kernel per step or in two
  * (HE pair + first Hp2, then Hp1 + second Hp2 + field solves). Both give
  * bitwise the same results; two kernels hold fewer registers. Default 1. */
