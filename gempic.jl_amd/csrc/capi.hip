@@ -1933,7 +1933,11 @@ int gempic_autotune(gempic_splitting *h, double dt, int64_t n_steps, const int *
         h->steps_since_sort = saved_since;
         return c;
     };
-    if (e == hipSuccess) e = copy_state(true);
+    bool saved = false;
+    if (e == hipSuccess) {
+        e = copy_state(true);
+        saved = e == hipSuccess;
+    }
 
     int rc = e == hipSuccess ? GEMPIC_OK
                              : fail(GEMPIC_ERR_DEVICE, std::string("autotune set-up: ") +
@@ -1976,7 +1980,7 @@ int gempic_autotune(gempic_splitting *h, double dt, int64_t n_steps, const int *
     if (rc) {
         // a failed trial leaves the caller's state and settings as they were
         const std::string msg = gempic_last_error();
-        if (snap && snap_id) (void)copy_state(false);
+        if (saved) (void)copy_state(false);
         h->override_blocks = saved_blocks;
         h->sort_interval = saved_interval;
         g_last_error = msg;
