@@ -2763,10 +2763,18 @@ __device__ __forceinline__ void adj_epi_body(const AdjArgs &A, int nblocks, int 
     double gdx = 0.0;
     const double *E2f = A.fields ? A.fields + H_E2 * nx : nullptr;
     const double *E2c = A.fields ? A.fields + H_E2P * nx : nullptr;
+    // grid dot products: wave 0's lanes take nodes j = lane, lane + 64, ...,
+    // and a butterfly adds the lanes in a fixed order (the sum is the same in
+    // every lane of wave 0; call from all of wave 0 only)
     auto dot_d = [&](const double *lb, const double *e) {      // lb . D e, D e[j] = e[j-1] - e[j]
         double r = 0.0;
-        for (int j = 0; j < nx; ++j) r += lb[j] * (e[j == 0 ? nx - 1 : j - 1] - e[j]);
-        return r;
+        for (int j = tid; j < nx; j += kWave) r += lb[j] * (e[j == 0 ? nx - 1 : j - 1] - e[j]);
+        return wave_sum(r);
+    };
+    auto dot = [&](const double *u, const double *v) {
+        double r = 0.0;
+        for (int j = tid; j < nx; j += kWave) r += u[j] * v[j];
+        return wave_sum(r);
     };
     if (KIND != AE_POIS && tid == 0) {
         G[A_LOSS * nx + AS_DX] = G[A_LOSS * nx + AS_DX] + red[AR_SCAL * nx + 0];
@@ -2797,7 +2805,10 @@ __device__ __forceinline__ void adj_epi_body(const AdjArgs &A, int nblocks, int 
             lE2[i] = lE2[i] + dE2[i];
         }
         __syncthreads();
-        if (tid == 0) gdx = gdx - (coef / dx) * dot_d(lB, E2f);
+        if (tid < kWave) {
+            const double r = dot_d(lB, E2f);
+            if (tid == 0) gdx = gdx - (coef / dx) * r;
+        }
         adj_b_from_e(lE2, coef, lB, nx);                       // U of the next step's HE
     }
     if constexpr (KIND == AE_HEAD) {
@@ -2808,11 +2819,13 @@ __device__ __forceinline__ void adj_epi_body(const AdjArgs &A, int nblocks, int 
             G[A_E2 * nx + i] = lE2[i];
             G[A_B * nx + i] = lB[i];
         }
-        if (tid == 0) {
-            double r = 0.0;                                    // (W^T lambda_E2) . B_0
-            for (int i = 0; i < nx; ++i) r += wA[i] * bend[i];
-            gdx = gdx - (coef / dx) * r;
-            G[A_LOSS * nx + AS_DX] = G[A_LOSS * nx + AS_DX] + gdx;
+        __syncthreads();
+        if (tid < kWave) {
+            const double r = dot(wA, bend);                    // (W^T lambda_E2) . B_0
+            if (tid == 0) {
+                gdx = gdx - (coef / dx) * r;
+                G[A_LOSS * nx + AS_DX] = G[A_LOSS * nx + AS_DX] + gdx;
+            }
         }
         return;
     }
@@ -2833,19 +2846,16 @@ __device__ __forceinline__ void adj_epi_body(const AdjArgs &A, int nblocks, int 
             lB[i] = b;
             G[A_MU1 * nx + i] = -w1[i] / dx;
         }
-        if (tid == 0) {
-            double r = 0.0, rh = 0.0;
-            for (int i = 0; i < nx; ++i) {
-                r += bend[i] * wM[i];
-                rh += wA[i] * bend[i];
-            }
-            G[A_LOSS * nx] = G[A_LOSS * nx] + wl * 0.5 * (r * dx);
-            gdx = gdx + wl * 0.5 * r - (KIND == AE_STEP ? 2.0 : 1.0) * (coef / dx) * rh;
-        }
         __syncthreads();
-        if (tid == 0) {
-            gdx = gdx - (coef / dx) * dot_d(lB, E2c);
-            G[A_LOSS * nx + AS_DX] = G[A_LOSS * nx + AS_DX] + gdx;
+        if (tid < kWave) {
+            const double r = dot(bend, wM), rh = dot(wA, bend);
+            const double rd = dot_d(lB, E2c);
+            if (tid == 0) {
+                G[A_LOSS * nx] = G[A_LOSS * nx] + wl * 0.5 * (r * dx);
+                gdx = gdx + wl * 0.5 * r - (KIND == AE_STEP ? 2.0 : 1.0) * (coef / dx) * rh;
+                gdx = gdx - (coef / dx) * rd;
+                G[A_LOSS * nx + AS_DX] = G[A_LOSS * nx + AS_DX] + gdx;
+            }
         }
         adj_b_from_e(lE2, coef, lB, nx);                       // U of this step's HE
         for (int i = tid; i < nx; i += nt) lE2[i] = lE2[i] + dE2[i];
