@@ -425,6 +425,7 @@ __device__ __forceinline__ void pass1d_body(const PassArgs &a, int bid, double *
     if constexpr (NEED_NEW) c_new.init(slots + (W0 + W1) * kWave);
 
     const double xmin = a.xmin, dx = a.dx, qm = a.qm;
+    const double rdx = 1.0 / dx;     // div_by: (x - xmin) / dx, correctly rounded
     // this workgroup's chunk: particles base .. base + cnt_here - 1, addressed
     // through per-chunk base pointers and a 32-bit index
     const int64_t base = (int64_t)bid * a.chunk;
@@ -461,7 +462,7 @@ __device__ __forceinline__ void pass1d_body(const PassArgs &a, int bid, double *
         // marker charge, as (w*charge)*common_weight (hamiltonian_splitting_1d2v.jl:62-64)
         const double q = (w * a.charge) * a.cw;
 
-        double xi = (x - xmin) / dx;
+        double xi = div_by(x - xmin, dx, rdx);
         if (valid && !pos_ok(xi)) errs |= E_INEXACT;
         const int ic = (int)xi;                          // trunc(Int, xi)
         const double off = xi - (double)ic;
@@ -508,7 +509,7 @@ __device__ __forceinline__ void pass1d_body(const PassArgs &a, int bid, double *
         if constexpr (HP1) {
             // operatorHp1 (hamiltonian_splitting_1d2v.jl:53-82 / _1d1v.jl:308-329)
             const double xn = x + a.dt_hp1 * v1;
-            const double xin = (xn - xmin) / dx;
+            const double xin = div_by(xn - xmin, dx, rdx);
             if (valid && !pos_ok(xin)) errs |= E_INEXACT;
             const int inw = V1 ? (int)floor(xin) : (int)xin;
             const double rn = xin - (double)inw;
@@ -547,7 +548,7 @@ __device__ __forceinline__ void pass1d_body(const PassArgs &a, int bid, double *
             // rho deposit at the new position (hamiltonian_splitting_1d2v.jl:93-102,
             // diagnostics.jl:189-193) and the second Hp2; the same division as
             // Hp1's when the mod left the position unchanged
-            const double xi2 = xi_new_ok ? xi_new : (x - xmin) / dx;
+            const double xi2 = xi_new_ok ? xi_new : div_by(x - xmin, dx, rdx);
             if (valid && RHO_ONLY && !pos_ok(xi2)) errs |= E_INEXACT;
             const int ic2 = (int)xi2;
             const double off2 = xi2 - (double)ic2;
@@ -2330,6 +2331,7 @@ __device__ __forceinline__ void adj1d_body(const AdjArgs &A, int bid, double *sm
     }
 
     const double xmin = a.xmin, dx = a.dx, qm = a.qm;
+    const double rdx = 1.0 / dx;     // div_by: every quotient by dx, correctly rounded
     const double dtqm = a.dtqm_he, g = a.dtqm_hp2, dt1 = a.dt_hp1;
     const double s0 = a.scaling0, s1 = a.scaling1;
     const int64_t base = (int64_t)bid * a.chunk;
@@ -2377,7 +2379,7 @@ __device__ __forceinline__ void adj1d_body(const AdjArgs &A, int bid, double *sm
             lv2 = A.lv2[gi];
         }
         const double q = (w * a.charge) * a.cw;
-        const double xi = (x - xmin) / dx;
+        const double xi = div_by(x - xmin, dx, rdx);
         const int ic = pos_ok(xi) ? (int)xi : 0;
         const double off = xi - (double)ic;
         const int key = imod(ic, nx);
@@ -2387,7 +2389,7 @@ __device__ __forceinline__ void adj1d_body(const AdjArgs &A, int bid, double *sm
         if constexpr (POIS) {
             double db0[W0];
             dbspline<P0>(off, db0);
-            const double dnu = gather_p<P0>(db0, sM1, key) / dx;
+            const double dnu = div_by(gather_p<P0>(db0, sM1, key), dx, rdx);
             if (valid) A.lx[gi] = q * s0 * dnu;
             continue;
         }
@@ -2395,8 +2397,8 @@ __device__ __forceinline__ void adj1d_body(const AdjArgs &A, int bid, double *sm
             double db1[W1], db0[W0];
             dbspline<P1>(off, db1);
             dbspline<P0>(off, db0);
-            const double de1 = gather_p<P1>(db1, sE1, key) / dx;
-            const double de2p = gather_p<P0>(db0, sE2p, key) / dx;
+            const double de1 = div_by(gather_p<P1>(db1, sE1, key), dx, rdx);
+            const double de2p = div_by(gather_p<P0>(db0, sE2p, key), dx, rdx);
             const double dhe = dtqm * (lv1 * de1 + lv2 * de2p);
             lx = lx + dhe;
             if (valid) acc_dx = acc_dx - xi * dhe;
@@ -2427,7 +2429,7 @@ __device__ __forceinline__ void adj1d_body(const AdjArgs &A, int bid, double *sm
         const double v2b = v2;
         v1 = v1 + g * v2 * bA;
         const double xn = x + dt1 * v1;
-        const double xin = (xn - xmin) / dx;
+        const double xin = div_by(xn - xmin, dx, rdx);
         const int inw = pos_ok(xin) ? (int)xin : 0;
         const double rn = xin - (double)inw;
         const int dseg = inw - ic;
@@ -2450,7 +2452,7 @@ __device__ __forceinline__ void adj1d_body(const AdjArgs &A, int bid, double *sm
         }
         const double x1 = jl_mod(xn, a.Lx);
         const bool same = __double_as_longlong(x1) == __double_as_longlong(xn);
-        const double xi2 = same ? xin : (x1 - xmin) / dx;
+        const double xi2 = same ? xin : div_by(x1 - xmin, dx, rdx);
         const int ic2 = pos_ok(xi2) ? (int)xi2 : 0;
         const double off2 = xi2 - (double)ic2;
         const int key2 = imod(ic2, nx);
@@ -2489,14 +2491,14 @@ __device__ __forceinline__ void adj1d_body(const AdjArgs &A, int bid, double *sm
             double lx1;                 // cotangent of the position after the mod
             {
                 const double mu = gather_p<P0>(c0, sM2, key2);
-                const double dmu = gather_p<P0>(dc0, sM2, key2) / dx;
-                const double db = gather_p<P1>(dc1, sB, key2) / dx;
+                const double dmu = div_by(gather_p<P0>(dc0, sM2, key2), dx, rdx);
+                const double db = div_by(gather_p<P1>(dc1, sB, key2), dx, rdx);
                 const double d2 = gl2 * db + q * s0 * v2d * dmu;
                 lx = lx + d2;
                 lv2 = lv2 + g * bB * lv1 + q * s0 * mu;
                 lx1 = lx;
                 if (valid) {
-                    acc_dx = acc_dx - xi2 * d2 - q * s0 * v2d * mu / dx;
+                    acc_dx = acc_dx - xi2 * d2 - div_by(q * s0 * v2d * mu, dx, rdx);
                     acc_w = acc_w + cwq * s0 * v2d * mu;
                 }
             }
@@ -2555,7 +2557,7 @@ __device__ __forceinline__ void adj1d_body(const AdjArgs &A, int bid, double *sm
                 }
                 if (valid) {
                     // d/d dx of int_x^xn f = (1/dx) int f - xin f(xn) + xi f(x)
-                    acc_dx = acc_dx - qm * lv2h * (ib / dx - xin * bN + xi * bA) +
+                    acc_dx = acc_dx - qm * lv2h * (div_by(ib, dx, rdx) - xin * bN + xi * bA) +
                              q * s1 * (xi * muO - xin * muN);
                     acc_w = acc_w + cwq * s1 * im1;
                     // x = mod(xn, Lx): d x / d Lx = the number of periods added
@@ -2575,14 +2577,14 @@ __device__ __forceinline__ void adj1d_body(const AdjArgs &A, int bid, double *sm
             }
             {
                 const double mu = gather_p<P0>(b0, sM2, key);
-                const double dmu = gather_p<P0>(db0, sM2, key) / dx;
-                const double db = gather_p<P1>(db1, sB, key) / dx;
+                const double dmu = div_by(gather_p<P0>(db0, sM2, key), dx, rdx);
+                const double db = div_by(gather_p<P1>(db1, sB, key), dx, rdx);
                 const double gl1 = g * v2b * lv1;
                 const double d1 = gl1 * db + q * s0 * v2b * dmu;
                 lx = lx + d1;
                 lv2 = lv2 + g * bA * lv1 + q * s0 * mu;
                 if (valid) {
-                    acc_dx = acc_dx - xi * d1 - q * s0 * v2b * mu / dx;
+                    acc_dx = acc_dx - xi * d1 - div_by(q * s0 * v2b * mu, dx, rdx);
                     acc_w = acc_w + cwq * s0 * v2b * mu;
                 }
 #pragma unroll
@@ -2608,13 +2610,13 @@ __device__ __forceinline__ void adj1d_body(const AdjArgs &A, int bid, double *sm
             }
             // HE of this step and of the previous one: same x and cotangents
             {
-                const double de1 = gather_p<P1>(db1, sE1, key) / dx;
-                const double de2 = gather_p<P0>(db0, sE2, key) / dx;
+                const double de1 = div_by(gather_p<P1>(db1, sE1, key), dx, rdx);
+                const double de2 = div_by(gather_p<P0>(db0, sE2, key), dx, rdx);
                 const double dh = dtqm * (lv1 * de1 + lv2 * de2);
                 lx = lx + dh;
                 double dhs = dh;
                 if constexpr (PREV) {
-                    const double de2p = gather_p<P0>(db0, sE2p, key) / dx;
+                    const double de2p = div_by(gather_p<P0>(db0, sE2p, key), dx, rdx);
                     const double dhp = dtqm * (lv1 * de1 + lv2 * de2p);
                     lx = lx + dhp;
                     dhs = dhs + dhp;
