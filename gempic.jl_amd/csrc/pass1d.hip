@@ -1245,6 +1245,7 @@ __global__ void __launch_bounds__(kBlock) k_boris1d(PassArgs a)
     }
 
     const double xmin = a.xmin, dx = a.dx;
+    const double rdx = 1.0 / dx;     // div_by: (x - xmin) / dx, correctly rounded
     const int64_t base = (int64_t)blockIdx.x * a.chunk;
     const int64_t left = a.n - base;
     const int cnt_here = left < a.chunk ? (int)left : (int)a.chunk;
@@ -1277,7 +1278,7 @@ __global__ void __launch_bounds__(kBlock) k_boris1d(PassArgs a)
         }
         if constexpr (KICK) {
             // evaluate (particle_mesh_coupling_1d.jl:438-453) at the old position
-            const double xi = (x - xmin) / dx;
+            const double xi = div_by(x - xmin, dx, rdx);
             if (valid && !pos_ok(xi)) errs |= E_INEXACT;
             const int ic = (int)xi;
             const double off = xi - (double)ic;
@@ -1319,7 +1320,7 @@ __global__ void __launch_bounds__(kBlock) k_boris1d(PassArgs a)
             const double q = (w * a.charge) * a.cw;
             const double xn = x + a.dt_hp1 * v1;
             const double xm = (x + xn) * 0.5;
-            const double xim = (xm - xmin) / dx;
+            const double xim = div_by(xm - xmin, dx, rdx);
             if (valid && !pos_ok(xim)) errs |= E_INEXACT;
             const int icm = (int)xim;                    // trunc(Int, xi)
             const double offm = xim - (double)icm;

@@ -65,6 +65,7 @@ struct Pmc2Args {
     int64_t chunk;              // particles per workgroup (multiple of 64)
     int nx, ny;
     double xmin, ymin, dx, dy, scaling;
+    double rdx, rdy;            // 1 / dx, 1 / dy (correctly rounded): div_by
     const double *x, *y, *w;    // w: wp = weight * charge (add_charge! argument)
     const double *f1, *f2;
     double *out1, *out2;
@@ -80,8 +81,8 @@ template <int P>
 __device__ __forceinline__ bool shape2(const Pmc2Args &a, double x, double y, int &sx, int &sy,
                                        double (&vx)[P + 1], double (&vy)[P + 1])
 {
-    const double xp = (x - a.xmin) / a.dx;
-    const double yp = (y - a.ymin) / a.dy;
+    const double xp = div_by(x - a.xmin, a.dx, a.rdx);
+    const double yp = div_by(y - a.ymin, a.dy, a.rdy);
     const bool ok = fabs(xp) < 1.0e9 && fabs(yp) < 1.0e9;
     const double cx = ok ? ceil(xp) : 0.0, cy = ok ? ceil(yp) : 0.0;
     const int ip = (int)cx, jp = (int)cy;
@@ -109,8 +110,8 @@ template <int P>
 __device__ __forceinline__ bool shape2_pp(const Pmc2Args &a, double x, double y, int &sx, int &sy,
                                           double (&vx)[P + 1], double (&vy)[P + 1])
 {
-    const double xp = (x - a.xmin) / a.dx;
-    const double yp = (y - a.ymin) / a.dy;
+    const double xp = div_by(x - a.xmin, a.dx, a.rdx);
+    const double yp = div_by(y - a.ymin, a.dy, a.rdy);
     const bool ok = fabs(xp) < 1.0e9 && fabs(yp) < 1.0e9;
     const int ip = ok ? (int)floor(xp) + 1 : 1, jp = ok ? (int)floor(yp) + 1 : 1;
     const double dxp = ok ? xp - (double)(ip - 1) : 0.0;
@@ -383,8 +384,8 @@ __global__ void __launch_bounds__(256) k_pmc2d_eval_pp(Pmc2Args a)
     int errs = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n;
          i += (int64_t)gridDim.x * blockDim.x) {
-        double xi = (a.x[i] - a.xmin) / a.dx;
-        double yi = (a.y[i] - a.ymin) / a.dy;
+        double xi = div_by(a.x[i] - a.xmin, a.dx, a.rdx);
+        double yi = div_by(a.y[i] - a.ymin, a.dy, a.rdy);
         const bool fin = fabs(xi) < 1.0e9 && fabs(yi) < 1.0e9;
         const int idx = fin ? (int)floor(xi) + 1 : 0, idy = fin ? (int)floor(yi) + 1 : 0;
         xi = xi - (double)(idx - 1);
@@ -1088,6 +1089,8 @@ int pmc2_geometry(int op, int64_t nx, int64_t ny, double xmin, double xmax, doub
     a.ymin = ymin;
     a.dx = (xmax - xmin) / (double)nx;
     a.dy = (ymax - ymin) / (double)ny;
+    a.rdx = 1.0 / a.dx;
+    a.rdy = 1.0 / a.dy;
     a.scaling = smoothing == GEMPIC_COLLOCATION ? 1.0 / (a.dx * a.dy) : 1.0;
     double pf[36];
     gh::splinepp_tables(degree, a.pc, pf);      // SplinePP(degree, n) (splinepp.jl:36-233)
