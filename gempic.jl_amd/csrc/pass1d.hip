@@ -956,6 +956,7 @@ __global__ void __launch_bounds__(kBlock) k_diag1d2v(PassArgs a)
     __syncthreads();
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     const int64_t base = (int64_t)blockIdx.x * a.chunk;
+    const double rdx = 1.0 / a.dx;   // div_by: (x - xmin) / dx, correctly rounded
     for (int it = 0; it < a.iters; ++it) {
         const int64_t i = base + (int64_t)it * kBlock + tid;
         if (i >= a.n) continue;
@@ -964,7 +965,7 @@ __global__ void __launch_bounds__(kBlock) k_diag1d2v(PassArgs a)
         acc[0] += (v1 * v1 + v2 * v2) * wm;
         acc[1] += v1 * wm;
         acc[2] += v2 * wm;
-        const double xi = (x - a.xmin) / a.dx;
+        const double xi = div_by(x - a.xmin, a.dx, rdx);
         const int ic = (int)xi;
         const double off = xi - (double)ic;
         const int key = imod(ic, nx);
@@ -2070,11 +2071,12 @@ __global__ void __launch_bounds__(kBlock) k_pmc(PmcArgs a)
     if constexpr (DEPOSIT) cache.init(slots);
     int errs = 0;
     const int64_t base = (int64_t)blockIdx.x * a.chunk;
+    const double rdx = 1.0 / a.dx;   // div_by: (x - xmin) / dx, correctly rounded
     for (int it = 0; it < a.iters; ++it) {
         const int64_t i = base + (int64_t)it * kBlock + tid;
         const bool valid = i < a.n;
         const double x = valid ? a.x[i] : a.xmin;
-        const double xi0 = (x - a.xmin) / a.dx;
+        const double xi0 = div_by(x - a.xmin, a.dx, rdx);
         if (valid && !pos_ok(xi0)) errs |= E_INEXACT;
         if constexpr (OP == PMC_EVAL || OP == PMC_EVAL_PP || OP == PMC_CHARGE ||
                       OP == PMC_CHARGE_PP) {
@@ -2105,7 +2107,7 @@ __global__ void __launch_bounds__(kBlock) k_pmc(PmcArgs a)
             double v = (valid && OP != PMC_CURRENT_NOB) ? a.out[i] : 0.0;
             const int io = (int)xi0;
             const double ro = xi0 - (double)io;
-            const double xin = (xn - a.xmin) / a.dx;
+            const double xin = div_by(xn - a.xmin, a.dx, rdx);
             if (valid && !pos_ok(xin)) errs |= E_INEXACT;
             const int inw = (OP == PMC_CURRENT_NOB) ? (int)floor(xin) : (int)xin;
             const double rn = xin - (double)inw;
