@@ -175,6 +175,27 @@ void or_maxwell2d_fem_rhs(or_maxwell2d *m, double (*f)(double, double, void *), 
 void or_maxwell2d_l2projection(or_maxwell2d *m, double (*f)(double, double, void *), void *ctx,
                                int component, int form, double *out);
 
+/* ---------- 2D3V Hamiltonian splitting (gempic_oracle2d3v.c; the reference's
+ * hamiltonian_splitting_2d3v.jl is empty) ---------- */
+typedef struct or_split2d or_split2d;
+/* particles: 6 x n column-major (x1, x2, v1, v2, v3, w), e and b: 3 arrays of
+ * nx*ny each; all ALIASED */
+or_split2d *or_split2d_create(int64_t nx, int64_t ny, double xmin, double xmax, double ymin,
+                              double ymax, int degree, int64_t n, double charge, double mass,
+                              double common_weight, double *particles, double *const *e,
+                              double *const *b);
+void or_split2d_destroy(or_split2d *h);
+const double *or_split2d_j(const or_split2d *h, int component);
+void or_split2d_operator_hb(or_split2d *h, double dt);
+void or_split2d_operator_he(or_split2d *h, double dt);
+void or_split2d_operator_hp1(or_split2d *h, double dt);
+void or_split2d_operator_hp2(or_split2d *h, double dt);
+void or_split2d_operator_hp3(or_split2d *h, double dt);
+void or_split2d_strang(or_split2d *h, double dt, int64_t nsteps);
+void or_split2d_rho(or_split2d *h, double *rho);
+void or_split2d_rho_from_e(or_split2d *h, double *rho);
+void or_split2d_energy(or_split2d *h, double *out7);
+
 /* ---------- hamiltonian_splitting*.jl ---------- */
 typedef struct or_split or_split;
 

@@ -131,6 +131,19 @@ def lib():
             "or_maxwell2d_compute_e_from_rho": (None, [_VP, _P, _PP]),
             "or_maxwell2d_fem_rhs": (None, [_VP, _FN2, _VP, _I, _I, _P]),
             "or_maxwell2d_l2projection": (None, [_VP, _FN2, _VP, _I, _I, _P]),
+            "or_split2d_create": (_VP, [_I64, _I64, _D, _D, _D, _D, _I, _I64, _D, _D, _D, _P,
+                                        _PP, _PP]),
+            "or_split2d_destroy": (None, [_VP]),
+            "or_split2d_j": (_P, [_VP, _I]),
+            "or_split2d_operator_hb": (None, [_VP, _D]),
+            "or_split2d_operator_he": (None, [_VP, _D]),
+            "or_split2d_operator_hp1": (None, [_VP, _D]),
+            "or_split2d_operator_hp2": (None, [_VP, _D]),
+            "or_split2d_operator_hp3": (None, [_VP, _D]),
+            "or_split2d_strang": (None, [_VP, _D, _I64]),
+            "or_split2d_rho": (None, [_VP, _P]),
+            "or_split2d_rho_from_e": (None, [_VP, _P]),
+            "or_split2d_energy": (None, [_VP, _P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -579,6 +592,68 @@ class Maxwell2D:
         cb = _FN2(lambda x, y, ctx: float(f(x, y)))
         out = np.zeros(self.nx * self.ny)
         lib().or_maxwell2d_l2projection(self.h, cb, None, component, form, _p(out))
+        return out
+
+
+class Splitting2D3V:
+    """The 2D3V Hamiltonian splitting (gempic_oracle2d3v.c; the reference's
+    hamiltonian_splitting_2d3v.jl is empty). particles (6 x n, rows x1, x2,
+    v1, v2, v3, w, Fortran order), e and b (3 arrays of nx*ny each) are
+    aliased and updated in place."""
+
+    def __init__(self, nx, ny, xmin, xmax, ymin, ymax, degree, particles, e, b, charge=1.0,
+                 mass=1.0, common_weight=0.0):
+        assert particles.flags.f_contiguous and particles.shape[0] == 6
+        self._keep = (particles, e, b)
+        self._pe, self._pb = _ptrs3(e), _ptrs3(b)
+        self.h = lib().or_split2d_create(nx, ny, xmin, xmax, ymin, ymax, degree,
+                                         particles.shape[1], charge, mass, common_weight,
+                                         _p(particles), self._pe, self._pb)
+        if not self.h:
+            raise ValueError("invalid 2D3V splitting arguments")
+        self.nx, self.ny = nx, ny
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_split2d_destroy(self.h)
+            self.h = None
+
+    def j(self, component):
+        return np.ctypeslib.as_array(lib().or_split2d_j(self.h, component),
+                                     shape=(self.nx * self.ny,)).copy()
+
+    def operatorHB(self, dt):
+        lib().or_split2d_operator_hb(self.h, dt)
+
+    def operatorHE(self, dt):
+        lib().or_split2d_operator_he(self.h, dt)
+
+    def operatorHp1(self, dt):
+        lib().or_split2d_operator_hp1(self.h, dt)
+
+    def operatorHp2(self, dt):
+        lib().or_split2d_operator_hp2(self.h, dt)
+
+    def operatorHp3(self, dt):
+        lib().or_split2d_operator_hp3(self.h, dt)
+
+    def strang_splitting(self, dt, nsteps):
+        lib().or_split2d_strang(self.h, dt, nsteps)
+
+    def rho(self):
+        out = np.zeros(self.nx * self.ny)
+        lib().or_split2d_rho(self.h, _p(out))
+        return out
+
+    def rho_from_e(self):
+        out = np.zeros(self.nx * self.ny)
+        lib().or_split2d_rho_from_e(self.h, _p(out))
+        return out
+
+    def energy(self):
+        """kinetic, 1/2 (E_c, M1 E_c) for c = 1..3, 1/2 (B_c, M2 B_c) for c = 1..3"""
+        out = np.zeros(7)
+        lib().or_split2d_energy(self.h, _p(out))
         return out
 
 
