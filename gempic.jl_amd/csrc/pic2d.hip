@@ -1196,7 +1196,10 @@ int dep_plan(int degree, bool det, bool pp, int64_t n, size_t nn, DepPlan &pl)
 }
 
 // Enqueue one verb on `s` over device arrays (a.x, a.y, ... set by the caller).
-int pmc2_enqueue(int op, int degree, bool deterministic, Pmc2Args a, double *rho, hipStream_t s)
+// With `fixed` the deposit runs in that workspace as it is (never grown): its
+// workgroups are capped so that their grids and the split sums fit.
+int pmc2_enqueue(int op, int degree, bool deterministic, Pmc2Args a, double *rho, hipStream_t s,
+                 const Ws2 *fixed = nullptr)
 {
     const int cus = abi_device_cus();
     const size_t nn = (size_t)a.nx * a.ny;
@@ -1206,14 +1209,25 @@ int pmc2_enqueue(int op, int degree, bool deterministic, Pmc2Args a, double *rho
     if (charge) {
         int rc = dep_plan(degree, deterministic, pp, a.n, nn, pl);
         if (rc) return rc;
+        if (fixed) {
+            const int64_t grids = (int64_t)(fixed->cap / nn);      // workgroup grids + split sums
+            if (grids < 2)
+                return abi_fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: the 2D grid does not fit in the fixed deposit scratch");
+            const int64_t bmax = grids >= 128 ? grids - 64 : grids / 2;
+            pl.blocks = (int)std::min<int64_t>(pl.blocks, bmax);
+        }
         const int64_t waves = (int64_t)pl.blocks * pl.nw;
         a.chunk = ((a.n + waves - 1) / waves + kWave - 1) / kWave * kWave;      // per wave
         nblk = (int)((a.n + (int64_t)pl.nw * a.chunk - 1) / ((int64_t)pl.nw * a.chunk));
         splits = std::min(nblk, 64);
     }
-    Ws2 *w = nullptr;
-    int rc = ws2_get((size_t)(nblk + splits) * nn, w);
-    if (rc) return rc;
+    const Ws2 *w = fixed;
+    if (!w) {
+        Ws2 *g = nullptr;
+        int rc = ws2_get((size_t)(nblk + splits) * nn, g);
+        if (rc) return rc;
+        w = g;
+    }
     a.err = w->err;
     a.part = w->part;
     if (charge) {
@@ -1367,9 +1381,104 @@ int pmc2_dev(int op, int64_t nx, int64_t ny, double xmin, double xmax, double ym
     return pmc2_finish(s);
 }
 
+// ---------------------------------------------------------------------------
+// The allocation-free deposit (gempic_pmc2d_add_charge_dev_noalloc): the
+// error flag and the deposit's workgroup grids and split sums live in one
+// file-scope device array, `scratch`, instead of the grown workspace. Each
+// GPU holds its own copy of the array (GEMPIC_NOALLOC_SCRATCH_BYTES) until
+// the process ends; host code reaches it only through the address that
+// hipGetSymbolAddress returns for the current device.
+// ---------------------------------------------------------------------------
+#ifndef GEMPIC_NOALLOC_SCRATCH_BYTES
+#define GEMPIC_NOALLOC_SCRATCH_BYTES (32u << 20)
+#endif
+constexpr size_t kNoallocHeader = 256;      // bytes: the int error flag, then padding
+static_assert(GEMPIC_NOALLOC_SCRATCH_BYTES % 256 == 0 && GEMPIC_NOALLOC_SCRATCH_BYTES > kNoallocHeader,
+              "scratch size");
+
+}  // namespace
+
+static __device__ __align__(256) double scratch[GEMPIC_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+namespace {
+
+__global__ void k_clear_flag(int *flag) { *flag = 0; }
+
+std::mutex g_noalloc_mu;
+Ws2 g_noalloc[64];                          // per device: views into that device's `scratch`
+
+// the current device's view of `scratch`; resolved (and its flag cleared on
+// `s`) at the first call on each device
+int noalloc_ws(hipStream_t s, const Ws2 *&w)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64)
+        return abi_fail(GEMPIC_ERR_DEVICE, "hipGetDevice failed");
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    Ws2 &v = g_noalloc[dev];
+    if (!v.err) {
+        void *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return abi_fail(GEMPIC_ERR_DEVICE, (std::string("deposit scratch: hipGetSymbolAddress: ") +
+                                                hipGetErrorString(e)).c_str());
+        hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, s, (int *)ptr);
+        e = hipGetLastError();
+        if (e != hipSuccess)
+            return abi_fail(GEMPIC_ERR_DEVICE, (std::string("deposit scratch: ") + hipGetErrorString(e)).c_str());
+        v.err = (int *)ptr;
+        v.part = (double *)((char *)ptr + kNoallocHeader);
+        v.cap = (GEMPIC_NOALLOC_SCRATCH_BYTES - kNoallocHeader) / sizeof(double);
+    }
+    w = &v;
+    return GEMPIC_OK;
+}
+
+// wait for `s`, then report the scratch's error flag and clear it
+int noalloc_finish(hipStream_t s, const Ws2 *w)
+{
+    int herr = 0;
+    hipError_t e = hipMemcpyAsync(&herr, w->err, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e == hipSuccess && herr) {
+        hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, s, w->err);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+    }
+    if (e != hipSuccess)
+        return abi_fail(GEMPIC_ERR_DEVICE, (std::string("2D deposit: ") + hipGetErrorString(e)).c_str());
+    if (herr & E_INEXACT)
+        return abi_fail(GEMPIC_ERR_INEXACT, "InexactError: ceil(Int, x) of a non-finite or out-of-range position");
+    return GEMPIC_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int gempic_pmc2d_add_charge_dev_noalloc(int64_t nx, int64_t ny, double xmin, double xmax,
+                                        double ymin, double ymax, int degree, int smoothing,
+                                        int64_t n, const double *x, const double *y,
+                                        const double *wp, double *rho, int flags, void *stream)
+{
+    if ((n > 0 && (!x || !y || !wp)) || !rho) return abi_fail(GEMPIC_ERR_ARGUMENT, "NULL array");
+    Pmc2Args a;
+    int rc = pmc2_geometry(P2_CHARGE, nx, ny, xmin, xmax, ymin, ymax, degree, smoothing, n, a);
+    if (rc) return rc;
+    rc = abi_check_device();
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    const Ws2 *w = nullptr;
+    rc = noalloc_ws(s, w);
+    if (rc) return rc;
+    if (n > 0) {
+        a.x = x; a.y = y; a.w = wp;
+        rc = pmc2_enqueue(P2_CHARGE, degree, (flags & GEMPIC_DETERMINISTIC) != 0, a, rho, s, w);
+        if (rc) return rc;
+    }
+    if (flags & GEMPIC_ASYNC) return GEMPIC_OK;
+    return noalloc_finish(s, w);
+}
 
 int gempic_pmc2d_add_charge(int64_t nx, int64_t ny, double xmin, double xmax, double ymin,
                             double ymax, int degree, int smoothing, int64_t n, const double *x,

@@ -110,6 +110,8 @@ SIGNATURES = {
     "gempic_pmc2d_evaluate_multiple_dev": (_I, [_I64, _I64, _D, _D, _D, _D, _I, _I, _I64, _VP, _VP,
                                                 _VP, _VP, _VP, _VP, _I, _VP]),
     "gempic_pmc2d_sync": (_I, [_VP]),
+    "gempic_pmc2d_add_charge_dev_noalloc": (_I, [_I64, _I64, _D, _D, _D, _D, _I, _I, _I64, _VP,
+                                                 _VP, _VP, _VP, _I, _VP]),
     "gempic_maxwell2d_create": (_I, [_I64, _D, _D, _I64, _D, _D, _I, _PP]),
     "gempic_maxwell2d_destroy": (_I, [_VP]),
     "gempic_maxwell2d_tables": (_I, [_VP, _P, _P, _P, _P]),

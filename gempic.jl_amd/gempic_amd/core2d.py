@@ -25,7 +25,7 @@ __all__ = [
     "TwoDGrid", "ParticleMeshCoupling2D", "evaluate_multiple", "TwoDMaxwell",
     "TwoDLinearSolverSplineMass", "solve", "multiply_mass_2dkron_", "compute_rho_from_e_",
     "compute_fem_rhs_", "l2projection", "spline_fem_mass_line", "spline_fem_compute_mass_eig",
-    "add_charge_device_", "evaluate_device", "evaluate_multiple_device", "pmc2d_sync",
+    "add_charge_device_", "add_charge_device_noalloc_", "evaluate_device", "evaluate_multiple_device", "pmc2d_sync",
     "b_to_pp_2d_",
 ]
 
@@ -211,6 +211,18 @@ def add_charge_device_(rho, pm: ParticleMeshCoupling2D, x, y, wp, deterministic=
             _dev(rho, pm.grid.nx * pm.grid.ny, "rho"))
     check(_lib.load().gempic_pmc2d_add_charge_dev(*pm._args(), n, *args,
                                                   _flags(deterministic, async_), _stream(stream)))
+
+
+def add_charge_device_noalloc_(rho, pm: ParticleMeshCoupling2D, x, y, wp, deterministic=False,
+                               async_=False, stream=None):
+    """add_charge_device_ through the allocation-free entry point: the
+    deposit's scratch is the library's fixed per-GPU array, so the call
+    allocates no device memory (calls overlapping on one GPU race)."""
+    n = x.size
+    args = (_dev(x, n, "x"), _dev(y, n, "y"), _dev(wp, n, "wp"),
+            _dev(rho, pm.grid.nx * pm.grid.ny, "rho"))
+    check(_lib.load().gempic_pmc2d_add_charge_dev_noalloc(
+        *pm._args(), n, *args, _flags(deterministic, async_), _stream(stream)))
 
 
 def evaluate_device(pm: ParticleMeshCoupling2D, x, y, field_dofs, out, async_=False,

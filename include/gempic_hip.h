@@ -379,6 +379,24 @@ int gempic_pmc2d_evaluate_multiple_dev(int64_t nx, int64_t ny, double xmin, doub
 /* wait for `stream` and report errors of earlier GEMPIC_ASYNC 2D calls */
 int gempic_pmc2d_sync(void *stream);
 
+/* Allocation-free add_charge! on device arrays. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Arguments, flags and results are those of
+ * gempic_pmc2d_add_charge_dev; the call makes no hipMalloc/hipFree. Its
+ * error flag, workgroup grids and split sums live in a fixed 32 MiB array of
+ * device memory that every GPU the library runs on holds until the process
+ * ends: the deposit uses at most (32 MiB - 256 B) / (8 nx ny) grids, so its
+ * workgroup count is capped by the grid size (nx*ny <= 4096, as for the
+ * other deposit). All calls on one GPU share that array and keep the last
+ * call's contents: calls that overlap on one GPU race. With GEMPIC_ASYNC an
+ * InexactError is reported by the next synchronous call of this entry point
+ * on the same GPU. */
+int gempic_pmc2d_add_charge_dev_noalloc(int64_t nx, int64_t ny, double xmin, double xmax,
+                                        double ymin, double ymax, int degree, int smoothing,
+                                        int64_t n, const double *x, const double *y,
+                                        const double *wp, double *rho, int flags, void *stream);
+
 /* ------------------------------------------------------------------------
  * TwoDMaxwell (maxwell_2d_fem.jl:11-88) with its TwoDPoisson
  * (poisson_2d_fem.jl:3-85) and TwoDLinearSolverSplineMass
