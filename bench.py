@@ -44,6 +44,41 @@ def make_splitting(G, w, sync=False):
     return h
 
 
+def arith_run(G, _lib, w, arith, warmup, steps, blocks, interval, sort_s):
+    """A fused-path run of `steps` Strang steps of w in the given arithmetic:
+    particle-steps/s (the sort charged per step as in the headline when
+    sort_s is given, else measured in the window), the pass's HIP-event time
+    and its roofline at 56 B per particle-step."""
+    L = _lib.load()
+    n = w["particles"].shape[1]
+    h = make_splitting(G, w)
+    h.set_arithmetic(arith)
+    _lib.check(L.gempic_set_launch(h._h, blocks))
+    _lib.check(L.gempic_set_sort_interval(h._h, interval))
+    if warmup:
+        _lib.check(L.gempic_strang_splitting(h._h, w["dt"], warmup, _lib.STRANG_FUSED))
+    _lib.check(L.gempic_sort_particles(h._h))
+    _lib.check(L.gempic_profile_enable(h._h, 10))
+    _lib.check(L.gempic_profile_reset(h._h))
+    t0 = time.perf_counter()
+    _lib.check(L.gempic_strang_splitting(h._h, w["dt"], steps, _lib.STRANG_FUSED))
+    _lib.check(L.gempic_synchronize(h._h))
+    el = time.perf_counter() - t0
+    if sort_s is not None and interval > 0:
+        el += (steps / interval - (steps - 1) // interval) * sort_s
+    ms, cnt = h.profile_read(0)
+    epi_ms, epi_cnt = h.profile_read(9)
+    h.profile(False)
+    pass_s = 1e-3 * ms / max(cnt, 1)
+    ach = BYTES_FUSED * n / pass_s / 1e9
+    del h
+    return {"arithmetic": arith, "particles": n, "steps": steps,
+            "value": n * steps / el, "ms_per_step": 1e3 * el / steps,
+            "pass_us": 1e6 * pass_s, "field_part_us": 1e3 * epi_ms / max(epi_cnt, 1),
+            "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": ach / HBM_PEAK_GBS, "bytes_per_particle_step": BYTES_FUSED}}
+
+
 def cpu_baseline(n, steps, seed=1):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as orc
@@ -413,6 +448,13 @@ def main():
     ap.add_argument("--substep", action="store_true", help="also time the sub-step path")
     ap.add_argument("--blocks", type=int, default=0, help="workgroups per particle pass (0 = auto)")
     ap.add_argument("--fusion", type=int, default=1, help="particle kernels per fused step (1 or 2)")
+    ap.add_argument("--arith", choices=("fused", "reference"), default="fused",
+                    help="arithmetic of the particle pass: fused (closed-form bases and fma, "
+                         "parity with the reference to 1e-12 per sub-step) or reference (the "
+                         "reference's rounding, bitwise per particle)")
+    ap.add_argument("--hbm-particles", type=int, default=16777216,
+                    help="particles of the state-larger-than-the-caches point (0 = skip it)")
+    ap.add_argument("--hbm-steps", type=int, default=200)
     ap.add_argument("--no-sampling", action="store_true", help="skip the sample! measurement")
     ap.add_argument("--boris-steps", type=int, default=200,
                     help="steps of the Boris splitting measurement (0 = skip it)")
@@ -457,6 +499,7 @@ def main():
     _lib.check(L.gempic_set_sort_interval(h._h, args.sort_interval))
     _lib.check(L.gempic_set_launch(h._h, args.blocks))
     _lib.check(L.gempic_set_fusion(h._h, args.fusion))
+    h.set_arithmetic(args.arith)
     dt = w["dt"]
     tune = None
     blocks, interval = args.blocks, args.sort_interval
@@ -476,6 +519,19 @@ def main():
     if args.warmup:
         _lib.check(L.gempic_strang_splitting(h._h, dt, args.warmup, _lib.STRANG_FUSED))
     _lib.check(L.gempic_synchronize(h._h))
+    # the particle sort, timed on its own (HIP events on the library's stream)
+    _lib.check(L.gempic_profile_enable(h._h, 1))
+    _lib.check(L.gempic_profile_reset(h._h))
+    for _ in range(3):
+        _lib.check(L.gempic_sort_particles(h._h))
+    sort_ms_tot, sort_cnt = h.profile_read(6)
+    sort_s = 1e-3 * sort_ms_tot / max(sort_cnt, 1)
+    # The timed window starts right after a sort: steps s = interval,
+    # 2 interval, ... inside it sort, floor((K - 1) / interval) of them. The
+    # figures below charge every step its share of a sort instead
+    # (sort time / interval), whether or not one falls in the window.
+    sorts_in_window = (args.steps - 1) // interval if interval > 0 else 0
+    sort_share_s = (args.steps / interval - sorts_in_window) * sort_s if interval > 0 else 0.0
     # HIP events around every 10th launch of each kernel: the kernel times
     # come from the timed run itself at a small cost to its stream
     _lib.check(L.gempic_profile_enable(h._h, 10))
@@ -484,7 +540,8 @@ def main():
     t0 = time.perf_counter()
     _lib.check(L.gempic_strang_splitting(h._h, dt, args.steps, _lib.STRANG_FUSED))
     _lib.check(L.gempic_synchronize(h._h))
-    elapsed = time.perf_counter() - t0
+    elapsed_window = time.perf_counter() - t0
+    elapsed = elapsed_window + sort_share_s
     fused_ms, fused_cnt = h.profile_read(0)
     kernels = {}
     for kid, name in enumerate(["fused_pass", "trailing_pass", "he", "hp2", "hp1", "grid",
@@ -498,6 +555,25 @@ def main():
     achieved = BYTES_FUSED * n / avg_s / 1e9
     print(f"[bench] {value:.4e} particle-steps/s, fused pass {avg_s * 1e6:.2f} us",
           file=sys.stderr, flush=True)
+
+    # the same timed run in the reference's arithmetic, on a fresh copy of the state
+    ref_arith = None
+    if args.arith != "reference":
+        print("[bench] the same run in the reference's arithmetic", file=sys.stderr, flush=True)
+        ref_arith = arith_run(G, _lib, w, "reference", args.warmup, args.steps, blocks, interval,
+                              sort_s)
+
+    hbm = None
+    if args.hbm_particles:
+        print(f"[bench] {args.hbm_particles} particles (state larger than the caches)",
+              file=sys.stderr, flush=True)
+        wh = workloads.weibel_1d2v(args.hbm_particles, seed=1)
+        hbm = arith_run(G, _lib, wh, args.arith, min(args.warmup, 20), args.hbm_steps, 0,
+                        interval, None)
+        hbm["workload"] = (f"1D2V Weibel, {args.hbm_particles} particles (the particle state, "
+                           f"{args.hbm_particles * 40 / 2**30:.2f} GiB with the sort buffers' "
+                           f"halves, exceeds the 256 MB last-level cache)")
+        del wh
 
     substep = None
     if args.substep:
@@ -575,7 +651,12 @@ def main():
         "config": {"workload": "1D2V Weibel instability", "particles": n, "nx": w["nx"],
                    "spline_degree": w["degree"], "dt": dt, "sort_interval": interval,
                    "blocks": blocks, "kernels_per_step": args.fusion,
-                   "path": "fused Strang step"},
+                   "arithmetic": args.arith, "path": "fused Strang step"},
+        "sort": {"us_per_sort": 1e6 * sort_s, "interval": interval,
+                 "sorts_in_timed_window": sorts_in_window,
+                 "share_added_us_per_step": 1e6 * sort_share_s / args.steps,
+                 "window_ms_per_step": 1e3 * elapsed_window / args.steps,
+                 "note": "value and ms_per_step charge each step sort time / interval"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_pass1d fused Strang pass",
@@ -597,6 +678,10 @@ def main():
         line["pieces_2d"] = p2d
     if substep is not None:
         line["substep_path_value"] = substep
+    if ref_arith is not None:
+        line["reference_arithmetic"] = ref_arith
+    if hbm is not None:
+        line["hbm_16m"] = hbm
     print(json.dumps(line))
 
 

@@ -136,6 +136,7 @@ struct gempic_splitting {
     int override_blocks = 0;          // gempic_set_launch
     int cus = 256;
     int kernels_per_step = 1;         // fused Strang step in 1 or 2 particle kernels
+    int arith = GEMPIC_ARITH_REFERENCE;  // arithmetic of the 1d2v passes of strang_splitting and the operators
     int64_t sort_interval = 20;
     int64_t steps_since_sort = 0;
     bool particles_set = false;
@@ -147,6 +148,7 @@ struct gempic_splitting {
     double *d_agrid = nullptr;
     double *d_wsnap = nullptr;        // weights of every sort epoch
     int *d_perm = nullptr;            // each epoch's sort: perm[new index] = old index
+    int64_t epoch_cap = 0;            // sort epochs d_wsnap and d_perm hold
     int64_t *d_ids = nullptr;         // the particle order (id) the call started from
     size_t hist_cap = 0, ckpt_cap = 0, seg_cap = 0, ids_cap = 0, wsnap_cap = 0, perm_cap = 0;
     int *cur_perm = nullptr;          // SortArgs::perm of the next sort
@@ -427,6 +429,22 @@ int gempic_device_count(void)
     return n;
 }
 
+int gempic_device_info(int *cus, char *arch, int arch_len)
+{
+    int rc = check_device();
+    if (rc) return rc;
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+        return fail(GEMPIC_ERR_DEVICE, "hipGetDeviceProperties failed");
+    if (cus) *cus = prop.multiProcessorCount;
+    if (arch && arch_len > 0) {
+        std::strncpy(arch, prop.gcnArchName, (size_t)arch_len - 1);
+        arch[arch_len - 1] = '\0';
+    }
+    return GEMPIC_OK;
+}
+
 int gempic_splitting_create(int64_t nx, double xmin, double xmax, int maxwell_degree, int degree0,
                             int smoothing0, int degree1, int smoothing1, int dim_v,
                             int64_t n_particles, int n_weights, double charge, double mass,
@@ -686,6 +704,9 @@ int gempic_get_j(gempic_splitting *h, double *j1, double *j2)
     return finish(h);
 }
 
+// the mode bit of the handle's arithmetic for its 1d2v particle passes
+static int fm(const gempic_splitting *h) { return h->arith == GEMPIC_ARITH_FUSED ? M_FAST : 0; }
+
 int gempic_operator_hb(gempic_splitting *h, double dt)
 {
     CHECK_H(h);
@@ -705,7 +726,7 @@ int gempic_operator_he(gempic_splitting *h, double dt)
     CHECK_H(h);
     CHECK_P(h);
     if (h->dim_v != 2) return fail(GEMPIC_ERR_ARGUMENT, "MethodError: operatorHE is defined for 1d2v only");
-    int rc = run_pass(h, M_HE, dt, 0.0, 0.0, 0, 2);
+    int rc = run_pass(h, fm(h) | M_HE, dt, 0.0, 0.0, 0, 2);
     if (rc) return rc;
     return finish(h);
 }
@@ -715,7 +736,7 @@ int gempic_operator_hp2(gempic_splitting *h, double dt)
     CHECK_H(h);
     CHECK_P(h);
     if (h->dim_v != 2) return fail(GEMPIC_ERR_ARGUMENT, "MethodError: operatorHp2 is defined for 1d2v only");
-    int rc = run_pass(h, M_HP2A, 0.0, dt, 0.0, 0, 3);
+    int rc = run_pass(h, fm(h) | M_HP2A, 0.0, dt, 0.0, 0, 3);
     if (rc) return rc;
     return finish(h);
 }
@@ -724,7 +745,7 @@ int gempic_operator_hp1(gempic_splitting *h, double dt)
 {
     CHECK_H(h);
     CHECK_P(h);
-    int mode = h->dim_v == 2 ? M_HP1 : (M_V1 | M_HP1);
+    int mode = h->dim_v == 2 ? (fm(h) | M_HP1) : (M_V1 | M_HP1);
     int rc = run_pass(h, mode, 0.0, 0.0, dt, 0, 4);
     if (rc) return rc;
     return finish(h);
@@ -750,11 +771,11 @@ int gempic_strang_splitting(gempic_splitting *h, double dt, int64_t n_steps, int
             for (int64_t s = 0; s < n_steps && !rc; ++s) {
                 rc = maybe_sort(h);
                 if (!rc) rc = run_grid(h, prog1(G_HB, hdt, S_B, S_B, S_E2));
-                if (!rc) rc = run_pass(h, M_HE, hdt, 0, 0, 0, 2);
-                if (!rc) rc = run_pass(h, M_HP2A, 0, hdt, 0, 0, 3);
-                if (!rc) rc = run_pass(h, M_HP1, 0, 0, 1.0 * dt, 0, 4);
-                if (!rc) rc = run_pass(h, M_HP2A, 0, hdt, 0, 0, 3);
-                if (!rc) rc = run_pass(h, M_HE, hdt, 0, 0, 0, 2);
+                if (!rc) rc = run_pass(h, fm(h) | M_HE, hdt, 0, 0, 0, 2);
+                if (!rc) rc = run_pass(h, fm(h) | M_HP2A, 0, hdt, 0, 0, 3);
+                if (!rc) rc = run_pass(h, fm(h) | M_HP1, 0, 0, 1.0 * dt, 0, 4);
+                if (!rc) rc = run_pass(h, fm(h) | M_HP2A, 0, hdt, 0, 0, 3);
+                if (!rc) rc = run_pass(h, fm(h) | M_HE, hdt, 0, 0, 0, 2);
                 if (!rc) rc = run_grid(h, prog1(G_HB, hdt, S_B, S_B, S_E2));
                 h->steps_since_sort++;
             }
@@ -768,7 +789,7 @@ int gempic_strang_splitting(gempic_splitting *h, double dt, int64_t n_steps, int
                 const int prev = s == 0 ? 0 : M_HE_PREV;
                 const int more = s + 1 < n_steps;
                 if (h->kernels_per_step == 1) {
-                    const int mode = prev | M_HE | M_HP2A | M_HP1 | M_HP2B;
+                    const int mode = fm(h) | prev | M_HE | M_HP2A | M_HP1 | M_HP2B;
                     if (!rc) rc = run_pass(h, mode, hdt, hdt, 1.0 * dt, more, 0);
                 } else {
                     // HE pair + first Hp2, then Hp1 + second Hp2 with the field solves;
@@ -778,7 +799,7 @@ int gempic_strang_splitting(gempic_splitting *h, double dt, int64_t n_steps, int
                 }
                 h->steps_since_sort++;
             }
-            if (!rc) rc = run_pass(h, M_HE_PREV, hdt, 0, 0, 0, 1);
+            if (!rc) rc = run_pass(h, fm(h) | M_HE_PREV, hdt, 0, 0, 0, 1);
         }
     } else {
         if (substep) {
@@ -862,24 +883,49 @@ int adj_epi(gempic_splitting *h, int kind, const AdjArgs &A, int blocks)
     return GEMPIC_OK;
 }
 
+// Sort epochs a reverse-mode call of T steps with checkpoints every K can
+// start: the first at step 1, then one per sort. Sorts run only where an
+// interval starts after the first (intervals 1 .. nck-1), the first of them
+// as early as interval 1 (the handle may enter the call with a sort due), the
+// others at least ceil(sort_interval / K) intervals apart; none when the
+// handle never sorts.
+int64_t adj_epoch_bound(const gempic_splitting *h, int64_t T, int64_t K)
+{
+    const int64_t nck = (T + K - 1) / K;
+    if (h->sort_interval <= 0 || nck < 2) return 1;
+    const int64_t gap = (h->sort_interval + K - 1) / K;
+    return std::min<int64_t>(nck, 2 + (nck - 2) / gap);
+}
+
+// the epoch about to start at a due sort must fit the reserved epochs
+int adj_epoch_check(const gempic_splitting *h, int64_t ne)
+{
+    const bool due = h->sort_interval > 0 && h->steps_since_sort >= h->sort_interval;
+    if (due && ne >= h->epoch_cap)
+        return fail(GEMPIC_ERR_ASSERTION, "AssertionError: more sort epochs than reserved in the reverse mode");
+    return GEMPIC_OK;
+}
+
 int adj_workspace(gempic_splitting *h, int64_t T, int64_t K)
 {
     const size_t n = (size_t)h->n, nx = (size_t)h->nx;
     const size_t nck = (size_t)((T + K - 1) / K);
+    const size_t nep = (size_t)adj_epoch_bound(h, T, K);
     int rc = grow(&h->d_hist, &h->hist_cap, (size_t)(T + 2) * NHIST * nx);
     if (!rc) rc = grow(&h->d_ckpt, &h->ckpt_cap, nck * 3 * n);
-    if (!rc) rc = grow(&h->d_wsnap, &h->wsnap_cap, nck * n);
+    if (!rc) rc = grow(&h->d_wsnap, &h->wsnap_cap, nep * n);
     if (!rc && h->ids_cap < n) {
         HIP_TRY(dalloc(&h->d_ids, n));
         h->ids_cap = n;
     }
-    if (!rc && h->perm_cap < nck * n) {
+    if (!rc && h->perm_cap < nep * n) {
         if (h->d_perm) (void)hipFree(h->d_perm);
         h->d_perm = nullptr;
         h->perm_cap = 0;
-        HIP_TRY(dalloc(&h->d_perm, nck * n));
-        h->perm_cap = nck * n;
+        HIP_TRY(dalloc(&h->d_perm, nep * n));
+        h->perm_cap = nep * n;
     }
+    if (!rc) h->epoch_cap = (int64_t)nep;
     if (!rc && K > 1) rc = grow(&h->d_seg, &h->seg_cap, (size_t)(K - 1) * 3 * n);
     if (!rc && !h->d_lam) HIP_TRY(dalloc(&h->d_lam, 3 * n));
     if (!rc && !h->d_agrid) HIP_TRY(dalloc(&h->d_agrid, (size_t)NASLOT * nx));
@@ -946,6 +992,8 @@ int gempic_strang_splitting_adjoint(gempic_splitting *h, double dt, int64_t n_st
             const size_t j = (size_t)((s - 1) / K);
             bool sorted = false;
             if (s > 1) {
+                rc = adj_epoch_check(h, ne);
+                if (rc) break;
                 h->cur_perm = h->d_perm + (size_t)ne * n;     // this epoch's, if a sort runs
                 rc = maybe_sort(h);
                 h->cur_perm = nullptr;
@@ -1623,6 +1671,10 @@ int gempic_ensemble_strang_splitting_adjoint(gempic_ensemble *e, const double *d
             const size_t j = (size_t)((s - 1) / K);
             std::vector<char> sorted((size_t)ni, 0);
             if (s > 1) {
+                for (int i = 0; i < ni; ++i) {
+                    rc = adj_epoch_check(e->hs[i], st[i].ne);
+                    if (rc) return rc;
+                }
                 for (int i = 0; i < ni; ++i)
                     e->hs[i]->cur_perm = e->hs[i]->d_perm + (size_t)st[i].ne * e->hs[i]->n;
                 bool any = false;
@@ -1864,7 +1916,22 @@ int gempic_set_fusion(gempic_splitting *h, int kernels_per_step)
     CHECK_H(h);
     if (kernels_per_step != 1 && kernels_per_step != 2)
         return fail(GEMPIC_ERR_ARGUMENT, "kernels_per_step must be 1 or 2");
+    if (kernels_per_step == 2 && h->arith != GEMPIC_ARITH_REFERENCE)
+        return fail(GEMPIC_ERR_ARGUMENT, "the two-kernel step runs in the reference's arithmetic only");
     h->kernels_per_step = kernels_per_step;
+    return GEMPIC_OK;
+}
+
+int gempic_set_arithmetic(gempic_splitting *h, int arith)
+{
+    CHECK_H(h);
+    if (arith != GEMPIC_ARITH_REFERENCE && arith != GEMPIC_ARITH_FUSED)
+        return fail(GEMPIC_ERR_ARGUMENT, "arithmetic must be GEMPIC_ARITH_REFERENCE or GEMPIC_ARITH_FUSED");
+    if (arith == GEMPIC_ARITH_FUSED && h->dim_v != 2)
+        return fail(GEMPIC_ERR_ARGUMENT, "fused arithmetic covers the 1d2v passes");
+    if (arith == GEMPIC_ARITH_FUSED && h->kernels_per_step != 1)
+        return fail(GEMPIC_ERR_ARGUMENT, "fused arithmetic needs the one-kernel fused step");
+    h->arith = arith;
     return GEMPIC_OK;
 }
 
@@ -1958,10 +2025,13 @@ int gempic_autotune(gempic_splitting *h, double dt, int64_t n_steps, const int *
                     rc = fail(GEMPIC_ERR_DEVICE, std::string("autotune: ") + hipGetErrorString(e));
                     break;
                 }
-                rc = gempic_strang_splitting(h, dt, n_steps, 0);
+                // queued without a host round trip, so the events time the GPU
+                // work only; the deferred error check follows the timing
+                rc = gempic_strang_splitting(h, dt, n_steps, GEMPIC_ASYNC);
                 if (!rc) e = hipEventRecord(ev[1], h->stream);
                 if (!rc && e == hipSuccess) e = hipEventSynchronize(ev[1]);
                 if (!rc && e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
+                if (!rc && e == hipSuccess) rc = finish(h);
                 if (!rc && e == hipSuccess) e = copy_state(false);
                 if (!rc && e != hipSuccess)
                     rc = fail(GEMPIC_ERR_DEVICE, std::string("autotune: ") + hipGetErrorString(e));

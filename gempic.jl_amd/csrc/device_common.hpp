@@ -14,18 +14,22 @@ namespace gk {
 
 constexpr int kWave = 64;
 
-// Base.mod(x::Float64, y::Float64) of Julia: r = rem(x, y) (exact);
-// zero takes y's sign; a nonzero r of the other sign gets y added.
-// Used at hamiltonian_splitting_1d2v.jl:79 and hamiltonian_splitting_1d1v.jl:91.
-// For y > 0 and |x| < 2y, rem(x, y) is x or x - sign(x)*y, and the latter is
-// exact (Sterbenz: y <= |x| < 2y), so the library fmod is only needed for
-// positions more than one period away.
-// a / b, correctly rounded, for a divisor b whose reciprocal rb = 1 / b
-// (correctly rounded) is known: q0 = a * rb is within one ulp of a / b, the
-// remainder a - q0 * b is exact as an fma, and q0 + r * rb rounds to the
-// correctly rounded quotient (Markstein, IBM J. Res. Develop. 34 (1990)
-// 111-119). Holds while a / b is a normal number; 3 FP64 instructions
-// against about 10 for an IEEE division.
+// a / b for a divisor b whose correctly rounded reciprocal rb = RN(1 / b) is
+// known, in 3 FP64 instructions against about 10 for an IEEE division
+// (Markstein's correction step, IBM J. Res. Develop. 34 (1990) 111-119):
+//   q0 = RN(a * rb), at most 1.5 ulp from a / b (rb's half-ulp error, scaled
+//        by a, can reach one ulp of the quotient, plus the rounding of q0);
+//   r  = a - q0 * b, exact as an fma while a / b is a normal number;
+//   q1 = RN(q0 + r * rb). Before its rounding, q0 + r * rb = a / b + (a / b - q0) * (b * rb - 1),
+//        an error of at most 1.5 ulp * 2^-53.
+// q1 is therefore the correctly rounded quotient unless a / b lies within
+// 1.5 * 2^-53 ulp of a midpoint between two doubles. Such quotients exist
+// only near a midpoint a / b cannot reach exactly (it lies at least 2^-54 ulp
+// from every midpoint), and none has been found: the quotient tests compare
+// this form with IEEE division on 32 million generic quotients and on the
+// region where the bound is loosest (b's significand near 2, a / b's
+// significand near 2), and the parity tests compare positions and cells
+// with the oracle's IEEE division bit for bit.
 __device__ __forceinline__ double div_by(double a, double b, double rb)
 {
     const double q0 = a * rb;
@@ -33,6 +37,12 @@ __device__ __forceinline__ double div_by(double a, double b, double rb)
     return __builtin_fma(r, rb, q0);
 }
 
+// Base.mod(x::Float64, y::Float64) of Julia: r = rem(x, y) (exact);
+// zero takes y's sign; a nonzero r of the other sign gets y added.
+// Used at hamiltonian_splitting_1d2v.jl:79 and hamiltonian_splitting_1d1v.jl:91.
+// For y > 0 and |x| < 2y, rem(x, y) is x or x - sign(x)*y, and the latter is
+// exact (Sterbenz: y <= |x| < 2y), so the library fmod is only needed for
+// positions more than one period away.
 __device__ __forceinline__ double jl_mod(double x, double y)
 {
     double r;
@@ -112,6 +122,82 @@ __device__ __forceinline__ void bspline_pair(double off, double (&b0)[P0 + 1], d
     } else {
         bspline<P1>(off, b1);
         bspline<P0>(off, b0);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fused arithmetic (the opt-in fast mode of the particle passes): the same
+// quantities as the reference's recurrences, from closed polynomial forms
+// evaluated with fma. They agree with the recurrences to a few ulp, not
+// bitwise.
+// ---------------------------------------------------------------------------
+
+// The P+1 uniform B-splines of degree P <= 3 at offset t in [0, 1], in the
+// order of uniform_bsplines_eval_basis! (b[0] is the spline ending at t = 1):
+//   P = 1: 1 - t, t
+//   P = 2: s^2/2, 1/2 + t s, t^2/2                        (s = 1 - t)
+//   P = 3: s^3/6, 2/3 + t^2 (t/2 - 1), 2/3 + s^2 (s/2 - 1), t^3/6
+template <int P>
+__device__ __forceinline__ void bspline_fast(double t, double (&b)[P + 1])
+{
+    static_assert(P >= 0 && P <= 3, "closed forms for degrees 0 to 3");
+    if constexpr (P == 0) {
+        b[0] = 1.0;
+    } else if constexpr (P == 1) {
+        b[0] = 1.0 - t;
+        b[1] = t;
+    } else if constexpr (P == 2) {
+        const double s = 1.0 - t;
+        b[0] = 0.5 * (s * s);
+        b[1] = __builtin_fma(t, s, 0.5);
+        b[2] = 0.5 * (t * t);
+    } else {
+        const double s = 1.0 - t, t2 = t * t, s2 = s * s;
+        constexpr double c6 = 1.0 / 6.0, c23 = 2.0 / 3.0;
+        b[0] = (s2 * s) * c6;
+        b[1] = __builtin_fma(t2, __builtin_fma(t, 0.5, -1.0), c23);
+        b[2] = __builtin_fma(s2, __builtin_fma(s, 0.5, -1.0), c23);
+        b[3] = (t2 * t) * c6;
+    }
+}
+
+// Both bases of a smoother pair from one offset (closed forms; the squares
+// are shared when P1 = P0 - 1)
+template <int P0, int P1>
+__device__ __forceinline__ void bspline_pair_fast(double t, double (&b0)[P0 + 1], double (&b1)[P1 + 1])
+{
+    if constexpr (P0 == 3 && P1 == 2) {
+        const double s = 1.0 - t, t2 = t * t, s2 = s * s;
+        constexpr double c6 = 1.0 / 6.0, c23 = 2.0 / 3.0;
+        b0[0] = (s2 * s) * c6;
+        b0[1] = __builtin_fma(t2, __builtin_fma(t, 0.5, -1.0), c23);
+        b0[2] = __builtin_fma(s2, __builtin_fma(s, 0.5, -1.0), c23);
+        b0[3] = (t2 * t) * c6;
+        b1[0] = 0.5 * s2;
+        b1[1] = __builtin_fma(t, s, 0.5);
+        b1[2] = 0.5 * t2;
+    } else {
+        bspline_fast<P0>(t, b0);
+        bspline_fast<P1>(t, b1);
+    }
+}
+
+// Second derivatives of those splines (zero below degree 2)
+template <int P>
+__device__ __forceinline__ void bspline_d2_fast(double t, double (&d)[P + 1])
+{
+    if constexpr (P < 2) {
+#pragma unroll
+        for (int k = 0; k <= P; ++k) d[k] = 0.0;
+    } else if constexpr (P == 2) {
+        d[0] = 1.0;
+        d[1] = -2.0;
+        d[2] = 1.0;
+    } else {
+        d[0] = 1.0 - t;
+        d[1] = __builtin_fma(3.0, t, -2.0);
+        d[2] = __builtin_fma(-3.0, t, 1.0);
+        d[3] = t;
     }
 }
 

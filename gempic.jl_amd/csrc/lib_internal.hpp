@@ -35,6 +35,9 @@ enum ModeBits : int {
     M_DIAG = 128,    // write_step! particle sums
     M_NOEPI = 256,   // write the deposit rows and stop (the next pass reduces them)
     M_WITH_J2A = 512,  // also reduce the Hp2 rows written by the previous pass
+    M_FAST = 1024,   // fused arithmetic: closed-form spline bases, fma, the line integral
+                     // by its midpoint rule with the curvature term (agrees with the
+                     // reference to rounding, not bitwise)
 };
 
 struct PassArgs {

@@ -344,6 +344,41 @@ __device__ __forceinline__ void segment(double lower, double upper, double sign,
     for (int k = 0; k <= P1; ++k) sv[k] = sv[k] * s1;
 }
 
+// The same segment in fused arithmetic: for splines of degree P1 <= 3 the
+// integral over [lower, upper] is exactly d (S(m) + d^2/24 S''(m)), d = upper -
+// lower, m the midpoint (the Gauss-Legendre rule of the reference is exact for
+// these polynomials too, so both give the same value up to rounding).
+template <int P1>
+__device__ __forceinline__ void segment_fast(double lower, double upper, double sign, double dx,
+                                             double (&sv)[P1 + 1])
+{
+    const double d = upper - lower;
+    const double m = __builtin_fma(0.5, d, lower);
+    bspline_fast<P1>(m, sv);
+    const double f = (sign * dx) * d;
+    if constexpr (P1 >= 2) {
+        double c[P1 + 1];
+        bspline_d2_fast<P1>(m, c);
+        const double h = (d * d) * (1.0 / 24.0);
+#pragma unroll
+        for (int k = 0; k <= P1; ++k) sv[k] = __builtin_fma(h, c[k], sv[k]) * f;
+    } else {
+#pragma unroll
+        for (int k = 0; k <= P1; ++k) sv[k] = sv[k] * f;
+    }
+}
+
+// gather_p in fused arithmetic
+template <int P>
+__device__ __forceinline__ double gather_pf(const double (&b)[P + 1], const double *fp, int cell)
+{
+    const double *f = fp + (cell - P);
+    double v = f[0] * b[0];
+#pragma unroll
+    for (int i = 1; i <= P; ++i) v = __builtin_fma(f[i], b[i], v);
+    return v;
+}
+
 // Segment s of add_current_update_v! (:296-376 / :471-529): the old cell,
 // the new cell, then the cells in between, each with its integration bounds.
 __device__ __forceinline__ void segment_bounds(int s, int io, double ro, int inw, double rn,
@@ -381,6 +416,8 @@ __device__ __forceinline__ void pass1d_body(const PassArgs &a, int bid, double *
     constexpr bool HP2B = (MODE & M_HP2B) != 0;
     constexpr bool V1 = (MODE & M_V1) != 0;
     constexpr bool RHO_ONLY = (MODE & M_RHO) != 0;
+    constexpr bool FAST = (MODE & M_FAST) != 0;
+    static_assert(!FAST || (!V1 && !RHO_ONLY), "fused arithmetic covers the 1d2v sub-steps");
     // operatorHp1 (1d2v) also deposits rho into j_dofs[2]
     // (hamiltonian_splitting_1d2v.jl:90-108); when Hp2 follows in the same pass
     // it clears j_dofs[2] first (:132-133) and nothing reads the rho between, so
@@ -470,7 +507,8 @@ __device__ __forceinline__ void pass1d_body(const PassArgs &a, int bid, double *
 
         double b1[W1];
         double b0[W0];
-        if constexpr ((HE_PREV || HE || HP2A) && !V1) bspline_pair<P0, P1>(off, b0, b1);
+        if constexpr (FAST && (HE_PREV || HE || HP2A)) bspline_pair_fast<P0, P1>(off, b0, b1);
+        else if constexpr ((HE_PREV || HE || HP2A) && !V1) bspline_pair<P0, P1>(off, b0, b1);
         else if constexpr (HE_PREV || HE || HP2A) bspline<P1>(off, b1);
 
         if constexpr (V1 && (HE_PREV || HE)) {
@@ -479,7 +517,21 @@ __device__ __forceinline__ void pass1d_body(const PassArgs &a, int bid, double *
             if (HE_PREV) v1 = v1 + a.dt_he * e1;
             if (HE) v1 = v1 + a.dt_he * e1;
         }
-        if constexpr (!V1 && (HE_PREV || HE)) {
+        if constexpr (FAST && (HE_PREV || HE)) {
+            // operatorHE 1d2v in fused arithmetic
+            const double e1 = gather_pf<P1>(b1, sE1, key_old);
+            const double dtqm = a.dtqm_he;
+            if (HE_PREV) {
+                const double e2 = gather_pf<P0>(b0, sE2p, key_old);
+                v1 = __builtin_fma(dtqm, e1, v1);
+                v2 = __builtin_fma(dtqm, e2, v2);
+            }
+            if (HE) {
+                const double e2 = gather_pf<P0>(b0, sE2, key_old);
+                v1 = __builtin_fma(dtqm, e1, v1);
+                v2 = __builtin_fma(dtqm, e2, v2);
+            }
+        } else if constexpr (!V1 && (HE_PREV || HE)) {
             // operatorHE 1d2v (hamiltonian_splitting_1d2v.jl:198-215)
             const double e1 = gather_p<P1>(b1, sE1, key_old);
             const double dtqm = a.dtqm_he;
@@ -496,19 +548,27 @@ __device__ __forceinline__ void pass1d_body(const PassArgs &a, int bid, double *
         }
         if constexpr (HP2A) {
             // operatorHp2 (hamiltonian_splitting_1d2v.jl:146-163)
-            const double bf = gather_p<P1>(b1, sB, key_old);
-            v1 = v1 + a.dtqm_hp2 * v2 * bf;
-            const double wq = q * v2;
             double vals[W0];
+            if constexpr (FAST) {
+                const double bf = gather_pf<P1>(b1, sB, key_old);
+                v1 = __builtin_fma(a.dtqm_hp2 * v2, bf, v1);
+                const double ws = (q * v2) * a.scaling0;
 #pragma unroll
-            for (int k = 0; k < W0; ++k) vals[k] = wq * b0[k] * a.scaling0;
+                for (int k = 0; k < W0; ++k) vals[k] = ws * b0[k];
+            } else {
+                const double bf = gather_p<P1>(b1, sB, key_old);
+                v1 = v1 + a.dtqm_hp2 * v2 * bf;
+                const double wq = q * v2;
+#pragma unroll
+                for (int k = 0; k < W0; ++k) vals[k] = wq * b0[k] * a.scaling0;
+            }
             cache_add<W0>(c_j2a, valid, key_old, vals, myrow + D_J2A * nx, nx);
         }
         double xi_new = 0.0;       // (x - xmin) / dx after Hp1, when the mod kept x_new
         bool xi_new_ok = false;
         if constexpr (HP1) {
             // operatorHp1 (hamiltonian_splitting_1d2v.jl:53-82 / _1d1v.jl:308-329)
-            const double xn = x + a.dt_hp1 * v1;
+            const double xn = FAST ? __builtin_fma(a.dt_hp1, v1, x) : x + a.dt_hp1 * v1;
             const double xin = div_by(xn - xmin, dx, rdx);
             if (valid && !pos_ok(xin)) errs |= E_INEXACT;
             const int inw = V1 ? (int)floor(xin) : (int)xin;
@@ -527,15 +587,27 @@ __device__ __forceinline__ void pass1d_body(const PassArgs &a, int bid, double *
                 int cell;
                 segment_bounds(s, ic, off, inw, rn, lo, up, sg, cell);
                 double sv[W1];
-                segment<P1, NQ>(lo, up, sg, a.qx, a.qw, dx, sv);
+                if constexpr (FAST) segment_fast<P1>(lo, up, sg, dx, sv);
+                else segment<P1, NQ>(lo, up, sg, a.qx, a.qw, dx, sv);
                 const int key = imod(cell, nx);
                 double vals[W1];
                 const double *bk = sB + (key - P1);
                 double v2s = v2;
+                if constexpr (FAST) {
+                    const double qs = q * a.scaling1;
+                    double sb = sv[0] * bk[0];
 #pragma unroll
-                for (int m = 0; m < W1; ++m) {
-                    vals[m] = q * sv[m] * a.scaling1;
-                    if (!V1) v2s = v2s - qm * sv[m] * bk[m];
+                    for (int m = 0; m < W1; ++m) {
+                        vals[m] = qs * sv[m];
+                        if (m > 0) sb = __builtin_fma(sv[m], bk[m], sb);
+                    }
+                    v2s = __builtin_fma(-qm, sb, v2);
+                } else {
+#pragma unroll
+                    for (int m = 0; m < W1; ++m) {
+                        vals[m] = q * sv[m] * a.scaling1;
+                        if (!V1) v2s = v2s - qm * sv[m] * bk[m];
+                    }
                 }
                 if (act) v2 = v2s;
                 cache_add<W1>(c_j1, act, key, vals, myrow + D_J1 * nx, nx);
@@ -555,17 +627,29 @@ __device__ __forceinline__ void pass1d_body(const PassArgs &a, int bid, double *
             const int key2 = imod(ic2, nx);
             double c0[W0];
             double c1[W1];
-            if constexpr (HP2B) bspline_pair<P0, P1>(off2, c0, c1);
+            if constexpr (FAST) bspline_pair_fast<P0, P1>(off2, c0, c1);
+            else if constexpr (HP2B) bspline_pair<P0, P1>(off2, c0, c1);
             else bspline<P0>(off2, c0);
             if constexpr (HP2B) {
-                const double bf = gather_p<P1>(c1, sB, key2);
-                v1 = v1 + a.dtqm_hp2 * v2 * bf;
+                if constexpr (FAST) {
+                    const double bf = gather_pf<P1>(c1, sB, key2);
+                    v1 = __builtin_fma(a.dtqm_hp2 * v2, bf, v1);
+                } else {
+                    const double bf = gather_p<P1>(c1, sB, key2);
+                    v1 = v1 + a.dtqm_hp2 * v2 * bf;
+                }
             }
             // rho: q * S0; Hp2's j2: (q * v2) * S0 (particle_mesh_coupling_1d.jl:120)
             const double wq = HP2B ? q * v2 : q;
             double vals[W0];
+            if constexpr (FAST) {
+                const double ws = wq * a.scaling0;
 #pragma unroll
-            for (int k = 0; k < W0; ++k) vals[k] = wq * c0[k] * a.scaling0;
+                for (int k = 0; k < W0; ++k) vals[k] = ws * c0[k];
+            } else {
+#pragma unroll
+                for (int k = 0; k < W0; ++k) vals[k] = wq * c0[k] * a.scaling0;
+            }
             cache_add<W0>(c_new, valid, key2, vals, myrow + NEW_ROW * nx, nx);
         }
         if (valid) {
@@ -955,6 +1039,7 @@ __global__ void __launch_bounds__(kBlock) k_diag1d2v(PassArgs a)
     }
     __syncthreads();
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    int errs = 0;
     const int64_t base = (int64_t)blockIdx.x * a.chunk;
     const double rdx = 1.0 / a.dx;   // div_by: (x - xmin) / dx, correctly rounded
     for (int it = 0; it < a.iters; ++it) {
@@ -965,7 +1050,11 @@ __global__ void __launch_bounds__(kBlock) k_diag1d2v(PassArgs a)
         acc[0] += (v1 * v1 + v2 * v2) * wm;
         acc[1] += v1 * wm;
         acc[2] += v2 * wm;
-        const double xi = div_by(x - a.xmin, a.dx, rdx);
+        double xi = div_by(x - a.xmin, a.dx, rdx);
+        if (!pos_ok(xi)) {      // evaluate's trunc(Int, xi) throws InexactError (particle_mesh_coupling_1d.jl:438-453)
+            errs |= E_INEXACT;
+            xi = 0.0;
+        }
         const int ic = (int)xi;
         const double off = xi - (double)ic;
         const int key = imod(ic, nx);
@@ -980,6 +1069,7 @@ __global__ void __launch_bounds__(kBlock) k_diag1d2v(PassArgs a)
         const double qb = (w * a.charge) * a.cw;
         acc[4] += qb * v1 * v2 * bf;
     }
+    if (errs) atomicOr(a.err, errs);
     // deterministic block sum: wave butterflies, then waves in order
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
@@ -1063,6 +1153,7 @@ static hipError_t launch_epi_pd(int mode, const PassArgs &a, int pass_blocks, hi
 hipError_t launch_epi(int p0, int p1, int mode, const PassArgs &a, int pass_blocks, hipStream_t s)
 {
     if (a.nx > kNxMax || pass_blocks < 1) return hipErrorInvalidValue;
+    mode &= ~M_FAST;                  // the field part is the same in either arithmetic
 #define GK_PAIR(A, B) \
     if (p0 == (A) && p1 == (B)) return launch_epi_pd<A, B>(mode, a, pass_blocks, s);
     GK_PAIR(1, 0)
@@ -1161,6 +1252,12 @@ static hipError_t launch_pass_pd(int mode, const PassArgs &a, int blocks, size_t
         GK_CASE(M_V1 | M_HE_PREV | M_HE | M_HP1)
         GK_CASE(M_V1 | M_HE_PREV)
         GK_CASE(M_RHO)
+        GK_CASE(M_FAST | M_HE)
+        GK_CASE(M_FAST | M_HP2A)
+        GK_CASE(M_FAST | M_HP1)
+        GK_CASE(M_FAST | M_HE | M_HP2A | M_HP1 | M_HP2B)
+        GK_CASE(M_FAST | M_HE_PREV | M_HE | M_HP2A | M_HP1 | M_HP2B)
+        GK_CASE(M_FAST | M_HE_PREV)
     case M_DIAG:
         if (blocks < 0) {
             int nb = 0;

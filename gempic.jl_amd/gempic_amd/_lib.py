@@ -109,6 +109,8 @@ SIGNATURES = {
                                        _VP, _I, _VP]),
     "gempic_pmc2d_evaluate_multiple_dev": (_I, [_I64, _I64, _D, _D, _D, _D, _I, _I, _I64, _VP, _VP,
                                                 _VP, _VP, _VP, _VP, _I, _VP]),
+    "gempic_set_arithmetic": (_I, [_VP, _I]),
+    "gempic_device_info": (_I, [ctypes.POINTER(ctypes.c_int), ctypes.c_char_p, _I]),
     "gempic_pmc2d_sync": (_I, [_VP]),
     "gempic_pmc2d_add_charge_dev_noalloc": (_I, [_I64, _I64, _D, _D, _D, _D, _I, _I, _I64, _VP,
                                                  _VP, _VP, _VP, _I, _VP]),

@@ -342,11 +342,23 @@ AUTOTUNE_BLOCKS = (0, 512, 1024, 2048)
 AUTOTUNE_SORT_INTERVALS = (10, 20, 40, 80)
 
 
-def autotune_key(V, n, nx, deg0, deg1, dt, n_steps, blocks, sort_intervals) -> str:
-    """Cache key of one autotuning problem: everything the timing depends on."""
+def autotune_key(V, n, nx, deg0, deg1, dt, n_steps, blocks, sort_intervals,
+                 kernels_per_step=1, arithmetic="reference", device="") -> str:
+    """Cache key of one autotuning problem: everything the timing depends on
+    (the problem, the candidates, the fused step's kernel split and
+    arithmetic, and the GPU: its architecture and compute-unit count)."""
     return (f"V={int(V)},n={int(n)},nx={int(nx)},deg={int(deg0)}/{int(deg1)},dt={float(dt)!r},"
             f"steps={int(n_steps)},blocks={list(map(int, blocks))},"
-            f"intervals={list(map(int, sort_intervals))}")
+            f"intervals={list(map(int, sort_intervals))},kernels={int(kernels_per_step)},"
+            f"arith={arithmetic},device={device}")
+
+
+def device_info() -> str:
+    """The current GPU as "<arch>/<CUs>cu" (the architecture name HIP reports)."""
+    arch = ctypes.create_string_buffer(64)
+    cus = ctypes.c_int()
+    check(_lib.load().gempic_device_info(ctypes.byref(cus), arch, 64))
+    return f"{arch.value.decode()}/{cus.value}cu"
 
 
 def autotune_cache_lookup(path: str, key: str):
@@ -418,6 +430,8 @@ class HamiltonianSplitting:
         nx = kernel_smoother_0.n_dofs
         self.j_dofs = [np.zeros(nx), np.zeros(nx)]
         self.sync = sync
+        self.kernels_per_step = 1
+        self.arithmetic = "reference"
         pg = particle_group
         mesh = maxwell_solver._mesh
         h = ctypes.c_void_p()
@@ -479,6 +493,22 @@ class HamiltonianSplitting:
         """Workgroups per particle pass (0 = automatic)."""
         check(_lib.load().gempic_set_launch(self._h, int(blocks)))
 
+    def set_fusion(self, kernels_per_step: int):
+        """The fused Strang step in one particle kernel (default) or two."""
+        check(_lib.load().gempic_set_fusion(self._h, int(kernels_per_step)))
+        self.kernels_per_step = int(kernels_per_step)
+
+    def set_arithmetic(self, arithmetic: str):
+        """"reference" (default): every product and sum rounded as in the
+        reference's Julia code, bitwise its per-particle results. "fused":
+        the 1d2v passes evaluate the same quantities with closed-form spline
+        bases and fused multiply-adds (equal to rounding, not bitwise)."""
+        code = {"reference": 0, "fused": 1}.get(arithmetic)
+        if code is None:
+            raise ValueError('arithmetic must be "reference" or "fused"')
+        check(_lib.load().gempic_set_arithmetic(self._h, code))
+        self.arithmetic = arithmetic
+
     def autotune(self, dt: float, n_steps: int | None = None,
                  blocks=AUTOTUNE_BLOCKS, sort_intervals=AUTOTUNE_SORT_INTERVALS,
                  cache: str | None = None) -> dict:
@@ -500,7 +530,7 @@ class HamiltonianSplitting:
         key = autotune_key(self.dims[1], self.particle_group.n_particles,
                            self.kernel_smoother_0.n_dofs, self.kernel_smoother_0.spline_degree,
                            self.kernel_smoother_1.spline_degree, dt, n_steps, blocks,
-                           sort_intervals)
+                           sort_intervals, self.kernels_per_step, self.arithmetic, device_info())
         if cache is not None:
             hit = autotune_cache_lookup(cache, key)
             if hit is not None:

@@ -53,6 +53,10 @@ const char *gempic_last_error(void);
 
 /* Number of visible GPUs (0 when none). */
 int gempic_device_count(void);
+/* The current GPU's compute units and architecture name (as HIP reports it,
+ * e.g. "gfx950:sramecc+:xnack-"); either may be NULL. No counterpart in the
+ * reference: the launch autotuner keys its cache on it. */
+int gempic_device_info(int *cus, char *arch, int arch_len);
 
 /* ------------------------------------------------------------------------
  * HamiltonianSplitting{1,V} (hamiltonian_splitting.jl:20-86) together with
@@ -227,6 +231,22 @@ int gempic_autotune(gempic_splitting *h, double dt, int64_t n_steps, const int *
  * (HE pair + first Hp2, then Hp1 + second Hp2 + field solves). Both give
  * bitwise the same results; two kernels hold fewer registers. Default 1. */
 int gempic_set_fusion(gempic_splitting *h, int kernels_per_step);
+
+/* Arithmetic of the 1d2v particle passes run by gempic_strang_splitting and
+ * the gempic_operator_* calls. GEMPIC_ARITH_REFERENCE (the default) rounds
+ * every product and sum as the reference's Julia code does
+ * (particle_mesh_coupling_1d.jl:261-453, low_level_bsplines.jl:63-80,
+ * hamiltonian_splitting_1d2v.jl:41-219): per-particle results are bitwise
+ * the reference's. GEMPIC_ARITH_FUSED has no counterpart in the reference: it
+ * evaluates the same quantities with fewer FP64 operations (closed-form
+ * spline bases, fused multiply-adds, each current segment's integral by its
+ * midpoint value plus the exact curvature term instead of Gauss-Legendre),
+ * which agree with the reference to rounding (a few ulp per operation), not
+ * bitwise. It needs the one-kernel fused step; the reverse mode, ensembles,
+ * the Boris splitting and 1d1v always use the reference's arithmetic. */
+#define GEMPIC_ARITH_REFERENCE 0
+#define GEMPIC_ARITH_FUSED 1
+int gempic_set_arithmetic(gempic_splitting *h, int arith);
 
 /* ------------------------------------------------------------------------
  * sample!(pg, alpha, k, sigma, mesh) (particle_sampling.jl:266-311) for

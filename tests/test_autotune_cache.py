@@ -10,7 +10,8 @@ def test_key_names_every_input():
     k0 = core.autotune_key(**base)
     for name, other in [("V", 1), ("n", 1048576), ("nx", 64), ("deg0", 2), ("deg1", 1),
                         ("dt", 0.1), ("n_steps", 40), ("blocks", [0, 1024]),
-                        ("sort_intervals", [20])]:
+                        ("sort_intervals", [20]), ("kernels_per_step", 2),
+                        ("arithmetic", "fused"), ("device", "gfx950/256cu")]:
         assert core.autotune_key(**dict(base, **{name: other})) != k0, name
     assert core.autotune_key(**base) == k0
 

@@ -4,7 +4,11 @@ parity with the oracle's plain `(x - xmin) / dx` rests on that being the
 correctly rounded quotient. This test checks it against IEEE division in C
 (the same FP64 operations, with fma from libm) on random quotients: positions
 across the domain, values near 0, arbitrary exponents, and near-multiples of
-dx (quotients next to an integer, where trunc() picks the cell).
+dx (quotients next to an integer, where trunc() picks the cell). A second
+generator draws from the region where the error bound of q0 is loosest: the
+divisor's significand just below 2 (so 1/dx's significand is just above 1
+and its half-ulp error is largest relative to the quotient) and the
+quotient's significand just below 2 as well.
 """
 import shutil
 import subprocess
@@ -57,6 +61,48 @@ def test_div_by_is_the_correctly_rounded_quotient(tmp_path):
     subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-o", str(exe), str(src), "-lm"],
                    check=True)
     out = subprocess.run([str(exe), "2000000"], check=True, capture_output=True, text=True)
+    assert int(out.stdout.strip()) == 0
+
+
+SRC_HARD = r"""
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+static uint64_t s = 0x9E3779B97F4A7C15ull;
+static uint64_t xs(void) { s ^= s << 13; s ^= s >> 7; s ^= s << 17; return s; }
+static double u01(void) { return (xs() >> 11) * 0x1.0p-53; }
+int main(int argc, char **argv) {
+    const long n = atol(argv[1]);
+    long bad = 0;
+    for (long i = 0; i < n; ++i) {
+        /* divisor: significand in [2 - 2^-8, 2), any exponent in [-8, 8] */
+        const double bsig = 2.0 - ldexp(u01(), -8);
+        const double b = ldexp(bsig, (int)(xs() % 17) - 8);
+        /* quotient target: significand in [2 - 2^-8, 2) */
+        const double t = ldexp(2.0 - ldexp(u01(), -8), (int)(xs() % 9) - 4);
+        double a = t * b;
+        if (xs() & 1) { uint64_t u; memcpy(&u, &a, 8); u += (xs() % 7) - 3; memcpy(&a, &u, 8); }
+        const double r = 1.0 / b;
+        const double q0 = a * r;
+        const double q = fma(fma(-q0, b, a), r, q0);
+        if (q != a / b) ++bad;
+    }
+    printf("%ld\n", bad);
+    return 0;
+}
+"""
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="needs gcc")
+def test_div_by_in_the_loosest_region(tmp_path):
+    src = tmp_path / "div_hard.c"
+    exe = tmp_path / "div_hard"
+    src.write_text(SRC_HARD)
+    subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-o", str(exe), str(src), "-lm"],
+                   check=True)
+    out = subprocess.run([str(exe), "16000000"], check=True, capture_output=True, text=True)
     assert int(out.stdout.strip()) == 0
 
 

@@ -84,6 +84,31 @@ def test_gpu_adjoint_matches_oracle(deg, K):
     assert abs(r.g_weight_sum - gw) <= REL * abs(gw), (r.g_weight_sum, gw)
 
 
+def test_gpu_adjoint_enters_with_a_sort_due():
+    """The handle enters the reverse mode with a sort already due, and the
+    sort interval spans two checkpoint intervals: a sort then runs at the
+    first interval boundary, so two sort epochs are needed although the
+    interval would allow only one sort after the first. The results match
+    the oracle's reverse mode from the same state."""
+    parts, e1, e2, b, L, nx = _hot(20000, seed=6)
+    dt, cw = 0.1, 0.01
+    h, pg = _gpu(parts, e1, e2, b, L, nx, (3, 2), cw, sync=False)
+    h.set_sort_interval(0)
+    G.strang_splitting_(h, dt, 5)                 # five steps since the upload's sort
+    h.to_host()                                   # (downloads do not sort)
+    start = pg.array.copy(order="F")
+    f0 = [h.e_dofs[0].copy(), h.e_dofs[1].copy(), h.b_dofs.copy()]
+    h.set_sort_interval(5)
+    r = G.strang_splitting_adjoint_(h, dt, 6, checkpoint_every=3, loss_weight=0.1)
+    osp = orc.Splitting(nx, 0.0, L, 3, 3, 2, start, *f0, common_weight=cw)
+    J, gp, ge1, ge2, gb = osp.strang_adjoint(dt, 6, 0.1)
+    assert abs(r.loss - J) <= 1e-12 * abs(J)
+    for row in range(3):
+        _close(r.g_particles[row], gp[row])
+    _close(r.g_e1, ge1)
+    _close(r.g_b, gb)
+
+
 def test_gpu_adjoint_loss_only_and_deterministic():
     parts, e1, e2, b, L, nx = _hot(30000, seed=4)
     outs = []

@@ -104,5 +104,6 @@ def test_autotune_cache_file(tmp_path):
     r2 = h2.autotune(0.05, n_steps=6, blocks=[0, 32], sort_intervals=[3], cache=path)
     assert r2["cached"] is True
     assert (r2["blocks"], r2["sort_interval"]) == (r1["blocks"], r1["sort_interval"])
-    key = core.autotune_key(2, 8192, 32, 3, 2, 0.05, 6, [0, 32], [3])
+    key = core.autotune_key(2, 8192, 32, 3, 2, 0.05, 6, [0, 32], [3], 1, "reference",
+                            core.device_info())
     assert core.autotune_cache_lookup(path, key)["blocks"] == r1["blocks"]

@@ -293,6 +293,16 @@ def test_gpu_nan_position_raises_inexact():
     h = G.HamiltonianSplitting(1, 2, mx, ks0, ks1, pg, [np.zeros(nx), np.zeros(nx)], np.zeros(nx))
     with pytest.raises(G.InexactError):
         G.operatorHE(h, 0.1)
+    # the diagnostics evaluate the fields at the positions too
+    with pytest.raises(G.InexactError):
+        G.solve_poisson_(np.zeros(nx), h, np.zeros(nx))
+    th = G.TimeHistoryDiagnostics(h)
+    with pytest.raises(G.InexactError):
+        G.write_step_(th, 0.0, 3, efield_poisson=np.zeros(nx))
+    # the error flag is cleared: a finite state runs again
+    pg.array[0, 17] = 0.5
+    out = G.write_step_(th, 0.0, 3, efield_poisson=np.zeros(nx))
+    assert np.all(np.isfinite(out))
 
 
 # ---------------------------------------------------------------------------
