@@ -79,7 +79,81 @@ def arith_run(G, _lib, w, arith, warmup, steps, blocks, interval, sort_s):
                          "frac": ach / HBM_PEAK_GBS, "bytes_per_particle_step": BYTES_FUSED}}
 
 
-def cpu_baseline(n, steps, seed=1):
+BYTES_2D3V = 472          # per particle-step of the 2D3V splitting's passes: HE 64 x 2,
+                          # Hp3 64 x 2, Hp2 72 x 2, Hp1 72 (48 B state read per pass,
+                          # the pushed components written)
+
+
+def split2d_run(G, n, steps, cpu_n, cpu_steps):
+    """The 2D3V Landau configuration (16,777,216 particles, 64 x 64, degree
+    3, dt = 0.05) through the 2D3V splitting: E from Poisson of the charge,
+    B = 0, `steps` Strang steps timed on the GPU; the oracle's 2D3V splitting
+    on a bounded sample of the same workload on one core."""
+    from gempic_amd import workloads
+    print(f"[bench] 2D3V Landau, {n} particles", file=sys.stderr, flush=True)
+    w = workloads.landau_2d3v(n, seed=3)
+    nx, ny = w["nx"], w["ny"]
+    mesh = G.TwoDGrid(w["xmin"], w["xmax"], nx, w["ymin"], w["ymax"], ny)
+    pg = G.ParticleGroup(2, 3, n, common_weight=w["common_weight"])
+    pg.array[:] = w["particles"]
+    mx = G.TwoDMaxwell(mesh, 3)
+    ks0 = G.ParticleMeshCoupling2D(pg, mesh, 3, "galerkin")
+    ks1 = G.ParticleMeshCoupling2D(pg, mesh, 2, "galerkin")
+    e = [np.zeros(nx * ny) for _ in range(3)]
+    b = [np.zeros(nx * ny) for _ in range(3)]
+    h = G.HamiltonianSplitting2D3V(mx, ks0, ks1, pg, e, b, sync=False)
+    rho = h.rho()
+    G.compute_e_from_rho_(e, mx, rho)
+    h.to_device(particles=False)
+    h.strang_splitting(w["dt"], 2)           # warmup
+    h.synchronize()
+    t0 = time.perf_counter()
+    h.strang_splitting(w["dt"], steps)
+    h.synchronize()
+    el = time.perf_counter() - t0
+    rate = n * steps / el
+    ach = BYTES_2D3V * rate / 1e9
+    out = {"workload": f"2D3V Landau damping, {n} particles, 64x64 on [0,4pi)^2, degree 3, "
+                       f"dt 0.05, {steps} steps",
+           "value": rate, "unit": "particle-steps/s", "ms_per_step": 1e3 * el / steps,
+           "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": ach / HBM_PEAK_GBS, "bytes_per_particle_step": BYTES_2D3V,
+                        "note": "whole-step time (all passes, field solves and sorts)"}}
+    del h
+    if cpu_n:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as orc
+        parts = np.asfortranarray(w["particles"][:, :cpu_n].copy())
+        parts[5] = w["particles"][5, :cpu_n]
+        ec = [a.copy() for a in e]
+        bc = [np.zeros(nx * ny) for _ in range(3)]
+        osp = orc.Splitting2D3V(nx, ny, w["xmin"], w["xmax"], w["ymin"], w["ymax"], 3, parts,
+                                ec, bc, common_weight=1.0 / cpu_n)
+        t0 = time.perf_counter()
+        osp.strang_splitting(w["dt"], cpu_steps)
+        secs = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": cpu_n * cpu_steps / secs, "unit": "particle-steps/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"{cpu_n} particles of the same sample x {cpu_steps} "
+                                         f"steps ({secs:.1f} s), the oracle's 2D3V splitting, "
+                                         "one thread"}
+    return out
+
+
+def cpu_threads():
+    """The host threads this process may use: its CPU affinity, capped by
+    OMP_NUM_THREADS where that is set."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def cpu_baseline(n, steps, seed=1, tasks=1):
+    """The oracle's 1D2V Strang splitting on the Weibel sample: one thread,
+    or `tasks` threads with the reference's nthreads() task split (equal
+    particle chunks, a buffer per chunk, buffers summed in chunk order)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as orc
     from gempic_amd import workloads
@@ -91,6 +165,8 @@ def cpu_baseline(n, steps, seed=1):
                         common_weight=w["common_weight"])
     rho = np.zeros(nx)
     osp.solve_poisson(e1, rho)
+    if tasks > 1:
+        osp.set_tasks(tasks)
     t0 = time.perf_counter()
     osp.strang_splitting(w["dt"], steps)
     dt = time.perf_counter() - t0
@@ -338,18 +414,20 @@ def adjoint_run(G, _lib, n, steps, every, cpu_n, cpu_steps):
     return out
 
 
-def ensemble_run(G, _lib, n_inst, steps, seq_steps, adj_steps=100, adj_every=10, seed=5):
+def ensemble_run(G, _lib, n_inst, steps, seq_steps, adj_steps=500, adj_every=25, seed=5):
     """Independent 1D2V Weibel instances: N uniform in [65536, 1048576],
     nx in {16, 32, 64}, k uniform in [1.0, 1.5], alpha and beta log-uniform
     in [1e-5, 1e-3] (seeded). Particles come from sample!(pg, alpha, k,
-    sigma, mesh) on the GPU (one sigma for both velocity components), E1
-    from Poisson, B0 = l2projection!(beta cos(kx)). dt = min(0.05, dx/2):
-    the degree-3 FEM Maxwell solver is unstable for dt > 0.64 dx, which
-    0.05 exceeds on the finest grids. Times `steps` batched Strang steps of
-    the whole ensemble against `seq_steps` steps run instance by instance,
-    then `adj_steps` steps forward against forward + reverse mode of every
-    instance (checkpoints every `adj_every` steps: about 85 GB for 146M
-    particles x 100 steps)."""
+    sigma1, mesh) on the GPU, sigma1 = 0.02 / sqrt(2), with v2 then scaled
+    to sigma2 = sqrt(12) sigma1 (the Weibel anisotropy); E1 from Poisson,
+    B0 = l2projection!(beta cos(kx)). dt = min(0.05, dx/2): the degree-3
+    FEM Maxwell solver is unstable for dt > 0.64 dx, which 0.05 exceeds on
+    the finest grids. Times `steps` batched Strang steps of the whole
+    ensemble against `seq_steps` steps run instance by instance, then
+    `adj_steps` steps forward against forward + reverse mode of every
+    instance (checkpoints every `adj_every` steps; for 146M particles x 500
+    steps at 25 the reverse mode holds about 195 GB: 20 checkpoints, a
+    24-state re-run buffer and 20 sort epochs, 1.33 KB per particle)."""
     import math
     from gempic_amd import workloads
     L_ = _lib.load()
@@ -371,6 +449,9 @@ def ensemble_run(G, _lib, n_inst, steps, seq_steps, adj_steps=100, adj_every=10,
         h = G.HamiltonianSplitting(1, 2, G.Maxwell1DFEM(mesh, 3), ks0, ks1, pg, [e1, e2], b,
                                    sync=False)
         G.sample_particles_(h, alpha, k, 0.02 / math.sqrt(2.0))
+        h.to_host(fields=False)
+        pg.array[2] *= math.sqrt(12.0)
+        h.to_device(fields=False)
         G.solve_poisson_(e1, h, np.zeros(nx))
         h.to_device(particles=False)
         hs.append(h)
@@ -381,6 +462,11 @@ def ensemble_run(G, _lib, n_inst, steps, seq_steps, adj_steps=100, adj_every=10,
     t0 = time.perf_counter()
     G.ensemble_strang_splitting_(ens, dts, steps)
     t_ens = time.perf_counter() - t0
+    # the same steps with the regular step replayed as a HIP graph
+    G.ensemble_strang_splitting_(ens, dts, 3, graph=True)    # capture
+    t0 = time.perf_counter()
+    G.ensemble_strang_splitting_(ens, dts, steps, graph=True)
+    t_graph = time.perf_counter() - t0
     t0 = time.perf_counter()
     for h, dt in zip(hs, dts):
         _lib.check(L_.gempic_strang_splitting(h._h, dt, seq_steps, _lib.STRANG_FUSED | _lib.ASYNC))
@@ -407,6 +493,7 @@ def ensemble_run(G, _lib, n_inst, steps, seq_steps, adj_steps=100, adj_every=10,
                         f"dt = min(0.05, dx/2)",
             "instances": n_inst, "particles": total, "steps": steps,
             "particle_steps_per_s": total * steps / t_ens,
+            "graph_particle_steps_per_s": total * steps / t_graph,
             "one_by_one_particle_steps_per_s": total * seq_steps / t_seq,
             "one_by_one_steps": seq_steps, "adjoint": adj}
 
@@ -444,6 +531,8 @@ def main():
                          "(HamiltonianSplitting.autotune on the same state, before the warmup)")
     ap.add_argument("--cpu-particles", type=int, default=262144)
     ap.add_argument("--cpu-steps", type=int, default=300)      # about 10 s on one core
+    ap.add_argument("--cpu-steps-mt", type=int, default=100,
+                    help="steps of the all-threads CPU baseline on the benchmark's particles")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--substep", action="store_true", help="also time the sub-step path")
     ap.add_argument("--blocks", type=int, default=0, help="workgroups per particle pass (0 = auto)")
@@ -455,6 +544,9 @@ def main():
     ap.add_argument("--hbm-particles", type=int, default=16777216,
                     help="particles of the state-larger-than-the-caches point (0 = skip it)")
     ap.add_argument("--hbm-steps", type=int, default=200)
+    ap.add_argument("--particles-2d3v", type=int, default=16777216,
+                    help="particles of the 2D3V Landau measurement (0 = skip it)")
+    ap.add_argument("--steps-2d3v", type=int, default=200)
     ap.add_argument("--no-sampling", action="store_true", help="skip the sample! measurement")
     ap.add_argument("--boris-steps", type=int, default=200,
                     help="steps of the Boris splitting measurement (0 = skip it)")
@@ -469,6 +561,8 @@ def main():
     ap.add_argument("--reps-2d", type=int, default=20)
     ap.add_argument("--only-2d", action="store_true",
                     help="print only the 2D pieces measurement (for profiling its kernels)")
+    ap.add_argument("--only-2d3v", action="store_true",
+                    help="print only the 2D3V Landau measurement (for profiling its kernels)")
     ap.add_argument("--cpu-particles-2d", type=int, default=4194304)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain copy of MIB MiB (HBM counter calibration) and exit")
@@ -490,6 +584,11 @@ def main():
     import gempic_amd as G
     from gempic_amd import workloads
     from gempic_amd import _lib
+
+    if args.only_2d3v:
+        print(json.dumps(split2d_run(G, args.particles_2d3v, args.steps_2d3v,
+                                     0 if args.no_cpu else 131072, 4)))
+        return
 
     n = args.particles
     print(f"[bench] generating 1D2V Weibel, {n} particles", file=sys.stderr, flush=True)
@@ -519,22 +618,32 @@ def main():
     if args.warmup:
         _lib.check(L.gempic_strang_splitting(h._h, dt, args.warmup, _lib.STRANG_FUSED))
     _lib.check(L.gempic_synchronize(h._h))
-    # the particle sort, timed on its own (HIP events on the library's stream)
+    # the particle sort, timed where it runs: HIP events on the library's
+    # stream around the sorts of three sort intervals of steps, each sorting
+    # particles that drifted for a whole interval; then one more sort, so
+    # that the timed window starts right after a sort
     _lib.check(L.gempic_profile_enable(h._h, 1))
     _lib.check(L.gempic_profile_reset(h._h))
-    for _ in range(3):
-        _lib.check(L.gempic_sort_particles(h._h))
+    if interval > 0:
+        _lib.check(L.gempic_strang_splitting(h._h, dt, 3 * interval + 1, _lib.STRANG_FUSED))
     sort_ms_tot, sort_cnt = h.profile_read(6)
+    if not sort_cnt:
+        for _ in range(3):
+            _lib.check(L.gempic_sort_particles(h._h))
+        sort_ms_tot, sort_cnt = h.profile_read(6)
     sort_s = 1e-3 * sort_ms_tot / max(sort_cnt, 1)
+    _lib.check(L.gempic_sort_particles(h._h))
+    _lib.check(L.gempic_synchronize(h._h))
     # The timed window starts right after a sort: steps s = interval,
     # 2 interval, ... inside it sort, floor((K - 1) / interval) of them. The
     # figures below charge every step its share of a sort instead
     # (sort time / interval), whether or not one falls in the window.
     sorts_in_window = (args.steps - 1) // interval if interval > 0 else 0
     sort_share_s = (args.steps / interval - sorts_in_window) * sort_s if interval > 0 else 0.0
-    # HIP events around every 10th launch of each kernel: the kernel times
+    # HIP events around every 10th launch of each kernel (every launch for
+    # short runs, so that at least 20 launches are timed): the kernel times
     # come from the timed run itself at a small cost to its stream
-    _lib.check(L.gempic_profile_enable(h._h, 10))
+    _lib.check(L.gempic_profile_enable(h._h, max(1, min(10, args.steps // 20))))
     _lib.check(L.gempic_profile_reset(h._h))
     print(f"[bench] timing {args.steps} steps", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
@@ -594,10 +703,24 @@ def main():
     if not args.no_cpu:
         print(f"[bench] CPU baseline: oracle, {args.cpu_particles} particles x "
               f"{args.cpu_steps} steps", file=sys.stderr, flush=True)
-        rate, secs = cpu_baseline(args.cpu_particles, args.cpu_steps)
-        cpu = {"value": rate, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+        rate1, secs1 = cpu_baseline(args.cpu_particles, args.cpu_steps)
+        one = {"value": rate1, "unit": "particle-steps/s", "cores": 1, "kind": "port",
                "sample": f"1D2V Weibel, {args.cpu_particles} particles x {args.cpu_steps} Strang "
-                         f"steps ({secs:.1f} s), C restatement of the reference, one thread"}
+                         f"steps ({secs1:.1f} s), C restatement of the reference, one thread"}
+        tasks = cpu_threads()
+        while tasks > 1 and n % tasks:
+            tasks -= 1
+        if tasks > 1:
+            print(f"[bench] CPU baseline: oracle on {tasks} threads, {n} particles x "
+                  f"{args.cpu_steps_mt} steps", file=sys.stderr, flush=True)
+            rate, secs = cpu_baseline(n, args.cpu_steps_mt, tasks=tasks)
+            cpu = {"value": rate, "unit": "particle-steps/s", "cores": tasks, "kind": "port",
+                   "sample": f"1D2V Weibel, the benchmark's {n} particles x {args.cpu_steps_mt} "
+                             f"Strang steps ({secs:.1f} s), C restatement of the reference with "
+                             f"its nthreads() task split on {tasks} threads",
+                   "one_thread": one}
+        else:
+            cpu = one
 
     boris = None
     if args.boris_steps:
@@ -630,6 +753,11 @@ def main():
         print("[bench] sample! into GPU memory", file=sys.stderr, flush=True)
         samp = sampling_run(G, n, 0 if args.no_cpu else 262144)
 
+    s2d = None
+    if args.particles_2d3v:
+        s2d = split2d_run(G, args.particles_2d3v, args.steps_2d3v,
+                          0 if args.no_cpu else 131072, 4)
+
     p2d = None
     if args.particles_2d:
         print(f"[bench] 2D pieces, {args.particles_2d} particles", file=sys.stderr, flush=True)
@@ -656,6 +784,7 @@ def main():
                  "sorts_in_timed_window": sorts_in_window,
                  "share_added_us_per_step": 1e6 * sort_share_s / args.steps,
                  "window_ms_per_step": 1e3 * elapsed_window / args.steps,
+                 "timed_on": f"{sort_cnt} sorts of particles that drifted for {interval} steps",
                  "note": "value and ms_per_step charge each step sort time / interval"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
@@ -682,6 +811,8 @@ def main():
         line["reference_arithmetic"] = ref_arith
     if hbm is not None:
         line["hbm_16m"] = hbm
+    if s2d is not None:
+        line["landau_2d3v"] = s2d
     print(json.dumps(line))
 
 

@@ -534,6 +534,7 @@ int gempic_splitting_create(int64_t nx, double xmin, double xmax, int maxwell_de
         put(T_INVM0, h->tabs.inv_mass0); put(T_INVM1, h->tabs.inv_mass1);
         put(T_AMP, h->tabs.eig_weak_ampere); put(T_POIS, h->tabs.eig_weak_poisson);
         put(T_M0, h->tabs.eig_mass0); put(T_M1, h->tabs.eig_mass1);
+        put(T_CM0, h->tabs.circ_m0); put(T_CM1, h->tabs.circ_m1); put(T_CAMP, h->tabs.circ_amp);
         ok(hipMemcpy(h->d_tabs, T.data(), sizeof(double) * T.size(), hipMemcpyHostToDevice));
     }
     if (e != hipSuccess) {
@@ -1319,6 +1320,11 @@ struct gempic_ensemble {
     int *d_tstart = nullptr, *d_gstart = nullptr;
     AdjArgs *d_aargs = nullptr;        // the batched reverse mode's arguments and tables
     int *d_abstart = nullptr, *d_aestart = nullptr;
+    // GEMPIC_GRAPH: the regular step (pass and field part) as a HIP graph,
+    // with the launch geometry it was captured for
+    hipGraphExec_t step_graph = nullptr;
+    int graph_blocks = 0, graph_groups = 0;
+    size_t graph_sh_pass = 0, graph_sh_epi = 0;
 };
 
 namespace {
@@ -1495,10 +1501,50 @@ int gempic_ensemble_destroy(gempic_ensemble *e)
                     e->d_aargs, e->d_abstart, e->d_aestart};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
+    if (e->step_graph) (void)hipGraphExecDestroy(e->step_graph);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
     return GEMPIC_OK;
 }
+
+namespace {
+
+// the regular Strang step of an ensemble (another step follows, the previous
+// one's HE pending) as a graph: captured on the ensemble's stream, then
+// replayed; captured again when the launch geometry changed
+int ens_step_graph(gempic_ensemble *e, const BatchArgs &b, int mode, size_t sh_pass, size_t sh_epi)
+{
+    const int ni = (int)e->hs.size();
+    if (e->step_graph && e->graph_blocks == e->bstart[ni] && e->graph_groups == e->estart[ni] &&
+        e->graph_sh_pass == sh_pass && e->graph_sh_epi == sh_epi)
+        return GEMPIC_OK;
+    if (e->step_graph) {
+        (void)hipGraphExecDestroy(e->step_graph);
+        e->step_graph = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    const hipError_t e1 = launch_pass_batch(e->p0, e->p1, mode, b, e->bstart[ni], sh_pass, e->stream);
+    const hipError_t e2 = launch_epi_batch(e->p0, e->p1, mode, b, e->estart[ni], sh_epi, e->stream);
+    const hipError_t e3 = hipStreamEndCapture(e->stream, &g);
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(GEMPIC_ERR_DEVICE, "capturing the ensemble step as a HIP graph failed");
+    }
+    const hipError_t e4 = hipGraphInstantiate(&e->step_graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e4 != hipSuccess) {
+        e->step_graph = nullptr;
+        return fail(GEMPIC_ERR_DEVICE, "instantiating the ensemble step graph failed");
+    }
+    e->graph_blocks = e->bstart[ni];
+    e->graph_groups = e->estart[ni];
+    e->graph_sh_pass = sh_pass;
+    e->graph_sh_epi = sh_epi;
+    return GEMPIC_OK;
+}
+
+}  // namespace
 
 int gempic_ensemble_strang_splitting(gempic_ensemble *e, const double *dt, int64_t n_steps, int flags)
 {
@@ -1537,8 +1583,14 @@ int gempic_ensemble_strang_splitting(gempic_ensemble *e, const double *dt, int64
         b.block_start = e->d_bstart;
         b.epi_start = e->d_estart;
         b.ninst = ni;
-        HIP_TRY(launch_pass_batch(e->p0, e->p1, mode, b, e->bstart[ni], sh_pass, e->stream));
-        HIP_TRY(launch_epi_batch(e->p0, e->p1, mode, b, e->estart[ni], sh_epi, e->stream));
+        if ((flags & GEMPIC_GRAPH) && s > 0 && s + 1 < n_steps) {
+            rc = ens_step_graph(e, b, mode, sh_pass, sh_epi);
+            if (rc) return rc;
+            HIP_TRY(hipGraphLaunch(e->step_graph, e->stream));
+        } else {
+            HIP_TRY(launch_pass_batch(e->p0, e->p1, mode, b, e->bstart[ni], sh_pass, e->stream));
+            HIP_TRY(launch_epi_batch(e->p0, e->p1, mode, b, e->estart[ni], sh_epi, e->stream));
+        }
         for (auto *h : e->hs) h->steps_since_sort++;
     }
     // the last step's trailing HE
@@ -1549,7 +1601,6 @@ int gempic_ensemble_strang_splitting(gempic_ensemble *e, const double *dt, int64
     b.ninst = ni;
     HIP_TRY(launch_pass_batch(e->p0, e->p1, M_HE_PREV, b, e->bstart[ni],
                               pass_shmem_bytes(e->max_nx, M_HE_PREV), e->stream));
-    (void)flags;
     HIP_TRY(hipStreamSynchronize(e->stream));
     for (auto *h : e->hs) {
         rc = finish(h);
@@ -2269,6 +2320,7 @@ int gempic_maxwell1d_create(int64_t nx, double xmin, double xmax, int degree, ge
         put(T_INVM0, m->tabs.inv_mass0); put(T_INVM1, m->tabs.inv_mass1);
         put(T_AMP, m->tabs.eig_weak_ampere); put(T_POIS, m->tabs.eig_weak_poisson);
         put(T_M0, m->tabs.eig_mass0); put(T_M1, m->tabs.eig_mass1);
+        put(T_CM0, m->tabs.circ_m0); put(T_CM1, m->tabs.circ_m1); put(T_CAMP, m->tabs.circ_amp);
         e = hipMemcpy(m->d_tabs, T.data(), sizeof(double) * T.size(), hipMemcpyHostToDevice);
     }
     if (e != hipSuccess) {
@@ -2276,6 +2328,46 @@ int gempic_maxwell1d_create(int64_t nx, double xmin, double xmax, int degree, ge
         return fail(GEMPIC_ERR_DEVICE, std::string("Maxwell1DFEM allocation: ") + hipGetErrorString(e));
     }
     *out = m;
+    return GEMPIC_OK;
+}
+
+int gempic_maxwell1d_compute_rhs_from_function(gempic_maxwell1d *m, gempic_fn1d f, void *ctx,
+                                               int degree, double *coefs)
+{
+    if (!m || !f || !coefs) return fail(GEMPIC_ERR_ARGUMENT, "NULL argument");
+    if (degree < 0 || degree > 3)
+        return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: degree must be 0..3");
+    // gausslegendre(degree + 1) on [0, 1] (closed forms) and the splines there
+    double x[4], w[4], bspl[4][4];
+    gh::gauss_legendre(degree + 1, x, w);
+    for (int k = 0; k <= degree; ++k) {
+        x[k] = 0.5 * (x[k] + 1.0);
+        w[k] = 0.5 * w[k];
+        gh::bspline_host(degree, x[k], bspl[k]);
+    }
+    for (int64_t i = 1; i <= m->nx; ++i) {
+        double coef = 0.0;
+        for (int j = 1; j <= degree + 1; ++j)
+            for (int k = 1; k <= degree + 1; ++k) {
+                const double xq = ((x[k - 1] + (double)i) + (double)j) - 2.0;
+                coef = coef + w[k - 1] * f(m->dx * xq, ctx) * bspl[k - 1][degree + 1 - j];
+            }
+        coefs[i - 1] = coef * m->dx;
+    }
+    return GEMPIC_OK;
+}
+
+int gempic_maxwell1d_l2projection(gempic_maxwell1d *m, gempic_fn1d f, void *ctx, int degree,
+                                  double *coefs)
+{
+    if (!m) return fail(GEMPIC_ERR_ARGUMENT, "null handle");
+    if (degree != m->degree && degree != m->degree - 1)
+        return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: degree " + std::to_string(degree) + " not available");
+    int rc = gempic_maxwell1d_compute_rhs_from_function(m, f, ctx, degree, coefs);
+    if (rc) return rc;
+    // the reference solves the mass system into its workspace and returns
+    // coefs_dofs ./ delta_x, the right-hand side over dx
+    for (int64_t i = 0; i < m->nx; ++i) coefs[i] = coefs[i] / m->dx;
     return GEMPIC_OK;
 }
 

@@ -51,6 +51,27 @@ inline bool gauss_legendre(int n, double *x, double *w)
     }
 }
 
+// uniform_bsplines_eval_basis (low_level_bsplines.jl:38-61) on the host: the
+// p+1 B-splines of degree p at offset x, with the reference's de Boor
+// recurrence and rounding.
+inline void bspline_host(int degree, double x, double *b)
+{
+    b[0] = 1.0;
+    for (int j = 1; j <= degree; ++j) {
+        double xx = -x;
+        const double jr = (double)j;
+        const double inv_j = 1.0 / jr;
+        double saved = 0.0;
+        for (int r = 0; r < j; ++r) {
+            xx = xx + 1.0;
+            const double temp = b[r] * inv_j;
+            b[r] = saved + xx * temp;
+            saved = (jr - xx) * temp;
+        }
+        b[j] = saved;
+    }
+}
+
 // Eulerian-number mass lines of the degree-p (0-form) and degree-(p-1)
 // (1-form) spline mass matrices, maxwell_1d_fem.jl:60-92.
 inline bool mass_lines(int degree, double *m0, double *m1)
@@ -76,7 +97,26 @@ struct Maxwell1DTables {
     std::vector<double> eig_mass0, eig_mass1, eig_weak_ampere, eig_weak_poisson;
     std::vector<double> inv_mass0, inv_mass1;   // 1/eig in the real slots, 0 elsewhere
     std::vector<double> cos_tab, sin_tab;       // cos/sin(2 pi t / n), t = 0..n-1
+    // first column c of the circulant matrix C with C rhs = solve_circulant(eig, rhs)
+    // (c = HC2R(eig) / n, in long double) for eig = inv_mass0, inv_mass1, eig_weak_ampere
+    std::vector<double> circ_m0, circ_m1, circ_amp;
 };
+
+// c[m] = (1/n) (e[0] + (-1)^m e[n/2] + 2 sum_{k=1}^{n/2-1} (e[k] cos(2 pi k m/n) - e[n-k] sin(2 pi k m/n)))
+inline std::vector<double> circulant_column(const std::vector<double> &e, int64_t n)
+{
+    std::vector<double> c((size_t)n, 0.0);
+    const long double tp = 2.0L * 3.14159265358979323846264338327950288L;
+    for (int64_t m = 0; m < n; ++m) {
+        long double s = (long double)e[0] + ((m & 1) ? -1.0L : 1.0L) * (long double)e[n / 2];
+        for (int64_t k = 1; k < n / 2; ++k) {
+            const long double ang = tp * (long double)((k * m) % n) / (long double)n;
+            s += 2.0L * ((long double)e[k] * cosl(ang) - (long double)e[n - k] * sinl(ang));
+        }
+        c[m] = (double)(s / (long double)n);
+    }
+    return c;
+}
 
 // Circulant eigenvalues in FFTW halfcomplex layout (real part of mode k at
 // k, imaginary at n-k), with the expression order of
@@ -149,6 +189,9 @@ inline bool maxwell1d_tables(int64_t n, int degree, Maxwell1DTables &t)
         t.cos_tab[s] = (double)cosl(ang);
         t.sin_tab[s] = (double)sinl(ang);
     }
+    t.circ_m0 = circulant_column(t.inv_mass0, n);
+    t.circ_m1 = circulant_column(t.inv_mass1, n);
+    t.circ_amp = circulant_column(t.eig_weak_ampere, n);
     return true;
 }
 

@@ -9,6 +9,12 @@ namespace gl {
 
 constexpr int kBlock = 256;               // threads per workgroup of the particle passes
 constexpr int kWaves = kBlock / 64;
+// iterations ahead that a 1D particle pass loads its particles (1 or 2)
+#ifndef GEMPIC_PASS_PREFETCH
+#define GEMPIC_PASS_PREFETCH 1
+#endif
+constexpr int kPrefetch = GEMPIC_PASS_PREFETCH;
+static_assert(kPrefetch == 1 || kPrefetch == 2, "prefetch depth 1 or 2");
 constexpr int kNxMax = 256;               // largest 1D grid handled by the device path
 constexpr int kGroup = 16;                // workgroups per first-level reduction group
 constexpr int kEpiGroupsMax = 64;         // workgroups of the deposit-sum kernel
@@ -19,7 +25,11 @@ constexpr int kAdjNxMax = 128;            // largest 1D grid of the reverse-mode
 enum GridSlot { S_E1 = 0, S_E2, S_B, S_E2P, S_J1, S_J2, S_RHO, S_EPOIS, S_E1N, S_E2N, S_E1M, S_E2M,
                 S_BM, NSLOT };
 // constant tables (nx doubles each)
-enum TabSlot { T_COS = 0, T_SIN, T_INVM0, T_INVM1, T_AMP, T_POIS, T_M0, T_M1, NTAB };
+// T_CM0, T_CM1, T_CAMP: first columns of the circulant matrices of the M0 and
+// M1 inverse-mass solves and of the weak Ampere solve (the fused-arithmetic
+// field part applies them as matrix-vector products)
+enum TabSlot { T_COS = 0, T_SIN, T_INVM0, T_INVM1, T_AMP, T_POIS, T_M0, T_M1, T_CM0, T_CM1, T_CAMP,
+               NTAB };
 // deposit slots of one pass
 enum DepSlot { D_J2A = 0, D_J1, D_RHO, D_J2B, NDEP };
 
@@ -274,4 +284,17 @@ int abi_fail(int code, const char *msg);
 int abi_check_device();
 int abi_device_cus();
 
+}  // namespace gl
+
+struct gempic_maxwell2d;
+
+namespace gl {
+// TwoDMaxwell operations on device arrays (pic2d.hip), queued on the
+// solver's own stream: compute_e_from_b!, compute_b_from_e!, E -= M1^{-1} j
+// (compute_e_from_j!) and compute_rho_from_e! (maxwell_2d_fem.jl:370-499)
+hipStream_t m2d_stream(gempic_maxwell2d *m);
+void m2d_e_from_b(gempic_maxwell2d *m, double dt, double *const *b, double *const *e);
+void m2d_b_from_e(gempic_maxwell2d *m, double dt, double *const *e, double *const *b);
+void m2d_e_from_j(gempic_maxwell2d *m, const double *j, int component, double *e);
+void m2d_rho_from_e(gempic_maxwell2d *m, double *const *e, double *rho);
 }  // namespace gl

@@ -4,9 +4,10 @@
 // particle arrays once, with the fields of the pass in LDS, applies the
 // sub-steps selected by MODE to each particle in registers, and accumulates
 // the charge/current deposits with the deterministic per-lane cell caches of
-// device_common.hpp. Deposits are reduced workgroup -> group -> grid in a
-// fixed order, and the last workgroup to finish runs the field solves of
-// the sub-steps (the "epilogue") so that a whole Strang step is one launch.
+// device_common.hpp, writing one partial row per workgroup. The field part
+// (k_epi1d, a second launch) sums the rows in a fixed order (workgroup slices,
+// then the last of its workgroups to finish sums the group rows) and runs the
+// field solves of the sub-steps, so a whole Strang step is two launches.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -445,6 +446,32 @@ __device__ __forceinline__ void pass1d_body(const PassArgs &a, int bid, double *
     // unpadded views (node k at s*[k])
     double *sE1 = pE1 + kPad, *sE2p = pE2p + kPad, *sE2 = pE2 + kPad, *sB = pB + kPad;
 
+    // this workgroup's chunk: particles base .. base + cnt_here - 1, addressed
+    // through per-chunk base pointers and a 32-bit index
+    const int64_t base = (int64_t)bid * a.chunk;
+    const int64_t left = a.n - base;
+    const int cnt_here = left < a.chunk ? (int)left : (int)a.chunk;
+    double *__restrict__ px = a.x + base;
+    double *__restrict__ pv1 = a.v1 + base;
+    double *__restrict__ pv2 = a.v2 + base;
+    const double *__restrict__ pw = a.w + base;
+
+    // software pipeline: the loads of iteration it + kPrefetch are issued
+    // before the work of iteration it; the first ones before the fields are
+    // staged, so that their latencies overlap
+    auto load_particle = [&](int j, double &lx, double &lv1, double &lv2, double &lw) {
+        if (j < cnt_here) {
+            lx = px[j];
+            lw = pw[j];
+            if (!RHO_ONLY) lv1 = pv1[j];
+            if (!V1 && !RHO_ONLY) lv2 = pv2[j];
+        }
+    };
+    double nx_x = 0.0, nx_v1 = 0.0, nx_v2 = 0.0, nx_w = 0.0;     // iteration it + 1
+    double n2_x = 0.0, n2_v1 = 0.0, n2_v2 = 0.0, n2_w = 0.0;     // it + 2 (kPrefetch 2)
+    if (a.iters > 0) load_particle(tid, nx_x, nx_v1, nx_v2, nx_w);
+    if (kPrefetch > 1 && a.iters > 1) load_particle(tid + kBlock, n2_x, n2_v1, n2_v2, n2_w);
+
     load_padded(pE1, a.grid + S_E1 * nx, nx);
     load_padded(pE2p, a.grid + S_E2P * nx, nx);
     load_padded(pE2, a.grid + S_E2 * nx, nx);
@@ -463,38 +490,24 @@ __device__ __forceinline__ void pass1d_body(const PassArgs &a, int bid, double *
 
     const double xmin = a.xmin, dx = a.dx, qm = a.qm;
     const double rdx = 1.0 / dx;     // div_by: (x - xmin) / dx, correctly rounded
-    // this workgroup's chunk: particles base .. base + cnt_here - 1, addressed
-    // through per-chunk base pointers and a 32-bit index
-    const int64_t base = (int64_t)bid * a.chunk;
-    const int64_t left = a.n - base;
-    const int cnt_here = left < a.chunk ? (int)left : (int)a.chunk;
-    double *__restrict__ px = a.x + base;
-    double *__restrict__ pv1 = a.v1 + base;
-    double *__restrict__ pv2 = a.v2 + base;
-    const double *__restrict__ pw = a.w + base;
     int errs = 0;
 
-    // software pipeline: the loads of iteration it+1 are issued before the
-    // work of iteration it
-    double nx_x = 0.0, nx_v1 = 0.0, nx_v2 = 0.0, nx_w = 0.0;
-    if (a.iters > 0 && tid < cnt_here) {
-        nx_x = px[tid];
-        nx_w = pw[tid];
-        if (!RHO_ONLY) nx_v1 = pv1[tid];
-        if (!V1 && !RHO_ONLY) nx_v2 = pv2[tid];
-    }
     for (int it = 0; it < a.iters; ++it) {
         const int i = it * kBlock + tid;
         const bool valid = i < cnt_here;
         double x = nx_x, v1 = nx_v1, v2 = nx_v2, w = nx_w;
-        {
-            const int in = i + kBlock;
-            if (it + 1 < a.iters && in < cnt_here) {
-                nx_x = px[in];
-                nx_w = pw[in];
-                if (!RHO_ONLY) nx_v1 = pv1[in];
-                if (!V1 && !RHO_ONLY) nx_v2 = pv2[in];
-            }
+        if constexpr (kPrefetch > 1) {
+            nx_x = n2_x;
+            nx_v1 = n2_v1;
+            nx_v2 = n2_v2;
+            nx_w = n2_w;
+            if (it + 2 < a.iters) load_particle(i + 2 * kBlock, n2_x, n2_v1, n2_v2, n2_w);
+        } else {
+            (void)n2_x;
+            (void)n2_v1;
+            (void)n2_v2;
+            (void)n2_w;
+            if (it + 1 < a.iters) load_particle(i + kBlock, nx_x, nx_v1, nx_v2, nx_w);
         }
         // marker charge, as (w*charge)*common_weight (hamiltonian_splitting_1d2v.jl:62-64)
         const double q = (w * a.charge) * a.cw;
@@ -703,7 +716,10 @@ __device__ __forceinline__ int batch_instance(const int *start, int ninst, int b
 }
 
 template <int P0, int P1, int MODE>
-__global__ void __launch_bounds__(kBlock) k_pass1d(PassArgs a)
+#ifndef GEMPIC_PASS_MIN_WAVES
+#define GEMPIC_PASS_MIN_WAVES 1
+#endif
+__global__ void __launch_bounds__(kBlock, GEMPIC_PASS_MIN_WAVES) k_pass1d(PassArgs a)
 {
     extern __shared__ double smem[];
     pass1d_body<P0, P1, MODE>(a, blockIdx.x, smem);
@@ -748,17 +764,18 @@ __device__ __forceinline__ int live_index(int e, int nx, unsigned mask)
     return r * nx + node;
 }
 
-// out[i] = sum of base[b*NDEP*nx + i] over b in [b0, b1) for the live
-// entries i of the rows in mask. Fixed order: the k threads of an entry take
-// consecutive slices of [b0, b1), and their slice sums are added in slice
-// order. Called by all threads; scratch holds blockDim.x doubles.
-__device__ void sum_rows_range(const double *base, int b0, int b1, int nx, unsigned mask,
-                               double *out, double *scratch)
+// out[i] = sum of base[b*NDEP*nx + i] over the n rows b = b0 + j * stride,
+// j in [0, n), for the live entries i of the rows in mask. Fixed order: the
+// k threads of an entry take consecutive slices of j, and their slice sums
+// are added in slice order. Called by all threads; scratch holds blockDim.x
+// doubles.
+__device__ void sum_rows(const double *base, int b0, int n, int stride, int nx, unsigned mask,
+                         double *out, double *scratch)
 {
     const int nt = blockDim.x, tid = threadIdx.x;
     const int len = NDEP * nx;
     const int E = __popc(mask) * nx;
-    const int n = b1 - b0;
+    const size_t rs = (size_t)stride * len;
     for (int e0 = 0; e0 < E; e0 += nt) {
         const int ec = E - e0 < nt ? E - e0 : nt;
         const int k = nt / ec;
@@ -768,19 +785,19 @@ __device__ void sum_rows_range(const double *base, int b0, int b1, int nx, unsig
         int idx = 0;
         if (h < k) {
             idx = live_index(e, nx, mask);
-            const int r0 = b0 + (int)(((long)n * h) / k);
-            const int r1 = b0 + (int)(((long)n * (h + 1)) / k);
-            const double *col = base + idx;
+            const int r0 = (int)(((long)n * h) / k);
+            const int r1 = (int)(((long)n * (h + 1)) / k);
+            const double *col = base + (size_t)b0 * len + idx;
             int b = r0;
             constexpr int B = 16;
             for (; b + B <= r1; b += B) {
                 double t[B];
 #pragma unroll
-                for (int u = 0; u < B; ++u) t[u] = col[(size_t)(b + u) * len];
+                for (int u = 0; u < B; ++u) t[u] = col[(size_t)(b + u) * rs];
 #pragma unroll
                 for (int u = 0; u < B; ++u) s += t[u];
             }
-            for (; b < r1; ++b) s += col[(size_t)b * len];
+            for (; b < r1; ++b) s += col[(size_t)b * rs];
         }
         scratch[tid] = s;
         __syncthreads();
@@ -791,6 +808,47 @@ __device__ void sum_rows_range(const double *base, int b0, int b1, int nx, unsig
         }
         __syncthreads();
     }
+}
+
+// sum_rows over the rows [b0, b1)
+__device__ __forceinline__ void sum_rows_range(const double *base, int b0, int b1, int nx,
+                                               unsigned mask, double *out, double *scratch)
+{
+    sum_rows(base, b0, b1 - b0, 1, nx, mask, out, scratch);
+}
+
+// The first level of the field part's sum: group g of ng sums the partial
+// rows of its contiguous slice of the nblocks workgroups. (Slices of the
+// rows written on the group's own XCD, b mod ng == g, measured the same.)
+__device__ __forceinline__ void sum_group_rows(const double *part, int nblocks, int g, int ng, int nx,
+                                               unsigned mask, double *out, double *scratch)
+{
+    const int b0 = (int)(((long)nblocks * g) / ng), b1 = (int)(((long)nblocks * (g + 1)) / ng);
+    sum_rows_range(part, b0, b1, nx, mask, out, scratch);
+}
+
+// (C rhs)[i] for the circulant C with first column c: sum_j c[(i - j) mod n] rhs[j],
+// in four interleaved partial sums
+__device__ __forceinline__ double circ_apply(const double *c, const double *rhs, int i, int n)
+{
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int t = i;                                    // (i - j) mod n for j = 0
+    int j = 0;
+    for (; j + 4 <= n; j += 4) {
+        const int t1 = t == 0 ? n - 1 : t - 1;
+        const int t2 = t1 == 0 ? n - 1 : t1 - 1;
+        const int t3 = t2 == 0 ? n - 1 : t2 - 1;
+        s0 = __builtin_fma(c[t], rhs[j], s0);
+        s1 = __builtin_fma(c[t1], rhs[j + 1], s1);
+        s2 = __builtin_fma(c[t2], rhs[j + 2], s2);
+        s3 = __builtin_fma(c[t3], rhs[j + 3], s3);
+        t = t3 == 0 ? n - 1 : t3 - 1;
+    }
+    for (; j < n; ++j) {
+        s0 = __builtin_fma(c[t], rhs[j], s0);
+        t = t == 0 ? n - 1 : t - 1;
+    }
+    return (s0 + s1) + (s2 + s3);
 }
 
 // workgroups of k_epi1d for nblocks partial rows
@@ -817,10 +875,12 @@ __device__ __forceinline__ void epi1d_body(const PassArgs &a, int nblocks, int g
     constexpr bool V1 = (MODE & M_V1) != 0;
     constexpr bool RHO_ONLY = (MODE & M_RHO) != 0;
     constexpr bool WITH_J2A = (MODE & M_WITH_J2A) != 0;
+    constexpr bool FAST = (MODE & M_FAST) != 0;
     constexpr bool DEP_RHO = (HP1 && !V1 && !HP2B) || RHO_ONLY;
     constexpr unsigned DEPMASK = (HP2A ? 1u << D_J2A : 0u) | (HP1 ? 1u << D_J1 : 0u) |
                                  (DEP_RHO ? 1u << D_RHO : 0u) | (HP2B ? 1u << D_J2B : 0u);
     constexpr unsigned REDMASK = DEPMASK | (WITH_J2A ? 1u << D_J2A : 0u);
+    static_assert(!FAST || (HE && HP2A && HP1 && HP2B && !V1), "fused arithmetic: the fused step");
     (void)HE_PREV;
     (void)HE;
 
@@ -842,8 +902,7 @@ __device__ __forceinline__ void epi1d_body(const PassArgs &a, int nblocks, int g
     double *scratch = w3 + 6 * nx;          // [blockDim.x]
 
     if constexpr (PHASE == 1) {
-        const int b0 = (int)(((long)nblocks * g) / ng), b1 = (int)(((long)nblocks * (g + 1)) / ng);
-        sum_rows_range(a.part, b0, b1, nx, REDMASK, a.gpart + (size_t)g * len, scratch);
+        sum_group_rows(a.part, nblocks, g, ng, nx, REDMASK, a.gpart + (size_t)g * len, scratch);
         return;
     }
 
@@ -860,8 +919,7 @@ __device__ __forceinline__ void epi1d_body(const PassArgs &a, int nblocks, int g
 
     if constexpr (PHASE == 0) {
         // group sums of a slice of the workgroups' rows
-        const int b0 = (int)(((long)nblocks * g) / ng), b1 = (int)(((long)nblocks * (g + 1)) / ng);
-        sum_rows_range(a.part, b0, b1, nx, REDMASK, a.gpart + (size_t)g * len, scratch);
+        sum_group_rows(a.part, nblocks, g, ng, nx, REDMASK, a.gpart + (size_t)g * len, scratch);
         __threadfence();
         __syncthreads();
         if (tid == 0) s_last = atomicAdd(&a.cnt[0], 1u) == (unsigned)(ng - 1);
@@ -909,6 +967,67 @@ __device__ __forceinline__ void epi1d_body(const PassArgs &a, int nblocks, int g
         }
         ep_e_from_j(sE1, r_j1, T + T_INVM1 * nx, dx, work, hc, T, nx);
         for (int i = tid; i < nx; i += nt) G[S_E1 * nx + i] = sE1[i];
+    } else if constexpr (FAST) {
+        // the fused step's field part in fused arithmetic: the two Hp2
+        // currents go through one M0^{-1} solve (linear, so the two updates
+        // of E2 become one), and every solve is the circulant's
+        // matrix-vector product (circ_apply)
+        double *rhs2 = spare;                          // dt (j2a + j2b)
+        for (int i = tid; i < nx; i += nt) rhs2[i] = (r_j2a[i] + r_j2b[i]) * a.dt_hp2;
+        __syncthreads();
+        const double rdx = 1.0 / dx;
+        for (int t = tid; t < 2 * nx; t += nt) {
+            const int g = t / nx, i = t - g * nx;
+            const double w = g == 0 ? circ_apply(T + T_CM0 * nx, rhs2, i, nx)
+                                    : circ_apply(T + T_CM1 * nx, r_j1, i, nx);
+            w3[g * nx + i] = w;
+        }
+        __syncthreads();
+        for (int i = tid; i < nx; i += nt) {
+            sE2[i] = __builtin_fma(-w3[i], rdx, sE2[i]);
+            sE1[i] = __builtin_fma(-w3[nx + i], rdx, sE1[i]);
+        }
+        __syncthreads();
+        const double coef = a.dt_he * rdx;
+        for (int i = tid; i < nx; i += nt) {
+            G[S_J1 * nx + i] = 0.0;
+            G[S_J2 * nx + i] = r_j2b[i] * a.dt_hp2;
+            G[S_E2P * nx + i] = sE2[i];
+            if (a.hist) a.hist[(NHIST + H_E2P) * nx + i] = sE2[i];
+            const int im = i == 0 ? nx - 1 : i - 1;
+            work[i] = __builtin_fma(coef, sE2[im] - sE2[i], sB[i]);     // HE's compute_b_from_e!
+        }
+        __syncthreads();
+        for (int i = tid; i < nx; i += nt) sB[i] = work[i];
+        if (a.hist)
+            for (int i = tid; i < nx; i += nt) a.hist[H_BEND * nx + i] = work[i];
+        __syncthreads();
+        // HB of this step and (a.more) of the next one: the weak curl of B once
+        const double times = a.more ? 2.0 * coef : coef;
+        for (int i = tid; i < nx; i += nt) hc[i] = circ_apply(T + T_CAMP * nx, sB, i, nx);
+        __syncthreads();
+        for (int i = tid; i < nx; i += nt) sE2[i] = __builtin_fma(times, hc[i], sE2[i]);
+        __syncthreads();
+        if (a.more) {
+            for (int i = tid; i < nx; i += nt) {
+                const int im = i == 0 ? nx - 1 : i - 1;
+                work[i] = __builtin_fma(coef, sE2[im] - sE2[i], sB[i]);
+            }
+            __syncthreads();
+            for (int i = tid; i < nx; i += nt) sB[i] = work[i];
+            __syncthreads();
+        }
+        for (int i = tid; i < nx; i += nt) {
+            G[S_E1 * nx + i] = sE1[i];
+            G[S_E2 * nx + i] = sE2[i];
+            G[S_B * nx + i] = sB[i];
+        }
+        if (a.hist)
+            for (int i = tid; i < nx; i += nt) {
+                a.hist[(NHIST + H_E1) * nx + i] = sE1[i];
+                a.hist[(NHIST + H_E2) * nx + i] = sE2[i];
+                a.hist[(NHIST + H_B) * nx + i] = sB[i];
+            }
     } else {
         // fused Strang step: Hp2 (1/2) - Hp1 - Hp2 (1/2) field solves, then
         // HE's B update and HB of this step, and (if another step follows)
@@ -1139,6 +1258,8 @@ static hipError_t launch_epi_pd(int mode, const PassArgs &a, int pass_blocks, hi
         GK_EPI(M_HP1)
         GK_EPI(M_HE | M_HP2A | M_HP1 | M_HP2B)
         GK_EPI(M_HE_PREV | M_HE | M_HP2A | M_HP1 | M_HP2B)
+        GK_EPI(M_FAST | M_HE | M_HP2A | M_HP1 | M_HP2B)
+        GK_EPI(M_FAST | M_HE_PREV | M_HE | M_HP2A | M_HP1 | M_HP2B)
         GK_EPI(M_HP1 | M_HP2B | M_WITH_J2A)
         GK_EPI(M_V1 | M_HP1)
         GK_EPI(M_V1 | M_HE | M_HP1)
@@ -1153,7 +1274,9 @@ static hipError_t launch_epi_pd(int mode, const PassArgs &a, int pass_blocks, hi
 hipError_t launch_epi(int p0, int p1, int mode, const PassArgs &a, int pass_blocks, hipStream_t s)
 {
     if (a.nx > kNxMax || pass_blocks < 1) return hipErrorInvalidValue;
-    mode &= ~M_FAST;                  // the field part is the same in either arithmetic
+    // the sub-step modes' field parts are the same in either arithmetic
+    if ((mode & M_FAST) && (mode & (M_HE | M_HP2A | M_HP1 | M_HP2B)) != (M_HE | M_HP2A | M_HP1 | M_HP2B))
+        mode &= ~M_FAST;
 #define GK_PAIR(A, B) \
     if (p0 == (A) && p1 == (B)) return launch_epi_pd<A, B>(mode, a, pass_blocks, s);
     GK_PAIR(1, 0)

@@ -815,6 +815,51 @@ int finish2(gempic_maxwell2d *m)
 
 size_t nn2(const gempic_maxwell2d *m) { return (size_t)m->nx * m->ny; }
 
+}  // namespace
+
+// TwoDMaxwell operations on device arrays, queued on the solver's stream
+// (used by the 2D3V splitting, split2d.hip)
+hipStream_t gl::m2d_stream(gempic_maxwell2d *m) { return m->stream; }
+
+void gl::m2d_e_from_b(gempic_maxwell2d *m, double dt, double *const *b, double *const *e)
+{
+    const Field2 f = m->f();
+    const unsigned gb = grid_blocks((int64_t)m->nx * m->ny);
+    mass2(m, false, true, b[0], m->buf(6));
+    mass2(m, true, false, b[1], m->buf(7));
+    mass2(m, true, true, b[2], m->buf(8));
+    hipLaunchKernelGGL(k_curl_b, dim3(gb), dim3(256), 0, m->stream, f, m->buf(6), m->buf(7),
+                       m->buf(8), m->buf(9), m->buf(10), m->buf(11));
+    for (int c = 0; c < 3; ++c) {
+        inv_mass(m, 1, c + 1, m->buf(9 + c), m->buf(12));
+        hipLaunchKernelGGL(k_axpy, dim3(gb), dim3(256), 0, m->stream, f, e[c], m->buf(12), dt);
+    }
+}
+
+void gl::m2d_b_from_e(gempic_maxwell2d *m, double dt, double *const *e, double *const *b)
+{
+    const unsigned gb = grid_blocks((int64_t)m->nx * m->ny);
+    hipLaunchKernelGGL(k_b_from_e, dim3(gb), dim3(256), 0, m->stream, m->f(), dt, e[0], e[1], e[2],
+                       b[0], b[1], b[2]);
+}
+
+void gl::m2d_e_from_j(gempic_maxwell2d *m, const double *j, int component, double *e)
+{
+    const unsigned gb = grid_blocks((int64_t)m->nx * m->ny);
+    inv_mass(m, 1, component, j, m->buf(2));
+    hipLaunchKernelGGL(k_sub, dim3(gb), dim3(256), 0, m->stream, m->f(), e, m->buf(2));
+}
+
+void gl::m2d_rho_from_e(gempic_maxwell2d *m, double *const *e, double *rho)
+{
+    const unsigned gb = grid_blocks((int64_t)m->nx * m->ny);
+    mass2(m, true, false, e[0], m->buf(2));
+    mass2(m, false, true, e[1], m->buf(3));
+    hipLaunchKernelGGL(k_div, dim3(gb), dim3(256), 0, m->stream, m->f(), m->buf(2), m->buf(3), rho);
+}
+
+namespace {
+
 #define H2D(dst, src) \
     (void)hipMemcpyAsync((dst), (src), sizeof(double) * nn2(m), hipMemcpyHostToDevice, m->stream)
 #define D2H(dst, src) \

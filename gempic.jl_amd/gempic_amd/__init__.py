@@ -8,4 +8,5 @@ include/gempic_hip.h (lib/libgempic_hip.so).
 """
 from .core import *  # noqa: F401,F403
 from .core import __all__  # noqa: F401
-from ._lib import GempicError, InexactError, LIB_PATH, load  # noqa: F401
+from ._lib import GempicError, InexactError, LIB_PATH  # noqa: F401
+from ._lib import load as load_library  # noqa: F401

@@ -24,6 +24,7 @@ COLLOCATION = 1
 STRANG_FUSED = 0
 STRANG_SUBSTEP = 1
 ASYNC = 16
+GRAPH = 64          # GEMPIC_GRAPH: an ensemble's regular steps as a HIP graph
 DETERMINISTIC = 32
 
 _D = ctypes.c_double
@@ -111,6 +112,25 @@ SIGNATURES = {
                                                 _VP, _VP, _VP, _VP, _I, _VP]),
     "gempic_set_arithmetic": (_I, [_VP, _I]),
     "gempic_device_info": (_I, [ctypes.POINTER(ctypes.c_int), ctypes.c_char_p, _I]),
+    "gempic_splitting2d_create": (_I, [_I64, _I64, _D, _D, _D, _D, _I, _I64, _D, _D, _D, _PP]),
+    "gempic_splitting2d_destroy": (_I, [_VP]),
+    "gempic_splitting2d_set_particles": (_I, [_VP, _P, _I64, _I64]),
+    "gempic_splitting2d_get_particles": (_I, [_VP, _P, _I64, _I64]),
+    "gempic_splitting2d_set_fields": (_I, [_VP, _P, _P, _P, _P, _P, _P]),
+    "gempic_splitting2d_get_fields": (_I, [_VP, _P, _P, _P, _P, _P, _P]),
+    "gempic_splitting2d_get_j": (_I, [_VP, _P, _P, _P]),
+    "gempic_splitting2d_operator_hb": (_I, [_VP, _D]),
+    "gempic_splitting2d_operator_he": (_I, [_VP, _D]),
+    "gempic_splitting2d_operator_hp1": (_I, [_VP, _D]),
+    "gempic_splitting2d_operator_hp2": (_I, [_VP, _D]),
+    "gempic_splitting2d_operator_hp3": (_I, [_VP, _D]),
+    "gempic_splitting2d_strang_splitting": (_I, [_VP, _D, _I64, _I]),
+    "gempic_splitting2d_synchronize": (_I, [_VP]),
+    "gempic_splitting2d_set_sort_interval": (_I, [_VP, _I64]),
+    "gempic_splitting2d_rho": (_I, [_VP, _P]),
+    "gempic_splitting2d_rho_from_e": (_I, [_VP, _P]),
+    "gempic_maxwell1d_compute_rhs_from_function": (_I, [_VP, _VP, _VP, _I, _P]),
+    "gempic_maxwell1d_l2projection": (_I, [_VP, _VP, _VP, _I, _P]),
     "gempic_pmc2d_sync": (_I, [_VP]),
     "gempic_pmc2d_add_charge_dev_noalloc": (_I, [_I64, _I64, _D, _D, _D, _D, _I, _I, _I64, _VP,
                                                  _VP, _VP, _VP, _I, _VP]),

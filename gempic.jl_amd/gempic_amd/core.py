@@ -10,11 +10,17 @@ from __future__ import annotations
 
 import ctypes
 import math
+import weakref
 
 import numpy as np
 
 from . import _lib
 from ._lib import check, ptr
+
+# particle group -> (weak reference to) the GPU splitting that holds it, so
+# that the diagnostics called with the group itself (solve_poisson!,
+# write_step!) run on that splitting's GPU copy
+_ON_GPU = weakref.WeakKeyDictionary()
 
 __all__ = [
     "OneDGrid", "ParticleGroup", "ParticleMeshCoupling1D", "Maxwell1DFEM",
@@ -23,7 +29,7 @@ __all__ = [
     "compute_e_from_rho_", "compute_e_from_j_", "compute_e_from_b_", "compute_b_from_e_",
     "compute_rhs_from_function_", "l2projection_", "inner_product", "l2norm_squared",
     "operatorHB", "operatorHE", "operatorHp1", "operatorHp2", "strang_splitting_",
-    "solve_poisson_", "TimeHistoryDiagnostics", "write_step_", "get_charge",
+    "solve_poisson_", "TimeHistoryDiagnostics", "write_step_", "get_charge", "save", "load",
     "HamiltonianSplittingBoris", "staggering_", "push_v_epart_", "push_v_bpart_",
     "push_x_accumulate_j_", "sample_", "sample_particles_", "strang_splitting_adjoint_",
     "get_adjoint", "solve_poisson_adjoint", "AdjointResult", "Ensemble",
@@ -72,6 +78,33 @@ def get_charge(p: ParticleGroup, i: int, i_wi: int = 0) -> float:
     """get_charge (particle_group.jl:67-69); 0-based particle index."""
     D, V = p.dims
     return p.charge * p.array[D + V + i_wi, i] * p.common_weight
+
+
+def save(file: str, step: int, p: ParticleGroup) -> str:
+    """save(file, step, p) (particle_group.jl:152-165): the particle state as
+    "x" (D x N), "v" (V x N) and "w" (n_weights x N) in
+    "<file>-<step:06d>.npz" (the reference writes the same three arrays to a
+    JLD2 file of that name; NumPy's archive is the format at hand here, and
+    Julia's NPZ package reads it). Call it after a synchronising call (or
+    to_host) so that the array holds the GPU state. Returns the path."""
+    D, V = p.dims
+    path = "%s-%06d.npz" % (file, int(step))
+    np.savez(path, x=np.asfortranarray(p.array[:D]), v=np.asfortranarray(p.array[D:D + V]),
+             w=np.asfortranarray(p.array[D + V:]))
+    return path
+
+
+def load(path: str, p: ParticleGroup) -> ParticleGroup:
+    """The state written by save, back into p.array (shapes must match); a
+    splitting then takes it with to_device (resume)."""
+    D, V = p.dims
+    with np.load(path, allow_pickle=False) as z:
+        x, v, w = z["x"], z["v"], z["w"]
+    if x.shape != (D, p.n_particles) or v.shape != (V, p.n_particles) or \
+            w.shape != (p.array.shape[0] - D - V, p.n_particles):
+        raise ValueError("the saved state does not match the particle group's shape")
+    p.array[:D], p.array[D:D + V], p.array[D + V:] = x, v, w
+    return p
 
 
 # ---------------------------------------------------------------------------
@@ -305,31 +338,27 @@ def _bspline_basis(degree: int, x: float):
     return b
 
 
+_FN1D = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_double, ctypes.c_void_p)
+
+
 def compute_rhs_from_function_(coefs_dofs, self: Maxwell1DFEM, func, degree: int):
-    """compute_rhs_from_function! (maxwell_1d_fem.jl:188-220): host set-up
-    (the reference evaluates the user's function on the host)."""
-    x, w = np.polynomial.legendre.leggauss(degree + 1)
-    x = 0.5 * (x + 1.0)
-    w = 0.5 * w
-    bspl = [_bspline_basis(degree, xk) for xk in x]
-    dx = self.delta_x
-    for i in range(1, self.n_dofs + 1):
-        coef = 0.0
-        for j in range(1, degree + 2):
-            for k in range(1, degree + 2):
-                coef = coef + w[k - 1] * func(dx * (((x[k - 1] + i) + j) - 2)) * \
-                    bspl[k - 1][degree + 2 - j - 1]
-        coefs_dofs[i - 1] = coef * dx
+    """compute_rhs_from_function! (maxwell_1d_fem.jl:188-220) by the library
+    (gempic_maxwell1d_compute_rhs_from_function), which calls func on the
+    host at the Gauss-Legendre points."""
+    cb = _FN1D(lambda x, ctx: float(func(x)))
+    check(_lib.load().gempic_maxwell1d_compute_rhs_from_function(self._h, cb, None, int(degree),
+                                                                 ptr(coefs_dofs)))
+    return coefs_dofs
 
 
 def l2projection_(coefs_dofs, self: Maxwell1DFEM, func, degree: int):
-    """l2projection! (maxwell_1d_fem.jl:347-374). The reference solves into
-    its workspace and then returns coefs_dofs ./ delta_x, i.e. the rhs
-    divided by dx; that result is kept."""
-    if degree not in (self.s_deg_0, self.s_deg_0 - 1):
-        raise ValueError(f"degree {degree} not available")
-    compute_rhs_from_function_(coefs_dofs, self, func, degree)
-    coefs_dofs[:] = coefs_dofs / self.delta_x
+    """l2projection! (maxwell_1d_fem.jl:347-374) by the library. The
+    reference solves into its workspace and then returns coefs_dofs ./
+    delta_x, i.e. the rhs divided by dx; that result is kept."""
+    cb = _FN1D(lambda x, ctx: float(func(x)))
+    check(_lib.load().gempic_maxwell1d_l2projection(self._h, cb, None, int(degree),
+                                                    ptr(coefs_dofs)))
+    return coefs_dofs
 
 
 # ---------------------------------------------------------------------------
@@ -430,6 +459,7 @@ class HamiltonianSplitting:
         nx = kernel_smoother_0.n_dofs
         self.j_dofs = [np.zeros(nx), np.zeros(nx)]
         self.sync = sync
+        _ON_GPU[particle_group] = weakref.ref(self)
         self.kernels_per_step = 1
         self.arithmetic = "reference"
         pg = particle_group
@@ -613,14 +643,16 @@ class Ensemble:
             self._e = None
 
 
-def ensemble_strang_splitting_(ens: Ensemble, dt, number_steps: int):
+def ensemble_strang_splitting_(ens: Ensemble, dt, number_steps: int, graph: bool = False):
     """strang_splitting!(h, dt_i, number_steps) for every splitting h of ens
-    (dt: one value for all, or one per splitting)."""
+    (dt: one value for all, or one per splitting). graph=True replays the
+    regular steps as a HIP graph of the step's launches (the same results)."""
     dts = np.broadcast_to(np.asarray(dt, dtype=np.float64), (len(ens.splittings),)).copy()
     for h in ens.splittings:
         if h.sync:
             h.to_device()
-    check(_lib.load().gempic_ensemble_strang_splitting(ens._e, ptr(dts), int(number_steps), 0))
+    check(_lib.load().gempic_ensemble_strang_splitting(ens._e, ptr(dts), int(number_steps),
+                                                       _lib.GRAPH if graph else 0))
     for h in ens.splittings:
         if h.sync:
             h.to_host()
@@ -798,34 +830,84 @@ def push_x_accumulate_j_(splitting: HamiltonianSplittingBoris, dt: float):
 # ---------------------------------------------------------------------------
 # diagnostics.jl
 
-def solve_poisson_(efield_dofs, h: HamiltonianSplitting, rho):
-    """solve_poisson!(efield, particle_group, kernel_smoother_0, maxwell, rho)
-    (diagnostics.jl:15-31) on the splitting's particles (the GPU copy)."""
+def _splitting_of(particle_group, kernel_smoother_0=None, maxwell_solver=None):
+    """The GPU splitting that holds this particle group (and, when given,
+    these smoother and solver objects)."""
+    ref = _ON_GPU.get(particle_group)
+    h = ref() if ref is not None else None
+    if h is None or (kernel_smoother_0 is not None and h.kernel_smoother_0 is not kernel_smoother_0) \
+            or (maxwell_solver is not None and h.maxwell_solver is not maxwell_solver):
+        raise ValueError("the particle group is not held by a HamiltonianSplitting on the GPU "
+                         "with this smoother and solver (the path has no CPU fallback)")
+    return h
+
+
+def solve_poisson_(efield_dofs, *args):
+    """solve_poisson!(efield_dofs, particle_group, kernel_smoother_0,
+    maxwell_solver, rho) (diagnostics.jl:15-31), the reference's signature:
+    the group's particles are deposited on the GPU by the splitting that holds
+    them. solve_poisson_(efield_dofs, splitting, rho) names the splitting
+    directly."""
+    if len(args) == 4:
+        pg, ks0, mx, rho = args
+        h = _splitting_of(pg, ks0, mx)
+    elif len(args) == 2:
+        h, rho = args
+    else:
+        raise TypeError("solve_poisson_(efield_dofs, particle_group, kernel_smoother_0, "
+                        "maxwell_solver, rho)")
     if h.sync:
         h.to_device(fields=False)
     check(_lib.load().gempic_solve_poisson(h._h, ptr(efield_dofs), ptr(rho)))
+    return efield_dofs
 
 
 class TimeHistoryDiagnostics:
-    """TimeHistoryDiagnostics (diagnostics.jl:128-170): rows of the 11
-    columns, filled by write_step_."""
+    """TimeHistoryDiagnostics(particle_group, maxwell_solver,
+    kernel_smoother_0, kernel_smoother_1) (diagnostics.jl:128-170): rows of
+    the 11 columns, filled by write_step_ on the GPU splitting that holds the
+    particle group. TimeHistoryDiagnostics(splitting) names it directly."""
 
     COLUMNS = ("Time", "KineticEnergy", "Momentum1", "Momentum2", "PotentialEnergyE1",
                "PotentialEnergyE2", "PotentialEnergyB3", "Transfer", "VVB", "Poynting",
                "ErrorPoisson")
 
-    def __init__(self, h: HamiltonianSplitting):
-        self.splitting = h
+    def __init__(self, *args):
+        if len(args) == 1:
+            self.splitting = args[0]
+        elif len(args) == 4:
+            pg, mx, ks0, ks1 = args
+            self.splitting = _splitting_of(pg, ks0, mx)
+            if self.splitting.kernel_smoother_1 is not ks1:
+                raise ValueError("kernel_smoother_1 is not the splitting's")
+        else:
+            raise TypeError("TimeHistoryDiagnostics(particle_group, maxwell_solver, "
+                            "kernel_smoother_0, kernel_smoother_1)")
+        self.particle_group = self.splitting.particle_group
         self.data = []
 
 
-def write_step_(thdiag: TimeHistoryDiagnostics, time: float, degree: int, efield_dofs_n=None,
-                efield_poisson=None):
-    """write_step! (diagnostics.jl:186-250) of the splitting's current fields
-    and particles; efield_dofs_n defaults to the current fields."""
+def write_step_(thdiag: TimeHistoryDiagnostics, time: float, degree: int, *fields,
+                efield_dofs_n=None, efield_poisson=None):
+    """write_step!(thdiag, time, degree, efield_dofs, bfield_dofs,
+    efield_dofs_n, efield_poisson) (diagnostics.jl:186-250), the reference's
+    signature: the given fields are uploaded and the row computed on the GPU.
+    Without the four field arguments the splitting's current fields are used
+    (efield_dofs_n and efield_poisson by keyword; defaults: the current
+    fields, and Poisson recomputed)."""
     h = thdiag.splitting
+    if len(fields) == 4:
+        efield_dofs, bfield_dofs, efield_dofs_n, efield_poisson = fields
+    elif fields:
+        raise TypeError("write_step_(thdiag, time, degree, efield_dofs, bfield_dofs, "
+                        "efield_dofs_n, efield_poisson)")
     if h.sync:
         h.to_device()
+    if len(fields) == 4:
+        check(_lib.load().gempic_set_fields(
+            h._h, ptr(np.ascontiguousarray(efield_dofs[0], dtype=np.float64)),
+            ptr(np.ascontiguousarray(efield_dofs[1], dtype=np.float64)),
+            ptr(np.ascontiguousarray(bfield_dofs, dtype=np.float64))))
     out = np.zeros(11)
     e1n = e2n = None
     if efield_dofs_n is not None:

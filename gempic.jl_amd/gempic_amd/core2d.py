@@ -26,7 +26,7 @@ __all__ = [
     "TwoDLinearSolverSplineMass", "solve", "multiply_mass_2dkron_", "compute_rho_from_e_",
     "compute_fem_rhs_", "l2projection", "spline_fem_mass_line", "spline_fem_compute_mass_eig",
     "add_charge_device_", "add_charge_device_noalloc_", "evaluate_device", "evaluate_multiple_device", "pmc2d_sync",
-    "b_to_pp_2d_",
+    "b_to_pp_2d_", "HamiltonianSplitting2D3V",
 ]
 
 
@@ -79,8 +79,10 @@ class ParticleMeshCoupling2D:
     (particle_mesh_coupling_2d.jl:12-45)."""
 
     def __init__(self, pg: ParticleGroup, grid: TwoDGrid, degree: int, smoothing_type: str):
-        if tuple(pg.dims) != (2, 2):
-            raise TypeError("ParticleMeshCoupling2D needs a ParticleGroup{2,2}")
+        # {2,2} as in the reference; {2,3} for the 2D3V splitting, which
+        # couples the same 2D splines to particles with three velocities
+        if tuple(pg.dims) not in ((2, 2), (2, 3)):
+            raise TypeError("ParticleMeshCoupling2D needs a ParticleGroup{2,2} or {2,3}")
         if smoothing_type not in _SMOOTHING:
             raise ValueError(f"Smoothing Type {smoothing_type} not implemented for "
                              "kernel_smoother_spline_2d.")
@@ -522,3 +524,121 @@ def l2projection(solver: TwoDMaxwell, f, component: int, form: int):
     if form == 2:
         return solve(solver.inv_mass_2[component - 1], rhs)
     return None
+
+
+# ---------------------------------------------------------------------------
+# The 2D3V Hamiltonian splitting (hamiltonian_splitting_2d3v.jl is empty in
+# the reference; the splitting of the GEMPIC paper on its 2D pieces)
+
+class HamiltonianSplitting2D3V:
+    """HamiltonianSplitting{2,3}(maxwell_solver, kernel_smoother_0,
+    kernel_smoother_1, particle_group, e_dofs, b_dofs), the 2D3V form of
+    hamiltonian_splitting.jl:20-86: ParticleGroup{2,3} (rows x1, x2, v1, v2,
+    v3, w), a TwoDMaxwell of degree p, the two smoothers of degrees p and
+    p-1 (galerkin), and the aliased field vectors e_dofs = [E1, E2, E3],
+    b_dofs = [B1, B2, B3] (nx*ny each, E1 of degrees (p-1, p), E2 (p, p-1),
+    E3 (p, p), B1 (p, p-1), B2 (p-1, p), B3 (p-1, p-1)). As the 1D splitting,
+    every call uploads the state, runs on the GPU and downloads it back
+    (sync=True), or leaves it on the GPU (sync=False, to_host/to_device)."""
+
+    def __init__(self, maxwell_solver: TwoDMaxwell, kernel_smoother_0, kernel_smoother_1,
+                 particle_group: ParticleGroup, e_dofs, b_dofs, sync: bool = True):
+        if tuple(particle_group.dims) != (2, 3):
+            raise AssertionError("the 2D3V splitting needs a ParticleGroup{2,3}")
+        p = maxwell_solver.s_deg_0
+        if kernel_smoother_0.degree != p or kernel_smoother_1.degree != p - 1:
+            raise AssertionError("smoothers of degrees p and p-1 of the Maxwell solver")
+        self.dims = (2, 3)
+        self.maxwell_solver = maxwell_solver
+        self.kernel_smoother_0 = kernel_smoother_0
+        self.kernel_smoother_1 = kernel_smoother_1
+        self.particle_group = particle_group
+        mesh = maxwell_solver.mesh
+        nn = mesh.nx * mesh.ny
+        self.e_dofs = [_grid(e, nn, "e_dofs") for e in e_dofs]
+        self.b_dofs = [_grid(b, nn, "b_dofs") for b in b_dofs]
+        self.j_dofs = [np.zeros(nn) for _ in range(3)]
+        self.sync = sync
+        pg = particle_group
+        h = ctypes.c_void_p()
+        check(_lib.load().gempic_splitting2d_create(
+            mesh.nx, mesh.ny, mesh.xmin, mesh.xmax, mesh.ymin, mesh.ymax, p, pg.n_particles,
+            pg.charge, pg.mass, pg.common_weight, ctypes.byref(h)))
+        self._h = h
+        self.to_device()
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            _lib.load().gempic_splitting2d_destroy(h)
+            self._h = None
+
+    def to_device(self, particles: bool = True, fields: bool = True):
+        L = _lib.load()
+        if particles:
+            a = self.particle_group.array
+            assert a.flags.f_contiguous
+            check(L.gempic_splitting2d_set_particles(self._h, ptr(a), a.shape[0], a.shape[1]))
+        if fields:
+            check(L.gempic_splitting2d_set_fields(self._h, *(ptr(f) for f in self.e_dofs),
+                                                  *(ptr(f) for f in self.b_dofs)))
+
+    def to_host(self, particles: bool = True, fields: bool = True):
+        L = _lib.load()
+        if particles:
+            a = self.particle_group.array
+            check(L.gempic_splitting2d_get_particles(self._h, ptr(a), a.shape[0], a.shape[1]))
+        if fields:
+            check(L.gempic_splitting2d_get_fields(self._h, *(ptr(f) for f in self.e_dofs),
+                                                  *(ptr(f) for f in self.b_dofs)))
+            check(L.gempic_splitting2d_get_j(self._h, *(ptr(j) for j in self.j_dofs)))
+
+    def _run(self, fn, *args):
+        if self.sync:
+            self.to_device()
+        check(fn(self._h, *args))
+        if self.sync:
+            self.to_host()
+
+    def set_sort_interval(self, k: int):
+        check(_lib.load().gempic_splitting2d_set_sort_interval(self._h, int(k)))
+
+    def operatorHB(self, dt: float):
+        self._run(_lib.load().gempic_splitting2d_operator_hb, float(dt))
+
+    def operatorHE(self, dt: float):
+        self._run(_lib.load().gempic_splitting2d_operator_he, float(dt))
+
+    def operatorHp1(self, dt: float):
+        self._run(_lib.load().gempic_splitting2d_operator_hp1, float(dt))
+
+    def operatorHp2(self, dt: float):
+        self._run(_lib.load().gempic_splitting2d_operator_hp2, float(dt))
+
+    def operatorHp3(self, dt: float):
+        self._run(_lib.load().gempic_splitting2d_operator_hp3, float(dt))
+
+    def strang_splitting(self, dt: float, number_steps: int, async_: bool = False):
+        """strang_splitting!(h, dt, number_steps): HB HE Hp3 Hp2 Hp1 Hp2 Hp3 HE HB."""
+        self._run(_lib.load().gempic_splitting2d_strang_splitting, float(dt), int(number_steps),
+                  16 if async_ else 0)
+
+    def synchronize(self):
+        check(_lib.load().gempic_splitting2d_synchronize(self._h))
+
+    def rho(self) -> np.ndarray:
+        """add_charge! of the particles on the GPU (degree (p, p), q = charge w
+        common_weight)."""
+        if self.sync:
+            self.to_device(fields=False)
+        out = np.zeros(self.maxwell_solver._nn)
+        check(_lib.load().gempic_splitting2d_rho(self._h, ptr(out)))
+        return out
+
+    def rho_from_e(self) -> np.ndarray:
+        """compute_rho_from_e! of the GPU fields."""
+        if self.sync:
+            self.to_device(particles=False)
+        out = np.zeros(self.maxwell_solver._nn)
+        check(_lib.load().gempic_splitting2d_rho_from_e(self._h, ptr(out)))
+        return out

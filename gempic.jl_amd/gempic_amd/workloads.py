@@ -16,6 +16,8 @@ import math
 
 import numpy as np
 
+from .core import _bspline_basis
+
 
 def sample_cos_density(rng, n, L, k, alpha):
     """Rejection sampling of 1 + alpha*cos(k x) on [0, L)."""
@@ -32,21 +34,21 @@ def sample_cos_density(rng, n, L, k, alpha):
     return out
 
 
-def bspline_basis(degree, x):
-    b = [0.0] * (degree + 1)
-    b[0] = 1.0
-    for j in range(1, degree + 1):
-        xx = -x
-        jr = float(j)
-        inv_j = 1.0 / jr
-        saved = 0.0
-        for r in range(j):
-            xx = xx + 1.0
-            temp = b[r] * inv_j
-            b[r] = saved + xx * temp
-            saved = (jr - xx) * temp
-        b[j] = saved
-    return b
+def _gauss_legendre(n):
+    """FastGaussQuadrature.gausslegendre(n) for n <= 4, in closed form (the
+    rule compute_rhs_from_function! takes, maxwell_1d_fem.jl:195)."""
+    if n == 1:
+        return [0.0], [2.0]
+    if n == 2:
+        return [-1.0 / math.sqrt(3.0), 1.0 / math.sqrt(3.0)], [1.0, 1.0]
+    if n == 3:
+        r = math.sqrt(3.0 / 5.0)
+        return [-r, 0.0, r], [5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0]
+    a = 2.0 / 7.0 * math.sqrt(6.0 / 5.0)
+    x = [-math.sqrt(3.0 / 7.0 + a), -math.sqrt(3.0 / 7.0 - a), math.sqrt(3.0 / 7.0 - a),
+         math.sqrt(3.0 / 7.0 + a)]
+    w0, w1 = (18.0 - math.sqrt(30.0)) / 36.0, (18.0 + math.sqrt(30.0)) / 36.0
+    return x, [w0, w1, w1, w0]
 
 
 def l2projection_cos(nx, L, degree, amp, k):
@@ -54,10 +56,10 @@ def l2projection_cos(nx, L, degree, amp, k):
     returns its right-hand side divided by dx (the mass solve lands in the
     workspace only)."""
     dx = L / nx
-    xg, wg = np.polynomial.legendre.leggauss(degree + 1)
-    xg = 0.5 * (xg + 1.0)
-    wg = 0.5 * wg
-    bspl = [bspline_basis(degree, xk) for xk in xg]
+    xg, wg = _gauss_legendre(degree + 1)
+    xg = [0.5 * (x + 1.0) for x in xg]
+    wg = [0.5 * w for w in wg]
+    bspl = [_bspline_basis(degree, xk) for xk in xg]
     out = np.zeros(nx)
     for i in range(1, nx + 1):
         coef = 0.0
@@ -131,3 +133,31 @@ def weibel_adjoint_1d2v(n, seed=1, nx=32, k=1.25, alpha=1e-4, beta=1e-4,
     return dict(xmin=0.0, xmax=L, nx=nx, degree=degree, particles=parts, b_pattern=p,
                 b0=beta * p, dx_dalpha=dx_dalpha, common_weight=1.0 / n, dt=dt, k=k,
                 alpha=alpha, beta=beta)
+
+
+def landau_2d3v(n, seed=1, nx=64, ny=64, k=0.5, alpha=0.01, degree=3):
+    """2D3V Landau damping: (x1, x2) in [0, 2pi/k)^2 with density
+    1 + alpha cos(k x1) cos(k x2) (rejection sampling), v ~ N(0, I_3),
+    weight = the domain's area (the reference sampler's volume weight,
+    particle_sampling.jl:279 in 1D), common_weight = 1/N. Rows x1, x2, v1,
+    v2, v3, w (ParticleGroup{2,3})."""
+    L = 2.0 * math.pi / k
+    rng = np.random.default_rng(seed)
+    x1 = np.empty(n)
+    x2 = np.empty(n)
+    filled = 0
+    bound = 1.0 + abs(alpha)
+    while filled < n:
+        m = int((n - filled) * 1.05) + 16
+        u, v = rng.random(m) * L, rng.random(m) * L
+        acc = rng.random(m) * bound < 1.0 + alpha * np.cos(k * u) * np.cos(k * v)
+        take = min(int(acc.sum()), n - filled)
+        x1[filled: filled + take] = u[acc][:take]
+        x2[filled: filled + take] = v[acc][:take]
+        filled += take
+    parts = np.empty((6, n), order="F")
+    parts[0], parts[1] = x1, x2
+    parts[2:5] = rng.standard_normal((3, n))
+    parts[5] = L * L
+    return dict(xmin=0.0, xmax=L, ymin=0.0, ymax=L, nx=nx, ny=ny, degree=degree,
+                particles=parts, common_weight=1.0 / n, dt=0.05)

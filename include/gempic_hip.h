@@ -100,8 +100,9 @@ int gempic_operator_hp2(gempic_splitting *h, double dt);
 /* strang_splitting! (hamiltonian_splitting.jl:98-108; _1d1v.jl:11-21).
  * GEMPIC_STRANG_FUSED (default) runs one particle pass per step: the
  * step-boundary HE pair and Hp2-Hp1-Hp2 read only B between two field
- * solves, so they are fused; the field solves run in the pass's last
- * workgroup. GEMPIC_STRANG_SUBSTEP launches the seven sub-steps one by one.
+ * solves, so they are fused; a second, small kernel then sums the
+ * workgroups' deposits and runs the field solves. GEMPIC_STRANG_SUBSTEP
+ * launches the seven sub-steps one by one.
  * Particle states and fields after each step are those of the sub-step
  * sequence. With GEMPIC_ASYNC the call returns once the work is queued. */
 #define GEMPIC_STRANG_FUSED 0
@@ -124,7 +125,12 @@ int gempic_synchronize(gempic_splitting *h);
 typedef struct gempic_ensemble gempic_ensemble;
 int gempic_ensemble_create(int n_instances, gempic_splitting *const *handles, gempic_ensemble **out);
 int gempic_ensemble_destroy(gempic_ensemble *e);
-/* dt: one time step per splitting (n_instances doubles) */
+/* dt: one time step per splitting (n_instances doubles). flags:
+ * GEMPIC_GRAPH replays the steps between the first and the last as a HIP
+ * graph of the step's launches (captured once per ensemble, again after a
+ * change of the launch geometry); the due particle sorts run between
+ * replays. Results equal those of the stream-queued steps bit for bit. */
+#define GEMPIC_GRAPH 64
 int gempic_ensemble_strang_splitting(gempic_ensemble *e, const double *dt, int64_t n_steps, int flags);
 /* The reverse mode of every splitting of the ensemble (gempic_strang_splitting_adjoint
  * per splitting, batched: one launch per pass, re-run and field part for all
@@ -332,6 +338,19 @@ int gempic_maxwell1d_compute_e_from_b(gempic_maxwell1d *m, double *e, double dt,
 /* compute_b_from_e! (:407-420) */
 int gempic_maxwell1d_compute_b_from_e(gempic_maxwell1d *m, double *b, double dt,
                                       const double *e);
+/* compute_rhs_from_function! (:188-220): coefs[i] = dx sum over the
+ * support of spline i and the degree+1 Gauss-Legendre points of each cell of
+ * w f(x) N_i(x); f is called on the host (a C callback, e.g. a Julia
+ * @cfunction, with ctx passed through). 0 <= degree <= 3. */
+typedef double (*gempic_fn1d)(double x, void *ctx);
+int gempic_maxwell1d_compute_rhs_from_function(gempic_maxwell1d *m, gempic_fn1d f, void *ctx,
+                                               int degree, double *coefs);
+/* l2projection! (:347-374) for degree s_deg_0 or s_deg_0 - 1. As in the
+ * reference, the result is the right-hand side over dx (the mass solve lands
+ * in the reference's workspace and is not returned). */
+int gempic_maxwell1d_l2projection(gempic_maxwell1d *m, gempic_fn1d f, void *ctx, int degree,
+                                  double *coefs);
+
 /* inner_product (:461-475) and l2norm_squared (:299-313) */
 int gempic_maxwell1d_inner_product(gempic_maxwell1d *m, const double *c1, const double *c2,
                                    int degree, double *out);
@@ -457,6 +476,63 @@ int gempic_maxwell2d_compute_rho_from_e(gempic_maxwell2d *m, const double *e1, c
 /* compute_e_from_rho! (:199-201 -> poisson_2d_fem.jl:237-262) */
 int gempic_maxwell2d_compute_e_from_rho(gempic_maxwell2d *m, const double *rho, double *e1,
                                         double *e2);
+
+/* ------------------------------------------------------------------------
+ * The 2D3V Hamiltonian splitting. The reference includes
+ * hamiltonian_splitting_2d3v.jl (src/GEMPIC.jl:33) but the file is empty;
+ * this is the GEMPIC paper's splitting H = Hp1 + Hp2 + Hp3 + HE + HB on the
+ * reference's 2D pieces, in the form of its 1D2V splitting
+ * (hamiltonian_splitting.jl:20-108, hamiltonian_splitting_1d2v.jl):
+ * ParticleGroup{2,3} (particle_group.jl:15-46; rows x1, x2, v1, v2, v3, w,
+ * ld = 6), the 2D coupling's shape factor (particle_mesh_coupling_2d.jl:54-82)
+ * with one degree per direction, and TwoDMaxwell(mesh, degree)
+ * (maxwell_2d_fem.jl:26-87) with E1 in splines of degrees (p-1, p), E2
+ * (p, p-1), E3 (p, p), B1 (p, p-1), B2 (p-1, p), B3 (p-1, p-1):
+ *   operatorHp1/Hp2: the position moves along x1 / x2 with the exact line
+ *     integral of the degree p-1 splines (as update_jv!,
+ *     particle_mesh_coupling_1d.jl:385-414), the current j1 / j2 and the
+ *     magnetic kicks of the other velocities, then E -= M1^{-1} j
+ *     (compute_e_from_j!, maxwell_2d_fem.jl:455-459);
+ *   operatorHp3: the kicks of v1, v2 by v3 B and the current j3 = dt q v3 S;
+ *   operatorHE: v += dt qm E(x), then compute_b_from_e! (:423-444);
+ *   operatorHB: compute_e_from_b! (:370-412);
+ *   strang_splitting: HB HE Hp3 Hp2 Hp1 Hp2 Hp3 HE HB (half steps around Hp1).
+ * Particles are kept sorted by cell on the device (every `interval` steps,
+ * default 20); get_particles restores the caller's order. Degrees 1..3,
+ * 2*degree+1 <= nx, ny and nx*ny <= 4096. The current deposits add in an
+ * order that depends on timing between wave flushes: results agree with the
+ * oracle to rounding, not bitwise.
+ * ------------------------------------------------------------------------ */
+typedef struct gempic_splitting2d gempic_splitting2d;
+int gempic_splitting2d_create(int64_t nx, int64_t ny, double xmin, double xmax, double ymin,
+                              double ymax, int degree, int64_t n_particles, double charge,
+                              double mass, double common_weight, gempic_splitting2d **out);
+int gempic_splitting2d_destroy(gempic_splitting2d *h);
+int gempic_splitting2d_set_particles(gempic_splitting2d *h, const double *aos, int64_t ld, int64_t n);
+int gempic_splitting2d_get_particles(gempic_splitting2d *h, double *aos, int64_t ld, int64_t n);
+/* e_dofs and b_dofs, nx*ny each (x fastest); any pointer may be NULL */
+int gempic_splitting2d_set_fields(gempic_splitting2d *h, const double *e1, const double *e2,
+                                  const double *e3, const double *b1, const double *b2,
+                                  const double *b3);
+int gempic_splitting2d_get_fields(gempic_splitting2d *h, double *e1, double *e2, double *e3,
+                                  double *b1, double *b2, double *b3);
+/* j_dofs as left by the last Hp1 / Hp2 / Hp3 (j3 includes its dt) */
+int gempic_splitting2d_get_j(gempic_splitting2d *h, double *j1, double *j2, double *j3);
+int gempic_splitting2d_operator_hb(gempic_splitting2d *h, double dt);
+int gempic_splitting2d_operator_he(gempic_splitting2d *h, double dt);
+int gempic_splitting2d_operator_hp1(gempic_splitting2d *h, double dt);
+int gempic_splitting2d_operator_hp2(gempic_splitting2d *h, double dt);
+int gempic_splitting2d_operator_hp3(gempic_splitting2d *h, double dt);
+/* flags: GEMPIC_ASYNC returns once the work is queued */
+int gempic_splitting2d_strang_splitting(gempic_splitting2d *h, double dt, int64_t n_steps, int flags);
+int gempic_splitting2d_synchronize(gempic_splitting2d *h);
+int gempic_splitting2d_set_sort_interval(gempic_splitting2d *h, int64_t interval);
+/* add_charge! (particle_mesh_coupling_2d.jl:94-105) of the handle's particles
+ * with degree (p, p) and q = charge w common_weight, into rho (nx*ny, not
+ * accumulated), and compute_rho_from_e! (maxwell_2d_fem.jl:468-499) of its
+ * fields: their difference is Gauss's law, constant over the splitting. */
+int gempic_splitting2d_rho(gempic_splitting2d *h, double *rho);
+int gempic_splitting2d_rho_from_e(gempic_splitting2d *h, double *rho);
 
 #ifdef __cplusplus
 }

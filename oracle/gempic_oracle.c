@@ -801,6 +801,8 @@ struct or_split {
     or_maxwell1d *m;
     or_pmc1d ks0, ks1;
     double Lx, x_min, delta_x;
+    int ntasks;              /* n_jobs = nthreads() (hamiltonian_splitting.jl:62) */
+    double *tbuf;            /* [ntasks][nx]: each task's buffer = zero(j_dofs) */
 };
 
 /* HamiltonianSplitting{D,V} constructor (hamiltonian_splitting.jl:41-86). */
@@ -837,11 +839,34 @@ or_split *or_split_create(int64_t nx, double xmin, double xmax, int maxwell_degr
     h->x_min = h->m->xmin;
     h->Lx = h->m->Lx;
     h->delta_x = h->Lx / (double)nx;
+    h->ntasks = 1;
     return h;
+}
+
+/* The reference's task split (hamiltonian_splitting.jl:61-66): n_jobs =
+ * nthreads(), @assert np % n_jobs == 0, chunks = partition(1:np, np / n_jobs).
+ * With ntasks > 1 the 1d2v Hp1 and Hp2 deposit each chunk into its own
+ * buffer on its own thread and sum the buffers in chunk order,
+ * reduce(+, fetch.(tasks)) (hamiltonian_splitting_1d2v.jl:48-90,141-164;
+ * Base's reduce over fewer than 1024 vectors is the left fold
+ * ((b1 + b2) + b3) + ...), and HE's @threads loop runs on the threads:
+ * the reference's own results with nthreads() = ntasks. */
+int or_split_set_tasks(or_split *h, int ntasks)
+{
+    if (ntasks < 1 || h->np % ntasks != 0) return -1;
+    free(h->tbuf);
+    h->tbuf = NULL;
+    h->ntasks = ntasks;
+    if (ntasks > 1) {
+        h->tbuf = (double *)calloc((size_t)ntasks * (size_t)h->m->n, sizeof(double));
+        if (!h->tbuf) { h->ntasks = 1; return -1; }
+    }
+    return 0;
 }
 
 int or_split_set_particles(or_split *h, double *particles, int64_t n)
 {
+    if (h->ntasks > 1 && n % h->ntasks != 0) return -1;
     h->a = particles;
     h->np = n;
     return 0;
@@ -853,6 +878,7 @@ void or_split_destroy(or_split *h)
     or_maxwell1d_destroy(h->m);
     free(h->j1); free(h->j2);
     free(h->e1m); free(h->e2m); free(h->bm);
+    free(h->tbuf);
     free(h);
 }
 
@@ -885,6 +911,7 @@ void or_operator_hb(or_split *h, double dt)
 void or_operator_he(or_split *h, double dt)
 {
     double qm = h->q_over_m;
+#pragma omp parallel for num_threads(h->ntasks) schedule(static) if (h->ntasks > 1)
     for (int64_t i = 0; i < h->np; ++i) {
         double v1 = PV1(h, i), v2 = PV2(h, i);
         double xi = PX(h, i);
@@ -896,14 +923,22 @@ void or_operator_he(or_split *h, double dt)
     or_compute_b_from_e(h->b, h->m, dt, h->e2);
 }
 
-/* operatorHp2 1D2V (hamiltonian_splitting_1d2v.jl:129-176), one task. */
-void or_operator_hp2(or_split *h, double dt)
+/* h->j = reduce(+, task buffers): the left fold in chunk order */
+static void reduce_tasks(const or_split *h, double *j)
 {
     int64_t nx = h->m->n;
-    memset(h->j1, 0, (size_t)nx * sizeof(double));
-    memset(h->j2, 0, (size_t)nx * sizeof(double));
+    for (int64_t k = 0; k < nx; ++k) {
+        double v = h->tbuf[k] + h->tbuf[nx + k];
+        for (int t = 2; t < h->ntasks; ++t) v = v + h->tbuf[(int64_t)t * nx + k];
+        j[k] = v;
+    }
+}
+
+/* Hp2's particle loop over particles [i0, i1) into j (one task's body) */
+static void hp2_chunk(or_split *h, double dt, int64_t i0, int64_t i1, double *j)
+{
     double qm = h->q_over_m;
-    for (int64_t i = 0; i < h->np; ++i) {
+    for (int64_t i = i0; i < i1; ++i) {
         double x1 = PX(h, i), v1 = PV1(h, i), v2 = PV2(h, i);
         double b = or_evaluate(&h->ks1, x1, h->b);
         v1 = v1 + dt * qm * v2 * b;
@@ -912,13 +947,60 @@ void or_operator_hp2(or_split *h, double dt)
         w = w * h->charge;
         w = w * h->common_weight;
         w = w * v2;
-        or_add_charge(h->j2, &h->ks0, x1, w);
+        or_add_charge(j, &h->ks0, x1, w);
+    }
+}
+
+/* operatorHp2 1D2V (hamiltonian_splitting_1d2v.jl:129-176). */
+void or_operator_hp2(or_split *h, double dt)
+{
+    int64_t nx = h->m->n;
+    memset(h->j1, 0, (size_t)nx * sizeof(double));
+    memset(h->j2, 0, (size_t)nx * sizeof(double));
+    if (h->ntasks == 1) {
+        hp2_chunk(h, dt, 0, h->np, h->j2);
+    } else {
+        int64_t c = h->np / h->ntasks;
+        memset(h->tbuf, 0, (size_t)h->ntasks * (size_t)nx * sizeof(double));
+#pragma omp parallel for num_threads(h->ntasks) schedule(static, 1)
+        for (int t = 0; t < h->ntasks; ++t) hp2_chunk(h, dt, t * c, (t + 1) * c, h->tbuf + t * nx);
+        reduce_tasks(h, h->j2);
     }
     for (int64_t k = 0; k < nx; ++k) h->j2[k] = h->j2[k] * dt;
     or_compute_e_from_j(h->e2, h->m, h->j2, 2);
 }
 
-/* operatorHp1: 1D2V (hamiltonian_splitting_1d2v.jl:41-112), one task;
+/* Hp1's first task loop over particles [i0, i1) into j
+ * (hamiltonian_splitting_1d2v.jl:48-86) */
+static void hp1_current_chunk(or_split *h, double dt, int64_t i0, int64_t i1, double *j)
+{
+    double qm = h->q_over_m;
+    for (int64_t i = i0; i < i1; ++i) {
+        double x_old = PX(h, i), v1_old = PV1(h, i), v2_old = PV2(h, i);
+        double x_new = x_old + dt * v1_old;
+        double wi = PW(h, i);
+        wi = wi * h->charge;
+        wi = wi * h->common_weight;
+        double v2_new = or_add_current_update_v(j, &h->ks1, x_old, x_new, wi, qm, h->b, v2_old);
+        x_new = or_jl_mod_f64(x_new, h->Lx);
+        PX(h, i) = x_new;
+        PV2(h, i) = v2_new;
+    }
+}
+
+/* Hp1's second task loop, the rho deposit (hamiltonian_splitting_1d2v.jl:90-106) */
+static void hp1_rho_chunk(or_split *h, int64_t i0, int64_t i1, double *j)
+{
+    for (int64_t i = i0; i < i1; ++i) {
+        double x = PX(h, i);
+        double w = PW(h, i);
+        w = w * h->charge;
+        w = w * h->common_weight;
+        or_add_charge(j, &h->ks0, x, w);
+    }
+}
+
+/* operatorHp1: 1D2V (hamiltonian_splitting_1d2v.jl:41-112);
  * 1D1V (hamiltonian_splitting_1d1v.jl:63-98). */
 void or_operator_hp1(or_split *h, double dt)
 {
@@ -940,23 +1022,20 @@ void or_operator_hp1(or_split *h, double dt)
         or_compute_e_from_j(h->e1, h->m, h->j1, 1);
         return;
     }
-    for (int64_t i = 0; i < h->np; ++i) {
-        double x_old = PX(h, i), v1_old = PV1(h, i), v2_old = PV2(h, i);
-        double x_new = x_old + dt * v1_old;
-        double wi = PW(h, i);
-        wi = wi * h->charge;
-        wi = wi * h->common_weight;
-        double v2_new = or_add_current_update_v(h->j1, &h->ks1, x_old, x_new, wi, qm, h->b, v2_old);
-        x_new = or_jl_mod_f64(x_new, h->Lx);
-        PX(h, i) = x_new;
-        PV2(h, i) = v2_new;
-    }
-    for (int64_t i = 0; i < h->np; ++i) {
-        double x = PX(h, i);
-        double w = PW(h, i);
-        w = w * h->charge;
-        w = w * h->common_weight;
-        or_add_charge(h->j2, &h->ks0, x, w);
+    if (h->ntasks == 1) {
+        hp1_current_chunk(h, dt, 0, h->np, h->j1);
+        hp1_rho_chunk(h, 0, h->np, h->j2);
+    } else {
+        int64_t c = h->np / h->ntasks;
+        memset(h->tbuf, 0, (size_t)h->ntasks * (size_t)nx * sizeof(double));
+#pragma omp parallel for num_threads(h->ntasks) schedule(static, 1)
+        for (int t = 0; t < h->ntasks; ++t)
+            hp1_current_chunk(h, dt, t * c, (t + 1) * c, h->tbuf + t * nx);
+        reduce_tasks(h, h->j1);
+        memset(h->tbuf, 0, (size_t)h->ntasks * (size_t)nx * sizeof(double));
+#pragma omp parallel for num_threads(h->ntasks) schedule(static, 1)
+        for (int t = 0; t < h->ntasks; ++t) hp1_rho_chunk(h, t * c, (t + 1) * c, h->tbuf + t * nx);
+        reduce_tasks(h, h->j2);
     }
     or_compute_e_from_j(h->e1, h->m, h->j1, 1);
 }

@@ -232,6 +232,11 @@ void or_operator_hp1(or_split *h, double dt);
 void or_operator_hp2(or_split *h, double dt);
 void or_strang_splitting(or_split *h, double dt, int64_t nsteps);
 int or_split_set_particles(or_split *h, double *particles, int64_t n_particles);
+/* n_jobs = nthreads() of the reference (hamiltonian_splitting.jl:61-66): the
+ * 1d2v Hp1/Hp2 deposits per chunk of n_particles / ntasks particles, on
+ * ntasks threads, summed in chunk order; HE on ntasks threads. Default 1.
+ * Returns -1 when n_particles % ntasks != 0 (the reference's @assert). */
+int or_split_set_tasks(or_split *h, int ntasks);
 
 /* ---------- diagnostics.jl ---------- */
 void or_solve_poisson(double *efield, or_split *h, double *rho);

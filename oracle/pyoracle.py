@@ -99,6 +99,7 @@ def lib():
             "or_boris_staggering": (None, [_VP, _D]),
             "or_boris_strang": (None, [_VP, _D, _I64]),
             "or_split_set_particles": (_I, [_VP, _P, _I64]),
+            "or_split_set_tasks": (_I, [_VP, _I]),
             "or_bspline_deriv": (None, [_I, _D, _P]),
             "or_strang_loss": (_D, [_VP, _D, _I64, _D]),
             "or_strang_adjoint": (_I, [_VP, _D, _I64, _D, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
@@ -364,6 +365,12 @@ class Splitting:
 
     def j(self, component):
         return np.ctypeslib.as_array(lib().or_split_j(self.h, component), shape=(self.nx,)).copy()
+
+    def set_tasks(self, ntasks):
+        """The reference's nthreads() task split of the 1d2v operators
+        (hamiltonian_splitting.jl:61-66)."""
+        if lib().or_split_set_tasks(self.h, int(ntasks)) != 0:
+            raise ValueError("n_particles % ntasks != 0")
 
     def operatorHB(self, dt):
         lib().or_operator_hb(self.h, dt)

@@ -238,8 +238,8 @@ def test_gpu_adjoint_one_step_intervals_store_every_state():
     _close(r.g_b, gb)
 
 
-def _k_run(k, n, steps, K):
-    w = workloads.weibel_adjoint_1d2v(n, seed=2, k=k, alpha=1e-2, beta=1e-3)
+def _k_run(k, n, steps, K, alpha=1e-2, beta=1e-3):
+    w = workloads.weibel_adjoint_1d2v(n, seed=2, k=k, alpha=alpha, beta=beta)
     nx = w["nx"]
     h, pg = _gpu(w["particles"], np.zeros(nx), np.zeros(nx), w["b0"], w["xmax"], nx, (3, 2),
                  w["common_weight"])
@@ -289,3 +289,132 @@ def test_gpu_adjoint_grid_limits(nx):
         hb, _ = _gpu(pb, e1b, e2b, bb, Lb, bad, (3, 2), cw)
         with pytest.raises(ValueError):
             G.strang_splitting_adjoint_(hb, dt, 2)
+
+
+@pytest.mark.parametrize("deg", [(3, 2), (2, 1)])
+def test_gpu_adjoint_matches_forward_mode(deg):
+    """The GPU reverse mode against dual-number forward mode
+    (oracle/tangent1d2v.py, itself equal to the oracle's reverse mode to
+    1e-12 and to central differences of its forward run): the gradient in a
+    random direction of every input group (positions, velocities, E1, E2, B)
+    with cotangents on the final state, 6 steps of a hot state with a
+    re-run interval (K = 4), to 1e-9 relative (the forwards differ in the
+    deposit summation order only)."""
+    import sys
+    import os
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(__file__)), "oracle"))
+    from tangent1d2v import Tangent1D2V
+    parts, e1, e2, b, L, nx = _hot(20000, seed=12)
+    n = parts.shape[1]
+    dt, steps, wl, cw = 0.1, 6, 0.1, 0.01
+    rng = np.random.default_rng(13)
+    cp = np.zeros_like(parts, order="F")
+    cp[:3] = rng.normal(size=(3, n))
+    ct = (rng.normal(size=nx), rng.normal(size=nx), rng.normal(size=nx))
+    d = [rng.normal(size=n) for _ in range(3)]
+    dF = [rng.normal(size=nx) for _ in range(3)]
+    h, pg = _gpu(parts, e1, e2, b, L, nx, deg, cw)
+    r = G.strang_splitting_adjoint_(h, dt, steps, checkpoint_every=4, loss_weight=wl,
+                                    ct_particles=cp, ct_e1=ct[0], ct_e2=ct[1], ct_b=ct[2])
+    g_dot = (sum(float(np.dot(r.g_particles[row], d[row])) for row in range(3)) +
+             float(np.dot(r.g_e1, dF[0])) + float(np.dot(r.g_e2, dF[1])) +
+             float(np.dot(r.g_b, dF[2])))
+    T = Tangent1D2V(nx, 0.0, L, deg[0], deg[1], common_weight=cw)
+    _, dJ, _ = T.strang_jvp(parts, e1, e2, b, d, *dF, dt, steps, wl, cp, *ct)
+    assert abs(g_dot - dJ) <= 1e-9 * abs(dJ), (g_dot, dJ)
+
+
+def test_gpu_adjoint_config4_scale_vs_central_differences():
+    """The adjoint configuration itself: 1,048,576 Weibel particles, 1000
+    steps, checkpoints every 50, alpha = beta = 1e-4, k = 1.25; dJ/dbeta,
+    dJ/dalpha, dJ/dk and a random direction in all initial velocities
+    against central differences of the GPU's own forward run with
+    h = 1e-6 |theta|.
+
+    dJ/dbeta is checked to 1e-6. The other three only to 1e-3, the
+    resolution of central differences at this horizon. The forward map is
+    C^1 but not C^2 (the degree-2 smoother's second derivative jumps where a
+    particle crosses a cell boundary), and over 1000 steps of the unstable
+    run J' changes by up to 1e-4 across |s| <= 4e-6 |theta| without a
+    smooth trend. Measured on the oracle (8192 particles, velocity
+    direction), where the dual-number forward mode equals the reverse mode
+    to 2e-12: the central quotient is off by 3e-6, 9e-8, 1e-9, 7e-6 and
+    3e-7 relative at h = 1e-9, 1e-8, 1e-6, 4e-6 and 1e-5. The beta direction
+    is smooth at this scale because the loss is dominated by the B seed.
+    The derivative itself is checked exactly at the full 1000-step horizon
+    against the oracle's reverse mode in
+    test_gpu_adjoint_full_horizon_matches_oracle_reverse_mode."""
+    n, steps, K = 1048576, 1000, 50
+    print(f"adjoint at the configuration's size: n={n} steps={steps} K={K}", flush=True)
+    w = workloads.weibel_adjoint_1d2v(n, seed=2)
+    r, gpois = _weibel_loss_and_grad(w, steps=steps, K=K)
+    dj_dbeta = float(r.g_b @ w["b_pattern"])
+    dj_dalpha = float((r.g_particles[0] + gpois) @ w["dx_dalpha"])
+    hb = 1e-6 * w["beta"]
+    fd_b = (_weibel_loss_and_grad(w, beta_shift=hb, steps=steps, K=K)[0].loss -
+            _weibel_loss_and_grad(w, beta_shift=-hb, steps=steps, K=K)[0].loss) / (2 * hb)
+    ha = 1e-6 * w["alpha"]
+    fd_a = (_weibel_loss_and_grad(w, alpha_shift=ha, steps=steps, K=K)[0].loss -
+            _weibel_loss_and_grad(w, alpha_shift=-ha, steps=steps, K=K)[0].loss) / (2 * ha)
+    # a random direction in the velocities
+    nx = w["nx"]
+    d = np.random.default_rng(4).normal(size=(2, n)) * 0.02
+
+    def vrun(eps):
+        parts = w["particles"].copy(order="F")
+        parts[1:3] += eps * d
+        h, pg = _gpu(parts, np.zeros(nx), np.zeros(nx), w["b0"], w["xmax"], nx, (3, 2),
+                     w["common_weight"])
+        e1 = np.zeros(nx)
+        G.solve_poisson_(e1, h, np.zeros(nx))
+        h.e_dofs[0][:] = e1
+        return G.strang_splitting_adjoint_(h, w["dt"], steps, checkpoint_every=K)
+    rv = vrun(0.0)
+    ad = float(np.sum(rv.g_particles[1:3] * d))
+    fd_v = (vrun(1e-6).loss - vrun(-1e-6).loss) / 2e-6
+    # the wavenumber
+    k = w["k"]
+    rk, wk, e1k = _k_run(k, n, steps, K, 1e-4, 1e-4)
+    L = wk["xmax"]
+    dj_dl = (rk.g_length + rk.g_weight_sum + float(rk.g_particles[0] @ wk["particles"][0]) / L +
+             float(rk.g_e1 @ e1k) / L)
+    dj_dk = -(L / k) * dj_dl
+    hk = 1e-6 * k
+    fd_k = (_k_run(k + hk, n, steps, K, 1e-4, 1e-4)[0].loss -
+            _k_run(k - hk, n, steps, K, 1e-4, 1e-4)[0].loss) / (2 * hk)
+    print(f"beta {dj_dbeta!r} {fd_b!r}; alpha {dj_dalpha!r} {fd_a!r}; velocities {ad!r} "
+          f"{fd_v!r}; k {dj_dk!r} {fd_k!r}", flush=True)
+    assert abs(fd_b - dj_dbeta) <= 1e-6 * abs(fd_b), (fd_b, dj_dbeta)
+    assert abs(fd_a - dj_dalpha) <= 1e-3 * abs(fd_a), (fd_a, dj_dalpha)
+    assert abs(fd_v - ad) <= 1e-3 * abs(fd_v), (fd_v, ad)
+    assert abs(fd_k - dj_dk) <= 1e-3 * abs(fd_k), (fd_k, dj_dk)
+
+
+def test_gpu_adjoint_full_horizon_matches_oracle_reverse_mode():
+    """The configuration's horizon (1000 steps, checkpoints every 50, the
+    Weibel adjoint state with alpha = beta = 1e-4, k = 1.25) on 8192
+    particles: the GPU's loss and every cotangent of the initial state
+    against the oracle's reverse mode, which equals the dual-number forward
+    mode at this horizon (tests/test_oracle_tangent.py). The two forward
+    runs differ only in the deposits' summation order, so the loss agrees to
+    1e-12 and each cotangent group to 1e-8 of its largest entry (the
+    summation-order differences grow through the unstable run and the
+    derivative is evaluated on the two slightly different trajectories)."""
+    n, steps, K = 8192, 1000, 50
+    print(f"adjoint over the full horizon vs the oracle: n={n} steps={steps} K={K}", flush=True)
+    w = workloads.weibel_adjoint_1d2v(n, seed=2)
+    nx, L, dt = w["nx"], w["xmax"], w["dt"]
+    e1 = np.zeros(nx)
+    orc.Splitting(nx, 0.0, L, 3, 3, 2, w["particles"].copy(order="F"), e1, np.zeros(nx),
+                  w["b0"].copy(), common_weight=w["common_weight"]).solve_poisson(e1, np.zeros(nx))
+    h, pg = _gpu(w["particles"], e1, np.zeros(nx), w["b0"], L, nx, (3, 2), w["common_weight"])
+    r = G.strang_splitting_adjoint_(h, dt, steps, checkpoint_every=K)
+    osp = orc.Splitting(nx, 0.0, L, 3, 3, 2, w["particles"].copy(order="F"), e1.copy(),
+                        np.zeros(nx), w["b0"].copy(), common_weight=w["common_weight"])
+    loss, gp, ge1, ge2, gb = osp.strang_adjoint(dt, steps, dt)
+    assert abs(r.loss - loss) <= 1e-12 * abs(loss), (r.loss, loss)
+    for row in range(3):
+        _close(r.g_particles[row], gp[row], 1e-8)
+    _close(r.g_e1, ge1, 1e-8)
+    _close(r.g_e2, ge2, 1e-8)
+    _close(r.g_b, gb, 1e-8)

@@ -343,3 +343,69 @@ def test_gpu_weibel_4m_properties():
     h.to_host(particles=False)
     d = h.e_dofs[0] - ep
     assert np.max(np.abs(d - d.mean())) < 1e-5 * np.max(np.abs(ep))
+
+
+@pytest.mark.parametrize("nx,degree", [(16, 2), (32, 3), (10, 2)])
+def test_gpu_rhs_from_function_and_l2projection(nx, degree):
+    """compute_rhs_from_function! and l2projection! (maxwell_1d_fem.jl:188-220,
+    347-374) through the library: the function is called back on the host at
+    the closed-form Gauss-Legendre points; bitwise equal to the oracle for the
+    same function (written as the oracle evaluates it), and the reference's
+    rhs / dx result of l2projection!."""
+    L = 2 * math.pi / 1.25
+    mesh = G.OneDGrid(0.0, L, nx)
+    m = G.Maxwell1DFEM(mesh, 3)
+    om = orc.Maxwell1D(0.0, L, nx, 3)
+    a_cos, a_sin, k, c = 0.3, -0.2, 1.25, 0.1
+
+    def f(x):
+        v = 0.0
+        v += a_cos * math.cos(k * x)
+        v += a_sin * math.sin(k * x)
+        return v + c
+    rhs = np.zeros(nx)
+    G.compute_rhs_from_function_(rhs, m, f, degree)
+    ref = np.zeros(nx)
+    om.compute_rhs_from_function(ref, degree, a_cos=a_cos, a_sin=a_sin, k=k, c=c)
+    np.testing.assert_array_equal(rhs, ref)
+    proj = np.zeros(nx)
+    G.l2projection_(proj, m, f, degree)
+    ref_p = np.zeros(nx)
+    om.l2projection(ref_p, degree, a_cos=a_cos, a_sin=a_sin, k=k, c=c)
+    np.testing.assert_array_equal(proj, ref_p)
+    np.testing.assert_array_equal(proj, rhs / m.delta_x)
+    # the benchmark's initial B (host set-up, the same quadrature)
+    b0 = workloads.l2projection_cos(nx, L, degree, 1e-4, 1.25)
+    G.l2projection_(proj, m, lambda x: 1e-4 * math.cos(1.25 * x), degree)
+    np.testing.assert_allclose(b0, proj, rtol=0, atol=1e-19)
+    with pytest.raises(ValueError):
+        G.l2projection_(proj, m, f, 1)
+
+
+def test_gpu_save_and_resume(tmp_path):
+    """save(file, step, pg) (particle_group.jl:152-165) after 5 steps, then a
+    new splitting resumed from the file (and the fields) runs 4 more steps:
+    the state of 9 uninterrupted steps, up to the deposits' summation order
+    (the resumed splitting sorts its particles by their positions after 5
+    steps, the uninterrupted one kept the order of its upload)."""
+    w = _hot_weibel(8192, seed=21)
+    a, pga, *_ = _setup_pair(w)
+    G.strang_splitting_(a, 0.1, 9)
+    b, pgb, *_ = _setup_pair(w)
+    G.strang_splitting_(b, 0.1, 5)
+    path = G.save(str(tmp_path / "weibel"), 5, pgb)
+    assert path.endswith("weibel-000005.npz")
+    fields = [b.e_dofs[0].copy(), b.e_dofs[1].copy(), b.b_dofs.copy()]
+    nx = w["nx"]
+    mesh = G.OneDGrid(w["xmin"], w["xmax"], nx)
+    pgc = G.ParticleGroup(1, 2, 8192, common_weight=w["common_weight"])
+    G.load(path, pgc)
+    np.testing.assert_array_equal(pgc.array, pgb.array)
+    c = G.HamiltonianSplitting(1, 2, G.Maxwell1DFEM(mesh, 3),
+                               G.ParticleMeshCoupling1D(mesh, 8192, 3, "galerkin"),
+                               G.ParticleMeshCoupling1D(mesh, 8192, 2, "galerkin"), pgc,
+                               fields[:2], fields[2])
+    G.strang_splitting_(c, 0.1, 4)
+    assert_rows_close(pgc.array, pga.array, 1e-11)
+    for got, ref in ((c.e_dofs[0], a.e_dofs[0]), (c.e_dofs[1], a.e_dofs[1]), (c.b_dofs, a.b_dofs)):
+        assert np.max(np.abs(got - ref)) <= 1e-11 * max(np.max(np.abs(ref)), 1.0)

@@ -14,7 +14,7 @@ oracle on the same seeded input:
 Tolerances. Per-particle arithmetic is the reference's, rounded the same
 way, so the only difference between the GPU and the oracle is the order in
 which deposits (rho, j) are summed. Where a quantity is compared at 1e-12
-that is the north-star bound, normwise: |a - b|_inf <= 1e-12 |b|_inf.
+that is the per-sub-step parity bound, normwise: |a - b|_inf <= 1e-12 |b|_inf.
 Over many steps the summation-order differences feed back into the
 particles and fields, and some diagnostics cancel (rho is a constant plus a
 1e-4 perturbation, so the Poisson field carries 1e4 times the relative
@@ -66,9 +66,11 @@ def _oracle(w, perm=None, degree0=3, degree1=2):
     return osp, parts, (e1, e2, b)
 
 
-def _trajectory_within_spread(got, ref, alt, what):
+def _trajectory_within_spread(got, ref, alt, what, floor_abs=0.0):
     """got, ref, alt: per-step values (a sequence of scalars or arrays);
-    the envelope form of _within_spread (module docstring)."""
+    the envelope form of _within_spread (module docstring), plus an absolute
+    floor for quantities that are themselves differences of nearly equal
+    fields."""
     env, run = 0.0, 0.0
     for s, (g, r, a) in enumerate(zip(got, ref, alt)):
         g, r, a = (np.atleast_1d(np.asarray(v, dtype=float)) for v in (g, r, a))
@@ -76,7 +78,7 @@ def _trajectory_within_spread(got, ref, alt, what):
         run = max(run, mag)
         scale = max(mag, 1e-6 * run, 1e-300)
         env = max(env, float(np.max(np.abs(a - r))) / scale)
-        tol = (SPREAD_FACTOR * env + SPREAD_FLOOR) * scale
+        tol = (SPREAD_FACTOR * env + SPREAD_FLOOR) * scale + floor_abs
         err = float(np.max(np.abs(g - r)))
         assert err <= tol, (f"{what} at step {s}: {err:.3e} > {tol:.3e} "
                             f"(permuted-oracle relative spread {env:.3e})")
@@ -146,23 +148,36 @@ def test_gpu_weibel_4m_write_step_every_step():
     th = G.TimeHistoryDiagnostics(h)
     dt = w["dt"]
     got, ref, alt = [], [], []
+    field_spread = 0.0
     for s in range(steps + 1):
         t = s * dt
         ep = np.zeros(nx)
         G.solve_poisson_(ep, h, np.zeros(nx))
         got.append(G.write_step_(th, t, 3, efield_poisson=ep))
+        pois = []
         for sp, (a1, a2, ab), out in ((osp, (o_e1, o_e2, o_b), ref), (psp, (p_e1, p_e2, p_b), alt)):
             epo = np.zeros(nx)
             sp.solve_poisson(epo, np.zeros(nx))
+            pois.append(epo)
             out.append(sp.write_step(t, 3, a1, a2, ab, a1, a2, epo))
+        # the summation order's spread of the two fields ErrorPoisson subtracts
+        field_spread = max(field_spread, float(np.max(np.abs(p_e1 - o_e1))),
+                           float(np.max(np.abs(pois[1] - pois[0]))))
         if s < steps:
             G.strang_splitting_(h, dt, 1)
             osp.strang_splitting(dt, 1)
             psp.strang_splitting(dt, 1)
     got, ref, alt = np.array(got), np.array(ref), np.array(alt)
     np.testing.assert_array_equal(got[:, 0], ref[:, 0])
-    for k in range(1, 11):
+    for k in range(1, 10):
         _trajectory_within_spread(got[:, k], ref[:, k], alt[:, k], WRITE_STEP_COLUMNS[k])
+    # ErrorPoisson = max |E1 - E_Poisson| is a difference of two fields, each
+    # carrying its deposit's summation-order rounding (the Poisson field of a
+    # charge that is a constant plus a 1e-4 perturbation: 1e4 times the
+    # rounding of rho): its floor is 16 times the permuted oracle's spread of
+    # those two fields
+    _trajectory_within_spread(got[:, 10], ref[:, 10], alt[:, 10], "ErrorPoisson",
+                              floor_abs=SPREAD_FACTOR * 2.0 * field_spread)
     # the energy of the Strang scheme: the same drift on both sides
     energy = lambda d: 0.5 * (d[:, 1] + d[:, 4] + d[:, 5] + d[:, 6])
     _trajectory_within_spread(energy(got), energy(ref), energy(alt), "total energy")
@@ -292,7 +307,7 @@ def test_gpu_pmc1d_verbs_with_b_field(degree, smoothing):
 # ---------------------------------------------------------------------------
 # Fused arithmetic (HamiltonianSplitting.set_arithmetic("fused")): the same
 # sub-steps with closed-form bases and fma, compared with the oracle at the
-# north-star bound. Per-particle results differ from the reference's by a few
+# 1e-12 per-sub-step bound. Per-particle results differ from the reference's by a few
 # ulp per operation instead of being bitwise equal.
 
 @pytest.mark.parametrize("pair", [(3, 2), (3, 3), (2, 1), (2, 2), (1, 0), (1, 1)])

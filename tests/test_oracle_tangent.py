@@ -72,3 +72,29 @@ def test_tangent_equals_central_differences_of_the_forward_run():
     fd = (loss(h) - loss(-h)) / (2 * h)
     assert abs(loss(0.0) - J) <= 1e-13 * abs(J)
     assert abs(fd - dJ) <= 1e-6 * abs(dJ), (fd, dJ)
+
+
+def test_tangent_equals_the_reverse_mode_over_the_adjoint_horizon():
+    """The Weibel adjoint state (alpha = beta = 1e-4, k = 1.25, E1 from
+    Poisson) over 1000 steps, on 2048 particles, in a random direction of
+    the velocities: forward and reverse mode agree to 1e-10 relative at the
+    horizon where central differences no longer resolve the derivative (the
+    map is C^1 only; see tests/test_gpu_adjoint.py)."""
+    from gempic_amd import workloads
+    n, steps = 2048, 1000
+    w = workloads.weibel_adjoint_1d2v(n, seed=2)
+    nx, L, dt = w["nx"], w["xmax"], w["dt"]
+    parts = np.asfortranarray(w["particles"].copy())
+    e1 = np.zeros(nx)
+    orc.Splitting(nx, 0.0, L, 3, 3, 2, parts.copy(order="F"), e1, np.zeros(nx), w["b0"].copy(),
+                  common_weight=w["common_weight"]).solve_poisson(e1, np.zeros(nx))
+    d = np.random.default_rng(1).normal(size=(2, n)) * 0.02
+    osp = orc.Splitting(nx, 0.0, L, 3, 3, 2, parts.copy(order="F"), e1.copy(), np.zeros(nx),
+                        w["b0"].copy(), common_weight=w["common_weight"])
+    J, gp, *_ = osp.strang_adjoint(dt, steps, dt)
+    rev = float(np.sum(gp[1:3] * d))
+    T = Tangent1D2V(nx, 0.0, L, 3, 2, common_weight=w["common_weight"])
+    Jt, dJ, _ = T.strang_jvp(parts, e1, np.zeros(nx), w["b0"], [np.zeros(n), d[0], d[1]],
+                             np.zeros(nx), np.zeros(nx), np.zeros(nx), dt, steps, dt)
+    assert abs(Jt - J) <= 1e-12 * abs(J)
+    assert abs(dJ - rev) <= 1e-10 * abs(rev), (dJ, rev)
