@@ -459,6 +459,8 @@ def ensemble_run(G, _lib, n_inst, steps, seq_steps, adj_steps=500, adj_every=25,
         total += n
     ens = G.Ensemble(hs)
     G.ensemble_strang_splitting_(ens, dts, 2)                # kernel loads
+    # particles per workgroup of every instance's pass, tuned on this state
+    tune = G.ensemble_autotune(ens, dts, 10, particles_per_workgroup=(2048, 4096, 8192, 16384))
     t0 = time.perf_counter()
     G.ensemble_strang_splitting_(ens, dts, steps)
     t_ens = time.perf_counter() - t0
@@ -495,7 +497,7 @@ def ensemble_run(G, _lib, n_inst, steps, seq_steps, adj_steps=500, adj_every=25,
             "particle_steps_per_s": total * steps / t_ens,
             "graph_particle_steps_per_s": total * steps / t_graph,
             "one_by_one_particle_steps_per_s": total * seq_steps / t_seq,
-            "one_by_one_steps": seq_steps, "adjoint": adj}
+            "one_by_one_steps": seq_steps, "autotune": tune, "adjoint": adj}
 
 
 def copy_calibration(mib):
@@ -640,10 +642,11 @@ def main():
     # (sort time / interval), whether or not one falls in the window.
     sorts_in_window = (args.steps - 1) // interval if interval > 0 else 0
     sort_share_s = (args.steps / interval - sorts_in_window) * sort_s if interval > 0 else 0.0
-    # HIP events around every 10th launch of each kernel (every launch for
-    # short runs, so that at least 20 launches are timed): the kernel times
-    # come from the timed run itself at a small cost to its stream
-    _lib.check(L.gempic_profile_enable(h._h, max(1, min(10, args.steps // 20))))
+    # HIP events around every 10th launch of each kernel (for runs of fewer
+    # than 50 steps, every (steps / 5)-th, so that at least 5 launches are
+    # timed): the kernel times come from the timed run itself, at the cost of
+    # a few microseconds on the sampled steps only
+    _lib.check(L.gempic_profile_enable(h._h, max(1, min(10, args.steps // 5))))
     _lib.check(L.gempic_profile_reset(h._h))
     print(f"[bench] timing {args.steps} steps", file=sys.stderr, flush=True)
     t0 = time.perf_counter()

@@ -535,6 +535,7 @@ int gempic_splitting_create(int64_t nx, double xmin, double xmax, int maxwell_de
         put(T_AMP, h->tabs.eig_weak_ampere); put(T_POIS, h->tabs.eig_weak_poisson);
         put(T_M0, h->tabs.eig_mass0); put(T_M1, h->tabs.eig_mass1);
         put(T_CM0, h->tabs.circ_m0); put(T_CM1, h->tabs.circ_m1); put(T_CAMP, h->tabs.circ_amp);
+        put(T_CMASS1, h->tabs.circ_mass1); put(T_CPOIS, h->tabs.circ_pois);
         ok(hipMemcpy(h->d_tabs, T.data(), sizeof(double) * T.size(), hipMemcpyHostToDevice));
     }
     if (e != hipSuccess) {
@@ -1610,6 +1611,123 @@ int gempic_ensemble_strang_splitting(gempic_ensemble *e, const double *dt, int64
 }
 
 
+// Launch autotuning of an ensemble: for each candidate number of particles
+// per workgroup, every instance's pass takes ceil(N / ppw) workgroups (at most
+// its reduction scratch); n_steps batched steps are timed twice from the same
+// state (restored after every trial, on the device) and the fastest
+// candidate stays set on the member handles.
+int gempic_ensemble_autotune(gempic_ensemble *e, const double *dt, int64_t n_steps, const int *ppw,
+                             int n_candidates, int *best_ppw, double *ms_per_step)
+{
+    if (!e) return fail(GEMPIC_ERR_ARGUMENT, "null ensemble");
+    if (!dt || !ppw) return fail(GEMPIC_ERR_ARGUMENT, "dt and the candidates must not be NULL");
+    if (n_steps < 1 || n_steps > 100000)
+        return fail(GEMPIC_ERR_ARGUMENT, "autotune: n_steps must be in [1, 100000]");
+    if (n_candidates < 1 || n_candidates > 64)
+        return fail(GEMPIC_ERR_ARGUMENT, "autotune: 1 to 64 candidates");
+    for (int c = 0; c < n_candidates; ++c)
+        if (ppw[c] < kBlock) return fail(GEMPIC_ERR_ARGUMENT, "autotune: a candidate is below 256 particles per workgroup");
+    if (hipSetDevice(e->device) != hipSuccess) return fail(GEMPIC_ERR_DEVICE, "hipSetDevice failed");
+    for (auto *h : e->hs)
+        if (!h->particles_set) return fail(GEMPIC_ERR_STATE, "particles not set");
+    const size_t ni = e->hs.size();
+    // snapshot of every instance: particles in device order, ids, field slots, steps since sort
+    std::vector<size_t> off(ni + 1, 0);
+    for (size_t i = 0; i < ni; ++i) {
+        const gempic_splitting *h = e->hs[i];
+        off[i + 1] = off[i] + 5 * (size_t)h->n + (size_t)NSLOT * (size_t)h->nx;
+    }
+    double *snap = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    for (auto *h : e->hs) HIP_TRY(hipStreamSynchronize(h->stream));
+    hipError_t err = dalloc(&snap, std::max<size_t>(off[ni], 1));
+    if (err == hipSuccess) err = hipEventCreate(&ev[0]);
+    if (err == hipSuccess) err = hipEventCreate(&ev[1]);
+    std::vector<int64_t> since(ni);
+    std::vector<int> saved_blocks(ni);
+    for (size_t i = 0; i < ni; ++i) {
+        since[i] = e->hs[i]->steps_since_sort;
+        saved_blocks[i] = e->hs[i]->override_blocks;
+    }
+    auto copy_state = [&](bool save) -> hipError_t {
+        hipError_t c = hipSuccess;
+        for (size_t i = 0; i < ni && c == hipSuccess; ++i) {
+            gempic_splitting *h = e->hs[i];
+            const size_t n = (size_t)h->n;
+            double *base = snap + off[i];
+            double *dev[4] = {h->d_x, h->d_v1, h->d_v2, h->d_w};
+            for (int k = 0; k < 4 && c == hipSuccess; ++k)
+                c = hipMemcpyAsync(save ? base + k * n : dev[k], save ? dev[k] : base + k * n,
+                                   sizeof(double) * n, hipMemcpyDeviceToDevice, e->stream);
+            if (c == hipSuccess)
+                c = hipMemcpyAsync(save ? (void *)(base + 4 * n) : (void *)h->d_id,
+                                   save ? (const void *)h->d_id : (const void *)(base + 4 * n),
+                                   sizeof(int64_t) * n, hipMemcpyDeviceToDevice, e->stream);
+            if (c == hipSuccess)
+                c = hipMemcpyAsync(save ? base + 5 * n : h->d_grid, save ? h->d_grid : base + 5 * n,
+                                   sizeof(double) * (size_t)NSLOT * (size_t)h->nx,
+                                   hipMemcpyDeviceToDevice, e->stream);
+            h->steps_since_sort = since[i];
+        }
+        if (c == hipSuccess) c = hipStreamSynchronize(e->stream);
+        return c;
+    };
+    bool saved = false;
+    if (err == hipSuccess) {
+        err = copy_state(true);
+        saved = err == hipSuccess;
+    }
+    int rc = err == hipSuccess ? GEMPIC_OK
+                               : fail(GEMPIC_ERR_DEVICE, std::string("ensemble autotune set-up: ") +
+                                                             hipGetErrorString(err));
+    double best = 0;
+    int bi = -1;
+    auto set_ppw = [&](int p) {
+        for (auto *h : e->hs) {
+            const int64_t b = (h->n + p - 1) / p;
+            h->override_blocks = (int)std::max<int64_t>(1, std::min<int64_t>(b, h->max_blocks));
+        }
+    };
+    for (int c = 0; c < n_candidates && !rc; ++c) {
+        set_ppw(ppw[c]);
+        float t = 0;
+        for (int r = 0; r < 2 && !rc; ++r) {
+            float ms = 0;
+            err = hipEventRecord(ev[0], e->stream);
+            if (err == hipSuccess) {
+                rc = gempic_ensemble_strang_splitting(e, dt, n_steps, 0);
+                if (!rc) err = hipEventRecord(ev[1], e->stream);
+                if (!rc && err == hipSuccess) err = hipEventSynchronize(ev[1]);
+                if (!rc && err == hipSuccess) err = hipEventElapsedTime(&ms, ev[0], ev[1]);
+                if (!rc && err == hipSuccess) err = copy_state(false);
+            }
+            if (!rc && err != hipSuccess)
+                rc = fail(GEMPIC_ERR_DEVICE, std::string("ensemble autotune: ") + hipGetErrorString(err));
+            if (r == 0 || ms < t) t = ms;
+        }
+        if (rc) break;
+        const double per = (double)t / (double)n_steps;
+        if (ms_per_step) ms_per_step[c] = per;
+        if (bi < 0 || per < best) {
+            best = per;
+            bi = c;
+        }
+    }
+    if (rc) {
+        const std::string msg = gempic_last_error();
+        if (saved) (void)copy_state(false);
+        for (size_t i = 0; i < ni; ++i) e->hs[i]->override_blocks = saved_blocks[i];
+        g_last_error = msg;
+    } else {
+        set_ppw(ppw[bi]);
+        if (best_ppw) *best_ppw = ppw[bi];
+    }
+    if (ev[0]) (void)hipEventDestroy(ev[0]);
+    if (ev[1]) (void)hipEventDestroy(ev[1]);
+    if (snap) (void)hipFree(snap);
+    return rc;
+}
+
 // Reverse mode of an ensemble: gempic_strang_splitting_adjoint for every
 // instance, with the recorded forward, the re-runs, the reverse passes and
 // their field parts batched over the instances (one launch each per step;
@@ -2321,6 +2439,7 @@ int gempic_maxwell1d_create(int64_t nx, double xmin, double xmax, int degree, ge
         put(T_AMP, m->tabs.eig_weak_ampere); put(T_POIS, m->tabs.eig_weak_poisson);
         put(T_M0, m->tabs.eig_mass0); put(T_M1, m->tabs.eig_mass1);
         put(T_CM0, m->tabs.circ_m0); put(T_CM1, m->tabs.circ_m1); put(T_CAMP, m->tabs.circ_amp);
+        put(T_CMASS1, m->tabs.circ_mass1); put(T_CPOIS, m->tabs.circ_pois);
         e = hipMemcpy(m->d_tabs, T.data(), sizeof(double) * T.size(), hipMemcpyHostToDevice);
     }
     if (e != hipSuccess) {

@@ -98,8 +98,9 @@ struct Maxwell1DTables {
     std::vector<double> inv_mass0, inv_mass1;   // 1/eig in the real slots, 0 elsewhere
     std::vector<double> cos_tab, sin_tab;       // cos/sin(2 pi t / n), t = 0..n-1
     // first column c of the circulant matrix C with C rhs = solve_circulant(eig, rhs)
-    // (c = HC2R(eig) / n, in long double) for eig = inv_mass0, inv_mass1, eig_weak_ampere
-    std::vector<double> circ_m0, circ_m1, circ_amp;
+    // (c = HC2R(eig) / n, in long double) for eig = inv_mass0, inv_mass1,
+    // eig_weak_ampere, eig_mass1 and eig_weak_poisson
+    std::vector<double> circ_m0, circ_m1, circ_amp, circ_mass1, circ_pois;
 };
 
 // c[m] = (1/n) (e[0] + (-1)^m e[n/2] + 2 sum_{k=1}^{n/2-1} (e[k] cos(2 pi k m/n) - e[n-k] sin(2 pi k m/n)))
@@ -192,6 +193,8 @@ inline bool maxwell1d_tables(int64_t n, int degree, Maxwell1DTables &t)
     t.circ_m0 = circulant_column(t.inv_mass0, n);
     t.circ_m1 = circulant_column(t.inv_mass1, n);
     t.circ_amp = circulant_column(t.eig_weak_ampere, n);
+    t.circ_mass1 = circulant_column(t.eig_mass1, n);
+    t.circ_pois = circulant_column(t.eig_weak_poisson, n);
     return true;
 }
 

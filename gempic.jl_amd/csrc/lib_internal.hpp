@@ -29,7 +29,7 @@ enum GridSlot { S_E1 = 0, S_E2, S_B, S_E2P, S_J1, S_J2, S_RHO, S_EPOIS, S_E1N, S
 // M1 inverse-mass solves and of the weak Ampere solve (the fused-arithmetic
 // field part applies them as matrix-vector products)
 enum TabSlot { T_COS = 0, T_SIN, T_INVM0, T_INVM1, T_AMP, T_POIS, T_M0, T_M1, T_CM0, T_CM1, T_CAMP,
-               NTAB };
+               T_CMASS1, T_CPOIS, NTAB };
 // deposit slots of one pass
 enum DepSlot { D_J2A = 0, D_J1, D_RHO, D_J2B, NDEP };
 

@@ -851,6 +851,29 @@ __device__ __forceinline__ double circ_apply(const double *c, const double *rhs,
     return (s0 + s1) + (s2 + s3);
 }
 
+// (C^T rhs)[i] for the circulant C with first column c: sum_j c[(j - i) mod n] rhs[j]
+__device__ __forceinline__ double circ_apply_t(const double *c, const double *rhs, int i, int n)
+{
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int t = i == 0 ? 0 : n - i;                   // (j - i) mod n for j = 0
+    int j = 0;
+    for (; j + 4 <= n; j += 4) {
+        const int t1 = t + 1 == n ? 0 : t + 1;
+        const int t2 = t1 + 1 == n ? 0 : t1 + 1;
+        const int t3 = t2 + 1 == n ? 0 : t2 + 1;
+        s0 = __builtin_fma(c[t], rhs[j], s0);
+        s1 = __builtin_fma(c[t1], rhs[j + 1], s1);
+        s2 = __builtin_fma(c[t2], rhs[j + 2], s2);
+        s3 = __builtin_fma(c[t3], rhs[j + 3], s3);
+        t = t3 + 1 == n ? 0 : t3 + 1;
+    }
+    for (; j < n; ++j) {
+        s0 = __builtin_fma(c[t], rhs[j], s0);
+        t = t + 1 == n ? 0 : t + 1;
+    }
+    return (s0 + s1) + (s2 + s3);
+}
+
 // workgroups of k_epi1d for nblocks partial rows
 int epi_groups(int nblocks)
 {
@@ -2937,8 +2960,7 @@ __device__ __forceinline__ void adj_epi_body(const AdjArgs &A, int nblocks, int 
     double *red = lB + nx;                  // [NDEP][nx]
     double *w3 = red + NDEP * nx;           // [6][nx]: three outputs, three workspaces
     double *T = w3 + 6 * nx;                // [NTAB][nx]
-    double *ampt = T + NTAB * nx;           // conjugate weak-Ampere symbol
-    double *bend = ampt + nx;
+    double *bend = T + NTAB * nx;
     double *scratch = bend + nx;            // [blockDim.x]
     double *G = A.agrid;
 
@@ -2951,7 +2973,6 @@ __device__ __forceinline__ void adj_epi_body(const AdjArgs &A, int nblocks, int 
     for (int i = tid; i < NTAB * nx; i += nt) T[i] = a.tabs[i];
     for (int i = tid; i < len; i += nt) red[i] = 0.0;
     __syncthreads();
-    for (int k = tid; k < nx; k += nt) ampt[k] = k > nx / 2 ? -T[T_AMP * nx + k] : T[T_AMP * nx + k];
 
     constexpr unsigned MASK = KIND == AE_TAIL ? (1u << AR_E1) | (1u << AR_E2) | (1u << AR_SCAL)
                               : KIND == AE_POIS ? 0u
@@ -3006,7 +3027,9 @@ __device__ __forceinline__ void adj_epi_body(const AdjArgs &A, int nblocks, int 
         G[A_LOSS * nx + AS_MOD] = G[A_LOSS * nx + AS_MOD] + red[AR_SCAL * nx + 1];
         G[A_LOSS * nx + AS_W] = G[A_LOSS * nx + AS_W] + red[AR_SCAL * nx + 2];
     }
-    const double *cs = T + T_COS * nx, *sn = T + T_SIN * nx;
+    // the field part's circulant operators are matrix-vector products with
+    // their first columns (circ_apply, circ_apply_t): the reverse mode agrees
+    // with the transform form to rounding, and has no transform phases
     double *wA = w3, *wM = w3 + nx, *w1 = w3 + 2 * nx;
 
     if constexpr (KIND == AE_POIS) {
@@ -3018,8 +3041,7 @@ __device__ __forceinline__ void adj_epi_body(const AdjArgs &A, int nblocks, int 
         }
         __syncthreads();
         adj_b_from_e(tmp, 1.0, bend, nx);
-        circ_solve(T + T_POIS * nx, tmp, wA, w3 + 4 * nx, cs, sn, nx);
-        for (int i = tid; i < nx; i += nt) G[A_NU * nx + i] = wA[i];
+        for (int i = tid; i < nx; i += nt) G[A_NU * nx + i] = circ_apply(T + T_CPOIS * nx, tmp, i, nx);
         return;
     }
     if constexpr (KIND == AE_STEP || KIND == AE_HEAD) {
@@ -3037,7 +3059,8 @@ __device__ __forceinline__ void adj_epi_body(const AdjArgs &A, int nblocks, int 
         adj_b_from_e(lE2, coef, lB, nx);                       // U of the next step's HE
     }
     if constexpr (KIND == AE_HEAD) {
-        circ_solve(ampt, lE2, wA, w3 + 3 * nx, cs, sn, nx);     // H of the first step
+        for (int i = tid; i < nx; i += nt) wA[i] = circ_apply_t(T + T_CAMP * nx, lE2, i, nx);
+        __syncthreads();                                        // H of the first step
         for (int i = tid; i < nx; i += nt) {
             lB[i] = lB[i] + coef * wA[i];
             G[A_E1 * nx + i] = lE1[i];
@@ -3058,10 +3081,15 @@ __device__ __forceinline__ void adj_epi_body(const AdjArgs &A, int nblocks, int 
         // lambda_E1 is final once the rows of this step's last HE are added
         for (int i = tid; i < nx; i += nt) lE1[i] = lE1[i] + dE1[i];
         __syncthreads();
-        const SolveArgs sa{ampt, lE2, 1.0, wA, w3 + 3 * nx};
-        const SolveArgs sm{T + T_M1 * nx, bend, 1.0, wM, w3 + 4 * nx};
-        const SolveArgs s1{T + T_INVM1 * nx, lE1, 1.0, w1, w3 + 5 * nx};
-        circ_solve3(sa, sm, s1, cs, sn, nx);
+        // W^T lambda_E2, M1 B_end and M1^-1 lambda_E1, each a matrix-vector
+        // product with the circulant's first column
+        for (int t = tid; t < 3 * nx; t += nt) {
+            const int k = t / nx, i = t - k * nx;
+            if (k == 0) wA[i] = circ_apply_t(T + T_CAMP * nx, lE2, i, nx);
+            else if (k == 1) wM[i] = circ_apply(T + T_CMASS1 * nx, bend, i, nx);
+            else w1[i] = circ_apply(T + T_CM1 * nx, lE1, i, nx);
+        }
+        __syncthreads();
         const double wl = A.wloss;
         for (int i = tid; i < nx; i += nt) {
             double b = lB[i];
@@ -3085,9 +3113,8 @@ __device__ __forceinline__ void adj_epi_body(const AdjArgs &A, int nblocks, int 
         adj_b_from_e(lE2, coef, lB, nx);                       // U of this step's HE
         for (int i = tid; i < nx; i += nt) lE2[i] = lE2[i] + dE2[i];
         __syncthreads();
-        circ_solve(T + T_INVM0 * nx, lE2, wA, w3 + 3 * nx, cs, sn, nx);
         for (int i = tid; i < nx; i += nt) {
-            G[A_MU2 * nx + i] = -(c / dx) * wA[i];
+            G[A_MU2 * nx + i] = -(c / dx) * circ_apply(T + T_CM0 * nx, lE2, i, nx);
             G[A_E1 * nx + i] = lE1[i];
             G[A_E2 * nx + i] = lE2[i];
             G[A_B * nx + i] = lB[i];
@@ -3123,7 +3150,7 @@ __global__ void __launch_bounds__(kBlock) k_adj_epi_batch_fin(AdjBatch b)
 
 size_t adj_epi_shmem_bytes(int nx)
 {
-    return sizeof(double) * ((size_t)(3 + NDEP + 6 + NTAB + 2) * nx + kBlock);
+    return sizeof(double) * ((size_t)(3 + NDEP + 6 + NTAB + 1) * nx + kBlock);
 }
 
 template <int P0, int P1>

@@ -14,10 +14,13 @@
 // (so a particle's stencil is constant offsets from its cell, no wrap), and
 // deposits its current into an LDS grid: each lane keeps the node sums of
 // the patch its particles fall in; when a lane's next contribution starts
-// another patch, the wave flushes (DPP sums into one patch when every
-// flushing lane holds the same one, LDS atomics otherwise). The workgroup
-// grids are summed over workgroups in a fixed order. Per-particle arithmetic
-// follows the oracle's operations in its order; the deposit's additions
+// another patch, the wave flushes (DPP sums into one patch when many
+// flushing lanes hold the same one, LDS atomics otherwise). The workgroup
+// grids are summed over workgroups in a fixed order. HE and Hp3 follow the
+// oracle's operations in its order; Hp1 and Hp2 take the path integrals of
+// the degree p-1 splines from their antiderivative (suffix sums of the
+// degree-p values at the two ends) where the oracle integrates piece by
+// piece, the same integrals rounded differently. The deposit's additions
 // happen in another order (and, between lanes that flush different patches,
 // in a timing-dependent one).
 #include <hip/hip_runtime.h>
@@ -48,7 +51,7 @@ namespace {
 #define GEMPIC_S2_THREADS_HP12 1024
 #endif
 #ifndef GEMPIC_S2_THREADS_HP3
-#define GEMPIC_S2_THREADS_HP3 1024
+#define GEMPIC_S2_THREADS_HP3 512
 #endif
 constexpr int s2_threads(int op)
 {
@@ -66,8 +69,6 @@ struct S2Args {
     int nx, ny;
     double xmin, ymin, dx, dy, rdx, rdy, Lx, Ly;
     double charge, cw, qm, dt;
-    double qx[3], qw[3];            // Gauss-Legendre rule of the degree p-1 line integrals
-    int nq;
     double *x1, *x2, *v1, *v2, *v3;
     const double *w;
     const double *f[3];             // fields read by the pass (staged in LDS)
@@ -129,6 +130,13 @@ struct Patch {
     double acc[WX * WY];
 };
 
+// Row stride of the workgroup's LDS deposit grid, in doubles: congruent to 4
+// modulo 16, so that the 16 nodes of a 4 x 4 patch (column a, row c at
+// a + 4 c modulo 16) fall in 16 different pairs of the 32 four-byte banks.
+// With rows of nx = 64 doubles every node of a column shared one bank.
+__host__ __device__ __forceinline__ int grid_stride(int nx) { return nx + ((20 - nx % 16) % 16); }
+
+// LDS index of node j of the patch that starts at node `key` = sx + sy * nx
 template <int WX, int WY>
 __device__ __forceinline__ int patch_node(int key, int j, int nx, int ny)
 {
@@ -137,8 +145,20 @@ __device__ __forceinline__ int patch_node(int key, int j, int nx, int ny)
     int gx = sx + a, gy = sy + c;
     if (gx >= nx) gx -= nx;
     if (gy >= ny) gy -= ny;
-    return gx + gy * nx;
+    return gx + gy * grid_stride(nx);
 }
+
+// Flush of the lanes with `evict` set. A patch held by at least kDppLanes of
+// them (checked for the patches of the first kFlushKeys evicting lanes in
+// turn) is summed over those lanes with DPP and added by 16 lanes at once;
+// the lanes of the other patches add their own nodes with LDS atomics
+// (same-node atomics in one instruction serialize, so a patch shared by
+// many lanes is cheaper as one sum).
+#ifndef GEMPIC_S2_DPP_LANES
+#define GEMPIC_S2_DPP_LANES 64
+#endif
+constexpr int kDppLanes = GEMPIC_S2_DPP_LANES;
+constexpr int kFlushKeys = 4;
 
 template <int WX, int WY>
 __device__ __forceinline__ void patch_flush(Patch<WX, WY> &pc, bool evict, double *grid,
@@ -146,17 +166,24 @@ __device__ __forceinline__ void patch_flush(Patch<WX, WY> &pc, bool evict, doubl
 {
     constexpr int WW = WX * WY;
     const int lane = lane_id();
-    const unsigned long long m = __ballot(evict);
-    if (!m) return;
-    const int k0 = __shfl(pc.key, __ffsll((long long)m) - 1, kWave);
-    if (__ballot(evict && pc.key != k0) == 0) {
+    unsigned long long pending = __ballot(evict);
+    if (!pending) return;
+    const unsigned long long all = pending;
+    for (int r = 0; r < kFlushKeys && pending; ++r) {
+        const int k0 = __shfl(pc.key, __ffsll((long long)pending) - 1, kWave);
+        const unsigned long long mk = __ballot(evict && pc.key == k0);
+        pending &= ~mk;
+        if (__popcll(mk) < (kDppLanes < __popcll(all) ? kDppLanes : __popcll(all))) continue;
+        const bool mine = evict && pc.key == k0;
 #pragma unroll
         for (int j = 0; j < WW; ++j) {
-            const double sj = wave_sum_lane63(evict ? pc.acc[j] : 0.0);
+            const double sj = wave_sum_lane63(mine ? pc.acc[j] : 0.0);
             if (lane == kWave - 1) stage[j] = sj;
         }
         if (lane < WW) atomicAdd(&grid[patch_node<WX, WY>(k0, lane, nx, ny)], stage[lane]);
-    } else if (evict) {
+        if (mine) evict = false;
+    }
+    if (evict) {
 #pragma unroll
         for (int j = 0; j < WW; ++j) atomicAdd(&grid[patch_node<WX, WY>(pc.key, j, nx, ny)], pc.acc[j]);
     }
@@ -190,40 +217,6 @@ __device__ __forceinline__ void patch_add(Patch<WX, WY> &pc, bool act, int key, 
     }
 }
 
-// One piece of a line integral along a direction: the path [xa, xb] (cell
-// coordinates) cut at integer coordinates, pieces in path order; piece s has
-// cell c and vals[k] = sign * d * int_lo^up S^{P-1}_k(t) dt (Gauss-Legendre,
-// as update_jv!, particle_mesh_coupling_1d.jl:385-414). Returns false when
-// piece s is empty (past the end, or of zero length).
-template <int P>
-__device__ __forceinline__ bool line_piece(const S2Args &a, double xa, double xb, double d, int s,
-                                           int &cell, double (&vals)[P])
-{
-    const bool fwd = xb >= xa;
-    const double sign = fwd ? 1.0 : -1.0;
-    const double lo_x = fwd ? xa : xb, hi_x = fwd ? xb : xa;
-    const int c0 = (int)floor(lo_x), c1 = (int)floor(hi_x);
-    const int c = fwd ? c0 + s : c1 - s;
-    double lo = lo_x - (double)c, up = hi_x - (double)c;
-    if (lo < 0.0) lo = 0.0;
-    if (up > 1.0) up = 1.0;
-    cell = c;
-    if (s > c1 - c0 || !(up > lo)) return false;
-    const double cc1 = 0.5 * (up - lo), cc2 = 0.5 * (up + lo);
-#pragma unroll
-    for (int k = 0; k < P; ++k) vals[k] = 0.0;
-    for (int q = 0; q < a.nq; ++q) {
-        double sv[P];
-        bspline<P - 1>(cc1 * a.qx[q] + cc2, sv);
-#pragma unroll
-        for (int k = 0; k < P; ++k) vals[k] = vals[k] + sv[k] * a.qw[q] * cc1;
-    }
-    const double sd = sign * d;
-#pragma unroll
-    for (int k = 0; k < P; ++k) vals[k] = vals[k] * sd;
-    return true;
-}
-
 // the largest number of cells a particle may cross in one push
 constexpr int kMaxCross2 = 1 << 16;
 
@@ -233,8 +226,8 @@ __global__ void __launch_bounds__(s2_threads(OP)) k_split2d(S2Args a)
     constexpr bool DEP = OP != S2_HE;
     constexpr int NF = OP == S2_HE ? 3 : (OP == S2_RHO ? 0 : 2);
     // deposit patch shape: j1 (P-1, P), j2 (P, P-1), j3 and rho (P, P)
-    constexpr int WX = OP == S2_HP1 ? P : P + 1;
-    constexpr int WY = OP == S2_HP2 ? P : P + 1;
+    constexpr int WX = P + 1;
+    constexpr int WY = P + 1;
     extern __shared__ __align__(16) double smem[];
     const int nx = a.nx, ny = a.ny, nn = nx * ny;
     const int W = nx + P;
@@ -242,12 +235,13 @@ __global__ void __launch_bounds__(s2_threads(OP)) k_split2d(S2Args a)
     const int lane = lane_id(), wave = threadIdx.x / kWave;
     double *fs0 = smem, *fs1 = smem + fsz, *fs2 = smem + 2 * fsz;
     double *grid = smem + NF * fsz;
-    double *stage = grid + (DEP ? nn : 0) + wave * 16;
+    const int gsz = grid_stride(nx) * ny;
+    double *stage = grid + (DEP ? gsz : 0) + wave * 16;
     if (NF > 0) stage_field<P>(fs0, a.f[0], nx, ny);
     if (NF > 1) stage_field<P>(fs1, a.f[1], nx, ny);
     if (NF > 2) stage_field<P>(fs2, a.f[2], nx, ny);
     if (DEP)
-        for (int k = threadIdx.x; k < nn; k += blockDim.x) grid[k] = 0.0;
+        for (int k = threadIdx.x; k < gsz; k += blockDim.x) grid[k] = 0.0;
     __syncthreads();
 
     Patch<WX, WY> pc;
@@ -312,52 +306,87 @@ __global__ void __launch_bounds__(s2_threads(OP)) k_split2d(S2Args a)
             const bool far = !(fabs(xa) < 1.0e9) || !(fabs(xb) < 1.0e9) ||
                              fabs(floor(xb) - floor(xa)) > (double)kMaxCross2;
             if (valid && far) errs |= E_FAR;
-            const int npiece = far ? 0 : 1 + (int)fabs(floor(xb) - floor(xa));
+            const double fa = far ? 0.0 : floor(xa), fb = far ? 0.0 : floor(xb);
+            const int ca = (int)fa, cb = (int)fb;
+            const int clo = ca < cb ? ca : cb, chi = ca < cb ? cb : ca;
+            // The path's integrals of the degree P-1 splines from their
+            // antiderivative F_k = sum_{m >= k} N^P_m (d/dx N^P_m = N^{P-1}_m -
+            // N^{P-1}_{m+1}): int_xa^xb N^{P-1}_k = F_k(xb) - F_k(xa), where F_k
+            // at an end in cell c is 1 for k <= c - P, 0 for k > c and otherwise
+            // the suffix sum from k - (c - P) of the degree-P values there. The
+            // nodes clo - P + 1 .. chi (P of them within one cell, P + 1 across
+            // one boundary) go in chunks of P + 1, one patch each: one chunk for
+            // nearly every particle, and no pieces.
+            const int nch = far ? 0 : (chi - clo + P + P) / (P + 1);
+            double sa[P + 1], sb[P + 1];        // suffix sums sa[j] = sum_{i >= j} ia[i]
+            {
+                double ia[P + 1], ib[P + 1];
+                bspline<P>(xa - fa, ia);
+                bspline<P>(xb - fb, ib);
+                sa[P] = ia[P];
+                sb[P] = ib[P];
+#pragma unroll
+                for (int j = P - 1; j >= 1; --j) {
+                    sa[j] = sa[j + 1] + ia[j];
+                    sb[j] = sb[j + 1] + ib[j];
+                }
+                sa[0] = 1.0;
+                sb[0] = 1.0;
+            }
+            // F at node k for an end in cell c: index j = k - c + P
+            auto F = [](const double (&sfx)[P + 1], int j) {
+                double f = j <= 0 ? 1.0 : 0.0;
+#pragma unroll
+                for (int t = 1; t <= P; ++t) f = j == t ? sfx[t] : f;
+                return f;
+            };
             double u = ALONG_X ? v2 : v1;         // Hp1: v2 (with B3); Hp2: v1 (with B3)
             double t3 = v3;                        // both: v3 (with B2 or B1)
-            for (int s = 0;; ++s) {
-                const bool want = valid && s < npiece;
-                if (!__ballot(want)) break;
-                int c = 0;
-                double val[P];
-                const bool act = want && line_piece<P>(a, xa, xb, d, s, c, val);
+            const int nd = ALONG_X ? nx : ny;
+            for (int m = 0;; ++m) {
+                const bool act = valid && m < nch;
+                if (!__ballot(act)) break;
+                const int kb = clo - P + 1 + m * (P + 1);          // first node of the chunk
+                double val[P + 1];
+#pragma unroll
+                for (int r = 0; r <= P; ++r)
+                    val[r] = d * (F(sb, kb + r - cb + P) - F(sa, kb + r - ca + P));
+                int k0 = imod(kb, nd);             // in [-P, nd - 1 - P]: within the staging
+                if (k0 + P > nd - 1) k0 -= nd;
                 if constexpr (ALONG_X) {
-                    // j1 += q (dx int S^{p-1}) S^p(x2); v3 += qm B2 ..; v2 -= qm B3 ..
-                    // first x node of the piece's cell, in [-(P-1), nx - P]: within the staging
-                    const int fx = imod(c, nx) - (P - 1);
-                    patch_begin<WX, WY>(pc, act, imod(fx, nx) + imod(cy - P, ny) * nx, grid, stage,
+                    // j1 += q (dx int S^{p-1}) S^p(x2); v3 += qm int B2 ..; v2 -= qm int B3 ..
+                    patch_begin<WX, WY>(pc, act, imod(k0, nx) + imod(cy - P, ny) * nx, grid, stage,
                                         nx, ny);
                     if (act) {
-                        const double *b2 = fs1 + (cy - P + P) * W + (fx + P);
-                        const double *b3 = fs0 + (cy - (P - 1) + P) * W + (fx + P);
+                        const double *b2 = fs1 + (cy - P + P) * W + (k0 + P);
+                        const double *b3 = fs0 + (cy - (P - 1) + P) * W + (k0 + P);
 #pragma unroll
-                        for (int aa = 0; aa < P; ++aa) {
+                        for (int r = 0; r <= P; ++r) {
 #pragma unroll
                             for (int cc = 0; cc <= P; ++cc) {
-                                pc.acc[aa * WY + cc] += q * val[aa] * by0[cc];
-                                t3 = t3 + qm * b2[cc * W + aa] * val[aa] * by0[cc];
+                                pc.acc[r * WY + cc] += q * val[r] * by0[cc];
+                                t3 = t3 + qm * b2[cc * W + r] * val[r] * by0[cc];
                             }
 #pragma unroll
-                            for (int cc = 0; cc < P; ++cc) u = u - qm * b3[cc * W + aa] * val[aa] * by1[cc];
+                            for (int cc = 0; cc < P; ++cc) u = u - qm * b3[cc * W + r] * val[r] * by1[cc];
                         }
                     }
                 } else {
-                    // j2 += q S^p(x1) (dy int S^{p-1}); v3 -= qm B1 ..; v1 += qm B3 ..
-                    const int fy = imod(c, ny) - (P - 1);
-                    patch_begin<WX, WY>(pc, act, imod(cx - P, nx) + imod(fy, ny) * nx, grid, stage,
+                    // j2 += q S^p(x1) (dy int S^{p-1}); v3 -= qm int B1 ..; v1 += qm int B3 ..
+                    patch_begin<WX, WY>(pc, act, imod(cx - P, nx) + imod(k0, ny) * nx, grid, stage,
                                         nx, ny);
                     if (act) {
-                        const double *b1 = fs1 + (fy + P) * W + (cx - P + P);
-                        const double *b3 = fs0 + (fy + P) * W + (cx - (P - 1) + P);
+                        const double *b1 = fs1 + (k0 + P) * W + (cx - P + P);
+                        const double *b3 = fs0 + (k0 + P) * W + (cx - (P - 1) + P);
 #pragma unroll
-                        for (int cc = 0; cc < P; ++cc) {
+                        for (int r = 0; r <= P; ++r) {
 #pragma unroll
                             for (int aa = 0; aa <= P; ++aa) {
-                                pc.acc[aa * WY + cc] += q * bx0[aa] * val[cc];
-                                t3 = t3 - qm * b1[cc * W + aa] * bx0[aa] * val[cc];
+                                pc.acc[aa * WY + r] += q * bx0[aa] * val[r];
+                                t3 = t3 - qm * b1[r * W + aa] * bx0[aa] * val[r];
                             }
 #pragma unroll
-                            for (int aa = 0; aa < P; ++aa) u = u + qm * b3[cc * W + aa] * bx1[aa] * val[cc];
+                            for (int aa = 0; aa < P; ++aa) u = u + qm * b3[r * W + aa] * bx1[aa] * val[r];
                         }
                     }
                 }
@@ -384,7 +413,11 @@ __global__ void __launch_bounds__(s2_threads(OP)) k_split2d(S2Args a)
         patch_flush<WX, WY>(pc, pc.key >= 0, grid, stage, nx, ny);
         __syncthreads();
         double *out = a.part + (size_t)blockIdx.x * nn;
-        for (int k = threadIdx.x; k < nn; k += blockDim.x) out[k] = grid[k];
+        const int gs = grid_stride(nx);
+        for (int k = threadIdx.x; k < nn; k += blockDim.x) {
+            const int gy = k / nx;
+            out[k] = grid[k + gy * (gs - nx)];
+        }
     }
 }
 
@@ -419,15 +452,44 @@ __device__ __forceinline__ int cell_key(double x1, double x2, const S2Args &a)
     return imod((int)ceil(xp) - 1, a.nx) + imod((int)ceil(yp) - 1, a.ny) * a.nx;
 }
 
+// Wave-aggregated LDS counts: the lanes of a wave with one key take
+// consecutive ranks in lane order from one atomic of the key's first lane
+// (particles kept sorted by cell give a wave few distinct keys); after
+// kAggKeys keys the remaining lanes add their own. Returns the lane's rank
+// (its slot among this key's counts so far); call from every lane.
+constexpr int kAggKeys = 8;
+
+__device__ __forceinline__ int wave_agg_rank(bool valid, int key, int *cnt)
+{
+    const int lane = lane_id();
+    const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (kWave - lane));
+    unsigned long long pending = __ballot(valid);
+    int rank = 0;
+    for (int r = 0; r < kAggKeys && pending; ++r) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const int k = __shfl(key, leader, kWave);
+        const unsigned long long m = __ballot(valid && key == k);
+        int b = 0;
+        if (lane == leader) b = atomicAdd(&cnt[k], __popcll(m));
+        b = __shfl(b, leader, kWave);
+        if (valid && key == k) rank = b + __popcll(m & below);
+        pending &= ~m;
+    }
+    if ((pending >> lane) & 1ull) rank = atomicAdd(&cnt[key], 1);
+    return rank;
+}
+
 __global__ void __launch_bounds__(1024) k_s2_count(S2Args a, int *count)
 {
     __shared__ int h[kNn2Max];
     const int nn = a.nx * a.ny;
     for (int k = threadIdx.x; k < nn; k += blockDim.x) h[k] = 0;
     __syncthreads();
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        atomicAdd(&h[cell_key(a.x1[i], a.x2[i], a)], 1);
+    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < a.n; i0 += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = i0 + threadIdx.x;
+        const bool valid = i < a.n;
+        (void)wave_agg_rank(valid, valid ? cell_key(a.x1[i], a.x2[i], a) : 0, h);
+    }
     __syncthreads();
     for (int k = threadIdx.x; k < nn; k += blockDim.x)
         if (h[k]) atomicAdd(&count[k], h[k]);
@@ -464,7 +526,8 @@ struct S2Perm {
 };
 
 // each round: 1024 particles of this workgroup take consecutive slots of
-// their cell, reserved per cell with one global atomic
+// their cell, reserved per cell with one global atomic; within a wave the
+// lanes of a cell take consecutive slots in lane order (coalesced stores)
 __global__ void __launch_bounds__(1024) k_s2_scatter(S2Args a, int *cursor, S2Perm pm)
 {
     __shared__ int cnt[kNn2Max];
@@ -476,11 +539,8 @@ __global__ void __launch_bounds__(1024) k_s2_scatter(S2Args a, int *cursor, S2Pe
         __syncthreads();
         const int64_t i = r0 + threadIdx.x;
         const bool valid = i < a.n;
-        int key = 0, rank = 0;
-        if (valid) {
-            key = cell_key(pm.src[0][i], pm.src[1][i], a);
-            rank = atomicAdd(&cnt[key], 1);
-        }
+        const int key = valid ? cell_key(pm.src[0][i], pm.src[1][i], a) : 0;
+        const int rank = wave_agg_rank(valid, key, cnt);
         __syncthreads();
         for (int k = threadIdx.x; k < nn; k += blockDim.x)
             if (cnt[k]) base[k] = atomicAdd(&cursor[k], cnt[k]);
@@ -533,8 +593,6 @@ struct gempic_splitting2d {
     int *d_err = nullptr, *d_count = nullptr, *d_cursor = nullptr;
     int blocks = 0;
     int64_t sort_interval = 20, steps_since_sort = 0;
-    double qx[3] = {0, 0, 0}, qw[3] = {0, 0, 0};
-    int nq = 1;
 };
 
 namespace {
@@ -555,7 +613,7 @@ size_t s2_shmem(int op, int nx, int ny, int p)
     const size_t fsz = (size_t)(nx + p) * (ny + p);
     const int nf = op == S2_HE ? 3 : (op == S2_RHO ? 0 : 2);
     const bool dep = op != S2_HE;
-    return sizeof(double) * (nf * fsz + (dep ? (size_t)nx * ny : 0) +
+    return sizeof(double) * (nf * fsz + (dep ? (size_t)grid_stride(nx) * ny : 0) +
                              (size_t)(s2_threads(op) / kWave) * 16);
 }
 
@@ -593,11 +651,6 @@ S2Args s2_args(gempic_splitting2d *h, double dt, int op)
     a.cw = h->cw;
     a.qm = h->qm;
     a.dt = dt;
-    for (int k = 0; k < 3; ++k) {
-        a.qx[k] = h->qx[k];
-        a.qw[k] = h->qw[k];
-    }
-    a.nq = h->nq;
     a.x1 = h->d_p[0];
     a.x2 = h->d_p[1];
     a.v1 = h->d_p[2];
@@ -749,8 +802,6 @@ int gempic_splitting2d_create(int64_t nx, int64_t ny, double xmin, double xmax, 
     h->mass = mass;
     h->cw = common_weight == 0.0 ? 1.0 / (double)std::max<int64_t>(n_particles, 1) : common_weight;
     h->qm = charge / mass;
-    h->nq = (degree + 1) / 2;           // n_quad_points of the degree p-1 smoother
-    gh::gauss_legendre(h->nq, h->qx, h->qw);
     int dev = 0, cus = 256, maxsh = 65536;
     if (hipGetDevice(&dev) == hipSuccess) {
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);

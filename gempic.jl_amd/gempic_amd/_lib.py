@@ -60,6 +60,8 @@ SIGNATURES = {
     "gempic_ensemble_create": (_I, [_I, ctypes.POINTER(_VP), ctypes.POINTER(_VP)]),
     "gempic_ensemble_destroy": (_I, [_VP]),
     "gempic_ensemble_strang_splitting": (_I, [_VP, _P, _I64, _I]),
+    "gempic_ensemble_autotune": (_I, [_VP, _P, _I64, ctypes.POINTER(_I), _I,
+                                      ctypes.POINTER(_I), _P]),
     "gempic_ensemble_strang_splitting_adjoint": (_I, [_VP, _P, _I64, _I64, _P, _I]),
     "gempic_set_sort_interval": (_I, [_VP, _I64]),
     "gempic_sort_particles": (_I, [_VP]),

@@ -33,7 +33,7 @@ __all__ = [
     "HamiltonianSplittingBoris", "staggering_", "push_v_epart_", "push_v_bpart_",
     "push_x_accumulate_j_", "sample_", "sample_particles_", "strang_splitting_adjoint_",
     "get_adjoint", "solve_poisson_adjoint", "AdjointResult", "Ensemble",
-    "ensemble_strang_splitting_", "ensemble_strang_splitting_adjoint_",
+    "ensemble_strang_splitting_", "ensemble_strang_splitting_adjoint_", "ensemble_autotune",
 ]
 
 
@@ -656,6 +656,29 @@ def ensemble_strang_splitting_(ens: Ensemble, dt, number_steps: int, graph: bool
     for h in ens.splittings:
         if h.sync:
             h.to_host()
+
+
+def ensemble_autotune(ens: Ensemble, dt, n_steps: int = 20,
+                      particles_per_workgroup=(2048, 4096, 8192, 16384)) -> dict:
+    """Times n_steps batched steps for each candidate number of particles
+    per workgroup (each instance's pass then takes ceil(N / ppw)
+    workgroups), twice each from the same state, which is restored; the
+    fastest stays set on the member splittings. Returns the choice and the
+    ms per step of every candidate."""
+    m = len(ens.splittings)
+    dts = np.broadcast_to(np.asarray(dt, dtype=np.float64), (m,)).copy()
+    for h in ens.splittings:
+        if h.sync:
+            h.to_device()
+    cand = (ctypes.c_int * len(particles_per_workgroup))(*[int(p) for p in particles_per_workgroup])
+    best = ctypes.c_int(0)
+    ms = np.zeros(len(particles_per_workgroup))
+    check(_lib.load().gempic_ensemble_autotune(ens._e, ptr(dts), int(n_steps), cand,
+                                               len(particles_per_workgroup), ctypes.byref(best),
+                                               ptr(ms)))
+    return {"particles_per_workgroup": int(best.value), "n_steps": int(n_steps),
+            "candidates": [int(p) for p in particles_per_workgroup],
+            "ms_per_step": [float(v) for v in ms]}
 
 
 def ensemble_strang_splitting_adjoint_(ens: Ensemble, dt, number_steps: int,

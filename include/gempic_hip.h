@@ -132,6 +132,15 @@ int gempic_ensemble_destroy(gempic_ensemble *e);
  * replays. Results equal those of the stream-queued steps bit for bit. */
 #define GEMPIC_GRAPH 64
 int gempic_ensemble_strang_splitting(gempic_ensemble *e, const double *dt, int64_t n_steps, int flags);
+/* Launch autotuning of an ensemble (no counterpart in the reference; it
+ * replaces the JULIA_NUM_THREADS knob of hamiltonian_splitting.jl:62 for a
+ * batch): for each candidate number of particles per workgroup (>= 256),
+ * every instance's pass takes ceil(N / ppw) workgroups; n_steps batched
+ * steps are timed twice per candidate from the same state, which is
+ * restored on the device after every trial; the fastest candidate stays set
+ * on the member handles. ms_per_step (n_candidates doubles) may be NULL. */
+int gempic_ensemble_autotune(gempic_ensemble *e, const double *dt, int64_t n_steps, const int *ppw,
+                             int n_candidates, int *best_ppw, double *ms_per_step);
 /* The reverse mode of every splitting of the ensemble (gempic_strang_splitting_adjoint
  * per splitting, batched: one launch per pass, re-run and field part for all
  * of them), with loss loss_weight[i] * sum_s 1/2 l2norm_squared(B_s, p-1)
@@ -489,10 +498,11 @@ int gempic_maxwell2d_compute_e_from_rho(gempic_maxwell2d *m, const double *rho, 
  * (maxwell_2d_fem.jl:26-87) with E1 in splines of degrees (p-1, p), E2
  * (p, p-1), E3 (p, p), B1 (p, p-1), B2 (p-1, p), B3 (p-1, p-1):
  *   operatorHp1/Hp2: the position moves along x1 / x2 with the exact line
- *     integral of the degree p-1 splines (as update_jv!,
- *     particle_mesh_coupling_1d.jl:385-414), the current j1 / j2 and the
- *     magnetic kicks of the other velocities, then E -= M1^{-1} j
- *     (compute_e_from_j!, maxwell_2d_fem.jl:455-459);
+ *     integrals of the degree p-1 splines (those of update_jv!,
+ *     particle_mesh_coupling_1d.jl:385-414, here from the splines'
+ *     antiderivative), the current j1 / j2 and the magnetic kicks of the
+ *     other velocities, then E -= M1^{-1} j (compute_e_from_j!,
+ *     maxwell_2d_fem.jl:455-459);
  *   operatorHp3: the kicks of v1, v2 by v3 B and the current j3 = dt q v3 S;
  *   operatorHE: v += dt qm E(x), then compute_b_from_e! (:423-444);
  *   operatorHB: compute_e_from_b! (:370-412);

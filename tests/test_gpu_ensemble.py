@@ -137,3 +137,30 @@ def test_gpu_ensemble_graph_replay_equals_stream_steps():
             np.testing.assert_array_equal(hg.e_dofs[0], he.e_dofs[0])
             np.testing.assert_array_equal(hg.e_dofs[1], he.e_dofs[1])
             np.testing.assert_array_equal(hg.b_dofs, he.b_dofs)
+
+
+def test_gpu_ensemble_autotune_restores_state_and_sets_the_choice():
+    """gempic_ensemble_autotune: every trial starts from the same state and
+    the state after tuning is the one before it (bitwise); the choice is one
+    of the candidates."""
+    specs = SPECS[:3]
+    dts = [_dt(nx) for (_, nx, _, _) in specs]
+    hs = [_instance(n, nx, k, seed) for (n, nx, k, seed) in specs]
+    ens = G.Ensemble([h for h, _, _ in hs])
+    G.ensemble_strang_splitting_(ens, dts, 3)
+    before = [(pg.array.copy(), h.e_dofs[0].copy(), h.e_dofs[1].copy(), h.b_dofs.copy())
+              for h, pg, _ in hs]
+    cands = (1024, 2048, 4096)
+    r = G.ensemble_autotune(ens, dts, n_steps=4, particles_per_workgroup=cands)
+    assert r["particles_per_workgroup"] in cands
+    assert len(r["ms_per_step"]) == 3 and all(v > 0 for v in r["ms_per_step"])
+    for (h, pg, _), ref in zip(hs, before):
+        h.to_host()
+        np.testing.assert_array_equal(pg.array, ref[0])
+        np.testing.assert_array_equal(h.e_dofs[0], ref[1])
+        np.testing.assert_array_equal(h.e_dofs[1], ref[2])
+        np.testing.assert_array_equal(h.b_dofs, ref[3])
+    # the ensemble runs on with the chosen launch
+    G.ensemble_strang_splitting_(ens, dts, 5)
+    for h, pg, _ in hs:
+        assert np.all(np.isfinite(pg.array)) and np.all(np.isfinite(h.b_dofs))
