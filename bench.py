@@ -458,7 +458,15 @@ def ensemble_run(G, _lib, n_inst, steps, seq_steps, adj_steps=500, adj_every=25,
         dts.append(min(0.05, 0.5 * L / nx))
         total += n
     ens = G.Ensemble(hs)
+    # the forward in the reference's arithmetic, then in the fused one (the
+    # reverse mode below runs in the reference's arithmetic)
     G.ensemble_strang_splitting_(ens, dts, 2)                # kernel loads
+    t0 = time.perf_counter()
+    G.ensemble_strang_splitting_(ens, dts, steps)
+    t_ref = time.perf_counter() - t0
+    for h in hs:
+        h.set_arithmetic("fused")
+    G.ensemble_strang_splitting_(ens, dts, 2)
     # particles per workgroup of every instance's pass, tuned on this state
     tune = G.ensemble_autotune(ens, dts, 10, particles_per_workgroup=(2048, 4096, 8192, 16384))
     t0 = time.perf_counter()
@@ -478,7 +486,10 @@ def ensemble_run(G, _lib, n_inst, steps, seq_steps, adj_steps=500, adj_every=25,
     adj = None
     if adj_steps:
         # forward and forward + reverse mode of every instance (loss: dt times the
-        # magnetic energy summed over the steps), same number of steps
+        # magnetic energy summed over the steps), same number of steps, both in
+        # the reference's arithmetic
+        for h in hs:
+            h.set_arithmetic("reference")
         t0 = time.perf_counter()
         G.ensemble_strang_splitting_(ens, dts, adj_steps)
         t_f = time.perf_counter() - t0
@@ -494,8 +505,10 @@ def ensemble_run(G, _lib, n_inst, steps, seq_steps, adj_steps=500, adj_every=25,
                         f"N in [65536, 1048576], nx in {{16, 32, 64}}, k in [1, 1.5], "
                         f"dt = min(0.05, dx/2)",
             "instances": n_inst, "particles": total, "steps": steps,
+            "arithmetic": "fused",
             "particle_steps_per_s": total * steps / t_ens,
             "graph_particle_steps_per_s": total * steps / t_graph,
+            "reference_arithmetic_particle_steps_per_s": total * steps / t_ref,
             "one_by_one_particle_steps_per_s": total * seq_steps / t_seq,
             "one_by_one_steps": seq_steps, "autotune": tune, "adjoint": adj}
 

@@ -1324,7 +1324,7 @@ struct gempic_ensemble {
     // GEMPIC_GRAPH: the regular step (pass and field part) as a HIP graph,
     // with the launch geometry it was captured for
     hipGraphExec_t step_graph = nullptr;
-    int graph_blocks = 0, graph_groups = 0;
+    int graph_blocks = 0, graph_groups = 0, graph_mode = -1;
     size_t graph_sh_pass = 0, graph_sh_epi = 0;
 };
 
@@ -1517,7 +1517,7 @@ int ens_step_graph(gempic_ensemble *e, const BatchArgs &b, int mode, size_t sh_p
 {
     const int ni = (int)e->hs.size();
     if (e->step_graph && e->graph_blocks == e->bstart[ni] && e->graph_groups == e->estart[ni] &&
-        e->graph_sh_pass == sh_pass && e->graph_sh_epi == sh_epi)
+        e->graph_mode == mode && e->graph_sh_pass == sh_pass && e->graph_sh_epi == sh_epi)
         return GEMPIC_OK;
     if (e->step_graph) {
         (void)hipGraphExecDestroy(e->step_graph);
@@ -1540,6 +1540,7 @@ int ens_step_graph(gempic_ensemble *e, const BatchArgs &b, int mode, size_t sh_p
     }
     e->graph_blocks = e->bstart[ni];
     e->graph_groups = e->estart[ni];
+    e->graph_mode = mode;
     e->graph_sh_pass = sh_pass;
     e->graph_sh_epi = sh_epi;
     return GEMPIC_OK;
@@ -1556,6 +1557,11 @@ int gempic_ensemble_strang_splitting(gempic_ensemble *e, const double *dt, int64
     for (auto *h : e->hs)
         if (!h->particles_set) return fail(GEMPIC_ERR_STATE, "particles not set");
     if (n_steps == 0) return GEMPIC_OK;
+    // the batch runs in the members' arithmetic, which they must share
+    const int fast = fm(e->hs[0]);
+    for (auto *h : e->hs)
+        if (fm(h) != fast)
+            return fail(GEMPIC_ERR_ARGUMENT, "ArgumentError: an ensemble's splittings must share one arithmetic");
     EnsStreams streams(e);
     const int ni = (int)e->hs.size();
     // HB(dt/2) and HE's B update of the first step, per instance
@@ -1578,7 +1584,7 @@ int gempic_ensemble_strang_splitting(gempic_ensemble *e, const double *dt, int64
             rc = ens_upload(e, dt);
             if (rc) return rc;
         }
-        const int mode = (s == 0 ? 0 : M_HE_PREV) | M_HE | M_HP2A | M_HP1 | M_HP2B;
+        const int mode = (s == 0 ? 0 : M_HE_PREV) | M_HE | M_HP2A | M_HP1 | M_HP2B | fast;
         BatchArgs b;
         b.args = e->d_args + (s + 1 < n_steps ? 0 : ni);
         b.block_start = e->d_bstart;
@@ -1600,7 +1606,7 @@ int gempic_ensemble_strang_splitting(gempic_ensemble *e, const double *dt, int64
     b.block_start = e->d_bstart;
     b.epi_start = e->d_estart;
     b.ninst = ni;
-    HIP_TRY(launch_pass_batch(e->p0, e->p1, M_HE_PREV, b, e->bstart[ni],
+    HIP_TRY(launch_pass_batch(e->p0, e->p1, M_HE_PREV | fast, b, e->bstart[ni],
                               pass_shmem_bytes(e->max_nx, M_HE_PREV), e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     for (auto *h : e->hs) {

@@ -1328,9 +1328,15 @@ static hipError_t launch_batch_pd(bool epi, int mode, const BatchArgs &b, int to
     switch (mode) {
         GK_B(M_HE | M_HP2A | M_HP1 | M_HP2B)
         GK_B(M_HE_PREV | M_HE | M_HP2A | M_HP1 | M_HP2B)
+        GK_B(M_FAST | M_HE | M_HP2A | M_HP1 | M_HP2B)
+        GK_B(M_FAST | M_HE_PREV | M_HE | M_HP2A | M_HP1 | M_HP2B)
     case M_HE_PREV:
         if (epi) return hipErrorInvalidValue;
         hipLaunchKernelGGL((k_pass1d_batch<P0, P1, M_HE_PREV>), dim3(total), dim3(kBlock), shmem, s, b);
+        return hipGetLastError();
+    case M_FAST | M_HE_PREV:
+        if (epi) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_pass1d_batch<P0, P1, M_FAST | M_HE_PREV>), dim3(total), dim3(kBlock), shmem, s, b);
         return hipGetLastError();
     default:
         return hipErrorInvalidValue;

@@ -257,8 +257,10 @@ int gempic_set_fusion(gempic_splitting *h, int kernels_per_step);
  * spline bases, fused multiply-adds, each current segment's integral by its
  * midpoint value plus the exact curvature term instead of Gauss-Legendre),
  * which agree with the reference to rounding (a few ulp per operation), not
- * bitwise. It needs the one-kernel fused step; the reverse mode, ensembles,
- * the Boris splitting and 1d1v always use the reference's arithmetic. */
+ * bitwise. It needs the one-kernel fused step. An ensemble's forward runs in
+ * its members' arithmetic, which they must share; the reverse mode (of a
+ * splitting or an ensemble), the Boris splitting and 1d1v always use the
+ * reference's arithmetic. */
 #define GEMPIC_ARITH_REFERENCE 0
 #define GEMPIC_ARITH_FUSED 1
 int gempic_set_arithmetic(gempic_splitting *h, int arith);

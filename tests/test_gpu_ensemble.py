@@ -164,3 +164,31 @@ def test_gpu_ensemble_autotune_restores_state_and_sets_the_choice():
     G.ensemble_strang_splitting_(ens, dts, 5)
     for h, pg, _ in hs:
         assert np.all(np.isfinite(pg.array)) and np.all(np.isfinite(h.b_dofs))
+
+
+def test_gpu_ensemble_in_fused_arithmetic_equals_standalone_runs():
+    """Splittings set to the fused arithmetic step as a batch in it: every
+    instance equals its own fused strang_splitting! bit for bit across a
+    sort; splittings of mixed arithmetics are rejected."""
+    specs = SPECS[:3]
+    dts = [_dt(nx) for (_, nx, _, _) in specs]
+    steps = 25
+    alone = []
+    for (n, nx, k, seed) in specs:
+        h, pg, w = _instance(n, nx, k, seed)
+        h.set_arithmetic("fused")
+        G.strang_splitting_(h, _dt(nx), steps)
+        alone.append((pg.array.copy(), h.e_dofs[0].copy(), h.e_dofs[1].copy(), h.b_dofs.copy()))
+    hs = [_instance(n, nx, k, seed) for (n, nx, k, seed) in specs]
+    for h, _, _ in hs:
+        h.set_arithmetic("fused")
+    ens = G.Ensemble([h for h, _, _ in hs])
+    G.ensemble_strang_splitting_(ens, dts, steps)
+    for (h, pg, _), ref in zip(hs, alone):
+        np.testing.assert_array_equal(pg.array, ref[0])
+        np.testing.assert_array_equal(h.e_dofs[0], ref[1])
+        np.testing.assert_array_equal(h.e_dofs[1], ref[2])
+        np.testing.assert_array_equal(h.b_dofs, ref[3])
+    hs[1][0].set_arithmetic("reference")
+    with pytest.raises(ValueError):
+        G.ensemble_strang_splitting_(ens, dts, 2)
