@@ -101,6 +101,50 @@ Launch plan(int64_t n, int max_blocks)
     return L;
 }
 
+#ifndef GEMPIC_GEN_SKEW
+#define GEMPIC_GEN_SKEW 0.25
+#endif
+constexpr double kGenSkew = GEMPIC_GEN_SKEW;
+
+// The particle pass's partition in generations. A launch of G * cus
+// workgroups (2 <= G <= kMaxGen) fills every CU G deep, workgroup b on CU
+// b mod cus beside b +- cus, ..., and a SIMD issues from its oldest wave
+// first: with equal chunks the last generation ends well after the first,
+// and each CU runs the end of the pass with few waves. Chunks that shrink
+// linearly from the first generation to the last (weights 1 + skew (G - 1 -
+// 2g) / (G - 1), in whole waves of 64 particles) let the generations end
+// together. The partition depends only on n, the launch and the CU count,
+// so every deposit's summation order stays a fixed function of the particle
+// order.
+void set_generations(PassArgs &a, const Launch &L, int cus)
+{
+    a.gen_blocks = 0;
+    if (kGenSkew <= 0.0 || cus < 1 || L.blocks % cus != 0) return;
+    const int G = L.blocks / cus;
+    if (G < 2 || G > kMaxGen) return;
+    // waves of 64 particles per column of G workgroups (one per generation)
+    const int64_t T = (a.n + (int64_t)64 * cus - 1) / ((int64_t)64 * cus);
+    if (T < 8 * G) return;
+    double wt[kMaxGen], wsum = 0.0;
+    for (int g = 0; g < G; ++g) {
+        wt[g] = 1.0 + kGenSkew * (double)(G - 1 - 2 * g) / (double)(G - 1);
+        wsum += wt[g];
+    }
+    int64_t acc = 0, waves[kMaxGen];
+    for (int g = 0; g + 1 < G; ++g) {
+        waves[g] = std::max<int64_t>(1, std::llround((double)T * wt[g] / wsum));
+        acc += waves[g];
+    }
+    if (T - acc < 1) return;
+    waves[G - 1] = T - acc;
+    a.gen_base[0] = 0;
+    for (int g = 0; g < G; ++g) {
+        a.gen_chunk[g] = waves[g] * 64;
+        if (g + 1 < G) a.gen_base[g + 1] = a.gen_base[g] + a.gen_chunk[g] * cus;
+    }
+    a.gen_blocks = cus;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -303,6 +347,7 @@ int run_pass(gempic_splitting *h, int mode, double dt_he, double dt_hp2, double 
     a.chunk = L.chunk;
     a.iters = L.iters;
     a.ngroups = L.ngroups;
+    set_generations(a, L, h->cus);
     a.dt_he = dt_he;
     a.dt_hp2 = dt_hp2;
     a.dt_hp1 = dt_hp1;
@@ -1347,6 +1392,7 @@ int ens_upload(gempic_ensemble *e, const double *dts)
         a.chunk = L.chunk;
         a.iters = L.iters;
         a.ngroups = L.ngroups;
+        set_generations(a, L, h->cus);
         a.dt_he = hdt;
         a.dt_hp2 = hdt;
         a.dt_hp1 = 1.0 * dt;
@@ -1815,6 +1861,7 @@ int gempic_ensemble_strang_splitting_adjoint(gempic_ensemble *e, const double *d
             a.chunk = L.chunk;
             a.iters = L.iters;
             a.ngroups = L.ngroups;
+            set_generations(a, L, h->cus);
             a.dt_he = hdt;
             a.dt_hp2 = hdt;
             a.dt_hp1 = 1.0 * dt[i];

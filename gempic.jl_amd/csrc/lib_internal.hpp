@@ -50,6 +50,8 @@ enum ModeBits : int {
                      // reference to rounding, not bitwise)
 };
 
+constexpr int kMaxGen = 8;        // generations of a particle pass partition (PassArgs)
+
 struct PassArgs {
     int64_t n;
     int64_t chunk;       // particles per workgroup (multiple of kBlock)
@@ -75,6 +77,13 @@ struct PassArgs {
                                   // step's end to hist[4][nx] and the next pass's fields
                                   // E1, E2, B, E2P to hist[5..8][nx] (nullptr: no record)
     double *cx, *cv1, *cv2;       // fused Strang pass: also store x, v1, v2 here (nullptr: no)
+    // Particle pass partition in generations (gen_blocks > 0): workgroups
+    // g * gen_blocks .. (g + 1) * gen_blocks - 1 form generation g and take
+    // gen_chunk[g] particles each (a multiple of 64), from gen_base[g] on;
+    // gen_blocks = 0 is the uniform partition by chunk.
+    int gen_blocks;
+    int64_t gen_chunk[kMaxGen];
+    int64_t gen_base[kMaxGen];
 };
 
 // error flag bits set by kernels

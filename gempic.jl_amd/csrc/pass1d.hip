@@ -447,10 +447,24 @@ __device__ __forceinline__ void pass1d_body(const PassArgs &a, int bid, double *
     double *sE1 = pE1 + kPad, *sE2p = pE2p + kPad, *sE2 = pE2 + kPad, *sB = pB + kPad;
 
     // this workgroup's chunk: particles base .. base + cnt_here - 1, addressed
-    // through per-chunk base pointers and a 32-bit index
-    const int64_t base = (int64_t)bid * a.chunk;
+    // through per-chunk base pointers and a 32-bit index. With generations
+    // (PassArgs::gen_blocks), workgroup bid = g * gen_blocks + r takes
+    // gen_chunk[g] particles from gen_base[g] + r * gen_chunk[g]. Iteration it
+    // gives lane l of wave wv particle it * kBlock + wv * 64 + l; a wave stops
+    // after its last iteration with a particle (no barrier inside the loop).
+    int64_t base, chunk;
+    if (a.gen_blocks > 0) {
+        const int g = bid / a.gen_blocks;
+        chunk = a.gen_chunk[g];
+        base = a.gen_base[g] + (int64_t)(bid - g * a.gen_blocks) * chunk;
+    } else {
+        chunk = a.chunk;
+        base = (int64_t)bid * a.chunk;
+    }
     const int64_t left = a.n - base;
-    const int cnt_here = left < a.chunk ? (int)left : (int)a.chunk;
+    const int cnt_here = left <= 0 ? 0 : left < chunk ? (int)left : (int)chunk;
+    const int wave_first = (tid / kWave) * kWave;
+    const int iters = cnt_here > wave_first ? (cnt_here - wave_first + kBlock - 1) / kBlock : 0;
     double *__restrict__ px = a.x + base;
     double *__restrict__ pv1 = a.v1 + base;
     double *__restrict__ pv2 = a.v2 + base;
@@ -469,8 +483,8 @@ __device__ __forceinline__ void pass1d_body(const PassArgs &a, int bid, double *
     };
     double nx_x = 0.0, nx_v1 = 0.0, nx_v2 = 0.0, nx_w = 0.0;     // iteration it + 1
     double n2_x = 0.0, n2_v1 = 0.0, n2_v2 = 0.0, n2_w = 0.0;     // it + 2 (kPrefetch 2)
-    if (a.iters > 0) load_particle(tid, nx_x, nx_v1, nx_v2, nx_w);
-    if (kPrefetch > 1 && a.iters > 1) load_particle(tid + kBlock, n2_x, n2_v1, n2_v2, n2_w);
+    if (iters > 0) load_particle(tid, nx_x, nx_v1, nx_v2, nx_w);
+    if (kPrefetch > 1 && iters > 1) load_particle(tid + kBlock, n2_x, n2_v1, n2_v2, n2_w);
 
     load_padded(pE1, a.grid + S_E1 * nx, nx);
     load_padded(pE2p, a.grid + S_E2P * nx, nx);
@@ -492,7 +506,7 @@ __device__ __forceinline__ void pass1d_body(const PassArgs &a, int bid, double *
     const double rdx = 1.0 / dx;     // div_by: (x - xmin) / dx, correctly rounded
     int errs = 0;
 
-    for (int it = 0; it < a.iters; ++it) {
+    for (int it = 0; it < iters; ++it) {
         const int i = it * kBlock + tid;
         const bool valid = i < cnt_here;
         double x = nx_x, v1 = nx_v1, v2 = nx_v2, w = nx_w;
@@ -501,13 +515,13 @@ __device__ __forceinline__ void pass1d_body(const PassArgs &a, int bid, double *
             nx_v1 = n2_v1;
             nx_v2 = n2_v2;
             nx_w = n2_w;
-            if (it + 2 < a.iters) load_particle(i + 2 * kBlock, n2_x, n2_v1, n2_v2, n2_w);
+            if (it + 2 < iters) load_particle(i + 2 * kBlock, n2_x, n2_v1, n2_v2, n2_w);
         } else {
             (void)n2_x;
             (void)n2_v1;
             (void)n2_v2;
             (void)n2_w;
-            if (it + 1 < a.iters) load_particle(i + kBlock, nx_x, nx_v1, nx_v2, nx_w);
+            if (it + 1 < iters) load_particle(i + kBlock, nx_x, nx_v1, nx_v2, nx_w);
         }
         // marker charge, as (w*charge)*common_weight (hamiltonian_splitting_1d2v.jl:62-64)
         const double q = (w * a.charge) * a.cw;
