@@ -104,7 +104,11 @@ Launch plan(int64_t n, int max_blocks)
 #ifndef GEMPIC_GEN_SKEW
 #define GEMPIC_GEN_SKEW 0.25
 #endif
-constexpr double kGenSkew = GEMPIC_GEN_SKEW;
+#ifndef GEMPIC_ADJ_GEN_SKEW
+#define GEMPIC_ADJ_GEN_SKEW 0.15
+#endif
+constexpr double kGenSkew = GEMPIC_GEN_SKEW;         // forward and Boris passes
+constexpr double kAdjGenSkew = GEMPIC_ADJ_GEN_SKEW;  // reverse-mode passes
 
 // The particle pass's partition in generations. A launch of G * cus
 // workgroups (2 <= G <= kMaxGen) fills every CU G deep, workgroup b on CU
@@ -116,10 +120,10 @@ constexpr double kGenSkew = GEMPIC_GEN_SKEW;
 // together. The partition depends only on n, the launch and the CU count,
 // so every deposit's summation order stays a fixed function of the particle
 // order.
-void set_generations(PassArgs &a, const Launch &L, int cus)
+void set_generations(PassArgs &a, const Launch &L, int cus, double skew = kGenSkew)
 {
     a.gen_blocks = 0;
-    if (kGenSkew <= 0.0 || cus < 1 || L.blocks % cus != 0) return;
+    if (skew <= 0.0 || cus < 1 || L.blocks % cus != 0) return;
     const int G = L.blocks / cus;
     if (G < 2 || G > kMaxGen) return;
     // waves of 64 particles per column of G workgroups (one per generation)
@@ -127,7 +131,7 @@ void set_generations(PassArgs &a, const Launch &L, int cus)
     if (T < 8 * G) return;
     double wt[kMaxGen], wsum = 0.0;
     for (int g = 0; g < G; ++g) {
-        wt[g] = 1.0 + kGenSkew * (double)(G - 1 - 2 * g) / (double)(G - 1);
+        wt[g] = 1.0 + skew * (double)(G - 1 - 2 * g) / (double)(G - 1);
         wsum += wt[g];
     }
     int64_t acc = 0, waves[kMaxGen];
@@ -901,6 +905,7 @@ AdjArgs adj_args(gempic_splitting *h, double dt, const Launch &L)
     A.p.chunk = L.chunk;
     A.p.iters = L.iters;
     A.p.ngroups = L.ngroups;
+    set_generations(A.p, L, h->cus, kAdjGenSkew);
     const double hdt = 0.5 * dt;
     A.p.dt_he = hdt;
     A.p.dt_hp2 = hdt;
@@ -1232,6 +1237,7 @@ int run_boris(gempic_splitting *h, int ops, double dt_e, double dt_b, double dt_
     a.chunk = L.chunk;
     a.iters = L.iters;
     a.ngroups = L.ngroups;
+    set_generations(a, L, h->cus);
     a.dtqm_he = dt_e * h->qm;              // dt * qm .* [E1, E2]
     a.dtqm_hp2 = h->qm * 0.5 * dt_b;       // qm * 0.5 * dt
     a.dt_hp1 = dt_x;

@@ -404,6 +404,37 @@ __device__ __forceinline__ void segment_bounds(int s, int io, double ro, int inw
 
 __device__ __forceinline__ bool pos_ok(double xi) { return fabs(xi) < 1.0e9; }
 
+// The particles of workgroup bid of a particle pass: base .. base + cnt - 1.
+// With generations (PassArgs::gen_blocks), workgroup bid = g * gen_blocks + r
+// takes gen_chunk[g] particles from gen_base[g] + r * gen_chunk[g]; else bid
+// takes `chunk` from bid * chunk. Iteration it gives lane l of wave wv the
+// particle it * kBlock + wv * 64 + l, and `iters` counts the calling wave's
+// iterations up to its last particle (the loops have no barrier inside, so
+// the waves of a workgroup may run different counts).
+struct PassChunk {
+    int64_t base;
+    int cnt, iters;
+};
+
+__device__ __forceinline__ PassChunk pass_chunk(const PassArgs &a, int bid)
+{
+    PassChunk c;
+    int64_t chunk;
+    if (a.gen_blocks > 0) {
+        const int g = bid / a.gen_blocks;
+        chunk = a.gen_chunk[g];
+        c.base = a.gen_base[g] + (int64_t)(bid - g * a.gen_blocks) * chunk;
+    } else {
+        chunk = a.chunk;
+        c.base = (int64_t)bid * a.chunk;
+    }
+    const int64_t left = a.n - c.base;
+    c.cnt = left <= 0 ? 0 : left < chunk ? (int)left : (int)chunk;
+    const int wave_first = (int)(threadIdx.x / kWave) * kWave;
+    c.iters = c.cnt > wave_first ? (c.cnt - wave_first + kBlock - 1) / kBlock : 0;
+    return c;
+}
+
 // ---------------------------------------------------------------------------
 // The particle pass
 // ---------------------------------------------------------------------------
@@ -447,24 +478,11 @@ __device__ __forceinline__ void pass1d_body(const PassArgs &a, int bid, double *
     double *sE1 = pE1 + kPad, *sE2p = pE2p + kPad, *sE2 = pE2 + kPad, *sB = pB + kPad;
 
     // this workgroup's chunk: particles base .. base + cnt_here - 1, addressed
-    // through per-chunk base pointers and a 32-bit index. With generations
-    // (PassArgs::gen_blocks), workgroup bid = g * gen_blocks + r takes
-    // gen_chunk[g] particles from gen_base[g] + r * gen_chunk[g]. Iteration it
-    // gives lane l of wave wv particle it * kBlock + wv * 64 + l; a wave stops
-    // after its last iteration with a particle (no barrier inside the loop).
-    int64_t base, chunk;
-    if (a.gen_blocks > 0) {
-        const int g = bid / a.gen_blocks;
-        chunk = a.gen_chunk[g];
-        base = a.gen_base[g] + (int64_t)(bid - g * a.gen_blocks) * chunk;
-    } else {
-        chunk = a.chunk;
-        base = (int64_t)bid * a.chunk;
-    }
-    const int64_t left = a.n - base;
-    const int cnt_here = left <= 0 ? 0 : left < chunk ? (int)left : (int)chunk;
-    const int wave_first = (tid / kWave) * kWave;
-    const int iters = cnt_here > wave_first ? (cnt_here - wave_first + kBlock - 1) / kBlock : 0;
+    // through per-chunk base pointers and a 32-bit index
+    const PassChunk pc = pass_chunk(a, bid);
+    const int64_t base = pc.base;
+    const int cnt_here = pc.cnt;
+    const int iters = pc.iters;
     double *__restrict__ px = a.x + base;
     double *__restrict__ pv1 = a.v1 + base;
     double *__restrict__ pv2 = a.v2 + base;
@@ -1510,9 +1528,10 @@ __global__ void __launch_bounds__(kBlock) k_boris1d(PassArgs a)
 
     const double xmin = a.xmin, dx = a.dx;
     const double rdx = 1.0 / dx;     // div_by: (x - xmin) / dx, correctly rounded
-    const int64_t base = (int64_t)blockIdx.x * a.chunk;
-    const int64_t left = a.n - base;
-    const int cnt_here = left < a.chunk ? (int)left : (int)a.chunk;
+    const PassChunk pc = pass_chunk(a, blockIdx.x);
+    const int64_t base = pc.base;
+    const int cnt_here = pc.cnt;
+    const int iters = pc.iters;
     double *__restrict__ px = a.x + base;
     double *__restrict__ pv1 = a.v1 + base;
     double *__restrict__ pv2 = a.v2 + base;
@@ -1520,20 +1539,20 @@ __global__ void __launch_bounds__(kBlock) k_boris1d(PassArgs a)
     int errs = 0;
 
     double nx_x = 0.0, nx_v1 = 0.0, nx_v2 = 0.0, nx_w = 0.0;
-    if (a.iters > 0 && tid < cnt_here) {
+    if (iters > 0 && tid < cnt_here) {
         nx_x = px[tid];
         nx_v1 = pv1[tid];
         nx_v2 = pv2[tid];
         if (PUSHX) nx_w = pw[tid];
     }
-    for (int it = 0; it < a.iters; ++it) {
+    for (int it = 0; it < iters; ++it) {
         const int i = it * kBlock + tid;
         const bool valid = i < cnt_here;
         double x = nx_x, v1 = nx_v1, v2 = nx_v2;
         const double w = nx_w;
         {
             const int in = i + kBlock;
-            if (it + 1 < a.iters && in < cnt_here) {
+            if (it + 1 < iters && in < cnt_here) {
                 nx_x = px[in];
                 nx_v1 = pv1[in];
                 nx_v2 = pv2[in];
@@ -2600,9 +2619,10 @@ __device__ __forceinline__ void adj1d_body(const AdjArgs &A, int bid, double *sm
     const double rdx = 1.0 / dx;     // div_by: every quotient by dx, correctly rounded
     const double dtqm = a.dtqm_he, g = a.dtqm_hp2, dt1 = a.dt_hp1;
     const double s0 = a.scaling0, s1 = a.scaling1;
-    const int64_t base = (int64_t)bid * a.chunk;
-    const int64_t left = a.n - base;
-    const int cnt_here = left < a.chunk ? (int)left : (int)a.chunk;
+    const PassChunk pc = pass_chunk(a, bid);
+    const int64_t base = pc.base;
+    const int cnt_here = pc.cnt;
+    const int iters = pc.iters;
 
     // the loss's explicit dependence on the cell size dx, on the domain
     // length through the mod, and on the particle weights (summed): per lane,
@@ -2614,7 +2634,7 @@ __device__ __forceinline__ void adj1d_body(const AdjArgs &A, int bid, double *sm
     // the work of iteration it (the cotangents are loaded at the top of their
     // own iteration; the forward re-run covers their latency)
     double nx_x = xmin, nx_v1 = 0.0, nx_v2 = 0.0, nx_w = 0.0;
-    if (a.iters > 0 && tid < cnt_here) {
+    if (iters > 0 && tid < cnt_here) {
         nx_x = A.x[base + tid];
         nx_w = a.w[base + tid];
         if constexpr (STEP) {
@@ -2622,14 +2642,14 @@ __device__ __forceinline__ void adj1d_body(const AdjArgs &A, int bid, double *sm
             nx_v2 = A.v2[base + tid];
         }
     }
-    for (int it = 0; it < a.iters; ++it) {
+    for (int it = 0; it < iters; ++it) {
         const int i = it * kBlock + tid;
         const bool valid = i < cnt_here;
         const int64_t gi = base + (valid ? i : 0);
         double x = nx_x, v1 = nx_v1, v2 = nx_v2, w = nx_w;
         {
             const int in = i + kBlock;
-            if (it + 1 < a.iters && in < cnt_here) {
+            if (it + 1 < iters && in < cnt_here) {
                 nx_x = A.x[base + in];
                 nx_w = a.w[base + in];
                 if constexpr (STEP) {
