@@ -404,6 +404,21 @@ __device__ __forceinline__ void segment_bounds(int s, int io, double ro, int inw
 
 __device__ __forceinline__ bool pos_ok(double xi) { return fabs(xi) < 1.0e9; }
 
+// Element i (0 <= i < 2^28) of a workgroup's slice of a particle array, with
+// the byte offset as a 32-bit unsigned value: the access is then the slice's
+// (uniform) base address plus a 32-bit lane offset, with no 64-bit address
+// arithmetic per lane.
+template <class T>
+__device__ __forceinline__ T &elem(T *p, int i)
+{
+    return *reinterpret_cast<T *>(reinterpret_cast<char *>(p) + (uint32_t)i * (uint32_t)sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ const T &elem(const T *p, int i)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(p) + (uint32_t)i * (uint32_t)sizeof(T));
+}
+
 // The particles of workgroup bid of a particle pass: base .. base + cnt - 1.
 // With generations (PassArgs::gen_blocks), workgroup bid = g * gen_blocks + r
 // takes gen_chunk[g] particles from gen_base[g] + r * gen_chunk[g]; else bid
@@ -493,10 +508,10 @@ __device__ __forceinline__ void pass1d_body(const PassArgs &a, int bid, double *
     // staged, so that their latencies overlap
     auto load_particle = [&](int j, double &lx, double &lv1, double &lv2, double &lw) {
         if (j < cnt_here) {
-            lx = px[j];
-            lw = pw[j];
-            if (!RHO_ONLY) lv1 = pv1[j];
-            if (!V1 && !RHO_ONLY) lv2 = pv2[j];
+            lx = elem(px, j);
+            lw = elem(pw, j);
+            if (!RHO_ONLY) lv1 = elem(pv1, j);
+            if (!V1 && !RHO_ONLY) lv2 = elem(pv2, j);
         }
     };
     double nx_x = 0.0, nx_v1 = 0.0, nx_v2 = 0.0, nx_w = 0.0;     // iteration it + 1
@@ -591,11 +606,17 @@ __device__ __forceinline__ void pass1d_body(const PassArgs &a, int bid, double *
                 v2 = v2 + dtqm * e2;
             }
         }
+        // fused arithmetic: B at the old cell's nodes, for Hp2 and Hp1's first piece
+        double bold[W1];
+        if constexpr (FAST && (HP2A || HP1)) {
+#pragma unroll
+            for (int k = 0; k < W1; ++k) bold[k] = sB[key_old - P1 + k];
+        }
         if constexpr (HP2A) {
             // operatorHp2 (hamiltonian_splitting_1d2v.jl:146-163)
             double vals[W0];
             if constexpr (FAST) {
-                const double bf = gather_pf<P1>(b1, sB, key_old);
+                const double bf = gather_pf<P1>(b1, bold, P1);
                 v1 = __builtin_fma(a.dtqm_hp2 * v2, bf, v1);
                 const double ws = (q * v2) * a.scaling0;
 #pragma unroll
@@ -624,6 +645,44 @@ __device__ __forceinline__ void pass1d_body(const PassArgs &a, int bid, double *
             const bool far = !pos_ok(xin) || !pos_ok(xi) || dseg > kMaxCellsPerStep ||
                              dseg < -kMaxCellsPerStep;
             if (valid && far) errs |= E_FAR;
+            if constexpr (FAST) {
+                // The path from xi to xin as signed integrals over its piece in
+                // each cell (the midpoint rule of segment_fast is odd in the
+                // length, so a piece run backwards is its negative): first the
+                // old cell, from the old offset to the new one when the cell is
+                // kept, else to the cell's edge in the direction of motion; then,
+                // for the lanes that cross, the new cell from its edge to the new
+                // offset and the whole cells in between.
+                const bool cross = dseg != 0 && !far;
+                const bool right = dseg > 0;
+                const double qs = q * a.scaling1;
+                auto piece = [&](bool act, int key, double from, double to, const double *bk) {
+                    double sv[W1];
+                    segment_fast<P1>(from, to, 1.0, dx, sv);
+                    double vals[W1];
+                    double sb = sv[0] * bk[0];
+#pragma unroll
+                    for (int m = 0; m < W1; ++m) {
+                        vals[m] = qs * sv[m];
+                        if (m > 0) sb = __builtin_fma(sv[m], bk[m], sb);
+                    }
+                    const double v2s = __builtin_fma(-qm, sb, v2);
+                    if (act) v2 = v2s;
+                    cache_add<W1>(c_j1, act, key, vals, myrow + D_J1 * nx, nx);
+                };
+                piece(valid, key_old, off, cross ? (right ? 1.0 : 0.0) : rn, bold);
+                if (__ballot(valid && cross)) {
+                    const int extra = cross ? (right ? dseg : -dseg) : 0;
+                    const double edge = right ? 0.0 : 1.0;
+                    for (int s = 1;; ++s) {
+                        const bool act = valid && s <= extra;
+                        if (!__ballot(act)) break;
+                        const int cell = s == 1 ? inw : (right ? ic + s - 1 : ic - s + 1);
+                        const int key = imod(cell, nx);
+                        piece(act, key, edge, s == 1 ? rn : 1.0 - edge, sB + (key - P1));
+                    }
+                }
+            } else {
             const int nseg = far ? 1 : 1 + (dseg < 0 ? -dseg : dseg);
             for (int s = 0;; ++s) {
                 const bool act = valid && s < nseg;
@@ -632,30 +691,19 @@ __device__ __forceinline__ void pass1d_body(const PassArgs &a, int bid, double *
                 int cell;
                 segment_bounds(s, ic, off, inw, rn, lo, up, sg, cell);
                 double sv[W1];
-                if constexpr (FAST) segment_fast<P1>(lo, up, sg, dx, sv);
-                else segment<P1, NQ>(lo, up, sg, a.qx, a.qw, dx, sv);
+                segment<P1, NQ>(lo, up, sg, a.qx, a.qw, dx, sv);
                 const int key = imod(cell, nx);
                 double vals[W1];
                 const double *bk = sB + (key - P1);
                 double v2s = v2;
-                if constexpr (FAST) {
-                    const double qs = q * a.scaling1;
-                    double sb = sv[0] * bk[0];
 #pragma unroll
-                    for (int m = 0; m < W1; ++m) {
-                        vals[m] = qs * sv[m];
-                        if (m > 0) sb = __builtin_fma(sv[m], bk[m], sb);
-                    }
-                    v2s = __builtin_fma(-qm, sb, v2);
-                } else {
-#pragma unroll
-                    for (int m = 0; m < W1; ++m) {
-                        vals[m] = q * sv[m] * a.scaling1;
-                        if (!V1) v2s = v2s - qm * sv[m] * bk[m];
-                    }
+                for (int m = 0; m < W1; ++m) {
+                    vals[m] = q * sv[m] * a.scaling1;
+                    if (!V1) v2s = v2s - qm * sv[m] * bk[m];
                 }
                 if (act) v2 = v2s;
                 cache_add<W1>(c_j1, act, key, vals, myrow + D_J1 * nx, nx);
+            }
             }
             x = jl_mod(xn, a.Lx);
             xi_new = xin;
@@ -698,9 +746,9 @@ __device__ __forceinline__ void pass1d_body(const PassArgs &a, int bid, double *
             cache_add<W0>(c_new, valid, key2, vals, myrow + NEW_ROW * nx, nx);
         }
         if (valid) {
-            if constexpr (HP1) px[i] = x;
-            if constexpr (HE_PREV || HE || HP2A || HP2B) pv1[i] = v1;
-            if constexpr (!V1 && (HE_PREV || HE || HP1)) pv2[i] = v2;
+            if constexpr (HP1) elem(px, i) = x;
+            if constexpr (HE_PREV || HE || HP2A || HP2B) elem(pv1, i) = v1;
+            if constexpr (!V1 && (HE_PREV || HE || HP1)) elem(pv2, i) = v2;
         }
         // a recorded fused step also keeps the state after the pass
         if constexpr (HE && HP2A && HP1 && HP2B && !V1)
