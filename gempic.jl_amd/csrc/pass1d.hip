@@ -632,6 +632,8 @@ __device__ __forceinline__ void pass1d_body(const PassArgs &a, int bid, double *
         }
         double xi_new = 0.0;       // (x - xmin) / dx after Hp1, when the mod kept x_new
         bool xi_new_ok = false;
+        int inw_new = 0;           // its cell (trunc) and offset
+        double rn_new = 0.0;
         if constexpr (HP1) {
             // operatorHp1 (hamiltonian_splitting_1d2v.jl:53-82 / _1d1v.jl:308-329)
             const double xn = FAST ? __builtin_fma(a.dt_hp1, v1, x) : x + a.dt_hp1 * v1;
@@ -708,16 +710,27 @@ __device__ __forceinline__ void pass1d_body(const PassArgs &a, int bid, double *
             x = jl_mod(xn, a.Lx);
             xi_new = xin;
             xi_new_ok = __double_as_longlong(x) == __double_as_longlong(xn);
+            inw_new = inw;
+            rn_new = rn;
         }
         if constexpr (NEED_NEW) {
             // rho deposit at the new position (hamiltonian_splitting_1d2v.jl:93-102,
             // diagnostics.jl:189-193) and the second Hp2; the same division as
             // Hp1's when the mod left the position unchanged
-            const double xi2 = xi_new_ok ? xi_new : div_by(x - xmin, dx, rdx);
-            if (valid && RHO_ONLY && !pos_ok(xi2)) errs |= E_INEXACT;
-            const int ic2 = (int)xi2;
-            const double off2 = xi2 - (double)ic2;
-            const int key2 = imod(ic2, nx);
+            // (when the mod kept every lane's position, Hp1's cell and offset are
+            // reused, and the old cell's key when no lane left its cell)
+            double off2;
+            int key2;
+            if (__ballot(valid && !xi_new_ok) == 0) {
+                off2 = rn_new;
+                key2 = __ballot(valid && inw_new != ic) == 0 ? key_old : imod(inw_new, nx);
+            } else {
+                const double xi2 = xi_new_ok ? xi_new : div_by(x - xmin, dx, rdx);
+                if (valid && RHO_ONLY && !pos_ok(xi2)) errs |= E_INEXACT;
+                const int ic2 = (int)xi2;
+                off2 = xi2 - (double)ic2;
+                key2 = imod(ic2, nx);
+            }
             double c0[W0];
             double c1[W1];
             if constexpr (FAST) bspline_pair_fast<P0, P1>(off2, c0, c1);
@@ -797,7 +810,7 @@ __device__ __forceinline__ int batch_instance(const int *start, int ninst, int b
 
 template <int P0, int P1, int MODE>
 #ifndef GEMPIC_PASS_MIN_WAVES
-#define GEMPIC_PASS_MIN_WAVES 1
+#define GEMPIC_PASS_MIN_WAVES 4
 #endif
 __global__ void __launch_bounds__(kBlock, GEMPIC_PASS_MIN_WAVES) k_pass1d(PassArgs a)
 {
