@@ -55,12 +55,11 @@ __device__ __forceinline__ double jl_mod(double x, double y)
     return r;
 }
 
-// floored integer modulus for n > 0; indices within one period of [0, n)
-// take a compare-and-add path instead of an integer division
+// floored integer modulus for n > 0; indices in [0, 2n) take a compare-and-
+// subtract path instead of an integer division
 __device__ __forceinline__ int imod(int a, int n)
 {
     int r = a >= n ? a - n : a;
-    r = r < 0 ? r + n : r;
     if ((unsigned)r >= (unsigned)n) {
         r = a % n;
         r = r < 0 ? r + n : r;

@@ -446,7 +446,9 @@ __device__ __forceinline__ PassChunk pass_chunk(const PassArgs &a, int bid)
     const int64_t left = a.n - c.base;
     c.cnt = left <= 0 ? 0 : left < chunk ? (int)left : (int)chunk;
     const int wave_first = (int)(threadIdx.x / kWave) * kWave;
-    c.iters = c.cnt > wave_first ? (c.cnt - wave_first + kBlock - 1) / kBlock : 0;
+    // the same in every lane of the wave: kept in a scalar register, so the
+    // loops over it branch on the scalar unit
+    c.iters = __builtin_amdgcn_readfirstlane(c.cnt > wave_first ? (c.cnt - wave_first + kBlock - 1) / kBlock : 0);
     return c;
 }
 
@@ -506,13 +508,14 @@ __device__ __forceinline__ void pass1d_body(const PassArgs &a, int bid, double *
     // software pipeline: the loads of iteration it + kPrefetch are issued
     // before the work of iteration it; the first ones before the fields are
     // staged, so that their latencies overlap
+    // (a lane past the chunk's end loads its last particle again: every lane
+    // loads, so the registers need no copies for the lanes that do not)
     auto load_particle = [&](int j, double &lx, double &lv1, double &lv2, double &lw) {
-        if (j < cnt_here) {
-            lx = elem(px, j);
-            lw = elem(pw, j);
-            if (!RHO_ONLY) lv1 = elem(pv1, j);
-            if (!V1 && !RHO_ONLY) lv2 = elem(pv2, j);
-        }
+        const int jj = j < cnt_here ? j : cnt_here - 1;
+        lx = elem(px, jj);
+        lw = elem(pw, jj);
+        if (!RHO_ONLY) lv1 = elem(pv1, jj);
+        if (!V1 && !RHO_ONLY) lv2 = elem(pv2, jj);
     };
     double nx_x = 0.0, nx_v1 = 0.0, nx_v2 = 0.0, nx_w = 0.0;     // iteration it + 1
     double n2_x = 0.0, n2_v1 = 0.0, n2_v2 = 0.0, n2_w = 0.0;     // it + 2 (kPrefetch 2)
@@ -537,7 +540,7 @@ __device__ __forceinline__ void pass1d_body(const PassArgs &a, int bid, double *
 
     const double xmin = a.xmin, dx = a.dx, qm = a.qm;
     const double rdx = 1.0 / dx;     // div_by: (x - xmin) / dx, correctly rounded
-    int errs = 0;
+    bool e_inexact = false, e_far = false;   // error flags of this lane's particles
 
     for (int it = 0; it < iters; ++it) {
         const int i = it * kBlock + tid;
@@ -560,7 +563,7 @@ __device__ __forceinline__ void pass1d_body(const PassArgs &a, int bid, double *
         const double q = (w * a.charge) * a.cw;
 
         double xi = div_by(x - xmin, dx, rdx);
-        if (valid && !pos_ok(xi)) errs |= E_INEXACT;
+        e_inexact |= valid && !pos_ok(xi);
         const int ic = (int)xi;                          // trunc(Int, xi)
         const double off = xi - (double)ic;
         const int key_old = imod(ic, nx);
@@ -638,7 +641,7 @@ __device__ __forceinline__ void pass1d_body(const PassArgs &a, int bid, double *
             // operatorHp1 (hamiltonian_splitting_1d2v.jl:53-82 / _1d1v.jl:308-329)
             const double xn = FAST ? __builtin_fma(a.dt_hp1, v1, x) : x + a.dt_hp1 * v1;
             const double xin = div_by(xn - xmin, dx, rdx);
-            if (valid && !pos_ok(xin)) errs |= E_INEXACT;
+            e_inexact |= valid && !pos_ok(xin);
             const int inw = V1 ? (int)floor(xin) : (int)xin;
             const double rn = xin - (double)inw;
             const int dseg = inw - ic;
@@ -646,7 +649,7 @@ __device__ __forceinline__ void pass1d_body(const PassArgs &a, int bid, double *
             // position) is reported instead of integrated segment by segment
             const bool far = !pos_ok(xin) || !pos_ok(xi) || dseg > kMaxCellsPerStep ||
                              dseg < -kMaxCellsPerStep;
-            if (valid && far) errs |= E_FAR;
+            e_far |= valid && far;
             if constexpr (FAST) {
                 // The path from xi to xin as signed integrals over its piece in
                 // each cell (the midpoint rule of segment_fast is odd in the
@@ -726,7 +729,7 @@ __device__ __forceinline__ void pass1d_body(const PassArgs &a, int bid, double *
                 key2 = __ballot(valid && inw_new != ic) == 0 ? key_old : imod(inw_new, nx);
             } else {
                 const double xi2 = xi_new_ok ? xi_new : div_by(x - xmin, dx, rdx);
-                if (valid && RHO_ONLY && !pos_ok(xi2)) errs |= E_INEXACT;
+                e_inexact |= valid && RHO_ONLY && !pos_ok(xi2);
                 const int ic2 = (int)xi2;
                 off2 = xi2 - (double)ic2;
                 key2 = imod(ic2, nx);
@@ -771,7 +774,7 @@ __device__ __forceinline__ void pass1d_body(const PassArgs &a, int bid, double *
                 a.cv2[base + i] = v2;
             }
     }
-    if (errs) atomicOr(a.err, errs);
+    if (e_inexact || e_far) atomicOr(a.err, (e_inexact ? (int)E_INEXACT : 0) | (e_far ? (int)E_FAR : 0));
 
     if constexpr (DEPMASK != 0u) {
         if constexpr (HP2A) cache_drain<W0>(c_j2a, myrow + D_J2A * nx, nx);
