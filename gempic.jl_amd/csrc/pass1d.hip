@@ -1604,10 +1604,10 @@ __global__ void __launch_bounds__(kBlock) k_boris1d(PassArgs a)
 
     double nx_x = 0.0, nx_v1 = 0.0, nx_v2 = 0.0, nx_w = 0.0;
     if (iters > 0 && tid < cnt_here) {
-        nx_x = px[tid];
-        nx_v1 = pv1[tid];
-        nx_v2 = pv2[tid];
-        if (PUSHX) nx_w = pw[tid];
+        nx_x = elem(px, tid);
+        nx_v1 = elem(pv1, tid);
+        nx_v2 = elem(pv2, tid);
+        if (PUSHX) nx_w = elem(pw, tid);
     }
     for (int it = 0; it < iters; ++it) {
         const int i = it * kBlock + tid;
@@ -1617,10 +1617,10 @@ __global__ void __launch_bounds__(kBlock) k_boris1d(PassArgs a)
         {
             const int in = i + kBlock;
             if (it + 1 < iters && in < cnt_here) {
-                nx_x = px[in];
-                nx_v1 = pv1[in];
-                nx_v2 = pv2[in];
-                if (PUSHX) nx_w = pw[in];
+                nx_x = elem(px, in);
+                nx_v1 = elem(pv1, in);
+                nx_v2 = elem(pv2, in);
+                if (PUSHX) nx_w = elem(pw, in);
             }
         }
         if constexpr (KICK) {
@@ -1686,10 +1686,10 @@ __global__ void __launch_bounds__(kBlock) k_boris1d(PassArgs a)
             x = jl_mod(xn, a.Lx);
         }
         if (valid) {
-            if constexpr (PUSHX) px[i] = x;
+            if constexpr (PUSHX) elem(px, i) = x;
             if constexpr (KICK) {
-                pv1[i] = v1;
-                pv2[i] = v2;
+                elem(pv1, i) = v1;
+                elem(pv2, i) = v2;
             }
         }
     }
@@ -2687,6 +2687,11 @@ __device__ __forceinline__ void adj1d_body(const AdjArgs &A, int bid, double *sm
     const int64_t base = pc.base;
     const int cnt_here = pc.cnt;
     const int iters = pc.iters;
+    // this workgroup's slices of the particle arrays (element access by elem())
+    const double *qx = A.x + base, *qw = a.w + base;
+    const double *qv1 = STEP ? A.v1 + base : nullptr, *qv2 = STEP ? A.v2 + base : nullptr;
+    double *qlx = A.lx + base;
+    double *qlv1 = DEP ? A.lv1 + base : nullptr, *qlv2 = DEP ? A.lv2 + base : nullptr;
 
     // the loss's explicit dependence on the cell size dx, on the domain
     // length through the mod, and on the particle weights (summed): per lane,
@@ -2699,34 +2704,33 @@ __device__ __forceinline__ void adj1d_body(const AdjArgs &A, int bid, double *sm
     // own iteration; the forward re-run covers their latency)
     double nx_x = xmin, nx_v1 = 0.0, nx_v2 = 0.0, nx_w = 0.0;
     if (iters > 0 && tid < cnt_here) {
-        nx_x = A.x[base + tid];
-        nx_w = a.w[base + tid];
+        nx_x = elem(qx, tid);
+        nx_w = elem(qw, tid);
         if constexpr (STEP) {
-            nx_v1 = A.v1[base + tid];
-            nx_v2 = A.v2[base + tid];
+            nx_v1 = elem(qv1, tid);
+            nx_v2 = elem(qv2, tid);
         }
     }
     for (int it = 0; it < iters; ++it) {
         const int i = it * kBlock + tid;
         const bool valid = i < cnt_here;
-        const int64_t gi = base + (valid ? i : 0);
         double x = nx_x, v1 = nx_v1, v2 = nx_v2, w = nx_w;
         {
             const int in = i + kBlock;
             if (it + 1 < iters && in < cnt_here) {
-                nx_x = A.x[base + in];
-                nx_w = a.w[base + in];
+                nx_x = elem(qx, in);
+                nx_w = elem(qw, in);
                 if constexpr (STEP) {
-                    nx_v1 = A.v1[base + in];
-                    nx_v2 = A.v2[base + in];
+                    nx_v1 = elem(qv1, in);
+                    nx_v2 = elem(qv2, in);
                 }
             }
         }
         double lx = 0.0, lv1 = 0.0, lv2 = 0.0;
-        if (valid && (DEP || POIS)) lx = A.lx[gi];
+        if (valid && (DEP || POIS)) lx = elem(qlx, i);
         if (valid && DEP) {
-            lv1 = A.lv1[gi];
-            lv2 = A.lv2[gi];
+            lv1 = elem(qlv1, i);
+            lv2 = elem(qlv2, i);
         }
         const double q = (w * a.charge) * a.cw;
         const double xi = div_by(x - xmin, dx, rdx);
@@ -2740,7 +2744,7 @@ __device__ __forceinline__ void adj1d_body(const AdjArgs &A, int bid, double *sm
             double db0[W0];
             dbspline<P0>(off, db0);
             const double dnu = div_by(gather_p<P0>(db0, sM1, key), dx, rdx);
-            if (valid) A.lx[gi] = q * s0 * dnu;
+            if (valid) elem(qlx, i) = q * s0 * dnu;
             continue;
         }
         if constexpr (HEONLY) {
@@ -2759,7 +2763,7 @@ __device__ __forceinline__ void adj1d_body(const AdjArgs &A, int bid, double *sm
             for (int k = 0; k < W0; ++k) vf[k] = (dtqm * lv2) * b0[k];
             cache_add<W1>(c_e1, valid, key, ve, myrow + AR_E1 * nx, nx);
             cache_add<W0>(c_e2, valid, key, vf, myrow + AR_E2 * nx, nx);
-            if (valid) A.lx[gi] = lx;
+            if (valid) elem(qlx, i) = lx;
             continue;
         }
 
@@ -2812,9 +2816,9 @@ __device__ __forceinline__ void adj1d_body(const AdjArgs &A, int bid, double *sm
         if constexpr (FWD) {
             v1 = v1 + g * v2 * bB;
             if (valid) {
-                A.xo[gi] = x1;
-                A.v1o[gi] = v1;
-                A.v2o[gi] = v2;
+                elem(A.xo + base, i) = x1;
+                elem(A.v1o + base, i) = v1;
+                elem(A.v2o + base, i) = v2;
             }
             continue;
         }
@@ -2978,9 +2982,9 @@ __device__ __forceinline__ void adj1d_body(const AdjArgs &A, int bid, double *sm
                 cache_add<W0>(c_e2, valid, key, vf, myrow + AR_E2 * nx, nx);
             }
             if (valid) {
-                A.lx[gi] = lx;
-                A.lv1[gi] = lv1;
-                A.lv2[gi] = lv2;
+                elem(qlx, i) = lx;
+                elem(qlv1, i) = lv1;
+                elem(qlv2, i) = lv2;
             }
         }
     }
