@@ -58,14 +58,23 @@ constexpr int s2_threads(int op)
     return (op == 1 || op == 2) ? GEMPIC_S2_THREADS_HP12 : (op == 3 ? GEMPIC_S2_THREADS_HP3 : 1024);
 }
 constexpr int kNn2Max = 4096;              // nx * ny of the device path (LDS grids)
+#ifndef GEMPIC_S2_GEN_SKEW
+#define GEMPIC_S2_GEN_SKEW 0.25
+#endif
+constexpr double kS2GenSkew = GEMPIC_S2_GEN_SKEW;   // chunk weights of a workgroup's waves (s2_args)
 constexpr int kSplits2 = 64;               // first-level groups of the workgroup-grid sum
 
 enum S2Op { S2_HE = 0, S2_HP1, S2_HP2, S2_HP3, S2_RHO };
 static_assert(S2_HP1 == 1 && S2_HP2 == 2 && S2_HP3 == 3, "s2_threads names the passes by value");
 
+constexpr int kS2MaxWaves = 16;            // waves of a 2D3V particle-pass workgroup, at most
+
 struct S2Args {
     int64_t n;
-    int64_t chunk;                  // particles per wave (multiple of 64)
+    int64_t chunk;                  // particles per wave on average (multiple of 64)
+    // wave w of a workgroup takes wch[w] particles from the workgroup's first
+    // (blockIdx.x * waves * chunk) plus woff[w] (both multiples of 64)
+    int64_t wch[kS2MaxWaves], woff[kS2MaxWaves];
     int nx, ny;
     double xmin, ymin, dx, dy, rdx, rdy, Lx, Ly;
     double charge, cw, qm, dt;
@@ -251,8 +260,9 @@ __global__ void __launch_bounds__(s2_threads(OP)) k_split2d(S2Args a)
     int errs = 0;
     const double qm = a.qm, dt = a.dt;
     constexpr int kW2 = s2_threads(OP) / kWave;
-    const int64_t wbase = ((int64_t)blockIdx.x * kW2 + wave) * a.chunk;
-    for (int64_t off = 0; off < a.chunk; off += kWave) {
+    const int64_t wbase = (int64_t)blockIdx.x * kW2 * a.chunk + a.woff[wave];
+    const int64_t wlen = a.wch[wave];
+    for (int64_t off = 0; off < wlen; off += kWave) {
         const int64_t i = wbase + off + lane;
         const bool valid = i < a.n;
         if (!__ballot(valid)) break;
@@ -659,9 +669,31 @@ S2Args s2_args(gempic_splitting2d *h, double dt, int op)
     a.w = h->d_p[5];
     a.part = h->d_part;
     a.err = h->d_err;
-    const int64_t waves = (int64_t)h->blocks * (s2_threads(op) / kWave);
+    const int kw = s2_threads(op) / kWave;
+    const int64_t waves = (int64_t)h->blocks * kw;
     a.chunk = ((h->n + waves - 1) / waves + kWave - 1) / kWave * kWave;
     if (a.chunk < kWave) a.chunk = kWave;
+    // The waves of a workgroup share its CU's 4 SIMDs, kw / 4 to a SIMD, and
+    // a SIMD issues from its oldest wave first: with equal chunks the later
+    // waves of each SIMD run the end of the pass with the SIMD to themselves.
+    // Wave w is generation g = w / 4 on its SIMD; chunks weighted 1 + skew
+    // (G - 1 - 2g) / (G - 1) let the generations end together (the
+    // workgroup's total stays kw * chunk).
+    const int G = kw / 4;
+    const int64_t per = a.chunk / kWave;              // waves of 64 particles per wave
+    int64_t acc = 0;
+    for (int w = 0; w < kw; ++w) {
+        int64_t c = per;
+        if (G >= 2 && kS2GenSkew > 0.0) {
+            const int g = w / 4;
+            const double wt = 1.0 + kS2GenSkew * (double)(G - 1 - 2 * g) / (double)(G - 1);
+            c = w + 1 < kw ? std::max<int64_t>(0, std::llround((double)per * wt)) : per * kw - acc;
+            if (c < 0) c = 0;
+        }
+        a.woff[w] = acc * kWave;
+        a.wch[w] = c * kWave;
+        acc += c;
+    }
     return a;
 }
 
