@@ -2028,15 +2028,26 @@ __device__ __forceinline__ void sort_gsum_body(const SortArgs &a, int g, int by)
 }
 
 // One workgroup: bin totals, their exclusive scan (the bin bases), and
-// gsum[g][b] <- base[b] + sum of gsum[g'][b] over g' < g.
+// gsum[g][b] <- base[b] + sum of gsum[g'][b] over g' < g. Each thread walks
+// its bin's column of group sums twice, kBasesBatch loads in flight.
+constexpr int kBasesBatch = 32;
 __device__ __forceinline__ void sort_bases_body(const SortArgs &a, int *s_scan)
 {
     const int nb = a.nbins;
     const int b = threadIdx.x;
     int *col = a.gsum + b;
     int total = 0;
-    if (b < nb)
-        for (int g = 0; g < a.ngroups; ++g) total += col[(size_t)g * nb];
+    if (b < nb) {
+        int g = 0;
+        for (; g + kBasesBatch <= a.ngroups; g += kBasesBatch) {
+            int v[kBasesBatch];
+#pragma unroll
+            for (int u = 0; u < kBasesBatch; ++u) v[u] = col[(size_t)(g + u) * nb];
+#pragma unroll
+            for (int u = 0; u < kBasesBatch; ++u) total += v[u];
+        }
+        for (; g < a.ngroups; ++g) total += col[(size_t)g * nb];
+    }
     s_scan[b] = b < nb ? total : 0;
     __syncthreads();
     for (int off = 1; off < kSortBinsMax; off <<= 1) {
@@ -2048,12 +2059,12 @@ __device__ __forceinline__ void sort_bases_body(const SortArgs &a, int *s_scan)
     if (b >= nb) return;
     int run = s_scan[b] - total;
     int g = 0;
-    for (; g + kSortBatch <= a.ngroups; g += kSortBatch) {
-        int v[kSortBatch];
+    for (; g + kBasesBatch <= a.ngroups; g += kBasesBatch) {
+        int v[kBasesBatch];
 #pragma unroll
-        for (int u = 0; u < kSortBatch; ++u) v[u] = col[(size_t)(g + u) * nb];
+        for (int u = 0; u < kBasesBatch; ++u) v[u] = col[(size_t)(g + u) * nb];
 #pragma unroll
-        for (int u = 0; u < kSortBatch; ++u) {
+        for (int u = 0; u < kBasesBatch; ++u) {
             col[(size_t)(g + u) * nb] = run;
             run += v[u];
         }
