@@ -16,13 +16,15 @@
 // the patch its particles fall in; when a lane's next contribution starts
 // another patch, the wave flushes (DPP sums into one patch when many
 // flushing lanes hold the same one, LDS atomics otherwise). The workgroup
-// grids are summed over workgroups in a fixed order. HE and Hp3 follow the
-// oracle's operations in its order; Hp1 and Hp2 take the path integrals of
-// the degree p-1 splines from their antiderivative (suffix sums of the
-// degree-p values at the two ends) where the oracle integrates piece by
-// piece, the same integrals rounded differently. The deposit's additions
-// happen in another order (and, between lanes that flush different patches,
-// in a timing-dependent one).
+// grids are summed over workgroups in a fixed order. The shape factors are
+// evaluated in closed form and the gathers and deposits summed with fused
+// multiply-adds (the oracle: de Boor, every product and sum rounded); Hp1
+// and Hp2 take the path integrals of the degree p-1 splines from their
+// antiderivative (suffix sums of the degree-p values at the two ends) where
+// the oracle integrates piece by piece: the same quantities, rounded
+// differently; the positions follow the oracle's operations exactly. The
+// deposit's additions happen in another order (and, between lanes that
+// flush different patches, in a timing-dependent one).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -96,7 +98,7 @@ __device__ __forceinline__ bool shape_dir(double pos, double lo, double d, doubl
     const bool ok = fabs(xp) < 1.0e9;
     const int ip = ok ? (int)ceil(xp) : 1;
     const double off = ok ? xp - (double)(ip - 1) : 1.0;
-    bspline_pair<P, P - 1>(off, b0, b1);
+    bspline_pair_fast<P, P - 1>(off, b0, b1);
     cell = imod(ip - 1, n);
     return ok;
 }
@@ -122,12 +124,16 @@ template <int P, int DX, int DY>
 __device__ __forceinline__ double gather2(const double *fs, int W, int cx, int cy,
                                           const double *bx, const double *by)
 {
+    // sum_c by[c] (sum_a f[c][a] bx[a]), with fused multiply-adds
     const double *base = fs + (cy - DY + P) * W + (cx - DX + P);
     double v = 0.0;
 #pragma unroll
-    for (int a = 0; a <= DX; ++a)
+    for (int c = 0; c <= DY; ++c) {
+        double r = 0.0;
 #pragma unroll
-        for (int c = 0; c <= DY; ++c) v += base[c * W + a] * bx[a] * by[c];
+        for (int a = 0; a <= DX; ++a) r = __builtin_fma(base[c * W + a], bx[a], r);
+        v = __builtin_fma(r, by[c], v);
+    }
     return v;
 }
 
@@ -284,25 +290,29 @@ __global__ void __launch_bounds__(s2_threads(OP)) k_split2d(S2Args a)
             const double e1 = gather2<P, P - 1, P>(fs0, W, cx, cy, bx1, by0);
             const double e2 = gather2<P, P, P - 1>(fs1, W, cx, cy, bx0, by1);
             const double e3 = gather2<P, P, P>(fs2, W, cx, cy, bx0, by0);
-            v1 = v1 + dt * qm * e1;
-            v2 = v2 + dt * qm * e2;
-            v3 = v3 + dt * qm * e3;
+            const double dtqm = dt * qm;
+            v1 = __builtin_fma(dtqm, e1, v1);
+            v2 = __builtin_fma(dtqm, e2, v2);
+            v3 = __builtin_fma(dtqm, e3, v3);
         } else if constexpr (OP == S2_HP3 || OP == S2_RHO) {
             double wq = q;
             if constexpr (OP == S2_HP3) {
                 const double b1 = gather2<P, P, P - 1>(fs0, W, cx, cy, bx0, by1);
                 const double b2 = gather2<P, P - 1, P>(fs1, W, cx, cy, bx1, by0);
-                v1 = v1 - dt * qm * v3 * b2;
-                v2 = v2 + dt * qm * v3 * b1;
+                const double dv = (dt * qm) * v3;
+                v1 = __builtin_fma(-dv, b2, v1);
+                v2 = __builtin_fma(dv, b1, v2);
                 wq = q * v3;
             }
             patch_begin<WX, WY>(pc, valid, imod(cx - P, nx) + imod(cy - P, ny) * nx, grid, stage,
                                 nx, ny);
             if (valid) {
 #pragma unroll
-                for (int aa = 0; aa <= P; ++aa)
+                for (int aa = 0; aa <= P; ++aa) {
+                    const double wb = wq * bx0[aa];
 #pragma unroll
-                    for (int c = 0; c <= P; ++c) pc.acc[aa * WY + c] += wq * bx0[aa] * by0[c];
+                    for (int c = 0; c <= P; ++c) pc.acc[aa * WY + c] = __builtin_fma(wb, by0[c], pc.acc[aa * WY + c]);
+                }
             }
         } else {
             // Hp1 moves x1 (fields fs0 = B3, fs1 = B2), Hp2 moves x2 (fs0 = B3, fs1 = B1)
@@ -331,8 +341,8 @@ __global__ void __launch_bounds__(s2_threads(OP)) k_split2d(S2Args a)
             double sa[P + 1], sb[P + 1];        // suffix sums sa[j] = sum_{i >= j} ia[i]
             {
                 double ia[P + 1], ib[P + 1];
-                bspline<P>(xa - fa, ia);
-                bspline<P>(xb - fb, ib);
+                bspline_fast<P>(xa - fa, ia);
+                bspline_fast<P>(xb - fb, ib);
                 sa[P] = ia[P];
                 sb[P] = ib[P];
 #pragma unroll
@@ -368,36 +378,54 @@ __global__ void __launch_bounds__(s2_threads(OP)) k_split2d(S2Args a)
                     patch_begin<WX, WY>(pc, act, imod(k0, nx) + imod(cy - P, ny) * nx, grid, stage,
                                         nx, ny);
                     if (act) {
+                        // per node r of the path: the deposit q val[r] S^p(x2), and
+                        // the B2 and B3 gathers along x2 weighted by val[r]
                         const double *b2 = fs1 + (cy - P + P) * W + (k0 + P);
                         const double *b3 = fs0 + (cy - (P - 1) + P) * W + (k0 + P);
+                        double s2 = 0.0, s3 = 0.0;
 #pragma unroll
                         for (int r = 0; r <= P; ++r) {
+                            const double qv = q * val[r];
+                            double g2 = 0.0, g3 = 0.0;
 #pragma unroll
                             for (int cc = 0; cc <= P; ++cc) {
-                                pc.acc[r * WY + cc] += q * val[r] * by0[cc];
-                                t3 = t3 + qm * b2[cc * W + r] * val[r] * by0[cc];
+                                pc.acc[r * WY + cc] = __builtin_fma(qv, by0[cc], pc.acc[r * WY + cc]);
+                                g2 = __builtin_fma(b2[cc * W + r], by0[cc], g2);
                             }
 #pragma unroll
-                            for (int cc = 0; cc < P; ++cc) u = u - qm * b3[cc * W + r] * val[r] * by1[cc];
+                            for (int cc = 0; cc < P; ++cc) g3 = __builtin_fma(b3[cc * W + r], by1[cc], g3);
+                            s2 = __builtin_fma(val[r], g2, s2);
+                            s3 = __builtin_fma(val[r], g3, s3);
                         }
+                        t3 = __builtin_fma(qm, s2, t3);
+                        u = __builtin_fma(-qm, s3, u);
                     }
                 } else {
                     // j2 += q S^p(x1) (dy int S^{p-1}); v3 -= qm int B1 ..; v1 += qm int B3 ..
                     patch_begin<WX, WY>(pc, act, imod(cx - P, nx) + imod(k0, ny) * nx, grid, stage,
                                         nx, ny);
                     if (act) {
+                        // per node r of the path: the deposit q S^p(x1) val[r], and
+                        // the B1 and B3 gathers along x1 weighted by val[r]
                         const double *b1 = fs1 + (k0 + P) * W + (cx - P + P);
                         const double *b3 = fs0 + (k0 + P) * W + (cx - (P - 1) + P);
+                        double s1 = 0.0, s3 = 0.0;
 #pragma unroll
                         for (int r = 0; r <= P; ++r) {
+                            const double qv = q * val[r];
+                            double g1 = 0.0, g3 = 0.0;
 #pragma unroll
                             for (int aa = 0; aa <= P; ++aa) {
-                                pc.acc[aa * WY + r] += q * bx0[aa] * val[r];
-                                t3 = t3 - qm * b1[r * W + aa] * bx0[aa] * val[r];
+                                pc.acc[aa * WY + r] = __builtin_fma(qv, bx0[aa], pc.acc[aa * WY + r]);
+                                g1 = __builtin_fma(b1[r * W + aa], bx0[aa], g1);
                             }
 #pragma unroll
-                            for (int aa = 0; aa < P; ++aa) u = u + qm * b3[r * W + aa] * bx1[aa] * val[r];
+                            for (int aa = 0; aa < P; ++aa) g3 = __builtin_fma(b3[r * W + aa], bx1[aa], g3);
+                            s1 = __builtin_fma(val[r], g1, s1);
+                            s3 = __builtin_fma(val[r], g3, s3);
                         }
+                        t3 = __builtin_fma(-qm, s1, t3);
+                        u = __builtin_fma(qm, s3, u);
                     }
                 }
             }

@@ -3,9 +3,12 @@ split2d.hip) through the C ABI, against the oracle's 2D3V splitting
 (oracle/gempic_oracle2d3v.c, pinned by Gauss's law, the second-order energy
 error and the reduction to the reference's 1D2V splitting).
 
-Tolerances. HE and Hp3 evaluate each particle's push in the oracle's
-operations and order, and read only fields, so from the same input their
-particle rows are bitwise the oracle's. Hp1 and Hp2 integrate the degree
+Tolerances. Hp1 and Hp2 move the positions with the oracle's operations,
+so from the same input the positions are bitwise the oracle's. The kicks
+evaluate the shape factors in closed form and sum the gathers with fused
+multiply-adds (the oracle: the de Boor recurrence, products and sums
+rounded one by one), so they agree to rounding (1e-12 of each row's
+largest value). Hp1 and Hp2 also integrate the degree
 p-1 splines along the path through their antiderivative (differences of
 suffix sums of the degree-p values at the two ends) where the oracle
 integrates piece by piece with Gauss-Legendre: the same integrals, rounded
