@@ -44,20 +44,27 @@ using namespace gk;
 namespace {
 
 // threads per workgroup of a particle pass: 1024 (4 waves per SIMD, at most
-// 128 VGPRs per lane) or 512 (2 waves per SIMD, up to 256 VGPRs). The
-// passes with a deposit and B gathers hold the lane's patch cache, the
-// shape factors and the staged-field reads at once, close to 128 registers.
-// The LDS of a workgroup (staged fields and the deposit grid, above 80 KB)
-// allows one workgroup per CU either way.
-#ifndef GEMPIC_S2_THREADS_HP12
-#define GEMPIC_S2_THREADS_HP12 1024
+// 128 VGPRs per lane), 768 (3 waves per SIMD, up to 168) or 512 (2 waves
+// per SIMD, up to 256). The passes with a deposit and B gathers hold the
+// lane's patch cache, the shape factors and the staged-field reads at once:
+// Hp2 fits 127 registers, Hp1 needs 160 (at 1024 threads it spilled 28
+// registers to scratch and took 634 us a pass on 16,777,216 particles,
+// the step 3.33 ms; at 768 threads the step takes 3.13 ms) and Hp3 136 (at
+// 768 threads the step takes 3.03 ms, at 512 3.13). The LDS of a
+// workgroup (staged fields and the deposit grid, above 80 KB) allows one
+// workgroup per CU in every case.
+#ifndef GEMPIC_S2_THREADS_HP1
+#define GEMPIC_S2_THREADS_HP1 768
+#endif
+#ifndef GEMPIC_S2_THREADS_HP2
+#define GEMPIC_S2_THREADS_HP2 1024
 #endif
 #ifndef GEMPIC_S2_THREADS_HP3
-#define GEMPIC_S2_THREADS_HP3 512
+#define GEMPIC_S2_THREADS_HP3 768
 #endif
 constexpr int s2_threads(int op)
 {
-    return (op == 1 || op == 2) ? GEMPIC_S2_THREADS_HP12 : (op == 3 ? GEMPIC_S2_THREADS_HP3 : 1024);
+    return op == 1 ? GEMPIC_S2_THREADS_HP1 : op == 2 ? GEMPIC_S2_THREADS_HP2 : (op == 3 ? GEMPIC_S2_THREADS_HP3 : 1024);
 }
 constexpr int kNn2Max = 4096;              // nx * ny of the device path (LDS grids)
 #ifndef GEMPIC_S2_GEN_SKEW
@@ -70,6 +77,11 @@ enum S2Op { S2_HE = 0, S2_HP1, S2_HP2, S2_HP3, S2_RHO };
 static_assert(S2_HP1 == 1 && S2_HP2 == 2 && S2_HP3 == 3, "s2_threads names the passes by value");
 
 constexpr int kS2MaxWaves = 16;            // waves of a 2D3V particle-pass workgroup, at most
+static_assert(s2_threads(S2_HE) <= kS2MaxWaves * 64 && s2_threads(S2_HP1) <= kS2MaxWaves * 64 &&
+                  s2_threads(S2_HP2) <= kS2MaxWaves * 64 && s2_threads(S2_HP3) <= kS2MaxWaves * 64 &&
+                  s2_threads(S2_HP1) % 256 == 0 && s2_threads(S2_HP2) % 256 == 0 &&
+                  s2_threads(S2_HP3) % 256 == 0,
+              "a particle pass runs whole waves on each of the 4 SIMDs, at most kS2MaxWaves");
 
 struct S2Args {
     int64_t n;
