@@ -608,6 +608,179 @@ __global__ void k_div(Field2 f, const double *w1, const double *w2, double *rho)
     }
 }
 
+// ---- fused field updates of the 2D3V splitting (grids of at most 4096
+// nodes): one workgroup holds a component's grid in LDS, rows padded to
+// nx + 1 doubles so that lanes on consecutive rows hit different banks.
+
+struct FieldTabs {
+    const double *l0x, *l1x, *l0y, *l1y;   // mass lines of degree p and p - 1 (times dx, dy)
+    const double *k0x, *k1x, *k0y, *k1y;   // inverse-mass convolution kernels (host_inv_mass_kernel)
+    int degree;
+};
+
+// k_mass<DIR_X> on a grid read through in(i, j), written to LDS (row
+// stride nx + 1), with k_mass's arithmetic
+template <bool DIR_X, class In>
+__device__ void mass_line(In in, double *out, const double *line, int deg, int nx, int ny)
+{
+    const int n = DIR_X ? nx : ny;
+    for (int k = threadIdx.x; k < nx * ny; k += blockDim.x) {
+        const int i = k % nx, j = k / nx;
+        const int r = DIR_X ? i : j;
+        double o = line[0] * (DIR_X ? in(r, j) : in(i, r));
+        for (int c = 1; c <= deg; ++c) {
+            int rp = r + c, rm = r - c;
+            if (rp >= n) rp -= n;
+            if (rm < 0) rm += n;
+            o += line[c] * (DIR_X ? in(rp, j) + in(rm, j) : in(i, rp) + in(i, rm));
+        }
+        out[i + j * (nx + 1)] = o;
+    }
+}
+
+// Circular convolution of a grid in LDS (row stride nx + 1) along x
+// (DIR_X) or y: out(i, j) = sum_m k[m] in(i - m, j), resp. in(i, j - m),
+// indices mod n. A work item takes the R outputs q + S r (S = n / R) of
+// one line: input q - t (mod n) feeds output q + S r through k[t + S r]
+// (mod n), so one pass over the line's n inputs serves all R. k2 is the
+// kernel twice over in LDS (k2[m] = k[m mod n] for m < 2n): its reads are
+// broadcasts, four consecutive values at a time. Lanes take consecutive
+// lines (rows for x, columns for y).
+template <bool DIR_X, int R, class Out>
+__device__ void circ_conv_r(const double *in, const double *k2, int nx, int ny, Out out)
+{
+    const int P = nx + 1;
+    const int n = DIR_X ? nx : ny, S = n / R, lines = DIR_X ? ny : nx;
+    const int st = DIR_X ? 1 : P;
+    for (int w = threadIdx.x; w < lines * S; w += blockDim.x) {
+        const int line = w % lines, q = w / lines;
+        const double *src = DIR_X ? in + line * P : in + line;
+        double acc[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[r] = 0.0;
+        int u = q, t = 0;
+        for (; t + 4 <= n; t += 4) {
+            double v[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                v[e] = src[u * st];
+                u = u == 0 ? n - 1 : u - 1;
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const double *kr = k2 + t + S * r;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) acc[r] = __builtin_fma(kr[e], v[e], acc[r]);
+            }
+        }
+        for (; t < n; ++t) {
+            const double v = src[u * st];
+            u = u == 0 ? n - 1 : u - 1;
+#pragma unroll
+            for (int r = 0; r < R; ++r) acc[r] = __builtin_fma(k2[t + S * r], v, acc[r]);
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (DIR_X) out(q + S * r, line, acc[r]);
+            else out(line, q + S * r, acc[r]);
+        }
+    }
+}
+
+template <bool DIR_X, class Out>
+__device__ void circ_conv(const double *in, const double *k2, int nx, int ny, Out out)
+{
+    const int n = DIR_X ? nx : ny;
+    if (n % 4 == 0) circ_conv_r<DIR_X, 4>(in, k2, nx, ny, out);
+    else if (n % 2 == 0) circ_conv_r<DIR_X, 2>(in, k2, nx, ny, out);
+    else circ_conv_r<DIR_X, 1>(in, k2, nx, ny, out);
+}
+
+// the inverse-mass kernels kx, ky twice over into LDS (kx2: 2 nx values, ky2: 2 ny)
+__device__ void stage_kernels(const double *kx, const double *ky, int nx, int ny, double *kx2, double *ky2)
+{
+    for (int m = threadIdx.x; m < 2 * nx; m += blockDim.x) kx2[m] = kx[m < nx ? m : m - nx];
+    for (int m = threadIdx.x; m < 2 * ny; m += blockDim.x) ky2[m] = ky[m < ny ? m : m - ny];
+}
+
+// compute_e_from_b! (maxwell_2d_fem.jl:370-411) for the 2D3V splitting:
+// workgroup c updates e_{c+1} += dt M1^-1 (curl M2 b)_{c+1}, with the mass
+// products and the curl of k_mass and k_curl_b and the inverse mass as the
+// circulant convolutions x then y (the DFT solve's matrix, in closed form)
+__global__ void __launch_bounds__(1024) k_fields_e_from_b(Field2 f, FieldTabs t, double dt,
+                                                          const double *b1, const double *b2,
+                                                          const double *b3, double *e1, double *e2,
+                                                          double *e3)
+{
+    extern __shared__ __align__(16) double lds[];
+    const int nx = f.nx, ny = f.ny, P = nx + 1, nn = nx * ny;
+    double *A = lds, *B = lds + (size_t)P * ny, *C = lds + 2 * (size_t)P * ny;
+    double *kx2 = lds + 3 * (size_t)P * ny, *ky2 = kx2 + 2 * nx;
+    const int c = blockIdx.x;
+    // the inverse mass of the 1-form's component c + 1 (degree p - 1 along
+    // its own direction)
+    stage_kernels(c == 0 ? t.k1x : t.k0x, c == 1 ? t.k1y : t.k0y, nx, ny, kx2, ky2);
+    auto glob = [nx](const double *g) { return [=](int i, int j) { return g[i + j * nx]; }; };
+    auto loc = [P](const double *a) { return [=](int i, int j) { return a[i + j * P]; }; };
+    if (c == 2) {
+        // u1 = M(x: p, y: p - 1) b1, u2 = M(x: p - 1, y: p) b2; c3 = d_y u1 - d_x u2
+        mass_line<true>(glob(b1), A, t.l0x, t.degree, nx, ny);
+        __syncthreads();
+        mass_line<false>(loc(A), B, t.l1y, t.degree - 1, nx, ny);
+        __syncthreads();
+        mass_line<true>(glob(b2), A, t.l1x, t.degree - 1, nx, ny);
+        __syncthreads();
+        mass_line<false>(loc(A), C, t.l0y, t.degree, nx, ny);
+        __syncthreads();
+        for (int k = threadIdx.x; k < nn; k += blockDim.x) {
+            const int i = k % nx, j = k / nx;
+            const int i1 = i + 1 == nx ? 0 : i + 1, j1 = j + 1 == ny ? 0 : j + 1;
+            A[i + j * P] = (B[i + j * P] - B[i + j1 * P]) / f.dy - (C[i + j * P] - C[i1 + j * P]) / f.dx;
+        }
+    } else {
+        // u3 = M(x: p - 1, y: p - 1) b3; c1 = -d_y u3, c2 = d_x u3
+        mass_line<true>(glob(b3), A, t.l1x, t.degree - 1, nx, ny);
+        __syncthreads();
+        mass_line<false>(loc(A), B, t.l1y, t.degree - 1, nx, ny);
+        __syncthreads();
+        for (int k = threadIdx.x; k < nn; k += blockDim.x) {
+            const int i = k % nx, j = k / nx;
+            const int i1 = i + 1 == nx ? 0 : i + 1, j1 = j + 1 == ny ? 0 : j + 1;
+            A[i + j * P] = c == 0 ? -(B[i + j * P] - B[i + j1 * P]) / f.dy
+                                  : (B[i + j * P] - B[i1 + j * P]) / f.dx;
+        }
+    }
+    __syncthreads();
+    // w = M1^-1 c, then e += dt w as k_axpy
+    double *e = c == 0 ? e1 : (c == 1 ? e2 : e3);
+    circ_conv<true>(A, kx2, nx, ny, [=](int i, int j, double v) { B[i + j * P] = v; });
+    __syncthreads();
+    circ_conv<false>(B, ky2, nx, ny, [=](int i, int j, double v) {
+        const int k = i + j * nx;
+        e[k] = e[k] + dt * v;
+    });
+}
+
+// compute_e_from_j! (maxwell_2d_fem.jl:455-459): e -= M1^-1 j for the
+// 1-form's component with inverse-mass kernels kx, ky, one workgroup
+__global__ void __launch_bounds__(1024) k_fields_e_from_j(Field2 f, const double *kx, const double *ky,
+                                                          const double *j, double *e)
+{
+    extern __shared__ __align__(16) double lds[];
+    const int nx = f.nx, ny = f.ny, P = nx + 1, nn = nx * ny;
+    double *A = lds, *B = lds + (size_t)P * ny;
+    double *kx2 = lds + 2 * (size_t)P * ny, *ky2 = kx2 + 2 * nx;
+    stage_kernels(kx, ky, nx, ny, kx2, ky2);
+    for (int k = threadIdx.x; k < nn; k += blockDim.x) A[k % nx + (k / nx) * P] = j[k];
+    __syncthreads();
+    circ_conv<true>(A, kx2, nx, ny, [=](int i, int jj, double v) { B[i + jj * P] = v; });
+    __syncthreads();
+    circ_conv<false>(B, ky2, nx, ny, [=](int i, int jj, double v) {
+        const int k = i + jj * nx;
+        e[k] = e[k] - v;
+    });
+}
+
 struct PoissonTabs {
     const double *eig0x, *eig0y, *dtm1x, *dtm1y, *d1re, *d1im, *d2re, *d2im;
 };
@@ -709,6 +882,20 @@ void host_mass_eig(int64_t n, int degree, const double *line, double *eig)
     }
 }
 
+// The inverse of a circulant mass matrix with eigenvalues eig[k] (its DFT
+// symbol, real and even: eig[k] = eig[n - k]) is the circulant matrix of
+// kinv[m] = (1/n) sum_k cos(2 pi k m / n) / eig[k], in long double.
+void host_inv_mass_kernel(int64_t n, const double *eig, double *kinv)
+{
+    const long double twopi = 6.283185307179586476925286766559005768L;
+    for (int64_t m = 0; m < n; ++m) {
+        long double t = 0.0L;
+        for (int64_t k = 0; k < n; ++k)
+            t += cosl(twopi * (long double)((k * m) % n) / (long double)n) / (long double)eig[k];
+        kinv[m] = (double)(t / (long double)n);
+    }
+}
+
 void host_twiddles(int64_t n, double *c, double *s)
 {
     for (int64_t i = 0; i < n; ++i) {
@@ -731,7 +918,8 @@ unsigned grid_blocks(int64_t nn) { return (unsigned)std::min<int64_t>(1024, (nn 
 // table slots of a TwoDMaxwell handle (each of length kNxMax)
 enum Tab2 {
     T2_CX = 0, T2_SX, T2_CY, T2_SY, T2_E0X, T2_E1X, T2_E0Y, T2_E1Y,
-    T2_DTMX, T2_DTMY, T2_D1R, T2_D1I, T2_D2R, T2_D2I, T2_L0X, T2_L1X, T2_L0Y, T2_L1Y, NT2
+    T2_DTMX, T2_DTMY, T2_D1R, T2_D1I, T2_D2R, T2_D2I, T2_L0X, T2_L1X, T2_L0Y, T2_L1Y,
+    T2_K0X, T2_K1X, T2_K0Y, T2_K1Y, NT2
 };
 
 struct gempic_maxwell2d {
@@ -741,6 +929,7 @@ struct gempic_maxwell2d {
     double *d_tab = nullptr;            // NT2 x kNxMax
     double *d_buf = nullptr;            // 16 work arrays of nx*ny
     double *d_scalar = nullptr;
+    bool fused = false;                 // the one-workgroup field kernels fit (k_fields_*)
     const double *tab(int s) const { return d_tab + (size_t)s * kNxMax; }
     double *buf(int i) const { return d_buf + (size_t)i * nx * ny; }
     Field2 f() const { return Field2{nx, ny, dx, dy}; }
@@ -805,6 +994,37 @@ void form_lines(int form, int component, bool &onex, bool &oney)
     else if (form == 3) { onex = oney = true; }
 }
 
+// LDS of a one-workgroup field kernel: `grids` grids with rows of nx + 1
+// and the two inverse-mass kernels twice over
+size_t fused_lds(const gempic_maxwell2d *m, int grids)
+{
+    return sizeof(double) * ((size_t)grids * (size_t)(m->nx + 1) * (size_t)m->ny + 2 * (size_t)(m->nx + m->ny));
+}
+
+FieldTabs field_tabs(const gempic_maxwell2d *m)
+{
+    return FieldTabs{m->tab(T2_L0X), m->tab(T2_L1X), m->tab(T2_L0Y), m->tab(T2_L1Y),
+                     m->tab(T2_K0X), m->tab(T2_K1X), m->tab(T2_K0Y), m->tab(T2_K1Y), m->degree};
+}
+
+// the one-workgroup field kernels take grids of at most kFusedNodes nodes
+// (three padded grids in a workgroup's LDS)
+constexpr int64_t kFusedNodes = 4096;
+
+bool fused_setup(gempic_maxwell2d *m)
+{
+    if ((int64_t)m->nx * m->ny > kFusedNodes) return false;
+    const int sh3 = (int)fused_lds(m, 3), sh2 = (int)fused_lds(m, 2);
+    if (hipFuncSetAttribute((const void *)k_fields_e_from_b, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            sh3) != hipSuccess ||
+        hipFuncSetAttribute((const void *)k_fields_e_from_j, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            sh2) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return true;
+}
+
 int finish2(gempic_maxwell2d *m)
 {
     hipError_t e = hipGetLastError();
@@ -824,6 +1044,12 @@ hipStream_t gl::m2d_stream(gempic_maxwell2d *m) { return m->stream; }
 void gl::m2d_e_from_b(gempic_maxwell2d *m, double dt, double *const *b, double *const *e)
 {
     const Field2 f = m->f();
+    if (m->fused) {
+        hipLaunchKernelGGL(k_fields_e_from_b, dim3(3), dim3(1024), fused_lds(m, 3), m->stream, f,
+                           field_tabs(m), dt, (const double *)b[0], (const double *)b[1],
+                           (const double *)b[2], e[0], e[1], e[2]);
+        return;
+    }
     const unsigned gb = grid_blocks((int64_t)m->nx * m->ny);
     mass2(m, false, true, b[0], m->buf(6));
     mass2(m, true, false, b[1], m->buf(7));
@@ -845,6 +1071,12 @@ void gl::m2d_b_from_e(gempic_maxwell2d *m, double dt, double *const *e, double *
 
 void gl::m2d_e_from_j(gempic_maxwell2d *m, const double *j, int component, double *e)
 {
+    if (m->fused) {
+        const FieldTabs t = field_tabs(m);
+        hipLaunchKernelGGL(k_fields_e_from_j, dim3(1), dim3(1024), fused_lds(m, 2), m->stream, m->f(),
+                           component == 1 ? t.k1x : t.k0x, component == 2 ? t.k1y : t.k0y, j, e);
+        return;
+    }
     const unsigned gb = grid_blocks((int64_t)m->nx * m->ny);
     inv_mass(m, 1, component, j, m->buf(2));
     hipLaunchKernelGGL(k_sub, dim3(gb), dim3(256), 0, m->stream, m->f(), e, m->buf(2));
@@ -907,6 +1139,10 @@ int gempic_maxwell2d_create(int64_t nx, double xmin, double xmax, int64_t ny, do
     host_mass_eig(ny, degree, slot(T2_L0Y), slot(T2_E0Y));
     host_mass_eig(nx, degree - 1, slot(T2_L1X), slot(T2_E1X));
     host_mass_eig(ny, degree - 1, slot(T2_L1Y), slot(T2_E1Y));
+    host_inv_mass_kernel(nx, slot(T2_E0X), slot(T2_K0X));
+    host_inv_mass_kernel(nx, slot(T2_E1X), slot(T2_K1X));
+    host_inv_mass_kernel(ny, slot(T2_E0Y), slot(T2_K0Y));
+    host_inv_mass_kernel(ny, slot(T2_E1Y), slot(T2_K1Y));
     // TwoDPoisson (poisson_2d_fem.jl:48-64)
     const double twopi = 6.283185307179586;
     for (int64_t j = 2; j <= nx; ++j) {
@@ -927,6 +1163,7 @@ int gempic_maxwell2d_create(int64_t nx, double xmin, double xmax, int64_t ny, do
     if (e == hipSuccess) e = dalloc2(&m->d_scalar, 8);
     if (e == hipSuccess)
         e = hipMemcpy(m->d_tab, T.data(), sizeof(double) * T.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) m->fused = fused_setup(m);
     if (e != hipSuccess) {
         gempic_maxwell2d_destroy(m);
         return abi_fail(GEMPIC_ERR_DEVICE, (std::string("TwoDMaxwell allocation: ") + hipGetErrorString(e)).c_str());

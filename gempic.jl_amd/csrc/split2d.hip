@@ -71,7 +71,10 @@ constexpr int kNn2Max = 4096;              // nx * ny of the device path (LDS gr
 #define GEMPIC_S2_GEN_SKEW 0.25
 #endif
 constexpr double kS2GenSkew = GEMPIC_S2_GEN_SKEW;   // chunk weights of a workgroup's waves (s2_args)
-constexpr int kSplits2 = 64;               // first-level groups of the workgroup-grid sum
+#ifndef GEMPIC_S2_SPLITS
+#define GEMPIC_S2_SPLITS 64
+#endif
+constexpr int kSplits2 = GEMPIC_S2_SPLITS;  // first-level groups of the workgroup-grid sum
 
 enum S2Op { S2_HE = 0, S2_HP1, S2_HP2, S2_HP3, S2_RHO };
 static_assert(S2_HP1 == 1 && S2_HP2 == 2 && S2_HP3 == 3, "s2_threads names the passes by value");
