@@ -305,5 +305,10 @@ hipStream_t m2d_stream(gempic_maxwell2d *m);
 void m2d_e_from_b(gempic_maxwell2d *m, double dt, double *const *b, double *const *e);
 void m2d_b_from_e(gempic_maxwell2d *m, double dt, double *const *e, double *const *b);
 void m2d_e_from_j(gempic_maxwell2d *m, const double *j, int component, double *e);
+// j = (sum of the partial grids part2[0 .. splits - 1], in order) * scale,
+// then e -= the inverse 1-form mass of j's component; false (nothing
+// queued) where the grid takes the general path (m2d_e_from_j)
+bool m2d_e_from_parts(gempic_maxwell2d *m, const double *part2, int splits, double scale, double *j,
+                      int component, double *e);
 void m2d_rho_from_e(gempic_maxwell2d *m, double *const *e, double *rho);
 }  // namespace gl

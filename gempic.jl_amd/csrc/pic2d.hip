@@ -608,177 +608,174 @@ __global__ void k_div(Field2 f, const double *w1, const double *w2, double *rho)
     }
 }
 
-// ---- fused field updates of the 2D3V splitting (grids of at most 4096
-// nodes): one workgroup holds a component's grid in LDS, rows padded to
-// nx + 1 doubles so that lanes on consecutive rows hit different banks.
+// ---- field updates of the 2D3V splitting in two kernels each: the first
+// takes blocks of kFieldRows rows (the mass products, the curl and the
+// current sum row by row, then the inverse mass along x), the second blocks
+// of kFieldCols columns (the inverse mass along y and the update of E). The
+// inverse mass of the DFT solve is a circulant matrix; its rows are the
+// kernels of host_inv_mass_kernel, applied as circular convolutions.
+
+constexpr int kFieldRows = 4, kFieldCols = 4, kFieldThreads = 256;
 
 struct FieldTabs {
     const double *l0x, *l1x, *l0y, *l1y;   // mass lines of degree p and p - 1 (times dx, dy)
-    const double *k0x, *k1x, *k0y, *k1y;   // inverse-mass convolution kernels (host_inv_mass_kernel)
+    const double *k0x, *k1x, *k0y, *k1y;   // inverse-mass convolution kernels
     int degree;
 };
 
-// k_mass<DIR_X> on a grid read through in(i, j), written to LDS (row
-// stride nx + 1), with k_mass's arithmetic
-template <bool DIR_X, class In>
-__device__ void mass_line(In in, double *out, const double *line, int deg, int nx, int ny)
+// out[rr][i] = sum_m k[m] in[rr][(i - m) mod nx] for `rows` rows of nx in LDS
+__device__ void conv_rows(const double *in, const double *k, int rows, int nx, double *out, int64_t ostride)
 {
-    const int n = DIR_X ? nx : ny;
-    for (int k = threadIdx.x; k < nx * ny; k += blockDim.x) {
-        const int i = k % nx, j = k / nx;
-        const int r = DIR_X ? i : j;
-        double o = line[0] * (DIR_X ? in(r, j) : in(i, r));
-        for (int c = 1; c <= deg; ++c) {
-            int rp = r + c, rm = r - c;
-            if (rp >= n) rp -= n;
-            if (rm < 0) rm += n;
-            o += line[c] * (DIR_X ? in(rp, j) + in(rm, j) : in(i, rp) + in(i, rm));
+    for (int t = threadIdx.x; t < rows * nx; t += blockDim.x) {
+        const int rr = t / nx, i = t - rr * nx;
+        const double *line = in + rr * nx;
+        double acc = 0.0;
+        int u = i;
+        for (int m = 0; m < nx; ++m) {
+            acc = __builtin_fma(k[m], line[u], acc);
+            u = u == 0 ? nx - 1 : u - 1;
         }
-        out[i + j * (nx + 1)] = o;
+        out[(int64_t)rr * ostride + i] = acc;
     }
 }
 
-// Circular convolution of a grid in LDS (row stride nx + 1) along x
-// (DIR_X) or y: out(i, j) = sum_m k[m] in(i - m, j), resp. in(i, j - m),
-// indices mod n. A work item takes the R outputs q + S r (S = n / R) of
-// one line: input q - t (mod n) feeds output q + S r through k[t + S r]
-// (mod n), so one pass over the line's n inputs serves all R. k2 is the
-// kernel twice over in LDS (k2[m] = k[m mod n] for m < 2n): its reads are
-// broadcasts, four consecutive values at a time. Lanes take consecutive
-// lines (rows for x, columns for y).
-template <bool DIR_X, int R, class Out>
-__device__ void circ_conv_r(const double *in, const double *k2, int nx, int ny, Out out)
-{
-    const int P = nx + 1;
-    const int n = DIR_X ? nx : ny, S = n / R, lines = DIR_X ? ny : nx;
-    const int st = DIR_X ? 1 : P;
-    for (int w = threadIdx.x; w < lines * S; w += blockDim.x) {
-        const int line = w % lines, q = w / lines;
-        const double *src = DIR_X ? in + line * P : in + line;
-        double acc[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) acc[r] = 0.0;
-        int u = q, t = 0;
-        for (; t + 4 <= n; t += 4) {
-            double v[4];
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                v[e] = src[u * st];
-                u = u == 0 ? n - 1 : u - 1;
-            }
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const double *kr = k2 + t + S * r;
-#pragma unroll
-                for (int e = 0; e < 4; ++e) acc[r] = __builtin_fma(kr[e], v[e], acc[r]);
-            }
-        }
-        for (; t < n; ++t) {
-            const double v = src[u * st];
-            u = u == 0 ? n - 1 : u - 1;
-#pragma unroll
-            for (int r = 0; r < R; ++r) acc[r] = __builtin_fma(k2[t + S * r], v, acc[r]);
-        }
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            if (DIR_X) out(q + S * r, line, acc[r]);
-            else out(line, q + S * r, acc[r]);
-        }
-    }
-}
-
-template <bool DIR_X, class Out>
-__device__ void circ_conv(const double *in, const double *k2, int nx, int ny, Out out)
-{
-    const int n = DIR_X ? nx : ny;
-    if (n % 4 == 0) circ_conv_r<DIR_X, 4>(in, k2, nx, ny, out);
-    else if (n % 2 == 0) circ_conv_r<DIR_X, 2>(in, k2, nx, ny, out);
-    else circ_conv_r<DIR_X, 1>(in, k2, nx, ny, out);
-}
-
-// the inverse-mass kernels kx, ky twice over into LDS (kx2: 2 nx values, ky2: 2 ny)
-__device__ void stage_kernels(const double *kx, const double *ky, int nx, int ny, double *kx2, double *ky2)
-{
-    for (int m = threadIdx.x; m < 2 * nx; m += blockDim.x) kx2[m] = kx[m < nx ? m : m - nx];
-    for (int m = threadIdx.x; m < 2 * ny; m += blockDim.x) ky2[m] = ky[m < ny ? m : m - ny];
-}
-
-// compute_e_from_b! (maxwell_2d_fem.jl:370-411) for the 2D3V splitting:
-// workgroup c updates e_{c+1} += dt M1^-1 (curl M2 b)_{c+1}, with the mass
-// products and the curl of k_mass and k_curl_b and the inverse mass as the
-// circulant convolutions x then y (the DFT solve's matrix, in closed form)
-__global__ void __launch_bounds__(1024) k_fields_e_from_b(Field2 f, FieldTabs t, double dt,
-                                                          const double *b1, const double *b2,
-                                                          const double *b3, double *e1, double *e2,
-                                                          double *e3)
+// compute_e_from_j! (maxwell_2d_fem.jl:455-459), first kernel: the rows
+// r0 .. r0 + rows - 1 of the current, j = (sum of the splits' partial grids
+// in order) * scale as k_s2_reduce2 forms it (written to jout), then their
+// inverse mass along x into tmp
+__global__ void __launch_bounds__(kFieldThreads) k_fj_rows(Field2 f, const double *part2, int splits,
+                                                           double scale, const double *kx, double *jout,
+                                                           double *tmp)
 {
     extern __shared__ __align__(16) double lds[];
-    const int nx = f.nx, ny = f.ny, P = nx + 1, nn = nx * ny;
-    double *A = lds, *B = lds + (size_t)P * ny, *C = lds + 2 * (size_t)P * ny;
-    double *kx2 = lds + 3 * (size_t)P * ny, *ky2 = kx2 + 2 * nx;
-    const int c = blockIdx.x;
-    // the inverse mass of the 1-form's component c + 1 (degree p - 1 along
-    // its own direction)
-    stage_kernels(c == 0 ? t.k1x : t.k0x, c == 1 ? t.k1y : t.k0y, nx, ny, kx2, ky2);
-    auto glob = [nx](const double *g) { return [=](int i, int j) { return g[i + j * nx]; }; };
-    auto loc = [P](const double *a) { return [=](int i, int j) { return a[i + j * P]; }; };
+    const int nx = f.nx, nn = nx * f.ny;
+    const int r0 = blockIdx.x * kFieldRows, rows = min(kFieldRows, f.ny - r0);
+    double *jr = lds, *kk = lds + kFieldRows * nx;
+    for (int m = threadIdx.x; m < nx; m += blockDim.x) kk[m] = kx[m];
+    for (int t = threadIdx.x; t < rows * nx; t += blockDim.x) {
+        const int k = r0 * nx + t;
+        double v = 0.0;
+#pragma unroll 8
+        for (int sp = 0; sp < splits; ++sp) v += part2[(size_t)sp * nn + k];
+        v = v * scale;
+        jout[k] = v;
+        jr[t] = v;
+    }
+    __syncthreads();
+    conv_rows(jr, kk, rows, nx, tmp + (size_t)r0 * nx, nx);
+}
+
+// second kernel of both updates: the columns c0 .. c0 + cols - 1 of tmp
+// (component blockIdx.y), their inverse mass along y, e = e + coef w
+// (coef = dt: k_axpy; coef = -1: k_sub, bitwise e - w)
+struct FieldCols {
+    const double *tmp[3];
+    const double *ky[3];
+    double *e[3];
+};
+
+__global__ void __launch_bounds__(kFieldThreads) k_f_cols(Field2 f, FieldCols a, double coef)
+{
+    extern __shared__ __align__(16) double lds[];
+    const int nx = f.nx, ny = f.ny, c = blockIdx.y;
+    const int c0 = blockIdx.x * kFieldCols, cols = min(kFieldCols, nx - c0);
+    double *col = lds, *kk = lds + kFieldCols * ny;
+    const double *tmp = a.tmp[c], *ky = a.ky[c];
+    double *e = a.e[c];
+    for (int m = threadIdx.x; m < ny; m += blockDim.x) kk[m] = ky[m];
+    for (int t = threadIdx.x; t < cols * ny; t += blockDim.x) {
+        const int j = t / cols, cc = t - j * cols;
+        col[cc * ny + j] = tmp[(size_t)j * nx + c0 + cc];
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < cols * ny; t += blockDim.x) {
+        const int cc = t / ny, j = t - cc * ny;
+        const double *line = col + cc * ny;
+        double acc = 0.0;
+        int u = j;
+        for (int m = 0; m < ny; ++m) {
+            acc = __builtin_fma(kk[m], line[u], acc);
+            u = u == 0 ? ny - 1 : u - 1;
+        }
+        const size_t k = (size_t)j * nx + c0 + cc;
+        e[k] = e[k] + coef * acc;
+    }
+}
+
+// rows of a mass product M(x: lx, dgx; y: ly, dgy) b for the output rows
+// r0 .. r0 + nout - 1 (mod ny), with k_mass's arithmetic: the rows of b
+// from r0 - p (H = nout + 2p of them) staged in sb, their products along x
+// in mx, then along y into u
+__device__ void mass_rows(const double *b, const double *lx, int dgx, const double *ly, int dgy, int p,
+                          int r0, int nout, int nx, int ny, double *sb, double *mx, double *u)
+{
+    const int H = nout + 2 * p;
+    for (int t = threadIdx.x; t < H * nx; t += blockDim.x) {
+        const int hr = t / nx, i = t - hr * nx;
+        int j = r0 - p + hr;
+        j = j < 0 ? j + ny : (j >= ny ? j - ny : j);
+        sb[t] = b[(size_t)j * nx + i];
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < H * nx; t += blockDim.x) {
+        const int hr = t / nx, i = t - hr * nx;
+        const double *in = sb + hr * nx;
+        double o = lx[0] * in[i];
+        for (int c = 1; c <= dgx; ++c) {
+            int ip = i + c, im = i - c;
+            if (ip >= nx) ip -= nx;
+            if (im < 0) im += nx;
+            o += lx[c] * (in[ip] + in[im]);
+        }
+        mx[t] = o;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < nout * nx; t += blockDim.x) {
+        const int rr = t / nx, i = t - rr * nx;
+        const double *in = mx + (rr + p) * nx + i;
+        double o = ly[0] * in[0];
+        for (int c = 1; c <= dgy; ++c) o += ly[c] * (in[c * nx] + in[-c * nx]);
+        u[t] = o;
+    }
+    __syncthreads();
+}
+
+// compute_e_from_b! (maxwell_2d_fem.jl:370-411), first kernel: component
+// blockIdx.y of curl(M2 b) on the rows r0 .. r0 + rows - 1 (the mass
+// products and the curl of k_mass and k_curl_b), then its inverse mass
+// along x into tmp[c]
+struct FieldRowsB {
+    const double *b[3];
+    double *tmp[3];
+};
+
+__global__ void __launch_bounds__(kFieldThreads) k_fb_rows(Field2 f, FieldTabs t, FieldRowsB a)
+{
+    extern __shared__ __align__(16) double lds[];
+    const int nx = f.nx, ny = f.ny, p = t.degree, c = blockIdx.y;
+    const int r0 = blockIdx.x * kFieldRows, rows = min(kFieldRows, ny - r0);
+    const int H = kFieldRows + 1 + 2 * p;
+    double *sb = lds, *mx = sb + H * nx, *u1 = mx + H * nx, *u2 = u1 + (kFieldRows + 1) * nx;
+    double *cu = u2 + (kFieldRows + 1) * nx, *kk = cu + kFieldRows * nx;
+    const double *kx = c == 0 ? t.k1x : t.k0x;
+    for (int m = threadIdx.x; m < nx; m += blockDim.x) kk[m] = kx[m];
+    // u at rows r0 .. r0 + rows (the curl's j + 1 included)
     if (c == 2) {
-        // u1 = M(x: p, y: p - 1) b1, u2 = M(x: p - 1, y: p) b2; c3 = d_y u1 - d_x u2
-        mass_line<true>(glob(b1), A, t.l0x, t.degree, nx, ny);
-        __syncthreads();
-        mass_line<false>(loc(A), B, t.l1y, t.degree - 1, nx, ny);
-        __syncthreads();
-        mass_line<true>(glob(b2), A, t.l1x, t.degree - 1, nx, ny);
-        __syncthreads();
-        mass_line<false>(loc(A), C, t.l0y, t.degree, nx, ny);
-        __syncthreads();
-        for (int k = threadIdx.x; k < nn; k += blockDim.x) {
-            const int i = k % nx, j = k / nx;
-            const int i1 = i + 1 == nx ? 0 : i + 1, j1 = j + 1 == ny ? 0 : j + 1;
-            A[i + j * P] = (B[i + j * P] - B[i + j1 * P]) / f.dy - (C[i + j * P] - C[i1 + j * P]) / f.dx;
-        }
+        mass_rows(a.b[0], t.l0x, p, t.l1y, p - 1, p, r0, rows + 1, nx, ny, sb, mx, u1);
+        mass_rows(a.b[1], t.l1x, p - 1, t.l0y, p, p, r0, rows + 1, nx, ny, sb, mx, u2);
     } else {
-        // u3 = M(x: p - 1, y: p - 1) b3; c1 = -d_y u3, c2 = d_x u3
-        mass_line<true>(glob(b3), A, t.l1x, t.degree - 1, nx, ny);
-        __syncthreads();
-        mass_line<false>(loc(A), B, t.l1y, t.degree - 1, nx, ny);
-        __syncthreads();
-        for (int k = threadIdx.x; k < nn; k += blockDim.x) {
-            const int i = k % nx, j = k / nx;
-            const int i1 = i + 1 == nx ? 0 : i + 1, j1 = j + 1 == ny ? 0 : j + 1;
-            A[i + j * P] = c == 0 ? -(B[i + j * P] - B[i + j1 * P]) / f.dy
-                                  : (B[i + j * P] - B[i1 + j * P]) / f.dx;
-        }
+        mass_rows(a.b[2], t.l1x, p - 1, t.l1y, p - 1, p, r0, rows + 1, nx, ny, sb, mx, u1);
+    }
+    for (int q = threadIdx.x; q < rows * nx; q += blockDim.x) {
+        const int rr = q / nx, i = q - rr * nx;
+        const int i1 = i + 1 == nx ? 0 : i + 1;
+        const double *w = u1 + rr * nx;
+        if (c == 0) cu[q] = -(w[i] - w[i + nx]) / f.dy;
+        else if (c == 1) cu[q] = (w[i] - w[i1]) / f.dx;
+        else cu[q] = (w[i] - w[i + nx]) / f.dy - (u2[rr * nx + i] - u2[rr * nx + i1]) / f.dx;
     }
     __syncthreads();
-    // w = M1^-1 c, then e += dt w as k_axpy
-    double *e = c == 0 ? e1 : (c == 1 ? e2 : e3);
-    circ_conv<true>(A, kx2, nx, ny, [=](int i, int j, double v) { B[i + j * P] = v; });
-    __syncthreads();
-    circ_conv<false>(B, ky2, nx, ny, [=](int i, int j, double v) {
-        const int k = i + j * nx;
-        e[k] = e[k] + dt * v;
-    });
-}
-
-// compute_e_from_j! (maxwell_2d_fem.jl:455-459): e -= M1^-1 j for the
-// 1-form's component with inverse-mass kernels kx, ky, one workgroup
-__global__ void __launch_bounds__(1024) k_fields_e_from_j(Field2 f, const double *kx, const double *ky,
-                                                          const double *j, double *e)
-{
-    extern __shared__ __align__(16) double lds[];
-    const int nx = f.nx, ny = f.ny, P = nx + 1, nn = nx * ny;
-    double *A = lds, *B = lds + (size_t)P * ny;
-    double *kx2 = lds + 2 * (size_t)P * ny, *ky2 = kx2 + 2 * nx;
-    stage_kernels(kx, ky, nx, ny, kx2, ky2);
-    for (int k = threadIdx.x; k < nn; k += blockDim.x) A[k % nx + (k / nx) * P] = j[k];
-    __syncthreads();
-    circ_conv<true>(A, kx2, nx, ny, [=](int i, int jj, double v) { B[i + jj * P] = v; });
-    __syncthreads();
-    circ_conv<false>(B, ky2, nx, ny, [=](int i, int jj, double v) {
-        const int k = i + jj * nx;
-        e[k] = e[k] - v;
-    });
+    conv_rows(cu, kk, rows, nx, a.tmp[c] + (size_t)r0 * nx, nx);
 }
 
 struct PoissonTabs {
@@ -929,7 +926,7 @@ struct gempic_maxwell2d {
     double *d_tab = nullptr;            // NT2 x kNxMax
     double *d_buf = nullptr;            // 16 work arrays of nx*ny
     double *d_scalar = nullptr;
-    bool fused = false;                 // the one-workgroup field kernels fit (k_fields_*)
+    bool fused = false;                 // the row and column field kernels take this grid (k_fb_rows ..)
     const double *tab(int s) const { return d_tab + (size_t)s * kNxMax; }
     double *buf(int i) const { return d_buf + (size_t)i * nx * ny; }
     Field2 f() const { return Field2{nx, ny, dx, dy}; }
@@ -994,12 +991,13 @@ void form_lines(int form, int component, bool &onex, bool &oney)
     else if (form == 3) { onex = oney = true; }
 }
 
-// LDS of a one-workgroup field kernel: `grids` grids with rows of nx + 1
-// and the two inverse-mass kernels twice over
-size_t fused_lds(const gempic_maxwell2d *m, int grids)
+size_t fb_rows_lds(const gempic_maxwell2d *m)
 {
-    return sizeof(double) * ((size_t)grids * (size_t)(m->nx + 1) * (size_t)m->ny + 2 * (size_t)(m->nx + m->ny));
+    const size_t H = kFieldRows + 1 + 2 * (size_t)m->degree;
+    return sizeof(double) * (2 * H + 2 * (kFieldRows + 1) + kFieldRows + 1) * (size_t)m->nx;
 }
+
+size_t f_cols_lds(const gempic_maxwell2d *m) { return sizeof(double) * (kFieldCols + 1) * (size_t)m->ny; }
 
 FieldTabs field_tabs(const gempic_maxwell2d *m)
 {
@@ -1007,22 +1005,27 @@ FieldTabs field_tabs(const gempic_maxwell2d *m)
                      m->tab(T2_K0X), m->tab(T2_K1X), m->tab(T2_K0Y), m->tab(T2_K1Y), m->degree};
 }
 
-// the one-workgroup field kernels take grids of at most kFusedNodes nodes
-// (three padded grids in a workgroup's LDS)
+// the row and column field kernels hold a few rows or columns in LDS; they
+// take the grids of the 2D3V splitting (at most 4096 nodes)
 constexpr int64_t kFusedNodes = 4096;
 
 bool fused_setup(gempic_maxwell2d *m)
 {
     if ((int64_t)m->nx * m->ny > kFusedNodes) return false;
-    const int sh3 = (int)fused_lds(m, 3), sh2 = (int)fused_lds(m, 2);
-    if (hipFuncSetAttribute((const void *)k_fields_e_from_b, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            sh3) != hipSuccess ||
-        hipFuncSetAttribute((const void *)k_fields_e_from_j, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            sh2) != hipSuccess) {
+    const size_t shb = fb_rows_lds(m), shc = f_cols_lds(m);
+    if (shb > 65536 &&
+        hipFuncSetAttribute((const void *)k_fb_rows, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shb) !=
+            hipSuccess) {
         (void)hipGetLastError();
         return false;
     }
-    return true;
+    if (shc > 65536 &&
+        hipFuncSetAttribute((const void *)k_f_cols, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shc) !=
+            hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return sizeof(double) * (size_t)(kFieldRows + 1) * m->nx <= 65536;
 }
 
 int finish2(gempic_maxwell2d *m)
@@ -1045,9 +1048,13 @@ void gl::m2d_e_from_b(gempic_maxwell2d *m, double dt, double *const *b, double *
 {
     const Field2 f = m->f();
     if (m->fused) {
-        hipLaunchKernelGGL(k_fields_e_from_b, dim3(3), dim3(1024), fused_lds(m, 3), m->stream, f,
-                           field_tabs(m), dt, (const double *)b[0], (const double *)b[1],
-                           (const double *)b[2], e[0], e[1], e[2]);
+        const FieldTabs t = field_tabs(m);
+        FieldRowsB ra{{b[0], b[1], b[2]}, {m->buf(9), m->buf(10), m->buf(11)}};
+        hipLaunchKernelGGL(k_fb_rows, dim3((m->ny + kFieldRows - 1) / kFieldRows, 3), dim3(kFieldThreads),
+                           fb_rows_lds(m), m->stream, f, t, ra);
+        FieldCols ca{{m->buf(9), m->buf(10), m->buf(11)}, {t.k0y, t.k1y, t.k0y}, {e[0], e[1], e[2]}};
+        hipLaunchKernelGGL(k_f_cols, dim3((m->nx + kFieldCols - 1) / kFieldCols, 3), dim3(kFieldThreads),
+                           f_cols_lds(m), m->stream, f, ca, dt);
         return;
     }
     const unsigned gb = grid_blocks((int64_t)m->nx * m->ny);
@@ -1071,15 +1078,25 @@ void gl::m2d_b_from_e(gempic_maxwell2d *m, double dt, double *const *e, double *
 
 void gl::m2d_e_from_j(gempic_maxwell2d *m, const double *j, int component, double *e)
 {
-    if (m->fused) {
-        const FieldTabs t = field_tabs(m);
-        hipLaunchKernelGGL(k_fields_e_from_j, dim3(1), dim3(1024), fused_lds(m, 2), m->stream, m->f(),
-                           component == 1 ? t.k1x : t.k0x, component == 2 ? t.k1y : t.k0y, j, e);
-        return;
-    }
     const unsigned gb = grid_blocks((int64_t)m->nx * m->ny);
     inv_mass(m, 1, component, j, m->buf(2));
     hipLaunchKernelGGL(k_sub, dim3(gb), dim3(256), 0, m->stream, m->f(), e, m->buf(2));
+}
+
+bool gl::m2d_e_from_parts(gempic_maxwell2d *m, const double *part2, int splits, double scale, double *j,
+                          int component, double *e)
+{
+    if (!m->fused) return false;
+    const Field2 f = m->f();
+    const FieldTabs t = field_tabs(m);
+    hipLaunchKernelGGL(k_fj_rows, dim3((m->ny + kFieldRows - 1) / kFieldRows), dim3(kFieldThreads),
+                       sizeof(double) * (size_t)(kFieldRows + 1) * m->nx, m->stream, f, part2, splits, scale,
+                       component == 1 ? t.k1x : t.k0x, j, m->buf(12));
+    FieldCols ca{{m->buf(12), nullptr, nullptr}, {component == 2 ? t.k1y : t.k0y, nullptr, nullptr},
+                 {e, nullptr, nullptr}};
+    hipLaunchKernelGGL(k_f_cols, dim3((m->nx + kFieldCols - 1) / kFieldCols, 1), dim3(kFieldThreads),
+                       f_cols_lds(m), m->stream, f, ca, -1.0);
+    return true;
 }
 
 void gl::m2d_rho_from_e(gempic_maxwell2d *m, double *const *e, double *rho)

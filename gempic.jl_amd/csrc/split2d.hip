@@ -72,7 +72,7 @@ constexpr int kNn2Max = 4096;              // nx * ny of the device path (LDS gr
 #endif
 constexpr double kS2GenSkew = GEMPIC_S2_GEN_SKEW;   // chunk weights of a workgroup's waves (s2_args)
 #ifndef GEMPIC_S2_SPLITS
-#define GEMPIC_S2_SPLITS 64
+#define GEMPIC_S2_SPLITS 16
 #endif
 constexpr int kSplits2 = GEMPIC_S2_SPLITS;  // first-level groups of the workgroup-grid sum
 
@@ -740,9 +740,11 @@ S2Args s2_args(gempic_splitting2d *h, double dt, int op)
     return a;
 }
 
-// one particle pass; with a deposit, its workgroup grids summed into `out` (times scale)
+// one particle pass; with a deposit, its workgroup grids summed into `out`
+// (times scale), and with e_comp > 0 then E_{e_comp} -= the inverse 1-form
+// mass of it (compute_e_from_j!)
 int s2_pass(gempic_splitting2d *h, int op, double dt, const double *f0, const double *f1,
-            const double *f2, double *out, double scale)
+            const double *f2, double *out, double scale, int e_comp = 0)
 {
     if (h->n == 0) {
         if (out) S2_TRY("2D3V deposit", hipMemsetAsync(out, 0, sizeof(double) * h->nx * h->ny, h->stream));
@@ -764,8 +766,11 @@ int s2_pass(gempic_splitting2d *h, int op, double dt, const double *f0, const do
         double *part2 = h->d_part + (size_t)nblk * nn;
         hipLaunchKernelGGL(k_s2_reduce1, dim3((unsigned)((nn + 255) / 256), (unsigned)splits), dim3(256), 0,
                            h->stream, (const double *)h->d_part, nblk, nn, part2);
-        hipLaunchKernelGGL(k_s2_reduce2, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, h->stream,
-                           (const double *)part2, splits, nn, scale, out);
+        if (!(e_comp > 0 && m2d_e_from_parts(h->m, part2, splits, scale, out, e_comp, h->d_e[e_comp - 1]))) {
+            hipLaunchKernelGGL(k_s2_reduce2, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, h->stream,
+                               (const double *)part2, splits, nn, scale, out);
+            if (e_comp > 0) m2d_e_from_j(h->m, out, e_comp, h->d_e[e_comp - 1]);
+        }
         S2_TRY("2D3V deposit sum", hipGetLastError());
     }
     return GEMPIC_OK;
@@ -827,12 +832,11 @@ int s2_hb(gempic_splitting2d *h, double dt)
 int s2_hp(gempic_splitting2d *h, int comp, double dt)
 {
     int rc;
-    if (comp == 1) rc = s2_pass(h, S2_HP1, dt, h->d_b[2], h->d_b[1], nullptr, h->d_j[0], 1.0);
-    else if (comp == 2) rc = s2_pass(h, S2_HP2, dt, h->d_b[2], h->d_b[0], nullptr, h->d_j[1], 1.0);
-    else rc = s2_pass(h, S2_HP3, dt, h->d_b[0], h->d_b[1], nullptr, h->d_j[2], dt);
-    if (rc) return rc;
-    m2d_e_from_j(h->m, h->d_j[comp - 1], comp, h->d_e[comp - 1]);   // compute_e_from_j!
-    return GEMPIC_OK;
+    // the pass, then compute_e_from_j!
+    if (comp == 1) rc = s2_pass(h, S2_HP1, dt, h->d_b[2], h->d_b[1], nullptr, h->d_j[0], 1.0, 1);
+    else if (comp == 2) rc = s2_pass(h, S2_HP2, dt, h->d_b[2], h->d_b[0], nullptr, h->d_j[1], 1.0, 2);
+    else rc = s2_pass(h, S2_HP3, dt, h->d_b[0], h->d_b[1], nullptr, h->d_j[2], dt, 3);
+    return rc;
 }
 
 }  // namespace
