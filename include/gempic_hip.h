@@ -538,6 +538,8 @@ int gempic_splitting2d_operator_hp3(gempic_splitting2d *h, double dt);
 /* flags: GEMPIC_ASYNC returns once the work is queued */
 int gempic_splitting2d_strang_splitting(gempic_splitting2d *h, double dt, int64_t n_steps, int flags);
 int gempic_splitting2d_synchronize(gempic_splitting2d *h);
+/* sort the particles by cell every `interval` Strang steps (0, the default:
+   never; the passes run faster on unordered particles) */
 int gempic_splitting2d_set_sort_interval(gempic_splitting2d *h, int64_t interval);
 /* add_charge! (particle_mesh_coupling_2d.jl:94-105) of the handle's particles
  * with degree (p, p) and q = charge w common_weight, into rho (nx*ny, not
