@@ -80,8 +80,9 @@ def arith_run(G, _lib, w, arith, warmup, steps, blocks, interval, sort_s):
 
 
 BYTES_2D3V = 472          # per particle-step of the 2D3V splitting's passes: HE 64 x 2,
-                          # Hp3 64 x 2, Hp2 72 x 2, Hp1 72 (48 B state read per pass,
-                          # the pushed components written)
+                          # Hp3 64 x 2, Hp2 72 x 2, Hp1 72 (the state read: 40 B by HE,
+                          # 48 B with the weights by the others; the pushed components
+                          # written)
 
 
 def split2d_run(G, n, steps, cpu_n, cpu_steps):
