@@ -67,6 +67,9 @@ constexpr int s2_threads(int op)
     return op == 1 ? GEMPIC_S2_THREADS_HP1 : op == 2 ? GEMPIC_S2_THREADS_HP2 : (op == 3 ? GEMPIC_S2_THREADS_HP3 : 1024);
 }
 constexpr int kNn2Max = 4096;              // nx * ny of the device path (LDS grids)
+#ifndef GEMPIC_S2_HE_PREFETCH
+#define GEMPIC_S2_HE_PREFETCH 1
+#endif
 #ifndef GEMPIC_S2_GEN_SKEW
 #define GEMPIC_S2_GEN_SKEW 0.25
 #endif
@@ -283,6 +286,43 @@ __global__ void __launch_bounds__(s2_threads(OP)) k_split2d(S2Args a)
     constexpr int kW2 = s2_threads(OP) / kWave;
     const int64_t wbase = (int64_t)blockIdx.x * kW2 * a.chunk + a.woff[wave];
     const int64_t wlen = a.wch[wave];
+    if constexpr (OP == S2_HE && GEMPIC_S2_HE_PREFETCH) {
+        // HE reads the state and the staged E only: the next particle's
+        // state is loaded while this one's gathers run
+        int64_t i = wbase + lane;
+        bool valid = 0 < wlen && i < a.n;
+        int64_t ii = valid ? i : 0;
+        double nx1 = a.x1[ii], nx2 = a.x2[ii], nv1 = a.v1[ii], nv2 = a.v2[ii], nv3 = a.v3[ii];
+        for (int64_t off = 0; off < wlen; off += kWave) {
+            if (!__ballot(valid)) break;
+            const double x1 = nx1, x2 = nx2;
+            double v1 = nv1, v2 = nv2, v3 = nv3;
+            const int64_t ic = i;
+            const bool vc = valid;
+            i += kWave;
+            valid = off + kWave < wlen && i < a.n;
+            ii = valid ? i : ii;
+            nx1 = a.x1[ii];
+            nx2 = a.x2[ii];
+            nv1 = a.v1[ii];
+            nv2 = a.v2[ii];
+            nv3 = a.v3[ii];
+            int cx, cy;
+            double bx0[P + 1], bx1[P], by0[P + 1], by1[P];
+            bool ok = shape_dir<P>(x1, a.xmin, a.dx, a.rdx, nx, cx, bx0, bx1);
+            ok = shape_dir<P>(x2, a.ymin, a.dy, a.rdy, ny, cy, by0, by1) && ok;
+            if (vc && !ok) errs |= E_INEXACT;
+            const double e1 = gather2<P, P - 1, P>(fs0, W, cx, cy, bx1, by0);
+            const double e2 = gather2<P, P, P - 1>(fs1, W, cx, cy, bx0, by1);
+            const double e3 = gather2<P, P, P>(fs2, W, cx, cy, bx0, by0);
+            const double dtqm = dt * qm;
+            if (vc) {
+                a.v1[ic] = __builtin_fma(dtqm, e1, v1);
+                a.v2[ic] = __builtin_fma(dtqm, e2, v2);
+                a.v3[ic] = __builtin_fma(dtqm, e3, v3);
+            }
+        }
+    } else
     for (int64_t off = 0; off < wlen; off += kWave) {
         const int64_t i = wbase + off + lane;
         const bool valid = i < a.n;
